@@ -1,0 +1,98 @@
+"""Thin torch-facing wrappers over the C ABI.
+
+Tensors are complex128 CUDA (HIP) tensors; the wrappers pass raw device
+pointers, element strides and the current HIP stream of the tensor's
+device.  No computation happens in Python.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+_WS = {}
+
+
+def stream_of(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def workspace(nbytes, device):
+    """A per-device scratch buffer of at least ``nbytes`` bytes (grown, never
+    shrunk).  Calls on one device and stream are ordered, so sharing it is
+    safe for stream-ordered use."""
+    key = torch.device(device)
+    buf = _WS.get(key)
+    if buf is None or buf.numel() < nbytes:
+        buf = torch.empty(max(int(nbytes), 1 << 20), dtype=torch.uint8, device=key)
+        _WS[key] = buf
+    return buf
+
+
+def _require(t, name):
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.complex128 or not t.is_cuda:
+        raise TypeError(f"{name} must be a complex128 device tensor")
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def zgemm_raw(*, m, n, k, a, a_layout, lda, b, b_layout, ldb, c, ldc, alpha=1.0, beta=0.0,
+              conj_a=False, conj_b=False, batch=1, stride_a=0, stride_b=0, stride_c=0,
+              reduce_batch=False, split_k=0):
+    """C = alpha * op(A) op(B) + beta * C on raw strides (see apy_zgemm)."""
+    lib = _lib.load()
+    alpha = complex(alpha)
+    beta = complex(beta)
+    d = _lib.GemmDesc(
+        m, n, k, a_layout, b_layout, int(conj_a), int(conj_b),
+        alpha.real, alpha.imag, beta.real, beta.imag,
+        a.data_ptr(), lda, stride_a,
+        b.data_ptr(), ldb, stride_b,
+        c.data_ptr(), ldc, stride_c,
+        batch, int(reduce_batch), int(split_k))
+    nbytes = lib.apy_zgemm_workspace(ctypes.byref(d))
+    ws = workspace(nbytes, c.device) if nbytes else None
+    st = lib.apy_zgemm(ctypes.byref(d), ctypes.c_void_p(ws.data_ptr() if ws is not None else 0),
+                       ctypes.c_size_t(nbytes), stream_of(c.device))
+    _lib.check(st, "apy_zgemm")
+    return c
+
+
+def _layout_a(A):
+    if A.stride(1) == 1:
+        return _lib.APY_MK, A.stride(0)
+    if A.stride(0) == 1:
+        return _lib.APY_KM, A.stride(1)
+    raise ValueError("A must have a unit stride along one dimension")
+
+
+def _layout_b(B):
+    if B.stride(1) == 1:
+        return _lib.APY_KN, B.stride(0)
+    if B.stride(0) == 1:
+        return _lib.APY_NK, B.stride(1)
+    raise ValueError("B must have a unit stride along one dimension")
+
+
+def mm(A, B, C=None, alpha=1.0, beta=0.0, conj_a=False, conj_b=False, split_k=0):
+    """C = alpha*A@B + beta*C for 2-D complex128 device tensors (views with a
+    unit stride along one dimension are accepted without copies)."""
+    _require(A, "A")
+    _require(B, "B")
+    m, k = A.shape
+    k2, n = B.shape
+    if k != k2:
+        raise ValueError("inner dimensions differ")
+    if C is None:
+        C = torch.empty((m, n), dtype=torch.complex128, device=A.device)
+        beta = 0.0
+    _require(C, "C")
+    if C.shape != (m, n) or (C.stride(1) != 1 and n > 1):
+        raise ValueError("C must be (m, n) with unit column stride")
+    al, lda = _layout_a(A)
+    bl, ldb = _layout_b(B)
+    return zgemm_raw(m=m, n=n, k=k, a=A, a_layout=al, lda=lda, b=B, b_layout=bl, ldb=ldb,
+                     c=C, ldc=C.stride(0), alpha=alpha, beta=beta, conj_a=conj_a, conj_b=conj_b,
+                     split_k=split_k)
