@@ -49,11 +49,57 @@ class GemmDesc(ctypes.Structure):
     ]
 
 
+class SoInts(ctypes.Structure):
+    _fields_ = [
+        ("O", c_i64), ("V", c_i64),
+        ("vvvv_p", c_vp), ("ring", c_vp), ("oooo_p", c_vp), ("ovoo_p", c_vp), ("vvvo_p", c_vp),
+        ("vovv", c_vp), ("ooov", c_vp), ("vvoo_t", c_vp), ("oovv", c_vp),
+    ]
+
+
+SO_BLOCKS = ("vvvv_p", "ring", "oooo_p", "ovoo_p", "vvvo_p", "vovv", "ooov", "vvoo_t", "oovv")
+
+APY_MODE_RESIDUAL = 0
+APY_MODE_SIGMA = 1
+APY_MAXVEC = 16
+
+
+class CisdSoArgs(ctypes.Structure):
+    _fields_ = [
+        ("O", c_i64), ("V", c_i64),
+        ("fock", c_vp), ("eps", c_vp),
+        ("ints", ctypes.POINTER(SoInts)),
+        ("hoo", c_vp),
+        ("singles", c_int), ("mode", c_int), ("fov_coupling", c_int),
+        ("energy", c_vp),
+        ("c0_re", c_dbl), ("c0_im", c_dbl),
+        ("t", c_vp), ("t_new", c_vp), ("out", c_vp),
+    ]
+
+
+_P_I64 = ctypes.POINTER(c_i64)
+_P_DBL = ctypes.POINTER(c_dbl)
+_P_VP = ctypes.POINTER(c_vp)
+
 # name -> (restype, argtypes); every symbol here is declared in include/apyib_amd.h
 SIGNATURES = {
     "apy_version": (c_int, []),
     "apy_zgemm_workspace": (c_size, [ctypes.POINTER(GemmDesc)]),
     "apy_zgemm": (c_int, [ctypes.POINTER(GemmDesc), c_vp, c_size, c_vp]),
+    "apy_so_ints_sizes": (c_int, [c_i64, c_i64, _P_I64]),
+    "apy_so_ints_from_full": (c_int, [c_i64, c_vp, ctypes.POINTER(SoInts), c_vp]),
+    "apy_so_ints_from_mo": (c_int, [c_i64, c_vp, ctypes.POINTER(SoInts), c_vp]),
+    "apy_synth_so_ints": (c_int, [ctypes.c_uint64, ctypes.c_uint64, c_dbl, c_int, c_dbl, c_int,
+                                  ctypes.POINTER(SoInts), c_vp]),
+    "apy_so_fock_from_mo": (c_int, [c_i64, c_vp, c_vp, c_vp]),
+    "apy_cisd_so_pair_operator": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "apy_cisd_so_workspace": (c_size, [c_i64, c_i64, c_int]),
+    "apy_cisd_so_step": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_vp]),
+    "apy_cisd_so_energy": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_zdot_many": (c_int, [c_int, _P_VP, c_vp, c_i64, c_int, c_vp, c_vp, c_size, c_vp]),
+    "apy_zdiffsq_segments": (c_int, [c_vp, c_vp, c_int, _P_I64, c_vp, c_vp, c_size, c_vp]),
+    "apy_zcombine": (c_int, [c_int, _P_VP, _P_DBL, c_i64, c_vp, c_vp]),
+    "apy_reduce_workspace": (c_size, [c_i64, c_int]),
 }
 
 _lib = None

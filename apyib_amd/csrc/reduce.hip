@@ -1,0 +1,177 @@
+// Deterministic vector reductions for DIIS (utils.py:104-140) and the
+// convergence test (ci_wfn.py:361-365): block partial sums in a fixed order,
+// then one finishing block, so repeated calls give identical bits.
+#include "common.h"
+
+namespace {
+
+constexpr int NB = 512;  // partial-sum blocks
+
+struct VecList {
+  const double2* p[APY_MAXVEC];
+};
+struct CoefList {
+  double re[APY_MAXVEC], im[APY_MAXVEC];
+};
+
+template <int K>
+__global__ __launch_bounds__(256) void dot_many_partial(VecList xs, const double2* __restrict__ y, int64_t len,
+                                                        int conj_x, double2* __restrict__ partial) {
+  double ar[K], ai[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) { ar[m] = 0.0; ai[m] = 0.0; }
+  const double cs = conj_x ? -1.0 : 1.0;
+  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < len; l += (int64_t)gridDim.x * blockDim.x) {
+    const double2 yv = y[l];
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      const double2 x = xs.p[m][l];
+      const double xi = cs * x.y;
+      ar[m] += x.x * yv.x - xi * yv.y;
+      ai[m] += x.x * yv.y + xi * yv.x;
+    }
+  }
+  __shared__ double sr[256], si[256];
+#pragma unroll 1
+  for (int m = 0; m < K; ++m) {
+    sr[threadIdx.x] = ar[m];
+    si[threadIdx.x] = ai[m];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) { sr[threadIdx.x] += sr[threadIdx.x + w]; si[threadIdx.x] += si[threadIdx.x + w]; }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * K + m] = make_double2(sr[0], si[0]);
+    __syncthreads();
+  }
+}
+
+__global__ void finalize_many(const double2* __restrict__ partial, int nb, int k, double2* out) {
+  __shared__ double sr[256], si[256];
+  for (int m = 0; m < k; ++m) {
+    double a = 0.0, b = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) { a += partial[(int64_t)q * k + m].x; b += partial[(int64_t)q * k + m].y; }
+    sr[threadIdx.x] = a;
+    si[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) { sr[threadIdx.x] += sr[threadIdx.x + w]; si[threadIdx.x] += si[threadIdx.x + w]; }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[m] = make_double2(sr[0], si[0]);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void diffsq_partial(const double2* __restrict__ x, const double2* __restrict__ y,
+                                                      int64_t beg, int64_t end, double2* __restrict__ partial) {
+  double a = 0.0, b = 0.0;
+  for (int64_t l = beg + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < end; l += (int64_t)gridDim.x * blockDim.x) {
+    const double dr = x[l].x - y[l].x, di = x[l].y - y[l].y;
+    a += dr * dr - di * di;
+    b += 2.0 * dr * di;
+  }
+  __shared__ double sr[256], si[256];
+  sr[threadIdx.x] = a;
+  si[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { sr[threadIdx.x] += sr[threadIdx.x + w]; si[threadIdx.x] += si[threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(sr[0], si[0]);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void combine_kernel(VecList xs, CoefList c, int64_t len, double2* __restrict__ y) {
+  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < len; l += (int64_t)gridDim.x * blockDim.x) {
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      const double2 x = xs.p[m][l];
+      a += c.re[m] * x.x - c.im[m] * x.y;
+      b += c.re[m] * x.y + c.im[m] * x.x;
+    }
+    y[l] = make_double2(a, b);
+  }
+}
+
+template <int K>
+void launch_dot(VecList v, const double2* y, int64_t len, int conj, double2* part, hipStream_t s) {
+  dot_many_partial<K><<<NB, 256, 0, s>>>(v, y, len, conj, part);
+}
+
+template <int K>
+void launch_comb(VecList v, CoefList c, int64_t len, double2* y, hipStream_t s) {
+  combine_kernel<K><<<apy_grid1d(len, 8192), 256, 0, s>>>(v, c, len, y);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t apy_reduce_workspace(int64_t len, int k) {
+  (void)len;
+  if (k < 1) k = 1;
+  return (size_t)NB * (size_t)k * sizeof(double2);
+}
+
+int apy_zdot_many(int k, const void* const* xs, const void* y, int64_t len, int conj_x, void* out, void* workspace,
+                  size_t ws_bytes, void* stream) {
+  if (k < 1 || k > APY_MAXVEC || !xs || !y || !out || len < 0) return APY_E_ARG;
+  if (!workspace || ws_bytes < apy_reduce_workspace(len, k)) return APY_E_WORKSPACE;
+  VecList v{};
+  for (int m = 0; m < k; ++m) {
+    if (!xs[m]) return APY_E_ARG;
+    v.p[m] = (const double2*)xs[m];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  double2* part = (double2*)workspace;
+  switch (k) {
+#define C(K) case K: launch_dot<K>(v, (const double2*)y, len, conj_x, part, s); break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
+#undef C
+  }
+  APY_LAUNCH_CHECK();
+  finalize_many<<<1, 256, 0, s>>>(part, NB, k, (double2*)out);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_zdiffsq_segments(const void* x, const void* y, int nseg, const int64_t* off, void* out, void* workspace,
+                         size_t ws_bytes, void* stream) {
+  if (!x || !y || !off || !out || nseg < 1 || nseg > APY_MAXVEC) return APY_E_ARG;
+  if (!workspace || ws_bytes < apy_reduce_workspace(0, 1)) return APY_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double2* part = (double2*)workspace;
+  for (int q = 0; q < nseg; ++q) {
+    if (off[q + 1] < off[q]) return APY_E_ARG;
+    diffsq_partial<<<NB, 256, 0, s>>>((const double2*)x, (const double2*)y, off[q], off[q + 1], part);
+    APY_LAUNCH_CHECK();
+    finalize_many<<<1, 256, 0, s>>>(part, NB, 1, (double2*)out + q);
+    APY_LAUNCH_CHECK();
+  }
+  return APY_OK;
+}
+
+int apy_zcombine(int k, const void* const* xs, const double* coef, int64_t len, void* y, void* stream) {
+  if (k < 1 || k > APY_MAXVEC || !xs || !coef || !y || len < 0) return APY_E_ARG;
+  VecList v{};
+  CoefList c{};
+  for (int m = 0; m < k; ++m) {
+    if (!xs[m]) return APY_E_ARG;
+    v.p[m] = (const double2*)xs[m];
+    c.re[m] = coef[2 * m];
+    c.im[m] = coef[2 * m + 1];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  switch (k) {
+#define C(K) case K: launch_comb<K>(v, c, len, (double2*)y, s); break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
+#undef C
+  }
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+}  // extern "C"

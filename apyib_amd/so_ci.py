@@ -1,0 +1,281 @@
+"""Spin-orbital CISD / CID on the GPU: device integrals and the reference's
+Jacobi + DIIS iteration (ci_wfn.py:171-416, utils.py:104-140).
+
+All tensor work runs in libapyib_amd.so; Python only sequences the calls,
+solves the (at most 9x9) bordered DIIS system and applies the reference's
+convergence test to a handful of scalars copied back per iteration.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .kernels import stream_of, workspace
+
+CDT = torch.complex128
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def npair(n):
+    return n * (n - 1) // 2
+
+
+ALL_BLOCKS = _lib.SO_BLOCKS
+CID_BLOCKS = ("vvvv_p", "ring", "oooo_p", "vvoo_t", "oovv")
+MP2_BLOCKS = ("vvoo_t", "oovv")
+
+
+class SOIntegrals:
+    """Antisymmetrized spin-orbital blocks <pq||rs> resident on one device,
+    in the layouts documented for apy_so_ints in include/apyib_amd.h."""
+
+    def __init__(self, O, V, device="cuda", blocks=ALL_BLOCKS):
+        lib = _lib.load()
+        self.O, self.V = int(O), int(V)
+        self.device = torch.device(device)
+        sizes = (ctypes.c_int64 * 9)()
+        _lib.check(lib.apy_so_ints_sizes(self.O, self.V, sizes), "apy_so_ints_sizes")
+        self.blocks = {}
+        for k, name in enumerate(_lib.SO_BLOCKS):
+            if name in blocks:
+                self.blocks[name] = torch.empty(int(sizes[k]), dtype=CDT, device=self.device)
+        self.struct = _lib.SoInts(self.O, self.V, *[
+            self.blocks[nm].data_ptr() if nm in self.blocks else None for nm in _lib.SO_BLOCKS])
+
+    def __getitem__(self, name):
+        return self.blocks[name]
+
+    def nbytes(self):
+        return sum(t.numel() * 16 for t in self.blocks.values())
+
+    @classmethod
+    def from_full(cls, A, O, blocks=ALL_BLOCKS):
+        """From a full antisymmetrized tensor A[n,n,n,n] = <pq||rs> (device)."""
+        n = A.shape[0]
+        self = cls(O, n - O, A.device, blocks)
+        A = A.contiguous()
+        _lib.check(_lib.load().apy_so_ints_from_full(n, _vp(A), ctypes.byref(self.struct), stream_of(A.device)),
+                   "apy_so_ints_from_full")
+        return self
+
+    @classmethod
+    def from_mo(cls, eri_mo, O, blocks=ALL_BLOCKS):
+        """From spatial chemists' MO integrals (pq|rs) over the active space
+        (utils.py:258-279), spin-expanded and antisymmetrized on the fly
+        (utils.py:317-365, ci_wfn.py:284, 312-333)."""
+        m = eri_mo.shape[0]
+        self = cls(O, 2 * m - O, eri_mo.device, blocks)
+        eri_mo = eri_mo.contiguous()
+        _lib.check(_lib.load().apy_so_ints_from_mo(m, _vp(eri_mo), ctypes.byref(self.struct),
+                                                   stream_of(eri_mo.device)), "apy_so_ints_from_mo")
+        return self
+
+    @classmethod
+    def synthetic(cls, O, V, seed, std=0.05, device="cuda", blocks=ALL_BLOCKS):
+        """Seeded synthetic blocks (generator of apyib_amd.synthetic)."""
+        self = cls(O, V, device, blocks)
+        self.add_synthetic(seed, tag=1, std=std, scale=1.0, accumulate=False)
+        return self
+
+    def add_synthetic(self, seed, tag=1, std=0.05, scale=1.0, accumulate=True, real_only=False):
+        _lib.check(_lib.load().apy_synth_so_ints(int(seed), int(tag), float(std), int(accumulate), float(scale),
+                                                 int(real_only), ctypes.byref(self.struct), stream_of(self.device)),
+                   "apy_synth_so_ints")
+        return self
+
+
+def so_fock_from_mo(f_mo):
+    """F_SO[p,q] = F_MO[p/2,q/2] delta(spin) (utils.py:283-313) on the device."""
+    m = f_mo.shape[0]
+    f_mo = f_mo.contiguous()
+    out = torch.empty((2 * m, 2 * m), dtype=CDT, device=f_mo.device)
+    _lib.check(_lib.load().apy_so_fock_from_mo(m, _vp(f_mo), _vp(out), stream_of(f_mo.device)),
+               "apy_so_fock_from_mo")
+    return out
+
+
+def _converged(iteration, dE, rms_list, params):
+    """ci_wfn.py:371-374: from iteration 2 on, |dE| < e_convergence and every
+    rms (a complex number: the square root of an un-conjugated sum of
+    squares) below d_convergence in NumPy's complex ordering."""
+    if iteration <= 1 or not abs(dE) < params['e_convergence']:
+        return False
+    return all(np.complex128(r) < params['d_convergence'] for r in rms_list)
+
+
+class SOCI:
+    """One spin-orbital CISD (singles=True) or CID problem on one device.
+
+    fock: [n,n] complex128 device tensor (spin-orbital Fock matrix)
+    eps:  [n] float64 orbital energies used in the denominators
+    ints: SOIntegrals with at least the blocks the method reads
+    fov_coupling: add the P(ij)P(ab) f_bj t_ia term the reference omits
+        (Hermitian operator; identical results for canonical HF orbitals).
+    """
+
+    def __init__(self, ints, fock, eps, singles=True, fov_coupling=False):
+        self.lib = _lib.load()
+        self.ints = ints
+        self.O, self.V = ints.O, ints.V
+        self.n = self.O + self.V
+        self.device = ints.device
+        self.singles = bool(singles)
+        self.fov = bool(fov_coupling)
+        self.fock = fock.to(self.device, CDT).contiguous()
+        self.eps = torch.as_tensor(eps, dtype=torch.float64).to(self.device).contiguous()
+        O, V = self.O, self.V
+        self.OV = O * V
+        self.L = O * V + O * O * V * V
+        self.hoo = torch.zeros(npair(O) * npair(O), dtype=CDT, device=self.device)
+        oooo = ints.blocks.get("oooo_p")
+        _lib.check(self.lib.apy_cisd_so_pair_operator(O, V, _vp(self.fock), _vp(oooo), _vp(self.hoo),
+                                                      stream_of(self.device)), "apy_cisd_so_pair_operator")
+        self.ws_bytes = int(self.lib.apy_cisd_so_workspace(O, V, int(self.singles)))
+        self.red_bytes = int(self.lib.apy_reduce_workspace(self.L, _lib.APY_MAXVEC))
+        self.energy_dev = torch.zeros(1, dtype=CDT, device=self.device)
+        self.zero_energy = torch.zeros(1, dtype=CDT, device=self.device)
+        self.scal = torch.zeros(_lib.APY_MAXVEC + 4, dtype=CDT, device=self.device)
+
+    # ------------------------------------------------------------ primitives
+    def new_vector(self):
+        return torch.zeros(self.L, dtype=CDT, device=self.device)
+
+    def split(self, t):
+        O, V = self.O, self.V
+        return t[:self.OV].view(O, V), t[self.OV:].view(O, O, V, V)
+
+    def step(self, t, t_new, out, energy=None, mode=_lib.APY_MODE_RESIDUAL, c0=1.0):
+        """One call of apy_cisd_so_step (residual update or sigma product)."""
+        c0 = complex(c0)
+        args = _lib.CisdSoArgs(
+            self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(), ctypes.pointer(self.ints.struct),
+            self.hoo.data_ptr(), int(self.singles), int(mode), int(self.fov),
+            (energy if energy is not None else self.energy_dev).data_ptr(), c0.real, c0.imag,
+            t.data_ptr(), t_new.data_ptr() if t_new is not None else None, out.data_ptr())
+        ws = workspace(self.ws_bytes, self.device)
+        _lib.check(self.lib.apy_cisd_so_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
+                   "apy_cisd_so_step")
+
+    def sigma(self, x, out, c0=1.0):
+        """out = H_N (c0, x): the CISD sigma with reference weight c0."""
+        self.step(x, None, out, mode=_lib.APY_MODE_SIGMA, c0=c0)
+
+    def energy(self, t, out=None):
+        out = self.energy_dev if out is None else out
+        ws = workspace(self.red_bytes, self.device)
+        _lib.check(self.lib.apy_cisd_so_energy(self.O, self.V, int(self.singles), _vp(self.fock),
+                                               _vp(self.ints["oovv"]), _vp(t), _vp(out), _vp(ws), self.red_bytes,
+                                               stream_of(self.device)), "apy_cisd_so_energy")
+        return out
+
+    def dots(self, xs, y, out, conj=True):
+        k = len(xs)
+        arr = (ctypes.c_void_p * k)(*[x.data_ptr() for x in xs])
+        ws = workspace(self.red_bytes, self.device)
+        _lib.check(self.lib.apy_zdot_many(k, arr, _vp(y), y.numel(), int(conj), _vp(out), _vp(ws), self.red_bytes,
+                                          stream_of(self.device)), "apy_zdot_many")
+
+    def combine(self, xs, coef, y):
+        k = len(xs)
+        arr = (ctypes.c_void_p * k)(*[x.data_ptr() for x in xs])
+        c = np.ascontiguousarray(np.asarray(coef, dtype=np.complex128)).view(np.float64)
+        _lib.check(self.lib.apy_zcombine(k, arr, c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), y.numel(),
+                                         _vp(y), stream_of(self.device)), "apy_zcombine")
+
+    def diffsq(self, x, y, out):
+        off = (ctypes.c_int64 * 3)(0, self.OV, self.L)
+        ws = workspace(self.red_bytes, self.device)
+        _lib.check(self.lib.apy_zdiffsq_segments(_vp(x), _vp(y), 2, off, _vp(out), _vp(ws), self.red_bytes,
+                                                 stream_of(self.device)), "apy_zdiffsq_segments")
+
+    # ---------------------------------------------------------------- solver
+    def initial_guess(self, t):
+        """t1 = F_vo^T / D_ia, t2 = <ab||ij> / D_ijab (ci_wfn.py:287, 290): one
+        residual update from zero amplitudes with E = 0."""
+        zero = self.new_vector()
+        scratch = self.new_vector()
+        self.step(zero, t, scratch, energy=self.zero_energy)
+        if not self.singles:
+            t[:self.OV].zero_()
+        return t
+
+    def solve(self, params, print_level=0, max_diis=7):
+        """The reference's iteration (ci_wfn.py:263-378 for CISD, :171-257
+        for CID).  Returns (E, t1, t2, iterations performed), E a Python
+        complex."""
+        dev = self.device
+        diis = bool(params['DIIS'])
+        nvec = max_diis + 1
+        t_cur = self.initial_guess(self.new_vector())
+        t_nxt = self.new_vector()
+        self.energy(t_cur)
+        E = complex(self.energy_dev.cpu()[0])
+        if diis:
+            Eh = torch.zeros((nvec, self.L), dtype=CDT, device=dev)
+            Th = torch.zeros((nvec, self.L), dtype=CDT, device=dev)
+            Bfull = {}
+            hist = []
+        else:
+            r = self.new_vector()
+        if print_level > 0:
+            print("\n Iter      E_elec(real)       E_elec(imaginary)        E(tot)           Delta_E(real)       "
+                  "Delta_E(imaginary)      RMS_T1(real)      RMS_T1(imaginary)      RMS_T2(real)      RMS_T2(imaginary)")
+        rms1 = rms2 = 0j
+        dE = 0j
+        it = 0
+        for it in range(1, params['max_iterations'] + 1):
+            E_old = E
+            if diis:
+                slot = (it - 1) % nvec
+                # drop the oldest while more than max_diis are stored (utils.py:109-112)
+                while len(hist) > max_diis:
+                    hist.pop(0)
+                self.step(t_cur, Th[slot], Eh[slot])
+                hist.append((it, slot))
+                # new row of B = E^H E (utils.py:121), cached for older pairs
+                self.dots([Eh[s] for _, s in hist], Eh[slot], self.scal)
+                row = self.scal[:len(hist)].cpu().numpy()
+                for (itm, _), b in zip(hist, row):
+                    Bfull[(itm, it)] = b
+                    Bfull[(it, itm)] = np.conj(b)
+                for key in [k for k in Bfull if k[0] < hist[0][0] or k[1] < hist[0][0]]:
+                    del Bfull[key]
+                nv = len(hist)
+                B = np.zeros((nv + 1, nv + 1), dtype=np.complex128)
+                for a_, (ia, _) in enumerate(hist):
+                    for b_, (ib, _) in enumerate(hist):
+                        B[a_, b_] = Bfull[(ia, ib)]
+                B[-1, :] = -1
+                B[:, -1] = -1
+                B[-1, -1] = 0
+                A = np.zeros(nv + 1)
+                A[-1] = -1
+                c = np.linalg.solve(B, A)                                       # utils.py:135
+                self.combine([Th[s] for _, s in hist], c[:-1], t_nxt)           # utils.py:138
+            else:
+                self.step(t_cur, t_nxt, r)
+            self.energy(t_nxt)
+            self.diffsq(t_cur, t_nxt, self.scal[1:3])
+            self.scal[0:1].copy_(self.energy_dev)
+            host = self.scal[:3].cpu().numpy()
+            E = complex(host[0])
+            rms1 = np.sqrt(host[1])
+            rms2 = np.sqrt(host[2])
+            dE = E_old - E
+            t_cur, t_nxt = t_nxt, t_cur
+            if print_level > 0:
+                print(" %02d %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f" % (
+                    it, E.real, E.imag, E.real, dE.real, dE.imag, rms1.real, rms1.imag, rms2.real, rms2.imag))
+            checks = [rms1, rms2] if self.singles else [rms2]
+            if _converged(it, dE, checks, params):
+                break
+            if it == params['max_iterations']:
+                if abs(dE) > params['e_convergence'] or rms2 > params['d_convergence']:
+                    print("Not converged.")
+        self.t = t_cur
+        t1, t2 = self.split(t_cur)
+        return E, t1, t2, it

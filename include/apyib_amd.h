@@ -74,6 +74,115 @@ typedef struct apy_gemm_desc {
 size_t apy_zgemm_workspace(const apy_gemm_desc* d);
 int apy_zgemm(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Antisymmetrized spin-orbital integral blocks <pq||rs> in the layouts the
+ * sigma kernels read.  O = occupied, V = virtual spin orbitals (n = O+V,
+ * occupied first), Po = O(O-1)/2, Pv = V(V-1)/2.  A pair index (p<q) of N
+ * orbitals is  p*(2N-p-1)/2 + (q-p-1)  (row-major upper triangle).
+ * The reference forms each block per term as ERI_SO[..] - ERI_SO.swapaxes
+ * (2,3)[..] (ci_wfn.py:312-333, 355); here each is built once.
+ * Any pointer may be NULL when the caller does not need that block.
+ * ---------------------------------------------------------------------- */
+typedef struct apy_so_ints {
+  int64_t O, V;
+  void* vvvv_p;  /* [Pv][Pv]      <ab||cd>, a<b rows, c<d columns  (ci_wfn.py:329)      */
+  void* ring;    /* [O][V][O][V]  ring[(j,b),(k,c)] = <kb||cj>      (ci_wfn.py:312,330-333) */
+  void* oooo_p;  /* [Po][Po]      <kl||ij>, (i<j) rows, (k<l) cols  (ci_wfn.py:328)      */
+  void* ovoo_p;  /* [O][V][Po]    <kb||ij>, i<j                     (ci_wfn.py:320-321)  */
+  void* vvvo_p;  /* [Pv][O][V]    <ab||cj> stored at [(a<b)][j][c]  (ci_wfn.py:322-323)  */
+  void* vovv;    /* [V][O][V][V]  <aj||cb>                          (ci_wfn.py:314)      */
+  void* ooov;    /* [O][O][O][V]  <kj||ib>                          (ci_wfn.py:315)      */
+  void* vvoo_t;  /* [O][O][V][V]  <ab||ij> stored at [i][j][a][b]   (ci_wfn.py:319)      */
+  void* oovv;    /* [O][O][V][V]  <ij||ab>                          (ci_wfn.py:355)      */
+} apy_so_ints;
+
+/* Number of complex elements of each block, in the order of the struct
+ * (vvvv_p, ring, oooo_p, ovoo_p, vvvo_p, vovv, ooov, vvoo_t, oovv). Host. */
+int apy_so_ints_sizes(int64_t O, int64_t V, int64_t sizes_out[9]);
+
+/* Blocks from a full antisymmetrized tensor A[n][n][n][n] = <pq||rs>. */
+int apy_so_ints_from_full(int64_t n, const void* A, const apy_so_ints* ints, void* stream);
+
+/* Blocks from spatial MO integrals in chemists' order, eri_mo[m][m][m][m]
+ * = (pq|rs) over the active orbitals (utils.py:258-279), fused with the
+ * spin expansion of compute_ERI_SO (utils.py:317-365), the Dirac swap
+ * (ci_wfn.py:284) and the antisymmetrization of each term.  SO p is
+ * spatial p/2 with spin p%2; O = occupied SOs.  Requires ints->O+ints->V
+ * == 2m. */
+int apy_so_ints_from_mo(int64_t m, const void* eri_mo, const apy_so_ints* ints, void* stream);
+
+/* Blocks of the seeded synthetic <pq||rs> (definition in
+ * apyib_amd/synthetic.py): std = complex std of the draw.  With
+ * accumulate = 1 the blocks are incremented by scale * (draw) instead of
+ * overwritten; real_only keeps the real part of the draw (a real,
+ * symmetric-orbital displacement).  tag selects the stream (1 = base). */
+int apy_synth_so_ints(uint64_t seed, uint64_t tag, double std, int accumulate, double scale,
+                      int real_only, const apy_so_ints* ints, void* stream);
+
+/* Spin-orbital Fock matrix F_SO[p,q] = F_MO[p/2,q/2] delta(spin p, spin q)
+ * (compute_F_SO, utils.py:283-313): f_mo is [m][m], f_so [2m][2m]. */
+int apy_so_fock_from_mo(int64_t m, const void* f_mo, void* f_so, void* stream);
+
+/* ------------------------------------------------------------------------
+ * CISD / CID in spin orbitals: one iteration of the reference's Jacobi
+ * step, or the plain sigma product (for Davidson).
+ *   residual mode (ci_wfn.py:309-337, CID :210-221):
+ *       r = sigma(1, t) - E t ;  t_new = t + r / D
+ *   sigma mode:  out = sigma(c0, t)   (out must not alias t)
+ * t and r are one vector [t1 (O*V) | t2 (O*O*V*V)], the reference's DIIS
+ * vector layout (ci_wfn.py:345-346); CID uses only the t2 part but keeps
+ * the same layout with a zero t1.  t2 must be antisymmetric in (i,j) and
+ * in (a,b); the diagonal entries (i==j or a==b) are never written.
+ * fov_coupling = 0 reproduces the reference operator, which omits the
+ * P(ij)P(ab) f_bj t_ia doubles-singles coupling (exact for canonical
+ * Hartree-Fock orbitals); 1 adds it, making the operator Hermitian.
+ * ---------------------------------------------------------------------- */
+#define APY_MODE_RESIDUAL 0
+#define APY_MODE_SIGMA 1
+
+typedef struct apy_cisd_so_args {
+  int64_t O, V;
+  const void* fock;        /* [n][n] spin-orbital Fock matrix           */
+  const double* eps;       /* [n] orbital energies for D (ci_wfn.py:268-271) */
+  const apy_so_ints* ints;
+  const void* hoo;         /* [Po][Po] from apy_cisd_so_pair_operator    */
+  int singles;             /* 1 CISD, 0 CID                             */
+  int mode;                /* APY_MODE_RESIDUAL or APY_MODE_SIGMA       */
+  int fov_coupling;
+  const void* energy;      /* device complex scalar E (residual mode)   */
+  double c0_re, c0_im;     /* driver weight (sigma mode; 1 in residual) */
+  const void* t;           /* [O*V + O^2*V^2] input amplitudes           */
+  void* t_new;             /* residual mode: t + r/D (may equal t)       */
+  void* out;               /* residual (residual mode) or sigma; != t    */
+} apy_cisd_so_args;
+
+/* Pair-space occupied operator: <kl||ij> plus the F_oo terms of the doubles
+ * residual (ci_wfn.py:326-328), [(i<j)][(k<l)]; built once per solve. */
+int apy_cisd_so_pair_operator(int64_t O, int64_t V, const void* fock, const void* oooo_p, void* hoo,
+                              void* stream);
+size_t apy_cisd_so_workspace(int64_t O, int64_t V, int singles);
+int apy_cisd_so_step(const apy_cisd_so_args* a, void* workspace, size_t ws_bytes, void* stream);
+
+/* E = sum_ia t1 f_ia + 1/4 sum t2 <ij||ab> (ci_wfn.py:355; CID :237 with
+ * singles = 0), written to the device complex scalar `energy`. */
+int apy_cisd_so_energy(int64_t O, int64_t V, int singles, const void* fock, const void* oovv, const void* t,
+                       void* energy, void* workspace, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Vector reductions used by DIIS and the convergence test.
+ * ---------------------------------------------------------------------- */
+#define APY_MAXVEC 16
+/* out[m] = sum_l conj?(x_m[l]) * y[l] for m < k   (utils.py:121) */
+int apy_zdot_many(int k, const void* const* xs, const void* y, int64_t len, int conj_x, void* out,
+                  void* workspace, size_t ws_bytes, void* stream);
+/* out[seg] = sum_l (x[l]-y[l])^2 over each of nseg segments [off[s], off[s+1])
+ * (un-conjugated, ci_wfn.py:361-365).  off is a host array of nseg+1. */
+int apy_zdiffsq_segments(const void* x, const void* y, int nseg, const int64_t* off, void* out,
+                         void* workspace, size_t ws_bytes, void* stream);
+/* y = sum_m c_m x_m for m < k, c host complex (re,im pairs) (utils.py:138) */
+int apy_zcombine(int k, const void* const* xs, const double* coef, int64_t len, void* y, void* stream);
+size_t apy_reduce_workspace(int64_t len, int k);
+
 #ifdef __cplusplus
 }
 #endif
