@@ -46,6 +46,7 @@ class GemmDesc(ctypes.Structure):
         ("batch", c_i64),
         ("reduce_batch", c_int),
         ("split_k", c_int),
+        ("tile", c_int),
     ]
 
 
@@ -74,6 +75,7 @@ class CisdSoArgs(ctypes.Structure):
         ("energy", c_vp),
         ("c0_re", c_dbl), ("c0_im", c_dbl),
         ("t", c_vp), ("t_new", c_vp), ("out", c_vp),
+        ("ev_begin", c_vp), ("ev_end", c_vp),
     ]
 
 
@@ -86,6 +88,10 @@ SIGNATURES = {
     "apy_version": (c_int, []),
     "apy_zgemm_workspace": (c_size, [ctypes.POINTER(GemmDesc)]),
     "apy_zgemm": (c_int, [ctypes.POINTER(GemmDesc), c_vp, c_size, c_vp]),
+    "apy_zgemm_plan": (c_int, [ctypes.POINTER(GemmDesc), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "apy_zgemm_tune": (c_int, [ctypes.POINTER(GemmDesc), c_vp, c_size, c_int, c_vp, ctypes.POINTER(c_int),
+                               ctypes.POINTER(c_int), ctypes.POINTER(c_dbl)]),
+    "apy_zgemm_tune_clear": (c_int, []),
     "apy_so_ints_sizes": (c_int, [c_i64, c_i64, _P_I64]),
     "apy_so_ints_from_full": (c_int, [c_i64, c_vp, ctypes.POINTER(SoInts), c_vp]),
     "apy_so_ints_from_mo": (c_int, [c_i64, c_vp, ctypes.POINTER(SoInts), c_vp]),
@@ -95,6 +101,7 @@ SIGNATURES = {
     "apy_cisd_so_pair_operator": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "apy_cisd_so_workspace": (c_size, [c_i64, c_i64, c_int]),
     "apy_cisd_so_step": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_vp]),
+    "apy_cisd_so_tune": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_int, c_vp]),
     "apy_cisd_so_energy": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_zdot_many": (c_int, [c_int, _P_VP, c_vp, c_i64, c_int, c_vp, c_vp, c_size, c_vp]),
     "apy_zdiffsq_segments": (c_int, [c_vp, c_vp, c_int, _P_I64, c_vp, c_vp, c_size, c_vp]),

@@ -576,7 +576,10 @@ int apy_cisd_so_step(const apy_cisd_so_args* A, void* workspace, size_t ws_bytes
   StepGemms G;
   step_gemms(O, V, singles, I, F, A->hoo, t, w, G);
   for (int q = 0; q < G.count; ++q) {
-    int s = apy_zgemm_launch(&G.g[q], w.gemm, w.gemm_bytes, st);
+    // G.g[0] is the particle-particle ladder, the dominant kernel.
+    int s = q == 0 ? apy_zgemm_launch(&G.g[q], w.gemm, w.gemm_bytes, st, (hipEvent_t)A->ev_begin,
+                                      (hipEvent_t)A->ev_end)
+                   : apy_zgemm_launch(&G.g[q], w.gemm, w.gemm_bytes, st);
     if (s != APY_OK) return s;
   }
 
@@ -604,6 +607,23 @@ int apy_cisd_so_step(const apy_cisd_so_args* A, void* workspace, size_t ws_bytes
     epi_singles<<<apy_grid1d(OV), 256, 0, st>>>(O, V, n, F, A->eps, (const double2*)A->energy, c0r, c0i, w.s1, t, tn,
                                                 out, A->mode);
     APY_LAUNCH_CHECK();
+  }
+  return APY_OK;
+}
+
+int apy_cisd_so_tune(const apy_cisd_so_args* A, void* workspace, size_t ws_bytes, int reps, void* stream) {
+  if (!A || !A->ints || !A->fock || !A->t || !A->hoo) return APY_E_ARG;
+  const int64_t O = A->O, V = A->V;
+  if (O < 2 || V < 2 || A->ints->O != O || A->ints->V != V) return APY_E_ARG;
+  const int singles = A->singles ? 1 : 0;
+  SigmaWS w = carve(O, V, singles, workspace);
+  if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
+  StepGemms G;
+  step_gemms(O, V, singles, A->ints, (const double2*)A->fock, A->hoo, (const double2*)A->t, w, G);
+  for (int q = 0; q < G.count; ++q) {
+    if (!G.g[q].a || !G.g[q].b) continue;
+    int s = apy_zgemm_tune_impl(&G.g[q], w.gemm, w.gemm_bytes, reps, (hipStream_t)stream, nullptr, nullptr, nullptr);
+    if (s != APY_OK) return s;
   }
   return APY_OK;
 }

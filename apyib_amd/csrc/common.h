@@ -47,5 +47,10 @@ inline unsigned apy_grid1d(int64_t n, int64_t cap = 65536) {
 }
 
 // Internal (C++) helpers shared between translation units.
-int apy_zgemm_launch(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, hipStream_t s);
+// ev0 / ev1 (optional) are recorded just before and after the main GEMM
+// kernel, excluding the split-K reduction.
+int apy_zgemm_launch(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, hipStream_t s,
+                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 size_t apy_zgemm_ws_bytes(const apy_gemm_desc* d);
+int apy_zgemm_tune_impl(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, int reps, hipStream_t s,
+                        int* best_cfg, int* best_split, double* best_ms);

@@ -5,27 +5,38 @@
 // 8 real flops per complex MAC, executed with v_mfma_f64_16x16x4f64.
 //
 // Tiling: 256 threads = 4 waves arranged 2x2; each wave owns a WM x WN
-// complex output tile built from 16x16 MFMA blocks.  K is staged through
-// LDS BK complex elements at a time, split into separate real and imaginary
-// planes.  Global loads for stage s+1 are issued into registers before the
-// MFMAs of stage s run (software pipelining with a single LDS buffer).
+// complex output tile built from 16x16 MFMA blocks.  K is staged through a
+// double-buffered LDS tile, BK complex elements at a time, with two register
+// stages in front of it: while the MFMAs consume LDS buffer s, stage s+1
+// waits in registers for the other buffer and stage s+2 is in flight from
+// memory; each stage needs a single barrier.
 //
 // MFMA f64 16x16x4 lane layout (checked on gfx950):
 //   A operand: lane l holds A[l%16][l/16]       (16 x 4)
 //   B operand: lane l holds B[l/16][l%16]       (4 x 16)
 //   D result : lane l, element r holds D[l/16 + 4r][l%16]
 //
-// Work decomposition: the flat block id is remapped so that consecutive
-// logical tiles land on the same XCD (workgroups are dealt round-robin
-// over the 8 XCDs), with the M tile fastest: the blocks that share a B
-// tile share that XCD's L2.
+// Work decomposition: (M tile, N tile, K slice, batch) units on a flat grid
+// whose block id is remapped so consecutive logical units land on the same
+// XCD (workgroups are dealt round-robin over the 8 XCDs), M tile fastest:
+// blocks sharing a B tile share that XCD's L2.  K slices (split-K) write
+// partial products to a workspace that one reduction kernel sums in a fixed
+// order, so results are deterministic for a given plan.
+//
+// Plans (tile config, split) come from a cost model over the device's
+// resident-block count, or from the tune cache filled by apy_zgemm_tune,
+// which times the candidate plans on the caller's operands.
 #include "common.h"
+
 #include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
 
 namespace {
 
-constexpr int BK = 16;        // complex K elements per LDS stage
-constexpr int BKP = BK + 1;   // padded LDS row length (doubles) for k-contiguous tiles
 constexpr int NTHR = 256;
 
 struct GemmArgs {
@@ -49,17 +60,22 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t total) {
   return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
-template <int BM, int BN, int WM, int WN, bool A_MK, bool B_NK>
+template <int BM, int BN, int WM, int WN, int BK, bool A_MK, bool B_NK>
 __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
   constexpr int RB = WM / 16, CB = WN / 16;
+  constexpr int BKP = BK + 1;  // padded row (complex elements) of k-contiguous LDS tiles
   constexpr int A_PER = BM * BK / NTHR;
   constexpr int B_PER = BN * BK / NTHR;
+  constexpr int A_LDS = BM * BKP;
   constexpr int B_LDS = B_NK ? BN * BKP : BK * BN;
   static_assert(BM == 2 * WM && BN == 2 * WN, "2x2 wave grid");
   static_assert(A_PER >= 1 && B_PER >= 1, "tile too small");
 
-  __shared__ double As_re[BM * BKP], As_im[BM * BKP];
-  __shared__ double Bs_re[B_LDS], Bs_im[B_LDS];
+  // Interleaved (re, im) tiles: one 16-byte LDS read yields both planes of
+  // an MFMA operand.  Row strides of BKP complex (272 B for BK=16) keep the
+  // 8-lane groups of a 128-bit LDS access on distinct banks.
+  __shared__ double2 As[2][A_LDS];
+  __shared__ double2 Bs[2][B_LDS];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -85,92 +101,101 @@ __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
 #pragma unroll
     for (int j = 0; j < CB; ++j) { acc_re[i][j] = (v4d){0, 0, 0, 0}; acc_im[i][j] = (v4d){0, 0, 0, 0}; }
 
-  double2 ra[A_PER], rb[B_PER];
+  // Two register stages: stage s+2 is in flight while stage s+1 waits in
+  // registers and stage s is consumed from LDS.
+  double2 ra[2][A_PER], rb[2][B_PER];
   const double csa = p.conj_a ? -1.0 : 1.0, csb = p.conj_b ? -1.0 : 1.0;
 
-  auto load = [&](int64_t k0) {
+  // `set` / `buf` arguments are std::integral_constant so that every
+  // register-array index is a compile-time constant (no scratch memory).
+  auto load = [&](int64_t k0, auto setc) {
+    constexpr int set = decltype(setc)::value;
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
-      int idx = tid + NTHR * q;
+      const int idx = tid + NTHR * q;
       int r, kk;
       if (A_MK) { r = idx / BK; kk = idx % BK; } else { kk = idx / BM; r = idx % BM; }
-      int64_t gm = m0 + r, gk = k0 + kk;
+      const int64_t gm = m0 + r, gk = k0 + kk;
       double2 v = make_double2(0.0, 0.0);
       if (gm < p.m && gk < kend) v = A_MK ? A[gm * p.lda + gk] : A[gk * p.lda + gm];
       v.y *= csa;
-      ra[q] = v;
+      ra[set][q] = v;
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
-      int idx = tid + NTHR * q;
+      const int idx = tid + NTHR * q;
       int c, kk;
       if (B_NK) { c = idx / BK; kk = idx % BK; } else { kk = idx / BN; c = idx % BN; }
-      int64_t gn = n0 + c, gk = k0 + kk;
+      const int64_t gn = n0 + c, gk = k0 + kk;
       double2 v = make_double2(0.0, 0.0);
       if (gn < p.n && gk < kend) v = B_NK ? B[gn * p.ldb + gk] : B[gk * p.ldb + gn];
       v.y *= csb;
-      rb[q] = v;
+      rb[set][q] = v;
     }
   };
-  auto store = [&]() {
+  auto store = [&](auto bufc) {
+    constexpr int buf = decltype(bufc)::value, set = buf;
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
-      int idx = tid + NTHR * q;
+      const int idx = tid + NTHR * q;
       int r, kk;
       if (A_MK) { r = idx / BK; kk = idx % BK; } else { kk = idx / BM; r = idx % BM; }
-      As_re[r * BKP + kk] = ra[q].x;
-      As_im[r * BKP + kk] = ra[q].y;
+      As[buf][r * BKP + kk] = ra[set][q];
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
-      int idx = tid + NTHR * q;
+      const int idx = tid + NTHR * q;
       int c, kk;
       if (B_NK) { c = idx / BK; kk = idx % BK; } else { kk = idx / BN; c = idx % BN; }
-      int o = B_NK ? c * BKP + kk : kk * BN + c;
-      Bs_re[o] = rb[q].x;
-      Bs_im[o] = rb[q].y;
+      Bs[buf][B_NK ? c * BKP + kk : kk * BN + c] = rb[set][q];
     }
   };
 
-  if (kbeg < kend) load(kbeg);
-  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
-    __syncthreads();  // previous stage's MFMA reads are done
-    store();
-    __syncthreads();
-    if (k0 + BK < kend) load(k0 + BK);  // in flight while the MFMAs below run
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  const int64_t nstages = (kend - kbeg + BK - 1) / BK;
+  if (nstages > 0) load(kbeg, I0{});
+  if (nstages > 1) load(kbeg + BK, I1{});
+  if (nstages > 0) store(I0{});
+  __syncthreads();
+  auto stage = [&](int64_t st, auto curc) {
+    constexpr int cur = decltype(curc)::value;
+    using Other = std::integral_constant<int, cur ^ 1>;
+    if (st + 2 < nstages) load(kbeg + (st + 2) * BK, curc);  // set `cur` was stored last stage
+    const double2* Ab = As[cur];
+    const double2* Bb = Bs[cur];
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kk = ks * 4 + (lane >> 4);
-      double are[RB], aim[RB], naim[RB], bre[CB], bim[CB];
+      double2 av[RB], bv[CB];
 #pragma unroll
-      for (int i = 0; i < RB; ++i) {
-        int row = wm * WM + i * 16 + (lane & 15);
-        are[i] = As_re[row * BKP + kk];
-        aim[i] = As_im[row * BKP + kk];
-        naim[i] = -aim[i];
-      }
+      for (int i = 0; i < RB; ++i) av[i] = Ab[(wm * WM + i * 16 + (lane & 15)) * BKP + kk];
 #pragma unroll
       for (int j = 0; j < CB; ++j) {
-        int col = wn * WN + j * 16 + (lane & 15);
-        int o = B_NK ? col * BKP + kk : kk * BN + col;
-        bre[j] = Bs_re[o];
-        bim[j] = Bs_im[o];
+        const int col = wn * WN + j * 16 + (lane & 15);
+        bv[j] = Bb[B_NK ? col * BKP + kk : kk * BN + col];
       }
 #pragma unroll
       for (int i = 0; i < RB; ++i)
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
-          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], acc_re[i][j], 0, 0, 0);
-          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], acc_im[i][j], 0, 0, 0);
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].x, acc_re[i][j], 0, 0, 0);
+          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].y, acc_im[i][j], 0, 0, 0);
         }
 #pragma unroll
       for (int i = 0; i < RB; ++i)
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
-          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(naim[i], bim[j], acc_re[i][j], 0, 0, 0);
-          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], acc_im[i][j], 0, 0, 0);
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[i].y, bv[j].y, acc_re[i][j], 0, 0, 0);
+          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].y, bv[j].x, acc_im[i][j], 0, 0, 0);
         }
     }
+    if (st + 1 < nstages) store(Other{});
+    __syncthreads();
+  };
+  for (int64_t st = 0; st < nstages; st += 2) {
+    stage(st, I0{});
+    if (st + 1 < nstages) stage(st + 1, I1{});
   }
 
   // Epilogue.
@@ -182,17 +207,17 @@ __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
     for (int j = 0; j < CB; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int64_t row = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
-        int64_t col = n0 + wn * WN + j * 16 + (lane & 15);
+        const int64_t row = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
+        const int64_t col = n0 + wn * WN + j * 16 + (lane & 15);
         if (row < p.m && col < p.n) {
-          double vr = acc_re[i][j][r], vi = acc_im[i][j][r];
+          const double vr = acc_re[i][j][r], vi = acc_im[i][j][r];
           double o_r = p.alr * vr - p.ali * vi, o_i = p.alr * vi + p.ali * vr;
           if (W) {
             W[row * p.n + col] = make_double2(o_r, o_i);
           } else {
             double2* cp = Cb + row * p.ldc + col;
             if (p.use_beta) {
-              double2 old = *cp;
+              const double2 old = *cp;
               o_r += p.ber * old.x - p.bei * old.y;
               o_i += p.ber * old.y + p.bei * old.x;
             }
@@ -209,14 +234,16 @@ __global__ __launch_bounds__(256) void zgemm_reduce(const double2* __restrict__ 
   const int64_t mn = m * n;
   const int64_t total = ngroups * mn;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t g = t / mn, e = t % mn;
-    int64_t row = e / n, col = e % n;
-    const double2* w = ws + g * group * mn + e;
+    const int64_t g = t / mn, e = t % mn;
+    const int64_t row = e / n, col = e % n;
     double sr = 0.0, si = 0.0;
-    for (int64_t q = 0; q < group; ++q) { double2 v = w[q * mn]; sr += v.x; si += v.y; }
+    if (group > 0) {
+      const double2* w = ws + g * group * mn + e;
+      for (int64_t q = 0; q < group; ++q) { const double2 v = w[q * mn]; sr += v.x; si += v.y; }
+    }
     double2* cp = c + g * sc + row * ldc + col;
     if (use_beta) {
-      double2 old = *cp;
+      const double2 old = *cp;
       sr += ber * old.x - bei * old.y;
       si += ber * old.y + bei * old.x;
     }
@@ -224,87 +251,226 @@ __global__ __launch_bounds__(256) void zgemm_reduce(const double2* __restrict__ 
   }
 }
 
+// ------------------------------------------------------------ tile configs
+struct Cfg {
+  int bm, bn, wm, wn, bk;
+  double eff;  // relative MFMA efficiency used by the cost model
+};
+constexpr int NCFG = 7;
+constexpr Cfg CFGS[NCFG + 1] = {
+    {0, 0, 0, 0, 0, 0.0},          // 0: automatic
+    {64, 64, 32, 32, 16, 0.68},    // 1
+    {32, 64, 16, 32, 16, 0.70},    // 2
+    {64, 128, 32, 64, 16, 0.58},   // 3
+    {128, 64, 64, 32, 16, 0.50},   // 4
+    {96, 64, 48, 32, 16, 0.58},    // 5
+    {64, 64, 32, 32, 32, 0.60},    // 6
+    {32, 32, 16, 16, 16, 0.70},    // 7
+};
+
+template <int CI, bool AM, bool BN_>
+struct KernelOf {
+  static constexpr Cfg c = CFGS[CI];
+  static void* ptr() { return (void*)zgemm_mfma<c.bm, c.bn, c.wm, c.wn, c.bk, AM, BN_>; }
+  static void launch(const GemmArgs& a, hipStream_t s) {
+    zgemm_mfma<c.bm, c.bn, c.wm, c.wn, c.bk, AM, BN_><<<dim3((unsigned)a.total), dim3(NTHR), 0, s>>>(a);
+  }
+};
+
+template <int CI>
+void launch_layout(const GemmArgs& a, bool am, bool bnk, hipStream_t s) {
+  if (am && bnk) KernelOf<CI, true, true>::launch(a, s);
+  else if (am) KernelOf<CI, true, false>::launch(a, s);
+  else if (bnk) KernelOf<CI, false, true>::launch(a, s);
+  else KernelOf<CI, false, false>::launch(a, s);
+}
+
+template <int CI>
+void* kernel_ptr(bool am, bool bnk) {
+  if (am && bnk) return KernelOf<CI, true, true>::ptr();
+  if (am) return KernelOf<CI, true, false>::ptr();
+  if (bnk) return KernelOf<CI, false, true>::ptr();
+  return KernelOf<CI, false, false>::ptr();
+}
+
+void launch_cfg(int cfg, const GemmArgs& a, bool am, bool bnk, hipStream_t s) {
+  switch (cfg) {
+    case 1: launch_layout<1>(a, am, bnk, s); break;
+    case 2: launch_layout<2>(a, am, bnk, s); break;
+    case 3: launch_layout<3>(a, am, bnk, s); break;
+    case 4: launch_layout<4>(a, am, bnk, s); break;
+    case 5: launch_layout<5>(a, am, bnk, s); break;
+    case 6: launch_layout<6>(a, am, bnk, s); break;
+    default: launch_layout<7>(a, am, bnk, s); break;
+  }
+}
+
+void* cfg_kernel(int cfg, bool am, bool bnk) {
+  switch (cfg) {
+    case 1: return kernel_ptr<1>(am, bnk);
+    case 2: return kernel_ptr<2>(am, bnk);
+    case 3: return kernel_ptr<3>(am, bnk);
+    case 4: return kernel_ptr<4>(am, bnk);
+    case 5: return kernel_ptr<5>(am, bnk);
+    case 6: return kernel_ptr<6>(am, bnk);
+    default: return kernel_ptr<7>(am, bnk);
+  }
+}
+
+// Device facts, queried once per device.
+struct DevInfo {
+  int cus = 256;
+  int occ[NCFG + 1][4] = {};
+  bool ready = false;
+};
+std::mutex g_mu;
+std::map<int, DevInfo> g_dev;
+
+using Key = std::tuple<int64_t, int64_t, int64_t, int, int, int64_t, int, int>;  // m,n,k,al,bl,batch,reduce,device
+std::map<Key, std::pair<int, int>> g_tuned;
+
+int current_device() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess) d = 0;
+  return d;
+}
+
+const DevInfo& dev_info() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int d = current_device();
+  DevInfo& I = g_dev[d];
+  if (!I.ready) {
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, d) == hipSuccess) I.cus = pr.multiProcessorCount;
+    for (int c = 1; c <= NCFG; ++c)
+      for (int l = 0; l < 4; ++l) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)cfg_kernel(c, l & 1, l >> 1), NTHR, 0) !=
+                hipSuccess ||
+            nb < 1)
+          nb = 1;
+        I.occ[c][l] = nb;
+      }
+    I.ready = true;
+  }
+  return I;
+}
+
+Key key_of(const apy_gemm_desc* d) {
+  return Key(d->m, d->n, d->k, d->a_layout, d->b_layout, d->batch, d->reduce_batch, current_device());
+}
+
 struct Plan {
-  int cfg;        // 0: 64x64 tiles, 1: 32x64 tiles
-  int bm, bn;
+  int cfg;
   int split;
   int64_t kchunk;
   int64_t mtiles, ntiles, total;
   bool use_ws;
   int64_t npart;
+  double cost;
 };
 
-Plan make_plan(const apy_gemm_desc* d) {
+// Largest split the workspace query reserves for: partial sums are capped
+// at ~768 MB; small outputs with long K (a few tiles) may use up to 256
+// slices so that the grid still fills the chip.
+int max_split(const apy_gemm_desc* d) {
+  const double mn_bytes = (double)d->m * (double)d->n * 16.0 * (double)d->batch;
+  const double cap = mn_bytes < 4.0e6 ? 256.0 : 16.0;
+  int s = (int)std::min(cap, std::max(1.0, std::floor(768.0e6 / std::max(mn_bytes, 1.0))));
+  return std::max(1, s);
+}
+
+Plan plan_for(const apy_gemm_desc* d, int cfg, int split) {
+  const Cfg& c = CFGS[cfg];
   Plan pl;
-  pl.cfg = (d->m <= 32) ? 1 : 0;
-  pl.bm = pl.cfg == 1 ? 32 : 64;
-  pl.bn = 64;
-  pl.mtiles = (d->m + pl.bm - 1) / pl.bm;
-  pl.ntiles = (d->n + pl.bn - 1) / pl.bn;
-  int64_t tiles = pl.mtiles * pl.ntiles * (d->reduce_batch ? d->batch : d->batch);
-  int split = d->split_k;
-  if (split <= 0) {
-    // Aim for >= ~2048 resident-work units on 256 CUs while keeping each
-    // K slice at least 8 stages long.
-    split = 1;
-    int64_t max_split = std::max<int64_t>(1, d->k / (8 * BK));
-    while (tiles * split < 2048 && split < max_split && split < 64) ++split;
-  }
+  pl.cfg = cfg;
+  pl.mtiles = (d->m + c.bm - 1) / c.bm;
+  pl.ntiles = (d->n + c.bn - 1) / c.bn;
   int64_t kchunk = (d->k + split - 1) / split;
-  kchunk = ((kchunk + BK - 1) / BK) * BK;
-  if (kchunk < BK) kchunk = BK;
+  kchunk = ((kchunk + c.bk - 1) / c.bk) * c.bk;
+  if (kchunk < c.bk) kchunk = c.bk;
   split = (int)((d->k + kchunk - 1) / kchunk);
   if (split < 1) split = 1;
   pl.split = split;
   pl.kchunk = kchunk;
   pl.total = pl.mtiles * pl.ntiles * split * d->batch;
-  pl.use_ws = (split > 1) || d->reduce_batch;
-  pl.npart = split * d->batch;
+  pl.use_ws = split > 1 || d->reduce_batch;
+  pl.npart = (int64_t)split * d->batch;
+  // Cost model: rounds of resident blocks x stages per block x stage time,
+  // plus the partial-sum traffic of the reduction.
+  const DevInfo& I = dev_info();
+  const int lay = (d->a_layout == APY_MK ? 1 : 0) | (d->b_layout == APY_NK ? 2 : 0);
+  const double slots = (double)I.cus * (double)I.occ[cfg][lay];
+  const double rounds = std::ceil((double)pl.total / slots);
+  const double stages = std::ceil((double)kchunk / c.bk);
+  const double peak_cu = 77.0e12 / 256.0;
+  const double t_stage = (double)I.occ[cfg][lay] * c.bm * c.bn * c.bk * 8.0 / (peak_cu * c.eff);
+  double t = rounds * stages * t_stage + 3.0e-6;
+  if (pl.use_ws) t += (double)(pl.npart + 2) * d->m * d->n * 16.0 / 4.0e12 + 4.0e-6;
+  pl.cost = t;
   return pl;
 }
 
-template <int BM, int BN, int WM, int WN>
-void launch_cfg(const GemmArgs& a, int a_layout, int b_layout, hipStream_t s) {
-  dim3 grid((unsigned)a.total), block(NTHR);
-  if (a_layout == APY_MK && b_layout == APY_NK) zgemm_mfma<BM, BN, WM, WN, true, true><<<grid, block, 0, s>>>(a);
-  else if (a_layout == APY_MK) zgemm_mfma<BM, BN, WM, WN, true, false><<<grid, block, 0, s>>>(a);
-  else if (b_layout == APY_NK) zgemm_mfma<BM, BN, WM, WN, false, true><<<grid, block, 0, s>>>(a);
-  else zgemm_mfma<BM, BN, WM, WN, false, false><<<grid, block, 0, s>>>(a);
+bool cfg_fits(const apy_gemm_desc* d, int cfg) {
+  const Cfg& c = CFGS[cfg];
+  // Tiles much larger than the matrix waste most of the MFMA work.
+  if (c.bm > 32 && d->m <= c.bm / 2) return false;
+  if (c.bn > 32 && d->n <= c.bn / 2) return false;
+  return true;
 }
 
-}  // namespace
-
-size_t apy_zgemm_ws_bytes(const apy_gemm_desc* d) {
-  if (!d || d->m <= 0 || d->n <= 0 || d->k <= 0 || d->batch <= 0) return 0;
-  Plan pl = make_plan(d);
-  if (!pl.use_ws) return 0;
-  return (size_t)pl.npart * (size_t)d->m * (size_t)d->n * sizeof(double2);
-}
-
-int apy_zgemm_launch(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, hipStream_t s) {
-  if (!d) return APY_E_ARG;
-  if (d->m < 0 || d->n < 0 || d->k < 0 || d->batch < 1) return APY_E_ARG;
-  if (d->a_layout != APY_MK && d->a_layout != APY_KM) return APY_E_ARG;
-  if (d->b_layout != APY_KN && d->b_layout != APY_NK) return APY_E_ARG;
-  if (d->m == 0 || d->n == 0) return APY_OK;
-  if (!d->c || ((uintptr_t)d->c & 15)) return APY_E_ARG;
-  if (d->k > 0 && (!d->a || !d->b || ((uintptr_t)d->a & 15) || ((uintptr_t)d->b & 15))) return APY_E_ARG;
-  const int use_beta = (d->beta_re != 0.0 || d->beta_im != 0.0) ? 1 : 0;
-  if (d->k == 0) {
-    // An empty sum contributes zero: C = beta*C.
-    int64_t nb = d->reduce_batch ? 1 : d->batch;
-    int64_t tot = nb * d->m * d->n;
-    // Reuse the reduce kernel with an empty group.
-    zgemm_reduce<<<apy_grid1d(tot), 256, 0, s>>>(nullptr, 0, nb, d->m, d->n, (double2*)d->c, d->ldc, d->stride_c,
-                                                 d->beta_re, d->beta_im, use_beta);
-    APY_LAUNCH_CHECK();
-    return APY_OK;
+Plan model_plan(const apy_gemm_desc* d) {
+  Plan best;
+  best.cost = 1e300;
+  best.cfg = 0;
+  const int smax = max_split(d);
+  for (int cfg = 1; cfg <= NCFG; ++cfg) {
+    if (!cfg_fits(d, cfg) && cfg != 7) continue;
+    for (int s = 1; s <= smax; ++s) {
+      if (s > 1 && (d->k + s - 1) / s < 4 * CFGS[cfg].bk) break;
+      Plan p = plan_for(d, cfg, s);
+      if (p.cost < best.cost) best = p;
+    }
   }
-  Plan pl = make_plan(d);
-  size_t need = pl.use_ws ? (size_t)pl.npart * (size_t)d->m * (size_t)d->n * sizeof(double2) : 0;
-  if (need > ws_bytes || (need && !workspace)) return APY_E_WORKSPACE;
-  if (pl.total > 0x7fffffffLL) return APY_E_ARG;
+  if (best.cfg == 0) best = plan_for(d, 7, 1);
+  return best;
+}
 
+Plan make_plan(const apy_gemm_desc* d) {
+  if (d->tile >= 1 && d->tile <= NCFG) {
+    int s = d->split_k > 0 ? std::min(d->split_k, max_split(d)) : 0;
+    if (s > 0) return plan_for(d, d->tile, s);
+    Plan best;
+    best.cost = 1e300;
+    for (int q = 1; q <= max_split(d); ++q) {
+      Plan p = plan_for(d, d->tile, q);
+      if (p.cost < best.cost) best = p;
+    }
+    return best;
+  }
+  int tcfg = 0, tsplit = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tuned.find(key_of(d));
+    if (it != g_tuned.end()) { tcfg = it->second.first; tsplit = it->second.second; }
+  }
+  if (tcfg > 0) return plan_for(d, tcfg, d->split_k > 0 ? std::min(d->split_k, max_split(d)) : tsplit);
+  if (d->split_k > 0) {
+    Plan m = model_plan(d);
+    return plan_for(d, m.cfg, std::min(d->split_k, max_split(d)));
+  }
+  return model_plan(d);
+}
+
+size_t ws_need(const apy_gemm_desc* d, const Plan& pl) {
+  return pl.use_ws ? (size_t)pl.npart * (size_t)d->m * (size_t)d->n * sizeof(double2) : 0;
+}
+
+int run_plan(const apy_gemm_desc* d, const Plan& pl, void* workspace, size_t ws_bytes, hipStream_t s,
+             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+  const int use_beta = (d->beta_re != 0.0 || d->beta_im != 0.0) ? 1 : 0;
+  if (ws_need(d, pl) > ws_bytes || (ws_need(d, pl) && !workspace)) return APY_E_WORKSPACE;
+  if (pl.total > 0x7fffffffLL) return APY_E_ARG;
   GemmArgs a;
   a.m = d->m; a.n = d->n; a.k = d->k;
   a.a = (const double2*)d->a; a.lda = d->lda; a.sa = d->stride_a;
@@ -315,20 +481,111 @@ int apy_zgemm_launch(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, h
   a.batch = d->batch; a.split = pl.split; a.kchunk = pl.kchunk;
   a.mtiles = pl.mtiles; a.ntiles = pl.ntiles; a.total = pl.total;
   a.ws = pl.use_ws ? (double2*)workspace : nullptr;
-
-  if (pl.cfg == 1) launch_cfg<32, 64, 16, 32>(a, d->a_layout, d->b_layout, s);
-  else launch_cfg<64, 64, 32, 32>(a, d->a_layout, d->b_layout, s);
+  if (ev0) APY_HIP_TRY(hipEventRecord(ev0, s));
+  launch_cfg(pl.cfg, a, d->a_layout == APY_MK, d->b_layout == APY_NK, s);
   APY_LAUNCH_CHECK();
-
+  if (ev1) APY_HIP_TRY(hipEventRecord(ev1, s));
   if (pl.use_ws) {
-    int64_t group = d->reduce_batch ? pl.npart : pl.split;
-    int64_t ngroups = d->reduce_batch ? 1 : d->batch;
-    int64_t tot = ngroups * d->m * d->n;
+    const int64_t group = d->reduce_batch ? pl.npart : pl.split;
+    const int64_t ngroups = d->reduce_batch ? 1 : d->batch;
+    const int64_t tot = ngroups * d->m * d->n;
     zgemm_reduce<<<apy_grid1d(tot), 256, 0, s>>>((const double2*)workspace, group, ngroups, d->m, d->n,
                                                  (double2*)d->c, d->ldc, d->stride_c, d->beta_re, d->beta_im,
                                                  use_beta);
     APY_LAUNCH_CHECK();
   }
+  return APY_OK;
+}
+
+int check_desc(const apy_gemm_desc* d) {
+  if (!d) return APY_E_ARG;
+  if (d->m < 0 || d->n < 0 || d->k < 0 || d->batch < 1) return APY_E_ARG;
+  if (d->a_layout != APY_MK && d->a_layout != APY_KM) return APY_E_ARG;
+  if (d->b_layout != APY_KN && d->b_layout != APY_NK) return APY_E_ARG;
+  if (d->tile < 0 || d->tile > NCFG || d->split_k < 0) return APY_E_ARG;
+  if (d->m > 0 && d->n > 0 && (!d->c || ((uintptr_t)d->c & 15))) return APY_E_ARG;
+  if (d->m > 0 && d->n > 0 && d->k > 0 && (!d->a || !d->b || ((uintptr_t)d->a & 15) || ((uintptr_t)d->b & 15)))
+    return APY_E_ARG;
+  return APY_OK;
+}
+
+}  // namespace
+
+size_t apy_zgemm_ws_bytes(const apy_gemm_desc* d) {
+  if (!d || d->m <= 0 || d->n <= 0 || d->k <= 0 || d->batch <= 0) return 0;
+  const int smax = max_split(d);
+  if (smax <= 1 && !d->reduce_batch) return 0;
+  return (size_t)smax * (size_t)d->batch * (size_t)d->m * (size_t)d->n * sizeof(double2);
+}
+
+int apy_zgemm_launch(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, hipStream_t s, hipEvent_t ev0,
+                     hipEvent_t ev1) {
+  int st = check_desc(d);
+  if (st != APY_OK) return st;
+  if (d->m == 0 || d->n == 0) return APY_OK;
+  if (d->k == 0) {
+    // An empty sum contributes zero: C = beta*C.
+    const int use_beta = (d->beta_re != 0.0 || d->beta_im != 0.0) ? 1 : 0;
+    const int64_t nb = d->reduce_batch ? 1 : d->batch;
+    zgemm_reduce<<<apy_grid1d(nb * d->m * d->n), 256, 0, s>>>(nullptr, 0, nb, d->m, d->n, (double2*)d->c, d->ldc,
+                                                               d->stride_c, d->beta_re, d->beta_im, use_beta);
+    APY_LAUNCH_CHECK();
+    return APY_OK;
+  }
+  return run_plan(d, make_plan(d), workspace, ws_bytes, s, ev0, ev1);
+}
+
+int apy_zgemm_tune_impl(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, int reps, hipStream_t s,
+                        int* best_cfg, int* best_split, double* best_ms) {
+  int st = check_desc(d);
+  if (st != APY_OK) return st;
+  if (d->m == 0 || d->n == 0 || d->k == 0) return APY_OK;
+  if (reps < 1) reps = 1;
+  hipEvent_t e0, e1;
+  APY_HIP_TRY(hipEventCreate(&e0));
+  APY_HIP_TRY(hipEventCreate(&e1));
+  double bms = 1e300;
+  int bc = 0, bs = 0;
+  const int smax = max_split(d);
+  for (int cfg = 1; cfg <= NCFG; ++cfg) {
+    if (!cfg_fits(d, cfg)) continue;
+    // the model's best split for this tile, and its neighbours
+    Plan m;
+    m.cost = 1e300;
+    for (int q = 1; q <= smax; ++q) {
+      if (q > 1 && (d->k + q - 1) / q < 4 * CFGS[cfg].bk) break;
+      Plan p = plan_for(d, cfg, q);
+      if (p.cost < m.cost) m = p;
+    }
+    const int cands[3] = {m.split, std::max(1, m.split / 2), std::min(smax, m.split * 2)};
+    for (int ci = 0; ci < 3; ++ci) {
+      if (ci > 0 && cands[ci] == cands[0]) continue;
+      if (ci == 2 && cands[2] == cands[1]) continue;
+      Plan p = plan_for(d, cfg, cands[ci]);
+      if (ws_need(d, p) > ws_bytes) continue;
+      st = run_plan(d, p, workspace, ws_bytes, s);  // warm-up
+      if (st != APY_OK) break;
+      APY_HIP_TRY(hipEventRecord(e0, s));
+      for (int r = 0; r < reps && st == APY_OK; ++r) st = run_plan(d, p, workspace, ws_bytes, s);
+      APY_HIP_TRY(hipEventRecord(e1, s));
+      APY_HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0.f;
+      APY_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      if (st != APY_OK) break;
+      if (ms / reps < bms) { bms = ms / reps; bc = cfg; bs = p.split; }
+    }
+    if (st != APY_OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != APY_OK) return st;
+  if (bc > 0) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_tuned[key_of(d)] = std::make_pair(bc, bs);
+  }
+  if (best_cfg) *best_cfg = bc;
+  if (best_split) *best_split = bs;
+  if (best_ms) *best_ms = bms;
   return APY_OK;
 }
 
@@ -340,6 +597,31 @@ size_t apy_zgemm_workspace(const apy_gemm_desc* d) { return apy_zgemm_ws_bytes(d
 
 int apy_zgemm(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, void* stream) {
   return apy_zgemm_launch(d, workspace, ws_bytes, (hipStream_t)stream);
+}
+
+int apy_zgemm_plan(const apy_gemm_desc* d, int* tile, int* split) {
+  int st = check_desc(d);
+  if (st != APY_OK) return st;
+  if (d->m == 0 || d->n == 0 || d->k == 0) {
+    if (tile) *tile = 0;
+    if (split) *split = 0;
+    return APY_OK;
+  }
+  Plan p = make_plan(d);
+  if (tile) *tile = p.cfg;
+  if (split) *split = p.split;
+  return APY_OK;
+}
+
+int apy_zgemm_tune(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, int reps, void* stream, int* tile,
+                   int* split, double* ms) {
+  return apy_zgemm_tune_impl(d, workspace, ws_bytes, reps, (hipStream_t)stream, tile, split, ms);
+}
+
+int apy_zgemm_tune_clear(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tuned.clear();
+  return APY_OK;
 }
 
 }  // extern "C"

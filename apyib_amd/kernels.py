@@ -40,8 +40,10 @@ def ptr(t):
 
 def zgemm_raw(*, m, n, k, a, a_layout, lda, b, b_layout, ldb, c, ldc, alpha=1.0, beta=0.0,
               conj_a=False, conj_b=False, batch=1, stride_a=0, stride_b=0, stride_c=0,
-              reduce_batch=False, split_k=0):
-    """C = alpha * op(A) op(B) + beta * C on raw strides (see apy_zgemm)."""
+              reduce_batch=False, split_k=0, tile=0, tune=False, reps=3):
+    """C = alpha * op(A) op(B) + beta * C on raw strides (see apy_zgemm).
+    With tune=True the candidate plans are timed first (apy_zgemm_tune) and
+    the fastest is cached for the shape; returns (tile, split, ms) then."""
     lib = _lib.load()
     alpha = complex(alpha)
     beta = complex(beta)
@@ -51,13 +53,33 @@ def zgemm_raw(*, m, n, k, a, a_layout, lda, b, b_layout, ldb, c, ldc, alpha=1.0,
         a.data_ptr(), lda, stride_a,
         b.data_ptr(), ldb, stride_b,
         c.data_ptr(), ldc, stride_c,
-        batch, int(reduce_batch), int(split_k))
+        batch, int(reduce_batch), int(split_k), int(tile))
     nbytes = lib.apy_zgemm_workspace(ctypes.byref(d))
     ws = workspace(nbytes, c.device) if nbytes else None
-    st = lib.apy_zgemm(ctypes.byref(d), ctypes.c_void_p(ws.data_ptr() if ws is not None else 0),
-                       ctypes.c_size_t(nbytes), stream_of(c.device))
+    wsp = ctypes.c_void_p(ws.data_ptr() if ws is not None else 0)
+    if tune:
+        ti, sp, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        st = lib.apy_zgemm_tune(ctypes.byref(d), wsp, ctypes.c_size_t(nbytes), int(reps), stream_of(c.device),
+                                ctypes.byref(ti), ctypes.byref(sp), ctypes.byref(ms))
+        _lib.check(st, "apy_zgemm_tune")
+        return ti.value, sp.value, ms.value
+    st = lib.apy_zgemm(ctypes.byref(d), wsp, ctypes.c_size_t(nbytes), stream_of(c.device))
     _lib.check(st, "apy_zgemm")
     return c
+
+
+def zgemm_plan(**kw):
+    """(tile, split) apy_zgemm would choose for the same arguments."""
+    lib = _lib.load()
+    alpha, beta = complex(kw.get('alpha', 1.0)), complex(kw.get('beta', 0.0))
+    a, b, c = kw['a'], kw['b'], kw['c']
+    d = _lib.GemmDesc(kw['m'], kw['n'], kw['k'], kw['a_layout'], kw['b_layout'], 0, 0, alpha.real, alpha.imag,
+                      beta.real, beta.imag, a.data_ptr(), kw['lda'], kw.get('stride_a', 0), b.data_ptr(), kw['ldb'],
+                      kw.get('stride_b', 0), c.data_ptr(), kw['ldc'], kw.get('stride_c', 0), kw.get('batch', 1),
+                      int(kw.get('reduce_batch', False)), int(kw.get('split_k', 0)), int(kw.get('tile', 0)))
+    ti, sp = ctypes.c_int(), ctypes.c_int()
+    _lib.check(lib.apy_zgemm_plan(ctypes.byref(d), ctypes.byref(ti), ctypes.byref(sp)), "apy_zgemm_plan")
+    return ti.value, sp.value
 
 
 def _layout_a(A):
@@ -76,7 +98,7 @@ def _layout_b(B):
     raise ValueError("B must have a unit stride along one dimension")
 
 
-def mm(A, B, C=None, alpha=1.0, beta=0.0, conj_a=False, conj_b=False, split_k=0):
+def mm(A, B, C=None, alpha=1.0, beta=0.0, conj_a=False, conj_b=False, split_k=0, tile=0, tune=False):
     """C = alpha*A@B + beta*C for 2-D complex128 device tensors (views with a
     unit stride along one dimension are accepted without copies)."""
     _require(A, "A")
@@ -95,4 +117,4 @@ def mm(A, B, C=None, alpha=1.0, beta=0.0, conj_a=False, conj_b=False, split_k=0)
     bl, ldb = _layout_b(B)
     return zgemm_raw(m=m, n=n, k=k, a=A, a_layout=al, lda=lda, b=B, b_layout=bl, ldb=ldb,
                      c=C, ldc=C.stride(0), alpha=alpha, beta=beta, conj_a=conj_a, conj_b=conj_b,
-                     split_k=split_k)
+                     split_k=split_k, tile=tile, tune=tune)

@@ -139,6 +139,8 @@ class SOCI:
         self.energy_dev = torch.zeros(1, dtype=CDT, device=self.device)
         self.zero_energy = torch.zeros(1, dtype=CDT, device=self.device)
         self.scal = torch.zeros(_lib.APY_MAXVEC + 4, dtype=CDT, device=self.device)
+        # optional (begin, end) hipEvent_t pair recorded around the PPL kernel
+        self.ev_dominant = None
 
     # ------------------------------------------------------------ primitives
     def new_vector(self):
@@ -148,17 +150,30 @@ class SOCI:
         O, V = self.O, self.V
         return t[:self.OV].view(O, V), t[self.OV:].view(O, O, V, V)
 
-    def step(self, t, t_new, out, energy=None, mode=_lib.APY_MODE_RESIDUAL, c0=1.0):
+    def step(self, t, t_new, out, energy=None, mode=_lib.APY_MODE_RESIDUAL, c0=1.0, ev_dominant=None):
         """One call of apy_cisd_so_step (residual update or sigma product)."""
         c0 = complex(c0)
+        ev = ev_dominant or self.ev_dominant or (None, None)
         args = _lib.CisdSoArgs(
             self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(), ctypes.pointer(self.ints.struct),
             self.hoo.data_ptr(), int(self.singles), int(mode), int(self.fov),
             (energy if energy is not None else self.energy_dev).data_ptr(), c0.real, c0.imag,
-            t.data_ptr(), t_new.data_ptr() if t_new is not None else None, out.data_ptr())
+            t.data_ptr(), t_new.data_ptr() if t_new is not None else None, out.data_ptr(), ev[0], ev[1])
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_so_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
                    "apy_cisd_so_step")
+
+    def tune(self, t=None, reps=2):
+        """Time the candidate GEMM plans of one step on this problem's
+        operands and cache the fastest per shape (process-wide)."""
+        t = self.new_vector() if t is None else t
+        args = _lib.CisdSoArgs(
+            self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(), ctypes.pointer(self.ints.struct),
+            self.hoo.data_ptr(), int(self.singles), _lib.APY_MODE_SIGMA, int(self.fov),
+            self.energy_dev.data_ptr(), 1.0, 0.0, t.data_ptr(), None, t.data_ptr(), None, None)
+        ws = workspace(self.ws_bytes, self.device)
+        _lib.check(self.lib.apy_cisd_so_tune(ctypes.byref(args), _vp(ws), self.ws_bytes, int(reps),
+                                             stream_of(self.device)), "apy_cisd_so_tune")
 
     def sigma(self, x, out, c0=1.0):
         """out = H_N (c0, x): the CISD sigma with reference weight c0."""
@@ -203,17 +218,21 @@ class SOCI:
             t[:self.OV].zero_()
         return t
 
-    def solve(self, params, print_level=0, max_diis=7):
-        """The reference's iteration (ci_wfn.py:263-378 for CISD, :171-257
-        for CID).  Returns (E, t1, t2, iterations performed), E a Python
-        complex."""
+    def iterations(self, diis=True, max_diis=7):
+        """Generator over the reference's iterations (ci_wfn.py:302-378): each
+        step is the Jacobi update (:336-337), the DIIS extrapolation
+        (:340-352, utils.py:104-140), the energy (:355) and the un-conjugated
+        rms of the change (:361-365).  Yields (it, E, dE, rms1, rms2) with E
+        complex; the current amplitudes are self.t.
+        """
         dev = self.device
-        diis = bool(params['DIIS'])
         nvec = max_diis + 1
         t_cur = self.initial_guess(self.new_vector())
         t_nxt = self.new_vector()
         self.energy(t_cur)
         E = complex(self.energy_dev.cpu()[0])
+        self.t = t_cur
+        yield 0, E, 0j, 0j, 0j
         if diis:
             Eh = torch.zeros((nvec, self.L), dtype=CDT, device=dev)
             Th = torch.zeros((nvec, self.L), dtype=CDT, device=dev)
@@ -221,13 +240,9 @@ class SOCI:
             hist = []
         else:
             r = self.new_vector()
-        if print_level > 0:
-            print("\n Iter      E_elec(real)       E_elec(imaginary)        E(tot)           Delta_E(real)       "
-                  "Delta_E(imaginary)      RMS_T1(real)      RMS_T1(imaginary)      RMS_T2(real)      RMS_T2(imaginary)")
-        rms1 = rms2 = 0j
-        dE = 0j
         it = 0
-        for it in range(1, params['max_iterations'] + 1):
+        while True:
+            it += 1
             E_old = E
             if diis:
                 slot = (it - 1) % nvec
@@ -237,7 +252,7 @@ class SOCI:
                 self.step(t_cur, Th[slot], Eh[slot])
                 hist.append((it, slot))
                 # new row of B = E^H E (utils.py:121), cached for older pairs
-                self.dots([Eh[s] for _, s in hist], Eh[slot], self.scal)
+                self.dots([Eh[q] for _, q in hist], Eh[slot], self.scal)
                 row = self.scal[:len(hist)].cpu().numpy()
                 for (itm, _), b in zip(hist, row):
                     Bfull[(itm, it)] = b
@@ -255,7 +270,7 @@ class SOCI:
                 A = np.zeros(nv + 1)
                 A[-1] = -1
                 c = np.linalg.solve(B, A)                                       # utils.py:135
-                self.combine([Th[s] for _, s in hist], c[:-1], t_nxt)           # utils.py:138
+                self.combine([Th[q] for _, q in hist], c[:-1], t_nxt)           # utils.py:138
             else:
                 self.step(t_cur, t_nxt, r)
             self.energy(t_nxt)
@@ -265,17 +280,64 @@ class SOCI:
             E = complex(host[0])
             rms1 = np.sqrt(host[1])
             rms2 = np.sqrt(host[2])
-            dE = E_old - E
             t_cur, t_nxt = t_nxt, t_cur
-            if print_level > 0:
-                print(" %02d %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f" % (
-                    it, E.real, E.imag, E.real, dE.real, dE.imag, rms1.real, rms1.imag, rms2.real, rms2.imag))
-            checks = [rms1, rms2] if self.singles else [rms2]
-            if _converged(it, dE, checks, params):
-                break
-            if it == params['max_iterations']:
-                if abs(dE) > params['e_convergence'] or rms2 > params['d_convergence']:
-                    print("Not converged.")
-        self.t = t_cur
-        t1, t2 = self.split(t_cur)
+            self.t = t_cur
+            yield it, E, E_old - E, rms1, rms2
+
+    def solve(self, params, print_level=0, max_diis=7, e_offset=0.0):
+        """The reference's iteration (ci_wfn.py:263-378 for CISD, :171-257
+        for CID) until its convergence test passes or max_iterations.
+        Returns (E, t1, t2, iterations performed), E a Python complex."""
+        if print_level > 0:
+            print("\n Iter      E_elec(real)       E_elec(imaginary)        E(tot)           Delta_E(real)       "
+                  "Delta_E(imaginary)      RMS_T1(real)      RMS_T1(imaginary)      RMS_T2(real)      RMS_T2(imaginary)")
+        gen = self.iterations(diis=bool(params['DIIS']), max_diis=max_diis)
+        _, E, _, _, _ = next(gen)
+        it = 0
+        dE = rms2 = 0j
+        if params['max_iterations'] >= 1:
+            for it, E, dE, rms1, rms2 in gen:
+                if print_level > 0:
+                    print(" %02d %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f" % (
+                        it, E.real, E.imag, (E + e_offset).real, dE.real, dE.imag, rms1.real, rms1.imag,
+                        rms2.real, rms2.imag))
+                checks = [rms1, rms2] if self.singles else [rms2]
+                if _converged(it, dE, checks, params):
+                    break
+                if it == params['max_iterations']:
+                    if abs(dE) > params['e_convergence'] or rms2 > params['d_convergence']:
+                        print("Not converged.")
+                    break
+        gen.close()
+        t1, t2 = self.split(self.t)
         return E, t1, t2, it
+
+
+def step_flops(O, V, singles=True):
+    """Real flops executed by one apy_cisd_so_step (8 per complex
+    multiply-add), per GEMM of the step; 'ppl' is the dominant kernel."""
+    Po, Pv, OV = npair(O), npair(V), O * V
+    f = {
+        'ppl': 8.0 * Po * Pv * Pv,
+        'hhl': 8.0 * Po * Pv * Po,
+        'ring': 8.0 * OV * OV * OV,
+        'fvv': 8.0 * O * O * V * V * V,
+    }
+    if singles:
+        f.update({
+            'u': 8.0 * V * V * Po * O,
+            'q': 8.0 * Pv * O * O * V,
+            's1_fvv': 8.0 * O * V * V,
+            's1_foo': 8.0 * O * V * O,
+            's1_ring': 8.0 * OV * OV,
+            's1_fov': 8.0 * O * OV * V,
+            's1_vovv': 8.0 * O * V * OV * V,
+            's1_ooov': 8.0 * O * O * O * V * V,
+        })
+    return f
+
+
+def dense_flops(O, V):
+    """The dense operation count of one sigma with every P(ij)/P(ab) image
+    counted once: 8 [O^2V^4 + O^4V^2 + O^3V^3 + 3 O^2V^3 + 3 O^3V^2 + 3 O^2V^2]."""
+    return 8.0 * (O**2 * V**4 + O**4 * V**2 + O**3 * V**3 + 3 * O**2 * V**3 + 3 * O**3 * V**2 + 3 * O**2 * V**2)

@@ -1,0 +1,256 @@
+#!/usr/bin/env python
+"""Benchmark of the spin-orbital CISD sigma on one MI355X.
+
+Workload (one GPU): synthetic complex128 CISD at O=20 occupied and V=160
+virtual spin orbitals (seeded <pq||rs>, complex std 0.05 Eh; Fock matrix
+with U[-1.5,-0.3] / U[0.2,3.0] Eh diagonal and 1e-3 Hermitian noise), the
+reference's Jacobi + DIIS iteration.  A step is one iteration: sigma and
+residual, Jacobi update, DIIS extrapolation, energy and convergence data.
+Inputs are generated on the device before timing; nothing crosses PCIe in
+the timed region except the handful of DIIS/energy scalars the iteration
+reads back each step.
+
+Prints one JSON line.  `value` is the executed FP64 work of the step's
+contractions (8 flops per complex multiply-add, antisymmetric pairs packed)
+divided by the measured time per step.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+METRIC = "CISD sigma TFLOP/s (cplx FP64); AAT set wall-time batched+adjoint vs plain loop"
+PEAK_FP64 = 78.6  # MI355X dense FP64 matrix, TFLOP/s (published)
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """Raw hipEvent_t handles (timing on the stream the kernels use)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self):
+        e = ctypes.c_void_p()
+        if self.hip.hipEventCreate(ctypes.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+        return e.value
+
+    def elapsed_ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        ms = ctypes.c_float()
+        if self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return ms.value
+
+    def destroy(self, e):
+        self.hip.hipEventDestroy(e)
+
+
+def read_traffic():
+    """HBM bytes per PPL launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "ppl_o20_v160_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(solver, O, V, flops_step, seed):
+    """One sigma of the reference algorithm (oracle NumPy restatement of
+    ci_wfn.py:309-334, dense einsums as written) on the host cores, on this
+    workload's integrals copied back from the device."""
+    import numpy as np
+    from oracle import so
+    from threadpoolctl import threadpool_info
+
+    ints = solver.ints
+    Po, Pv = O * (O - 1) // 2, V * (V - 1) // 2
+    io, jo = np.triu_indices(O, 1)
+    ia, ib = np.triu_indices(V, 1)
+
+    def host(name):
+        return ints[name].cpu().numpy()
+
+    t0 = time.perf_counter()
+    W = {}
+    W['oovv'] = host('oovv').reshape(O, O, V, V)
+    W['vvoo'] = host('vvoo_t').reshape(O, O, V, V).transpose(2, 3, 0, 1).copy()
+    W['ovvo'] = host('ring').reshape(O, V, O, V).transpose(2, 1, 3, 0).copy()
+    W['vovv'] = host('vovv').reshape(V, O, V, V)
+    W['ooov'] = host('ooov').reshape(O, O, O, V)
+    P = host('oooo_p').reshape(Po, Po)           # [(i<j)][(k<l)] = <kl||ij>
+    X = np.zeros((O, O, O, O), dtype=np.complex128)   # X[i,j,k,l] = <kl||ij>
+    Xv = X.reshape(O * O, O * O)
+    r1, r2 = io * O + jo, jo * O + io
+    Xv[np.ix_(r1, r1)] = P
+    Xv[np.ix_(r1, r2)] = -P
+    Xv[np.ix_(r2, r1)] = -P
+    Xv[np.ix_(r2, r2)] = P
+    W['oooo'] = X.transpose(2, 3, 0, 1).copy()
+    Pk = host('ovoo_p').reshape(O, V, Po)          # [k][b][(i<j)]
+    ovoo = np.zeros((O, V, O, O), dtype=np.complex128)
+    ovoo[:, :, io, jo] = Pk
+    ovoo[:, :, jo, io] = -Pk
+    W['ovoo'] = ovoo
+    W['vooo'] = -ovoo.transpose(1, 0, 2, 3).copy()
+    Pq = host('vvvo_p').reshape(Pv, O, V)          # [(a<b)][j][c] = <ab||cj>
+    vvvo = np.zeros((V, V, V, O), dtype=np.complex128)
+    vvvo[ia, ib] = Pq.transpose(0, 2, 1)
+    vvvo[ib, ia] = -Pq.transpose(0, 2, 1)
+    W['vvvo'] = vvvo
+    W['vvov'] = -vvvo.transpose(0, 1, 3, 2).copy()
+    Pvv = host('vvvv_p').reshape(Pv, Pv)
+    vvvv = np.zeros((V * V, V * V), dtype=np.complex128)
+    ra, rb = ia * V + ib, ib * V + ia
+    vvvv[np.ix_(ra, ra)] = Pvv
+    vvvv[np.ix_(ra, rb)] = -Pvv
+    vvvv[np.ix_(rb, ra)] = -Pvv
+    vvvv[np.ix_(rb, rb)] = Pvv
+    del Pvv
+    W['vvvv'] = vvvv.reshape(V, V, V, V)
+    F = solver.fock.cpu().numpy()
+    rng = np.random.default_rng(seed)
+    t1 = 0.01 * (rng.standard_normal((O, V)) + 1j * rng.standard_normal((O, V)))
+    t2 = 0.01 * (rng.standard_normal((O, O, V, V)) + 1j * rng.standard_normal((O, O, V, V)))
+    t2 = t2 - t2.transpose(1, 0, 2, 3)
+    t2 = t2 - t2.transpose(0, 1, 3, 2)
+    log(f"cpu baseline: host blocks ready in {time.perf_counter() - t0:.1f} s")
+    threads = max([i.get('num_threads', 1) for i in threadpool_info()] or [1])
+    t0 = time.perf_counter()
+    so.cisd_so_sigma_blocks(F, W, t1, t2, O)
+    sec = time.perf_counter() - t0
+    log(f"cpu baseline: one sigma {sec:.2f} s on {threads} BLAS threads")
+    return {
+        "value": flops_step / sec / 1e12,
+        "unit": "TFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"one CISD sigma (NumPy restatement of ci_wfn.py:309-334, dense einsums as written) at "
+                   f"O={O},V={V} on this workload's integrals, {sec:.2f} s; rate = the GPU step's executed "
+                   f"flops / CPU time"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--O", type=int, default=20)
+    ap.add_argument("--V", type=int, default=160)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-tune", action="store_true", help="skip the GEMM autotuner")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported: "
+              "the sharded design is described in DESIGN.md and is not built", file=sys.stderr)
+        sys.exit(2)
+
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    import torch
+    from apyib_amd import synthetic
+    from apyib_amd.so_ci import SOCI, SOIntegrals, dense_flops, step_flops
+
+    dev = torch.device("cuda:0")
+    O, V = args.O, args.V
+    t0 = time.perf_counter()
+    ints = SOIntegrals.synthetic(O, V, args.seed, device=dev)
+    F, eps = synthetic.fock(O, V, args.seed)
+    solver = SOCI(ints, torch.from_numpy(F).to(dev), eps, singles=True)
+    torch.cuda.synchronize()
+    log(f"setup: integrals {ints.nbytes() / 1e9:.2f} GB generated on device in {time.perf_counter() - t0:.2f} s")
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        solver.tune()
+        torch.cuda.synchronize()
+        log(f"autotune: {time.perf_counter() - t0:.1f} s")
+
+    fl = step_flops(O, V, True)
+    flops_step = sum(fl.values())
+    hip = HipEvents()
+    K, W = args.steps, args.warmup
+    evs = [(hip.create(), hip.create()) for _ in range(K)]
+
+    gen = solver.iterations(diis=True)
+    next(gen)  # initial guess and energy
+    for _ in range(W):
+        next(gen)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    E = None
+    for k in range(K):
+        solver.ev_dominant = evs[k]
+        it, E, dE, r1, r2 = next(gen)
+    torch.cuda.synchronize()
+    solver.ev_dominant = None
+    elapsed = time.perf_counter() - t_start
+    gen.close()
+    ms_step = elapsed * 1e3 / K
+    ppl_ms = [hip.elapsed_ms(a, b) for a, b in evs]
+    ppl_avg = sum(ppl_ms) / len(ppl_ms)
+    for a, b in evs:
+        hip.destroy(a)
+        hip.destroy(b)
+    achieved = fl['ppl'] / (ppl_avg * 1e-3) / 1e12
+    value = flops_step / (ms_step * 1e-3) / 1e12
+    log(f"step {ms_step:.3f} ms, PPL kernel {ppl_avg:.3f} ms ({achieved:.1f} TFLOP/s), E = {E}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(solver, O, V, flops_step, args.seed)
+        except MemoryError as ex:  # host without ~20 GB free
+            log(f"cpu baseline skipped: {ex!r}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "complex128",
+        "data": "synthetic",
+        "config": {
+            "workload": f"CISD (spin-orbital) Jacobi+DIIS iteration, seeded complex128 Hamiltonian, O={O} V={V}",
+            "O": O, "V": V, "seed": args.seed, "solver": "jacobi+diis (ci_wfn.py:263-378)",
+            "step_flops_executed": flops_step,
+            "step_flops_dense_equivalent": dense_flops(O, V),
+        },
+        "dense_equivalent_tflops": round(dense_flops(O, V) / (ms_step * 1e-3) / 1e12, 3),
+        "roofline": {
+            "kernel": "zgemm_mfma (particle-particle ladder, packed pairs)",
+            "bound": "mfma",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP64,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64, 4),
+            "traffic": read_traffic(),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

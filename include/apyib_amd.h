@@ -68,11 +68,22 @@ typedef struct apy_gemm_desc {
   int64_t batch;      /* >= 1 */
   int reduce_batch;   /* 0 or 1 */
   int split_k;        /* 0 = choose automatically, else number of K slices */
+  int tile;           /* 0 = choose automatically, else tile config 1..7 */
 } apy_gemm_desc;
 
-/* Workspace (bytes) apy_zgemm needs for this descriptor (0 if none). */
+/* Workspace (bytes) apy_zgemm needs for this descriptor (0 if none); it
+ * covers every plan the tuner may pick for the shape. */
 size_t apy_zgemm_workspace(const apy_gemm_desc* d);
 int apy_zgemm(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, void* stream);
+/* The (tile config, split) apy_zgemm would use for d. Host only. */
+int apy_zgemm_plan(const apy_gemm_desc* d, int* tile, int* split);
+/* Autotuner: time the candidate plans for d's shape on d's operands (C is
+ * overwritten), keep the fastest in a process-wide cache (mutex guarded)
+ * that later calls with the same shape use.  Synchronises the stream.
+ * Outputs (may be NULL): chosen tile, split, and its time in ms. */
+int apy_zgemm_tune(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, int reps, void* stream, int* tile,
+                   int* split, double* ms);
+int apy_zgemm_tune_clear(void);
 
 /* ------------------------------------------------------------------------
  * Antisymmetrized spin-orbital integral blocks <pq||rs> in the layouts the
@@ -154,6 +165,8 @@ typedef struct apy_cisd_so_args {
   const void* t;           /* [O*V + O^2*V^2] input amplitudes           */
   void* t_new;             /* residual mode: t + r/D (may equal t)       */
   void* out;               /* residual (residual mode) or sigma; != t    */
+  void* ev_begin;          /* optional hipEvent_t recorded just before and */
+  void* ev_end;            /* just after the PPL GEMM kernel (profiling)   */
 } apy_cisd_so_args;
 
 /* Pair-space occupied operator: <kl||ij> plus the F_oo terms of the doubles
@@ -162,6 +175,9 @@ int apy_cisd_so_pair_operator(int64_t O, int64_t V, const void* fock, const void
                               void* stream);
 size_t apy_cisd_so_workspace(int64_t O, int64_t V, int singles);
 int apy_cisd_so_step(const apy_cisd_so_args* a, void* workspace, size_t ws_bytes, void* stream);
+/* Autotune every GEMM of a step for this (O, V) on the operands in `a`
+ * (apy_zgemm_tune; only workspace is written).  Synchronises the stream. */
+int apy_cisd_so_tune(const apy_cisd_so_args* a, void* workspace, size_t ws_bytes, int reps, void* stream);
 
 /* E = sum_ia t1 f_ia + 1/4 sum t2 <ij||ab> (ci_wfn.py:355; CID :237 with
  * singles = 0), written to the device complex scalar `energy`. */
