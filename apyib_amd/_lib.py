@@ -79,6 +79,13 @@ class CisdSoArgs(ctypes.Structure):
     ]
 
 
+class Mp2Instance(ctypes.Structure):
+    _fields_ = [
+        ("O", c_i64), ("V", c_i64),
+        ("eps", c_vp), ("vvoo_t", c_vp), ("oovv", c_vp), ("t2", c_vp),
+    ]
+
+
 _P_I64 = ctypes.POINTER(c_i64)
 _P_DBL = ctypes.POINTER(c_dbl)
 _P_VP = ctypes.POINTER(c_vp)
@@ -107,6 +114,18 @@ SIGNATURES = {
     "apy_zdiffsq_segments": (c_int, [c_vp, c_vp, c_int, _P_I64, c_vp, c_vp, c_size, c_vp]),
     "apy_zcombine": (c_int, [c_int, _P_VP, _P_DBL, c_i64, c_vp, c_vp]),
     "apy_reduce_workspace": (c_size, [c_i64, c_int]),
+    "apy_mp2_so_batched_workspace": (c_size, [c_int, ctypes.POINTER(Mp2Instance)]),
+    "apy_mp2_so_batched": (c_int, [c_int, ctypes.POINTER(Mp2Instance), c_vp, c_vp, c_size, c_vp]),
+    "apy_mp2_so": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_mp2_spatial_workspace": (c_size, []),
+    "apy_mp2_spatial": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_eri_mo_workspace": (c_size, [c_i64, c_i64]),
+    "apy_eri_mo": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_fock_build": (c_int, [c_i64, c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),
+    "apy_mo_overlap_workspace": (c_size, [c_i64, c_i64, c_i64]),
+    "apy_mo_overlap": (c_int, [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_size,
+                               c_vp]),
+    "apy_phase_align": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp]),
 }
 
 _lib = None

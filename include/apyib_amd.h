@@ -185,6 +185,62 @@ int apy_cisd_so_energy(int64_t O, int64_t V, int singles, const void* fock, cons
                        void* energy, void* workspace, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * AO -> MO transforms, Fock build, overlaps, phase (utils.py:217-279, 370-446)
+ * AO arrays are complex128 (a real AO tensor is passed with zero imaginary
+ * parts).  C is [nbf][ncol] row-major.
+ * ---------------------------------------------------------------------- */
+/* (pq|rs) = sum conj(C_mp) C_nq conj(C_lr) C_gs (mn|lg)  (compute_ERI_MO,
+ * utils.py:258-279): four quarter transforms on the matrix cores.
+ * eri_ao [nbf]^4, C [nbf][nact] (active columns), eri_mo [nact]^4. */
+size_t apy_eri_mo_workspace(int64_t nbf, int64_t nact);
+int apy_eri_mo(int64_t nbf, int64_t nact, const void* eri_ao, const void* C, void* eri_mo, void* workspace,
+               size_t ws_bytes, void* stream);
+/* F = h + scale * sum_ls D_ls (2 (mn|ls) - (ml|ns))  (utils.py:241,249 with
+ * scale 1; hf_wfn.py:81 with scale 1/2).  h may be NULL (zero). */
+int apy_fock_build(int64_t nbf, const void* h, const void* eri_ao, const void* D, double scale, void* F,
+                   void* stream);
+/* out_b = C_bra,b^H S_b C_ket,b for b < batch (compute_mo_overlap,
+ * utils.py:381); C_* are [nbf][nmo], S [nbf][nbf], out [nmo][nmo]; strides
+ * are element counts between batch entries (0 = shared). */
+size_t apy_mo_overlap_workspace(int64_t nbf, int64_t nmo, int64_t batch);
+int apy_mo_overlap(int64_t nbf, int64_t nmo, int64_t batch, const void* c_bra, int64_t stride_bra, const void* s_ao,
+                   int64_t stride_s, const void* c_ket, int64_t stride_ket, void* out, void* workspace,
+                   size_t ws_bytes, void* stream);
+/* out[:, m] = C[:, m] / (S_mm / |S_mm|)  (compute_phase, utils.py:427-446);
+ * S is the MO overlap C0^H S_AO C with leading dimension lds. */
+int apy_phase_align(int64_t nbf, int64_t ncol, const void* s_mo, int64_t lds, const void* C, void* out,
+                    void* stream);
+
+/* ------------------------------------------------------------------------
+ * MP2 (mp2_wfn.py:42-87).
+ * Spin orbitals (solve_MP2_SO, :64-87):  t = <ab||ij>/D,  E = 1/4 sum <ij||ab> t
+ * Spatial (solve_MP2, :42-59), from chemists' MO integrals (pq|rs) over the
+ *   active orbitals [m][m][m][m], occupied first:
+ *   t[i,j,a,b] = (ai|bj)/D,  E = sum (2 (ia|jb) - (ib|ja)) t
+ * D_ijab = eps_i + eps_j - eps_a - eps_b (mp2_wfn.py:39, 66-68).
+ * `energy` / `energies` are device complex scalars.
+ * ---------------------------------------------------------------------- */
+typedef struct apy_mp2_instance {
+  int64_t O, V;
+  const double* eps;    /* [O+V] */
+  const void* vvoo_t;   /* [O][O][V][V] <ab||ij> at [i][j][a][b] */
+  const void* oovv;     /* [O][O][V][V] <ij||ab> */
+  void* t2;             /* [O][O][V][V] output */
+} apy_mp2_instance;
+
+/* n instances of any (O, V) in one launch per 40 instances: the batched
+ * finite-difference form (each instance is one displaced wavefunction).
+ * `inst` is a host array; energies[k] receives instance k's energy. */
+size_t apy_mp2_so_batched_workspace(int n, const apy_mp2_instance* inst);
+int apy_mp2_so_batched(int n, const apy_mp2_instance* inst, void* energies, void* workspace, size_t ws_bytes,
+                       void* stream);
+int apy_mp2_so(int64_t O, int64_t V, const double* eps, const void* vvoo_t, const void* oovv, void* t2,
+               void* energy, void* workspace, size_t ws_bytes, void* stream);
+size_t apy_mp2_spatial_workspace(void);
+int apy_mp2_spatial(int64_t O, int64_t V, const double* eps, const void* eri_mo, void* t2, void* energy,
+                    void* workspace, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * Vector reductions used by DIIS and the convergence test.
  * ---------------------------------------------------------------------- */
 #define APY_MAXVEC 16

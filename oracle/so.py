@@ -63,7 +63,8 @@ def cisd_so_energy(F, eri, t1, t2, O):
 def cisd_so_residual(F, eri, t1, t2, E, O):
     """Residuals r_T1, r_T2 of ci_wfn.py:309-334 (sigma minus E*t)."""
     o, v = _slices(O, F.shape[0])
-    r1 = F.swapaxes(0, 1)[o, v].copy()                                          # :309
+    dt = np.result_type(F, eri, t1, t2)
+    r1 = F.swapaxes(0, 1)[o, v].astype(dt)                                      # :309
     r1 -= contract('ji,ja->ia', F[o, o], t1)                                    # :310
     r1 += contract('ab,ib->ia', F[v, v], t1)                                    # :311
     r1 += contract('jabi,jb->ia', asym(eri, o, v, v, o), t1)                    # :312
@@ -72,7 +73,7 @@ def cisd_so_residual(F, eri, t1, t2, E, O):
     r1 -= 0.5 * contract('kjib,kjab->ia', asym(eri, o, o, o, v), t2)            # :315
     r1 -= E * t1                                                                # :316
 
-    r2 = driver_t2(eri, O)                                                      # :319
+    r2 = driver_t2(eri, O).astype(dt)                                           # :319
     r2 -= contract('kbij,ka->ijab', asym(eri, o, v, o, o), t1)                  # :320
     r2 -= contract('akij,kb->ijab', asym(eri, v, o, o, o), t1)                  # :321
     r2 += contract('abcj,ic->ijab', asym(eri, v, v, v, o), t1)                  # :322
@@ -100,7 +101,7 @@ def cid_so_energy(eri, t2, O):
 def cid_so_residual(F, eri, t2, E, O):
     """r_T2 of ci_wfn.py:210-220."""
     o, v = _slices(O, F.shape[0])
-    r2 = driver_t2(eri, O)                                                      # :210
+    r2 = driver_t2(eri, O).astype(np.result_type(F, eri, t2))                   # :210
     r2 += contract('ijae,be->ijab', t2, F[v, v]) + contract('ijeb,ae->ijab', t2, F[v, v])   # :211
     r2 -= contract('imab,mj->ijab', t2, F[o, o]) + contract('mjab,mi->ijab', t2, F[o, o])   # :212
     r2 += 0.5 * contract('mnab,mnij->ijab', t2, asym(eri, o, o, o, o))          # :213

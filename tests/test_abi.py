@@ -1,0 +1,63 @@
+"""The C-ABI library loads without a GPU and exports every function
+include/apyib_amd.h declares; the ctypes table matches the header.  No
+compute calls (CPU only)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from apyib_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "apyib_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(apy_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_functions():
+    names = declared_functions()
+    assert "apy_zgemm" in names and "apy_cisd_so_step" in names
+    assert len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_table_matches_header():
+    assert set(_lib.SIGNATURES) == set(declared_functions())
+
+
+def test_load_binds_signatures():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    assert lib.apy_version() >= 100
+    sizes = (ctypes.c_int64 * 9)()
+    assert lib.apy_so_ints_sizes(20, 160, sizes) == 0
+    Po, Pv = 190, 12720
+    assert list(sizes) == [Pv * Pv, 20 * 160 * 20 * 160, Po * Po, 20 * 160 * Po, Pv * 20 * 160,
+                           160 * 20 * 160 * 160, 20 * 20 * 20 * 160, 20 * 20 * 160 * 160, 20 * 20 * 160 * 160]
+    assert lib.apy_so_ints_sizes(-1, 3, sizes) == _lib.APY_E_ARG
+
+
+def test_missing_library_raises(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(_lib.NativeError):
+        _lib.load()
+
+
+def test_struct_layouts_match_header_sizes():
+    # int64 x3 + int x4 + double x4 + (ptr,int64,int64) x3 + int64 + int x3 (padded)
+    assert ctypes.sizeof(_lib.GemmDesc) == 8 * 3 + 4 * 4 + 8 * 4 + 24 * 3 + 8 + 4 * 3 + 4
+    assert ctypes.sizeof(_lib.SoInts) == 8 * 2 + 8 * 9
