@@ -58,6 +58,22 @@ __global__ void phase_kernel(int64_t nb, int64_t ncol, const double2* __restrict
   }
 }
 
+// out[p,q,r,s] = eri[p/2,q/2,r/2,s/2] delta(sp,sq) delta(sr,ss)  (utils.py:332-363)
+__global__ __launch_bounds__(256) void eri_so_kernel(int64_t m, const double2* __restrict__ eri,
+                                                     double2* __restrict__ out) {
+  const int64_t n = 2 * m, total = n * n * n * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r0 = t;
+    const int64_t s = r0 % n; r0 /= n;
+    const int64_t r = r0 % n; r0 /= n;
+    const int64_t q = r0 % n;
+    const int64_t p = r0 / n;
+    double2 v = make_double2(0.0, 0.0);
+    if (((p ^ q) & 1) == 0 && ((r ^ s) & 1) == 0) v = eri[(((p >> 1) * m + (q >> 1)) * m + (r >> 1)) * m + (s >> 1)];
+    out[t] = v;
+  }
+}
+
 apy_gemm_desc gd(int64_t m, int64_t n, int64_t k, int al, const void* a, int64_t lda, int bl, const void* b,
                  int64_t ldb, void* c, int64_t ldc) {
   apy_gemm_desc d{};
@@ -126,6 +142,15 @@ int apy_eri_mo(int64_t nbf, int64_t nact, const void* eri_ao, const void* C, voi
     int s = apy_zgemm_launch(&P.g[q], gw, P0.gemm_bytes, (hipStream_t)stream);
     if (s != APY_OK) return s;
   }
+  return APY_OK;
+}
+
+int apy_eri_so_from_mo(int64_t m, const void* eri_mo, void* eri_so, void* stream) {
+  if (m < 1 || !eri_mo || !eri_so) return APY_E_ARG;
+  const int64_t n = 2 * m;
+  eri_so_kernel<<<apy_grid1d(n * n * n * n, 1 << 20), 256, 0, (hipStream_t)stream>>>(m, (const double2*)eri_mo,
+                                                                                    (double2*)eri_so);
+  APY_LAUNCH_CHECK();
   return APY_OK;
 }
 

@@ -1,0 +1,59 @@
+"""Atomic-orbital Hamiltonian with apyib's interface (hamiltonian.py:8-70),
+built from this package's own integral engine instead of psi4.
+
+Attributes match the reference: T, V, ERI, S (NumPy, AO basis), E_nuc,
+basis_set (nbf(), n_frozen_core()), molecule (natom(),
+true_atomic_number(i), nuclear_repulsion_energy(field)).  Uniform electric
+and magnetic fields enter V as in hamiltonian.py:41-70:
+    V -= F_el . mu_el,   mu_el = -<m| r |n>           (electron charge -1)
+    V -= F_mag . mu_mag, mu_mag = -1/2 i <m| L |n>     with L the angular
+         momentum integrals in the convention of hamiltonian.py:55-57.
+The phase-space extensions (hamiltonian.py:72-538) are not part of this path.
+"""
+import numpy as np
+
+from . import gaussian
+from .molecule import BasisSet, Molecule
+
+# <m| L |n> as passed to -1/2 i (.) in hamiltonian.py:57: the real
+# antisymmetric matrix of r x nabla times this sign (fixed by the reference's
+# finite-difference AAT goldens, see tests/test_gpu_aat.py).
+ANGULAR_MOMENTUM_SIGN = 1.0
+
+
+class Hamiltonian(object):
+    def __init__(self, parameters):
+        self.molecule = Molecule.from_string(parameters['geom'])
+        self.basis_set = BasisSet(self.molecule, parameters['basis'])
+        fns = self.basis_set.functions
+        charges = list(zip(self.molecule.Z, self.molecule.coords))
+        self.S, self.T, self.V = gaussian.build_one_electron(fns, charges)
+        self.ERI = gaussian.build_eri(fns)
+
+        F_el = list(parameters.get('F_el', [0.0, 0.0, 0.0]))
+        F_mag = list(parameters.get('F_mag', [0.0, 0.0, 0.0]))
+        E_field = any(f != 0.0 for f in F_el)
+        M_field = any(f != 0.0 for f in F_mag)
+        if E_field:
+            D = gaussian.build_dipole(fns)
+            self.mu_el = [-D[a] for a in range(3)]
+        if M_field:
+            L = ANGULAR_MOMENTUM_SIGN * gaussian.build_angular_momentum(fns)
+            self.mu_mag = [-0.5j * L[a] for a in range(3)]
+        self.E_nuc = self.molecule.nuclear_repulsion_energy([-f for f in F_el])
+        for a in range(3):
+            if E_field:
+                self.V = self.V - F_el[a] * self.mu_el[a]
+            if M_field:
+                self.V = self.V - F_mag[a] * self.mu_mag[a]
+        self._device_cache = {}
+
+    def device_eri(self, device):
+        """The AO integrals as a complex128 device tensor (uploaded once)."""
+        import torch
+        key = str(device)
+        t = self._device_cache.get(key)
+        if t is None:
+            t = torch.from_numpy(np.ascontiguousarray(self.ERI, dtype=np.complex128)).to(device)
+            self._device_cache[key] = t
+        return t

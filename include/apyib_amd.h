@@ -195,6 +195,10 @@ int apy_cisd_so_energy(int64_t O, int64_t V, int singles, const void* fock, cons
 size_t apy_eri_mo_workspace(int64_t nbf, int64_t nact);
 int apy_eri_mo(int64_t nbf, int64_t nact, const void* eri_ao, const void* C, void* eri_mo, void* workspace,
                size_t ws_bytes, void* stream);
+/* Spin-orbital chemists' integrals (compute_ERI_SO, utils.py:317-365):
+ * eri_so[p,q,r,s] = eri_mo[p/2,q/2,r/2,s/2] delta(sp,sq) delta(sr,ss),
+ * eri_mo [m]^4, eri_so [2m]^4 (not antisymmetrized). */
+int apy_eri_so_from_mo(int64_t m, const void* eri_mo, void* eri_so, void* stream);
 /* F = h + scale * sum_ls D_ls (2 (mn|ls) - (ml|ns))  (utils.py:241,249 with
  * scale 1; hf_wfn.py:81 with scale 1/2).  h may be NULL (zero). */
 int apy_fock_build(int64_t nbf, const void* h, const void* eri_ao, const void* D, double scale, void* F,
