@@ -86,6 +86,15 @@ class Mp2Instance(ctypes.Structure):
     ]
 
 
+class Mp2NcArgs(ctypes.Structure):
+    _fields_ = [
+        ("O", c_i64), ("V", c_i64),
+        ("fock", c_vp), ("eps", c_vp),
+        ("transpose_f", c_int), ("drv_scale", c_dbl),
+        ("drv", c_vp), ("x", c_vp), ("x_new", c_vp), ("rnorm", c_vp),
+    ]
+
+
 _P_I64 = ctypes.POINTER(c_i64)
 _P_DBL = ctypes.POINTER(c_dbl)
 _P_VP = ctypes.POINTER(c_vp)
@@ -119,6 +128,10 @@ SIGNATURES = {
     "apy_mp2_so": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_mp2_spatial_workspace": (c_size, []),
     "apy_mp2_spatial": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_mp2_nc_workspace": (c_size, [c_i64, c_i64]),
+    "apy_mp2_nc_step": (c_int, [ctypes.POINTER(Mp2NcArgs), c_vp, c_size, c_vp]),
+    "apy_mp2_nc_density_workspace": (c_size, [c_i64, c_i64]),
+    "apy_mp2_nc_densities": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_eri_mo_workspace": (c_size, [c_i64, c_i64]),
     "apy_eri_mo": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_fock_build": (c_int, [c_i64, c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),

@@ -245,6 +245,34 @@ int apy_mp2_spatial(int64_t O, int64_t V, const double* eps, const void* eri_mo,
                     void* workspace, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Orbital-invariant MP2 and its adjoint (lambda) — the energy derivative of
+ * analytic.py:307-338 without unrolling iterations.
+ *   R(x) = s*drv + P(ab) x F_vv^T - P(ij) F_oo^T x   (ci_wfn.py:211-212)
+ *   amplitudes: drv = <ab||ij> at [i][j][a][b], s = 1,   F
+ *   adjoint:    drv = <ij||ab>,                s = 1/4, F^T (transpose_f)
+ * One Jacobi step x_new = x + R(x)/D; rnorm (device complex scalar) gets
+ * (sum |R|^2, 0).  x_new may equal x.  E = 1/4 sum <ij||ab> t; at diagonal
+ * F this is solve_MP2_SO (mp2_wfn.py:64-87).
+ * ---------------------------------------------------------------------- */
+typedef struct apy_mp2_nc_args {
+  int64_t O, V;
+  const void* fock;     /* [n][n] */
+  const double* eps;    /* [n] preconditioner D_ijab */
+  int transpose_f;
+  double drv_scale;
+  const void* drv;      /* [O][O][V][V] */
+  const void* x;
+  void* x_new;
+  void* rnorm;
+} apy_mp2_nc_args;
+size_t apy_mp2_nc_workspace(int64_t O, int64_t V);
+int apy_mp2_nc_step(const apy_mp2_nc_args* a, void* workspace, size_t ws_bytes, void* stream);
+/* grad_f [n][n] = dE/dF: vv block 2 sum lam t, oo block -2 sum t lam, rest 0. */
+size_t apy_mp2_nc_density_workspace(int64_t O, int64_t V);
+int apy_mp2_nc_densities(int64_t O, int64_t V, const void* t, const void* lam, void* grad_f, void* workspace,
+                         size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * Vector reductions used by DIIS and the convergence test.
  * ---------------------------------------------------------------------- */
 #define APY_MAXVEC 16
