@@ -95,6 +95,22 @@ class Mp2NcArgs(ctypes.Structure):
     ]
 
 
+APY_AAT_NTERMS = 20
+
+
+class AatAxpbyOp(ctypes.Structure):
+    _fields_ = [
+        ("n", c_i64), ("a", c_vp), ("b", c_vp), ("out", c_vp),
+        ("ca", c_dbl), ("cb", c_dbl), ("conj", c_int),
+    ]
+
+
+class AatJob(ctypes.Structure):
+    _fields_ = [
+        ("gamma", c_i64), ("b1", c_vp), ("b2", c_vp), ("k1", c_vp), ("k2", c_vp),
+    ]
+
+
 _P_I64 = ctypes.POINTER(c_i64)
 _P_DBL = ctypes.POINTER(c_dbl)
 _P_VP = ctypes.POINTER(c_vp)
@@ -140,6 +156,16 @@ SIGNATURES = {
     "apy_mo_overlap": (c_int, [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_size,
                                c_vp]),
     "apy_phase_align": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "apy_aat_gamma_record": (c_i64, [c_i64, c_i64, c_i64]),
+    "apy_aat_gamma_workspace": (c_size, [c_i64, c_i64, c_i64]),
+    "apy_aat_gamma": (c_int, [c_i64, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_size, c_vp]),
+    "apy_aat_axpby_workspace": (c_size, [c_i64]),
+    "apy_aat_axpby": (c_int, [c_i64, ctypes.POINTER(AatAxpbyOp), c_vp, c_size, c_vp]),
+    "apy_aat_terms_workspace": (c_size, [c_i64, c_i64, c_i64]),
+    "apy_aat_terms": (c_int, [c_i64, c_i64, c_vp, c_i64, c_i64, c_i64, ctypes.POINTER(AatJob), c_vp, c_vp, c_size,
+                               c_vp]),
+    "apy_aat_norms_workspace": (c_size, [c_i64]),
+    "apy_aat_norms": (c_int, [c_i64, c_i64, c_i64, _P_VP, _P_VP, c_vp, c_vp, c_size, c_vp]),
 }
 
 _lib = None

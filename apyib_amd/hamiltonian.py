@@ -16,9 +16,11 @@ from . import gaussian
 from .molecule import BasisSet, Molecule
 
 # <m| L |n> as passed to -1/2 i (.) in hamiltonian.py:57: the real
-# antisymmetric matrix of r x nabla times this sign (fixed by the reference's
-# finite-difference AAT goldens, see tests/test_gpu_aat.py).
-ANGULAR_MOMENTUM_SIGN = 1.0
+# antisymmetric matrix of r x nabla times this sign.  -1 reproduces the
+# reference's finite-difference AAT arrays (test_011_AAT.py,
+# test_013_AAT_parallel.py); +1 flips the sign of every element
+# (tests/test_oracle_aat.py).
+ANGULAR_MOMENTUM_SIGN = -1.0
 
 
 class Hamiltonian(object):

@@ -287,6 +287,73 @@ int apy_zdiffsq_segments(const void* x, const void* y, int nseg, const int64_t* 
 int apy_zcombine(int k, const void* const* xs, const double* coef, int64_t len, void* y, void* stream);
 size_t apy_reduce_workspace(int64_t len, int k);
 
+/* ------------------------------------------------------------------------
+ * Finite-difference atomic axial tensors (replaces AAT.compute_all_dets,
+ * aats.py:558-642, and the determinant-amplitude sums of
+ * compute_spatial_aats, aats.py:646-1055, and compute_SO_I_*,
+ * aats.py:120-516).
+ *
+ * Overlaps S (n x n, rows = bra orbitals, columns = ket orbitals, the first
+ * no of each occupied, the first nf of those frozen) are reduced to a record
+ *   [d = det S[o,o]] [x: oa x nv] [y: oa x nv] [P: oa x oa] [W: nv x nv]
+ * (oa = no - nf) with P = S[o,o]^-1 (active block), x[i][a] = (S[v,o] P)[a][i],
+ * y = (P S[o,v])[active rows], W = S[v,v] - S[v,o] P S[o,v]: every
+ * determinant with up to two occupied rows and two occupied columns
+ * replaced is d times a minor built from these.  d = 0 (exactly singular
+ * occupied block) zeroes the whole record.
+ * ---------------------------------------------------------------------- */
+#define APY_AAT_NTERMS 20
+/* double2 elements per record (a multiple of 16) */
+int64_t apy_aat_gamma_record(int64_t no, int64_t nf, int64_t nv);
+size_t apy_aat_gamma_workspace(int64_t n, int64_t no, int64_t batch);
+/* gamma[b * record] for S[b * stride_s], b < batch; no <= 256 */
+int apy_aat_gamma(int64_t n, int64_t no, int64_t nf, int64_t batch, const void* s, int64_t stride_s, void* gamma,
+                  void* workspace, size_t ws_bytes, void* stream);
+
+/* out = conj?(ca * a + cb * b) over n complex elements; b may be NULL.
+ * Forms the normalised finite-difference amplitudes of aats.py:680-704. */
+typedef struct {
+  int64_t n;
+  const void* a;
+  const void* b;
+  void* out;
+  double ca, cb;
+  int conj;
+} apy_aat_axpby_op;
+size_t apy_aat_axpby_workspace(int64_t count);
+int apy_aat_axpby(int64_t count, const apy_aat_axpby_op* ops, void* workspace, size_t ws_bytes, void* stream);
+
+/* One overlap class: bra amplitudes (b1 [oa,nv], b2 [oa,oa,nv,nv], already
+ * conjugated) and ket amplitudes (k1, k2) against record `gamma`; NULL
+ * amplitudes are zero.  out[job * APY_AAT_NTERMS + q], with
+ * A(t) = (t - t^ab - t^ij + t^ijab)/4, beta = sum_jb b2[ijab] x[jb],
+ * beta' = sum_ia b2[ijab] x[ia], beta^ = sum_jb A(b2)[ijab] x[jb],
+ * kA = sum_ld k2[klcd] y[ld], kB = sum_kc k2[klcd] y[kc],
+ * k^ = sum_lc A(k2)[klcd] y[lc], H(M) = P^T M W^T, T = (P^T P^T W W) k2:
+ *   q: 0 d  1 b1.x  2 k1.y  3 beta.x  4 beta^.x  5 kA.y  6 -k^.y
+ *      7 b1.H(k1)  8 beta.H(k1)  9 beta^.H(k1)  10 b1.H(k^)  11 b1.H(kA)
+ *      12 beta^.H(k^)  13 beta^.H(kA)  14 beta.H(k^)  15 beta'.H(k^)
+ *      16 beta'.H(kB)  17 beta.H(kA)  18 b2.A(T)  19 b2.T
+ * (bilinear sums, no conjugation).  The job table is a host array; each
+ * job's gamma index must be below ngamma, the number of records. */
+typedef struct {
+  int64_t gamma;
+  const void* b1;
+  const void* b2;
+  const void* k1;
+  const void* k2;
+} apy_aat_job;
+size_t apy_aat_terms_workspace(int64_t oa, int64_t nv, int64_t njobs);
+int apy_aat_terms(int64_t oa, int64_t nv, const void* gamma, int64_t record, int64_t ngamma, int64_t njobs,
+                  const apy_aat_job* jobs, void* out, void* workspace, size_t ws_bytes, void* stream);
+
+/* out[3 s + (0,1,2)] = sum |t1|^2, sum |t2|^2, Re sum conj(t2[ijab]) t2[ijba]
+ * for each amplitude set s (t1[s] may be NULL), normalisations of
+ * aats.py:134-157, 652-669.  t1, t2 are host arrays of device pointers. */
+size_t apy_aat_norms_workspace(int64_t nset);
+int apy_aat_norms(int64_t oa, int64_t nv, int64_t nset, const void* const* t1, const void* const* t2, void* out,
+                  void* workspace, size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

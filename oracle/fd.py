@@ -1,0 +1,139 @@
+"""Finite-difference AAT driver of apyib restated on the CPU (oracle only):
+fin_diff.compute_AAT (fin_diff.py:267-372), phase_corrected_energy
+(energy.py:84-150), the overlap sets of AAT.__init__ (aats.py:23-115) and the
+element loop of parallel.compute_parallel_aats (parallel.py:16-48).
+
+Each displaced wavefunction is an RHF solve (oracle.scf), phase-aligned to
+the unperturbed orbitals (compute_phase), then the correlated solve of the
+requested method (oracle.spatial / oracle.so).  Integrals come from the
+package's host integral engine (apyib_amd.hamiltonian, which applies the
+field terms of hamiltonian.py:41-70).
+
+Pinned by the reference's golden AAT arrays (tests/test_oracle_aat.py).
+"""
+import copy
+
+import numpy as np
+
+from apyib_amd import gaussian
+from apyib_amd.hamiltonian import Hamiltonian
+from apyib_amd.molecule import Molecule
+
+from . import aats, scf, so, spatial, transforms
+
+
+def _ndocc(H):
+    return int(sum(H.molecule.true_atomic_number(a) for a in range(H.molecule.natom()))) // 2
+
+
+def solve(parameters, C0=None, funcs0=None):
+    """RHF (+ phase alignment when C0 is given) + the correlated method.
+    Returns (C, basis functions, T_list = [1, t1, t2], E_list)."""
+    H = Hamiltonian(parameters)
+    ndocc = _ndocc(H)
+    nbf = H.basis_set.nbf()
+    E_scf, C, eps, _ = scf.solve_scf(H.T, H.V, H.ERI, H.S, H.E_nuc, ndocc, parameters)
+    funcs = H.basis_set.functions
+    if C0 is not None:
+        C = transforms.compute_phase(gaussian.build_mixed_overlap(funcs0, funcs), C0, C)
+    m = parameters['method']
+    nfzc = H.basis_set.n_frozen_core() if parameters.get('freeze_core', False) else 0
+    E, t1, t2 = 0, 0, 0
+    if m != 'RHF':
+        C_list, _ = transforms.get_slices(nbf, ndocc, nfzc, m)
+        F_mo, _ = transforms.compute_F_MO(H.T + H.V, H.ERI, C, C_list, bool(nfzc))
+        eri_mo = transforms.compute_ERI_MO(H.ERI, C, C_list)
+        e_act = eps[nfzc:]
+        O = ndocc - nfzc
+        if m in ('MP2', 'CID', 'CISD'):
+            eri = eri_mo.swapaxes(1, 2)
+            if m == 'MP2':
+                E, t2 = spatial.solve_mp2(eri, e_act, O)
+            elif m == 'CID':
+                E, t2, _ = spatial.solve_cid(F_mo, eri, e_act, O, parameters)
+            else:
+                E, t1, t2, _ = spatial.solve_cisd(F_mo, eri, e_act, O, parameters)
+        else:
+            eri = transforms.compute_ERI_SO(eri_mo).swapaxes(1, 2)
+            F = transforms.compute_F_SO(F_mo)
+            e = np.repeat(e_act, 2)
+            O2 = 2 * O
+            if m == 'MP2_SO':
+                E, t2 = so.solve_mp2_so(eri, e[:O2], e[O2:], O2)
+            elif m == 'CID_SO':
+                E, t2, _ = so.solve_cid_so(F, eri, e[:O2], e[O2:], O2, parameters)
+            else:
+                E, t1, t2, _ = so.solve_cisd_so(F, eri, e[:O2], e[O2:], O2, parameters)
+    return C, funcs, [1, t1, t2], [E_scf, E, H.E_nuc], ndocc, nbf, nfzc
+
+
+def compute_AAT(parameters, dR, dH):
+    """fin_diff.py:267-372 on a copy of `parameters`: +dR then -dR on each
+    Cartesian coordinate (alpha = 3*atom + xyz), then +dH then -dH on each
+    F_mag component.  Returns the unperturbed solve and the four lists."""
+    p = copy.deepcopy(parameters)
+    p.setdefault('F_el', [0.0, 0.0, 0.0])
+    p.setdefault('F_mag', [0.0, 0.0, 0.0])
+    C0, f0, T0, E0, ndocc, nbf, nfzc = solve(p)
+    mol = Molecule.from_string(p['geom'])
+    geom = mol.coords.copy()
+    out = {'u': (C0, f0, T0), 'np': [], 'nn': [], 'mp': [], 'mn': []}
+    for sign, key in ((+1, 'np'), (-1, 'nn')):
+        for alpha in range(3 * mol.natom()):
+            g = geom.copy()
+            g[alpha // 3, alpha % 3] += sign * dR
+            q = dict(p, geom=mol.with_coords(g).geometry_string())
+            C, f, T, *_ = solve(q, C0, f0)
+            out[key].append((C, f, T))
+    for sign, key in ((+1, 'mp'), (-1, 'mn')):
+        for beta in range(3):
+            F = list(p['F_mag'])
+            F[beta] += sign * dH
+            q = dict(p, F_mag=F)
+            C, f, T, *_ = solve(q, C0, f0)
+            out[key].append((C, f, T))
+    return out, ndocc, nbf, nfzc
+
+
+def _ovl(bra, ket):
+    Cb, fb, _ = bra
+    Ck, fk, _ = ket
+    return transforms.compute_mo_overlap(gaussian.build_mixed_overlap(fb, fk), Cb, Ck)
+
+
+def overlap_sets(wfns, so_form):
+    """AAT.__init__ (aats.py:23-115): uu, up/un[beta], pu/nu[alpha],
+    pp/pn/np/nn[alpha][beta] (SO-expanded for the spin-orbital form)."""
+    f = (lambda S: transforms.compute_so_overlap(S)) if so_form else (lambda S: S)
+    u = wfns['u']
+    nA = len(wfns['np'])
+    ov = {'uu': f(_ovl(u, u)),
+          'up': [f(_ovl(u, w)) for w in wfns['mp']], 'un': [f(_ovl(u, w)) for w in wfns['mn']],
+          'pu': [f(_ovl(w, u)) for w in wfns['np']], 'nu': [f(_ovl(w, u)) for w in wfns['nn']]}
+    for kb, kn in (('pp', ('np', 'mp')), ('pn', ('np', 'mn')), ('np', ('nn', 'mp')), ('nn', ('nn', 'mn'))):
+        ov[kb] = [[f(_ovl(wfns[kn[0]][a], wfns[kn[1]][b])) for b in range(3)] for a in range(nA)]
+    return ov
+
+
+def amplitude_sets(wfns):
+    return {k: (v[2] if k == 'u' else [w[2] for w in v]) for k, v in wfns.items()}
+
+
+def compute_aats(parameters, dR, dH, normalization='full'):
+    """parallel.compute_parallel_aats (parallel.py:16-48) without the process
+    pool: the [3N, 3] AAT, spatial or spin-orbital by method."""
+    wfns, ndocc, nbf, nfzc = compute_AAT(parameters, dR, dH)
+    m = parameters['method']
+    so_form = m.endswith('_SO')
+    ov = overlap_sets(wfns, so_form)
+    T = amplitude_sets(wfns)
+    nA = len(wfns['np'])
+    I = np.zeros((nA, 3))
+    cache = {}
+    for a in range(nA):
+        for b in range(3):
+            if so_form:
+                I[a, b] = aats.so_aat(m, 2 * ndocc, 2 * (nbf - ndocc), ov, T, dR, dH, a, b, normalization, cache)
+            else:
+                I[a, b] = aats.spatial_aat(m, ndocc, nbf - ndocc, nfzc, ov, T, dR, dH, a, b, normalization, cache)
+    return I
