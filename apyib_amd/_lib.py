@@ -95,6 +95,13 @@ class Mp2NcArgs(ctypes.Structure):
     ]
 
 
+class Mp2NcInstance(ctypes.Structure):
+    _fields_ = [
+        ("O", c_i64), ("V", c_i64),
+        ("fock", c_vp), ("eps", c_vp), ("drv", c_vp), ("x", c_vp), ("x_new", c_vp), ("w", c_vp),
+    ]
+
+
 APY_AAT_NTERMS = 20
 
 
@@ -156,6 +163,10 @@ SIGNATURES = {
     "apy_mo_overlap": (c_int, [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_size,
                                c_vp]),
     "apy_phase_align": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "apy_mp2_nc_batched_workspace": (c_size, [c_int, ctypes.POINTER(Mp2NcInstance)]),
+    "apy_mp2_nc_batched_setup": (c_int, [c_int, ctypes.POINTER(Mp2NcInstance), c_vp, c_size, _P_I64, c_vp]),
+    "apy_mp2_nc_batched_step": (c_int, [c_int, c_i64, c_int, c_dbl, c_int, c_vp, c_vp, c_vp]),
+    "apy_mp2_nc_batched_energy": (c_int, [c_int, c_i64, c_int, c_dbl, c_vp, c_vp, c_vp]),
     "apy_aat_gamma_record": (c_i64, [c_i64, c_i64, c_i64]),
     "apy_aat_gamma_workspace": (c_size, [c_i64, c_i64, c_i64]),
     "apy_aat_gamma": (c_int, [c_i64, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_size, c_vp]),

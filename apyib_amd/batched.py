@@ -1,0 +1,226 @@
+"""Finite-difference instance sets solved as one batch on the GPU.
+
+The reference forms every finite-difference tensor from independent
+solves, one per +/- displacement (fin_diff.py:267-372, a Python loop over
+energy()/phase_corrected_energy()).  Here the instances of a set are
+solved together: `MP2NCBatch` runs the orbital-invariant spin-orbital MP2
+iteration (ci_wfn.py:210-212 driven by <ab||ij>; solve_MP2_SO at diagonal
+F, mp2_wfn.py:64-87) for all instances, of possibly different (O, V), in
+one launch per iteration (apy_mp2_nc_batched_step), with chunks of
+iterations captured in a HIP graph so the host checks convergence once per
+chunk.  `plain_solve` runs the same kernels one instance at a time with a
+convergence check every iteration: the per-displacement loop.
+
+`fd_mp2_instances` builds a seeded finite-difference set: a base system
+plus +/- h along nuclear-like (real symmetric F and real <pq||rs>
+perturbation) and field-like (i x antisymmetric F) directions.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib, synthetic
+from .kernels import stream_of, workspace
+from .so_ci import MP2_BLOCKS, SOIntegrals
+
+CDT = torch.complex128
+
+
+class Instance:
+    """One MP2 system on the device: F [n,n], eps [n], drv = <ab||ij> at
+    [i,j,a,b], w = <ij||ab>."""
+
+    def __init__(self, O, V, F, eps, drv, w):
+        self.O, self.V = int(O), int(V)
+        self.F = F.contiguous()
+        self.eps = eps.contiguous()
+        self.drv = drv.contiguous()
+        self.w = w.contiguous() if w is not None else None
+
+
+class MP2NCBatch:
+    def __init__(self, instances, transpose_f=False, drv_scale=1.0):
+        self.lib = _lib.load()
+        self.inst = list(instances)
+        self.n = len(self.inst)
+        self.dev = self.inst[0].F.device
+        self.transpose_f = int(bool(transpose_f))
+        self.drv_scale = float(drv_scale)
+        sizes = [I.O * I.O * I.V * I.V for I in self.inst]
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self._xbuf = torch.zeros(int(offs[-1]), dtype=CDT, device=self.dev)
+        self._xnbuf = torch.empty(int(offs[-1]), dtype=CDT, device=self.dev)
+        self.x = [self._xbuf[offs[k]:offs[k + 1]] for k in range(self.n)]
+        self.x_new = [self._xnbuf[offs[k]:offs[k + 1]] for k in range(self.n)]
+        tab = (_lib.Mp2NcInstance * self.n)()
+        for k, I in enumerate(self.inst):
+            tab[k] = _lib.Mp2NcInstance(I.O, I.V, I.F.data_ptr(), I.eps.data_ptr(), I.drv.data_ptr(),
+                                        self.x[k].data_ptr(), self.x_new[k].data_ptr(),
+                                        I.w.data_ptr() if I.w is not None else None)
+        self.table = tab
+        nbytes = int(self.lib.apy_mp2_nc_batched_workspace(self.n, tab))
+        # a private workspace: the tables stay valid while graphs replay
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+        items = ctypes.c_int64(0)
+        _lib.check(self.lib.apy_mp2_nc_batched_setup(self.n, tab, ctypes.c_void_p(self.ws.data_ptr()), nbytes,
+                                                     ctypes.byref(items), stream_of(self.dev)),
+                   "apy_mp2_nc_batched_setup")
+        self.items = items.value
+        self.rnorm = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+        self.energy_out = torch.zeros(self.n, dtype=CDT, device=self.dev)
+        self.iterations = 0
+        self._graph = None
+
+    def step(self, reverse):
+        _lib.check(self.lib.apy_mp2_nc_batched_step(self.n, self.items, self.transpose_f, self.drv_scale,
+                                                    int(reverse), ctypes.c_void_p(self.rnorm.data_ptr()),
+                                                    ctypes.c_void_p(self.ws.data_ptr()), stream_of(self.dev)),
+                   "apy_mp2_nc_batched_step")
+
+    def _capture(self, chunk):
+        """chunk (even) iterations x -> x_new -> x ... -> x as one graph."""
+        torch.cuda.synchronize(self.dev)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for q in range(chunk):
+                self.step(q % 2)
+        return g
+
+    def reset(self):
+        self._xbuf.zero_()
+        self.iterations = 0
+
+    def solve(self, tol=1e-12, maxit=200, chunk=4, graph=False):
+        """Iterate all instances until every residual norm is below tol,
+        checked once per chunk of iterations (chunk even; the amplitudes end
+        in x).  graph=True replays the chunk as a captured HIP graph (worth
+        it when the same batch is solved repeatedly).  Returns the iteration
+        count."""
+        chunk = max(2, chunk + (chunk % 2))
+        if graph and (self._graph is None or self._graph[0] != chunk):
+            self._graph = (chunk, self._capture(chunk))
+            self.reset()
+        it = 0
+        while it < maxit:
+            if graph:
+                self._graph[1].replay()
+            else:
+                for q in range(chunk):
+                    self.step(q % 2)
+            it += chunk
+            if float(self.rnorm.max().cpu()) ** 0.5 < tol:
+                break
+        self.iterations = it
+        return it
+
+    def solve_unbatched_steps(self, tol=1e-12, maxit=200):
+        """Step-by-step iteration with a host check after every step (the
+        plain-loop protocol), used on single-instance batches."""
+        it = 0
+        rev = 0
+        for it in range(1, maxit + 1):
+            self.step(rev)
+            rev ^= 1
+            if it > 1 and float(self.rnorm.max().cpu()) ** 0.5 < tol:
+                break
+        if rev == 1:                            # last step wrote x_new: copy back
+            for x, xn in zip(self.x, self.x_new):
+                x.copy_(xn)
+        self.iterations = it
+        return it
+
+    def energies(self, scale=0.25):
+        _lib.check(self.lib.apy_mp2_nc_batched_energy(self.n, self.items, 0, float(scale),
+                                                      ctypes.c_void_p(self.energy_out.data_ptr()),
+                                                      ctypes.c_void_p(self.ws.data_ptr()), stream_of(self.dev)),
+                   "apy_mp2_nc_batched_energy")
+        return self.energy_out
+
+
+def mp2_adjoint_directional(base, dF, dW, dV, tol=1e-12, maxit=200):
+    """Directional derivatives of the MP2 energy of `base` along directions
+    (dF_k [n,n], dW_k = d<ij||ab>, dV_k = d<ab||ij>; flattened device
+    tensors, dW/dV may be None) from one amplitude solve, one lambda solve
+    (the same batched kernels with F^T and the driver <ij||ab>/4) and the
+    gradient densities (apy_mp2_nc_densities):
+        dE = sum dE/dF . dF + sum t/4 . dW + sum lambda . dV."""
+    lib = _lib.load()
+    dev = base.F.device
+    O, V = base.O, base.V
+    tb = MP2NCBatch([base])
+    tb.solve(tol, maxit)
+    lb = MP2NCBatch([Instance(O, V, base.F, base.eps, base.w, None)], transpose_f=True, drv_scale=0.25)
+    lb.solve(tol, maxit)
+    t, lam = tb.x[0], lb.x[0]
+    n = O + V
+    G = torch.empty(n * n, dtype=CDT, device=dev)
+    nb = int(lib.apy_mp2_nc_density_workspace(O, V))
+    ws = workspace(nb, dev)
+    _lib.check(lib.apy_mp2_nc_densities(O, V, ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(lam.data_ptr()),
+                                        ctypes.c_void_p(G.data_ptr()), ctypes.c_void_p(ws.data_ptr()), nb,
+                                        stream_of(dev)), "apy_mp2_nc_densities")
+    K = len(dF)
+    out = np.zeros(K, dtype=np.complex128)
+    part = torch.empty(_lib.APY_MAXVEC, dtype=CDT, device=dev)
+    for xs, y, scale in ((dF, G, 1.0), (dW, t, 0.25), (dV, lam, 1.0)):
+        if xs is None:
+            continue
+        for s0 in range(0, K, _lib.APY_MAXVEC):
+            chunk = xs[s0:s0 + _lib.APY_MAXVEC]
+            arr = (ctypes.c_void_p * len(chunk))(*[x.data_ptr() for x in chunk])
+            nb = int(lib.apy_reduce_workspace(y.numel(), len(chunk)))
+            ws = workspace(nb, dev)
+            _lib.check(lib.apy_zdot_many(len(chunk), arr, ctypes.c_void_p(y.data_ptr()), y.numel(), 0,
+                                         ctypes.c_void_p(part.data_ptr()), ctypes.c_void_p(ws.data_ptr()), nb,
+                                         stream_of(dev)), "apy_zdot_many")
+            out[s0:s0 + len(chunk)] += scale * part[:len(chunk)].cpu().numpy()
+    return out, (G, t, lam)
+
+
+def plain_solve(instances, tol=1e-12, maxit=200):
+    """The per-displacement loop: the same kernels, one instance at a time,
+    a convergence check after every iteration.  Returns (energies, iters)."""
+    E, its = [], []
+    for I in instances:
+        b = MP2NCBatch([I])
+        its.append(b.solve_unbatched_steps(tol, maxit))
+        E.append(b.energies()[0])
+    return torch.stack(E), its
+
+
+def base_system(O, V, seed, device, std=0.05):
+    """(F, eps, ints) of a seeded synthetic system (MP2 blocks only)."""
+    F, eps = synthetic.fock(O, V, seed)
+    ints = SOIntegrals.synthetic(O, V, seed, std=std, device=device, blocks=MP2_BLOCKS)
+    return (torch.from_numpy(F).to(device), torch.from_numpy(np.asarray(eps, dtype=np.float64)).to(device), ints)
+
+
+def fd_mp2_instances(O, V, seed, natom, h=1e-4, device="cuda", sizes=None):
+    """A finite-difference set of 2 (3 natom + 3) instances: for each
+    nuclear-like direction k (real symmetric dF_k, real <pq||rs> draw with
+    tag 100 + k) and each field-like direction (i x antisymmetric dF), the
+    base system displaced by +h and -h.  With `sizes` (a list of (O, V) per
+    direction), each direction uses its own base system of that size (the
+    uneven-size stress case).  Returns (instances, directions) where
+    directions[d] = (kind, seed_d, (O, V)) and instances 2d, 2d+1 are +h, -h."""
+    ndir = 3 * natom + 3
+    dirs, out = [], []
+    cache = {}
+    for d in range(ndir):
+        Od, Vd = sizes[d] if sizes is not None else (O, V)
+        if (Od, Vd) not in cache:
+            cache[(Od, Vd)] = base_system(Od, Vd, seed + 7919 * Od + 104729 * Vd, device)
+        F0, eps, _ = cache[(Od, Vd)]
+        kind = 'nuclear' if d < 3 * natom else 'field'
+        sd = seed * 1000 + d
+        dF = torch.from_numpy(synthetic.fock_perturbation(Od + Vd, sd, kind)).to(device)
+        dirs.append((kind, sd, (Od, Vd)))
+        for sgn in (+1, -1):
+            ints = SOIntegrals.synthetic(Od, Vd, seed + 7919 * Od + 104729 * Vd, device=device, blocks=MP2_BLOCKS)
+            if kind == 'nuclear':
+                ints.add_synthetic(sd, tag=100 + d, std=1.0, scale=sgn * h, accumulate=True, real_only=True)
+            n = Od * Od * Vd * Vd
+            out.append(Instance(Od, Vd, F0 + sgn * h * dF, eps, ints.blocks['vvoo_t'].view(n),
+                                ints.blocks['oovv'].view(n)))
+    return out, dirs

@@ -1,0 +1,265 @@
+// Batched orbital-invariant spin-orbital MP2 over independent instances of
+// possibly different sizes (the +/- displaced systems of a finite-difference
+// set), one launch per iteration for the whole set.
+//
+// Per instance k (O_k, V_k, F_k, eps_k, drv_k) the iteration is the Jacobi
+// step of the non-canonical MP2 equations (ci_wfn.py:210-212 with the
+// driver <ab||ij>, solve_MP2_SO at diagonal F, mp2_wfn.py:64-87):
+//   r_ij[a,b] = s drv_ij[a,b] + Y_ij[a,b] - Y_ij[b,a] - Z_ij[a,b] + Z_ji[a,b]
+//   Y_ij[a,b] = sum_e x_ij[a,e] Fvv(b,e),   Z_ij[a,b] = sum_m Foo(m,j) x_im[a,b]
+//   x_new = x + r / (e_i + e_j - e_a - e_b)
+// with Fvv(b,e) = F[O+b][O+e], Foo(m,j) = F[m][j]; transpose_f uses F^T
+// (the adjoint lambda equations).  One workgroup owns one (instance, i, j)
+// pair: it stages x_ij in LDS, forms the V x V residual block and writes
+// x_new_ij and its squared residual norm; a second kernel reduces the pair
+// norms per instance (deterministic order).  All tables live in the
+// workspace, written once by apy_mp2_nc_batched_setup, so a sequence of steps
+// can be captured in a HIP graph.
+#include "common.h"
+
+#include <vector>
+
+namespace {
+
+constexpr int BLK = 256;
+constexpr int LDS_V = 64;          // x_ij staged in LDS when V <= LDS_V
+
+struct Item {
+  int32_t inst, i, j, pad;
+};
+
+struct Header {
+  int64_t n, items;
+};
+
+inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct WsLayout {
+  size_t inst, item, off, part, total;
+};
+
+inline WsLayout layout(int n, int64_t items) {
+  WsLayout L;
+  L.inst = al256(sizeof(Header));
+  L.item = L.inst + al256((size_t)n * sizeof(apy_mp2_nc_instance));
+  L.off = L.item + al256((size_t)items * sizeof(Item));
+  L.part = L.off + al256((size_t)(n + 1) * sizeof(int64_t));
+  L.total = L.part + al256((size_t)items * sizeof(double));
+  return L;
+}
+
+__device__ inline void cfma(double2& acc, double2 a, double2 b) {
+  acc.x += a.x * b.x - a.y * b.y;
+  acc.y += a.x * b.y + a.y * b.x;
+}
+
+// One (instance, i, j) pair per workgroup.  Y_ij = x_ij Fvv^T is formed in
+// 2 x 2 blocks: a thread owns the block pair {(A,B), (B,A)} (A <= B), so
+// it holds Y[a][b] and Y[b][a] for its elements in registers and needs no
+// second pass; x_ij is staged in LDS when V <= LDS_V.
+__global__ __launch_bounds__(BLK) void nc_pair_step(const apy_mp2_nc_instance* __restrict__ insts,
+                                                    const Item* __restrict__ items, int transpose_f, double s,
+                                                    int reverse, double* __restrict__ part) {
+  __shared__ double2 xs[LDS_V * LDS_V];
+  __shared__ double red[BLK];
+  const Item it = items[blockIdx.x];
+  const apy_mp2_nc_instance I = insts[it.inst];
+  const int64_t O = I.O, V = I.V, n = O + V;
+  const int64_t i = it.i, j = it.j;
+  const double2* F = (const double2*)I.fock;
+  const double* eps = (const double*)I.eps;
+  const double2* drv = (const double2*)I.drv;
+  const double2* x = (const double2*)(reverse ? I.x_new : I.x);
+  double2* xn = (double2*)(reverse ? I.x : I.x_new);
+  const int64_t VV = V * V;
+  const double2* xij = x + (i * O + j) * VV;
+  double2* xnij = xn + (i * O + j) * VV;
+  double acc = 0.0;
+  if (i == j) {
+    for (int64_t e = threadIdx.x; e < VV; e += BLK) xnij[e] = xij[e];
+  } else {
+    const bool lds = V <= LDS_V;
+    if (lds)
+      for (int64_t e = threadIdx.x; e < VV; e += BLK) xs[e] = xij[e];
+    __syncthreads();
+    const double2* X = lds ? (const double2*)xs : xij;
+    // Fvv(b, q): F[O+b][O+q] (or F[O+q][O+b] for F^T)
+    const int64_t fr = transpose_f ? 1 : n, fc = transpose_f ? n : 1;
+    const double2* Fv = F + O * n + O;
+    const int64_t nb = (V + 1) / 2;
+    const int64_t npair = nb * (nb + 1) / 2;
+    for (int64_t p = threadIdx.x; p < npair; p += BLK) {
+      // unordered block pair (A, B), A <= B, from the triangular index p
+      int64_t A = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+      while (A * (A + 1) / 2 > p) --A;
+      while ((A + 1) * (A + 2) / 2 <= p) ++A;
+      const int64_t Bq = p - A * (A + 1) / 2;          // Bq <= A
+      const int64_t a0 = 2 * Bq, b0 = 2 * A;            // rows of block 1 (a0..a0+1), block 2 (b0..b0+1)
+      const bool a1ok = a0 + 1 < V, b1ok = b0 + 1 < V;
+      double2 yab[2][2], yba[2][2];
+      for (int u = 0; u < 2; ++u)
+        for (int w = 0; w < 2; ++w) { yab[u][w] = make_double2(0.0, 0.0); yba[u][w] = make_double2(0.0, 0.0); }
+      for (int64_t q = 0; q < V; ++q) {
+        const double2 xa0 = X[a0 * V + q], xa1 = a1ok ? X[(a0 + 1) * V + q] : make_double2(0.0, 0.0);
+        const double2 xb0 = X[b0 * V + q], xb1 = b1ok ? X[(b0 + 1) * V + q] : make_double2(0.0, 0.0);
+        const double2 fa0 = Fv[a0 * fr + q * fc], fa1 = a1ok ? Fv[(a0 + 1) * fr + q * fc] : make_double2(0.0, 0.0);
+        const double2 fb0 = Fv[b0 * fr + q * fc], fb1 = b1ok ? Fv[(b0 + 1) * fr + q * fc] : make_double2(0.0, 0.0);
+        cfma(yab[0][0], xa0, fb0); cfma(yab[0][1], xa0, fb1);
+        cfma(yab[1][0], xa1, fb0); cfma(yab[1][1], xa1, fb1);
+        cfma(yba[0][0], xb0, fa0); cfma(yba[0][1], xb0, fa1);
+        cfma(yba[1][0], xb1, fa0); cfma(yba[1][1], xb1, fa1);
+      }
+      // elements (a, b) of block (a0.., b0..) and their mirrors (b, a); on the
+      // diagonal block (A == B) the mirror is the same block: visit it once.
+      for (int side = 0; side < 2; ++side) {
+        if (side == 1 && a0 == b0) break;
+        for (int u = 0; u < 2; ++u)
+          for (int w = 0; w < 2; ++w) {
+            const int64_t a = side == 0 ? a0 + u : b0 + w;
+            const int64_t b = side == 0 ? b0 + w : a0 + u;
+            if (a >= V || b >= V) continue;
+            const int64_t e = a * V + b;
+            if (a == b) { xnij[e] = X[e]; continue; }
+            // Y[a][b] - Y[b][a]: side 0 -> yab[u][w] - yba[w][u]; side 1 -> yba[w][u] - yab[u][w]
+            double2 y = side == 0 ? make_double2(yab[u][w].x - yba[w][u].x, yab[u][w].y - yba[w][u].y)
+                                  : make_double2(yba[w][u].x - yab[u][w].x, yba[w][u].y - yab[u][w].y);
+            double2 zij = make_double2(0.0, 0.0), zji = make_double2(0.0, 0.0);
+            for (int64_t m = 0; m < O; ++m) {
+              const double2 fj = transpose_f ? F[j * n + m] : F[m * n + j];
+              const double2 fi = transpose_f ? F[i * n + m] : F[m * n + i];
+              cfma(zij, fj, x[(i * O + m) * VV + e]);
+              cfma(zji, fi, x[(j * O + m) * VV + e]);
+            }
+            const double2 d = drv[(i * O + j) * VV + e];
+            const double rr = s * d.x + y.x - zij.x + zji.x;
+            const double ri = s * d.y + y.y - zij.y + zji.y;
+            const double D = eps[i] + eps[j] - eps[O + a] - eps[O + b];
+            const double2 xv = X[e];
+            xnij[e] = make_double2(xv.x + rr / D, xv.y + ri / D);
+            acc += rr * rr + ri * ri;
+          }
+      }
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(BLK) void per_instance_sum(const int64_t* __restrict__ off,
+                                                        const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[BLK];
+  const int64_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+  double acc = 0.0;
+  for (int64_t q = b + threadIdx.x; q < e; q += BLK) acc += part[q];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// energy[k] = scale * sum w_k x_k  (bilinear)
+__global__ __launch_bounds__(BLK) void nc_energy(const apy_mp2_nc_instance* __restrict__ insts, int use_x_new,
+                                                 double scale, double2* __restrict__ out) {
+  __shared__ double2 red[BLK];
+  const apy_mp2_nc_instance I = insts[blockIdx.x];
+  const double2* w = (const double2*)I.w;
+  const double2* x = (const double2*)(use_x_new ? I.x_new : I.x);
+  const int64_t N = I.O * I.O * I.V * I.V;
+  double2 acc = make_double2(0.0, 0.0);
+  if (w)
+    for (int64_t e = threadIdx.x; e < N; e += BLK) cfma(acc, w[e], x[e]);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int q = BLK / 2; q > 0; q >>= 1) {
+    if ((int)threadIdx.x < q) {
+      red[threadIdx.x].x += red[threadIdx.x + q].x;
+      red[threadIdx.x].y += red[threadIdx.x + q].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = make_double2(scale * red[0].x, scale * red[0].y);
+}
+
+int64_t count_items(int n, const apy_mp2_nc_instance* inst) {
+  int64_t items = 0;
+  for (int k = 0; k < n; ++k) items += inst[k].O * inst[k].O;
+  return items;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t apy_mp2_nc_batched_workspace(int n, const apy_mp2_nc_instance* inst) {
+  if (n < 1 || !inst) return 0;
+  return layout(n, count_items(n, inst)).total;
+}
+
+int apy_mp2_nc_batched_setup(int n, const apy_mp2_nc_instance* inst, void* workspace, size_t ws_bytes,
+                             int64_t* n_items, void* stream) {
+  if (n < 1 || n > 65535 || !inst || !n_items) return APY_E_ARG;
+  for (int k = 0; k < n; ++k) {
+    const apy_mp2_nc_instance& I = inst[k];
+    if (I.O < 1 || I.V < 1 || I.O > 32767 || !I.fock || !I.eps || !I.drv || !I.x || !I.x_new || I.x == I.x_new)
+      return APY_E_ARG;
+  }
+  const int64_t items = count_items(n, inst);
+  if (items > 2147483647) return APY_E_ARG;
+  const WsLayout L = layout(n, items);
+  if (!workspace || ws_bytes < L.total) return APY_E_WORKSPACE;
+  std::vector<Item> it;
+  it.reserve(items);
+  std::vector<int64_t> off(n + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    off[k] = (int64_t)it.size();
+    for (int32_t i = 0; i < (int32_t)inst[k].O; ++i)
+      for (int32_t j = 0; j < (int32_t)inst[k].O; ++j) it.push_back(Item{k, i, j, 0});
+  }
+  off[n] = (int64_t)it.size();
+  Header h{n, items};
+  char* w = (char*)workspace;
+  hipStream_t st = (hipStream_t)stream;
+  // pageable sources: each copy returns once the host buffer has been read
+  APY_HIP_TRY(hipMemcpyAsync(w, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  APY_HIP_TRY(hipMemcpyAsync(w + L.inst, inst, n * sizeof(apy_mp2_nc_instance), hipMemcpyHostToDevice, st));
+  APY_HIP_TRY(hipMemcpyAsync(w + L.item, it.data(), items * sizeof(Item), hipMemcpyHostToDevice, st));
+  APY_HIP_TRY(hipMemcpyAsync(w + L.off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  APY_HIP_TRY(hipStreamSynchronize(st));
+  *n_items = items;
+  return APY_OK;
+}
+
+int apy_mp2_nc_batched_step(int n, int64_t n_items, int transpose_f, double drv_scale, int reverse, void* rnorm,
+                            const void* workspace, void* stream) {
+  if (n < 1 || n > 65535 || n_items < 1 || n_items > 2147483647 || !rnorm || !workspace) return APY_E_ARG;
+  const WsLayout L = layout(n, n_items);
+  const char* w = (const char*)workspace;
+  hipStream_t st = (hipStream_t)stream;
+  double* part = (double*)(w + L.part);
+  nc_pair_step<<<(unsigned)n_items, BLK, 0, st>>>((const apy_mp2_nc_instance*)(w + L.inst),
+                                                   (const Item*)(w + L.item), transpose_f, drv_scale, reverse, part);
+  APY_LAUNCH_CHECK();
+  per_instance_sum<<<(unsigned)n, BLK, 0, st>>>((const int64_t*)(w + L.off), part, (double*)rnorm);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_mp2_nc_batched_energy(int n, int64_t n_items, int use_x_new, double scale, void* energy,
+                              const void* workspace, void* stream) {
+  if (n < 1 || n > 65535 || n_items < 1 || !energy || !workspace) return APY_E_ARG;
+  const WsLayout L = layout(n, n_items);
+  nc_energy<<<(unsigned)n, BLK, 0, (hipStream_t)stream>>>(
+      (const apy_mp2_nc_instance*)((const char*)workspace + L.inst), use_x_new, scale, (double2*)energy);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+}  // extern "C"

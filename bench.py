@@ -144,6 +144,73 @@ def cpu_baseline(solver, O, V, flops_step, seed):
     }
 
 
+def aat_set_measurement(dev, seed, tol=1e-12):
+    """Second metric: a finite-difference tensor set solved batched (one
+    launch per iteration for all instances, graph-captured chunks) plus the
+    adjoint gradient, against the plain per-displacement loop (the same
+    kernels, one instance at a time, host convergence check per iteration).
+
+    Set A: 24 instances at O=10, V=38 (3 atoms: 9 nuclear + 3 field
+    directions, +/- 1e-4).  Set B: 36 instances at O~16, V~60 with seeded
+    uneven sizes (18 directions).  MP2 (orbital-invariant spin-orbital)."""
+    import numpy as np
+    import torch
+    from apyib_amd.batched import Instance, MP2NCBatch, fd_mp2_instances, mp2_adjoint_directional, plain_solve
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    res = {}
+    rng = np.random.default_rng(seed + 17)
+    sets = {
+        "A": dict(O=10, V=38, natom=3, sizes=None),
+        "B": dict(O=16, V=60, natom=5, sizes=[(int(rng.integers(14, 19)), int(rng.integers(54, 67)))
+                                              for _ in range(18)]),
+    }
+    for name, cfg in sets.items():
+        insts, dirs = fd_mp2_instances(cfg["O"], cfg["V"], seed, cfg["natom"], h=1e-4, device=dev,
+                                       sizes=cfg["sizes"])
+        plain_solve(insts[:2], tol)                                   # warm both paths once
+        MP2NCBatch(insts[:2]).solve(tol)
+        (Ep, its), t_plain = timed(lambda: plain_solve(insts, tol))
+
+        def batched():
+            b = MP2NCBatch(insts)
+            b.solve(tol, maxit=400)
+            return b.energies().clone(), b.iterations
+        (Eb, itb), t_batch = timed(batched)
+        entry = {"instances": len(insts), "directions": len(dirs),
+                 "sizes": sorted({(I.O, I.V) for I in insts}) if cfg["sizes"] else [(cfg["O"], cfg["V"])],
+                 "plain_loop_s": round(t_plain, 5), "batched_s": round(t_batch, 5),
+                 "speedup_batched": round(t_plain / t_batch, 2),
+                 "iterations_plain_max": int(max(its)), "iterations_batched": int(itb),
+                 "max_abs_dE_batched_vs_plain": float(torch.abs(Eb - Ep).max().cpu())}
+        if cfg["sizes"] is None:
+            h = 1e-4
+            fd = ((Ep[0::2] - Ep[1::2]) / (2 * h)).cpu().numpy()
+            P, M = insts[0], insts[1]
+            base = Instance(cfg["O"], cfg["V"], 0.5 * (P.F + M.F), P.eps, 0.5 * (P.drv + M.drv), 0.5 * (P.w + M.w))
+            dF = [((insts[2 * k].F - insts[2 * k + 1].F) / (2 * h)).reshape(-1) for k in range(len(dirs))]
+            dW = [(insts[2 * k].w - insts[2 * k + 1].w) / (2 * h) for k in range(len(dirs))]
+            dV = [(insts[2 * k].drv - insts[2 * k + 1].drv) / (2 * h) for k in range(len(dirs))]
+            mp2_adjoint_directional(base, dF[:1], dW[:1], dV[:1], tol)        # warm
+            (an, (G, t, lam)), t_adj = timed(lambda: mp2_adjoint_directional(base, dF, dW, dV, tol))
+            # error relative to the natural scale of each directional sum, sum |dE/dx| |dx|
+            scale = np.array([float((torch.sum((G * dF[k]).abs()) + 0.25 * torch.sum((t * dW[k]).abs())
+                                     + torch.sum((lam * dV[k]).abs())).cpu()) for k in range(len(dirs))])
+            err = np.abs(an - fd) / scale
+            entry.update({"adjoint_s": round(t_adj, 5), "speedup_adjoint_vs_plain_fd": round(t_plain / t_adj, 2),
+                          "max_err_adjoint_vs_fd_rel_to_scale": float(err.max()),
+                          "fd_step": h})
+        res[name] = entry
+        log(f"aat set {name}: {entry}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +221,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-tune", action="store_true", help="skip the GEMM autotuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-aat-set", action="store_true", help="skip the batched finite-difference set timing")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -211,6 +279,10 @@ def main():
     value = flops_step / (ms_step * 1e-3) / 1e12
     log(f"step {ms_step:.3f} ms, PPL kernel {ppl_avg:.3f} ms ({achieved:.1f} TFLOP/s), E = {E}")
 
+    aat_set = None
+    if not args.no_aat_set:
+        aat_set = aat_set_measurement(dev, args.seed)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -248,6 +320,7 @@ def main():
             "traffic": read_traffic(),
         },
         "cpu_baseline": cpu,
+        "aat_set": aat_set,
     }
     print(json.dumps(out), flush=True)
 

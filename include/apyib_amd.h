@@ -273,6 +273,35 @@ int apy_mp2_nc_densities(int64_t O, int64_t V, const void* t, const void* lam, v
                          size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * The same non-canonical MP2 (or lambda) iteration batched over independent
+ * instances of possibly different (O, V): the +/- displaced systems of a
+ * finite-difference set (fin_diff.py:267-372 solves them one at a time).
+ * One workgroup per (instance, i, j) pair; every step covers all instances.
+ * setup uploads the instance and pair tables into the workspace (host
+ * arrays; synchronises the stream) and returns the pair count; step and
+ * energy read only device memory, so a sequence of them can be captured in
+ * a HIP graph.  reverse = 1 iterates x_new -> x (ping-pong without
+ * re-setup).  rnorm[k] = sum |r_k|^2 (double); energy[k] = scale *
+ * sum w_k x_k (complex; w may be NULL).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int64_t O, V;
+  const void* fock;  /* [(O+V)][(O+V)] complex */
+  const void* eps;   /* [O+V] double: Jacobi denominators */
+  const void* drv;   /* [O][O][V][V] driver (<ab||ij> at [i,j,a,b], or <ij||ab>) */
+  void* x;           /* [O][O][V][V] amplitudes */
+  void* x_new;       /* [O][O][V][V] (distinct from x) */
+  const void* w;     /* [O][O][V][V] <ij||ab> for the energy, or NULL */
+} apy_mp2_nc_instance;
+size_t apy_mp2_nc_batched_workspace(int n, const apy_mp2_nc_instance* inst);
+int apy_mp2_nc_batched_setup(int n, const apy_mp2_nc_instance* inst, void* workspace, size_t ws_bytes,
+                             int64_t* n_items, void* stream);
+int apy_mp2_nc_batched_step(int n, int64_t n_items, int transpose_f, double drv_scale, int reverse, void* rnorm,
+                            const void* workspace, void* stream);
+int apy_mp2_nc_batched_energy(int n, int64_t n_items, int use_x_new, double scale, void* energy,
+                              const void* workspace, void* stream);
+
+/* ------------------------------------------------------------------------
  * Vector reductions used by DIIS and the convergence test.
  * ---------------------------------------------------------------------- */
 #define APY_MAXVEC 16

@@ -1,0 +1,131 @@
+"""Batched finite-difference MP2 instance sets on the GPU: the one-launch
+batched iteration against the same kernels run per instance, against the
+oracle, and the finite-difference derivatives it gives against the adjoint
+gradient."""
+import numpy as np
+import pytest
+import torch
+
+from apyib_amd import synthetic
+from apyib_amd.adjoint import MP2Adjoint
+from apyib_amd.batched import Instance, MP2NCBatch, fd_mp2_instances, plain_solve
+from oracle import adjoint
+
+pytestmark = pytest.mark.gpu
+
+
+def _host_instance(O, V, seed, dev):
+    F, A, eps = synthetic.system(O, V, seed)
+    W = A[:O, :O, O:, O:].copy()
+    Vd = A[O:, O:, :O, :O].transpose(2, 3, 0, 1).copy()
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    inst = Instance(O, V, d(F), d(np.asarray(eps, dtype=np.float64)), d(Vd).reshape(-1), d(W).reshape(-1))
+    return inst, (F, W, Vd, eps)
+
+
+def test_batched_equals_per_instance_bitwise(dev):
+    """Uneven sizes; the same number of iterations batched (graph) and one
+    instance at a time gives identical amplitudes."""
+    sizes = [(4, 6), (3, 9), (6, 5), (2, 11), (5, 7)]
+    insts = [_host_instance(O, V, 10 + k, dev)[0] for k, (O, V) in enumerate(sizes)]
+    b = MP2NCBatch(insts)
+    b.solve(tol=0.0, maxit=12, chunk=6)          # exactly 12 iterations
+    for k, I in enumerate(insts):
+        s = MP2NCBatch([I])
+        for q in range(12):
+            s.step(q % 2)
+        assert torch.equal(b.x[k], s.x[0])
+    assert b.iterations == 12
+
+
+def test_batched_converges_to_oracle(dev):
+    sizes = [(4, 6), (3, 9), (6, 5)]
+    data = [_host_instance(O, V, 20 + k, dev) for k, (O, V) in enumerate(sizes)]
+    b = MP2NCBatch([d[0] for d in data])
+    b.solve(tol=1e-13, maxit=400, chunk=8)
+    E = b.energies().cpu().numpy()
+    for k, (_, (F, W, Vd, eps)) in enumerate(data):
+        E_ref, t_ref = adjoint.energy(F, W, Vd, eps, sizes[k][0])
+        assert abs(E[k] - E_ref) < 1e-12
+        assert np.abs(b.x[k].cpu().numpy().reshape(t_ref.shape) - t_ref).max() < 1e-11
+
+
+def test_lambda_batch_matches_adjoint(dev):
+    inst, (F, W, Vd, eps) = _host_instance(5, 8, 3, dev)
+    lam_inst = Instance(5, 8, inst.F, inst.eps, inst.w, None)
+    b = MP2NCBatch([lam_inst], transpose_f=True, drv_scale=0.25)
+    b.solve(tol=1e-14, maxit=400)
+    _, _, lam_ref, _, _, _ = adjoint.gradients(F, W, Vd, eps, 5)
+    assert np.abs(b.x[0].cpu().numpy().reshape(lam_ref.shape) - lam_ref).max() < 1e-12
+
+
+def test_plain_loop_energies_match_batched(dev):
+    insts, _ = fd_mp2_instances(6, 14, 5, natom=1, device=dev)
+    b = MP2NCBatch(insts)
+    b.solve(tol=1e-12, maxit=400)
+    Eb = b.energies()
+    Ep, its = plain_solve(insts, tol=1e-12)
+    assert torch.abs(Eb - Ep).max().item() < 1e-12
+    assert len(its) == len(insts)
+
+
+def test_fd_derivatives_match_adjoint(dev):
+    """Central differences from the batched +/- set against the adjoint
+    directional derivatives at the base point (h = 1e-4; the central
+    difference error is O(h^2))."""
+    O, V, seed, h = 6, 14, 8, 1e-4
+    insts, dirs = fd_mp2_instances(O, V, seed, natom=1, h=h, device=dev)
+    b = MP2NCBatch(insts)
+    b.solve(tol=1e-14, maxit=600)
+    E = b.energies().cpu().numpy()
+    fd = [(E[2 * d] - E[2 * d + 1]) / (2 * h) for d in range(len(dirs))]
+    # base point = average of the +/- pair of direction 0 (F and integrals)
+    Ip, Im = insts[0], insts[1]
+    F0 = 0.5 * (Ip.F + Im.F)
+    W0 = (0.5 * (Ip.w + Im.w)).view(O, O, V, V)
+    V0 = (0.5 * (Ip.drv + Im.drv)).view(O, O, V, V)
+    s = MP2Adjoint(F0, W0, V0, Ip.eps, O, tol=1e-14)
+    s.energy()
+    GF, GW, GV = s.gradients()
+    for d in range(len(dirs)):
+        P, M = insts[2 * d], insts[2 * d + 1]
+        dF = (P.F - M.F) / (2 * h)
+        dW = ((P.w - M.w) / (2 * h)).view(O, O, V, V)
+        dV = ((P.drv - M.drv) / (2 * h)).view(O, O, V, V)
+        an = complex((torch.sum(GF * dF) + torch.sum(GW * dW) + torch.sum(GV * dV)).cpu())
+        scale = float((torch.sum((GF * dF).abs()) + torch.sum((GW * dW).abs()) + torch.sum((GV * dV).abs())).cpu())
+        assert abs(fd[d] - an) < 1e-7 * scale, (d, fd[d], an)
+
+
+def test_graph_replay_equals_direct_launches(dev):
+    """The same chunk replayed from a captured HIP graph (several replays,
+    with eager work in between) gives the directly launched iterates."""
+    sizes = [(4, 6), (3, 9)]
+    insts = [_host_instance(O, V, 40 + k, dev)[0] for k, (O, V) in enumerate(sizes)]
+    a, b = MP2NCBatch(insts), MP2NCBatch(insts)
+    a.solve(tol=0.0, maxit=12, chunk=4, graph=True)
+    b.solve(tol=0.0, maxit=12, chunk=4, graph=False)
+    for k in range(len(insts)):
+        assert torch.equal(a.x[k], b.x[k])
+    assert torch.equal(a.rnorm, b.rnorm)
+
+
+def test_adjoint_directional_matches_fd(dev):
+    from apyib_amd.batched import mp2_adjoint_directional
+    O, V, seed, h = 5, 12, 9, 1e-4
+    insts, dirs = fd_mp2_instances(O, V, seed, natom=1, h=h, device=dev)
+    b = MP2NCBatch(insts)
+    b.solve(tol=1e-14, maxit=600)
+    E = b.energies().cpu().numpy()
+    P, M = insts[0], insts[1]
+    base = Instance(O, V, 0.5 * (P.F + M.F), P.eps, 0.5 * (P.drv + M.drv), 0.5 * (P.w + M.w))
+    nd = len(dirs)
+    dF = [((insts[2 * k].F - insts[2 * k + 1].F) / (2 * h)).reshape(-1) for k in range(nd)]
+    dW = [(insts[2 * k].w - insts[2 * k + 1].w) / (2 * h) for k in range(nd)]
+    dV = [(insts[2 * k].drv - insts[2 * k + 1].drv) / (2 * h) for k in range(nd)]
+    an, (G, t, lam) = mp2_adjoint_directional(base, dF, dW, dV, tol=1e-14)
+    for k in range(nd):
+        fd = (E[2 * k] - E[2 * k + 1]) / (2 * h)
+        scale = float((torch.sum((G * dF[k]).abs()) + 0.25 * torch.sum((t * dW[k]).abs())
+                       + torch.sum((lam * dV[k]).abs())).cpu())
+        assert abs(an[k] - fd) < 1e-7 * scale
