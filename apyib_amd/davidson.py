@@ -1,0 +1,197 @@
+"""Lowest root of the Hermitian spin-orbital CISD matrix by Davidson
+iteration, and its energy gradient with respect to F, on the GPU.
+
+The operator is the reference's CISD (ci_wfn.py:309-334) with the
+P(ij)P(ab) f_bj t_ia coupling included (SOCI(fov_coupling=True)), so the
+matrix is Hermitian and the lowest root variational; with canonical
+Hartree-Fock orbitals (F_ov = 0) it is the reference's solution.  The
+reference's Jacobi + DIIS iteration diverges on systems far from
+diagonal dominance; Davidson converges there.
+
+Vectors live on the device with layout [c1 | c2 | c0] (length L + 1);
+inner products count each determinant once (c2 dense, weight 1/4).  Every
+vector operation is a library call: sigma (apy_cisd_so_step, SIGMA mode),
+the reference row (apy_cisd_so_energy), CI-metric dot products
+(apy_zdot_many over the three segments), linear combinations
+(apy_zcombine) and the preconditioner (apy_ci_precondition).  The k x k
+projected eigenproblem is solved on the host.
+
+The gradient (adjoint) needs no iteration history: for the variational
+root, dE/dF = D, the one-particle density of the normalised vector
+(apy_cisd_so_opd); the left eigenvector is the conjugate of the right one.
+`CISDEnergy` wraps E(F) with this backward in a torch.autograd.Function.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .kernels import stream_of, workspace
+
+CDT = torch.complex128
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class CISDDavidson:
+    def __init__(self, soci, max_space=16):
+        if not soci.fov or not soci.singles:
+            raise ValueError("Davidson needs the Hermitian CISD operator: SOCI(..., singles=True, fov_coupling=True)")
+        self.soci = soci
+        self.lib = soci.lib
+        self.dev = soci.device
+        self.O, self.V = soci.O, soci.V
+        self.OV, self.L = soci.OV, soci.L
+        self.max_space = int(max_space)
+        self.B = torch.zeros((self.max_space, self.L + 1), dtype=CDT, device=self.dev)
+        self.S = torch.zeros((self.max_space, self.L + 1), dtype=CDT, device=self.dev)
+        self.work = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
+        self.part = torch.zeros(_lib.APY_MAXVEC, dtype=CDT, device=self.dev)
+        self.c = None
+        self.E = None
+        self.iterations = 0
+
+    # -------------------------------------------------------------- primitives
+    def _sigma(self, q):
+        """S[q] = H_N B[q] (needs the host value of the reference weight)."""
+        b, s = self.B[q], self.S[q]
+        c0 = complex(b[self.L].cpu())
+        self.soci.sigma(b[:self.L], s[:self.L], c0=c0)
+        self.soci.energy(b[:self.L], out=s[self.L:])
+
+    def dots(self, xs, y):
+        """CI-metric <x_m|y> for m < len(xs) (host complex array)."""
+        out = np.zeros(len(xs), dtype=np.complex128)
+        segs = ((0, self.OV, 1.0), (self.OV, self.L, 0.25), (self.L, self.L + 1, 1.0))
+        for s0 in range(0, len(xs), _lib.APY_MAXVEC):
+            chunk = xs[s0:s0 + _lib.APY_MAXVEC]
+            acc = 0
+            for lo, hi, w in segs:
+                arr = (ctypes.c_void_p * len(chunk))(*[x[lo:hi].data_ptr() for x in chunk])
+                nb = int(self.lib.apy_reduce_workspace(hi - lo, len(chunk)))
+                ws = workspace(nb, self.dev)
+                _lib.check(self.lib.apy_zdot_many(len(chunk), arr, _vp(y[lo:hi]), hi - lo, 1, _vp(self.part), _vp(ws),
+                                                  nb, stream_of(self.dev)), "apy_zdot_many")
+                acc = acc + w * self.part[:len(chunk)].cpu().numpy()
+            out[s0:s0 + len(chunk)] = acc
+        return out
+
+    def combine(self, xs, coef, y):
+        """y = sum c_m x_m.  y may be one of the x_m when len(xs) fits one
+        call (the kernel reads every input at an index before writing it)."""
+        M = _lib.APY_MAXVEC
+        if len(xs) > M and any(x.data_ptr() == y.data_ptr() for x in xs):
+            raise ValueError("combine: output aliases an input of a multi-call combination")
+        bounds = [(0, min(M, len(xs)))]
+        while bounds[-1][1] < len(xs):               # later calls accumulate through y: M - 1 inputs each
+            lo = bounds[-1][1]
+            bounds.append((lo, min(lo + M - 1, len(xs))))
+        for lo, hi in bounds:
+            chunk = list(xs[lo:hi])
+            c = list(coef[lo:hi])
+            if lo > 0:
+                chunk.append(y)
+                c.append(1.0)
+            arr = (ctypes.c_void_p * len(chunk))(*[x.data_ptr() for x in chunk])
+            cc = np.ascontiguousarray(np.asarray(c, dtype=np.complex128)).view(np.float64)
+            _lib.check(self.lib.apy_zcombine(len(chunk), arr, cc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                             y.numel(), _vp(y), stream_of(self.dev)), "apy_zcombine")
+
+    # ------------------------------------------------------------------ solve
+    def solve(self, e_conv=1e-12, r_conv=1e-8, maxit=200):
+        """Lowest root: stops when the Ritz value changes by less than e_conv
+        and the residual norm is below r_conv (the residual reaches a floor
+        near 1e-10 set by rounding in sigma; r_conv must stay above it).
+        Returns the correlation energy (real); self.c holds the normalised
+        vector [c1 | c2 | c0]; self.history the (Ritz value, residual norm)
+        of every iteration."""
+        B, S = self.B, self.S
+        B.zero_()
+        S.zero_()
+        B[0, self.L] = 1.0                          # the reference determinant
+        self._sigma(0)
+        k = 1
+        G = np.zeros((self.max_space, self.max_space), dtype=np.complex128)
+        G[0, 0] = self.dots([B[0]], S[0])[0]
+        theta_old = None
+        self.history = []
+        it = 0
+        x = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
+        r = self.work
+        for it in range(1, maxit + 1):
+            Gk = G[:k, :k]
+            w, U = np.linalg.eigh(0.5 * (Gk + Gk.conj().T))
+            theta, y = float(w[0]), U[:, 0]
+            self.combine([B[j] for j in range(k)], y, x)
+            self.combine([S[j] for j in range(k)] + [x], list(y) + [-theta], r)
+            rn = float(np.sqrt(abs(self.dots([r], r)[0])))
+            self.history.append((theta, rn))
+            if theta_old is not None and abs(theta - theta_old) < e_conv and rn < r_conv:
+                break
+            theta_old = theta
+            if k == self.max_space:                 # restart from the Ritz vector, with a fresh sigma
+                B[0].copy_(x)
+                self._sigma(0)
+                G[0, 0] = self.dots([B[0]], S[0])[0]
+                k = 1
+            d = B[k]
+            _lib.check(self.lib.apy_ci_precondition(self.O, self.V, _vp(self.soci.eps), theta, _vp(r), _vp(d),
+                                                    stream_of(self.dev)), "apy_ci_precondition")
+            for _ in range(2):                      # Gram-Schmidt against the basis, twice
+                c = self.dots([B[j] for j in range(k)], d)
+                self.combine([B[j] for j in range(k)] + [d], list(-c) + [1.0], d)
+            nd = float(np.sqrt(abs(self.dots([d], d)[0])))
+            self.combine([d], [1.0 / nd], d)
+            self._sigma(k)
+            col = self.dots([B[j] for j in range(k + 1)], S[k])
+            G[:k + 1, k] = col
+            G[k, :k + 1] = np.conj(col)
+            k += 1
+        nx = float(np.sqrt(abs(self.dots([x], x)[0])))
+        self.combine([x], [1.0 / nx], x)
+        self.c = x
+        self.E = theta
+        self.iterations = it
+        return theta
+
+    def amplitudes(self):
+        """(t1, t2) in intermediate normalisation (c / c0)."""
+        c0 = complex(self.c[self.L].cpu())
+        t = self.c[:self.L] / c0
+        return t[:self.OV].view(self.O, self.V), t[self.OV:].view(self.O, self.O, self.V, self.V)
+
+    def gradient_fock(self):
+        """dE/dF [n, n] (apy_cisd_so_opd on the normalised root)."""
+        n = self.O + self.V
+        D = torch.empty((n, n), dtype=CDT, device=self.dev)
+        nb = int(self.lib.apy_cisd_so_opd_workspace(self.O, self.V))
+        ws = workspace(nb, self.dev)
+        _lib.check(self.lib.apy_cisd_so_opd(self.O, self.V, _vp(self.c), _vp(D), _vp(ws), nb, stream_of(self.dev)),
+                   "apy_cisd_so_opd")
+        return D
+
+
+class CISDEnergy(torch.autograd.Function):
+    """E(F) of the lowest Hermitian CISD root at fixed integrals, with the
+    adjoint backward dE/dF = D.  E is real and F Hermitian; for a real
+    loss the vector-Jacobian product is grad_output * conj(D)."""
+
+    @staticmethod
+    def forward(ctx, fock, ints, eps, e_conv=1e-12, r_conv=1e-9):
+        from .so_ci import SOCI
+        dav = CISDDavidson(SOCI(ints, fock.detach(), eps, singles=True, fov_coupling=True))
+        E = dav.solve(e_conv, r_conv)
+        ctx.dav = dav
+        return torch.tensor(E, dtype=torch.float64, device=fock.device)
+
+    @staticmethod
+    def backward(ctx, gE):
+        D = ctx.dav.gradient_fock()
+        return gE * D.conj(), None, None, None, None
+
+
+def cisd_energy(fock, ints, eps):
+    return CISDEnergy.apply(fock, ints, eps)

@@ -273,6 +273,23 @@ int apy_mp2_nc_densities(int64_t O, int64_t V, const void* t, const void* lam, v
                          size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Davidson support and the CISD energy gradient with respect to F.
+ * Vectors have length L + 1, L = O V + O^2 V^2: [c1 | c2 (dense,
+ * antisymmetric) | c0] -- the apy_cisd_so_step layout plus the reference
+ * weight.
+ *   apy_ci_precondition: out = r / (theta - D) with D = e_a - e_i (singles),
+ *     e_a + e_b - e_i - e_j (doubles, 0 where i == j or a == b), 0 (c0).
+ *   apy_cisd_so_opd: D[p][q] = dE/dF[p][q] of the normalised vector c (the
+ *     one-particle density of the adjoint sketch at ci_wfn.py:380-394;
+ *     the lowest root of the Hermitian CISD matrix is variational, so this
+ *     is the exact gradient): D is [(O+V)][(O+V)], fully written.
+ * ---------------------------------------------------------------------- */
+int apy_ci_precondition(int64_t O, int64_t V, const void* eps, double theta, const void* r, void* out,
+                        void* stream);
+size_t apy_cisd_so_opd_workspace(int64_t O, int64_t V);
+int apy_cisd_so_opd(int64_t O, int64_t V, const void* c, void* D, void* workspace, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * The same non-canonical MP2 (or lambda) iteration batched over independent
  * instances of possibly different (O, V): the +/- displaced systems of a
  * finite-difference set (fin_diff.py:267-372 solves them one at a time).

@@ -1,0 +1,112 @@
+"""Davidson lowest root of the Hermitian CISD matrix on the GPU and its
+adjoint gradient dE/dF: against the explicit CISD matrix, the NumPy
+Davidson oracle, the reference's Jacobi iteration (canonical case) and
+central finite differences."""
+import numpy as np
+import pytest
+import torch
+
+from apyib_amd import synthetic
+from apyib_amd.davidson import CISDDavidson, cisd_energy
+from apyib_amd.so_ci import SOCI, SOIntegrals
+from oracle import brute, davidson, so
+from tests.helpers import PARAMS, brillouin
+
+pytestmark = pytest.mark.gpu
+
+
+def _d(x, dev):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def _system(O, V, seed, dev, noise=0.05):
+    F, A, eps = synthetic.system(O, V, seed, noise=noise)
+    ints = SOIntegrals.synthetic(O, V, seed, device=dev)
+    return F, A, eps, ints
+
+
+def _dav(F, eps, ints, dev, max_space=16):
+    return CISDDavidson(SOCI(ints, _d(F, dev), eps, singles=True, fov_coupling=True), max_space=max_space)
+
+
+@pytest.mark.parametrize("O,V,seed", [(2, 3, 1), (3, 4, 2), (4, 4, 3)])
+def test_lowest_root_matches_explicit_matrix(dev, O, V, seed):
+    F, A, eps, ints = _system(O, V, seed, dev)
+    E_ref, t1_ref, t2_ref = brute.lowest_root(F, A, O)
+    dav = _dav(F, eps, ints, dev)
+    E = dav.solve(e_conv=1e-13, r_conv=1e-8)
+    t1, t2 = dav.amplitudes()
+    assert abs(E - E_ref) < 1e-11
+    assert np.abs(t1.cpu().numpy() - t1_ref).max() < 1e-8
+    assert np.abs(t2.cpu().numpy() - t2_ref).max() < 1e-8
+
+
+@pytest.mark.parametrize("O,V,seed,max_space", [(6, 14, 4, 16), (8, 20, 5, 6)])
+def test_matches_numpy_davidson(dev, O, V, seed, max_space):
+    """Medium sizes (with restarts for max_space=6) against the oracle."""
+    F, A, eps, ints = _system(O, V, seed, dev)
+    W = so.blocks_from_eri(A / 2, O)
+    E_ref, (c0, c1, c2), _ = davidson.davidson(F, W, O, tol=1e-13, rtol=1e-10)
+    dav = _dav(F, eps, ints, dev, max_space)
+    E = dav.solve(e_conv=1e-13, r_conv=1e-8)
+    t1, t2 = dav.amplitudes()
+    print("history", dav.iterations, dav.history[-5:], "oracle", E_ref)
+    assert abs(E - E_ref) < 1e-10
+    assert np.abs(t1.cpu().numpy() - c1 / c0).max() < 1e-8
+    assert np.abs(t2.cpu().numpy() - c2 / c0).max() < 1e-8
+
+
+def test_canonical_case_equals_reference_iteration(dev):
+    """F_ov = 0: the Hermitian root is the reference's Jacobi+DIIS solution."""
+    O, V, seed = 4, 8, 6
+    F, A, eps, ints = _system(O, V, seed, dev, noise=1e-3)
+    F = brillouin(F, O)
+    E_ref, _, _, _ = so.solve_cisd_so(F, A / 2, eps[:O], eps[O:], O, PARAMS)
+    dav = _dav(F, eps, ints, dev)
+    assert abs(dav.solve(e_conv=1e-13, r_conv=1e-8) - E_ref) < 1e-10
+
+
+def test_fock_gradient_matches_oracle_density(dev):
+    O, V, seed = 5, 9, 7
+    F, A, eps, ints = _system(O, V, seed, dev)
+    W = so.blocks_from_eri(A / 2, O)
+    _, c, _ = davidson.davidson(F, W, O, tol=1e-13, rtol=1e-11)
+    D_ref = davidson.one_particle_density(c, O)
+    dav = _dav(F, eps, ints, dev)
+    dav.solve(e_conv=1e-13, r_conv=1e-9)
+    D = dav.gradient_fock().cpu().numpy()
+    # D is invariant to the phase of the vector and first order in its error
+    # (the residual of either solve stays above ~1e-10)
+    assert np.abs(D - D_ref).max() < 2e-8
+
+
+def test_fock_gradient_matches_central_differences(dev):
+    """dE along Hermitian one-electron directions (real symmetric and
+    i x antisymmetric, as in the nuclear and field perturbations), h = 1e-4."""
+    O, V, seed, h = 6, 12, 8, 1e-4
+    F, A, eps, ints = _system(O, V, seed, dev)
+    dav = _dav(F, eps, ints, dev)
+    dav.solve(e_conv=1e-13, r_conv=1e-8)
+    D = dav.gradient_fock()
+    n = O + V
+    for k, kind in enumerate(('nuclear', 'field', 'nuclear', 'field')):
+        dF = synthetic.fock_perturbation(n, 50 + k, kind)
+        an = complex(torch.sum(D * _d(dF, dev)).cpu())
+        Ep = _dav(F + h * dF, eps, ints, dev).solve(e_conv=1e-13, r_conv=1e-8)
+        Em = _dav(F - h * dF, eps, ints, dev).solve(e_conv=1e-13, r_conv=1e-8)
+        fd = (Ep - Em) / (2 * h)
+        scale = float(torch.sum((D * _d(dF, dev)).abs()).cpu())
+        assert abs(an.imag) < 1e-10 * scale
+        assert abs(fd - an.real) < 1e-7 * scale, (kind, fd, an)
+
+
+def test_autograd_function(dev):
+    O, V, seed = 4, 7, 9
+    F, A, eps, ints = _system(O, V, seed, dev)
+    Ft = _d(F, dev).requires_grad_(True)
+    E = cisd_energy(Ft, ints, eps)
+    (g,) = torch.autograd.grad(E, [Ft])
+    dav = _dav(F, eps, ints, dev)
+    dav.solve(e_conv=1e-12, r_conv=1e-9)
+    D = dav.gradient_fock()
+    assert torch.abs(g - D.conj()).max().item() < 1e-8
