@@ -79,6 +79,18 @@ class CisdSoArgs(ctypes.Structure):
     ]
 
 
+class CisdSoSigmaBatchArgs(ctypes.Structure):
+    _fields_ = [
+        ("O", c_i64), ("V", c_i64),
+        ("nb", c_int), ("singles", c_int), ("fov_coupling", c_int),
+        ("ints", ctypes.POINTER(SoInts)),
+        ("fock", ctypes.POINTER(c_vp)), ("hoo", ctypes.POINTER(c_vp)),
+        ("c0", ctypes.POINTER(c_dbl)),
+        ("t", ctypes.POINTER(c_vp)), ("out", ctypes.POINTER(c_vp)),
+        ("ev_begin", c_vp), ("ev_end", c_vp),
+    ]
+
+
 class Mp2Instance(ctypes.Structure):
     _fields_ = [
         ("O", c_i64), ("V", c_i64),
@@ -141,6 +153,8 @@ SIGNATURES = {
     "apy_cisd_so_workspace": (c_size, [c_i64, c_i64, c_int]),
     "apy_cisd_so_step": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_vp]),
     "apy_cisd_so_tune": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_int, c_vp]),
+    "apy_cisd_so_sigma_batch_workspace": (c_size, [c_i64, c_i64, c_int, c_int]),
+    "apy_cisd_so_sigma_batch": (c_int, [ctypes.POINTER(CisdSoSigmaBatchArgs), c_vp, c_size, c_vp]),
     "apy_cisd_so_energy": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_zdot_many": (c_int, [c_int, _P_VP, c_vp, c_i64, c_int, c_vp, c_vp, c_size, c_vp]),
     "apy_zdiffsq_segments": (c_int, [c_vp, c_vp, c_int, _P_I64, c_vp, c_vp, c_size, c_vp]),

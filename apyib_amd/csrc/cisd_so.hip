@@ -19,6 +19,9 @@
 #include "common.h"
 #include "synth.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace {
 
 __host__ __device__ inline int64_t npair(int64_t N) { return N * (N - 1) / 2; }
@@ -247,6 +250,7 @@ struct EpiArgs {
   const double2* Y;
   const double2* U;
   const double2* Q;
+  int64_t ldq;             // row stride of Q[((ab),j)][i] (O for one instance)
   const double2* F;
   const double2* t1;
   int fov;
@@ -289,8 +293,8 @@ __global__ __launch_bounds__(256) void epi_doubles(EpiArgs e) {
       acc(s, e.U[(b * V + a) * e.Po + Pij]);
     }
     if (e.Q) {
-      acc(s, e.Q[(Pab * O + j) * O + i]);
-      dec(s, e.Q[(Pab * O + i) * O + j]);
+      acc(s, e.Q[(Pab * O + j) * e.ldq + i]);
+      dec(s, e.Q[(Pab * O + i) * e.ldq + j]);
     }
     if (e.fov) {
       const int64_t n = e.n;
@@ -591,6 +595,7 @@ int apy_cisd_so_step(const apy_cisd_so_args* A, void* workspace, size_t ws_bytes
   e.Rp = w.rp; e.X = w.X; e.Y = w.Y;
   e.U = singles ? w.U : nullptr;
   e.Q = singles ? w.Q : nullptr;
+  e.ldq = O;
   e.F = F;
   e.t1 = t;
   e.fov = (singles && A->fov_coupling) ? 1 : 0;
@@ -638,6 +643,181 @@ int apy_cisd_so_energy(int64_t O, int64_t V, int singles, const void* fock, cons
   APY_LAUNCH_CHECK();
   finalize_sum<<<1, 256, 0, st>>>((const double2*)workspace, ENERGY_BLOCKS, (double2*)energy);
   APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched sigma
+// Sigma products of nb instances that share the integrals (the field-
+// displaced copies of one system): the particle-particle ladder, the ring
+// term, the vvvo.t1 term and the ring.t1 singles term each run as ONE GEMM
+// with the instances stacked along M (or N), so the integral blocks are read
+// once per batch; the Fock-dependent products and the epilogues run per
+// instance.
+namespace {
+
+struct BatchWS {
+  double2 *tp, *rp, *tr, *X, *Y, *U, *Q, *s1, *t1;
+  void* gemm;
+  size_t gemm_bytes, total;
+};
+
+void batch_gemms(int64_t O, int64_t V, int nb, int singles, const apy_so_ints* I, const double2* const* F,
+                 const double2* const* hoo, const double2* const* t, const BatchWS& w, std::vector<apy_gemm_desc>& G) {
+  const int64_t Po = npair(O), Pv = npair(V), OV = O * V, n = O + V, L2 = O * O * V * V;
+  G.clear();
+  apy_gemm_desc d;
+  // folded over instances
+  fill(d, nb * Po, Pv, Pv, APY_MK, w.tp, Pv, APY_NK, I ? I->vvvv_p : nullptr, Pv, w.rp, Pv, 1.0, 0.0);
+  G.push_back(d);                                                              // [0] PPL
+  fill(d, nb * OV, OV, OV, APY_MK, w.tr, OV, APY_NK, I ? I->ring : nullptr, OV, w.X, OV, 1.0, 0.0);
+  G.push_back(d);                                                              // [1] ring
+  if (singles) {
+    fill(d, Pv * O, nb * O, V, APY_MK, I ? I->vvvo_p : nullptr, V, APY_NK, w.t1, V, w.Q, nb * O, 1.0, 0.0);
+    G.push_back(d);                                                            // [2] Q
+  }
+  for (int k = 0; k < nb; ++k) {
+    const double2* tk = t ? t[k] : nullptr;
+    const double2* t1 = tk;
+    const double2* t2 = tk ? tk + OV : nullptr;
+    const double2* Fk = F ? F[k] : nullptr;
+    fill(d, Po, Pv, Po, APY_MK, hoo ? hoo[k] : nullptr, Po, APY_KN, w.tp ? w.tp + k * Po * Pv : nullptr, Pv,
+         w.rp ? w.rp + k * Po * Pv : nullptr, Pv, 1.0, 1.0);
+    G.push_back(d);                                                            // HHL + F_oo
+    fill(d, O * O * V, V, V, APY_MK, t2, V, APY_NK, Fk ? Fk + O * n + O : nullptr, n, w.Y ? w.Y + k * L2 : nullptr, V,
+         1.0, 0.0);
+    G.push_back(d);                                                            // F_vv
+    if (!singles) continue;
+    double2* s1 = w.s1 ? w.s1 + k * OV : nullptr;
+    fill(d, V, V * Po, O, APY_KM, t1, V, APY_KN, I ? I->ovoo_p : nullptr, V * Po, w.U ? w.U + k * V * V * Po : nullptr,
+         V * Po, 1.0, 0.0);
+    G.push_back(d);                                                            // U
+    fill(d, O, V, V, APY_MK, t1, V, APY_NK, Fk ? Fk + O * n + O : nullptr, n, s1, V, 1.0, 0.0);
+    G.push_back(d);                                                            // s1 = t1 F_vv^T
+    fill(d, O, V, O, APY_KM, Fk, n, APY_KN, t1, V, s1, V, -1.0, 1.0);
+    G.push_back(d);                                                            // s1 -= F_oo^T t1
+    fill(d, OV, 1, V, APY_MK, t2, V, APY_KN, Fk ? Fk + O : nullptr, 1, s1, 1, -1.0, 1.0);
+    d.batch = O; d.stride_a = OV * V; d.stride_b = n; d.stride_c = 0; d.reduce_batch = 1;
+    G.push_back(d);                                                            // s1 += F_jb t2
+    fill(d, O, V, OV * V, APY_MK, t2, OV * V, APY_NK, I ? I->vovv : nullptr, OV * V, s1, V, 0.5, 1.0);
+    G.push_back(d);                                                            // s1 += 1/2 t2 vovv
+    fill(d, O, V, V, APY_MK, I ? I->ooov : nullptr, V, APY_NK, t2, V, s1, V, -0.5, 1.0);
+    d.batch = O * O; d.stride_a = OV; d.stride_b = V * V; d.stride_c = 0; d.reduce_batch = 1;
+    G.push_back(d);                                                            // s1 -= 1/2 ooov t2
+  }
+  if (singles) {
+    fill(d, nb, OV, OV, APY_MK, w.t1, OV, APY_NK, I ? I->ring : nullptr, OV, w.s1, OV, 1.0, 1.0);
+    G.push_back(d);                                                            // s1 += ring . t1 (all instances)
+  }
+}
+
+BatchWS carve_batch(int64_t O, int64_t V, int nb, int singles, void* base) {
+  const int64_t Po = npair(O), Pv = npair(V), OV = O * V;
+  BatchWS w{};
+  size_t off = 0;
+  char* b = (char*)base;
+  auto take = [&](size_t elems) -> double2* {
+    double2* p = b ? (double2*)(b + off) : nullptr;
+    off += al256(elems * sizeof(double2));
+    return p;
+  };
+  w.tp = take((size_t)nb * Po * Pv);
+  w.rp = take((size_t)nb * Po * Pv);
+  w.tr = take((size_t)nb * OV * OV);
+  w.X = take((size_t)nb * OV * OV);
+  w.Y = take((size_t)nb * O * O * V * V);
+  if (singles) {
+    w.U = take((size_t)nb * V * V * Po);
+    w.Q = take((size_t)Pv * O * nb * O);
+    w.s1 = take((size_t)nb * OV);
+    w.t1 = take((size_t)nb * OV);
+  }
+  std::vector<apy_gemm_desc> G;
+  BatchWS shape = w;
+  if (!b) {  // sizing pass: non-null dummies so every descriptor is generated
+    shape.tp = shape.rp = shape.tr = shape.X = shape.Y = shape.U = shape.Q = shape.s1 = shape.t1 = nullptr;
+  }
+  batch_gemms(O, V, nb, singles, nullptr, nullptr, nullptr, nullptr, shape, G);
+  size_t g = 0;
+  for (const apy_gemm_desc& d : G) g = std::max(g, apy_zgemm_ws_bytes(&d));
+  w.gemm = b ? (void*)(b + off) : nullptr;
+  w.gemm_bytes = g;
+  off += al256(g);
+  w.total = off;
+  return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t apy_cisd_so_sigma_batch_workspace(int64_t O, int64_t V, int nb, int singles) {
+  if (O < 2 || V < 2 || nb < 1) return 0;
+  return carve_batch(O, V, nb, singles ? 1 : 0, nullptr).total;
+}
+
+int apy_cisd_so_sigma_batch(const apy_cisd_so_sigma_batch_args* A, void* workspace, size_t ws_bytes, void* stream) {
+  if (!A || !A->ints || !A->fock || !A->hoo || !A->c0 || !A->t || !A->out) return APY_E_ARG;
+  const int64_t O = A->O, V = A->V;
+  const int nb = A->nb;
+  if (O < 2 || V < 2 || nb < 1 || nb > 4096 || A->ints->O != O || A->ints->V != V) return APY_E_ARG;
+  const int singles = A->singles ? 1 : 0;
+  const apy_so_ints* I = A->ints;
+  if (!I->vvvv_p || !I->ring || !I->vvoo_t) return APY_E_ARG;
+  if (singles && (!I->ovoo_p || !I->vvvo_p || !I->vovv || !I->ooov)) return APY_E_ARG;
+  for (int k = 0; k < nb; ++k)
+    if (!A->fock[k] || !A->hoo[k] || !A->t[k] || !A->out[k] || A->out[k] == A->t[k]) return APY_E_ARG;
+  BatchWS w = carve_batch(O, V, nb, singles, workspace);
+  if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t Po = npair(O), Pv = npair(V), OV = O * V, n = O + V;
+  std::vector<const double2*> F(nb), H(nb), T(nb);
+  for (int k = 0; k < nb; ++k) {
+    F[k] = (const double2*)A->fock[k];
+    H[k] = (const double2*)A->hoo[k];
+    T[k] = (const double2*)A->t[k];
+    pack_t2<<<apy_grid1d(Po * Pv, 1 << 20), 256, 0, st>>>(T[k] + OV, w.tp + k * Po * Pv, O, V);
+    APY_LAUNCH_CHECK();
+    ring_perm<<<apy_grid1d(OV * OV, 1 << 20), 256, 0, st>>>(T[k] + OV, w.tr + k * OV * OV, O, V);
+    APY_LAUNCH_CHECK();
+    if (singles) APY_HIP_TRY(hipMemcpyAsync(w.t1 + k * OV, T[k], OV * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  }
+  std::vector<apy_gemm_desc> G;
+  batch_gemms(O, V, nb, singles, I, F.data(), H.data(), T.data(), w, G);
+  for (size_t q = 0; q < G.size(); ++q) {
+    const int s = q == 0 ? apy_zgemm_launch(&G[q], w.gemm, w.gemm_bytes, st, (hipEvent_t)A->ev_begin,
+                                            (hipEvent_t)A->ev_end)
+                         : apy_zgemm_launch(&G[q], w.gemm, w.gemm_bytes, st);
+    if (s != APY_OK) return s;
+  }
+  for (int k = 0; k < nb; ++k) {
+    double2* out = (double2*)A->out[k];
+    EpiArgs e;
+    e.O = O; e.V = V; e.Po = Po; e.Pv = Pv; e.n = n;
+    e.drv = (const double2*)I->vvoo_t;
+    e.Rp = w.rp + k * Po * Pv; e.X = w.X + k * OV * OV; e.Y = w.Y + k * O * O * V * V;
+    e.U = singles ? w.U + k * V * V * Po : nullptr;
+    e.Q = singles ? w.Q + k * O : nullptr;
+    e.ldq = (int64_t)nb * O;
+    e.F = F[k];
+    e.t1 = T[k];
+    e.fov = (singles && A->fov_coupling) ? 1 : 0;
+    e.eps = nullptr;
+    e.E = nullptr;
+    e.c0r = A->c0[2 * k]; e.c0i = A->c0[2 * k + 1];
+    e.t2 = T[k] + OV;
+    e.t2n = nullptr;
+    e.out2 = out + OV;
+    e.mode = APY_MODE_SIGMA;
+    epi_doubles<<<apy_grid1d(Po * Pv, 1 << 20), 256, 0, st>>>(e);
+    APY_LAUNCH_CHECK();
+    if (singles) {
+      epi_singles<<<apy_grid1d(OV), 256, 0, st>>>(O, V, n, F[k], nullptr, nullptr, e.c0r, e.c0i, w.s1 + k * OV,
+                                                  T[k], nullptr, out, APY_MODE_SIGMA);
+      APY_LAUNCH_CHECK();
+    }
+  }
   return APY_OK;
 }
 

@@ -195,3 +195,111 @@ class CISDEnergy(torch.autograd.Function):
 
 def cisd_energy(fock, ints, eps):
     return CISDEnergy.apply(fock, ints, eps)
+
+
+class MultiDavidson:
+    """Lowest roots of several Hermitian CISD problems that share the
+    integrals (the +/- field-displaced copies of one system) solved
+    together: every iteration applies sigma to the new vectors of all
+    unconverged instances in one apy_cisd_so_sigma_batch call, so the
+    integral blocks stream through the matrix cores once per iteration for
+    the whole set.  Each instance keeps its own subspace, Ritz pair and
+    convergence test (those of CISDDavidson).  With one instance this is
+    the per-displacement solve on the same kernels."""
+
+    def __init__(self, ints, focks, eps, max_space=12):
+        from .so_ci import SOCI
+        self.ints = ints
+        self.dav = [CISDDavidson(SOCI(ints, F, eps, singles=True, fov_coupling=True), max_space) for F in focks]
+        self.lib = self.dav[0].lib
+        self.dev = self.dav[0].dev
+        self.O, self.V, self.L = self.dav[0].O, self.dav[0].V, self.dav[0].L
+        self.ev_dominant = None
+
+    def _sigma_batch(self, entries):
+        """S[slot] = H_N B[slot] for every (instance, slot) in entries."""
+        nb = len(entries)
+        L = self.L
+        c0 = torch.stack([d.B[q][L] for d, q in entries]).cpu().numpy()
+        f = (ctypes.c_void_p * nb)(*[d.soci.fock.data_ptr() for d, _ in entries])
+        h = (ctypes.c_void_p * nb)(*[d.soci.hoo.data_ptr() for d, _ in entries])
+        t = (ctypes.c_void_p * nb)(*[d.B[q].data_ptr() for d, q in entries])
+        o = (ctypes.c_void_p * nb)(*[d.S[q].data_ptr() for d, q in entries])
+        cc = np.ascontiguousarray(c0.astype(np.complex128)).view(np.float64)
+        ev = self.ev_dominant or (None, None)
+        args = _lib.CisdSoSigmaBatchArgs(self.O, self.V, nb, 1, 1, ctypes.pointer(self.ints.struct), f, h,
+                                         cc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), t, o, ev[0], ev[1])
+        nbytes = int(self.lib.apy_cisd_so_sigma_batch_workspace(self.O, self.V, nb, 1))
+        ws = workspace(nbytes, self.dev)
+        _lib.check(self.lib.apy_cisd_so_sigma_batch(ctypes.byref(args), _vp(ws), nbytes, stream_of(self.dev)),
+                   "apy_cisd_so_sigma_batch")
+        for d, q in entries:
+            d.soci.energy(d.B[q][:L], out=d.S[q][L:])
+
+    def solve(self, e_conv=1e-12, r_conv=1e-8, maxit=200):
+        """Returns the array of correlation energies; each instance's
+        normalised vector is in dav[k].c (iteration counts in dav[k].iterations)."""
+        L = self.L
+        st = []
+        for d in self.dav:
+            d.B.zero_()
+            d.S.zero_()
+            d.B[0, L] = 1.0
+            d.history = []
+            st.append({'k': 1, 'G': np.zeros((d.max_space, d.max_space), np.complex128), 'theta_old': None,
+                       'x': torch.zeros(L + 1, dtype=CDT, device=self.dev), 'done': False})
+        self._sigma_batch([(d, 0) for d in self.dav])
+        for d, s in zip(self.dav, st):
+            s['G'][0, 0] = d.dots([d.B[0]], d.S[0])[0]
+        for it in range(1, maxit + 1):
+            entries, cols = [], []
+            for d, s in zip(self.dav, st):
+                if s['done']:
+                    continue
+                k, G, x, r = s['k'], s['G'], s['x'], d.work
+                Gk = G[:k, :k]
+                w, U = np.linalg.eigh(0.5 * (Gk + Gk.conj().T))
+                theta, y = float(w[0]), U[:, 0]
+                d.combine([d.B[j] for j in range(k)], y, x)
+                d.combine([d.S[j] for j in range(k)] + [x], list(y) + [-theta], r)
+                rn = float(np.sqrt(abs(d.dots([r], r)[0])))
+                d.history.append((theta, rn))
+                d.iterations = it
+                d.E = theta
+                if s['theta_old'] is not None and abs(theta - s['theta_old']) < e_conv and rn < r_conv:
+                    s['done'] = True
+                    continue
+                s['theta_old'] = theta
+                if k == d.max_space:                         # restart from the Ritz vector
+                    d.B[0].copy_(x)
+                    k = 1
+                    entries.append((d, 0))
+                    s['restart'] = True
+                dv = d.B[k]
+                _lib.check(self.lib.apy_ci_precondition(self.O, self.V, _vp(d.soci.eps), theta, _vp(r), _vp(dv),
+                                                        stream_of(self.dev)), "apy_ci_precondition")
+                for _ in range(2):
+                    c = d.dots([d.B[j] for j in range(k)], dv)
+                    d.combine([d.B[j] for j in range(k)] + [dv], list(-c) + [1.0], dv)
+                nd = float(np.sqrt(abs(d.dots([dv], dv)[0])))
+                d.combine([dv], [1.0 / nd], dv)
+                s['k'] = k
+                entries.append((d, k))
+                cols.append((d, s))
+            if not entries:
+                break
+            self._sigma_batch(entries)
+            for d, s in cols:
+                k, G = s['k'], s['G']
+                if s.pop('restart', False):
+                    G[0, 0] = d.dots([d.B[0]], d.S[0])[0]
+                col = d.dots([d.B[j] for j in range(k + 1)], d.S[k])
+                G[:k + 1, k] = col
+                G[k, :k + 1] = np.conj(col)
+                s['k'] = k + 1
+        for d, s in zip(self.dav, st):
+            x = s['x']
+            nx = float(np.sqrt(abs(d.dots([x], x)[0])))
+            d.combine([x], [1.0 / nx], x)
+            d.c = x
+        return np.array([d.E for d in self.dav])

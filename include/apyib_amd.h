@@ -179,6 +179,29 @@ int apy_cisd_so_step(const apy_cisd_so_args* a, void* workspace, size_t ws_bytes
  * (apy_zgemm_tune; only workspace is written).  Synchronises the stream. */
 int apy_cisd_so_tune(const apy_cisd_so_args* a, void* workspace, size_t ws_bytes, int reps, void* stream);
 
+/* Sigma products (SIGMA mode) of nb instances sharing `ints` and (O, V),
+ * each with its own Fock matrix (the +/- field-displaced copies of one
+ * system, fin_diff.py:335-370).  The particle-particle ladder, ring, vvvo.t1
+ * and ring.t1 products run as single GEMMs with the instances stacked, so
+ * each integral block is read once per batch.  Host arrays of nb device
+ * pointers; c0 is a host array of nb (re, im) pairs.  The optional events
+ * bracket the stacked PPL kernel. */
+typedef struct apy_cisd_so_sigma_batch_args {
+  int64_t O, V;
+  int nb;
+  int singles, fov_coupling;
+  const apy_so_ints* ints;
+  const void* const* fock;
+  const void* const* hoo;
+  const double* c0;
+  const void* const* t;
+  void* const* out;
+  void* ev_begin;
+  void* ev_end;
+} apy_cisd_so_sigma_batch_args;
+size_t apy_cisd_so_sigma_batch_workspace(int64_t O, int64_t V, int nb, int singles);
+int apy_cisd_so_sigma_batch(const apy_cisd_so_sigma_batch_args* a, void* workspace, size_t ws_bytes, void* stream);
+
 /* E = sum_ia t1 f_ia + 1/4 sum t2 <ij||ab> (ci_wfn.py:355; CID :237 with
  * singles = 0), written to the device complex scalar `energy`. */
 int apy_cisd_so_energy(int64_t O, int64_t V, int singles, const void* fock, const void* oovv, const void* t,
