@@ -53,10 +53,12 @@ class CISDDavidson:
         self.c = None
         self.E = None
         self.iterations = 0
+        self.nsigma = 0
 
     # -------------------------------------------------------------- primitives
     def _sigma(self, q):
         """S[q] = H_N B[q] (needs the host value of the reference weight)."""
+        self.nsigma += 1
         b, s = self.B[q], self.S[q]
         c0 = complex(b[self.L].cpu())
         self.soci.sigma(b[:self.L], s[:self.L], c0=c0)
@@ -101,13 +103,12 @@ class CISDDavidson:
                                              y.numel(), _vp(y), stream_of(self.dev)), "apy_zcombine")
 
     # ------------------------------------------------------------------ solve
-    def solve(self, e_conv=1e-12, r_conv=1e-8, maxit=200):
-        """Lowest root: stops when the Ritz value changes by less than e_conv
-        and the residual norm is below r_conv (the residual reaches a floor
-        near 1e-10 set by rounding in sigma; r_conv must stay above it).
-        Returns the correlation energy (real); self.c holds the normalised
-        vector [c1 | c2 | c0]; self.history the (Ritz value, residual norm)
-        of every iteration."""
+    def iterate(self):
+        """Generator over Davidson iterations: each yields (theta, residual
+        norm) after forming the Ritz pair and, unless the caller stops,
+        extends the subspace by one preconditioned, orthonormalised vector
+        and its sigma.  The current Ritz vector (unnormalised only by
+        rounding) is self.x."""
         B, S = self.B, self.S
         B.zero_()
         S.zero_()
@@ -116,22 +117,17 @@ class CISDDavidson:
         k = 1
         G = np.zeros((self.max_space, self.max_space), dtype=np.complex128)
         G[0, 0] = self.dots([B[0]], S[0])[0]
-        theta_old = None
-        self.history = []
-        it = 0
-        x = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
+        self.x = x = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
         r = self.work
-        for it in range(1, maxit + 1):
+        while True:
             Gk = G[:k, :k]
             w, U = np.linalg.eigh(0.5 * (Gk + Gk.conj().T))
             theta, y = float(w[0]), U[:, 0]
             self.combine([B[j] for j in range(k)], y, x)
             self.combine([S[j] for j in range(k)] + [x], list(y) + [-theta], r)
             rn = float(np.sqrt(abs(self.dots([r], r)[0])))
-            self.history.append((theta, rn))
-            if theta_old is not None and abs(theta - theta_old) < e_conv and rn < r_conv:
-                break
-            theta_old = theta
+            self.E = theta
+            yield theta, rn
             if k == self.max_space:                 # restart from the Ritz vector, with a fresh sigma
                 B[0].copy_(x)
                 self._sigma(0)
@@ -150,10 +146,33 @@ class CISDDavidson:
             G[:k + 1, k] = col
             G[k, :k + 1] = np.conj(col)
             k += 1
+
+    def finish(self):
+        """Normalise the current Ritz vector into self.c."""
+        x = self.x
         nx = float(np.sqrt(abs(self.dots([x], x)[0])))
         self.combine([x], [1.0 / nx], x)
         self.c = x
-        self.E = theta
+        return self.c
+
+    def solve(self, e_conv=1e-12, r_conv=1e-8, maxit=200):
+        """Lowest root: stops when the Ritz value changes by less than e_conv
+        and the residual norm is below r_conv (the residual reaches a floor
+        near 1e-10 set by rounding in sigma; r_conv must stay above it).
+        Returns the correlation energy (real); self.c holds the normalised
+        vector [c1 | c2 | c0]; self.history the (Ritz value, residual norm)
+        of every iteration."""
+        self.history = []
+        theta_old = None
+        it = 0
+        gen = self.iterate()
+        for it, (theta, rn) in enumerate(gen, start=1):
+            self.history.append((theta, rn))
+            if (theta_old is not None and abs(theta - theta_old) < e_conv and rn < r_conv) or it >= maxit:
+                break
+            theta_old = theta
+        gen.close()
+        self.finish()
         self.iterations = it
         return theta
 

@@ -4,15 +4,18 @@
 Workload (one GPU): synthetic complex128 CISD at O=20 occupied and V=160
 virtual spin orbitals (seeded <pq||rs>, complex std 0.05 Eh; Fock matrix
 with U[-1.5,-0.3] / U[0.2,3.0] Eh diagonal and 1e-3 Hermitian noise), the
-reference's Jacobi + DIIS iteration.  A step is one iteration: sigma and
-residual, Jacobi update, DIIS extrapolation, energy and convergence data.
-Inputs are generated on the device before timing; nothing crosses PCIe in
-the timed region except the handful of DIIS/energy scalars the iteration
-reads back each step.
+lowest root by Davidson iteration on the Hermitian CISD matrix.  A step is
+one Davidson iteration: the sigma of the new vector (and the restart sigma
+every max_space iterations), the CI-metric projections, Ritz pair,
+residual, preconditioner and Gram-Schmidt.  Inputs are generated on the
+device before timing; nothing crosses PCIe in the timed region except the
+small projected matrices and scalars the iteration reads back.
 
 Prints one JSON line.  `value` is the executed FP64 work of the step's
-contractions (8 flops per complex multiply-add, antisymmetric pairs packed)
-divided by the measured time per step.
+sigma contractions (8 flops per complex multiply-add, antisymmetric pairs
+packed) divided by the measured time per step.  `aat_set` holds the second
+metric: finite-difference instance sets solved batched and by the adjoint
+against the plain per-instance loop on the same kernels.
 """
 import argparse
 import ctypes
@@ -211,6 +214,81 @@ def aat_set_measurement(dev, seed, tol=1e-12):
     return res
 
 
+def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
+    """CISD field-derivative set sharing one integral set: 6 instances (3
+    field components x +/- h, i x antisymmetric one-electron
+    perturbations) of the Hermitian CISD lowest root.  Plain loop: one
+    Davidson solve per instance on the same kernels (batch of one);
+    batched: all 6 in one multi-instance Davidson (stacked PPL/ring GEMMs);
+    adjoint: one solve at the base point plus the one-particle density
+    (dE/dF), giving all three derivatives."""
+    import numpy as np
+    import torch
+    from apyib_amd import synthetic
+    from apyib_amd.davidson import MultiDavidson
+    from apyib_amd.so_ci import SOIntegrals
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    ints = SOIntegrals.synthetic(O, V, seed, device=dev)
+    F, eps = synthetic.fock(O, V, seed)
+    n = O + V
+    F0 = torch.from_numpy(F).to(dev)
+    dirs = [torch.from_numpy(synthetic.fock_perturbation(n, seed * 1000 + 500 + k, 'field')).to(dev) for k in range(3)]
+    focks = [F0 + sgn * h * dF for dF in dirs for sgn in (+1, -1)]
+    torch.cuda.synchronize()
+    log(f"cisd field set: integrals {ints.nbytes() / 1e9:.2f} GB in {time.perf_counter() - t0:.1f} s")
+    conv = dict(e_conv=1e-12, r_conv=1e-6)
+
+    def plain():
+        E, its = [], []
+        for Fk in focks:
+            md = MultiDavidson(ints, [Fk], eps, max_space)
+            E.append(md.solve(**conv)[0])
+            its.append(md.dav[0].iterations)
+            del md
+        return np.array(E), its
+    (Ep, its_p), t_plain = timed(plain)
+    log(f"cisd field set: plain loop {t_plain:.2f} s, iterations {its_p}")
+
+    def batched():
+        md = MultiDavidson(ints, focks, eps, max_space)
+        E = md.solve(**conv)
+        its = [d.iterations for d in md.dav]
+        del md
+        return E, its
+    (Eb, its_b), t_batch = timed(batched)
+    log(f"cisd field set: batched {t_batch:.2f} s, iterations {its_b}")
+
+    def adjoint():
+        md = MultiDavidson(ints, [F0], eps, max_space)
+        md.solve(**conv)
+        D = md.dav[0].gradient_fock()
+        g = torch.stack([torch.sum(D * dF) for dF in dirs]).cpu().numpy()
+        scale = np.array([float(torch.sum((D * dF).abs()).cpu()) for dF in dirs])
+        del md
+        return g, scale
+    (g, scale), t_adj = timed(adjoint)
+    fd_b = (Eb[0::2] - Eb[1::2]) / (2 * h)
+    fd_p = (Ep[0::2] - Ep[1::2]) / (2 * h)
+    res = {"workload": f"Hermitian CISD lowest root, O={O} V={V}, 6 instances (3 field components x +/-{h}) "
+                       f"sharing one seeded <pq||rs>", "instances": 6,
+           "plain_loop_s": round(t_plain, 3), "batched_s": round(t_batch, 3), "adjoint_s": round(t_adj, 3),
+           "speedup_batched": round(t_plain / t_batch, 2), "speedup_adjoint_vs_plain_fd": round(t_plain / t_adj, 2),
+           "iterations_plain": its_p, "iterations_batched": its_b,
+           "max_abs_dE_batched_vs_plain": float(np.abs(Eb - Ep).max()),
+           "max_err_adjoint_vs_fd_rel_to_scale": float((np.abs(g.real - fd_b) / scale).max()),
+           "max_err_fd_batched_vs_plain_rel_to_scale": float((np.abs(fd_b - fd_p) / scale).max())}
+    log(f"cisd field set: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,8 +298,10 @@ def main():
     ap.add_argument("--V", type=int, default=160)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-tune", action="store_true", help="skip the GEMM autotuner")
+    ap.add_argument("--max-space", type=int, default=12, help="Davidson subspace size before restart")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-aat-set", action="store_true", help="skip the batched finite-difference set timing")
+    ap.add_argument("--no-cisd-set", action="store_true", help="skip the CISD field-derivative set timing")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -233,6 +313,7 @@ def main():
     import numpy as np
     import torch
     from apyib_amd import synthetic
+    from apyib_amd.davidson import CISDDavidson
     from apyib_amd.so_ci import SOCI, SOIntegrals, dense_flops, step_flops
 
     dev = torch.device("cuda:0")
@@ -240,7 +321,7 @@ def main():
     t0 = time.perf_counter()
     ints = SOIntegrals.synthetic(O, V, args.seed, device=dev)
     F, eps = synthetic.fock(O, V, args.seed)
-    solver = SOCI(ints, torch.from_numpy(F).to(dev), eps, singles=True)
+    solver = SOCI(ints, torch.from_numpy(F).to(dev), eps, singles=True, fov_coupling=True)
     torch.cuda.synchronize()
     log(f"setup: integrals {ints.nbytes() / 1e9:.2f} GB generated on device in {time.perf_counter() - t0:.2f} s")
     if not args.no_tune:
@@ -250,24 +331,30 @@ def main():
         log(f"autotune: {time.perf_counter() - t0:.1f} s")
 
     fl = step_flops(O, V, True)
-    flops_step = sum(fl.values())
+    flops_sigma = sum(fl.values())
     hip = HipEvents()
     K, W = args.steps, args.warmup
     evs = [(hip.create(), hip.create()) for _ in range(K)]
 
-    gen = solver.iterations(diis=True)
-    next(gen)  # initial guess and energy
+    # a step = one Davidson iteration of the lowest root: sigma of the new
+    # vector (plus the restart sigma every max_space iterations), CI-metric
+    # projections, Ritz pair, residual, preconditioner, Gram-Schmidt
+    dav = CISDDavidson(solver, max_space=args.max_space)
+    gen = dav.iterate()
+    next(gen)                       # reference vector, its sigma and the first Ritz pair
     for _ in range(W):
         next(gen)
     torch.cuda.synchronize()
+    n0 = dav.nsigma
     t_start = time.perf_counter()
-    E = None
+    theta = rn = None
     for k in range(K):
         solver.ev_dominant = evs[k]
-        it, E, dE, r1, r2 = next(gen)
+        theta, rn = next(gen)
     torch.cuda.synchronize()
     solver.ev_dominant = None
     elapsed = time.perf_counter() - t_start
+    nsig = dav.nsigma - n0
     gen.close()
     ms_step = elapsed * 1e3 / K
     ppl_ms = [hip.elapsed_ms(a, b) for a, b in evs]
@@ -276,17 +363,21 @@ def main():
         hip.destroy(a)
         hip.destroy(b)
     achieved = fl['ppl'] / (ppl_avg * 1e-3) / 1e12
+    flops_step = flops_sigma * nsig / K
     value = flops_step / (ms_step * 1e-3) / 1e12
-    log(f"step {ms_step:.3f} ms, PPL kernel {ppl_avg:.3f} ms ({achieved:.1f} TFLOP/s), E = {E}")
+    log(f"step {ms_step:.3f} ms ({nsig} sigma in {K} iterations), PPL kernel {ppl_avg:.3f} ms "
+        f"({achieved:.1f} TFLOP/s), E_corr = {theta:.12f}, residual {rn:.2e}")
 
     aat_set = None
     if not args.no_aat_set:
         aat_set = aat_set_measurement(dev, args.seed)
+        if not args.no_cisd_set:
+            aat_set["C"] = cisd_field_set_measurement(dev, args.seed)
 
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(solver, O, V, flops_step, args.seed)
+            cpu = cpu_baseline(solver, O, V, flops_sigma, args.seed)
         except MemoryError as ex:  # host without ~20 GB free
             log(f"cpu baseline skipped: {ex!r}")
 
@@ -304,12 +395,15 @@ def main():
         "dtype": "complex128",
         "data": "synthetic",
         "config": {
-            "workload": f"CISD (spin-orbital) Jacobi+DIIS iteration, seeded complex128 Hamiltonian, O={O} V={V}",
-            "O": O, "V": V, "seed": args.seed, "solver": "jacobi+diis (ci_wfn.py:263-378)",
+            "workload": (f"CISD lowest root (spin-orbital, Hermitian) by Davidson, seeded complex128 Hamiltonian, "
+                         f"O={O} V={V}; one step = one Davidson iteration"),
+            "O": O, "V": V, "seed": args.seed, "solver": f"davidson (max_space {args.max_space})",
+            "sigma_per_step": nsig / K,
             "step_flops_executed": flops_step,
-            "step_flops_dense_equivalent": dense_flops(O, V),
+            "sigma_flops_dense_equivalent": dense_flops(O, V),
+            "E_corr_after_steps": theta, "residual_after_steps": rn,
         },
-        "dense_equivalent_tflops": round(dense_flops(O, V) / (ms_step * 1e-3) / 1e12, 3),
+        "dense_equivalent_tflops": round(dense_flops(O, V) * nsig / K / (ms_step * 1e-3) / 1e12, 3),
         "roofline": {
             "kernel": "zgemm_mfma (particle-particle ladder, packed pairs)",
             "bound": "mfma",
