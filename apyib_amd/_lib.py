@@ -143,6 +143,8 @@ SIGNATURES = {
     "apy_zgemm_tune": (c_int, [ctypes.POINTER(GemmDesc), c_vp, c_size, c_int, c_vp, ctypes.POINTER(c_int),
                                ctypes.POINTER(c_int), ctypes.POINTER(c_dbl)]),
     "apy_zgemm_tune_clear": (c_int, []),
+    "apy_zgemm_tune_set": (c_int, [ctypes.POINTER(GemmDesc), c_int, c_int]),
+    "apy_zgemm_tune_export": (c_int, [_P_I64, c_i64, _P_I64]),
     "apy_so_ints_sizes": (c_int, [c_i64, c_i64, _P_I64]),
     "apy_so_ints_from_full": (c_int, [c_i64, c_vp, ctypes.POINTER(SoInts), c_vp]),
     "apy_so_ints_from_mo": (c_int, [c_i64, c_vp, ctypes.POINTER(SoInts), c_vp]),

@@ -624,4 +624,36 @@ int apy_zgemm_tune_clear(void) {
   return APY_OK;
 }
 
+int apy_zgemm_tune_set(const apy_gemm_desc* d, int tile, int split) {
+  // shape-only: the descriptor's pointers are not used
+  if (!d || d->m < 1 || d->n < 1 || d->k < 1 || d->batch < 1) return APY_E_ARG;
+  if ((d->a_layout != APY_MK && d->a_layout != APY_KM) || (d->b_layout != APY_KN && d->b_layout != APY_NK))
+    return APY_E_ARG;
+  if (tile < 1 || tile > NCFG || split < 1) return APY_E_ARG;
+  const int s = std::min(split, max_split(d));
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tuned[key_of(d)] = std::make_pair(tile, s);
+  return APY_OK;
+}
+
+int apy_zgemm_tune_export(int64_t* rows, int64_t capacity, int64_t* count) {
+  if (!count || capacity < 0 || (capacity > 0 && !rows)) return APY_E_ARG;
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int dev = current_device();
+  int64_t q = 0;
+  for (const auto& kv : g_tuned) {
+    const Key& k = kv.first;
+    if (std::get<7>(k) != dev) continue;
+    if (q < capacity) {
+      int64_t* r = rows + 9 * q;
+      r[0] = std::get<0>(k); r[1] = std::get<1>(k); r[2] = std::get<2>(k);
+      r[3] = std::get<3>(k); r[4] = std::get<4>(k); r[5] = std::get<5>(k); r[6] = std::get<6>(k);
+      r[7] = kv.second.first; r[8] = kv.second.second;
+    }
+    ++q;
+  }
+  *count = q;
+  return APY_OK;
+}
+
 }  // extern "C"

@@ -118,3 +118,35 @@ def mm(A, B, C=None, alpha=1.0, beta=0.0, conj_a=False, conj_b=False, split_k=0,
     return zgemm_raw(m=m, n=n, k=k, a=A, a_layout=al, lda=lda, b=B, b_layout=bl, ldb=ldb,
                      c=C, ldc=C.stride(0), alpha=alpha, beta=beta, conj_a=conj_a, conj_b=conj_b,
                      split_k=split_k, tile=tile, tune=tune)
+
+
+def plans_export():
+    """The GEMM plans cached on the current device (autotuner results or
+    loaded), as a list of dicts."""
+    lib = _lib.load()
+    n = ctypes.c_int64(0)
+    _lib.check(lib.apy_zgemm_tune_export(None, 0, ctypes.byref(n)), "apy_zgemm_tune_export")
+    rows = (ctypes.c_int64 * (9 * max(n.value, 1)))()
+    _lib.check(lib.apy_zgemm_tune_export(rows, n.value, ctypes.byref(n)), "apy_zgemm_tune_export")
+    names = ("m", "n", "k", "a_layout", "b_layout", "batch", "reduce_batch", "tile", "split")
+    return [dict(zip(names, rows[9 * q:9 * q + 9])) for q in range(n.value)]
+
+
+def plans_load(entries):
+    """Install saved plans (the output of plans_export) in the process-wide
+    cache; later GEMMs of those shapes use them instead of the cost model."""
+    lib = _lib.load()
+    for e in entries:
+        d = _lib.GemmDesc()
+        d.m, d.n, d.k = e["m"], e["n"], e["k"]
+        d.a_layout, d.b_layout = e["a_layout"], e["b_layout"]
+        d.batch, d.reduce_batch = max(1, e["batch"]), e["reduce_batch"]
+        d.lda = e["k"] if e["a_layout"] == _lib.APY_MK else e["m"]
+        d.ldb = e["n"] if e["b_layout"] == _lib.APY_KN else e["k"]
+        d.ldc = e["n"]
+        d.stride_a = d.m * d.k
+        d.stride_b = d.k * d.n
+        d.stride_c = 0 if e["reduce_batch"] else d.m * d.n
+        d.alpha_re = 1.0
+        d.a = d.b = d.c = None                # shape-only descriptor (never launched)
+        _lib.check(lib.apy_zgemm_tune_set(ctypes.byref(d), int(e["tile"]), int(e["split"])), "apy_zgemm_tune_set")

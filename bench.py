@@ -297,7 +297,11 @@ def main():
     ap.add_argument("--O", type=int, default=20)
     ap.add_argument("--V", type=int, default=160)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--no-tune", action="store_true", help="skip the GEMM autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="use neither saved plans nor the autotuner (cost model)")
+    ap.add_argument("--tune", action="store_true", help="re-run the GEMM autotuner instead of loading saved plans")
+    ap.add_argument("--plans", default=os.path.join(ROOT, "apyib_amd", "plans_gfx950.json"),
+                    help="saved GEMM plans (loaded unless --tune or --no-tune)")
+    ap.add_argument("--save-plans", default=None, help="write the tuned plans here (with --tune)")
     ap.add_argument("--max-space", type=int, default=12, help="Davidson subspace size before restart")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-aat-set", action="store_true", help="skip the batched finite-difference set timing")
@@ -324,11 +328,21 @@ def main():
     solver = SOCI(ints, torch.from_numpy(F).to(dev), eps, singles=True, fov_coupling=True)
     torch.cuda.synchronize()
     log(f"setup: integrals {ints.nbytes() / 1e9:.2f} GB generated on device in {time.perf_counter() - t0:.2f} s")
+    from apyib_amd.kernels import plans_export, plans_load
+    plans = "model"
     if not args.no_tune:
-        t0 = time.perf_counter()
-        solver.tune()
-        torch.cuda.synchronize()
-        log(f"autotune: {time.perf_counter() - t0:.1f} s")
+        if not args.tune and os.path.exists(args.plans):
+            plans_load(json.load(open(args.plans)))
+            plans = os.path.relpath(args.plans, ROOT)
+            log(f"GEMM plans loaded from {plans}")
+        else:
+            t0 = time.perf_counter()
+            solver.tune(reps=5)
+            torch.cuda.synchronize()
+            plans = "autotuned in this run"
+            log(f"autotune: {time.perf_counter() - t0:.1f} s")
+            if args.save_plans:
+                json.dump(plans_export(), open(args.save_plans, "w"), indent=1)
 
     fl = step_flops(O, V, True)
     flops_sigma = sum(fl.values())
@@ -398,6 +412,7 @@ def main():
             "workload": (f"CISD lowest root (spin-orbital, Hermitian) by Davidson, seeded complex128 Hamiltonian, "
                          f"O={O} V={V}; one step = one Davidson iteration"),
             "O": O, "V": V, "seed": args.seed, "solver": f"davidson (max_space {args.max_space})",
+            "gemm_plans": plans,
             "sigma_per_step": nsig / K,
             "step_flops_executed": flops_step,
             "sigma_flops_dense_equivalent": dense_flops(O, V),

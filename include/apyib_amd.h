@@ -84,6 +84,13 @@ int apy_zgemm_plan(const apy_gemm_desc* d, int* tile, int* split);
 int apy_zgemm_tune(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, int reps, void* stream, int* tile,
                    int* split, double* ms);
 int apy_zgemm_tune_clear(void);
+/* Set the cached plan for d's shape (m, n, k, layouts, batch, reduce) on
+ * the current device, e.g. from a saved table.  Host only. */
+int apy_zgemm_tune_set(const apy_gemm_desc* d, int tile, int split);
+/* Export the current device's cached plans, 9 int64 per entry:
+ * m, n, k, a_layout, b_layout, batch, reduce_batch, tile, split.  *count is
+ * the number of entries (rows are written up to capacity).  Host only. */
+int apy_zgemm_tune_export(int64_t* rows, int64_t capacity, int64_t* count);
 
 /* ------------------------------------------------------------------------
  * Antisymmetrized spin-orbital integral blocks <pq||rs> in the layouts the
