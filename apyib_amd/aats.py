@@ -82,11 +82,18 @@ def spatial_terms(r):
     }
 
 
+def so_terms(r, A0, K0):
+    """<bra|ket> / d of one spin-orbital class split into the reference's
+    terms (compute_SO_I_00, _0S, _0D, _S0, _D0, _SS, _DS, _SD, _DD)."""
+    sx, sy, axx, kyy = r[SX], r[SY], r[AXX], r[KYY]
+    return {'00': A0 * K0, '0S': A0 * sy, '0D': A0 * 0.5 * kyy, 'S0': K0 * sx, 'D0': K0 * 0.5 * axx,
+            'SS': sx * sy + r[S1], 'DS': r[S3] + 0.5 * sy * axx, 'SD': -r[S4] + 0.5 * sx * kyy,
+            'DD': 0.25 * r[E1] + 0.25 * axx * kyy - r[S6]}
+
+
 def so_overlap(r, A0, K0):
     """<bra|ket> / d of one spin-orbital class."""
-    sx, sy, axx, kyy = r[SX], r[SY], r[AXX], r[KYY]
-    return (A0 * K0 + A0 * (sy + 0.5 * kyy) + K0 * (sx + 0.5 * axx) + sx * sy + r[S1] + r[S3] + 0.5 * sy * axx
-            - r[S4] + 0.5 * sx * kyy + 0.25 * r[E1] + 0.25 * axx * kyy - r[S6])
+    return sum(so_terms(r, A0, K0).values())
 
 
 class AAT(object):
@@ -218,13 +225,17 @@ class AAT(object):
             jobs.append((idx, bra[bk], ket[kk]))
             meta.append((alpha, beta, key, bk, kk, sign, (N, N_np, N_nn, N_mp, N_mn)))
 
-    def _run(self, elements, normalization):
-        """Evaluate the elements [(alpha, beta)] -> array of AATs."""
-        I = np.zeros(len(elements))
+    TERMS = ('00', '0S', '0D', 'S0', 'D0', 'SS', 'SD', 'DS', 'DD')
+
+    def _run_terms(self, elements, normalization):
+        """Evaluate the elements [(alpha, beta)] -> {term: array of AAT
+        contributions} (the reference's I_00, I_0S, ... of each element)."""
+        out = {t: np.zeros(len(elements)) for t in self.TERMS}
+        f = 1 / (4 * self.nuc_pert_strength * self.mag_pert_strength)
         if self.method == 'RHF':
             for q, (a, b) in enumerate(elements):
-                I[q] = self._i00(a, b, (1.0,) * 5)
-            return I
+                out['00'][q] = self._i00(a, b, (1.0,) * 5)
+            return out
         combos = _Combos(self)
         jobs, meta = [], []
         for a, b in elements:
@@ -233,22 +244,31 @@ class AAT(object):
         R = self._terms(jobs)
         acc = {}
         for r, (a, b, key, bk, kk, sign, Ns) in zip(R, meta):
-            acc[(a, b)] = acc.get((a, b), 0.0) + sign * self._class_value(r, key, bk, kk, Ns)
+            e = acc.setdefault((a, b), {t: 0.0 for t in self.TERMS})
+            for t, v in self._class_value(r, key, bk, kk, Ns).items():
+                e[t] = e[t] + sign * v
         for q, (a, b) in enumerate(elements):
             Ns = meta[len(CLASSES) * q][6]
-            # the spin-orbital class overlaps already hold the reference-reference
-            # term of pp/pn/np/nn (A0 K0 in so_overlap)
-            I_full = acc[(a, b)] if self.so else self._i00_complex(a, b, Ns) + acc[(a, b)]
-            I[q] = (1 / (4 * self.nuc_pert_strength * self.mag_pert_strength)) * np.imag(I_full)
-        return I
+            if not self.so:              # the SO class overlaps hold the reference-reference term themselves
+                acc[(a, b)]['00'] = acc[(a, b)]['00'] + self._i00_complex(a, b, Ns)
+            for t in self.TERMS:
+                out[t][q] = f * np.imag(acc[(a, b)][t])
+        return out
+
+    def _run(self, elements, normalization):
+        """Evaluate the elements [(alpha, beta)] -> array of AATs."""
+        T = self._run_terms(elements, normalization)
+        return sum(T[t] for t in self.TERMS)
 
     def _class_value(self, r, key, bk, kk, Ns):
+        """{term: contribution} of one overlap class (before the class sign)."""
         N, N_np, N_nn, N_mp, N_mn = Ns
         d = r[D]
         if self.so:
             nb = {'u': N, 'p': N_np, 'n': N_nn}[key[0]]
             nk = {'u': N, 'p': N_mp, 'n': N_mn}[key[1]]
-            return nb * nk * d * so_overlap(r, 1.0 if bk == 't' else 0.0, 1.0 if kk == 't' else 0.0)
+            return {t: nb * nk * d * v for t, v in so_terms(r, 1.0 if bk == 't' else 0.0,
+                                                            1.0 if kk == 't' else 0.0).items()}
         T = spatial_terms(r)
         names = ['DD']
         if self.method == 'CISD':
@@ -261,7 +281,7 @@ class AAT(object):
                 names.append('0D')
                 T['0S'] = T['0S'] * {'p': N_np, 'n': N_nn}[key[0]]
                 names.append('0S')
-        return d * d * sum(T[k] for k in names)
+        return {k: d * d * T[k] for k in names}
 
     def _i00_complex(self, alpha, beta, Ns):
         """HF term (aats.py:672-677; SO aats.py:161-179)."""
@@ -301,6 +321,53 @@ class AAT(object):
         if not self.so:
             raise ValueError("compute_SO_aats needs an _SO method")
         return float(self._run([(alpha, beta)], normalization)[0])
+
+    def _so_term(self, term, alpha, beta, normalization):
+        if not self.so and self.method != 'RHF':
+            raise ValueError("the compute_SO_I_* terms need an _SO method")
+        return float(self._run_terms([(alpha, beta)], normalization)[term][0])
+
+    def compute_SO_I_00(self, alpha, beta, normalization='full'):
+        """aats.py:161-179."""
+        return self._so_term('00', alpha, beta, normalization)
+
+    def compute_SO_I_0D(self, alpha, beta, normalization='full'):
+        """aats.py:183-211."""
+        return self._so_term('0D', alpha, beta, normalization)
+
+    def compute_SO_I_D0(self, alpha, beta, normalization='full'):
+        """aats.py:215-243."""
+        return self._so_term('D0', alpha, beta, normalization)
+
+    def compute_SO_I_DD(self, alpha, beta, normalization='full'):
+        """aats.py:247-302."""
+        return self._so_term('DD', alpha, beta, normalization)
+
+    def compute_SO_I_0S(self, alpha, beta, normalization='full'):
+        """aats.py:306-332."""
+        return self._so_term('0S', alpha, beta, normalization)
+
+    def compute_SO_I_S0(self, alpha, beta, normalization='full'):
+        """aats.py:336-362."""
+        return self._so_term('S0', alpha, beta, normalization)
+
+    def compute_SO_I_SS(self, alpha, beta, normalization='full'):
+        """aats.py:366-412."""
+        return self._so_term('SS', alpha, beta, normalization)
+
+    def compute_SO_I_SD(self, alpha, beta, normalization='full'):
+        """aats.py:416-464."""
+        return self._so_term('SD', alpha, beta, normalization)
+
+    def compute_SO_I_DS(self, alpha, beta, normalization='full'):
+        """aats.py:468-516."""
+        return self._so_term('DS', alpha, beta, normalization)
+
+    def compute_aat_terms(self, normalization='full'):
+        """{term: [3N, 3]} for the whole tensor in one batch."""
+        nA = 3 * self.natom
+        els = [(a, b) for a in range(nA) for b in range(3)]
+        return {t: v.reshape(nA, 3) for t, v in self._run_terms(els, normalization).items()}
 
     def compute_aats(self, normalization='full'):
         """The [3N, 3] tensor in one batch (the element map of

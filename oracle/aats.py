@@ -148,17 +148,21 @@ def normalizations(method, T, alpha, beta, normalization):
     return tuple(_spatial_norm(t, method == 'CISD') for t in sets)
 
 
-def spatial_aat(method, no, nv, nf, overlaps, T, dR, dH, alpha, beta, normalization='full', dets_cache=None):
+def spatial_aat(method, no, nv, nf, overlaps, T, dR, dH, alpha, beta, normalization='full', dets_cache=None,
+                split=False):
     """One AAT element of compute_spatial_aats (aats.py:646-1055).
 
     overlaps: {'uu': S, 'up': [S]*3, 'un', 'pu': [S]*3N, 'nu',
     'pp': [[S]*3]*3N, 'pn', 'np', 'nn'} spatial MO overlaps; T: {'u': [t0,t1,t2],
     'np': [...]*3N, 'nn', 'mp': [...]*3, 'mn'}.  dets_cache (dict) reuses
-    compute_all_dets of shared overlaps across elements."""
+    compute_all_dets of shared overlaps across elements.  split=True returns
+    {'00', '0S', 'S0', '0D', 'D0', 'SS', 'SD', 'DS', 'DD'} -> the reference's
+    I_00, I_0S, ... (aats.py:677-1047) instead of their sum."""
     N, N_np, N_nn, N_mp, N_mn = normalizations(method, T, alpha, beta, normalization)
     d2 = lambda S: np.linalg.det(S[:no, :no]) ** 2
-    I = (d2(overlaps['pp'][alpha][beta]) * N_np * N_mp - d2(overlaps['pn'][alpha][beta]) * N_np * N_mn
-         - d2(overlaps['np'][alpha][beta]) * N_nn * N_mp + d2(overlaps['nn'][alpha][beta]) * N_nn * N_mn)
+    parts = {k: 0.0 for k in SPLIT_TERMS}
+    parts['00'] = (d2(overlaps['pp'][alpha][beta]) * N_np * N_mp - d2(overlaps['pn'][alpha][beta]) * N_np * N_mn
+                   - d2(overlaps['np'][alpha][beta]) * N_nn * N_mp + d2(overlaps['nn'][alpha][beta]) * N_nn * N_mn)
     if method != 'RHF':
         cisd = method == 'CISD'
         na = no - nf
@@ -203,8 +207,19 @@ def spatial_aat(method, no, nv, nf, overlaps, T, dR, dH, alpha, beta, normalizat
                     terms += ['0S', '0D']
                     both.update({k: v for k, v in spatial_class_terms(dets, None, ket, nb, nk).items()
                                  if k in ('0S', '0D')})
-            I = I + sign * sum(both[k] for k in terms)
-    return (1 / (4 * dR * dH)) * np.imag(I)
+            for k in terms:
+                parts[k] = parts[k] + sign * both[k]
+    return _finish(parts, dR, dH, split)
+
+
+SPLIT_TERMS = ('00', '0S', 'S0', '0D', 'D0', 'SS', 'SD', 'DS', 'DD')
+
+
+def _finish(parts, dR, dH, split):
+    f = 1 / (4 * dR * dH)
+    if split:
+        return {k: f * np.imag(v) for k, v in parts.items()}
+    return f * np.imag(sum(parts[k] for k in SPLIT_TERMS))
 
 
 # ----------------------------------------------------------------------------------------------------------
@@ -251,9 +266,12 @@ def _so_coeffs(states, c0, t1, t2):
     return out
 
 
-def so_aat(method, nocc, nvir, overlaps, T, dR, dH, alpha, beta, normalization='full', det_cache=None):
+def so_aat(method, nocc, nvir, overlaps, T, dR, dH, alpha, beta, normalization='full', det_cache=None,
+           split=False):
     """One AAT element of compute_SO_aats (aats.py:520-554) from the SO
-    overlaps (the reference's compute_so_overlap of each MO overlap)."""
+    overlaps (the reference's compute_so_overlap of each MO overlap).
+    split=True returns the bra-segment x ket-segment parts {'00', '0S', ...}
+    -> compute_SO_I_00, compute_SO_I_0S, ... (aats.py:161-516)."""
     N, N_np, N_nn, N_mp, N_mn = normalizations(method, T, alpha, beta, normalization)
     singles = method == 'CISD_SO'
     states = _so_states(nocc, nvir, singles) if method != 'RHF' else [()]
@@ -277,10 +295,19 @@ def so_aat(method, nocc, nvir, overlaps, T, dR, dH, alpha, beta, normalization='
             det_cache[key] = val
         return val
 
-    I = tc @ (D(('pp', alpha, beta)) * N_np * N_mp - D(('pn', alpha, beta)) * N_np * N_mn
-              - D(('np', alpha, beta)) * N_nn * N_mp + D(('nn', alpha, beta)) * N_nn * N_mn) @ t
+    seg = {'0': slice(0, 1), 'S': slice(1, 1 + (nocc * nvir if singles else 0)),
+           'D': slice(1 + (nocc * nvir if singles else 0), len(states))}
+    parts = {k: 0.0 for k in SPLIT_TERMS}
+
+    def add(bra, M, ket):
+        for k in SPLIT_TERMS:
+            b, c = seg[k[0]], seg[k[1]]
+            parts[k] = parts[k] + bra[b] @ M[b, c] @ ket[c]
+
+    add(tc, D(('pp', alpha, beta)) * N_np * N_mp - D(('pn', alpha, beta)) * N_np * N_mn
+        - D(('np', alpha, beta)) * N_nn * N_mp + D(('nn', alpha, beta)) * N_nn * N_mn, t)
     if method != 'RHF':
-        I = I + dR_ @ D(('uu',)) @ dH_ * N * N
-        I = I + dR_ @ (D(('up', beta)) * N * N_mp - D(('un', beta)) * N * N_mn) @ t
-        I = I + tc @ (D(('pu', alpha)) * N_np * N - D(('nu', alpha)) * N_nn * N) @ dH_
-    return (1 / (4 * dR * dH)) * np.imag(I)
+        add(dR_, D(('uu',)) * N * N, dH_)
+        add(dR_, D(('up', beta)) * N * N_mp - D(('un', beta)) * N * N_mn, t)
+        add(tc, D(('pu', alpha)) * N_np * N - D(('nu', alpha)) * N_nn * N, dH_)
+    return _finish(parts, dR, dH, split)

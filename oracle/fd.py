@@ -119,21 +119,24 @@ def amplitude_sets(wfns):
     return {k: (v[2] if k == 'u' else [w[2] for w in v]) for k, v in wfns.items()}
 
 
-def compute_aats(parameters, dR, dH, normalization='full'):
+def compute_aats(parameters, dR, dH, normalization='full', split=False):
     """parallel.compute_parallel_aats (parallel.py:16-48) without the process
-    pool: the [3N, 3] AAT, spatial or spin-orbital by method."""
+    pool: the [3N, 3] AAT, spatial or spin-orbital by method.  split=True
+    returns {term: [3N, 3]} with the reference's I_00, I_0S, ... parts."""
     wfns, ndocc, nbf, nfzc = compute_AAT(parameters, dR, dH)
     m = parameters['method']
     so_form = m.endswith('_SO')
     ov = overlap_sets(wfns, so_form)
     T = amplitude_sets(wfns)
     nA = len(wfns['np'])
-    I = np.zeros((nA, 3))
+    I = {k: np.zeros((nA, 3)) for k in aats.SPLIT_TERMS}
     cache = {}
     for a in range(nA):
         for b in range(3):
             if so_form:
-                I[a, b] = aats.so_aat(m, 2 * ndocc, 2 * (nbf - ndocc), ov, T, dR, dH, a, b, normalization, cache)
+                v = aats.so_aat(m, 2 * ndocc, 2 * (nbf - ndocc), ov, T, dR, dH, a, b, normalization, cache, True)
             else:
-                I[a, b] = aats.spatial_aat(m, ndocc, nbf - ndocc, nfzc, ov, T, dR, dH, a, b, normalization, cache)
-    return I
+                v = aats.spatial_aat(m, ndocc, nbf - ndocc, nfzc, ov, T, dR, dH, a, b, normalization, cache, True)
+            for k in aats.SPLIT_TERMS:
+                I[k][a, b] = v[k]
+    return I if split else sum(I[k] for k in aats.SPLIT_TERMS)

@@ -111,3 +111,20 @@ def test_spin_orbital_determinants_factor_into_spatial():
             assert abs(D[q, 0] - ia[i // 2, a // 2] * d0) < 1e-13
         if i % 2 != a % 2:                      # spin flip: zero
             assert abs(D[q, 0]) < 1e-14
+
+
+@pytest.mark.parametrize("e", [e for e in GOLDEN if e["components"] and runnable(e)], ids=golden_id)
+def test_oracle_aat_terms_golden(e):
+    """The reference's tests also hold the AAT split into I_00, I_0D, I_D0 and
+    I_DD.  Spin-orbital: each part agrees.  Spatial: I_00 and I_DD agree; the
+    spatial code (aats.py:713-1047) has no 0D/D0 parts for MP2 and CID, and
+    the stored 0D and D0 arrays cancel."""
+    T = fd.compute_aats(golden_params(e), e["dR"], e["dH"], e["normalization"], split=True)
+    c = {k[2:4]: np.array(v) for k, v in e["components"].items()}
+    tol = golden_tolerance(e)
+    parts = ("00", "0D", "D0", "DD") if e["form"] == "SO" else ("00", "DD")
+    for k in parts:
+        assert np.abs(T[k] - c[k]).max() < tol, k
+    assert np.abs(T["0D"] + T["D0"] - c["0D"] - c["D0"]).max() < tol
+    for k in ("0S", "S0", "SS", "SD", "DS"):     # no singles in MP2 and CID
+        assert not np.any(T[k])
