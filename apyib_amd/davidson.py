@@ -18,8 +18,11 @@ projected eigenproblem is solved on the host.
 
 The gradient (adjoint) needs no iteration history: for the variational
 root, dE/dF = D, the one-particle density of the normalised vector
-(apy_cisd_so_opd); the left eigenvector is the conjugate of the right one.
-`CISDEnergy` wraps E(F) with this backward in a torch.autograd.Function.
+(apy_cisd_so_opd), and dE/d<pq||rs> = G, its two-particle density
+(apy_cisd_so_tpd, block layouts; apy_so_ints_grad_to_full for the whole
+tensor); the left eigenvector is the conjugate of the right one.
+`CISDEnergy` / `CISDEnergyFull` wrap E(F) and E(F, A) with this backward
+in torch.autograd.Functions.
 """
 import ctypes
 
@@ -193,6 +196,31 @@ class CISDDavidson:
         return D
 
 
+    def gradient_integrals(self, blocks=None):
+        """dE/d<pq||rs> in the block layouts of the device integrals
+        (apy_cisd_so_tpd on the normalised root): an SOIntegrals holding,
+        per stored entry, the derivative with respect to that entry, so
+        that dE = sum over blocks of grad . d(block) for perturbations
+        antisymmetric in (p,q) and (r,s)."""
+        from .so_ci import ALL_BLOCKS, SOIntegrals
+        G = SOIntegrals(self.O, self.V, self.dev, blocks or ALL_BLOCKS)
+        nb = int(self.lib.apy_cisd_so_tpd_workspace(self.O, self.V))
+        ws = workspace(nb, self.dev)
+        _lib.check(self.lib.apy_cisd_so_tpd(self.O, self.V, _vp(self.c), ctypes.byref(G.struct), _vp(ws), nb,
+                                            stream_of(self.dev)), "apy_cisd_so_tpd")
+        return G
+
+
+def full_gradient(G):
+    """Whole-tensor dE/dA [n,n,n,n] (antisymmetric in each index pair) from
+    the block gradients of CISDDavidson.gradient_integrals."""
+    n = G.O + G.V
+    out = torch.empty((n, n, n, n), dtype=CDT, device=G.device)
+    _lib.check(_lib.load().apy_so_ints_grad_to_full(n, ctypes.byref(G.struct), _vp(out), stream_of(G.device)),
+               "apy_so_ints_grad_to_full")
+    return out
+
+
 class CISDEnergy(torch.autograd.Function):
     """E(F) of the lowest Hermitian CISD root at fixed integrals, with the
     adjoint backward dE/dF = D.  E is real and F Hermitian; for a real
@@ -212,7 +240,38 @@ class CISDEnergy(torch.autograd.Function):
         return gE * D.conj(), None, None, None, None
 
 
-def cisd_energy(fock, ints, eps):
+class CISDEnergyFull(torch.autograd.Function):
+    """E(F, A) of the lowest Hermitian CISD root with A = <pq||rs> the whole
+    antisymmetrised tensor [n,n,n,n] (device), with the adjoint backward
+    dE/dF = D and dE/dA = G (two-particle density, antisymmetric in each
+    index pair).  For a real loss the vector-Jacobian products are
+    grad_output * conj(D) and grad_output * conj(G)."""
+
+    @staticmethod
+    def forward(ctx, fock, A, eps, O, e_conv=1e-12, r_conv=1e-9):
+        from .so_ci import SOCI, SOIntegrals
+        ints = SOIntegrals.from_full(A.detach(), int(O))
+        dav = CISDDavidson(SOCI(ints, fock.detach(), eps, singles=True, fov_coupling=True))
+        E = dav.solve(e_conv, r_conv)
+        ctx.dav = dav
+        return torch.tensor(E, dtype=torch.float64, device=fock.device)
+
+    @staticmethod
+    def backward(ctx, gE):
+        D = ctx.dav.gradient_fock()
+        G = full_gradient(ctx.dav.gradient_integrals())
+        return gE * D.conj(), gE * G.conj(), None, None, None, None
+
+
+def cisd_energy(fock, ints, eps, O=None):
+    """Lowest Hermitian CISD root energy behind torch.autograd: `ints` is an
+    SOIntegrals (gradient with respect to F only) or the whole tensor
+    A = <pq||rs> [n,n,n,n] with O occupied spin orbitals (gradients with
+    respect to F and A)."""
+    if isinstance(ints, torch.Tensor):
+        if O is None:
+            raise ValueError("cisd_energy with a whole integral tensor needs O")
+        return CISDEnergyFull.apply(fock, ints, eps, O)
     return CISDEnergy.apply(fock, ints, eps)
 
 

@@ -319,6 +319,26 @@ int apy_ci_precondition(int64_t O, int64_t V, const void* eps, double theta, con
 size_t apy_cisd_so_opd_workspace(int64_t O, int64_t V);
 int apy_cisd_so_opd(int64_t O, int64_t V, const void* c, void* D, void* workspace, size_t ws_bytes, void* stream);
 
+/* Two-particle density of the normalised vector c: the gradient of the
+ * lowest-root energy with respect to the antisymmetrised integrals
+ * <pq||rs> (the TPD of the adjoint sketch at ci_wfn.py:396-409, there for
+ * the Dirac integrals), written into `grad` in the block layouts of
+ * apy_so_ints (any block pointer may be NULL to skip it).  Each stored
+ * entry is the derivative with respect to that stored entry, i.e. the sum
+ * over the full-tensor positions it stands for, so that
+ *   dE = sum_blocks sum_entries grad_blk * d(ints_blk)
+ * for every perturbation antisymmetric in (p,q) and (r,s); with the OPD,
+ * E = sum F D + sum_blocks ints_blk . grad_blk.  vvvv_p, oooo_p, ring and
+ * vovv are GEMMs on the matrix cores.  O, V >= 2. */
+size_t apy_cisd_so_tpd_workspace(int64_t O, int64_t V);
+int apy_cisd_so_tpd(int64_t O, int64_t V, const void* c, const apy_so_ints* grad, void* workspace, size_t ws_bytes,
+                    void* stream);
+/* Whole-tensor gradient G[n][n][n][n] (n = O + V) from block gradients (all
+ * nine blocks required): the adjoint of apy_so_ints_from_full for
+ * antisymmetric perturbations, G antisymmetric in (p,q) and (r,s), zero on
+ * p == q or r == s, so that dE = sum G dA. */
+int apy_so_ints_grad_to_full(int64_t n, const apy_so_ints* grad, void* full, void* stream);
+
 /* ------------------------------------------------------------------------
  * The same non-canonical MP2 (or lambda) iteration batched over independent
  * instances of possibly different (O, V): the +/- displaced systems of a

@@ -1,5 +1,5 @@
-"""Lowest CISD root by Davidson iteration and its energy gradient, NumPy
-(oracle only).
+"""Lowest CISD root by Davidson iteration and its energy gradients, NumPy
+(oracle only; test infrastructure, never the product path).
 
 The operator is the Hermitian spin-orbital CISD matrix (the reference's
 ci_wfn.py:309-334 terms plus the P(ij)P(ab) f_bj t_ia coupling that the
@@ -17,9 +17,14 @@ form of the reference's adjoint sketch (ci_wfn.py:380-394):
     D_vo[a,i] =  conj(c1[i,a]) c0 + sum conj(c2[i,j,a,b]) c1[j,b]
 so that dE = sum_pq D[p,q] dF[p,q] for any one-electron perturbation.
 
+The gradient with respect to the antisymmetrised integrals A = <pq||rs>
+is the two-particle density G (two_particle_density; block layouts of the
+device integrals in tpd_blocks).
+
 Pinned by: the lowest eigenpair of the explicit CISD matrix (oracle.brute)
-and, for D, Hellmann-Feynman derivatives of that matrix's lowest eigenvalue
-with respect to each F[p,q] (tests/test_oracle_davidson.py).
+and, for D and G, Hellmann-Feynman derivatives of that matrix's lowest
+eigenvalue with respect to each F[p,q] and along each antisymmetrised unit
+direction of A (tests/test_oracle_davidson.py).
 """
 import numpy as np
 
@@ -67,7 +72,11 @@ def davidson(F, W, O, tol=1e-12, rtol=1e-9, max_space=20, maxit=200):
         if theta_old is not None and abs(theta - theta_old) < tol and rn < rtol:
             break
         theta_old = theta
-        d = (r[0] / theta if theta != 0 else r[0], r[1] / (theta - D1), np.where(D2 != 0, r[2] / (theta - D2), 0))
+        d2 = r[2] / (theta - D2)
+        # keep the correction in the antisymmetric doubles space: rounding
+        # outside it is amplified by small denominators and never removed
+        d2 = 0.25 * (d2 - d2.swapaxes(0, 1) - d2.swapaxes(2, 3) + d2.swapaxes(0, 1).swapaxes(2, 3))
+        d = (r[0] / theta if theta != 0 else r[0], r[1] / (theta - D1), d2)
         if k >= max_space:
             B, S = [x], [tuple(sum(y[j] * S[j][q] for j in range(k)) for q in range(3))]
         for _ in range(2):
@@ -95,3 +104,92 @@ def one_particle_density(c, O):
     D[o, v] = np.conj(c0) * c1 + contract('jb,ijab->ia', np.conj(c1), c2)
     D[v, o] = np.conj(c1.T) * c0 + contract('ijab,jb->ai', np.conj(c2), c1)
     return D
+
+
+def _antisym(G):
+    return 0.25 * (G - G.swapaxes(0, 1) - G.swapaxes(2, 3) + G.swapaxes(0, 1).swapaxes(2, 3))
+
+
+def two_particle_density(c, O):
+    """G[p,q,r,s] = dE/dA[p,q,r,s] of the normalised CISD vector c = (c0, c1,
+    c2), A = <pq||rs>, antisymmetric in (p,q) and in (r,s), so that
+    dE = sum G dA for every antisymmetric dA and E = sum F D + sum A G.
+
+    The blocks follow the reference's TPD sketch (ci_wfn.py:396-409, written
+    there for the plain Dirac integrals) restated for the antisymmetrised
+    blocks as the sigma equations read them (ci_wfn.py:309-334 plus the
+    reference row); each block is the derivative of conj(c) . H c with
+    respect to the block the sigma reads, then the whole tensor is
+    antisymmetrised."""
+    c0, c1, c2 = c
+    V = c1.shape[1]
+    n = O + V
+    o, v = slice(0, O), slice(O, n)
+    cc1, cc2 = np.conj(c1), np.conj(c2)
+    G = np.zeros((n, n, n, n), complex)
+    G[v, v, v, v] = 0.125 * contract('ijab,ijcd->abcd', cc2, c2)
+    G[o, o, o, o] = 0.125 * contract('ijab,klab->klij', cc2, c2)
+    G[o, v, v, o] = contract('ia,jb->jabi', cc1, c1) + contract('ijab,ikac->kbcj', cc2, c2)
+    G[v, o, v, v] = 0.5 * contract('ia,ijcb->ajcb', cc1, c2)
+    G[o, o, o, v] = -0.5 * contract('ia,kjab->kjib', cc1, c2)
+    G[v, v, o, o] = 0.25 * c0 * cc2.transpose(2, 3, 0, 1)
+    G[o, v, o, o] = -0.25 * contract('ijab,ka->kbij', cc2, c1)
+    G[v, o, o, o] = -0.25 * contract('ijab,kb->akij', cc2, c1)
+    G[v, v, v, o] = 0.25 * contract('ijab,ic->abcj', cc2, c1)
+    G[v, v, o, v] = 0.25 * contract('ijab,jc->abic', cc2, c1)
+    G[o, o, v, v] = 0.25 * np.conj(c0) * c2
+    return _antisym(G)
+
+
+def _pairs(N):
+    return [(p, q) for p in range(N) for q in range(p + 1, N)]
+
+
+def tpd_blocks(G, O):
+    """The density in the packed block layouts of the device integrals
+    (vvvv_p, ring, oooo_p, ovoo_p, vvvo_p, vovv, ooov, vvoo_t, oovv): each
+    stored entry carries the sum over the full-tensor positions it stands
+    for (4 for the packed blocks and ring, 2 for vovv and ooov, 1 for
+    vvoo_t and oovv), so that dE = sum_blocks sum Gblk * dblk."""
+    n = G.shape[0]
+    V = n - O
+    Po, Pv = _pairs(O), _pairs(V)
+    oi = np.array([p for p, _ in Po], int), np.array([q for _, q in Po], int)
+    vi = np.array([p for p, _ in Pv], int) + O, np.array([q for _, q in Pv], int) + O
+    o, v = slice(0, O), slice(O, n)
+    out = {}
+    out['vvvv_p'] = 4 * G[vi[0][:, None], vi[1][:, None], vi[0][None, :], vi[1][None, :]]
+    # ring[(j,b),(k,c)] = <kb||cj>
+    out['ring'] = 4 * G[o, v, v, o].transpose(3, 1, 0, 2)
+    # oooo_p[(i<j)][(k<l)] = <kl||ij>
+    out['oooo_p'] = 4 * G[oi[0][None, :], oi[1][None, :], oi[0][:, None], oi[1][:, None]]
+    # ovoo_p[k][b][(i<j)] = <kb||ij>
+    out['ovoo_p'] = 4 * G[o, v][:, :, oi[0], oi[1]]
+    # vvvo_p[(a<b)][j][c] = <ab||cj>
+    out['vvvo_p'] = 4 * G[vi[0], vi[1]][:, v, o].transpose(0, 2, 1)
+    out['vovv'] = 2 * G[v, o, v, v]
+    out['ooov'] = 2 * G[o, o, o, v]
+    out['vvoo_t'] = G[v, v, o, o].transpose(2, 3, 0, 1)
+    out['oovv'] = G[o, o, v, v].copy()
+    return out
+
+
+def integral_blocks(A, O):
+    """The same block layouts of the integrals themselves (A = <pq||rs>)."""
+    n = A.shape[0]
+    V = n - O
+    Po, Pv = _pairs(O), _pairs(V)
+    oi = np.array([p for p, _ in Po], int), np.array([q for _, q in Po], int)
+    vi = np.array([p for p, _ in Pv], int) + O, np.array([q for _, q in Pv], int) + O
+    o, v = slice(0, O), slice(O, n)
+    return {
+        'vvvv_p': A[vi[0][:, None], vi[1][:, None], vi[0][None, :], vi[1][None, :]],
+        'ring': A[o, v, v, o].transpose(3, 1, 0, 2),
+        'oooo_p': A[oi[0][None, :], oi[1][None, :], oi[0][:, None], oi[1][:, None]],
+        'ovoo_p': A[o, v][:, :, oi[0], oi[1]],
+        'vvvo_p': A[vi[0], vi[1]][:, v, o].transpose(0, 2, 1),
+        'vovv': A[v, o, v, v].copy(),
+        'ooov': A[o, o, o, v].copy(),
+        'vvoo_t': A[v, v, o, o].transpose(2, 3, 0, 1),
+        'oovv': A[o, o, v, v].copy(),
+    }

@@ -60,3 +60,53 @@ def test_one_particle_density_is_energy_gradient(O, V, seed):
             Hpq, Eref_pq = brute.hamiltonian(E1, Z, sp)
             g = np.vdot(u, Hpq @ u) - Eref_pq
             assert abs(D[p, q] - g) < 1e-9, (p, q)
+
+
+def _unit_asym(n, p, q, r, s):
+    X = np.zeros((n, n, n, n), complex)
+    X[p, q, r, s] += 1.0
+    X[q, p, r, s] -= 1.0
+    X[p, q, s, r] -= 1.0
+    X[q, p, s, r] += 1.0
+    return X
+
+
+@pytest.mark.parametrize("O,V,seed", [(2, 3, 5), (3, 3, 6)])
+def test_two_particle_density_is_energy_gradient(O, V, seed):
+    """G[p,q,r,s] = dE_corr/dA along the antisymmetrised unit direction at
+    (p<q, r<s): c^H (dH) c - dE_ref with the explicit matrix (h depends on A
+    through -sum_m <pm||qm>), every position of every o/v block, including
+    directions that break Hermiticity (first-order perturbation theory holds
+    for any direction at a Hermitian point)."""
+    F, A, W = _system(O, V, seed)
+    n = O + V
+    sp = brute.CISpace(n, O, singles=True, doubles=True)
+    H, _ = brute.hamiltonian(F, A, sp)
+    w, U = np.linalg.eigh(H)
+    u = U[:, 0]
+    _, c, _ = davidson.davidson(F, W, O, tol=1e-13, rtol=1e-11)
+    G = davidson.two_particle_density(c, O)
+    Z = np.zeros((n, n), complex)
+    for p in range(n):
+        for q in range(p + 1, n):
+            for r in range(n):
+                for s in range(r + 1, n):
+                    X = _unit_asym(n, p, q, r, s)
+                    Hx, Ex = brute.hamiltonian(Z, X, sp)
+                    g = np.vdot(u, Hx @ u) - Ex
+                    assert abs(np.sum(G * X) - g) < 1e-9, (p, q, r, s)
+
+
+def test_density_energy_identity_and_blocks():
+    """E_corr = sum F D + sum A G (the reference sketch's check,
+    ci_wfn.py:411-414), and the block layouts carry the same sum."""
+    O, V = 3, 4
+    F, A, W = _system(O, V, 8)
+    E, c, _ = davidson.davidson(F, W, O, tol=1e-13, rtol=1e-11)
+    D = davidson.one_particle_density(c, O)
+    G = davidson.two_particle_density(c, O)
+    assert abs(np.sum(F * D) + np.sum(A * G) - E) < 1e-12
+    Gb, Ab = davidson.tpd_blocks(G, O), davidson.integral_blocks(A, O)
+    assert abs(sum(np.sum(Gb[k] * Ab[k]) for k in Gb) - np.sum(A * G)) < 1e-13
+    # G is antisymmetric in each index pair
+    assert np.abs(G + G.swapaxes(0, 1)).max() < 1e-15 and np.abs(G + G.swapaxes(2, 3)).max() < 1e-15
