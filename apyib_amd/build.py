@@ -22,6 +22,9 @@ FLAGS = [
     "-munsafe-fp-atomics",
     # Packed FP32 instructions stay disabled for every kernel in this library.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: no copies between the VGPR and
+    # accumulation register files around the GEMM main loops.
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-result",
 ]
 

@@ -9,7 +9,14 @@
 // double-buffered LDS tile, BK complex elements at a time, with two register
 // stages in front of it: while the MFMAs consume LDS buffer s, stage s+1
 // waits in registers for the other buffer and stage s+2 is in flight from
-// memory; each stage needs a single barrier.
+// memory; each stage needs a single barrier.  The loads of a stage carry
+// no data dependence (clamped addresses; bounds mask and conjugation are
+// applied when the registers go to LDS) and the loop body has no
+// conditional compute, so the wait counts before each LDS store leave the
+// newest stage's loads in flight.  The BK = 8 tiles (configs 8, 9) halve
+// the LDS per block, which raises the resident blocks per CU; on the
+// 190 x 12720 x 12720 ladder product they run at 0.85 of the FP64 MFMA
+// peak.
 //
 // MFMA f64 16x16x4 lane layout (checked on gfx950):
 //   A operand: lane l holds A[l%16][l/16]       (16 x 4)
@@ -60,6 +67,13 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t total) {
   return (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
+// v (conjugated when sg holds the sign bit) if in, else 0; branch-free.
+__device__ inline double2 masked(double2 v, bool in, unsigned long long sg) {
+  const unsigned long long m = in ? ~0ull : 0ull;
+  return make_double2(__longlong_as_double(__double_as_longlong(v.x) & m),
+                      __longlong_as_double((__double_as_longlong(v.y) ^ sg) & m));
+}
+
 template <int BM, int BN, int WM, int WN, int BK, bool A_MK, bool B_NK>
 __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
   constexpr int RB = WM / 16, CB = WN / 16;
@@ -104,10 +118,16 @@ __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
   // Two register stages: stage s+2 is in flight while stage s+1 waits in
   // registers and stage s is consumed from LDS.
   double2 ra[2][A_PER], rb[2][B_PER];
-  const double csa = p.conj_a ? -1.0 : 1.0, csb = p.conj_b ? -1.0 : 1.0;
+  // conjugation as a sign-bit flip of the imaginary part
+  const unsigned long long sga = p.conj_a ? 0x8000000000000000ull : 0ull;
+  const unsigned long long sgb = p.conj_b ? 0x8000000000000000ull : 0ull;
 
   // `set` / `buf` arguments are std::integral_constant so that every
   // register-array index is a compile-time constant (no scratch memory).
+  // Loads read clamped (always in-bounds) addresses and carry no data
+  // dependence, so a stage's loads stay in flight while the previous stage
+  // computes; the bounds mask and the conjugation sign are applied when the
+  // registers are written to LDS one stage later.
   auto load = [&](int64_t k0, auto setc) {
     constexpr int set = decltype(setc)::value;
 #pragma unroll
@@ -115,53 +135,58 @@ __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
       const int idx = tid + NTHR * q;
       int r, kk;
       if (A_MK) { r = idx / BK; kk = idx % BK; } else { kk = idx / BM; r = idx % BM; }
-      const int64_t gm = m0 + r, gk = k0 + kk;
-      double2 v = make_double2(0.0, 0.0);
-      if (gm < p.m && gk < kend) v = A_MK ? A[gm * p.lda + gk] : A[gk * p.lda + gm];
-      v.y *= csa;
-      ra[set][q] = v;
+      const int64_t gm = min(m0 + r, p.m - 1), gk = min(k0 + kk, kend - 1);
+      ra[set][q] = A_MK ? A[gm * p.lda + gk] : A[gk * p.lda + gm];
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
       const int idx = tid + NTHR * q;
       int c, kk;
       if (B_NK) { c = idx / BK; kk = idx % BK; } else { kk = idx / BN; c = idx % BN; }
-      const int64_t gn = n0 + c, gk = k0 + kk;
-      double2 v = make_double2(0.0, 0.0);
-      if (gn < p.n && gk < kend) v = B_NK ? B[gn * p.ldb + gk] : B[gk * p.ldb + gn];
-      v.y *= csb;
-      rb[set][q] = v;
+      const int64_t gn = min(n0 + c, p.n - 1), gk = min(k0 + kk, kend - 1);
+      rb[set][q] = B_NK ? B[gn * p.ldb + gk] : B[gk * p.ldb + gn];
     }
   };
-  auto store = [&](auto bufc) {
+  auto store = [&](auto bufc, int64_t k0) {
     constexpr int buf = decltype(bufc)::value, set = buf;
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
       const int idx = tid + NTHR * q;
       int r, kk;
       if (A_MK) { r = idx / BK; kk = idx % BK; } else { kk = idx / BM; r = idx % BM; }
-      As[buf][r * BKP + kk] = ra[set][q];
+      const bool in = (m0 + r < p.m) && (k0 + kk < kend);
+      As[buf][r * BKP + kk] = masked(ra[set][q], in, sga);
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
       const int idx = tid + NTHR * q;
       int c, kk;
       if (B_NK) { c = idx / BK; kk = idx % BK; } else { kk = idx / BN; c = idx % BN; }
-      Bs[buf][B_NK ? c * BKP + kk : kk * BN + c] = rb[set][q];
+      const bool in = (n0 + c < p.n) && (k0 + kk < kend);
+      Bs[buf][B_NK ? c * BKP + kk : kk * BN + c] = masked(rb[set][q], in, sgb);
     }
   };
 
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
-  const int64_t nstages = (kend - kbeg + BK - 1) / BK;
-  if (nstages > 0) load(kbeg, I0{});
-  if (nstages > 1) load(kbeg + BK, I1{});
-  if (nstages > 0) store(I0{});
+  // An even number of stages (a padded last stage reads zeros) keeps the
+  // two-stage loop body free of conditional compute, so the accumulators
+  // stay in the matrix-core registers across iterations.
+  const int64_t nreal = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  const int64_t nstages = (nreal + 1) & ~(int64_t)1;
+  if (nstages > 0) {
+    load(kbeg, I0{});
+    load(kbeg + BK, I1{});
+    store(I0{}, kbeg);
+  }
   __syncthreads();
   auto stage = [&](int64_t st, auto curc) {
     constexpr int cur = decltype(curc)::value;
     using Other = std::integral_constant<int, cur ^ 1>;
-    if (st + 2 < nstages) load(kbeg + (st + 2) * BK, curc);  // set `cur` was stored last stage
+    // set `cur` was stored last stage; past the end the clamped loads
+    // re-read the last stage's lines and are masked at the store.  No branch
+    // here: the wait counts at the stores then cover only the older loads.
+    load(kbeg + (st + 2) * BK, curc);
     const double2* Ab = As[cur];
     const double2* Bb = Bs[cur];
 #pragma unroll
@@ -190,12 +215,12 @@ __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
           acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].y, bv[j].x, acc_im[i][j], 0, 0, 0);
         }
     }
-    if (st + 1 < nstages) store(Other{});
+    store(Other{}, kbeg + (st + 1) * BK);  // past the end: writes zeros
     __syncthreads();
   };
   for (int64_t st = 0; st < nstages; st += 2) {
     stage(st, I0{});
-    if (st + 1 < nstages) stage(st + 1, I1{});
+    stage(st + 1, I1{});
   }
 
   // Epilogue.
@@ -256,16 +281,19 @@ struct Cfg {
   int bm, bn, wm, wn, bk;
   double eff;  // relative MFMA efficiency used by the cost model
 };
-constexpr int NCFG = 7;
+constexpr int NCFG = 9;
 constexpr Cfg CFGS[NCFG + 1] = {
+    // eff: measured fraction of the FP64 MFMA peak on large products
     {0, 0, 0, 0, 0, 0.0},          // 0: automatic
-    {64, 64, 32, 32, 16, 0.68},    // 1
-    {32, 64, 16, 32, 16, 0.70},    // 2
-    {64, 128, 32, 64, 16, 0.58},   // 3
-    {128, 64, 64, 32, 16, 0.50},   // 4
-    {96, 64, 48, 32, 16, 0.58},    // 5
-    {64, 64, 32, 32, 32, 0.60},    // 6
-    {32, 32, 16, 16, 16, 0.70},    // 7
+    {64, 64, 32, 32, 16, 0.83},    // 1
+    {32, 64, 16, 32, 16, 0.82},    // 2
+    {64, 128, 32, 64, 16, 0.72},   // 3
+    {128, 64, 64, 32, 16, 0.60},   // 4
+    {96, 64, 48, 32, 16, 0.71},    // 5
+    {64, 64, 32, 32, 32, 0.69},    // 6
+    {32, 32, 16, 16, 16, 0.76},    // 7
+    {64, 64, 32, 32, 8, 0.855},    // 8: half the LDS of 1, more blocks per CU
+    {32, 64, 16, 32, 8, 0.81},     // 9
 };
 
 template <int CI, bool AM, bool BN_>
@@ -301,6 +329,8 @@ void launch_cfg(int cfg, const GemmArgs& a, bool am, bool bnk, hipStream_t s) {
     case 4: launch_layout<4>(a, am, bnk, s); break;
     case 5: launch_layout<5>(a, am, bnk, s); break;
     case 6: launch_layout<6>(a, am, bnk, s); break;
+    case 8: launch_layout<8>(a, am, bnk, s); break;
+    case 9: launch_layout<9>(a, am, bnk, s); break;
     default: launch_layout<7>(a, am, bnk, s); break;
   }
 }
@@ -313,6 +343,8 @@ void* cfg_kernel(int cfg, bool am, bool bnk) {
     case 4: return kernel_ptr<4>(am, bnk);
     case 5: return kernel_ptr<5>(am, bnk);
     case 6: return kernel_ptr<6>(am, bnk);
+    case 8: return kernel_ptr<8>(am, bnk);
+    case 9: return kernel_ptr<9>(am, bnk);
     default: return kernel_ptr<7>(am, bnk);
   }
 }
