@@ -186,6 +186,7 @@ SIGNATURES = {
     "apy_ci_precondition": (c_int, [c_i64, c_i64, c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "apy_cisd_so_opd_workspace": (c_size, [c_i64, c_i64]),
     "apy_cisd_so_opd": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_ci_dots": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_cisd_so_tpd_workspace": (c_size, [c_i64, c_i64]),
     "apy_cisd_so_tpd": (c_int, [c_i64, c_i64, c_vp, ctypes.POINTER(SoInts), c_vp, c_size, c_vp]),
     "apy_so_ints_grad_to_full": (c_int, [c_i64, ctypes.POINTER(SoInts), c_vp, c_vp]),

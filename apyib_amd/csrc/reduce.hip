@@ -96,6 +96,45 @@ __global__ __launch_bounds__(256) void combine_kernel(VecList xs, CoefList c, in
   }
 }
 
+// CI-metric dots over the vector layout [c1 (OV) | c2 (O^2 V^2, weight 1/4) | c0]:
+// sum_l w_l conj(x_m[l]) y[l] in one pass.
+template <int K>
+__global__ __launch_bounds__(256) void ci_dot_partial(VecList xs, const double2* __restrict__ y, int64_t ov,
+                                                      int64_t len, double2* __restrict__ partial) {
+  double ar[K], ai[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) { ar[m] = 0.0; ai[m] = 0.0; }
+  const int64_t l2 = len - 1;  // c0 sits at the end
+  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < len; l += (int64_t)gridDim.x * blockDim.x) {
+    const double2 yv = y[l];
+    const double w = (l >= ov && l < l2) ? 0.25 : 1.0;
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      const double2 x = xs.p[m][l];
+      ar[m] += w * (x.x * yv.x + x.y * yv.y);
+      ai[m] += w * (x.x * yv.y - x.y * yv.x);
+    }
+  }
+  __shared__ double sr[256], si[256];
+#pragma unroll 1
+  for (int m = 0; m < K; ++m) {
+    sr[threadIdx.x] = ar[m];
+    si[threadIdx.x] = ai[m];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) { sr[threadIdx.x] += sr[threadIdx.x + w]; si[threadIdx.x] += si[threadIdx.x + w]; }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * K + m] = make_double2(sr[0], si[0]);
+    __syncthreads();
+  }
+}
+
+template <int K>
+void launch_ci_dot(VecList v, const double2* y, int64_t ov, int64_t len, double2* part, hipStream_t s) {
+  ci_dot_partial<K><<<NB, 256, 0, s>>>(v, y, ov, len, part);
+}
+
 template <int K>
 void launch_dot(VecList v, const double2* y, int64_t len, int conj, double2* part, hipStream_t s) {
   dot_many_partial<K><<<NB, 256, 0, s>>>(v, y, len, conj, part);
@@ -129,6 +168,29 @@ int apy_zdot_many(int k, const void* const* xs, const void* y, int64_t len, int 
   double2* part = (double2*)workspace;
   switch (k) {
 #define C(K) case K: launch_dot<K>(v, (const double2*)y, len, conj_x, part, s); break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
+#undef C
+  }
+  APY_LAUNCH_CHECK();
+  finalize_many<<<1, 256, 0, s>>>(part, NB, k, (double2*)out);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_ci_dots(int64_t O, int64_t V, int k, const void* const* xs, const void* y, void* out, void* workspace,
+                size_t ws_bytes, void* stream) {
+  if (O < 1 || V < 1 || k < 1 || k > APY_MAXVEC || !xs || !y || !out) return APY_E_ARG;
+  if (!workspace || ws_bytes < apy_reduce_workspace(0, k)) return APY_E_WORKSPACE;
+  const int64_t ov = O * V, len = ov + ov * ov + 1;
+  VecList v{};
+  for (int m = 0; m < k; ++m) {
+    if (!xs[m]) return APY_E_ARG;
+    v.p[m] = (const double2*)xs[m];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  double2* part = (double2*)workspace;
+  switch (k) {
+#define C(K) case K: launch_ci_dot<K>(v, (const double2*)y, ov, len, part, s); break;
     C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
 #undef C
   }

@@ -12,7 +12,7 @@ Vectors live on the device with layout [c1 | c2 | c0] (length L + 1);
 inner products count each determinant once (c2 dense, weight 1/4).  Every
 vector operation is a library call: sigma (apy_cisd_so_step, SIGMA mode),
 the reference row (apy_cisd_so_energy), CI-metric dot products
-(apy_zdot_many over the three segments), linear combinations
+(apy_ci_dots, one pass with the c2 weight), linear combinations
 (apy_zcombine) and the preconditioner (apy_ci_precondition).  The k x k
 projected eigenproblem is solved on the host.
 
@@ -52,7 +52,7 @@ class CISDDavidson:
         self.B = torch.zeros((self.max_space, self.L + 1), dtype=CDT, device=self.dev)
         self.S = torch.zeros((self.max_space, self.L + 1), dtype=CDT, device=self.dev)
         self.work = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
-        self.part = torch.zeros(_lib.APY_MAXVEC, dtype=CDT, device=self.dev)
+        self.part = torch.zeros(max(_lib.APY_MAXVEC, self.max_space + 1), dtype=CDT, device=self.dev)
         self.c = None
         self.E = None
         self.iterations = 0
@@ -68,20 +68,19 @@ class CISDDavidson:
         self.soci.energy(b[:self.L], out=s[self.L:])
 
     def dots(self, xs, y):
-        """CI-metric <x_m|y> for m < len(xs) (host complex array)."""
+        """CI-metric <x_m|y> for m < len(xs) (host complex array): one
+        apy_ci_dots pass per APY_MAXVEC vectors, one host copy."""
         out = np.zeros(len(xs), dtype=np.complex128)
-        segs = ((0, self.OV, 1.0), (self.OV, self.L, 0.25), (self.L, self.L + 1, 1.0))
-        for s0 in range(0, len(xs), _lib.APY_MAXVEC):
-            chunk = xs[s0:s0 + _lib.APY_MAXVEC]
-            acc = 0
-            for lo, hi, w in segs:
-                arr = (ctypes.c_void_p * len(chunk))(*[x[lo:hi].data_ptr() for x in chunk])
-                nb = int(self.lib.apy_reduce_workspace(hi - lo, len(chunk)))
-                ws = workspace(nb, self.dev)
-                _lib.check(self.lib.apy_zdot_many(len(chunk), arr, _vp(y[lo:hi]), hi - lo, 1, _vp(self.part), _vp(ws),
-                                                  nb, stream_of(self.dev)), "apy_zdot_many")
-                acc = acc + w * self.part[:len(chunk)].cpu().numpy()
-            out[s0:s0 + len(chunk)] = acc
+        M = _lib.APY_MAXVEC
+        for s0 in range(0, len(xs), M):
+            chunk = xs[s0:s0 + M]
+            arr = (ctypes.c_void_p * len(chunk))(*[x.data_ptr() for x in chunk])
+            nb = int(self.lib.apy_reduce_workspace(0, len(chunk)))
+            ws = workspace(nb, self.dev)
+            _lib.check(self.lib.apy_ci_dots(self.O, self.V, len(chunk), arr, _vp(y), _vp(self.part[s0:]), _vp(ws), nb,
+                                            stream_of(self.dev)), "apy_ci_dots")
+        n = len(xs)
+        out[:] = self.part[:n].cpu().numpy()
         return out
 
     def combine(self, xs, coef, y):

@@ -316,6 +316,12 @@ int apy_mp2_nc_densities(int64_t O, int64_t V, const void* t, const void* lam, v
  * ---------------------------------------------------------------------- */
 int apy_ci_precondition(int64_t O, int64_t V, const void* eps, double theta, const void* r, void* out,
                         void* stream);
+/* CI-metric inner products of Davidson vectors (layout [c1 | c2 | c0],
+ * length O V + O^2 V^2 + 1): out[m] = sum_l w_l conj(x_m[l]) y[l] with
+ * w = 1/4 on the dense c2 block (each determinant counted once), 1 else;
+ * k <= APY_MAXVEC, workspace apy_reduce_workspace(0, k); deterministic. */
+int apy_ci_dots(int64_t O, int64_t V, int k, const void* const* xs, const void* y, void* out, void* workspace,
+                size_t ws_bytes, void* stream);
 size_t apy_cisd_so_opd_workspace(int64_t O, int64_t V);
 int apy_cisd_so_opd(int64_t O, int64_t V, const void* c, void* D, void* workspace, size_t ws_bytes, void* stream);
 
