@@ -60,6 +60,20 @@ class SoInts(ctypes.Structure):
 
 SO_BLOCKS = ("vvvv_p", "ring", "oooo_p", "ovoo_p", "vvvo_p", "vovv", "ooov", "vvoo_t", "oovv")
 
+SP_BLOCKS = ("vvvv", "oooo", "w21", "w3", "vvoo", "l", "vvvo", "vooo", "s1", "w4", "w5")
+
+
+class SpInts(ctypes.Structure):
+    _fields_ = [("o", c_i64), ("v", c_i64)] + [(b, c_vp) for b in SP_BLOCKS]
+
+
+class CisdSpArgs(ctypes.Structure):
+    _fields_ = [
+        ("o", c_i64), ("v", c_i64), ("fock", c_vp), ("eps", c_vp), ("ints", ctypes.POINTER(SpInts)),
+        ("hoo", c_vp), ("g", c_vp), ("singles", c_int), ("energy", c_vp), ("t", c_vp), ("t_new", c_vp),
+        ("out", c_vp),
+    ]
+
 APY_MODE_RESIDUAL = 0
 APY_MODE_SIGMA = 1
 APY_MAXVEC = 16
@@ -187,6 +201,13 @@ SIGNATURES = {
     "apy_cisd_so_opd_workspace": (c_size, [c_i64, c_i64]),
     "apy_cisd_so_opd": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_ci_dots": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_sp_ints_sizes": (c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),
+    "apy_sp_ints_from_mo": (c_int, [c_i64, c_vp, ctypes.POINTER(SpInts), c_vp]),
+    "apy_cisd_sp_prepare": (c_int, [c_i64, c_i64, c_vp, ctypes.POINTER(SpInts), c_vp, c_vp, c_vp]),
+    "apy_cisd_sp_workspace": (c_size, [c_i64, c_i64, c_int]),
+    "apy_cisd_sp_step": (c_int, [ctypes.POINTER(CisdSpArgs), c_vp, c_size, c_vp]),
+    "apy_cisd_sp_energy_workspace": (c_size, []),
+    "apy_cisd_sp_energy": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_cisd_so_tpd_workspace": (c_size, [c_i64, c_i64]),
     "apy_cisd_so_tpd": (c_int, [c_i64, c_i64, c_vp, ctypes.POINTER(SoInts), c_vp, c_size, c_vp]),
     "apy_so_ints_grad_to_full": (c_int, [c_i64, ctypes.POINTER(SoInts), c_vp, c_vp]),

@@ -2,18 +2,16 @@
 GPU.
 
 The spin-orbital solvers run the reference's Jacobi + DIIS iteration with
-the sigma of apy_cisd_so_step.  The spin-adapted (spatial) solvers solve the
-same equations in spin orbitals and return the closed-shell amplitudes
-t1[i,a] = t1[i alpha, a alpha] and t2[i,j,a,b] = t2[i alpha, j beta, a alpha,
-b beta], which are the spatial amplitudes of ci_wfn.py:420-574 and
-:51-167 (spin adaptation is exact for the closed-shell RHF reference; the
-energies agree at convergence, as the reference's tests pin both forms to
-the same golden numbers).
+the sigma of apy_cisd_so_step; the spin-adapted (spatial) solvers run the
+same iteration over spatial amplitudes with the residual of
+apy_cisd_sp_step (ci_wfn.py:420-574 and :51-167), so both follow the
+reference iterate for iterate.
 """
 import numpy as np
 
 from . import transforms as tr
 from .so_ci import CID_BLOCKS, SOCI, SOIntegrals
+from .spatial_ci import SPIntegrals, SpatialCI
 from .utils import compute_ERI_MO_device, compute_F_MO_device, device, get_slices
 
 
@@ -67,24 +65,27 @@ class ci_wfn(object):
         E, _, t2, _ = s.solve(self.parameters, print_level, e_offset=self._e_offset())
         return np.complex128(E), t2.cpu().numpy()
 
+    def _spatial_solver(self, singles):
+        ints = SPIntegrals.from_mo(self._ERI_MO, len(self.eps_o), singles=singles)
+        eps = np.concatenate([self.eps_o, self.eps_v])
+        return SpatialCI(ints, self._F_MO, eps, singles=singles)
+
     def solve_CISD(self, print_level=0):
-        """ci_wfn.py:420-574 (closed-shell amplitudes from the spin-orbital
-        solution).  Returns (E_CISD, t1, t2)."""
-        s = self._so_solver(True)
+        """ci_wfn.py:420-574.  Returns (E_CISD, t1, t2)."""
+        s = self._spatial_solver(True)
         E, t1, t2, _ = s.solve(self.parameters, print_level, e_offset=self._e_offset())
-        t1s = t1[0::2, 0::2].cpu().numpy()
-        t2s = t2[0::2, 1::2, 0::2, 1::2].cpu().numpy()
+        t1s = t1.cpu().numpy()
+        t2s = t2.cpu().numpy()
         print("t-Amplitude Data:")
         print("Maximum t1: ", np.max(t1s))
         print("Maximum t2: ", np.max(t2s))
         return np.complex128(E), t1s, t2s
 
     def solve_CID(self, print_level=0):
-        """ci_wfn.py:51-167 (closed-shell amplitudes from the spin-orbital
-        solution).  Returns (E_CID, t2)."""
-        s = self._so_solver(False)
+        """ci_wfn.py:51-167.  Returns (E_CID, t2)."""
+        s = self._spatial_solver(False)
         E, _, t2, _ = s.solve(self.parameters, print_level, e_offset=self._e_offset())
-        t2s = t2[0::2, 1::2, 0::2, 1::2].cpu().numpy()
+        t2s = t2.cpu().numpy()
         print("t-Amplitude Data:")
         print("Maximum t2: ", np.max(t2s))
         return np.complex128(E), t2s

@@ -1,0 +1,466 @@
+// Spin-adapted closed-shell CISD / CID (the reference's solve_CISD and
+// solve_CID, ci_wfn.py:420-574 and :51-167) on spatial orbitals: o occupied
+// and v virtual spatial orbitals, Dirac integrals <pq|rs> = (pr|qs) of the
+// complex MO tensor, amplitudes t1[i][a] and t2[i][j][a][b] with the pair
+// symmetry t2[i][j][a][b] = t2[j][i][b][a].
+//
+// The doubles residual of ci_wfn.py:466-482 is split into one half R and
+// its pair image, R_ijab + R_jiba, wherever the reference writes both:
+//   <ab|cd> t_ijcd (PPL) + <kl|ij> t_klab - F_ki t_kjab - F_kj t_ikab (HHL,
+//   both folded into one pair operator) are symmetric already;
+//   F_bc t_ijac (Y) and its image F_ac t_ijcb;
+//   the six ring einsums: with T[(i,a),(k,c)] = t_ikac, U[(i,a),(k,c)] = t_ikca,
+//     X1 = T W2 - U W1,  W2[(k,c),(j,b)] = 2<kb|cj> - <kb|jc>, W1 = <kb|cj>,
+//     X2 = U W3,         W3[(k,c),(j,a)] = <ka|jc>,
+//     r += X1[(ia),(jb)] + X1[(jb),(ia)] - X2[(ib),(ja)] - X2[(ja),(ib)];
+//   singles into doubles: Q1[i][(a,b,j)] = sum_c <ab|cj> t_ic,
+//     Q2[(a,i,j)][b] = sum_k <ak|ij> t_kb,  r += Q1 - Q2 + images.
+// Each product is one complex GEMM on the FP64 matrix cores (zgemm.hip);
+// the PPL (o^2 x v^2 x v^2) dominates.  The singles residual (ci_wfn.py:
+// 457-464) is six GEMMs into one accumulator; the energy (:495) one
+// deterministic reduction.  The CID residual of ci_wfn.py:83-93 (halved
+// terms, then r + r^T over the pair) is the doubles part of the same
+// expression with t1 = 0.
+#include "common.h"
+
+#include <algorithm>
+
+namespace {
+
+enum SpKind { S_VVVV = 0, S_OOOO, S_W21, S_W3, S_VVOO, S_L, S_VVVO, S_VOOO, S_S1, S_W4, S_W5, S_COUNT };
+
+__host__ __device__ inline int64_t sp_size(int kind, int64_t o, int64_t v) {
+  const int64_t ov = o * v;
+  switch (kind) {
+    case S_VVVV: return v * v * v * v;
+    case S_OOOO: return o * o * o * o;
+    case S_W21: return 2 * ov * ov;
+    case S_W3: return ov * ov;
+    case S_VVOO: return ov * ov;
+    case S_L: return ov * ov;
+    case S_VVVO: return v * v * o * v;
+    case S_VOOO: return v * o * o * o;
+    case S_S1: return ov * ov;
+    case S_W4: return v * o * v * v;
+    case S_W5: return o * o * o * v;
+  }
+  return 0;
+}
+
+void* sp_ptr(const apy_sp_ints* I, int kind) {
+  switch (kind) {
+    case S_VVVV: return I->vvvv;
+    case S_OOOO: return I->oooo;
+    case S_W21: return I->w21;
+    case S_W3: return I->w3;
+    case S_VVOO: return I->vvoo;
+    case S_L: return I->l;
+    case S_VVVO: return I->vvvo;
+    case S_VOOO: return I->vooo;
+    case S_S1: return I->s1;
+    case S_W4: return I->w4;
+    case S_W5: return I->w5;
+  }
+  return nullptr;
+}
+
+// <pq|rs> = (pr|qs) of the chemists' tensor E[m][m][m][m]
+struct Dirac {
+  const double2* E;
+  int64_t m;
+  __device__ double2 operator()(int64_t p, int64_t q, int64_t r, int64_t s) const {
+    return E[((p * m + r) * m + q) * m + s];
+  }
+};
+
+__device__ inline double2 d2lin(double a, double2 x, double b, double2 y) {
+  return make_double2(a * x.x + b * y.x, a * x.y + b * y.y);
+}
+
+__global__ __launch_bounds__(256) void sp_build_kernel(Dirac g, int kind, int64_t o, int64_t v, double2* out,
+                                                       int64_t total) {
+  const int64_t ov = o * v;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    double2 val;
+    switch (kind) {
+      case S_VVVV: {  // [(a,b)][(c,d)] = <ab|cd>
+        const int64_t d = t % v, c = (t / v) % v, b = (t / (v * v)) % v, a = t / (v * v * v);
+        val = g(o + a, o + b, o + c, o + d);
+        break;
+      }
+      case S_OOOO: {  // [(i,j)][(k,l)] = <kl|ij>
+        const int64_t l = t % o, k = (t / o) % o, j = (t / (o * o)) % o, i = t / (o * o * o);
+        val = g(k, l, i, j);
+        break;
+      }
+      case S_W21: {  // rows (k,c): 2<kb|cj> - <kb|jc>; rows ov + (k,c): -<kb|cj>; columns (j,b)
+        const int64_t row = t / ov, col = t % ov;
+        const int64_t j = col / v, b = col % v;
+        const int64_t r = row % ov, k = r / v, c = r % v;
+        const double2 x = g(k, o + b, o + c, j);
+        val = row < ov ? d2lin(2.0, x, -1.0, g(k, o + b, j, o + c)) : make_double2(-x.x, -x.y);
+        break;
+      }
+      case S_W3: {  // [(k,c)][(j,a)] = <ka|jc>
+        const int64_t k = t / (v * ov), c = (t / ov) % v, j = (t / v) % o, a = t % v;
+        val = g(k, o + a, j, o + c);
+        break;
+      }
+      case S_VVOO: {  // [(i,j)][(a,b)] = <ab|ij>
+        const int64_t b = t % v, a = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
+        val = g(o + a, o + b, i, j);
+        break;
+      }
+      case S_L: {  // [(i,j)][(a,b)] = 2<ij|ab> - <ij|ba>
+        const int64_t b = t % v, a = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
+        val = d2lin(2.0, g(i, j, o + a, o + b), -1.0, g(i, j, o + b, o + a));
+        break;
+      }
+      case S_VVVO: {  // [(a,b,j)][c] = <ab|cj>
+        const int64_t c = t % v, j = (t / v) % o, b = (t / ov) % v, a = t / (ov * v);
+        val = g(o + a, o + b, o + c, j);
+        break;
+      }
+      case S_VOOO: {  // [(a,i,j)][k] = <ak|ij>
+        const int64_t k = t % o, j = (t / o) % o, i = (t / (o * o)) % o, a = t / (o * o * o);
+        val = g(o + a, k, i, j);
+        break;
+      }
+      case S_S1: {  // [(i,a)][(j,b)] = 2<ja|bi> - <ja|ib>
+        const int64_t row = t / ov, col = t % ov;
+        const int64_t i = row / v, a = row % v, j = col / v, b = col % v;
+        val = d2lin(2.0, g(j, o + a, o + b, i), -1.0, g(j, o + a, i, o + b));
+        break;
+      }
+      case S_W4: {  // [a][(j,b,c)] = 2<aj|bc> - <aj|cb>
+        const int64_t c = t % v, b = (t / v) % v, j = (t / (v * v)) % o, a = t / (v * v * o);
+        val = d2lin(2.0, g(o + a, j, o + b, o + c), -1.0, g(o + a, j, o + c, o + b));
+        break;
+      }
+      default: {  // S_W5 [i][(j,k,b)] = 2<kj|ib> - <kj|bi>
+        const int64_t b = t % v, k = (t / v) % o, j = (t / ov) % o, i = t / (ov * o);
+        val = d2lin(2.0, g(k, j, i, o + b), -1.0, g(k, j, o + b, i));
+        break;
+      }
+    }
+    out[t] = val;
+  }
+}
+
+// Hoo[(i,j)][(k,l)] = <kl|ij> - F[k][i] d(l,j) - F[l][j] d(k,i);  g = [2 F_ov | -F_ov]
+__global__ __launch_bounds__(256) void sp_prepare_kernel(int64_t o, int64_t v, const double2* __restrict__ F,
+                                                         const double2* __restrict__ oooo, double2* __restrict__ hoo,
+                                                         double2* __restrict__ gvec) {
+  const int64_t n = o + v, o4 = o * o * o * o, ov = o * v;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < o4 + 2 * ov;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (t < o4) {
+      const int64_t l = t % o, k = (t / o) % o, j = (t / (o * o)) % o, i = t / (o * o * o);
+      double2 h = oooo[t];
+      if (l == j) { const double2 f = F[k * n + i]; h.x -= f.x; h.y -= f.y; }
+      if (k == i) { const double2 f = F[l * n + j]; h.x -= f.x; h.y -= f.y; }
+      hoo[t] = h;
+    } else {
+      const int64_t q = t - o4, e = q % ov, i = e / v, a = e % v;
+      const double2 f = F[i * n + o + a];
+      gvec[q] = q < ov ? make_double2(2.0 * f.x, 2.0 * f.y) : make_double2(-f.x, -f.y);
+    }
+  }
+}
+
+// TU[(i,a)][(k,c)] = t_ikac, TU[(i,a)][ov + (k,c)] = t_ikca
+__global__ __launch_bounds__(256) void sp_tu_kernel(int64_t o, int64_t v, const double2* __restrict__ t2,
+                                                    double2* __restrict__ TU) {
+  const int64_t ov = o * v, total = 2 * ov * ov;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = t / (2 * ov), col = t % (2 * ov);
+    const int64_t i = row / v, a = row % v;
+    const int64_t cc = col % ov, k = cc / v, c = cc % v;
+    TU[t] = col < ov ? t2[((i * o + k) * v + a) * v + c] : t2[((i * o + k) * v + c) * v + a];
+  }
+}
+
+struct SpEpi {
+  int64_t o, v;
+  const double* eps;
+  const double2* vvoo;
+  const double2* R;
+  const double2* Y;
+  const double2* X1;
+  const double2* X2;
+  const double2* Q1;
+  const double2* Q2;
+  const double2* t2;
+  const double2* energy;
+  double2* t2_new;
+  double2* r2;
+};
+
+__global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
+  const int64_t o = e.o, v = e.v, ov = o * v, total = ov * ov;
+  const double2 E = *e.energy;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t % v, a = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
+    double2 r = e.vvoo[t];
+    const double2 p = e.R[t];
+    r.x += p.x; r.y += p.y;
+    const double2 y1 = e.Y[t], y2 = e.Y[((j * o + i) * v + b) * v + a];
+    r.x += y1.x + y2.x; r.y += y1.y + y2.y;
+    const double2 x1 = e.X1[(i * v + a) * ov + j * v + b], x1t = e.X1[(j * v + b) * ov + i * v + a];
+    const double2 x2 = e.X2[(i * v + b) * ov + j * v + a], x2t = e.X2[(j * v + a) * ov + i * v + b];
+    r.x += x1.x + x1t.x - x2.x - x2t.x;
+    r.y += x1.y + x1t.y - x2.y - x2t.y;
+    if (e.Q1) {
+      const double2 q1 = e.Q1[i * (v * ov) + (a * v + b) * o + j], q1t = e.Q1[j * (v * ov) + (b * v + a) * o + i];
+      const double2 q2 = e.Q2[((a * o + i) * o + j) * v + b], q2t = e.Q2[((b * o + j) * o + i) * v + a];
+      r.x += q1.x + q1t.x - q2.x - q2t.x;
+      r.y += q1.y + q1t.y - q2.y - q2t.y;
+    }
+    const double2 x = e.t2[t];
+    r.x -= E.x * x.x - E.y * x.y;
+    r.y -= E.x * x.y + E.y * x.x;
+    const double D = e.eps[i] + e.eps[j] - e.eps[o + a] - e.eps[o + b];
+    e.r2[t] = r;
+    e.t2_new[t] = make_double2(x.x + r.x / D, x.y + r.y / D);
+  }
+}
+
+// r1 = F[a][i] + acc - E t1 (ci_wfn.py:457-464); t1_new = t1 + r1 / D_ia
+__global__ __launch_bounds__(256) void sp_epi_singles(int64_t o, int64_t v, const double* __restrict__ eps,
+                                                      const double2* __restrict__ F, const double2* __restrict__ acc,
+                                                      const double2* __restrict__ t1, const double2* __restrict__ energy,
+                                                      double2* __restrict__ t1_new, double2* __restrict__ r1) {
+  const int64_t n = o + v, ov = o * v;
+  const double2 E = *energy;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ov; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / v, a = t % v;
+    const double2 f = F[(o + a) * n + i], s = acc[t], x = t1[t];
+    double2 r = make_double2(f.x + s.x - (E.x * x.x - E.y * x.y), f.y + s.y - (E.x * x.y + E.y * x.x));
+    const double D = eps[i] - eps[o + a];
+    r1[t] = r;
+    t1_new[t] = make_double2(x.x + r.x / D, x.y + r.y / D);
+  }
+}
+
+__global__ __launch_bounds__(256) void sp_zero_kernel(int64_t n, double2* __restrict__ a, double2* __restrict__ b) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    a[t] = make_double2(0.0, 0.0);
+    b[t] = make_double2(0.0, 0.0);
+  }
+}
+
+constexpr int EN_BLOCKS = 256;
+
+// E = sum g[0:ov] t1 + sum L t2 (no conjugation; g[0:ov] = 2 F_ov)
+__global__ __launch_bounds__(256) void sp_energy_partial(int64_t ov, int singles, const double2* __restrict__ gvec,
+                                                         const double2* __restrict__ L, const double2* __restrict__ t,
+                                                         double2* __restrict__ partial) {
+  const int64_t n2 = ov * ov;
+  double sr = 0.0, si = 0.0;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ov + n2; q += (int64_t)gridDim.x * blockDim.x) {
+    double2 w, x = t[q];
+    if (q < ov) {
+      if (!singles) continue;
+      w = gvec[q];
+    } else {
+      w = L[q - ov];
+    }
+    sr += w.x * x.x - w.y * x.y;
+    si += w.x * x.y + w.y * x.x;
+  }
+  __shared__ double hr[256], hi[256];
+  hr[threadIdx.x] = sr;
+  hi[threadIdx.x] = si;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { hr[threadIdx.x] += hr[threadIdx.x + w]; hi[threadIdx.x] += hi[threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(hr[0], hi[0]);
+}
+
+__global__ void sp_energy_final(const double2* __restrict__ partial, int np, double2* out) {
+  __shared__ double hr[256], hi[256];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < np; q += blockDim.x) { a += partial[q].x; b += partial[q].y; }
+  hr[threadIdx.x] = a;
+  hi[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { hr[threadIdx.x] += hr[threadIdx.x + w]; hi[threadIdx.x] += hi[threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = make_double2(hr[0], hi[0]);
+}
+
+apy_gemm_desc gd(int64_t m, int64_t nn, int64_t k, int al, const void* a, int64_t lda, int bl, const void* b,
+                 int64_t ldb, void* c, int64_t ldc, double alpha, double beta) {
+  apy_gemm_desc d{};
+  d.m = m; d.n = nn; d.k = k;
+  d.a_layout = al; d.b_layout = bl;
+  d.alpha_re = alpha; d.beta_re = beta;
+  d.a = a; d.lda = lda; d.b = b; d.ldb = ldb; d.c = c; d.ldc = ldc;
+  d.batch = 1;
+  return d;
+}
+
+struct SpWS {
+  size_t R, Y, X1, X2, TU, Q1, Q2, acc, gemm, total;
+};
+
+constexpr int NSPG = 13;
+
+// The GEMMs of one step in order (pointers null when only sizing).
+int sp_gemms(int64_t o, int64_t v, int singles, const apy_cisd_sp_args* A, char* base, const SpWS* w,
+             apy_gemm_desc g[NSPG]) {
+  const int64_t n = o + v, ov = o * v, oo = o * o, vv = v * v;
+  auto P = [&](size_t off) -> double2* { return base ? (double2*)(base + off) : nullptr; };
+  const double2* t1 = A ? (const double2*)A->t : nullptr;
+  const double2* t2 = t1 ? t1 + ov : nullptr;
+  const double2* F = A ? (const double2*)A->fock : nullptr;
+  const double2* Fvv = F ? F + o * n + o : nullptr;
+  const apy_sp_ints* I = A ? A->ints : nullptr;
+  double2* R = P(w->R);
+  double2* TU = P(w->TU);
+  double2* acc = P(w->acc);
+  int q = 0;
+  g[q++] = gd(oo, vv, vv, APY_MK, t2, vv, APY_NK, I ? I->vvvv : nullptr, vv, R, vv, 1.0, 0.0);         // PPL
+  g[q++] = gd(oo, vv, oo, APY_MK, A ? A->hoo : nullptr, oo, APY_KN, t2, vv, R, vv, 1.0, 1.0);          // HHL + F_oo
+  g[q++] = gd(oo * v, v, v, APY_MK, t2, v, APY_NK, Fvv, n, P(w->Y), v, 1.0, 0.0);                      // F_vv
+  g[q++] = gd(ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, I ? I->w21 : nullptr, ov, P(w->X1), ov, 1.0, 0.0);
+  g[q++] = gd(ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2), ov,
+              1.0, 0.0);
+  if (singles) {
+    g[q++] = gd(o, vv * o, v, APY_MK, t1, v, APY_NK, I ? I->vvvo : nullptr, v, P(w->Q1), vv * o, 1.0, 0.0);
+    g[q++] = gd(v * oo, v, o, APY_MK, I ? I->vooo : nullptr, o, APY_KN, t1, v, P(w->Q2), v, 1.0, 0.0);
+    g[q++] = gd(ov, 1, ov, APY_MK, I ? I->s1 : nullptr, ov, APY_KN, t1, 1, acc, 1, 1.0, 0.0);
+    g[q++] = gd(ov, 1, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, A ? A->g : nullptr, 1, acc, 1, 1.0, 1.0);
+    g[q++] = gd(o, v, ov * v, APY_MK, t2, ov * v, APY_NK, I ? I->w4 : nullptr, ov * v, acc, v, 1.0, 1.0);
+    g[q++] = gd(o, v, oo * v, APY_MK, I ? I->w5 : nullptr, oo * v, APY_KN, t2, v, acc, v, -1.0, 1.0);
+    g[q++] = gd(o, v, o, APY_KM, F, n, APY_KN, t1, v, acc, v, -1.0, 1.0);                              // -F_oo^T t1
+    g[q++] = gd(o, v, v, APY_MK, t1, v, APY_NK, Fvv, n, acc, v, 1.0, 1.0);                             // t1 F_vv^T
+  }
+  return q;
+}
+
+SpWS sp_carve(int64_t o, int64_t v, int singles) {
+  const size_t z = 16, ov = (size_t)(o * v), d2 = ov * ov * z;
+  auto rnd = [](size_t b) { return (b + 255) / 256 * 256; };
+  SpWS w{};
+  size_t off = 0;
+  w.R = off; off += rnd(d2);
+  w.Y = off; off += rnd(d2);
+  w.X1 = off; off += rnd(d2);
+  w.X2 = off; off += rnd(d2);
+  w.TU = off; off += rnd(2 * d2);
+  if (singles) {
+    w.Q1 = off; off += rnd(d2);
+    w.Q2 = off; off += rnd(d2);
+    w.acc = off; off += rnd(ov * z);
+  }
+  w.gemm = off;
+  apy_gemm_desc g[NSPG];
+  const int q = sp_gemms(o, v, singles, nullptr, nullptr, &w, g);
+  size_t m = 0;
+  for (int k = 0; k < q; ++k) m = std::max(m, apy_zgemm_ws_bytes(&g[k]));
+  w.total = off + std::max(m, (size_t)EN_BLOCKS * z);
+  return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[11]) {
+  if (o < 1 || v < 1 || !sizes_out) return APY_E_ARG;
+  for (int k = 0; k < S_COUNT; ++k) sizes_out[k] = sp_size(k, o, v);
+  return APY_OK;
+}
+
+int apy_sp_ints_from_mo(int64_t m, const void* eri_mo, const apy_sp_ints* ints, void* stream) {
+  if (!eri_mo || !ints || ints->o < 1 || ints->v < 1 || ints->o + ints->v != m) return APY_E_ARG;
+  Dirac g{(const double2*)eri_mo, m};
+  for (int kind = 0; kind < S_COUNT; ++kind) {
+    double2* out = (double2*)sp_ptr(ints, kind);
+    if (!out) continue;
+    const int64_t total = sp_size(kind, ints->o, ints->v);
+    sp_build_kernel<<<apy_grid1d(total, 1 << 20), 256, 0, (hipStream_t)stream>>>(g, kind, ints->o, ints->v, out,
+                                                                                 total);
+    APY_LAUNCH_CHECK();
+  }
+  return APY_OK;
+}
+
+int apy_cisd_sp_prepare(int64_t o, int64_t v, const void* fock, const apy_sp_ints* ints, void* hoo, void* g,
+                        void* stream) {
+  if (o < 1 || v < 1 || !fock || !ints || !ints->oooo || !hoo || !g || ints->o != o || ints->v != v)
+    return APY_E_ARG;
+  const int64_t total = o * o * o * o + 2 * o * v;
+  sp_prepare_kernel<<<apy_grid1d(total), 256, 0, (hipStream_t)stream>>>(
+      o, v, (const double2*)fock, (const double2*)ints->oooo, (double2*)hoo, (double2*)g);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+size_t apy_cisd_sp_workspace(int64_t o, int64_t v, int singles) {
+  if (o < 1 || v < 1) return 0;
+  return sp_carve(o, v, singles ? 1 : 0).total;
+}
+
+int apy_cisd_sp_step(const apy_cisd_sp_args* A, void* workspace, size_t ws_bytes, void* stream) {
+  if (!A || !A->ints || !A->fock || !A->eps || !A->hoo || !A->g || !A->t || !A->t_new || !A->out || !A->energy)
+    return APY_E_ARG;
+  const int64_t o = A->o, v = A->v;
+  const apy_sp_ints* I = A->ints;
+  if (o < 1 || v < 1 || I->o != o || I->v != v) return APY_E_ARG;
+  if (A->out == A->t || A->t_new == A->out) return APY_E_ARG;
+  const int singles = A->singles ? 1 : 0;
+  if (!I->vvvv || !I->w21 || !I->w3 || !I->vvoo) return APY_E_ARG;
+  if (singles && (!I->vvvo || !I->vooo || !I->s1 || !I->w4 || !I->w5)) return APY_E_ARG;
+  const SpWS w = sp_carve(o, v, singles);
+  if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  const int64_t ov = o * v;
+  const double2* t = (const double2*)A->t;
+  sp_tu_kernel<<<apy_grid1d(2 * ov * ov), 256, 0, st>>>(o, v, t + ov, (double2*)(base + w.TU));
+  APY_LAUNCH_CHECK();
+  apy_gemm_desc g[NSPG];
+  const int q = sp_gemms(o, v, singles, A, base, &w, g);
+  for (int k = 0; k < q; ++k) {
+    const int s = apy_zgemm_launch(&g[k], base + w.gemm, ws_bytes - w.gemm, st);
+    if (s != APY_OK) return s;
+  }
+  SpEpi e{o, v, A->eps, (const double2*)I->vvoo, (const double2*)(base + w.R), (const double2*)(base + w.Y),
+          (const double2*)(base + w.X1), (const double2*)(base + w.X2),
+          singles ? (const double2*)(base + w.Q1) : nullptr, singles ? (const double2*)(base + w.Q2) : nullptr,
+          t + ov, (const double2*)A->energy, (double2*)A->t_new + ov, (double2*)A->out + ov};
+  sp_epi_doubles<<<apy_grid1d(ov * ov), 256, 0, st>>>(e);
+  APY_LAUNCH_CHECK();
+  if (singles) {
+    sp_epi_singles<<<apy_grid1d(ov), 256, 0, st>>>(o, v, A->eps, (const double2*)A->fock,
+                                                   (const double2*)(base + w.acc), t, (const double2*)A->energy,
+                                                   (double2*)A->t_new, (double2*)A->out);
+  } else {  // CID keeps the singles block of the vector at zero
+    sp_zero_kernel<<<apy_grid1d(ov), 256, 0, st>>>(ov, (double2*)A->t_new, (double2*)A->out);
+  }
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+size_t apy_cisd_sp_energy_workspace(void) { return (size_t)EN_BLOCKS * 16; }
+
+int apy_cisd_sp_energy(int64_t o, int64_t v, int singles, const void* g, const void* L, const void* t, void* energy,
+                       void* workspace, size_t ws_bytes, void* stream) {
+  if (o < 1 || v < 1 || !L || !t || !energy || (singles && !g)) return APY_E_ARG;
+  if (!workspace || ws_bytes < apy_cisd_sp_energy_workspace()) return APY_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  sp_energy_partial<<<EN_BLOCKS, 256, 0, st>>>(o * v, singles ? 1 : 0, (const double2*)g, (const double2*)L,
+                                               (const double2*)t, (double2*)workspace);
+  APY_LAUNCH_CHECK();
+  sp_energy_final<<<1, 256, 0, st>>>((const double2*)workspace, EN_BLOCKS, (double2*)energy);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+}  // extern "C"

@@ -142,6 +142,63 @@ int apy_synth_so_ints(uint64_t seed, uint64_t tag, double std, int accumulate, d
 int apy_so_fock_from_mo(int64_t m, const void* f_mo, void* f_so, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Spin-adapted closed-shell CISD / CID (ci_wfn.py:420-574 and :51-167) on
+ * spatial orbitals: o occupied, v virtual; Dirac integrals <pq|rs> =
+ * (pr|qs) of the chemists' MO tensor (utils.py:258-279, ci_wfn.py:433).
+ * Amplitude vector [t1 (o*v) | t2 (o*o*v*v)], t2[i][j][a][b] with
+ * t2[i][j][a][b] = t2[j][i][b][a] (the solver keeps it so; the doubles
+ * residual is assembled as a half and its pair image).
+ * ---------------------------------------------------------------------- */
+typedef struct apy_sp_ints {
+  int64_t o, v;
+  void* vvvv;  /* [(a,b)][(c,d)]      <ab|cd>                             (ci_wfn.py:475) */
+  void* oooo;  /* [(i,j)][(k,l)]      <kl|ij>                             (ci_wfn.py:474) */
+  void* w21;   /* [2ov][(j,b)]  rows (k,c): 2<kb|cj> - <kb|jc>; rows ov+(k,c): -<kb|cj>  (:476-481) */
+  void* w3;    /* [(k,c)][(j,a)]      <ka|jc>                             (ci_wfn.py:478, 482) */
+  void* vvoo;  /* [(i,j)][(a,b)]      <ab|ij>  (doubles driver)            (ci_wfn.py:466) */
+  void* l;     /* [(i,j)][(a,b)]      2<ij|ab> - <ij|ba>  (energy)          (ci_wfn.py:495) */
+  void* vvvo;  /* [(a,b,j)][c]        <ab|cj>                             (ci_wfn.py:467-468) */
+  void* vooo;  /* [(a,i,j)][k]        <ak|ij>                             (ci_wfn.py:469-470) */
+  void* s1;    /* [(i,a)][(j,b)]      2<ja|bi> - <ja|ib>                  (ci_wfn.py:460) */
+  void* w4;    /* [a][(j,b,c)]        2<aj|bc> - <aj|cb>                  (ci_wfn.py:462) */
+  void* w5;    /* [i][(j,k,b)]        2<kj|ib> - <kj|bi>                  (ci_wfn.py:463) */
+} apy_sp_ints;
+
+/* Complex elements of each block, in struct order.  Host. */
+int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[11]);
+/* Blocks from eri_mo[m][m][m][m] = (pq|rs), m = o + v (NULL blocks skipped). */
+int apy_sp_ints_from_mo(int64_t m, const void* eri_mo, const apy_sp_ints* ints, void* stream);
+/* Per-solve operands that depend on the Fock matrix fock[(o+v)][(o+v)]:
+ * hoo[(i,j)][(k,l)] = <kl|ij> - F[k][i] d(l,j) - F[l][j] d(k,i) (o^4) and
+ * g = [2 F_ov | -F_ov] (2 o v). */
+int apy_cisd_sp_prepare(int64_t o, int64_t v, const void* fock, const apy_sp_ints* ints, void* hoo, void* g,
+                        void* stream);
+
+typedef struct apy_cisd_sp_args {
+  int64_t o, v;
+  const void* fock;         /* [n][n] MO Fock matrix, n = o + v           */
+  const double* eps;        /* [n] orbital energies for D_ia, D_ijab       */
+  const apy_sp_ints* ints;
+  const void* hoo;          /* from apy_cisd_sp_prepare                    */
+  const void* g;            /* from apy_cisd_sp_prepare                    */
+  int singles;              /* 1 CISD, 0 CID (t1 block kept at zero)       */
+  const void* energy;       /* device complex scalar E of the iteration    */
+  const void* t;            /* [o*v + o*o*v*v] amplitudes                  */
+  void* t_new;              /* t + r / D (must differ from out)            */
+  void* out;                /* residual r (must differ from t)             */
+} apy_cisd_sp_args;
+
+/* One Jacobi step of ci_wfn.py:457-486 (CID :83-96):  r = sigma(t) - E t,
+ * t_new = t + r / D. */
+size_t apy_cisd_sp_workspace(int64_t o, int64_t v, int singles);
+int apy_cisd_sp_step(const apy_cisd_sp_args* a, void* workspace, size_t ws_bytes, void* stream);
+/* E = sum 2 F_ov t1 + sum (2<ij|ab> - <ij|ba>) t2 (ci_wfn.py:495, CID :112);
+ * deterministic two-pass sum into the device scalar `energy`. */
+size_t apy_cisd_sp_energy_workspace(void);
+int apy_cisd_sp_energy(int64_t o, int64_t v, int singles, const void* g, const void* L, const void* t, void* energy,
+                       void* workspace, size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
  * CISD / CID in spin orbitals: one iteration of the reference's Jacobi
  * step, or the plain sigma product (for Davidson).
  *   residual mode (ci_wfn.py:309-337, CID :210-221):

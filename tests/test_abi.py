@@ -61,3 +61,41 @@ def test_struct_layouts_match_header_sizes():
     # int64 x3 + int x4 + double x4 + (ptr,int64,int64) x3 + int64 + int x3 (padded)
     assert ctypes.sizeof(_lib.GemmDesc) == 8 * 3 + 4 * 4 + 8 * 4 + 24 * 3 + 8 + 4 * 3 + 4
     assert ctypes.sizeof(_lib.SoInts) == 8 * 2 + 8 * 9
+
+
+def test_spatial_sizes_and_struct_layouts():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    o, v = 5, 9
+    ov = o * v
+    sizes = (ctypes.c_int64 * 11)()
+    assert lib.apy_sp_ints_sizes(o, v, sizes) == 0
+    assert list(sizes) == [v ** 4, o ** 4, 2 * ov * ov, ov * ov, ov * ov, ov * ov, v * v * o * v, v * o ** 3,
+                           ov * ov, v * o * v * v, o ** 3 * v]
+    assert lib.apy_sp_ints_sizes(0, v, sizes) == _lib.APY_E_ARG
+    assert ctypes.sizeof(_lib.SpInts) == 8 * 2 + 8 * 11
+    # o, v, 5 pointers, int (padded), 4 pointers
+    assert ctypes.sizeof(_lib.CisdSpArgs) == 8 * 2 + 8 * 5 + 8 + 8 * 4
+
+
+def test_argument_errors_return_before_any_device_work():
+    """Null or inconsistent arguments come back as APY_E_ARG (or
+    APY_E_WORKSPACE) from the host-side checks, so these calls are safe
+    without a GPU."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    E = _lib.APY_E_ARG
+    assert lib.apy_cisd_sp_step(None, None, 0, None) == E
+    assert lib.apy_cisd_so_step(None, None, 0, None) == E
+    assert lib.apy_zgemm(None, None, 0, None) == E
+    assert lib.apy_sp_ints_from_mo(10, None, None, None) == E
+    assert lib.apy_cisd_so_tpd(2, 3, None, None, None, 0, None) == E
+    assert lib.apy_ci_dots(2, 3, 0, None, None, None, None, 0, None) == E
+    assert lib.apy_ci_dots(2, 3, _lib.APY_MAXVEC + 1, None, None, None, None, 0, None) == E
+    assert lib.apy_cisd_sp_energy(0, 3, 1, None, None, None, None, None, 0, None) == E
+    # a spatial args struct whose integrals disagree with (o, v)
+    ints = _lib.SpInts(3, 4)
+    args = _lib.CisdSpArgs(3, 5, 1, 1, ctypes.pointer(ints), 1, 1, 1, 1, 1, 2, 3)
+    assert lib.apy_cisd_sp_step(ctypes.byref(args), None, 0, None) == E
