@@ -60,7 +60,7 @@ class SoInts(ctypes.Structure):
 
 SO_BLOCKS = ("vvvv_p", "ring", "oooo_p", "ovoo_p", "vvvo_p", "vovv", "ooov", "vvoo_t", "oovv")
 
-SP_BLOCKS = ("vvvv", "oooo", "w21", "w3", "vvoo", "l", "vvvo", "vooo", "s1", "w4", "w5")
+SP_BLOCKS = ("vvvs", "vvva", "oooo", "w21", "w3", "vvoo", "l", "vvvo", "vooo", "s1", "w4", "w5")
 
 
 class SpInts(ctypes.Structure):

@@ -6,8 +6,14 @@
 //
 // The doubles residual of ci_wfn.py:466-482 is split into one half R and
 // its pair image, R_ijab + R_jiba, wherever the reference writes both:
-//   <ab|cd> t_ijcd (PPL) + <kl|ij> t_klab - F_ki t_kjab - F_kj t_ikab (HHL,
-//   both folded into one pair operator) are symmetric already;
+//   <ab|cd> t_ijcd (PPL) is split into pair-symmetric and antisymmetric
+//     parts: with W+[(a<=b),(c<=d)] = <ab|cd> + <ab|dc> (c<d; <ab|cc> on the
+//     diagonal), W-[(a<b),(c<d)] = <ab|cd> - <ab|dc>, T+[(i<=j),(c<=d)] =
+//     (t_ijcd + t_ijdc)/2, T-[(i<j),(c<d)] = (t_ijcd - t_ijdc)/2,
+//     P+ = T+ W+^T and P- = T- W-^T give sum_cd <ab|cd> t_ijcd =
+//     P+[(ij)][(ab)] + s_ij s_ab P-[(i<j)][(a<b)] (s = sign of the order),
+//     two GEMMs of a quarter of the plain o^2 v^4 flops between them;
+//   <kl|ij> t_klab - F_ki t_kjab - F_kj t_ikab (HHL, one pair operator);
 //   F_bc t_ijac (Y) and its image F_ac t_ijcb;
 //   the six ring einsums: with T[(i,a),(k,c)] = t_ikac, U[(i,a),(k,c)] = t_ikca,
 //     X1 = T W2 - U W1,  W2[(k,c),(j,b)] = 2<kb|cj> - <kb|jc>, W1 = <kb|cj>,
@@ -16,7 +22,7 @@
 //   singles into doubles: Q1[i][(a,b,j)] = sum_c <ab|cj> t_ic,
 //     Q2[(a,i,j)][b] = sum_k <ak|ij> t_kb,  r += Q1 - Q2 + images.
 // Each product is one complex GEMM on the FP64 matrix cores (zgemm.hip);
-// the PPL (o^2 x v^2 x v^2) dominates.  The singles residual (ci_wfn.py:
+// the two ladder GEMMs dominate.  The singles residual (ci_wfn.py:
 // 457-464) is six GEMMs into one accumulator; the energy (:495) one
 // deterministic reduction.  The CID residual of ci_wfn.py:83-93 (halved
 // terms, then r + r^T over the pair) is the doubles part of the same
@@ -27,12 +33,19 @@
 
 namespace {
 
-enum SpKind { S_VVVV = 0, S_OOOO, S_W21, S_W3, S_VVOO, S_L, S_VVVO, S_VOOO, S_S1, S_W4, S_W5, S_COUNT };
+enum SpKind { S_VVVS = 0, S_VVVA, S_OOOO, S_W21, S_W3, S_VVOO, S_L, S_VVVO, S_VOOO, S_S1, S_W4, S_W5, S_COUNT };
+
+// pairs x <= y (symmetric) and x < y (antisymmetric) among N, row-major
+__host__ __device__ inline int64_t nsym(int64_t N) { return N * (N + 1) / 2; }
+__host__ __device__ inline int64_t nanti(int64_t N) { return N * (N - 1) / 2; }
+__host__ __device__ inline int64_t isym(int64_t x, int64_t y, int64_t N) { return x * (2 * N - x + 1) / 2 + y - x; }
+__host__ __device__ inline int64_t ianti(int64_t x, int64_t y, int64_t N) { return x * (2 * N - x - 1) / 2 + y - x - 1; }
 
 __host__ __device__ inline int64_t sp_size(int kind, int64_t o, int64_t v) {
   const int64_t ov = o * v;
   switch (kind) {
-    case S_VVVV: return v * v * v * v;
+    case S_VVVS: return nsym(v) * nsym(v);
+    case S_VVVA: return nanti(v) * nanti(v);
     case S_OOOO: return o * o * o * o;
     case S_W21: return 2 * ov * ov;
     case S_W3: return ov * ov;
@@ -49,7 +62,8 @@ __host__ __device__ inline int64_t sp_size(int kind, int64_t o, int64_t v) {
 
 void* sp_ptr(const apy_sp_ints* I, int kind) {
   switch (kind) {
-    case S_VVVV: return I->vvvv;
+    case S_VVVS: return I->vvvs;
+    case S_VVVA: return I->vvva;
     case S_OOOO: return I->oooo;
     case S_W21: return I->w21;
     case S_W3: return I->w3;
@@ -83,10 +97,9 @@ __global__ __launch_bounds__(256) void sp_build_kernel(Dirac g, int kind, int64_
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
     double2 val;
     switch (kind) {
-      case S_VVVV: {  // [(a,b)][(c,d)] = <ab|cd>
-        const int64_t d = t % v, c = (t / v) % v, b = (t / (v * v)) % v, a = t / (v * v * v);
-        val = g(o + a, o + b, o + c, o + d);
-        break;
+      case S_VVVS:
+      case S_VVVA: {  // handled by sp_ladder_kernel
+        continue;
       }
       case S_OOOO: {  // [(i,j)][(k,l)] = <kl|ij>
         const int64_t l = t % o, k = (t / o) % o, j = (t / (o * o)) % o, i = t / (o * o * o);
@@ -147,6 +160,37 @@ __global__ __launch_bounds__(256) void sp_build_kernel(Dirac g, int kind, int64_
   }
 }
 
+// W+ and W- from one pass over (a,b,c,d) in v^4 (threads outside a<=b, c<=d exit)
+__global__ __launch_bounds__(256) void sp_ladder_kernel(Dirac g, int64_t o, int64_t v, double2* __restrict__ ws,
+                                                        double2* __restrict__ wa) {
+  const int64_t total = v * v * v * v, sv = nsym(v), av = nanti(v);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = t % v, c = (t / v) % v, b = (t / (v * v)) % v, a = t / (v * v * v);
+    if (a > b || c > d) continue;
+    const double2 x = g(o + a, o + b, o + c, o + d);
+    if (c == d) {
+      if (ws) ws[isym(a, b, v) * sv + isym(c, d, v)] = x;
+      continue;
+    }
+    const double2 y = g(o + a, o + b, o + d, o + c);
+    if (ws) ws[isym(a, b, v) * sv + isym(c, d, v)] = make_double2(x.x + y.x, x.y + y.y);
+    if (wa && a < b) wa[ianti(a, b, v) * av + ianti(c, d, v)] = make_double2(x.x - y.x, x.y - y.y);
+  }
+}
+
+// T+[(i<=j)][(c<=d)] = (t_ijcd + t_ijdc)/2, T-[(i<j)][(c<d)] = (t_ijcd - t_ijdc)/2
+__global__ __launch_bounds__(256) void sp_pack_kernel(int64_t o, int64_t v, const double2* __restrict__ t2,
+                                                      double2* __restrict__ ts, double2* __restrict__ ta) {
+  const int64_t total = o * o * v * v, sv = nsym(v), av = nanti(v);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t d = t % v, c = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
+    if (i > j || c > d) continue;
+    const double2 x = t2[t], y = t2[((i * o + j) * v + d) * v + c];
+    ts[isym(i, j, o) * sv + isym(c, d, v)] = c == d ? x : make_double2(0.5 * (x.x + y.x), 0.5 * (x.y + y.y));
+    if (i < j && c < d) ta[ianti(i, j, o) * av + ianti(c, d, v)] = make_double2(0.5 * (x.x - y.x), 0.5 * (x.y - y.y));
+  }
+}
+
 // Hoo[(i,j)][(k,l)] = <kl|ij> - F[k][i] d(l,j) - F[l][j] d(k,i);  g = [2 F_ov | -F_ov]
 __global__ __launch_bounds__(256) void sp_prepare_kernel(int64_t o, int64_t v, const double2* __restrict__ F,
                                                          const double2* __restrict__ oooo, double2* __restrict__ hoo,
@@ -184,6 +228,8 @@ struct SpEpi {
   int64_t o, v;
   const double* eps;
   const double2* vvoo;
+  const double2* Ps;  // P+ [(i<=j)][(a<=b)]
+  const double2* Pa;  // P- [(i<j)][(a<b)]
   const double2* R;
   const double2* Y;
   const double2* X1;
@@ -204,6 +250,16 @@ __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
     double2 r = e.vvoo[t];
     const double2 p = e.R[t];
     r.x += p.x; r.y += p.y;
+    {  // ladder: P+ + s_ij s_ab P-
+      const int64_t i0 = i < j ? i : j, j0 = i < j ? j : i, a0 = a < b ? a : b, b0 = a < b ? b : a;
+      const double2 ps = e.Ps[isym(i0, j0, o) * nsym(v) + isym(a0, b0, v)];
+      r.x += ps.x; r.y += ps.y;
+      if (i != j && a != b) {
+        const double2 pa = e.Pa[ianti(i0, j0, o) * nanti(v) + ianti(a0, b0, v)];
+        const double sg = ((i < j) == (a < b)) ? 1.0 : -1.0;
+        r.x += sg * pa.x; r.y += sg * pa.y;
+      }
+    }
     const double2 y1 = e.Y[t], y2 = e.Y[((j * o + i) * v + b) * v + a];
     r.x += y1.x + y2.x; r.y += y1.y + y2.y;
     const double2 x1 = e.X1[(i * v + a) * ov + j * v + b], x1t = e.X1[(j * v + b) * ov + i * v + a];
@@ -305,10 +361,10 @@ apy_gemm_desc gd(int64_t m, int64_t nn, int64_t k, int al, const void* a, int64_
 }
 
 struct SpWS {
-  size_t R, Y, X1, X2, TU, Q1, Q2, acc, gemm, total;
+  size_t R, Y, X1, X2, TU, Ts, Ta, Ps, Pa, Q1, Q2, acc, gemm, total;
 };
 
-constexpr int NSPG = 13;
+constexpr int NSPG = 14;
 
 // The GEMMs of one step in order (pointers null when only sizing).
 int sp_gemms(int64_t o, int64_t v, int singles, const apy_cisd_sp_args* A, char* base, const SpWS* w,
@@ -324,8 +380,11 @@ int sp_gemms(int64_t o, int64_t v, int singles, const apy_cisd_sp_args* A, char*
   double2* TU = P(w->TU);
   double2* acc = P(w->acc);
   int q = 0;
-  g[q++] = gd(oo, vv, vv, APY_MK, t2, vv, APY_NK, I ? I->vvvv : nullptr, vv, R, vv, 1.0, 0.0);         // PPL
-  g[q++] = gd(oo, vv, oo, APY_MK, A ? A->hoo : nullptr, oo, APY_KN, t2, vv, R, vv, 1.0, 1.0);          // HHL + F_oo
+  const int64_t so = nsym(o), ao = nanti(o), sv = nsym(v), av = nanti(v);
+  g[q++] = gd(so, sv, sv, APY_MK, P(w->Ts), sv, APY_NK, I ? I->vvvs : nullptr, sv, P(w->Ps), sv, 1.0, 0.0);  // P+
+  if (ao > 0 && av > 0)
+    g[q++] = gd(ao, av, av, APY_MK, P(w->Ta), av, APY_NK, I ? I->vvva : nullptr, av, P(w->Pa), av, 1.0, 0.0);  // P-
+  g[q++] = gd(oo, vv, oo, APY_MK, A ? A->hoo : nullptr, oo, APY_KN, t2, vv, R, vv, 1.0, 0.0);          // HHL + F_oo
   g[q++] = gd(oo * v, v, v, APY_MK, t2, v, APY_NK, Fvv, n, P(w->Y), v, 1.0, 0.0);                      // F_vv
   g[q++] = gd(ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, I ? I->w21 : nullptr, ov, P(w->X1), ov, 1.0, 0.0);
   g[q++] = gd(ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2), ov,
@@ -353,6 +412,11 @@ SpWS sp_carve(int64_t o, int64_t v, int singles) {
   w.X1 = off; off += rnd(d2);
   w.X2 = off; off += rnd(d2);
   w.TU = off; off += rnd(2 * d2);
+  const size_t so = (size_t)nsym(o), ao = (size_t)nanti(o), sv = (size_t)nsym(v), av = (size_t)nanti(v);
+  w.Ts = off; off += rnd(so * sv * z);
+  w.Ta = off; off += rnd(ao * av * z + z);
+  w.Ps = off; off += rnd(so * sv * z);
+  w.Pa = off; off += rnd(ao * av * z + z);
   if (singles) {
     w.Q1 = off; off += rnd(d2);
     w.Q2 = off; off += rnd(d2);
@@ -371,7 +435,7 @@ SpWS sp_carve(int64_t o, int64_t v, int singles) {
 
 extern "C" {
 
-int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[11]) {
+int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[12]) {
   if (o < 1 || v < 1 || !sizes_out) return APY_E_ARG;
   for (int k = 0; k < S_COUNT; ++k) sizes_out[k] = sp_size(k, o, v);
   return APY_OK;
@@ -380,7 +444,13 @@ int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[11]) {
 int apy_sp_ints_from_mo(int64_t m, const void* eri_mo, const apy_sp_ints* ints, void* stream) {
   if (!eri_mo || !ints || ints->o < 1 || ints->v < 1 || ints->o + ints->v != m) return APY_E_ARG;
   Dirac g{(const double2*)eri_mo, m};
-  for (int kind = 0; kind < S_COUNT; ++kind) {
+  if (ints->vvvs || ints->vvva) {
+    const int64_t v = ints->v;
+    sp_ladder_kernel<<<apy_grid1d(v * v * v * v, 1 << 20), 256, 0, (hipStream_t)stream>>>(
+        g, ints->o, v, (double2*)ints->vvvs, (double2*)ints->vvva);
+    APY_LAUNCH_CHECK();
+  }
+  for (int kind = S_OOOO; kind < S_COUNT; ++kind) {
     double2* out = (double2*)sp_ptr(ints, kind);
     if (!out) continue;
     const int64_t total = sp_size(kind, ints->o, ints->v);
@@ -415,7 +485,7 @@ int apy_cisd_sp_step(const apy_cisd_sp_args* A, void* workspace, size_t ws_bytes
   if (o < 1 || v < 1 || I->o != o || I->v != v) return APY_E_ARG;
   if (A->out == A->t || A->t_new == A->out) return APY_E_ARG;
   const int singles = A->singles ? 1 : 0;
-  if (!I->vvvv || !I->w21 || !I->w3 || !I->vvoo) return APY_E_ARG;
+  if (!I->vvvs || (v > 1 && !I->vvva) || !I->w21 || !I->w3 || !I->vvoo) return APY_E_ARG;
   if (singles && (!I->vvvo || !I->vooo || !I->s1 || !I->w4 || !I->w5)) return APY_E_ARG;
   const SpWS w = sp_carve(o, v, singles);
   if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
@@ -425,13 +495,16 @@ int apy_cisd_sp_step(const apy_cisd_sp_args* A, void* workspace, size_t ws_bytes
   const double2* t = (const double2*)A->t;
   sp_tu_kernel<<<apy_grid1d(2 * ov * ov), 256, 0, st>>>(o, v, t + ov, (double2*)(base + w.TU));
   APY_LAUNCH_CHECK();
+  sp_pack_kernel<<<apy_grid1d(ov * ov), 256, 0, st>>>(o, v, t + ov, (double2*)(base + w.Ts), (double2*)(base + w.Ta));
+  APY_LAUNCH_CHECK();
   apy_gemm_desc g[NSPG];
   const int q = sp_gemms(o, v, singles, A, base, &w, g);
   for (int k = 0; k < q; ++k) {
     const int s = apy_zgemm_launch(&g[k], base + w.gemm, ws_bytes - w.gemm, st);
     if (s != APY_OK) return s;
   }
-  SpEpi e{o, v, A->eps, (const double2*)I->vvoo, (const double2*)(base + w.R), (const double2*)(base + w.Y),
+  SpEpi e{o, v, A->eps, (const double2*)I->vvoo, (const double2*)(base + w.Ps), (const double2*)(base + w.Pa),
+          (const double2*)(base + w.R), (const double2*)(base + w.Y),
           (const double2*)(base + w.X1), (const double2*)(base + w.X2),
           singles ? (const double2*)(base + w.Q1) : nullptr, singles ? (const double2*)(base + w.Q2) : nullptr,
           t + ov, (const double2*)A->energy, (double2*)A->t_new + ov, (double2*)A->out + ov};

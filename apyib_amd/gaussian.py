@@ -307,3 +307,96 @@ def build_dipole(basis, origin=(0.0, 0.0, 0.0)):
         for j in range(i + 1):
             D[:, i, j] = D[:, j, i] = dipole(basis[i], basis[j], origin)
     return D
+
+
+# ----------------------------------------------------------------------------------------------------------
+# First derivatives with respect to nuclear positions (the integrals psi4's MintsHelper gives to
+# analytic_aats.py:84-96: mo_oei_deriv1, mo_tei_deriv1, mo_overlap_half_deriv1)
+
+class RawGaussian:
+    """Contracted Cartesian Gaussian with the coefficients taken as given
+    (no renormalisation): the terms of a differentiated basis function."""
+
+    def __init__(self, center, lmn, exps, coefs):
+        self.center = center
+        self.lmn = tuple(lmn)
+        self.exps = exps
+        self.coefs = coefs
+
+
+def center_derivative(f, d):
+    """d f / d A_d for f centred at A: sum_p c_p [2 a_p x_A^(l+1) - l x_A^(l-1)]
+    e^(-a_p r_A^2) along axis d, as a list of RawGaussian terms."""
+    up = list(f.lmn)
+    up[d] += 1
+    out = [RawGaussian(f.center, up, f.exps, 2.0 * f.exps * f.coefs)]
+    if f.lmn[d] > 0:
+        dn = list(f.lmn)
+        dn[d] -= 1
+        out.append(RawGaussian(f.center, dn, f.exps, -float(f.lmn[d]) * f.coefs))
+    return out
+
+
+def _dsum(fn, terms, *rest):
+    return sum(fn(t, *rest) for t in terms)
+
+
+def build_deriv1(basis, atom_of, charges):
+    """First-derivative AO integrals for every atom A and axis x:
+        S[A][x], T[A][x], V[A][x]  d/dR_Ax of S, T, V (V with the derivative
+                                   of the nuclear attraction operator of A)
+        half_S[A][x][m, n] = <d m/dR_Ax | n>  (bra functions only)
+        ERI[A][x]                  d/dR_Ax of (mn|ls), chemists' order
+    A basis function moves with its atom (d m/dR_A = d m/dA for m on A);
+    the operator term uses translational invariance of <m|V_C|n>:
+    d/dC = -(d/dA + d/dB)."""
+    n = len(basis)
+    natom = len(charges)
+    atom_of = np.asarray(atom_of)
+    dS = np.zeros((3, n, n))                  # <d m/dx | n> for every m (its own centre)
+    dT = np.zeros((3, n, n))
+    dVC = np.zeros((natom, 3, n, n))          # <d m/dx | V_C | n>
+    for x in range(3):
+        for m in range(n):
+            terms = center_derivative(basis[m], x)
+            for k in range(n):
+                dS[x, m, k] = _dsum(overlap, terms, basis[k])
+                dT[x, m, k] = _dsum(kinetic, terms, basis[k])
+                for C, (Z, R) in enumerate(charges):
+                    dVC[C, x, m, k] = _dsum(potential, terms, basis[k], [(Z, R)])
+    dV = dVC.sum(0)
+    # D[x][m,n,l,s] = (d m/dx n | l s), (l<=s) computed, (l,s) symmetric
+    D = np.zeros((3, n, n, n, n))
+    for x in range(3):
+        for m in range(n):
+            terms = center_derivative(basis[m], x)
+            for q in range(n):
+                for l in range(n):
+                    for s in range(l + 1):
+                        v = sum(eri(t, basis[q], basis[l], basis[s]) for t in terms)
+                        D[x, m, q, l, s] = D[x, m, q, s, l] = v
+    out = {'S': [], 'T': [], 'V': [], 'half_S': [], 'ERI': []}
+    for A in range(natom):
+        on = (atom_of == A).astype(float)
+        mS, mT, mV, mH, mE = [], [], [], [], []
+        for x in range(3):
+            bra = on[:, None] * dS[x]
+            mH.append(bra)
+            mS.append(bra + bra.T)
+            t = on[:, None] * dT[x]
+            mT.append(t + t.T)
+            v = on[:, None] * dV[x]
+            op = dVC[A, x] + dVC[A, x].T
+            mV.append(v + v.T - op)
+            Dx = D[x]
+            e = (on[:, None, None, None] * Dx
+                 + on[None, :, None, None] * Dx.transpose(1, 0, 2, 3)
+                 + on[None, None, :, None] * Dx.transpose(2, 3, 0, 1)
+                 + on[None, None, None, :] * Dx.transpose(2, 3, 1, 0))
+            mE.append(e)
+        out['S'].append(mS)
+        out['T'].append(mT)
+        out['V'].append(mV)
+        out['half_S'].append(mH)
+        out['ERI'].append(mE)
+    return out

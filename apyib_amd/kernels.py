@@ -6,6 +6,7 @@ device.  No computation happens in Python.
 """
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -118,6 +119,27 @@ def mm(A, B, C=None, alpha=1.0, beta=0.0, conj_a=False, conj_b=False, split_k=0,
     return zgemm_raw(m=m, n=n, k=k, a=A, a_layout=al, lda=lda, b=B, b_layout=bl, ldb=ldb,
                      c=C, ldc=C.stride(0), alpha=alpha, beta=beta, conj_a=conj_a, conj_b=conj_b,
                      split_k=split_k, tile=tile, tune=tune)
+
+
+def zdot_many(xs, y, conj=False):
+    """[sum_l op(x_m[l]) y[l] for x_m in xs] (apy_zdot_many, op = conj when
+    conj=True; deterministic), as a host complex array."""
+    lib = _lib.load()
+    y = y.reshape(-1)
+    out = np.zeros(len(xs), dtype=np.complex128)
+    part = torch.empty(_lib.APY_MAXVEC, dtype=torch.complex128, device=y.device)
+    nb = int(lib.apy_reduce_workspace(y.numel(), _lib.APY_MAXVEC))
+    ws = workspace(nb, y.device)
+    for s0 in range(0, len(xs), _lib.APY_MAXVEC):
+        chunk = [x.reshape(-1) for x in xs[s0:s0 + _lib.APY_MAXVEC]]
+        if any(x.numel() != y.numel() for x in chunk):
+            raise ValueError("zdot_many: length mismatch")
+        arr = (ctypes.c_void_p * len(chunk))(*[x.data_ptr() for x in chunk])
+        _lib.check(lib.apy_zdot_many(len(chunk), arr, ctypes.c_void_p(y.data_ptr()), y.numel(), int(conj),
+                                     ctypes.c_void_p(part.data_ptr()), ctypes.c_void_p(ws.data_ptr()), nb,
+                                     stream_of(y.device)), "apy_zdot_many")
+        out[s0:s0 + len(chunk)] = part[:len(chunk)].cpu().numpy()
+    return out
 
 
 def plans_export():

@@ -16,7 +16,7 @@ from .kernels import stream_of, workspace
 from .so_ci import SOCI
 
 CDT = torch.complex128
-CID_SP_BLOCKS = ("vvvv", "oooo", "w21", "w3", "vvoo", "l")
+CID_SP_BLOCKS = ("vvvs", "vvva", "oooo", "w21", "w3", "vvoo", "l")
 
 
 def _vp(t):

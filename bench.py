@@ -289,6 +289,134 @@ def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
     return res
 
 
+def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10):
+    """Spin-adapted closed-shell CISD residual steps (apy_cisd_sp_step) at
+    o=10, v=80 spatial orbitals (the closed-shell counterpart of the
+    O=20, V=160 spin-orbital workload), seeded chemists' integrals with the
+    particle-exchange symmetry, generated on the host and copied once."""
+    import numpy as np
+    import torch
+    from apyib_amd.spatial_ci import SPIntegrals, SpatialCI
+
+    rng = np.random.default_rng(seed + 31)
+    n = o + v
+    E4 = 0.05 * (rng.standard_normal((n,) * 4) + 1j * rng.standard_normal((n,) * 4))
+    E4 = torch.from_numpy(0.5 * (E4 + E4.transpose(2, 3, 0, 1))).to(dev)
+    eps = np.concatenate([np.sort(rng.uniform(-1.5, -0.3, o)), np.sort(rng.uniform(0.2, 3.0, v))])
+    F = torch.from_numpy(np.diag(eps).astype(complex)).to(dev)
+    ci = SpatialCI(SPIntegrals.from_mo(E4, o), F, eps)
+    del E4
+    t = ci.initial_guess(ci.new_vector())
+    tn, r = ci.new_vector(), ci.new_vector()
+    ci.energy(t)
+    for _ in range(2):
+        ci.step(t, tn, r)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ci.step(t, tn, r)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    ov, oo, vv = o * v, o * o, v * v
+    flops = 8.0 * (oo * vv * vv + oo * vv * oo + oo * v * v * v + 3 * ov ** 3 + o * vv * o * v + v * oo * v * o
+                   + ov * ov + 2 * ov * ov + o * v * ov * v + o * v * oo * v + o * v * o + o * v * v)
+    res = {"workload": f"spin-adapted CISD residual step (ci_wfn.py:457-486), o={o} v={v} spatial orbitals",
+           "ms_per_step": round(ms, 3), "executed_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+           "ppl_flops": 8.0 * oo * vv * vv, "step_flops": flops}
+    log(f"spatial cisd: {res}")
+    return res
+
+
+def mp2_adjoint_set_measurement(dev, seed, O=30, V=270, ninst=8, ndir=200, nfd=2, h=1e-5, tol=1e-10):
+    """MP2 adjoint set: `ninst` orbital-invariant spin-orbital MP2 instances
+    at O=30, V=270 (only <ij||ab>, <ab||ij> and F), each with `ndir` seeded
+    Hermitian directions (the last 4 on <pq||rs>: real draws with its pair
+    antisymmetry, the rest on F).  Adjoint: one amplitude solve, one lambda
+    solve and the gradient densities per instance give all directional
+    derivatives.  Plain loop: central differences, two solves per
+    direction, timed on `nfd` directions per instance (one on F, one on
+    <pq||rs>) and scaled to `ndir`; those directions also check the adjoint
+    against the differences."""
+    import numpy as np
+    import torch
+    from apyib_amd import synthetic
+    from apyib_amd.adjoint import MP2Adjoint
+    from apyib_amd.kernels import zdot_many
+    from apyib_amd.so_ci import MP2_BLOCKS, SOIntegrals
+
+    def sync_time(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    n, N2 = O + V, O * O * V * V
+    t_adj = t_fd = 0.0
+    errs, gens = [], 0.0
+    for k in range(ninst):
+        sk = seed * 100 + k
+        F, eps = synthetic.fock(O, V, sk)
+        F = torch.from_numpy(F).to(dev)
+        ints = SOIntegrals.synthetic(O, V, sk, device=dev, blocks=MP2_BLOCKS)
+        W, Vd = ints["oovv"].view(N2), ints["vvoo_t"].view(N2)
+        dFs = [torch.from_numpy(synthetic.fock_perturbation(n, sk * 1000 + d, "nuclear" if d % 2 else "field"))
+               .to(dev) for d in range(ndir - 4)]
+        if k == 0:                                                   # warm the kernels and plans once
+            MP2Adjoint(F, W, Vd, eps, O, tol=tol).gradients()
+
+        def adjoint():
+            s = MP2Adjoint(F, W, Vd, eps, O, tol=tol)
+            s.energy()
+            G, gW, gV = s.gradients()
+            out = list(zdot_many(dFs, G))
+            for d in range(4):                                       # <pq||rs> directions, generated on the device
+                dI = SOIntegrals(O, V, dev, MP2_BLOCKS)
+                dI.add_synthetic(sk * 1000 + 900 + d, tag=300 + d, std=1.0, accumulate=False, real_only=True)
+                out.append(zdot_many([dI["oovv"]], gW)[0] + zdot_many([dI["vvoo_t"]], gV)[0])
+                del dI
+            return np.array(out), G, gW, gV
+        (g, G, gW, gV), ta = sync_time(adjoint)
+        t_adj += ta
+
+        def fd_pair(which):
+            Ev = []
+            for sgn in (+1, -1):
+                if which == "F":
+                    Ev.append(complex(MP2Adjoint(F + sgn * h * dFs[0], W, Vd, eps, O, tol=tol).energy()))
+                else:
+                    P = SOIntegrals.synthetic(O, V, sk, device=dev, blocks=MP2_BLOCKS)
+                    P.add_synthetic(sk * 1000 + 900, tag=300, std=1.0, scale=sgn * h, accumulate=True,
+                                    real_only=True)
+                    Ev.append(complex(MP2Adjoint(F, P["oovv"].view(N2), P["vvoo_t"].view(N2), eps, O,
+                                                 tol=tol).energy()))
+                    del P
+            return (Ev[0] - Ev[1]) / (2 * h)
+        (fF, fW), tf = sync_time(lambda: (fd_pair("F"), fd_pair("W")))
+        t_fd += tf
+        scF = float(torch.sum((G.view(-1) * dFs[0].view(-1)).abs()))
+        dI = SOIntegrals(O, V, dev, MP2_BLOCKS)
+        dI.add_synthetic(sk * 1000 + 900, tag=300, std=1.0, accumulate=False, real_only=True)
+        scW = float(torch.sum((gW * dI["oovv"].view(N2)).abs()) + torch.sum((gV * dI["vvoo_t"].view(N2)).abs()))
+        del dI
+        errs += [abs(g[0].real - fF.real) / scF, abs(g[ndir - 4].real - fW.real) / scW]
+        log(f"mp2 adjoint set: instance {k}: adjoint {ta:.3f} s, 2 FD pairs {tf:.3f} s, "
+            f"err {errs[-2]:.2e} {errs[-1]:.2e}")
+        del ints, W, Vd, G, gW, gV
+        torch.cuda.empty_cache()
+    plain = t_fd * ndir / nfd
+    res = {"workload": f"orbital-invariant spin-orbital MP2 energy, {ninst} instances O={O} V={V}, {ndir} seeded "
+                       f"Hermitian directions each ({ndir - 4} on F, 4 on <pq||rs>)",
+           "instances": ninst, "directions_per_instance": ndir,
+           "adjoint_s": round(t_adj, 3),
+           "plain_fd_s_measured": round(t_fd, 3), "plain_fd_directions_measured": nfd * ninst,
+           "plain_fd_s_scaled": round(plain, 2),
+           "speedup_adjoint_vs_plain_fd": round(plain / t_adj, 1),
+           "fd_step": h, "max_err_adjoint_vs_fd_rel_to_scale": float(max(errs))}
+    log(f"mp2 adjoint set: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,6 +434,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-aat-set", action="store_true", help="skip the batched finite-difference set timing")
     ap.add_argument("--no-cisd-set", action="store_true", help="skip the CISD field-derivative set timing")
+    ap.add_argument("--no-mp2-adjoint-set", action="store_true",
+                    help="skip the MP2 adjoint set (8 instances O=30 V=270)")
+    ap.add_argument("--no-spatial", action="store_true", help="skip the spin-adapted CISD step timing")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -387,6 +518,9 @@ def main():
         aat_set = aat_set_measurement(dev, args.seed)
         if not args.no_cisd_set:
             aat_set["C"] = cisd_field_set_measurement(dev, args.seed)
+        if not args.no_mp2_adjoint_set:
+            aat_set["D"] = mp2_adjoint_set_measurement(dev, args.seed)
+    spatial = None if args.no_spatial else spatial_cisd_measurement(dev, args.seed)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -430,6 +564,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "aat_set": aat_set,
+        "spatial_cisd": spatial,
     }
     print(json.dumps(out), flush=True)
 

@@ -68,7 +68,7 @@ typedef struct apy_gemm_desc {
   int64_t batch;      /* >= 1 */
   int reduce_batch;   /* 0 or 1 */
   int split_k;        /* 0 = choose automatically, else number of K slices */
-  int tile;           /* 0 = choose automatically, else tile config 1..9 */
+  int tile;           /* 0 = choose automatically, else tile config 1..11 */
 } apy_gemm_desc;
 
 /* Workspace (bytes) apy_zgemm needs for this descriptor (0 if none); it
@@ -151,7 +151,8 @@ int apy_so_fock_from_mo(int64_t m, const void* f_mo, void* f_so, void* stream);
  * ---------------------------------------------------------------------- */
 typedef struct apy_sp_ints {
   int64_t o, v;
-  void* vvvv;  /* [(a,b)][(c,d)]      <ab|cd>                             (ci_wfn.py:475) */
+  void* vvvs;  /* [(a<=b)][(c<=d)]    <ab|cd> + <ab|dc> (c<d), <ab|cc>     (ci_wfn.py:475) */
+  void* vvva;  /* [(a<b)][(c<d)]      <ab|cd> - <ab|dc>                   (ci_wfn.py:475) */
   void* oooo;  /* [(i,j)][(k,l)]      <kl|ij>                             (ci_wfn.py:474) */
   void* w21;   /* [2ov][(j,b)]  rows (k,c): 2<kb|cj> - <kb|jc>; rows ov+(k,c): -<kb|cj>  (:476-481) */
   void* w3;    /* [(k,c)][(j,a)]      <ka|jc>                             (ci_wfn.py:478, 482) */
@@ -164,8 +165,10 @@ typedef struct apy_sp_ints {
   void* w5;    /* [i][(j,k,b)]        2<kj|ib> - <kj|bi>                  (ci_wfn.py:463) */
 } apy_sp_ints;
 
-/* Complex elements of each block, in struct order.  Host. */
-int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[11]);
+/* Complex elements of each block, in struct order.  Pairs are packed row
+ * by row: (x<=y) at x(2N-x+1)/2 + y - x, (x<y) at x(2N-x-1)/2 + y - x - 1.
+ * Host. */
+int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[12]);
 /* Blocks from eri_mo[m][m][m][m] = (pq|rs), m = o + v (NULL blocks skipped). */
 int apy_sp_ints_from_mo(int64_t m, const void* eri_mo, const apy_sp_ints* ints, void* stream);
 /* Per-solve operands that depend on the Fock matrix fock[(o+v)][(o+v)]:

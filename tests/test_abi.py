@@ -69,12 +69,13 @@ def test_spatial_sizes_and_struct_layouts():
     lib = _lib.load()
     o, v = 5, 9
     ov = o * v
-    sizes = (ctypes.c_int64 * 11)()
+    sizes = (ctypes.c_int64 * 12)()
     assert lib.apy_sp_ints_sizes(o, v, sizes) == 0
-    assert list(sizes) == [v ** 4, o ** 4, 2 * ov * ov, ov * ov, ov * ov, ov * ov, v * v * o * v, v * o ** 3,
-                           ov * ov, v * o * v * v, o ** 3 * v]
+    sv, av = v * (v + 1) // 2, v * (v - 1) // 2
+    assert list(sizes) == [sv * sv, av * av, o ** 4, 2 * ov * ov, ov * ov, ov * ov, ov * ov, v * v * o * v,
+                           v * o ** 3, ov * ov, v * o * v * v, o ** 3 * v]
     assert lib.apy_sp_ints_sizes(0, v, sizes) == _lib.APY_E_ARG
-    assert ctypes.sizeof(_lib.SpInts) == 8 * 2 + 8 * 11
+    assert ctypes.sizeof(_lib.SpInts) == 8 * 2 + 8 * 12
     # o, v, 5 pointers, int (padded), 4 pointers
     assert ctypes.sizeof(_lib.CisdSpArgs) == 8 * 2 + 8 * 5 + 8 + 8 * 4
 

@@ -1,0 +1,18 @@
+"""Analytic Hartree-Fock AATs through the drop-in
+apyib_amd.analytic_aats.analytic_derivative (GPU transforms of the MO and
+derivative integrals) against the reference tests' arrays and the oracle."""
+import numpy as np
+import pytest
+
+from apyib_amd.analytic_aats import analytic_derivative
+from tests.test_oracle_analytic import GOLDEN, golden_id, golden_params, runnable
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("e", [e for e in GOLDEN if e["call"]["function"] == "compute_RHF_AATs" and runnable(e)],
+                         ids=golden_id)
+def test_rhf_analytic_aat_golden(dev, e):
+    ad = analytic_derivative(golden_params(e))
+    I = ad.compute_RHF_AATs(orbitals=e["call"]["orbitals"])
+    assert np.abs(I - np.array(e["aat"])).max() < e["tolerance"]
