@@ -281,7 +281,7 @@ struct Cfg {
   int bm, bn, wm, wn, bk;
   double eff;  // relative MFMA efficiency used by the cost model
 };
-constexpr int NCFG = 11;
+constexpr int NCFG = 9;
 constexpr Cfg CFGS[NCFG + 1] = {
     // eff: measured fraction of the FP64 MFMA peak on large products
     {0, 0, 0, 0, 0, 0.0},          // 0: automatic
@@ -294,8 +294,6 @@ constexpr Cfg CFGS[NCFG + 1] = {
     {32, 32, 16, 16, 16, 0.76},    // 7
     {64, 64, 32, 32, 8, 0.855},    // 8: half the LDS of 1, more blocks per CU
     {32, 64, 16, 32, 8, 0.81},     // 9
-    {64, 128, 32, 64, 8, 0.80},    // 10
-    {128, 64, 64, 32, 8, 0.80},    // 11
 };
 
 template <int CI, bool AM, bool BN_>
@@ -333,8 +331,6 @@ void launch_cfg(int cfg, const GemmArgs& a, bool am, bool bnk, hipStream_t s) {
     case 6: launch_layout<6>(a, am, bnk, s); break;
     case 8: launch_layout<8>(a, am, bnk, s); break;
     case 9: launch_layout<9>(a, am, bnk, s); break;
-    case 10: launch_layout<10>(a, am, bnk, s); break;
-    case 11: launch_layout<11>(a, am, bnk, s); break;
     default: launch_layout<7>(a, am, bnk, s); break;
   }
 }
@@ -349,8 +345,6 @@ void* cfg_kernel(int cfg, bool am, bool bnk) {
     case 6: return kernel_ptr<6>(am, bnk);
     case 8: return kernel_ptr<8>(am, bnk);
     case 9: return kernel_ptr<9>(am, bnk);
-    case 10: return kernel_ptr<10>(am, bnk);
-    case 11: return kernel_ptr<11>(am, bnk);
     default: return kernel_ptr<7>(am, bnk);
   }
 }

@@ -68,7 +68,7 @@ typedef struct apy_gemm_desc {
   int64_t batch;      /* >= 1 */
   int reduce_batch;   /* 0 or 1 */
   int split_k;        /* 0 = choose automatically, else number of K slices */
-  int tile;           /* 0 = choose automatically, else tile config 1..11 */
+  int tile;           /* 0 = choose automatically, else tile config 1..9 */
 } apy_gemm_desc;
 
 /* Workspace (bytes) apy_zgemm needs for this descriptor (0 if none); it
