@@ -1,28 +1,28 @@
-"""Analytic atomic axial tensors, Hartree-Fock level (oracle only; test
-infrastructure, never the product path).
+"""Analytic atomic axial tensors, Hartree-Fock and MP2 levels (oracle only;
+test infrastructure, never the product path).
 
-Restates analytic_aats.py:30-212 (compute_RHF_AATs) in NumPy on the
-package's host integrals: the coupled-perturbed Hartree-Fock (CPHF)
-coefficients U for each nuclear displacement (skeleton derivative
-integrals, analytic_aats.py:84-141) and each magnetic-field component
-(the -1/2 L operator, :143-186), and
-
-    AAT_HF[lambda alpha][beta] = 2 sum_em U_H[beta][e,m] (U_R[lambda alpha][e,m]
-                                                         + half_S[lambda alpha][m,e])
-
-(:190-196), e over active virtuals, m over all occupied orbitals.
+Restates analytic_aats.py:30-212 (compute_RHF_AATs) and :215-550
+(compute_MP2_AATs) in NumPy on the package's host integrals: the
+coupled-perturbed Hartree-Fock (CPHF) coefficients U for each nuclear
+displacement (skeleton derivative integrals, :84-141, :412-470) and each
+magnetic-field component (the -1/2 L operator, :143-186, :278-330), with
+the dependent (occupied-occupied, virtual-virtual) blocks fixed by the
+'orbitals' option as the reference does; the first-order Fock matrices,
+integrals and MP2 amplitudes (:332-358, :471-505); and the AAT terms
+(HF, the four doubles terms and the normalisation term, :507-546).
 
 Pinned by: the reference's analytic AAT arrays for H2O2/STO-3G
-(test_018_RHF_Analytic_AAT.py:7, :62; tests/golden/analytic_aat_goldens.json)
-and, for the derivative integrals, central differences of the integrals
-themselves (tests/test_oracle_analytic.py).
+(test_018_RHF_Analytic_AAT.py:7, :62; test_019_MP2_Analytic_AAT.py:7, :62,
+:117, :172; tests/golden/analytic_aat_goldens.json) and, for the derivative
+integrals, central differences of the integrals themselves
+(tests/test_oracle_analytic.py).
 """
 import numpy as np
 
 from apyib_amd import gaussian
 from apyib_amd.hamiltonian import ANGULAR_MOMENTUM_SIGN, Hamiltonian
 
-from . import scf, transforms
+from . import scf, spatial, transforms
 
 
 def contract(expr, *ops):
@@ -41,86 +41,206 @@ def mo_tei(ERI, Cb, Ck):
     return contract('mp,mqrs->pqrs', np.conj(Cb), X)
 
 
-def cphf_setup(C, H, ndocc):
-    """(F, ERI Dirac, A, Ginv_nuclear, Ginv_magnetic) of analytic_aats.py:56-77, 145-148."""
-    nbf = C.shape[0]
-    o, v = slice(0, ndocc), slice(ndocc, nbf)
-    no, nv = ndocc, nbf - ndocc
-    h = np.conj(C).T @ (H.T + H.V) @ C
-    ERI = mo_tei(H.ERI, C, C).swapaxes(1, 2)                      # <pq|rs>
-    F = h + contract('piqi->pq', 2 * ERI[:, o, :, o] - ERI.swapaxes(2, 3)[:, o, :, o])
-    X = 2 * ERI - ERI.swapaxes(2, 3)
-    A = (X + X.swapaxes(1, 3)).swapaxes(1, 2)
-    Amag = (-X + X.swapaxes(1, 3)).swapaxes(1, 2)
-    diag = contract('ab,ij,aibj->aibj', np.eye(nv), np.eye(no),
-                    F[v, v].reshape(nv, 1, nv, 1) - F[o, o].reshape(1, no, 1, no))
-    G = np.linalg.inv((diag + A[v, o, v, o]).reshape(nv * no, nv * no))
-    Gm = np.linalg.inv((diag + Amag[v, o, v, o]).reshape(nv * no, nv * no))
-    return F, ERI, A, G, Gm
+_DERIV_CACHE = {}
 
 
-def nuclear_U(C, H, ndocc, F, A, G, deriv):
-    """U_R (vo block with the o,v coupling of analytic_aats.py:114-118) and the
-    MO half-derivative overlaps, for every (atom, axis) in the order of the
-    reference's loops."""
-    nbf = C.shape[0]
-    o, v = slice(0, ndocc), slice(ndocc, nbf)
-    no, nv = ndocc, nbf - ndocc
-    U_R, half = [], []
-    for N1 in range(len(deriv['S'])):
-        for a in range(3):
-            mo = lambda M: C.T @ M @ C                                 # psi4 mo_oei_deriv1 (real C)
-            T1, V1, S1 = mo(deriv['T'][N1][a]), mo(deriv['V'][N1][a]), mo(deriv['S'][N1][a])
-            E1 = contract('mp,nq,lr,gs,mnlg->pqrs', C, C, C, C, deriv['ERI'][N1][a]).swapaxes(1, 2)
-            F1 = T1 + V1 + contract('piqi->pq', 2 * E1[:, o, :, o] - E1.swapaxes(2, 3)[:, o, :, o])
-            B = (-F1[v, o] + contract('ai,ii->ai', S1[v, o], F[o, o])
-                 + 0.5 * contract('mn,amin->ai', S1[o, o], A.swapaxes(1, 2)[v, o, o, o]))
-            U = np.zeros((nbf, nbf), dtype=np.result_type(G, B))
-            U[v, o] += (G @ B.reshape(nv * no)).reshape(nv, no)
-            U[o, v] -= U[v, o].T + S1[o, v]
-            U_R.append(U)
-            half.append(C.T @ deriv['half_S'][N1][a] @ C)
-    return U_R, half
+def _deriv1(H, parameters):
+    key = (parameters['geom'], parameters['basis'])
+    if key not in _DERIV_CACHE:
+        charges = list(zip(H.molecule.Z, H.molecule.coords))
+        _DERIV_CACHE[key] = gaussian.build_deriv1(H.basis_set.functions, H.basis_set.atom_of, charges)
+    return _DERIV_CACHE[key]
 
 
-def magnetic_U(C, H, ndocc, Gm):
-    """U_H (vo and ov blocks, analytic_aats.py:150-164) for the three field
-    components; the operator is -1/2 L with L the reference's
-    ao_angular_momentum (sign fixed by the finite-difference AAT goldens)."""
-    nbf = C.shape[0]
-    o, v = slice(0, ndocc), slice(ndocc, nbf)
-    no, nv = ndocc, nbf - ndocc
-    L = ANGULAR_MOMENTUM_SIGN * gaussian.build_angular_momentum(H.basis_set.functions)
-    out = []
-    for a in range(3):
-        mu = np.conj(C).T @ (-0.5 * L[a]) @ C
-        U = np.zeros((nbf, nbf), dtype=np.result_type(Gm, mu))
-        U[v, o] += (Gm @ mu[v, o].reshape(nv * no)).reshape(nv, no)
+class Context:
+    """SCF solution, MO integrals and the CPHF matrices of one molecule."""
+
+    def __init__(self, parameters):
+        H = Hamiltonian(parameters)
+        self.H = H
+        self.no = no = _ndocc(H)
+        self.nbf = nbf = H.basis_set.nbf()
+        self.nv = nv = nbf - no
+        _, C, eps, _ = scf.solve_scf(H.T, H.V, H.ERI, H.S, H.E_nuc, no, parameters)
+        self.C = C = np.real_if_close(C)
+        self.eps = np.real(eps)
+        self.nfzc = H.basis_set.n_frozen_core() if parameters.get('freeze_core', False) else 0
+        self.freeze_core = bool(parameters.get('freeze_core', False))
+        self.C_list, _ = transforms.get_slices(nbf, no, self.nfzc, parameters['method'])
+        self.o, self.v = slice(0, no), slice(no, nbf)
+        o, v = self.o, self.v
+        self.h = np.conj(C).T @ (H.T + H.V) @ C
+        self.ERI = ERI = mo_tei(H.ERI, C, C).swapaxes(1, 2)                  # <pq|rs>
+        self.F = self.h + contract('piqi->pq', 2 * ERI[:, o, :, o] - ERI.swapaxes(2, 3)[:, o, :, o])
+        X = 2 * ERI - ERI.swapaxes(2, 3)
+        self.A = (X + X.swapaxes(1, 3)).swapaxes(1, 2)
+        self.Amag = (-X + X.swapaxes(1, 3)).swapaxes(1, 2)
+        diag = contract('ab,ij,aibj->aibj', np.eye(nv), np.eye(no),
+                        self.F[v, v].reshape(nv, 1, nv, 1) - self.F[o, o].reshape(1, no, 1, no))
+        self.G = np.linalg.inv((diag + self.A[v, o, v, o]).reshape(nv * no, nv * no))
+        self.Gm = np.linalg.inv((diag + self.Amag[v, o, v, o]).reshape(nv * no, nv * no))
+        self.natom = H.molecule.natom()
+        self.deriv = _deriv1(H, parameters)
+        self.L = ANGULAR_MOMENTUM_SIGN * gaussian.build_angular_momentum(H.basis_set.functions)
+
+    def _dep_denoms(self):
+        o, v = self.o, self.v
+        Do = (self.eps[o] - self.eps[o].reshape(-1, 1)) + np.eye(self.no)
+        Dv = (self.eps[v] - self.eps[v].reshape(-1, 1)) + np.eye(self.nv)
+        return Do, Dv
+
+    def magnetic(self, b, orbitals):
+        """(U_h, h_core) for field component b (analytic_aats.py:278-313)."""
+        o, v, no, nv, nbf = self.o, self.v, self.no, self.nv, self.nbf
+        C = self.C
+        h_core = C.T @ (-0.5 * self.L[b]) @ C
+        U = np.zeros((nbf, nbf))
+        U[v, o] += (self.Gm @ h_core[v, o].reshape(nv * no)).reshape(nv, no)
         U[o, v] += U[v, o].T
-        out.append(U)
+        Am = self.Amag.swapaxes(1, 2)
+        if self.freeze_core or orbitals == 'canonical':
+            Do, Dv = self._dep_denoms()
+            U[o, o] += (-h_core[o, o] + contract('em,iejm->ij', U[v, o], Am[o, v, o, o])) / Do
+            U[v, v] += (-h_core[v, v] + contract('em,aebm->ab', U[v, o], Am[v, v, v, o])) / Dv
+            for j in range(nbf):
+                U[j, j] = 0
+        if orbitals == 'non-canonical':
+            f_, o_, v_, _ = self.C_list
+            U[f_, f_] = 0
+            U[o_, o_] = 0
+            U[v_, v_] = 0
+        return U, h_core
+
+    def nuclear(self, N1, a, orbitals):
+        """(U_R, F_core, S_core, ERI_core Dirac, half_S) for atom N1, axis a
+        (analytic_aats.py:412-470)."""
+        o, v, no, nv, nbf = self.o, self.v, self.no, self.nv, self.nbf
+        C, F, A = self.C, self.F, self.A.swapaxes(1, 2)
+        d = self.deriv
+        T1, V1, S1 = (C.T @ d[k][N1][a] @ C for k in ('T', 'V', 'S'))
+        E1 = contract('mp,nq,lr,gs,mnlg->pqrs', C, C, C, C, d['ERI'][N1][a]).swapaxes(1, 2)
+        F1 = T1 + V1 + contract('piqi->pq', 2 * E1[:, o, :, o] - E1.swapaxes(2, 3)[:, o, :, o])
+        B = -F1[v, o] + contract('ai,ii->ai', S1[v, o], F[o, o]) + 0.5 * contract('mn,amin->ai', S1[o, o], A[v, o, o, o])
+        U = np.zeros((nbf, nbf))
+        U[v, o] += (self.G @ B.reshape(nv * no)).reshape(nv, no)
+        U[o, v] -= U[v, o].T + S1[o, v]
+        if self.freeze_core or orbitals == 'canonical':
+            Do, Dv = self._dep_denoms()
+            Bo = (F1[o, o] - contract('ij,jj->ij', S1[o, o], F[o, o]) + contract('em,iejm->ij', U[v, o], A[o, v, o, o])
+                  - 0.5 * contract('mn,imjn->ij', S1[o, o], A[o, o, o, o]))
+            U[o, o] += Bo / Do
+            Bv = (F1[v, v] - contract('ab,bb->ab', S1[v, v], F[v, v]) + contract('em,aebm->ab', U[v, o], A[v, v, v, o])
+                  - 0.5 * contract('mn,ambn->ab', S1[o, o], A[v, o, v, o]))
+            U[v, v] += Bv / Dv
+            for j in range(nbf):
+                U[j, j] = -0.5 * S1[j, j]
+        if orbitals == 'non-canonical':
+            f_, o_, v_, _ = self.C_list
+            U[f_, f_] = -0.5 * S1[f_, f_]
+            U[o_, o_] = -0.5 * S1[o_, o_]
+            U[v_, v_] = -0.5 * S1[v_, v_]
+        half = C.T @ d['half_S'][N1][a] @ C
+        return U, F1, S1, E1, half
+
+
+def rhf_aat(parameters, orbitals='non-canonical'):
+    """[3N, 3] Hartree-Fock AAT (compute_RHF_AATs, analytic_aats.py:30-212):
+    2 sum_em U_H[e,m] (U_R[e,m] + half_S[m,e]), e active virtual, m occupied."""
+    ctx = Context(parameters)
+    o, v_ = ctx.o, ctx.C_list[2]
+    U_H = [ctx.magnetic(b, orbitals)[0] for b in range(3)]
+    AAT = np.zeros((3 * ctx.natom, 3))
+    for N1 in range(ctx.natom):
+        for a in range(3):
+            U_R, _, _, _, half = ctx.nuclear(N1, a, orbitals)
+            for b in range(3):
+                AAT[3 * N1 + a, b] += 2 * contract('em,em', U_H[b][v_, o], U_R[v_, o] + half[o, v_].T)
+    return AAT
+
+
+def _dERI(U, ERI, sgn):
+    """sum over the four indices of U contracted into <pq|rs>
+    (analytic_aats.py:344-347 with sgn = (-1, -1, +1, +1) pattern for the
+    field, :479-483 all +1 for nuclei)."""
+    out = sgn[0] * contract('tp,tqrs->pqrs', U, ERI)
+    out = out + sgn[1] * contract('tq,ptrs->pqrs', U, ERI)
+    out = out + sgn[2] * contract('tr,pqts->pqrs', U, ERI)
+    out = out + sgn[3] * contract('ts,pqrt->pqrs', U, ERI)
     return out
 
 
-def rhf_aat(parameters):
-    """[3N, 3] Hartree-Fock AAT (compute_RHF_AATs, analytic_aats.py:30-212);
-    the result does not depend on the 'orbitals' option (only the vo blocks
-    of U enter)."""
-    H = Hamiltonian(parameters)
-    ndocc = _ndocc(H)
-    nbf = H.basis_set.nbf()
-    _, C, _, _ = scf.solve_scf(H.T, H.V, H.ERI, H.S, H.E_nuc, ndocc, parameters)
-    C = np.real_if_close(C)
-    nfzc = H.basis_set.n_frozen_core() if parameters.get('freeze_core', False) else 0
-    F, ERI, A, G, Gm = cphf_setup(C, H, ndocc)
-    charges = list(zip(H.molecule.Z, H.molecule.coords))
-    deriv = gaussian.build_deriv1(H.basis_set.functions, H.basis_set.atom_of, charges)
-    U_R, half = nuclear_U(C, H, ndocc, F, A, G, deriv)
-    U_H = magnetic_U(C, H, ndocc, Gm)
-    o, va = slice(0, ndocc), slice(ndocc, nbf)
-    del nfzc                                   # frozen core changes only the correlated slices
-    natom = H.molecule.natom()
-    AAT = np.zeros((3 * natom, 3))
-    for la in range(3 * natom):
-        for b in range(3):
-            AAT[la, b] += np.real(2 * contract('em,em', U_H[b][va, o], U_R[la][va, o] + half[la][o, va].T))
+def mp2_aat(parameters, normalization='full', orbitals='non-canonical'):
+    """[3N, 3] MP2 AAT (compute_MP2_AATs, analytic_aats.py:215-550)."""
+    ctx = Context(parameters)
+    o, v = ctx.o, ctx.v
+    f_, o_, v_, t_ = ctx.C_list
+    eps = ctx.eps
+    ERI = ctx.ERI
+    # canonical MP2 amplitudes over the active orbitals (mp2_wfn.py:42-59)
+    eri_act = ERI[t_, t_, t_, t_]
+    O = ctx.no - ctx.nfzc
+    _, t2 = spatial.solve_mp2(eri_act, eps[t_], O)
+    eo, ev = eps[o_], eps[v_]
+    D = eo.reshape(-1, 1, 1, 1) + eo.reshape(-1, 1, 1) - ev.reshape(-1, 1) - ev
+    tt = 2 * t2 - t2.swapaxes(2, 3)
+    N = 1.0 if normalization == 'intermediate' else 1 / np.sqrt(1 + contract('ijab,ijab', t2, tt))
+    Am = ctx.Amag.swapaxes(1, 2)
+    A = ctx.A.swapaxes(1, 2)
+    U_H, dT2_dH = [], []
+    for b in range(3):
+        U, h_core = ctx.magnetic(b, orbitals)
+        df = np.zeros((ctx.nbf, ctx.nbf))
+        df[o, o] += (-h_core[o, o] + U[o, o] * eps[o].reshape(-1, 1) - U[o, o].T * eps[o]
+                     + contract('em,iejm->ij', U[v, o], Am[o, v, o, o]))
+        df[v, v] += (-h_core[v, v] + U[v, v] * eps[v].reshape(-1, 1) - U[v, v].T * eps[v]
+                     + contract('em,aebm->ab', U[v, o], Am[v, v, v, o]))
+        dE = _dERI(U, ERI, (-1, -1, 1, 1))
+        dt = dE.swapaxes(0, 2).swapaxes(1, 3)[o_, o_, v_, v_].copy()
+        dt += contract('ac,ijcb->ijab', df[v_, v_], t2) + contract('bc,ijac->ijab', df[v_, v_], t2)
+        dt -= contract('ki,kjab->ijab', df[o_, o_], t2) + contract('kj,ikab->ijab', df[o_, o_], t2)
+        U_H.append(U)
+        dT2_dH.append(dt / D)
+    AAT = np.zeros((3 * ctx.natom, 3))
+    for N1 in range(ctx.natom):
+        for a in range(3):
+            U_R, F1, S1, E1, half = ctx.nuclear(N1, a, orbitals)
+            df = np.zeros((ctx.nbf, ctx.nbf))
+            df[o, o] += (F1[o, o] + U_R[o, o] * eps[o].reshape(-1, 1) + U_R[o, o].T * eps[o]
+                         + contract('em,iejm->ij', U_R[v, o], A[o, v, o, o])
+                         - 0.5 * contract('mn,imjn->ij', S1[o, o], A[o, o, o, o]))
+            df[v, v] += (F1[v, v] + U_R[v, v] * eps[v].reshape(-1, 1) + U_R[v, v].T * eps[v]
+                         + contract('em,aebm->ab', U_R[v, o], A[v, v, v, o])
+                         - 0.5 * contract('mn,ambn->ab', S1[o, o], A[v, o, v, o]))
+            dE = E1 + _dERI(U_R, ERI, (1, 1, 1, 1))
+            dt = dE[o_, o_, v_, v_].copy()
+            dt -= contract('kjab,ik->ijab', t2, df[o_, o_]) + contract('ikab,kj->ijab', t2, df[o_, o_])
+            dt += contract('ijcb,ac->ijab', t2, df[v_, v_]) + contract('ijac,cb->ijab', t2, df[v_, v_])
+            dt = dt / D
+            dtt = 2 * dt - dt.swapaxes(2, 3)
+            N_R = -(1 / np.sqrt((1 + contract('ijab,ijab', np.conj(t2), tt)) ** 3))
+            N_R *= 0.5 * (contract('ijab,ijab', np.conj(dt), tt) + contract('ijab,ijab', dt, np.conj(tt)))
+            R = U_R + half.T                      # U_R[p,q] + half_S[q,p]
+            for b in range(3):
+                UH = U_H[b]
+                dH = dT2_dH[b]
+                dHH = 2 * dH - dH.swapaxes(2, 3)
+                val = N ** 2 * 2 * contract('em,em', UH[v_, o], R[v_, o])
+                val += N ** 2 * contract('ijab,ijab', dtt, dH)
+                if orbitals == 'canonical':
+                    val -= N ** 2 * 2.0 * contract('ijab,kjab,ki', dtt, t2, UH[o_, o_])
+                    val += N ** 2 * 2.0 * contract('ijab,ijcb,ac', dtt, t2, UH[v_, v_])
+                val -= N ** 2 * 2.0 * contract('ijab,kjab,ki', dHH, t2, R[o_, o_])
+                val += N ** 2 * 2.0 * contract('ijab,ijcb,ac', dHH, t2, R[v_, v_])
+                val += N ** 2 * 2.0 * contract('ijab,kjab,km,im', t2, tt, UH[o_, o], R[o_, o])
+                if orbitals == 'canonical':
+                    val += N ** 2 * 2.0 * contract('ijab,ijcb,ec,ea', t2, tt, UH[v_, v_], R[v_, v_])
+                val += N ** 2 * 2.0 * contract('ijab,ijab,em,em', t2, tt, UH[v_, o], R[v_, o])
+                val -= N ** 2 * 2.0 * contract('ijab,imab,ej,em', t2, tt, UH[v_, o_], R[v_, o_])
+                val -= N ** 2 * 2.0 * contract('ijab,ijae,bm,em', t2, tt, UH[v_, o], R[v_, o])
+                if normalization == 'full':
+                    if orbitals == 'canonical':
+                        val -= N * N_R * 2.0 * contract('ijab,kjab,ki', tt, t2, UH[o_, o_])
+                        val += N * N_R * 2.0 * contract('ijab,ijcb,ac', tt, t2, UH[v_, v_])
+                    val += N * N_R * contract('ijab,ijab', tt, dH)
+                AAT[3 * N1 + a, b] += np.real(val)
     return AAT
