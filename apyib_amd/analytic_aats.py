@@ -1,20 +1,23 @@
 """Analytic atomic axial tensors with apyib's interface (analytic_aats.py).
 
-`analytic_derivative(parameters)` with `compute_RHF_AATs(orbitals)` and
-`compute_MP2_AATs(normalization, orbitals)` as in the reference
-(analytic_aats.py:14-550): the AAT from coupled-perturbed Hartree-Fock
-coefficients for every nuclear displacement and magnetic-field component
-and, for MP2, the first-order amplitudes, with no displaced wavefunctions.
+`analytic_derivative(parameters)` with `compute_RHF_AATs(orbitals)`,
+`compute_MP2_AATs`, `compute_CISD_AATs` and `compute_CID_AATs`
+(normalization, orbitals) as in the reference (analytic_aats.py:14-1927):
+the AAT from coupled-perturbed Hartree-Fock coefficients for every nuclear
+displacement and magnetic-field component and, for the correlated methods,
+the first-order amplitudes, with no displaced wavefunctions.
 
 Skeleton first-derivative integrals (the reference's psi4 mo_oei_deriv1,
 mo_tei_deriv1 and mo_overlap_half_deriv1) come from the package's host
 integral engine (gaussian.build_deriv1).  On the GPU: the AO -> MO
 transforms of the two-electron integrals and of every derivative integral
-set (apy_eri_mo), the one-electron transforms, the MP2 amplitudes, and the
+set (apy_eri_mo), the one-electron transforms, the MP2 amplitudes, the
 four U-contractions of the first-order integrals per perturbation (n^5
-each, complex GEMMs).  The (nv*no)^2 CPHF matrices are inverted and the
-O(o^2 v^2 n) amplitude-level contractions done on the host, as the
-reference does.
+each, complex GEMMs), the CISD/CID amplitudes and, per perturbation, the
+first-order CISD/CID amplitude equations (spatial_ci.ResponseCI: the
+spin-adapted step with its linear-response term, Jacobi + DIIS).  The
+(nv*no)^2 CPHF matrices are inverted and the O(o^3 v^2) amplitude-level
+AAT contractions done on the host, as the reference does.
 """
 import numpy as np
 import torch
@@ -23,8 +26,10 @@ from . import gaussian
 from . import transforms as tr
 from .hamiltonian import ANGULAR_MOMENTUM_SIGN, Hamiltonian
 from .hf_wfn import hf_wfn
+from .ci_wfn import ci_wfn
 from .kernels import zgemm_raw
 from .mp2_wfn import mp2_wfn
+from .spatial_ci import ResponseCI, SPIntegrals
 from .utils import device, get_slices
 from . import _lib
 
@@ -157,6 +162,168 @@ class analytic_derivative(object):
         half = self._mo1(d['half_S'][N1][a])
         return U, F1, S1, E1d, half
 
+    def _df_magnetic(self, U, h_core):
+        """Field derivative of the Fock matrix, oo and vv blocks
+        (analytic_aats.py:300-308, :720-728)."""
+        o, v, eps = self.o, self.v, self.eps
+        Am = self.Amag.swapaxes(1, 2)
+        df = np.zeros((self.nbf, self.nbf))
+        df[o, o] += (-h_core[o, o] + U[o, o] * eps[o].reshape(-1, 1) - U[o, o].T * eps[o]
+                     + _c('em,iejm->ij', U[v, o], Am[o, v, o, o]))
+        df[v, v] += (-h_core[v, v] + U[v, v] * eps[v].reshape(-1, 1) - U[v, v].T * eps[v]
+                     + _c('em,aebm->ab', U[v, o], Am[v, v, v, o]))
+        return df
+
+    def _df_nuclear(self, U_R, F1, S1):
+        """Nuclear derivative of the Fock matrix, oo and vv blocks
+        (analytic_aats.py:457-469, :964-974)."""
+        o, v, eps = self.o, self.v, self.eps
+        A = self.A.swapaxes(1, 2)
+        df = np.zeros((self.nbf, self.nbf))
+        df[o, o] += (F1[o, o] + U_R[o, o] * eps[o].reshape(-1, 1) + U_R[o, o].T * eps[o]
+                     + _c('em,iejm->ij', U_R[v, o], A[o, v, o, o])
+                     - 0.5 * _c('mn,imjn->ij', S1[o, o], A[o, o, o, o]))
+        df[v, v] += (F1[v, v] + U_R[v, v] * eps[v].reshape(-1, 1) + U_R[v, v].T * eps[v]
+                     + _c('em,aebm->ab', U_R[v, o], A[v, v, v, o])
+                     - 0.5 * _c('mn,ambn->ab', S1[o, o], A[v, o, v, o]))
+        return df
+
+    def _doubles_part(self, orbitals, normalization, N, N_R, t2, dt2_R, dt2_H, UH, R):
+        """Doubles/doubles and doubles normalisation terms of one AAT element
+        (MP2 analytic_aats.py:491-528, CID :1865-1907, CISD :1226-1249,
+        :1290-1308); R[p,q] = U_R[p,q] + half_S[q,p]."""
+        o = self.o
+        _, o_, v_, _ = self.C_list
+        tt = 2 * t2 - t2.swapaxes(2, 3)
+        dtt = 2 * dt2_R - dt2_R.swapaxes(2, 3)
+        dHH = 2 * dt2_H - dt2_H.swapaxes(2, 3)
+        can = orbitals == 'canonical'
+        val = N ** 2 * _c('ijab,ijab', dtt, dt2_H)
+        if can:
+            val -= N ** 2 * 2.0 * _c('ijab,kjab,ki', dtt, t2, UH[o_, o_])
+            val += N ** 2 * 2.0 * _c('ijab,ijcb,ac', dtt, t2, UH[v_, v_])
+        val -= N ** 2 * 2.0 * _c('ijab,kjab,ki', dHH, t2, R[o_, o_])
+        val += N ** 2 * 2.0 * _c('ijab,ijcb,ac', dHH, t2, R[v_, v_])
+        val += N ** 2 * 2.0 * _c('ijab,kjab,km,im', t2, tt, UH[o_, o], R[o_, o])
+        if can:
+            val += N ** 2 * 2.0 * _c('ijab,ijcb,ec,ea', t2, tt, UH[v_, v_], R[v_, v_])
+        val += N ** 2 * 2.0 * _c('ijab,ijab,em,em', t2, tt, UH[v_, o], R[v_, o])
+        val -= N ** 2 * 2.0 * _c('ijab,imab,ej,em', t2, tt, UH[v_, o_], R[v_, o_])
+        val -= N ** 2 * 2.0 * _c('ijab,ijae,bm,em', t2, tt, UH[v_, o], R[v_, o])
+        if normalization == 'full':
+            if can:
+                val -= N * N_R * 2.0 * _c('ijab,kjab,ki', tt, t2, UH[o_, o_])
+                val += N * N_R * 2.0 * _c('ijab,ijcb,ac', tt, t2, UH[v_, v_])
+            val += N * N_R * _c('ijab,ijab', tt, dt2_H)
+        return val
+
+    def _singles_part(self, orbitals, normalization, N, N_R, t1, t2, dt1_R, dt2_R, dt1_H, dt2_H, UH, R):
+        """Singles/reference, singles/singles, singles/doubles terms and the
+        singles normalisation terms of one CISD AAT element
+        (analytic_aats.py:1163-1224 and :1271-1288 canonical, :1252-1310
+        non-canonical), as (coefficient, subscripts, operands) rows."""
+        o = self.o
+        _, o_, v_, _ = self.C_list
+        tt = 2 * t2 - t2.swapaxes(2, 3)
+        dtt = 2 * dt2_R - dt2_R.swapaxes(2, 3)
+        dHH = 2 * dt2_H - dt2_H.swapaxes(2, 3)
+        Hvo, Hvo_, Hov_, Hvv, Hoo, Hoo_ = UH[v_, o], UH[v_, o_], UH[o_, v_], UH[v_, v_], UH[o_, o], UH[o_, o_]
+        Rvo, Rvo_, Rov_, Rvv, Roo, Roo_ = R[v_, o], R[v_, o_], R[o_, v_], R[v_, v_], R[o_, o], R[o_, o_]
+        if orbitals == 'canonical':
+            rows = [
+                (2, 'ia,ai', dt1_R, Hvo_), (2, 'ia,ei,ea', t1, Hvo_, Rvv), (-2, 'ia,am,im', t1, Hvo, Roo),
+                (2, 'kc,ck', dt1_H, Rvo_), (2, 'kc,fc,fk', t1, Hvv, Rvo_), (-2, 'kc,kn,cn', t1, Hoo, Rvo),
+                (2, 'ia,ia', dt1_R, dt1_H), (2, 'kc,cf,kf', dt1_R, Hvv, t1), (-2, 'kc,nk,nc', dt1_R, Hoo_, t1),
+                (2, 'ia,ae,ie', dt1_H, Rvv, t1), (-2, 'ia,mi,ma', dt1_H, Roo_, t1),
+                (2, 'kc,fc,fa,ka', t1, Hvv, Rvv, t1), (-2, 'kc,fc,ik,if', t1, Hvv, Roo_, t1),
+                (-2, 'kc,kn,ca,na', t1, Hoo_, Rvv, t1), (2, 'kc,kn,in,ic', t1, Hoo, Roo, t1),
+                (4, 'kc,kc,ia,ia', t1, Hov_, Rov_, t1), (4, 'kc,fn,fn,kc', t1, Hvo, Rvo, t1),
+                (-2, 'kc,fn,fk,nc', t1, Hvo_, Rvo_, t1), (-2, 'kc,fn,cn,kf', t1, Hvo, Rvo, t1),
+                (4, 'kc,fn,ck,nf', t1, Hvo_, Rvo_, t1),
+                (2, 'ijab,bj,ia', dtt, Hvo_, t1), (2, 'kc,ia,ikac', dt1_H, Rov_, tt),
+                (2, 'kc,fc,ia,ikaf', t1, Hvv, Rov_, tt), (-2, 'kc,kn,ia,inac', t1, Hoo_, Rov_, tt),
+                (-2, 'kc,fn,ik,incf', t1, Hvo_, Roo_, tt), (-2, 'kc,fn,in,ikfc', t1, Hvo, Roo, tt),
+                (2, 'kc,fn,ca,knaf', t1, Hvo_, Rvv, tt), (2, 'kc,fn,fa,knca', t1, Hvo_, Rvv, tt),
+                (2, 'ia,kc,ikac', dt1_R, Hov_, tt), (2, 'klcd,dl,kc', dHH, Rvo_, t1),
+                (2, 'ia,kc,ea,kice', t1, Hov_, Rvv, tt), (-2, 'ia,kc,im,kmca', t1, Hov_, Roo_, tt),
+                (2, 'ia,ac,em,imce', t1, Hvv, Rvo_, tt), (2, 'ia,ec,em,imac', t1, Hvv, Rvo_, tt),
+                (-2, 'ia,ki,em,kmae', t1, Hoo_, Rvo_, tt), (-2, 'ia,km,em,kiea', t1, Hoo, Rvo, tt),
+            ]
+            norm = [(2, 'ia,ai', t1, Hvo_), (2, 'kc,kc', t1, Hov_), (2, 'ia,ia', t1, dt1_H),
+                    (2, 'kc,cf,kf', t1, Hvv, t1), (-2, 'kc,nk,nc', t1, Hoo_, t1),
+                    (2, 'ijab,bj,ia', tt, Hvo_, t1), (2, 'ia,kc,ikac', t1, Hov_, tt)]
+        else:
+            rows = [
+                (2, 'ia,ai', dt1_R, Hvo_), (2, 'ia,ei,ea', t1, Hvo_, Rvv), (-2, 'ia,am,im', t1, Hvo, Roo),
+                (2, 'ia,ai', dt1_H, Rvo_), (-2, 'ia,im,am', t1, Hoo, Rvo),
+                (2, 'ia,ia', dt1_R, dt1_H), (2, 'ia,ae,ie', dt1_H, Rvv, t1), (-2, 'ia,mi,ma', dt1_H, Roo_, t1),
+                (2, 'ia,im,km,ka', t1, Hoo, Roo, t1), (4, 'ia,ia,kc,kc', t1, Hov_, Rov_, t1),
+                (4, 'ia,em,em,ia', t1, Hvo, Rvo, t1), (-2, 'ia,em,ei,ma', t1, Hvo_, Rvo_, t1),
+                (-2, 'ia,em,am,ie', t1, Hvo, Rvo, t1), (4, 'ia,em,ai,me', t1, Hvo_, Rvo_, t1),
+                (2, 'ijab,bj,ia', dtt, Hvo_, t1), (2, 'ia,kc,ikac', dt1_H, Rov_, tt),
+                (-2, 'ia,em,km,kiea', t1, Hvo, Roo, tt), (2, 'ia,em,ec,imac', t1, Hvo_, Rvv, tt),
+                (2, 'ia,kc,ikac', dt1_R, Hov_, tt), (2, 'ijab,bj,ia', dHH, Rvo_, t1),
+                (-2, 'ia,km,em,kiea', t1, Hoo, Rvo, tt),
+            ]
+            norm = [(4, 'ia,ai', t1, Hvo_), (2, 'ia,ia', t1, dt1_H), (4, 'ijab,bj,ia', tt, Hvo_, t1)]
+        val = N ** 2 * sum(c * _c(e, *ops) for c, e, *ops in rows)
+        if normalization == 'full':
+            val += N * N_R * sum(c * _c(e, *ops) for c, e, *ops in norm)
+        return val
+
+    def _ci_aats(self, singles, normalization, orbitals, print_level):
+        """Shared body of compute_CISD_AATs / compute_CID_AATs."""
+        if orbitals not in ('canonical', 'non-canonical'):
+            raise ValueError(f"orbitals {orbitals!r}")
+        self._setup()
+        o = self.o
+        f_, o_, v_, t_ = self.C_list
+        wfn_ci = ci_wfn(self.parameters, self.wfn)
+        solver = wfn_ci._spatial_solver(singles)
+        E, t1d, t2d, _ = solver.solve(self.parameters, print_level, e_offset=wfn_ci._e_offset())
+        t_ref = solver.t.clone()
+        t1 = np.real_if_close(t1d.cpu().numpy()) if singles else None
+        t2 = np.real_if_close(t2d.cpu().numpy())
+        O = solver.O
+        tt = 2 * t2 - t2.swapaxes(2, 3)
+        norm2 = 1 + _c('ijab,ijab', np.conj(t2), tt) + (2 * _c('ia,ia', np.conj(t1), t1) if singles else 0)
+        N = 1 / np.sqrt(norm2)          # the full factor in both normalisations (analytic_aats.py:628, :1462)
+
+        def respond(df, dEd):
+            dints = SPIntegrals.from_mo(dEd[t_, t_, t_, t_].permute(0, 2, 1, 3), O, singles)   # chemists' order
+            dF = tr.to_dev(np.ascontiguousarray(df[t_, t_]).astype(np.complex128), self.dev)
+            r = ResponseCI(solver.ints, solver.fock, solver.eps, singles, t_ref, E, dints, dF)
+            _, dt1, dt2, _ = r.solve(self.parameters, print_level)
+            return (np.real_if_close(dt1.cpu().numpy()) if singles else None), np.real_if_close(dt2.cpu().numpy())
+
+        U_H, dT1_dH, dT2_dH = [], [], []
+        for b in range(3):
+            U, h_core = self._magnetic(b, orbitals)
+            Ud = tr.to_dev(U.astype(np.complex128), self.dev)
+            dt1, dt2 = respond(self._df_magnetic(U, h_core), _dERI(Ud, self.Ed, (-1.0, -1.0, 1.0, 1.0)))
+            U_H.append(U)
+            dT1_dH.append(dt1)
+            dT2_dH.append(dt2)
+        AAT = np.zeros((self.natom * 3, 3))
+        for N1 in range(self.natom):
+            for a in range(3):
+                U_R, F1, S1, E1d, half = self._nuclear(N1, a, orbitals)
+                Ud = tr.to_dev(U_R.astype(np.complex128), self.dev)
+                dt1, dt2 = respond(self._df_nuclear(U_R, F1, S1), E1d + _dERI(Ud, self.Ed, (1.0, 1.0, 1.0, 1.0)))
+                s = _c('ijab,ijab', np.conj(dt2), tt) + _c('ijab,ijab', dt2, np.conj(tt))
+                if singles:
+                    s += 2 * _c('ia,ia', np.conj(dt1), t1) + 2 * _c('ia,ia', dt1, np.conj(t1))
+                N_R = -(1 / np.sqrt(norm2 ** 3)) * 0.5 * s
+                R = U_R + half.T
+                for b in range(3):
+                    val = N ** 2 * 2 * _c('em,em', U_H[b][v_, o], R[v_, o])
+                    val += self._doubles_part(orbitals, normalization, N, N_R, t2, dt2, dT2_dH[b], U_H[b], R)
+                    if singles:
+                        val += self._singles_part(orbitals, normalization, N, N_R, t1, t2, dt1, dt2, dT1_dH[b],
+                                                  dT2_dH[b], U_H[b], R)
+                    AAT[3 * N1 + a][b] += np.real(val)
+        return AAT
+
     # ------------------------------------------------------------------- AATs
     def compute_RHF_AATs(self, orbitals='non-canonical'):
         """analytic_aats.py:30-212.  Returns the [3N, 3] Hartree-Fock AAT
@@ -190,16 +357,10 @@ class analytic_derivative(object):
         D = eo.reshape(-1, 1, 1, 1) + eo.reshape(-1, 1, 1) - ev.reshape(-1, 1) - ev
         tt = 2 * t2 - t2.swapaxes(2, 3)
         N = 1.0 if normalization == 'intermediate' else 1 / np.sqrt(1 + _c('ijab,ijab', t2, tt))
-        Am, A = self.Amag.swapaxes(1, 2), self.A.swapaxes(1, 2)
-        nbf = self.nbf
         U_H, dT2_dH = [], []
         for b in range(3):
             U, h_core = self._magnetic(b, orbitals)
-            df = np.zeros((nbf, nbf))
-            df[o, o] += (-h_core[o, o] + U[o, o] * eps[o].reshape(-1, 1) - U[o, o].T * eps[o]
-                         + _c('em,iejm->ij', U[v, o], Am[o, v, o, o]))
-            df[v, v] += (-h_core[v, v] + U[v, v] * eps[v].reshape(-1, 1) - U[v, v].T * eps[v]
-                         + _c('em,aebm->ab', U[v, o], Am[v, v, v, o]))
+            df = self._df_magnetic(U, h_core)
             dE = np.real_if_close(_dERI(tr.to_dev(U.astype(np.complex128), self.dev), self.Ed,
                                         (-1.0, -1.0, 1.0, 1.0)).cpu().numpy())
             dt = dE.swapaxes(0, 2).swapaxes(1, 3)[o_, o_, v_, v_].astype(np.result_type(dE, t2))
@@ -211,43 +372,32 @@ class analytic_derivative(object):
         for N1 in range(self.natom):
             for a in range(3):
                 U_R, F1, S1, E1d, half = self._nuclear(N1, a, orbitals)
-                df = np.zeros((nbf, nbf))
-                df[o, o] += (F1[o, o] + U_R[o, o] * eps[o].reshape(-1, 1) + U_R[o, o].T * eps[o]
-                             + _c('em,iejm->ij', U_R[v, o], A[o, v, o, o])
-                             - 0.5 * _c('mn,imjn->ij', S1[o, o], A[o, o, o, o]))
-                df[v, v] += (F1[v, v] + U_R[v, v] * eps[v].reshape(-1, 1) + U_R[v, v].T * eps[v]
-                             + _c('em,aebm->ab', U_R[v, o], A[v, v, v, o])
-                             - 0.5 * _c('mn,ambn->ab', S1[o, o], A[v, o, v, o]))
+                df = self._df_nuclear(U_R, F1, S1)
                 dE = np.real_if_close((E1d + _dERI(tr.to_dev(U_R.astype(np.complex128), self.dev), self.Ed,
                                                    (1.0, 1.0, 1.0, 1.0))).cpu().numpy())
                 dt = dE[o_, o_, v_, v_].astype(np.result_type(dE, t2))
                 dt -= _c('kjab,ik->ijab', t2, df[o_, o_]) + _c('ikab,kj->ijab', t2, df[o_, o_])
                 dt += _c('ijcb,ac->ijab', t2, df[v_, v_]) + _c('ijac,cb->ijab', t2, df[v_, v_])
                 dt = dt / D
-                dtt = 2 * dt - dt.swapaxes(2, 3)
                 N_R = -(1 / np.sqrt((1 + _c('ijab,ijab', np.conj(t2), tt)) ** 3))
                 N_R *= 0.5 * (_c('ijab,ijab', np.conj(dt), tt) + _c('ijab,ijab', dt, np.conj(tt)))
                 R = U_R + half.T
                 for b in range(3):
-                    UH, dH = U_H[b], dT2_dH[b]
-                    dHH = 2 * dH - dH.swapaxes(2, 3)
-                    val = N ** 2 * 2 * _c('em,em', UH[v_, o], R[v_, o])
-                    val += N ** 2 * _c('ijab,ijab', dtt, dH)
-                    if orbitals == 'canonical':
-                        val -= N ** 2 * 2.0 * _c('ijab,kjab,ki', dtt, t2, UH[o_, o_])
-                        val += N ** 2 * 2.0 * _c('ijab,ijcb,ac', dtt, t2, UH[v_, v_])
-                    val -= N ** 2 * 2.0 * _c('ijab,kjab,ki', dHH, t2, R[o_, o_])
-                    val += N ** 2 * 2.0 * _c('ijab,ijcb,ac', dHH, t2, R[v_, v_])
-                    val += N ** 2 * 2.0 * _c('ijab,kjab,km,im', t2, tt, UH[o_, o], R[o_, o])
-                    if orbitals == 'canonical':
-                        val += N ** 2 * 2.0 * _c('ijab,ijcb,ec,ea', t2, tt, UH[v_, v_], R[v_, v_])
-                    val += N ** 2 * 2.0 * _c('ijab,ijab,em,em', t2, tt, UH[v_, o], R[v_, o])
-                    val -= N ** 2 * 2.0 * _c('ijab,imab,ej,em', t2, tt, UH[v_, o_], R[v_, o_])
-                    val -= N ** 2 * 2.0 * _c('ijab,ijae,bm,em', t2, tt, UH[v_, o], R[v_, o])
-                    if normalization == 'full':
-                        if orbitals == 'canonical':
-                            val -= N * N_R * 2.0 * _c('ijab,kjab,ki', tt, t2, UH[o_, o_])
-                            val += N * N_R * 2.0 * _c('ijab,ijcb,ac', tt, t2, UH[v_, v_])
-                        val += N * N_R * _c('ijab,ijab', tt, dH)
+                    val = N ** 2 * 2 * _c('em,em', U_H[b][v_, o], R[v_, o])
+                    val += self._doubles_part(orbitals, normalization, N, N_R, t2, dt, dT2_dH[b], U_H[b], R)
                     AAT[3 * N1 + a][b] += np.real(val)
         return AAT
+
+    def compute_CISD_AATs(self, normalization='full', orbitals='non-canonical', print_level=0):
+        """analytic_aats.py:553-1391.  Returns the [3N, 3] CISD AAT: HF,
+        singles/reference, reference/singles, singles/singles,
+        doubles/singles, singles/doubles, doubles/doubles and normalisation
+        terms, from first-order CISD amplitudes solved on the GPU for every
+        field component and nuclear displacement."""
+        return self._ci_aats(True, normalization, orbitals, print_level)
+
+    def compute_CID_AATs(self, normalization='full', orbitals='non-canonical', print_level=0):
+        """analytic_aats.py:1392-1927.  Returns the [3N, 3] CID AAT (HF,
+        doubles/doubles and normalisation terms) from first-order CID
+        amplitudes solved on the GPU."""
+        return self._ci_aats(False, normalization, orbitals, print_level)

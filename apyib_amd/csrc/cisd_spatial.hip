@@ -240,7 +240,17 @@ struct SpEpi {
   const double2* energy;
   double2* t2_new;
   double2* r2;
+  const double2* rb;  // optional response term (doubles part): r += rb - (*re + rs) rt
+  const double2* rt;
+  const double2* re;
+  double2 rs;
 };
+
+// scalar of the response term: *re (null: 0) + rs
+__device__ __forceinline__ double2 resp_scalar(const double2* re, double2 rs) {
+  if (re) { const double2 x = *re; rs.x += x.x; rs.y += x.y; }
+  return rs;
+}
 
 __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
   const int64_t o = e.o, v = e.v, ov = o * v, total = ov * ov;
@@ -275,6 +285,11 @@ __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
     const double2 x = e.t2[t];
     r.x -= E.x * x.x - E.y * x.y;
     r.y -= E.x * x.y + E.y * x.x;
+    if (e.rb) {
+      const double2 s = resp_scalar(e.re, e.rs), bb = e.rb[t], y = e.rt[t];
+      r.x += bb.x - (s.x * y.x - s.y * y.y);
+      r.y += bb.y - (s.x * y.y + s.y * y.x);
+    }
     const double D = e.eps[i] + e.eps[j] - e.eps[o + a] - e.eps[o + b];
     e.r2[t] = r;
     e.t2_new[t] = make_double2(x.x + r.x / D, x.y + r.y / D);
@@ -285,13 +300,20 @@ __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
 __global__ __launch_bounds__(256) void sp_epi_singles(int64_t o, int64_t v, const double* __restrict__ eps,
                                                       const double2* __restrict__ F, const double2* __restrict__ acc,
                                                       const double2* __restrict__ t1, const double2* __restrict__ energy,
-                                                      double2* __restrict__ t1_new, double2* __restrict__ r1) {
+                                                      double2* __restrict__ t1_new, double2* __restrict__ r1,
+                                                      const double2* __restrict__ rb, const double2* __restrict__ rt,
+                                                      const double2* __restrict__ re, double2 rs) {
   const int64_t n = o + v, ov = o * v;
   const double2 E = *energy;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ov; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t / v, a = t % v;
     const double2 f = F[(o + a) * n + i], s = acc[t], x = t1[t];
     double2 r = make_double2(f.x + s.x - (E.x * x.x - E.y * x.y), f.y + s.y - (E.x * x.y + E.y * x.x));
+    if (rb) {
+      const double2 q = resp_scalar(re, rs), bb = rb[t], y = rt[t];
+      r.x += bb.x - (q.x * y.x - q.y * y.y);
+      r.y += bb.y - (q.x * y.y + q.y * y.x);
+    }
     const double D = eps[i] - eps[o + a];
     r1[t] = r;
     t1_new[t] = make_double2(x.x + r.x / D, x.y + r.y / D);
@@ -484,6 +506,9 @@ int apy_cisd_sp_step(const apy_cisd_sp_args* A, void* workspace, size_t ws_bytes
   const apy_sp_ints* I = A->ints;
   if (o < 1 || v < 1 || I->o != o || I->v != v) return APY_E_ARG;
   if (A->out == A->t || A->t_new == A->out) return APY_E_ARG;
+  if (A->resp_b && (!A->resp_t || A->resp_b == A->out || A->resp_b == A->t_new || A->resp_t == A->out ||
+                    A->resp_t == A->t_new))
+    return APY_E_ARG;
   const int singles = A->singles ? 1 : 0;
   if (!I->vvvs || (v > 1 && !I->vvva) || !I->w21 || !I->w3 || !I->vvoo) return APY_E_ARG;
   if (singles && (!I->vvvo || !I->vooo || !I->s1 || !I->w4 || !I->w5)) return APY_E_ARG;
@@ -503,17 +528,22 @@ int apy_cisd_sp_step(const apy_cisd_sp_args* A, void* workspace, size_t ws_bytes
     const int s = apy_zgemm_launch(&g[k], base + w.gemm, ws_bytes - w.gemm, st);
     if (s != APY_OK) return s;
   }
+  const double2* rb = (const double2*)A->resp_b;
+  const double2* rt = (const double2*)A->resp_t;
+  const double2 rs = make_double2(A->resp_shift_re, A->resp_shift_im);
   SpEpi e{o, v, A->eps, (const double2*)I->vvoo, (const double2*)(base + w.Ps), (const double2*)(base + w.Pa),
           (const double2*)(base + w.R), (const double2*)(base + w.Y),
           (const double2*)(base + w.X1), (const double2*)(base + w.X2),
           singles ? (const double2*)(base + w.Q1) : nullptr, singles ? (const double2*)(base + w.Q2) : nullptr,
-          t + ov, (const double2*)A->energy, (double2*)A->t_new + ov, (double2*)A->out + ov};
+          t + ov, (const double2*)A->energy, (double2*)A->t_new + ov, (double2*)A->out + ov,
+          rb ? rb + ov : nullptr, rb ? rt + ov : nullptr, (const double2*)A->resp_energy, rs};
   sp_epi_doubles<<<apy_grid1d(ov * ov), 256, 0, st>>>(e);
   APY_LAUNCH_CHECK();
   if (singles) {
     sp_epi_singles<<<apy_grid1d(ov), 256, 0, st>>>(o, v, A->eps, (const double2*)A->fock,
                                                    (const double2*)(base + w.acc), t, (const double2*)A->energy,
-                                                   (double2*)A->t_new, (double2*)A->out);
+                                                   (double2*)A->t_new, (double2*)A->out, rb, rt,
+                                                   (const double2*)A->resp_energy, rs);
   } else {  // CID keeps the singles block of the vector at zero
     sp_zero_kernel<<<apy_grid1d(ov), 256, 0, st>>>(ov, (double2*)A->t_new, (double2*)A->out);
   }

@@ -108,3 +108,62 @@ class SpatialCI(SOCI):
 
     def tune(self, *a, **k):
         raise NotImplementedError("the spin-adapted solver uses the cost-model GEMM plans")
+
+
+class ResponseCI(SpatialCI):
+    """First-order CISD (singles=True) or CID amplitudes dt for one
+    perturbation (dF, d<pq|rs>) of a converged spin-adapted solution t with
+    correlation energy E (analytic_aats.py:733-890, :980-1143; CID
+    :1540-1660): the reference's Jacobi + DIIS iteration on
+
+        r = R_dH(1, t) - s t + [sigma_H(0, dt) - E dt],
+        s = E_dH(t) + E_H(dt)           (projected energy derivative),
+
+    run as apy_cisd_sp_step on dt with the response term resp_b =
+    R_dH(1, t) - R_H(0) (the perturbed residual of t with s = 0, less this
+    Hamiltonian's driver), resp_t = t, resp_energy = E_H(dt) of the
+    iteration and resp_shift = E_dH(t).  The iteration's "energy" is E_H(dt);
+    solve() returns it with E_dH(t) added.  The guess is one such step from
+    dt = 0 (the reference seeds its guess with the density gradient instead;
+    the converged amplitudes are the same).
+
+    ints, fock, eps: the unperturbed problem (as for SpatialCI); t: its
+    converged amplitude vector on the device; dints: SPIntegrals of the
+    perturbed <pq|rs>; dfock: [n, n] derivative Fock matrix (device)."""
+
+    def __init__(self, ints, fock, eps, singles, t, E, dints, dfock):
+        super().__init__(ints, fock, eps, singles)
+        self.t_ref = t.contiguous()
+        self.E_dev = torch.full((1,), complex(E), dtype=CDT, device=self.device)
+        pert = SpatialCI(dints, dfock, self.eps, singles)
+        zero = self.new_vector()
+        scratch = self.new_vector()
+        r_pert = self.new_vector()
+        drv = self.new_vector()
+        pert.step(self.t_ref, scratch, r_pert, energy=self.zero_energy)
+        SpatialCI.step(self, zero, scratch, drv, energy=self.zero_energy)
+        self.b = self.new_vector()
+        self.combine([r_pert, drv], [1.0, -1.0], self.b)
+        self.shift = complex(pert.energy(self.t_ref, torch.zeros(1, dtype=CDT, device=self.device)).cpu()[0])
+
+    def step(self, t, t_new, out, energy=None, **_):
+        args = _lib.CisdSpArgs(self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(),
+                               ctypes.pointer(self.ints.struct), self.hoo.data_ptr(), self.g.data_ptr(),
+                               int(self.singles), self.E_dev.data_ptr(), t.data_ptr(), t_new.data_ptr(),
+                               out.data_ptr(), self.b.data_ptr(), self.t_ref.data_ptr(),
+                               (energy if energy is not None else self.energy_dev).data_ptr(),
+                               self.shift.real, self.shift.imag)
+        ws = workspace(self.ws_bytes, self.device)
+        _lib.check(self.lib.apy_cisd_sp_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
+                   "apy_cisd_sp_step")
+
+    def initial_guess(self, t):
+        scratch = self.new_vector()
+        self.step(self.new_vector(), t, scratch, energy=self.zero_energy)
+        return t
+
+    def solve(self, params, print_level=0, max_diis=7, e_offset=0.0):
+        """Returns (dE, dt1, dt2, iterations): dE the converged projected
+        energy derivative, dt1/dt2 views of the device amplitude vector."""
+        E, dt1, dt2, it = super().solve(params, print_level, max_diis, e_offset)
+        return E + self.shift, dt1, dt2, it

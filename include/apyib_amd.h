@@ -189,10 +189,21 @@ typedef struct apy_cisd_sp_args {
   const void* t;            /* [o*v + o*o*v*v] amplitudes                  */
   void* t_new;              /* t + r / D (must differ from out)            */
   void* out;                /* residual r (must differ from t)             */
+  /* Optional linear-response term (resp_b null: off), added to r before
+   * the update:  r += resp_b - (*resp_energy + resp_shift) * resp_t.
+   * resp_b, resp_t: vectors in the amplitude layout; resp_energy: device
+   * complex scalar (null reads as zero).  With resp_b = the perturbed
+   * Hamiltonian's residual of (1, t) minus this Hamiltonian's driver and
+   * resp_t = t, a step on dt is one iteration of the first-order amplitude
+   * equations of analytic_aats.py:785-826 (CID :1582-1610). */
+  const void* resp_b;
+  const void* resp_t;
+  const void* resp_energy;
+  double resp_shift_re, resp_shift_im;
 } apy_cisd_sp_args;
 
-/* One Jacobi step of ci_wfn.py:457-486 (CID :83-96):  r = sigma(t) - E t,
- * t_new = t + r / D. */
+/* One Jacobi step of ci_wfn.py:457-486 (CID :83-96):  r = sigma(t) - E t
+ * (+ the response term above), t_new = t + r / D. */
 size_t apy_cisd_sp_workspace(int64_t o, int64_t v, int singles);
 int apy_cisd_sp_step(const apy_cisd_sp_args* a, void* workspace, size_t ws_bytes, void* stream);
 /* E = sum 2 F_ov t1 + sum (2<ij|ab> - <ij|ba>) t2 (ci_wfn.py:495, CID :112);

@@ -76,8 +76,8 @@ def test_spatial_sizes_and_struct_layouts():
                            v * o ** 3, ov * ov, v * o * v * v, o ** 3 * v]
     assert lib.apy_sp_ints_sizes(0, v, sizes) == _lib.APY_E_ARG
     assert ctypes.sizeof(_lib.SpInts) == 8 * 2 + 8 * 12
-    # o, v, 5 pointers, int (padded), 4 pointers
-    assert ctypes.sizeof(_lib.CisdSpArgs) == 8 * 2 + 8 * 5 + 8 + 8 * 4
+    # o, v, 5 pointers, int (padded), 4 pointers, 3 response pointers, 2 doubles
+    assert ctypes.sizeof(_lib.CisdSpArgs) == 8 * 2 + 8 * 5 + 8 + 8 * 4 + 8 * 3 + 8 * 2
 
 
 def test_argument_errors_return_before_any_device_work():
@@ -100,3 +100,9 @@ def test_argument_errors_return_before_any_device_work():
     ints = _lib.SpInts(3, 4)
     args = _lib.CisdSpArgs(3, 5, 1, 1, ctypes.pointer(ints), 1, 1, 1, 1, 1, 2, 3)
     assert lib.apy_cisd_sp_step(ctypes.byref(args), None, 0, None) == E
+    # a consistent struct whose response term has no reference vector, or
+    # writes over one of its own inputs
+    ints = _lib.SpInts(3, 5, *([1] * len(_lib.SP_BLOCKS)))
+    for rb, rt in ((7, None), (7, 2), (3, 9), (9, 3)):
+        args = _lib.CisdSpArgs(3, 5, 1, 1, ctypes.pointer(ints), 1, 1, 1, 1, 1, 2, 3, rb, rt)
+        assert lib.apy_cisd_sp_step(ctypes.byref(args), None, 0, None) == E

@@ -1,6 +1,7 @@
-"""Analytic Hartree-Fock and MP2 AATs through the drop-in
+"""Analytic Hartree-Fock, MP2, CID and CISD AATs through the drop-in
 apyib_amd.analytic_aats.analytic_derivative (GPU transforms of the MO and
-derivative integrals) against the reference tests' arrays and the oracle."""
+derivative integrals, first-order CI amplitudes) against the reference
+tests' arrays."""
 import numpy as np
 import pytest
 
@@ -10,14 +11,12 @@ from tests.test_oracle_analytic import GOLDEN, golden_id, golden_params, runnabl
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("e", [e for e in GOLDEN if runnable(e)
-                               and e["call"]["function"] in ("compute_RHF_AATs", "compute_MP2_AATs")],
-                         ids=golden_id)
+@pytest.mark.parametrize("e", [e for e in GOLDEN if runnable(e)], ids=golden_id)
 def test_analytic_aat_golden(dev, e):
     ad = analytic_derivative(golden_params(e))
     c = e["call"]
     if c["function"] == "compute_RHF_AATs":
         I = ad.compute_RHF_AATs(orbitals=c["orbitals"])
     else:
-        I = ad.compute_MP2_AATs(normalization=c["normalization"], orbitals=c["orbitals"])
+        I = getattr(ad, c["function"])(normalization=c["normalization"], orbitals=c["orbitals"])
     assert np.abs(I - np.array(e["aat"])).max() < e["tolerance"]
