@@ -14,8 +14,9 @@ transforms of the two-electron integrals and of every derivative integral
 set (apy_eri_mo), the one-electron transforms, the MP2 amplitudes, the
 four U-contractions of the first-order integrals per perturbation (n^5
 each, complex GEMMs), the CISD/CID amplitudes and, per perturbation, the
-first-order CISD/CID amplitude equations (spatial_ci.ResponseCI: the
-spin-adapted step with its linear-response term, Jacobi + DIIS).  The
+first-order CISD/CID amplitude equations of all 3N+3 perturbations as one
+set (spatial_ci.ResponseSet: the batched spin-adapted step with its
+linear-response term, Jacobi + DIIS per perturbation).  The
 (nv*no)^2 CPHF matrices are inverted and the O(o^3 v^2) amplitude-level
 AAT contractions done on the host, as the reference does.
 """
@@ -29,7 +30,7 @@ from .hf_wfn import hf_wfn
 from .ci_wfn import ci_wfn
 from .kernels import zgemm_raw
 from .mp2_wfn import mp2_wfn
-from .spatial_ci import ResponseCI, SPIntegrals
+from .spatial_ci import ResponseSet, SPIntegrals
 from .utils import device, get_slices
 from . import _lib
 
@@ -289,39 +290,43 @@ class analytic_derivative(object):
         norm2 = 1 + _c('ijab,ijab', np.conj(t2), tt) + (2 * _c('ia,ia', np.conj(t1), t1) if singles else 0)
         N = 1 / np.sqrt(norm2)          # the full factor in both normalisations (analytic_aats.py:628, :1462)
 
-        def respond(df, dEd):
-            dints = SPIntegrals.from_mo(dEd[t_, t_, t_, t_].permute(0, 2, 1, 3), O, singles)   # chemists' order
-            dF = tr.to_dev(np.ascontiguousarray(df[t_, t_]).astype(np.complex128), self.dev)
-            r = ResponseCI(solver.ints, solver.fock, solver.eps, singles, t_ref, E, dints, dF)
-            _, dt1, dt2, _ = r.solve(self.parameters, print_level)
-            return (np.real_if_close(dt1.cpu().numpy()) if singles else None), np.real_if_close(dt2.cpu().numpy())
+        # every field and nuclear perturbation's first-order amplitudes, iterated as one set
+        rs = ResponseSet(solver.ints, solver.fock, solver.eps, singles, t_ref, E)
 
-        U_H, dT1_dH, dT2_dH = [], [], []
+        def add(df, dEd):
+            dints = SPIntegrals.from_mo(dEd[t_, t_, t_, t_].permute(0, 2, 1, 3), O, singles)   # chemists' order
+            rs.add(dints, tr.to_dev(np.ascontiguousarray(df[t_, t_]).astype(np.complex128), self.dev))
+
+        U_H, nuc = [], []
         for b in range(3):
             U, h_core = self._magnetic(b, orbitals)
             Ud = tr.to_dev(U.astype(np.complex128), self.dev)
-            dt1, dt2 = respond(self._df_magnetic(U, h_core), _dERI(Ud, self.Ed, (-1.0, -1.0, 1.0, 1.0)))
+            add(self._df_magnetic(U, h_core), _dERI(Ud, self.Ed, (-1.0, -1.0, 1.0, 1.0)))
             U_H.append(U)
-            dT1_dH.append(dt1)
-            dT2_dH.append(dt2)
-        AAT = np.zeros((self.natom * 3, 3))
         for N1 in range(self.natom):
             for a in range(3):
                 U_R, F1, S1, E1d, half = self._nuclear(N1, a, orbitals)
                 Ud = tr.to_dev(U_R.astype(np.complex128), self.dev)
-                dt1, dt2 = respond(self._df_nuclear(U_R, F1, S1), E1d + _dERI(Ud, self.Ed, (1.0, 1.0, 1.0, 1.0)))
-                s = _c('ijab,ijab', np.conj(dt2), tt) + _c('ijab,ijab', dt2, np.conj(tt))
+                add(self._df_nuclear(U_R, F1, S1), E1d + _dERI(Ud, self.Ed, (1.0, 1.0, 1.0, 1.0)))
+                nuc.append((N1, a, U_R, half))
+        _, dt1s, dt2s, _ = rs.solve(self.parameters, print_level)
+        dT1 = list(np.real_if_close(dt1s.cpu().numpy())) if singles else [None] * dt2s.shape[0]
+        dT2 = list(np.real_if_close(dt2s.cpu().numpy()))
+        AAT = np.zeros((self.natom * 3, 3))
+        for k, (N1, a, U_R, half) in enumerate(nuc):
+            dt1, dt2 = dT1[3 + k], dT2[3 + k]
+            s = _c('ijab,ijab', np.conj(dt2), tt) + _c('ijab,ijab', dt2, np.conj(tt))
+            if singles:
+                s += 2 * _c('ia,ia', np.conj(dt1), t1) + 2 * _c('ia,ia', dt1, np.conj(t1))
+            N_R = -(1 / np.sqrt(norm2 ** 3)) * 0.5 * s
+            R = U_R + half.T
+            for b in range(3):
+                val = N ** 2 * 2 * _c('em,em', U_H[b][v_, o], R[v_, o])
+                val += self._doubles_part(orbitals, normalization, N, N_R, t2, dt2, dT2[b], U_H[b], R)
                 if singles:
-                    s += 2 * _c('ia,ia', np.conj(dt1), t1) + 2 * _c('ia,ia', dt1, np.conj(t1))
-                N_R = -(1 / np.sqrt(norm2 ** 3)) * 0.5 * s
-                R = U_R + half.T
-                for b in range(3):
-                    val = N ** 2 * 2 * _c('em,em', U_H[b][v_, o], R[v_, o])
-                    val += self._doubles_part(orbitals, normalization, N, N_R, t2, dt2, dT2_dH[b], U_H[b], R)
-                    if singles:
-                        val += self._singles_part(orbitals, normalization, N, N_R, t1, t2, dt1, dt2, dT1_dH[b],
-                                                  dT2_dH[b], U_H[b], R)
-                    AAT[3 * N1 + a][b] += np.real(val)
+                    val += self._singles_part(orbitals, normalization, N, N_R, t1, t2, dt1, dt2, dT1[b], dT2[b],
+                                              U_H[b], R)
+                AAT[3 * N1 + a][b] += np.real(val)
         return AAT
 
     # ------------------------------------------------------------------- AATs

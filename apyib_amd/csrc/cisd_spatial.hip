@@ -179,15 +179,22 @@ __global__ __launch_bounds__(256) void sp_ladder_kernel(Dirac g, int64_t o, int6
 }
 
 // T+[(i<=j)][(c<=d)] = (t_ijcd + t_ijdc)/2, T-[(i<j)][(c<d)] = (t_ijcd - t_ijdc)/2
-__global__ __launch_bounds__(256) void sp_pack_kernel(int64_t o, int64_t v, const double2* __restrict__ t2,
-                                                      double2* __restrict__ ts, double2* __restrict__ ta) {
-  const int64_t total = o * o * v * v, sv = nsym(v), av = nanti(v);
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+// T+ / T- of nb amplitude vectors (stride L, doubles at offset o*v) into [nb][(i<=j)][(c<=d)] and
+// [nb][(i<j)][(c<d)]
+__global__ __launch_bounds__(256) void sp_pack_kernel(int64_t o, int64_t v, int64_t nb, int64_t L,
+                                                      const double2* __restrict__ tv, double2* __restrict__ ts,
+                                                      double2* __restrict__ ta) {
+  const int64_t per = o * o * v * v, total = nb * per, sv = nsym(v), av = nanti(v), so = nsym(o), ao = nanti(o);
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = u / per, t = u % per;
     const int64_t d = t % v, c = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
     if (i > j || c > d) continue;
+    const double2* t2 = tv + q * L + o * v;
     const double2 x = t2[t], y = t2[((i * o + j) * v + d) * v + c];
-    ts[isym(i, j, o) * sv + isym(c, d, v)] = c == d ? x : make_double2(0.5 * (x.x + y.x), 0.5 * (x.y + y.y));
-    if (i < j && c < d) ta[ianti(i, j, o) * av + ianti(c, d, v)] = make_double2(0.5 * (x.x - y.x), 0.5 * (x.y - y.y));
+    ts[q * so * sv + isym(i, j, o) * sv + isym(c, d, v)] =
+        c == d ? x : make_double2(0.5 * (x.x + y.x), 0.5 * (x.y + y.y));
+    if (i < j && c < d)
+      ta[q * ao * av + ianti(i, j, o) * av + ianti(c, d, v)] = make_double2(0.5 * (x.x - y.x), 0.5 * (x.y - y.y));
   }
 }
 
@@ -212,20 +219,24 @@ __global__ __launch_bounds__(256) void sp_prepare_kernel(int64_t o, int64_t v, c
   }
 }
 
-// TU[(i,a)][(k,c)] = t_ikac, TU[(i,a)][ov + (k,c)] = t_ikca
-__global__ __launch_bounds__(256) void sp_tu_kernel(int64_t o, int64_t v, const double2* __restrict__ t2,
-                                                    double2* __restrict__ TU) {
-  const int64_t ov = o * v, total = 2 * ov * ov;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+// TU[q][(i,a)][(k,c)] = t_ikac, TU[q][(i,a)][ov + (k,c)] = t_ikca for the nb vectors (stride L)
+__global__ __launch_bounds__(256) void sp_tu_kernel(int64_t o, int64_t v, int64_t nb, int64_t L,
+                                                    const double2* __restrict__ tv, double2* __restrict__ TU) {
+  const int64_t ov = o * v, per = 2 * ov * ov, total = nb * per;
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = u / per, t = u % per;
     const int64_t row = t / (2 * ov), col = t % (2 * ov);
     const int64_t i = row / v, a = row % v;
     const int64_t cc = col % ov, k = cc / v, c = cc % v;
-    TU[t] = col < ov ? t2[((i * o + k) * v + a) * v + c] : t2[((i * o + k) * v + c) * v + a];
+    const double2* t2 = tv + q * L + ov;
+    TU[u] = col < ov ? t2[((i * o + k) * v + a) * v + c] : t2[((i * o + k) * v + c) * v + a];
   }
 }
 
+// Epilogue operands; every per-instance array is instance 0's, the others follow at the region
+// strides (so*sv for Ps, ao*av for Pa, (ov)^2 for R, Y, X1, X2, Q1, Q2; L for the vectors).
 struct SpEpi {
-  int64_t o, v;
+  int64_t o, v, nb, L;
   const double* eps;
   const double2* vvoo;
   const double2* Ps;  // P+ [(i<=j)][(a<=b)]
@@ -240,100 +251,116 @@ struct SpEpi {
   const double2* energy;
   double2* t2_new;
   double2* r2;
-  const double2* rb;  // optional response term (doubles part): r += rb - (*re + rs) rt
-  const double2* rt;
+  const double2* rb;  // optional response term (doubles part): r += rb - (re[q] + rs + es[q]) rt
+  const double2* rt;  // shared by the instances
   const double2* re;
   double2 rs;
+  const double2* es;
 };
 
-// scalar of the response term: *re (null: 0) + rs
-__device__ __forceinline__ double2 resp_scalar(const double2* re, double2 rs) {
-  if (re) { const double2 x = *re; rs.x += x.x; rs.y += x.y; }
+// scalar of the response term of instance q: re[q] + rs + es[q] (null arrays read as zero)
+__device__ __forceinline__ double2 resp_scalar(const double2* re, double2 rs, const double2* es, int64_t q) {
+  if (re) { const double2 x = re[q]; rs.x += x.x; rs.y += x.y; }
+  if (es) { const double2 x = es[q]; rs.x += x.x; rs.y += x.y; }
   return rs;
 }
 
 __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
-  const int64_t o = e.o, v = e.v, ov = o * v, total = ov * ov;
-  const double2 E = *e.energy;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t o = e.o, v = e.v, ov = o * v, per = ov * ov, total = e.nb * per;
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = u / per, t = u % per, qd = q * per, qv = q * e.L;
     const int64_t b = t % v, a = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
+    const double2 E = e.energy[q];
     double2 r = e.vvoo[t];
-    const double2 p = e.R[t];
+    const double2 p = e.R[u];
     r.x += p.x; r.y += p.y;
     {  // ladder: P+ + s_ij s_ab P-
       const int64_t i0 = i < j ? i : j, j0 = i < j ? j : i, a0 = a < b ? a : b, b0 = a < b ? b : a;
-      const double2 ps = e.Ps[isym(i0, j0, o) * nsym(v) + isym(a0, b0, v)];
+      const double2 ps = e.Ps[q * nsym(o) * nsym(v) + isym(i0, j0, o) * nsym(v) + isym(a0, b0, v)];
       r.x += ps.x; r.y += ps.y;
       if (i != j && a != b) {
-        const double2 pa = e.Pa[ianti(i0, j0, o) * nanti(v) + ianti(a0, b0, v)];
+        const double2 pa = e.Pa[q * nanti(o) * nanti(v) + ianti(i0, j0, o) * nanti(v) + ianti(a0, b0, v)];
         const double sg = ((i < j) == (a < b)) ? 1.0 : -1.0;
         r.x += sg * pa.x; r.y += sg * pa.y;
       }
     }
-    const double2 y1 = e.Y[t], y2 = e.Y[((j * o + i) * v + b) * v + a];
+    const double2* Y = e.Y + qd;
+    const double2 y1 = Y[t], y2 = Y[((j * o + i) * v + b) * v + a];
     r.x += y1.x + y2.x; r.y += y1.y + y2.y;
-    const double2 x1 = e.X1[(i * v + a) * ov + j * v + b], x1t = e.X1[(j * v + b) * ov + i * v + a];
-    const double2 x2 = e.X2[(i * v + b) * ov + j * v + a], x2t = e.X2[(j * v + a) * ov + i * v + b];
+    const double2* X1 = e.X1 + qd;
+    const double2* X2 = e.X2 + qd;
+    const double2 x1 = X1[(i * v + a) * ov + j * v + b], x1t = X1[(j * v + b) * ov + i * v + a];
+    const double2 x2 = X2[(i * v + b) * ov + j * v + a], x2t = X2[(j * v + a) * ov + i * v + b];
     r.x += x1.x + x1t.x - x2.x - x2t.x;
     r.y += x1.y + x1t.y - x2.y - x2t.y;
     if (e.Q1) {
-      const double2 q1 = e.Q1[i * (v * ov) + (a * v + b) * o + j], q1t = e.Q1[j * (v * ov) + (b * v + a) * o + i];
-      const double2 q2 = e.Q2[((a * o + i) * o + j) * v + b], q2t = e.Q2[((b * o + j) * o + i) * v + a];
+      const double2* Q1 = e.Q1 + qd;
+      const double2* Q2 = e.Q2 + qd;
+      const double2 q1 = Q1[i * (v * ov) + (a * v + b) * o + j], q1t = Q1[j * (v * ov) + (b * v + a) * o + i];
+      const double2 q2 = Q2[((a * o + i) * o + j) * v + b], q2t = Q2[((b * o + j) * o + i) * v + a];
       r.x += q1.x + q1t.x - q2.x - q2t.x;
       r.y += q1.y + q1t.y - q2.y - q2t.y;
     }
-    const double2 x = e.t2[t];
+    const double2 x = e.t2[qv + t];
     r.x -= E.x * x.x - E.y * x.y;
     r.y -= E.x * x.y + E.y * x.x;
     if (e.rb) {
-      const double2 s = resp_scalar(e.re, e.rs), bb = e.rb[t], y = e.rt[t];
+      const double2 s = resp_scalar(e.re, e.rs, e.es, q), bb = e.rb[qv + t], y = e.rt[t];
       r.x += bb.x - (s.x * y.x - s.y * y.y);
       r.y += bb.y - (s.x * y.y + s.y * y.x);
     }
     const double D = e.eps[i] + e.eps[j] - e.eps[o + a] - e.eps[o + b];
-    e.r2[t] = r;
-    e.t2_new[t] = make_double2(x.x + r.x / D, x.y + r.y / D);
+    e.r2[qv + t] = r;
+    e.t2_new[qv + t] = make_double2(x.x + r.x / D, x.y + r.y / D);
   }
 }
 
-// r1 = F[a][i] + acc - E t1 (ci_wfn.py:457-464); t1_new = t1 + r1 / D_ia
-__global__ __launch_bounds__(256) void sp_epi_singles(int64_t o, int64_t v, const double* __restrict__ eps,
-                                                      const double2* __restrict__ F, const double2* __restrict__ acc,
-                                                      const double2* __restrict__ t1, const double2* __restrict__ energy,
-                                                      double2* __restrict__ t1_new, double2* __restrict__ r1,
-                                                      const double2* __restrict__ rb, const double2* __restrict__ rt,
-                                                      const double2* __restrict__ re, double2 rs) {
-  const int64_t n = o + v, ov = o * v;
-  const double2 E = *energy;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ov; t += (int64_t)gridDim.x * blockDim.x) {
+// r1 = F[a][i] + acc - E t1 (ci_wfn.py:457-464); t1_new = t1 + r1 / D_ia, per instance (acc [nb][ov],
+// vectors L apart)
+__global__ __launch_bounds__(256) void sp_epi_singles(int64_t o, int64_t v, int64_t nb, int64_t L,
+                                                      const double* __restrict__ eps, const double2* __restrict__ F,
+                                                      const double2* __restrict__ acc, const double2* __restrict__ t1,
+                                                      const double2* __restrict__ energy, double2* __restrict__ t1_new,
+                                                      double2* __restrict__ r1, const double2* __restrict__ rb,
+                                                      const double2* __restrict__ rt, const double2* __restrict__ re,
+                                                      double2 rs, const double2* __restrict__ es) {
+  const int64_t n = o + v, ov = o * v, total = nb * ov;
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = u / ov, t = u % ov, qv = q * L;
     const int64_t i = t / v, a = t % v;
-    const double2 f = F[(o + a) * n + i], s = acc[t], x = t1[t];
+    const double2 E = energy[q];
+    const double2 f = F[(o + a) * n + i], s = acc[u], x = t1[qv + t];
     double2 r = make_double2(f.x + s.x - (E.x * x.x - E.y * x.y), f.y + s.y - (E.x * x.y + E.y * x.x));
     if (rb) {
-      const double2 q = resp_scalar(re, rs), bb = rb[t], y = rt[t];
-      r.x += bb.x - (q.x * y.x - q.y * y.y);
-      r.y += bb.y - (q.x * y.y + q.y * y.x);
+      const double2 c = resp_scalar(re, rs, es, q), bb = rb[qv + t], y = rt[t];
+      r.x += bb.x - (c.x * y.x - c.y * y.y);
+      r.y += bb.y - (c.x * y.y + c.y * y.x);
     }
     const double D = eps[i] - eps[o + a];
-    r1[t] = r;
-    t1_new[t] = make_double2(x.x + r.x / D, x.y + r.y / D);
+    r1[qv + t] = r;
+    t1_new[qv + t] = make_double2(x.x + r.x / D, x.y + r.y / D);
   }
 }
 
-__global__ __launch_bounds__(256) void sp_zero_kernel(int64_t n, double2* __restrict__ a, double2* __restrict__ b) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-    a[t] = make_double2(0.0, 0.0);
-    b[t] = make_double2(0.0, 0.0);
+// zero the first n entries of nb vectors L apart in a and b (the CID singles block)
+__global__ __launch_bounds__(256) void sp_zero_kernel(int64_t n, int64_t nb, int64_t L, double2* __restrict__ a,
+                                                      double2* __restrict__ b) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nb * n; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t at = (u / n) * L + u % n;
+    a[at] = make_double2(0.0, 0.0);
+    b[at] = make_double2(0.0, 0.0);
   }
 }
 
 constexpr int EN_BLOCKS = 256;
 
-// E = sum g[0:ov] t1 + sum L t2 (no conjugation; g[0:ov] = 2 F_ov)
+// E = sum g[0:ov] t1 + sum L t2 (no conjugation; g[0:ov] = 2 F_ov); instance blockIdx.y of vectors
+// `stride` apart, partial sums [instance][EN_BLOCKS]
 __global__ __launch_bounds__(256) void sp_energy_partial(int64_t ov, int singles, const double2* __restrict__ gvec,
-                                                         const double2* __restrict__ L, const double2* __restrict__ t,
-                                                         double2* __restrict__ partial) {
+                                                         const double2* __restrict__ L, const double2* __restrict__ tv,
+                                                         int64_t stride, double2* __restrict__ partial) {
   const int64_t n2 = ov * ov;
+  const double2* t = tv + blockIdx.y * stride;
   double sr = 0.0, si = 0.0;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ov + n2; q += (int64_t)gridDim.x * blockDim.x) {
     double2 w, x = t[q];
@@ -354,10 +381,13 @@ __global__ __launch_bounds__(256) void sp_energy_partial(int64_t ov, int singles
     if ((int)threadIdx.x < w) { hr[threadIdx.x] += hr[threadIdx.x + w]; hi[threadIdx.x] += hi[threadIdx.x + w]; }
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(hr[0], hi[0]);
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(hr[0], hi[0]);
 }
 
-__global__ void sp_energy_final(const double2* __restrict__ partial, int np, double2* out) {
+// out[blockIdx.x] = sum of that instance's np partials
+__global__ __launch_bounds__(256) void sp_energy_final(const double2* __restrict__ partial_all, int np, double2* out_all) {
+  const double2* partial = partial_all + (int64_t)blockIdx.x * np;
+  double2* out = out_all + blockIdx.x;
   __shared__ double hr[256], hi[256];
   double a = 0.0, b = 0.0;
   for (int q = threadIdx.x; q < np; q += blockDim.x) { a += partial[q].x; b += partial[q].y; }
@@ -372,26 +402,32 @@ __global__ void sp_energy_final(const double2* __restrict__ partial, int np, dou
 }
 
 apy_gemm_desc gd(int64_t m, int64_t nn, int64_t k, int al, const void* a, int64_t lda, int bl, const void* b,
-                 int64_t ldb, void* c, int64_t ldc, double alpha, double beta) {
+                 int64_t ldb, void* c, int64_t ldc, double alpha, double beta, int64_t batch = 1, int64_t sa = 0,
+                 int64_t sb = 0, int64_t sc = 0) {
   apy_gemm_desc d{};
   d.m = m; d.n = nn; d.k = k;
   d.a_layout = al; d.b_layout = bl;
   d.alpha_re = alpha; d.beta_re = beta;
   d.a = a; d.lda = lda; d.b = b; d.ldb = ldb; d.c = c; d.ldc = ldc;
-  d.batch = 1;
+  d.batch = batch;
+  if (batch > 1) { d.stride_a = sa; d.stride_b = sb; d.stride_c = sc; }
   return d;
 }
 
+// Workspace regions: each holds nb instances one after another (R, Y, X1, X2, Q1, Q2: (ov)^2 each;
+// TU: 2 (ov)^2; Ts/Ps: so*sv; Ta/Pa: ao*av; acc: ov), then the GEMM scratch.
 struct SpWS {
   size_t R, Y, X1, X2, TU, Ts, Ta, Ps, Pa, Q1, Q2, acc, gemm, total;
 };
 
 constexpr int NSPG = 14;
 
-// The GEMMs of one step in order (pointers null when only sizing).
-int sp_gemms(int64_t o, int64_t v, int singles, const apy_cisd_sp_args* A, char* base, const SpWS* w,
-             apy_gemm_desc g[NSPG]) {
-  const int64_t n = o + v, ov = o * v, oo = o * o, vv = v * v;
+// The GEMMs of one step over nb vectors L apart, in order (pointers null when only sizing).  The
+// ladder and ring GEMMs stack the instances along M; the ones reading the vectors directly run as
+// batched GEMMs with the vector stride.
+int sp_gemms(int64_t o, int64_t v, int singles, int64_t nb, int64_t L, const apy_cisd_sp_args* A, char* base,
+             const SpWS* w, apy_gemm_desc g[NSPG]) {
+  const int64_t n = o + v, ov = o * v, oo = o * o, vv = v * v, d2 = ov * ov;
   auto P = [&](size_t off) -> double2* { return base ? (double2*)(base + off) : nullptr; };
   const double2* t1 = A ? (const double2*)A->t : nullptr;
   const double2* t2 = t1 ? t1 + ov : nullptr;
@@ -403,54 +439,119 @@ int sp_gemms(int64_t o, int64_t v, int singles, const apy_cisd_sp_args* A, char*
   double2* acc = P(w->acc);
   int q = 0;
   const int64_t so = nsym(o), ao = nanti(o), sv = nsym(v), av = nanti(v);
-  g[q++] = gd(so, sv, sv, APY_MK, P(w->Ts), sv, APY_NK, I ? I->vvvs : nullptr, sv, P(w->Ps), sv, 1.0, 0.0);  // P+
+  g[q++] = gd(nb * so, sv, sv, APY_MK, P(w->Ts), sv, APY_NK, I ? I->vvvs : nullptr, sv, P(w->Ps), sv, 1.0, 0.0);  // P+
   if (ao > 0 && av > 0)
-    g[q++] = gd(ao, av, av, APY_MK, P(w->Ta), av, APY_NK, I ? I->vvva : nullptr, av, P(w->Pa), av, 1.0, 0.0);  // P-
-  g[q++] = gd(oo, vv, oo, APY_MK, A ? A->hoo : nullptr, oo, APY_KN, t2, vv, R, vv, 1.0, 0.0);          // HHL + F_oo
-  g[q++] = gd(oo * v, v, v, APY_MK, t2, v, APY_NK, Fvv, n, P(w->Y), v, 1.0, 0.0);                      // F_vv
-  g[q++] = gd(ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, I ? I->w21 : nullptr, ov, P(w->X1), ov, 1.0, 0.0);
-  g[q++] = gd(ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2), ov,
+    g[q++] = gd(nb * ao, av, av, APY_MK, P(w->Ta), av, APY_NK, I ? I->vvva : nullptr, av, P(w->Pa), av, 1.0, 0.0);  // P-
+  g[q++] = gd(oo, vv, oo, APY_MK, A ? A->hoo : nullptr, oo, APY_KN, t2, vv, R, vv, 1.0, 0.0, nb, 0, L, d2);  // HHL+F_oo
+  g[q++] = gd(oo * v, v, v, APY_MK, t2, v, APY_NK, Fvv, n, P(w->Y), v, 1.0, 0.0, nb, L, 0, d2);             // F_vv
+  g[q++] = gd(nb * ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, I ? I->w21 : nullptr, ov, P(w->X1), ov, 1.0, 0.0);
+  g[q++] = gd(nb * ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2), ov,
               1.0, 0.0);
   if (singles) {
-    g[q++] = gd(o, vv * o, v, APY_MK, t1, v, APY_NK, I ? I->vvvo : nullptr, v, P(w->Q1), vv * o, 1.0, 0.0);
-    g[q++] = gd(v * oo, v, o, APY_MK, I ? I->vooo : nullptr, o, APY_KN, t1, v, P(w->Q2), v, 1.0, 0.0);
-    g[q++] = gd(ov, 1, ov, APY_MK, I ? I->s1 : nullptr, ov, APY_KN, t1, 1, acc, 1, 1.0, 0.0);
-    g[q++] = gd(ov, 1, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, A ? A->g : nullptr, 1, acc, 1, 1.0, 1.0);
-    g[q++] = gd(o, v, ov * v, APY_MK, t2, ov * v, APY_NK, I ? I->w4 : nullptr, ov * v, acc, v, 1.0, 1.0);
-    g[q++] = gd(o, v, oo * v, APY_MK, I ? I->w5 : nullptr, oo * v, APY_KN, t2, v, acc, v, -1.0, 1.0);
-    g[q++] = gd(o, v, o, APY_KM, F, n, APY_KN, t1, v, acc, v, -1.0, 1.0);                              // -F_oo^T t1
-    g[q++] = gd(o, v, v, APY_MK, t1, v, APY_NK, Fvv, n, acc, v, 1.0, 1.0);                             // t1 F_vv^T
+    g[q++] = gd(o, vv * o, v, APY_MK, t1, v, APY_NK, I ? I->vvvo : nullptr, v, P(w->Q1), vv * o, 1.0, 0.0, nb, L, 0, d2);
+    g[q++] = gd(v * oo, v, o, APY_MK, I ? I->vooo : nullptr, o, APY_KN, t1, v, P(w->Q2), v, 1.0, 0.0, nb, 0, L, d2);
+    g[q++] = gd(ov, 1, ov, APY_MK, I ? I->s1 : nullptr, ov, APY_KN, t1, 1, acc, 1, 1.0, 0.0, nb, 0, L, ov);
+    g[q++] = gd(ov, 1, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, A ? A->g : nullptr, 1, acc, 1, 1.0, 1.0, nb, 2 * d2, 0, ov);
+    g[q++] = gd(o, v, ov * v, APY_MK, t2, ov * v, APY_NK, I ? I->w4 : nullptr, ov * v, acc, v, 1.0, 1.0, nb, L, 0, ov);
+    g[q++] = gd(o, v, oo * v, APY_MK, I ? I->w5 : nullptr, oo * v, APY_KN, t2, v, acc, v, -1.0, 1.0, nb, 0, L, ov);
+    g[q++] = gd(o, v, o, APY_KM, F, n, APY_KN, t1, v, acc, v, -1.0, 1.0, nb, 0, L, ov);   // -F_oo^T t1
+    g[q++] = gd(o, v, v, APY_MK, t1, v, APY_NK, Fvv, n, acc, v, 1.0, 1.0, nb, L, 0, ov);  // t1 F_vv^T
   }
   return q;
 }
 
-SpWS sp_carve(int64_t o, int64_t v, int singles) {
-  const size_t z = 16, ov = (size_t)(o * v), d2 = ov * ov * z;
+SpWS sp_carve(int64_t o, int64_t v, int singles, int64_t nb) {
+  const size_t z = 16, ov = (size_t)(o * v), d2 = ov * ov * z, B = (size_t)nb;
   auto rnd = [](size_t b) { return (b + 255) / 256 * 256; };
   SpWS w{};
   size_t off = 0;
-  w.R = off; off += rnd(d2);
-  w.Y = off; off += rnd(d2);
-  w.X1 = off; off += rnd(d2);
-  w.X2 = off; off += rnd(d2);
-  w.TU = off; off += rnd(2 * d2);
+  w.R = off; off += rnd(B * d2);
+  w.Y = off; off += rnd(B * d2);
+  w.X1 = off; off += rnd(B * d2);
+  w.X2 = off; off += rnd(B * d2);
+  w.TU = off; off += rnd(B * 2 * d2);
   const size_t so = (size_t)nsym(o), ao = (size_t)nanti(o), sv = (size_t)nsym(v), av = (size_t)nanti(v);
-  w.Ts = off; off += rnd(so * sv * z);
-  w.Ta = off; off += rnd(ao * av * z + z);
-  w.Ps = off; off += rnd(so * sv * z);
-  w.Pa = off; off += rnd(ao * av * z + z);
+  w.Ts = off; off += rnd(B * so * sv * z);
+  w.Ta = off; off += rnd(B * ao * av * z + z);
+  w.Ps = off; off += rnd(B * so * sv * z);
+  w.Pa = off; off += rnd(B * ao * av * z + z);
   if (singles) {
-    w.Q1 = off; off += rnd(d2);
-    w.Q2 = off; off += rnd(d2);
-    w.acc = off; off += rnd(ov * z);
+    w.Q1 = off; off += rnd(B * d2);
+    w.Q2 = off; off += rnd(B * d2);
+    w.acc = off; off += rnd(B * ov * z);
   }
   w.gemm = off;
   apy_gemm_desc g[NSPG];
-  const int q = sp_gemms(o, v, singles, nullptr, nullptr, &w, g);
+  const int q = sp_gemms(o, v, singles, nb, (int64_t)(ov + ov * ov), nullptr, nullptr, &w, g);
   size_t m = 0;
   for (int k = 0; k < q; ++k) m = std::max(m, apy_zgemm_ws_bytes(&g[k]));
   w.total = off + std::max(m, (size_t)EN_BLOCKS * z);
   return w;
+}
+
+// [p, p + n) and [q, q + m) overlap (byte ranges)
+bool overlap(const void* p, size_t n, const void* q, size_t m) {
+  const char *a = (const char*)p, *b = (const char*)q;
+  return a && b && a < b + m && b < a + n;
+}
+
+int sp_step_impl(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, void* workspace, size_t ws_bytes,
+                 void* stream) {
+  if (!A || !A->ints || !A->fock || !A->eps || !A->hoo || !A->g || !A->t || !A->t_new || !A->out || !A->energy ||
+      nb < 1)
+    return APY_E_ARG;
+  const int64_t o = A->o, v = A->v;
+  const apy_sp_ints* I = A->ints;
+  if (o < 1 || v < 1 || I->o != o || I->v != v) return APY_E_ARG;
+  const int64_t ov = o * v, L = ov + ov * ov;
+  const size_t vb = (size_t)(nb * L) * 16, one = (size_t)L * 16;
+  // t_new may be t itself (in-place update); no other overlap among the vectors written and read
+  if (overlap(A->out, vb, A->t, vb) || overlap(A->t_new, vb, A->out, vb)) return APY_E_ARG;
+  if (A->t_new != A->t && overlap(A->t_new, vb, A->t, vb)) return APY_E_ARG;
+  if (A->resp_b && (!A->resp_t || overlap(A->resp_b, vb, A->out, vb) || overlap(A->resp_b, vb, A->t_new, vb) ||
+                    overlap(A->resp_t, one, A->out, vb) || overlap(A->resp_t, one, A->t_new, vb)))
+    return APY_E_ARG;
+  const int singles = A->singles ? 1 : 0;
+  if (!I->vvvs || (v > 1 && !I->vvva) || !I->w21 || !I->w3 || !I->vvoo) return APY_E_ARG;
+  if (singles && (!I->vvvo || !I->vooo || !I->s1 || !I->w4 || !I->w5)) return APY_E_ARG;
+  const SpWS w = sp_carve(o, v, singles, nb);
+  if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  const double2* t = (const double2*)A->t;
+  sp_tu_kernel<<<apy_grid1d(nb * 2 * ov * ov), 256, 0, st>>>(o, v, nb, L, t, (double2*)(base + w.TU));
+  APY_LAUNCH_CHECK();
+  sp_pack_kernel<<<apy_grid1d(nb * ov * ov), 256, 0, st>>>(o, v, nb, L, t, (double2*)(base + w.Ts),
+                                                            (double2*)(base + w.Ta));
+  APY_LAUNCH_CHECK();
+  apy_gemm_desc g[NSPG];
+  const int q = sp_gemms(o, v, singles, nb, L, A, base, &w, g);
+  for (int k = 0; k < q; ++k) {
+    const int s = apy_zgemm_launch(&g[k], base + w.gemm, ws_bytes - w.gemm, st);
+    if (s != APY_OK) return s;
+  }
+  const double2* rb = (const double2*)A->resp_b;
+  const double2* rt = (const double2*)A->resp_t;
+  const double2* re = (const double2*)A->resp_energy;
+  const double2* es = (const double2*)resp_shift;
+  const double2 rs = make_double2(A->resp_shift_re, A->resp_shift_im);
+  SpEpi e{o, v, nb, L, A->eps, (const double2*)I->vvoo, (const double2*)(base + w.Ps), (const double2*)(base + w.Pa),
+          (const double2*)(base + w.R), (const double2*)(base + w.Y),
+          (const double2*)(base + w.X1), (const double2*)(base + w.X2),
+          singles ? (const double2*)(base + w.Q1) : nullptr, singles ? (const double2*)(base + w.Q2) : nullptr,
+          t + ov, (const double2*)A->energy, (double2*)A->t_new + ov, (double2*)A->out + ov,
+          rb ? rb + ov : nullptr, rb ? rt + ov : nullptr, re, rs, es};
+  sp_epi_doubles<<<apy_grid1d(nb * ov * ov), 256, 0, st>>>(e);
+  APY_LAUNCH_CHECK();
+  if (singles) {
+    sp_epi_singles<<<apy_grid1d(nb * ov), 256, 0, st>>>(o, v, nb, L, A->eps, (const double2*)A->fock,
+                                                        (const double2*)(base + w.acc), t, (const double2*)A->energy,
+                                                        (double2*)A->t_new, (double2*)A->out, rb, rt, re, rs, es);
+  } else {  // CID keeps the singles block of the vectors at zero
+    sp_zero_kernel<<<apy_grid1d(nb * ov), 256, 0, st>>>(ov, nb, L, (double2*)A->t_new, (double2*)A->out);
+  }
+  APY_LAUNCH_CHECK();
+  return APY_OK;
 }
 
 }  // namespace
@@ -496,72 +597,41 @@ int apy_cisd_sp_prepare(int64_t o, int64_t v, const void* fock, const apy_sp_int
 
 size_t apy_cisd_sp_workspace(int64_t o, int64_t v, int singles) {
   if (o < 1 || v < 1) return 0;
-  return sp_carve(o, v, singles ? 1 : 0).total;
+  return sp_carve(o, v, singles ? 1 : 0, 1).total;
 }
 
 int apy_cisd_sp_step(const apy_cisd_sp_args* A, void* workspace, size_t ws_bytes, void* stream) {
-  if (!A || !A->ints || !A->fock || !A->eps || !A->hoo || !A->g || !A->t || !A->t_new || !A->out || !A->energy)
-    return APY_E_ARG;
-  const int64_t o = A->o, v = A->v;
-  const apy_sp_ints* I = A->ints;
-  if (o < 1 || v < 1 || I->o != o || I->v != v) return APY_E_ARG;
-  if (A->out == A->t || A->t_new == A->out) return APY_E_ARG;
-  if (A->resp_b && (!A->resp_t || A->resp_b == A->out || A->resp_b == A->t_new || A->resp_t == A->out ||
-                    A->resp_t == A->t_new))
-    return APY_E_ARG;
-  const int singles = A->singles ? 1 : 0;
-  if (!I->vvvs || (v > 1 && !I->vvva) || !I->w21 || !I->w3 || !I->vvoo) return APY_E_ARG;
-  if (singles && (!I->vvvo || !I->vooo || !I->s1 || !I->w4 || !I->w5)) return APY_E_ARG;
-  const SpWS w = sp_carve(o, v, singles);
-  if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
-  hipStream_t st = (hipStream_t)stream;
-  char* base = (char*)workspace;
-  const int64_t ov = o * v;
-  const double2* t = (const double2*)A->t;
-  sp_tu_kernel<<<apy_grid1d(2 * ov * ov), 256, 0, st>>>(o, v, t + ov, (double2*)(base + w.TU));
-  APY_LAUNCH_CHECK();
-  sp_pack_kernel<<<apy_grid1d(ov * ov), 256, 0, st>>>(o, v, t + ov, (double2*)(base + w.Ts), (double2*)(base + w.Ta));
-  APY_LAUNCH_CHECK();
-  apy_gemm_desc g[NSPG];
-  const int q = sp_gemms(o, v, singles, A, base, &w, g);
-  for (int k = 0; k < q; ++k) {
-    const int s = apy_zgemm_launch(&g[k], base + w.gemm, ws_bytes - w.gemm, st);
-    if (s != APY_OK) return s;
-  }
-  const double2* rb = (const double2*)A->resp_b;
-  const double2* rt = (const double2*)A->resp_t;
-  const double2 rs = make_double2(A->resp_shift_re, A->resp_shift_im);
-  SpEpi e{o, v, A->eps, (const double2*)I->vvoo, (const double2*)(base + w.Ps), (const double2*)(base + w.Pa),
-          (const double2*)(base + w.R), (const double2*)(base + w.Y),
-          (const double2*)(base + w.X1), (const double2*)(base + w.X2),
-          singles ? (const double2*)(base + w.Q1) : nullptr, singles ? (const double2*)(base + w.Q2) : nullptr,
-          t + ov, (const double2*)A->energy, (double2*)A->t_new + ov, (double2*)A->out + ov,
-          rb ? rb + ov : nullptr, rb ? rt + ov : nullptr, (const double2*)A->resp_energy, rs};
-  sp_epi_doubles<<<apy_grid1d(ov * ov), 256, 0, st>>>(e);
-  APY_LAUNCH_CHECK();
-  if (singles) {
-    sp_epi_singles<<<apy_grid1d(ov), 256, 0, st>>>(o, v, A->eps, (const double2*)A->fock,
-                                                   (const double2*)(base + w.acc), t, (const double2*)A->energy,
-                                                   (double2*)A->t_new, (double2*)A->out, rb, rt,
-                                                   (const double2*)A->resp_energy, rs);
-  } else {  // CID keeps the singles block of the vector at zero
-    sp_zero_kernel<<<apy_grid1d(ov), 256, 0, st>>>(ov, (double2*)A->t_new, (double2*)A->out);
-  }
-  APY_LAUNCH_CHECK();
-  return APY_OK;
+  return sp_step_impl(A, 1, nullptr, workspace, ws_bytes, stream);
+}
+
+size_t apy_cisd_sp_batch_workspace(int64_t o, int64_t v, int singles, int64_t nb) {
+  if (o < 1 || v < 1 || nb < 1) return 0;
+  return sp_carve(o, v, singles ? 1 : 0, nb).total;
+}
+
+int apy_cisd_sp_step_batch(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, void* workspace,
+                           size_t ws_bytes, void* stream) {
+  return sp_step_impl(A, nb, resp_shift, workspace, ws_bytes, stream);
 }
 
 size_t apy_cisd_sp_energy_workspace(void) { return (size_t)EN_BLOCKS * 16; }
 
 int apy_cisd_sp_energy(int64_t o, int64_t v, int singles, const void* g, const void* L, const void* t, void* energy,
                        void* workspace, size_t ws_bytes, void* stream) {
-  if (o < 1 || v < 1 || !L || !t || !energy || (singles && !g)) return APY_E_ARG;
-  if (!workspace || ws_bytes < apy_cisd_sp_energy_workspace()) return APY_E_WORKSPACE;
+  return apy_cisd_sp_energy_batch(o, v, singles, 1, g, L, t, energy, workspace, ws_bytes, stream);
+}
+
+int apy_cisd_sp_energy_batch(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L,
+                             const void* t, void* energy, void* workspace, size_t ws_bytes, void* stream) {
+  if (o < 1 || v < 1 || nb < 1 || nb > 65535 || !L || !t || !energy || (singles && !g)) return APY_E_ARG;
+  if (!workspace || ws_bytes < (size_t)nb * apy_cisd_sp_energy_workspace()) return APY_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
-  sp_energy_partial<<<EN_BLOCKS, 256, 0, st>>>(o * v, singles ? 1 : 0, (const double2*)g, (const double2*)L,
-                                               (const double2*)t, (double2*)workspace);
+  const int64_t ov = o * v;
+  sp_energy_partial<<<dim3(EN_BLOCKS, (unsigned)nb), 256, 0, st>>>(ov, singles ? 1 : 0, (const double2*)g,
+                                                                    (const double2*)L, (const double2*)t,
+                                                                    ov + ov * ov, (double2*)workspace);
   APY_LAUNCH_CHECK();
-  sp_energy_final<<<1, 256, 0, st>>>((const double2*)workspace, EN_BLOCKS, (double2*)energy);
+  sp_energy_final<<<(unsigned)nb, 256, 0, st>>>((const double2*)workspace, EN_BLOCKS, (double2*)energy);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }

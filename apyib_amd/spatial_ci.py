@@ -167,3 +167,159 @@ class ResponseCI(SpatialCI):
         energy derivative, dt1/dt2 views of the device amplitude vector."""
         E, dt1, dt2, it = super().solve(params, print_level, max_diis, e_offset)
         return E + self.shift, dt1, dt2, it
+
+
+class ResponseSet:
+    """First-order CISD / CID amplitudes of several perturbations of one
+    converged spin-adapted solution, iterated together: each iteration is
+    one apy_cisd_sp_step_batch over all perturbations (the ladder and ring
+    GEMMs stacked along M), then per perturbation the reference's DIIS
+    extrapolation (utils.py:104-140) and convergence test
+    (analytic_aats.py:869-882); a perturbation whose test has passed keeps
+    its amplitudes from then on, as the reference's loop over perturbations
+    would leave them.  Each perturbation's equations and guess are those of
+    ResponseCI.
+
+    ints, fock, eps, singles, t, E: the unperturbed problem and solution;
+    add(dints, dfock) registers one perturbation (its residual of t is
+    formed at once, so dints can be released afterwards)."""
+
+    def __init__(self, ints, fock, eps, singles, t, E):
+        self.base = SpatialCI(ints, fock, eps, singles)
+        self.lib = self.base.lib
+        self.singles = bool(singles)
+        self.device = self.base.device
+        self.O, self.V, self.L, self.OV = self.base.O, self.base.V, self.base.L, self.base.OV
+        self.t_ref = t.contiguous()
+        self.E = complex(E)
+        self.zero = torch.zeros(1, dtype=CDT, device=self.device)
+        scratch = self.base.new_vector()
+        self.drv = self.base.new_vector()
+        SpatialCI.step(self.base, self.base.new_vector(), scratch, self.drv, energy=self.zero)
+        self.b, self.shift = [], []
+
+    def add(self, dints, dfock):
+        pert = SpatialCI(dints, dfock, self.base.eps, self.singles)
+        scratch, r = self.base.new_vector(), self.base.new_vector()
+        pert.step(self.t_ref, scratch, r, energy=self.zero)
+        b = self.base.new_vector()
+        self.base.combine([r, self.drv], [1.0, -1.0], b)
+        self.b.append(b)
+        self.shift.append(complex(pert.energy(self.t_ref, torch.zeros(1, dtype=CDT, device=self.device)).cpu()[0]))
+
+    def _step(self, x, x_new, out, s):
+        nb = x.shape[0]
+        base = self.base
+        args = _lib.CisdSpArgs(self.O, self.V, base.fock.data_ptr(), base.eps.data_ptr(),
+                               ctypes.pointer(base.ints.struct), base.hoo.data_ptr(), base.g.data_ptr(),
+                               int(self.singles), self.E_arr.data_ptr(), x.data_ptr(), x_new.data_ptr(),
+                               out.data_ptr(), self.B.data_ptr(), self.t_ref.data_ptr(), s.data_ptr(), 0.0, 0.0)
+        nbytes = int(self.lib.apy_cisd_sp_batch_workspace(self.O, self.V, int(self.singles), nb))
+        ws = workspace(nbytes, self.device)
+        _lib.check(self.lib.apy_cisd_sp_step_batch(ctypes.byref(args), nb, _vp(self.shift_dev), _vp(ws), nbytes,
+                                                   stream_of(self.device)), "apy_cisd_sp_step_batch")
+
+    def _energies(self, x, out):
+        nb = x.shape[0]
+        nbytes = nb * int(self.lib.apy_cisd_sp_energy_workspace())
+        ws = workspace(nbytes, self.device)
+        _lib.check(self.lib.apy_cisd_sp_energy_batch(self.O, self.V, int(self.singles), nb, _vp(self.base.g),
+                                                     _vp(self.base.ints["l"]), _vp(x), _vp(out), _vp(ws), nbytes,
+                                                     stream_of(self.device)), "apy_cisd_sp_energy_batch")
+
+    def solve(self, params, print_level=0, max_diis=7):
+        """Returns (dE [nb] complex host array, dt1 [nb,o,v] or None,
+        dt2 [nb,o,o,v,v] device views, iterations per perturbation)."""
+        from .so_ci import _converged
+        nb, L, dev = len(self.b), self.L, self.device
+        self.B = torch.stack(self.b)
+        self.b = []
+        self.shift_dev = torch.tensor(self.shift, dtype=CDT, device=dev)
+        self.E_arr = torch.full((nb,), self.E, dtype=CDT, device=dev)
+        diis = bool(params['DIIS'])
+        nvec = max_diis + 1
+        X = torch.zeros((nb, L), dtype=CDT, device=dev)
+        Xn = torch.zeros((nb, L), dtype=CDT, device=dev)
+        s = torch.zeros(nb, dtype=CDT, device=dev)
+        s_new = torch.zeros(nb, dtype=CDT, device=dev)
+        R = torch.zeros((nb, L), dtype=CDT, device=dev)
+        self._step(Xn, X, R, s)                        # guess: one step from dt = 0 (s = 0)
+        Xn.copy_(X)
+        self._energies(X, s)
+        E = s.cpu().numpy()
+        if diis:
+            Th = torch.zeros((nvec, nb, L), dtype=CDT, device=dev)
+            Eh = torch.zeros((nvec, nb, L), dtype=CDT, device=dev)
+        hist = [[] for _ in range(nb)]
+        Bfull = [dict() for _ in range(nb)]
+        scal = torch.zeros((nb, _lib.APY_MAXVEC), dtype=CDT, device=dev)
+        dsq = torch.zeros((nb, 2), dtype=CDT, device=dev)
+        active = [True] * nb
+        iters = [0] * nb
+        red = int(self.lib.apy_reduce_workspace(L, _lib.APY_MAXVEC))
+        off = (ctypes.c_int64 * 3)(0, self.OV, L)
+        it = 0
+        while any(active) and it < params['max_iterations']:
+            it += 1
+            act = [q for q in range(nb) if active[q]]
+            if diis:
+                slot = (it - 1) % nvec
+                self._step(X, Th[slot], Eh[slot], s)
+                for q in act:
+                    while len(hist[q]) > max_diis:
+                        hist[q].pop(0)
+                    hist[q].append((it, slot))
+                    xs = [Eh[h, q] for _, h in hist[q]]
+                    arr = (ctypes.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+                    ws = workspace(red, dev)
+                    _lib.check(self.lib.apy_zdot_many(len(xs), arr, _vp(Eh[slot, q]), L, 1, _vp(scal[q]), _vp(ws),
+                                                      red, stream_of(dev)), "apy_zdot_many")
+                rows = scal.cpu().numpy()
+                for q in act:
+                    for (itm, _), bq in zip(hist[q], rows[q]):
+                        Bfull[q][(itm, it)] = bq
+                        Bfull[q][(it, itm)] = np.conj(bq)
+                    first = hist[q][0][0]
+                    for key in [k for k in Bfull[q] if k[0] < first or k[1] < first]:
+                        del Bfull[q][key]
+                    k = len(hist[q])
+                    Bm = np.zeros((k + 1, k + 1), dtype=np.complex128)
+                    for a_, (ia, _) in enumerate(hist[q]):
+                        for b_, (ib, _) in enumerate(hist[q]):
+                            Bm[a_, b_] = Bfull[q][(ia, ib)]
+                    Bm[-1, :] = -1
+                    Bm[:, -1] = -1
+                    Bm[-1, -1] = 0
+                    rhs = np.zeros(k + 1)
+                    rhs[-1] = -1
+                    c = np.linalg.solve(Bm, rhs)
+                    self.base.combine([Th[h, q] for _, h in hist[q]], c[:-1], Xn[q])
+            else:
+                self._step(X, Xn, R, s)
+            self._energies(Xn, s_new)
+            for q in act:
+                ws = workspace(red, dev)
+                _lib.check(self.lib.apy_zdiffsq_segments(_vp(X[q]), _vp(Xn[q]), 2, off, _vp(dsq[q]), _vp(ws), red,
+                                                         stream_of(dev)), "apy_zdiffsq_segments")
+            E_new = s_new.cpu().numpy()
+            d = dsq.cpu().numpy()
+            for q in act:
+                X[q].copy_(Xn[q])
+                s[q:q + 1].copy_(s_new[q:q + 1])
+                rms1, rms2 = np.sqrt(d[q, 0]), np.sqrt(d[q, 1])
+                dE = E[q] - E_new[q]
+                E[q] = E_new[q]
+                iters[q] = it
+                if print_level > 0:
+                    print(" %02d %3d %20.12f %20.12f %20.12f" % (it, q, (E[q] + self.shift[q]).real, dE.real,
+                                                                 rms2.real))
+                if _converged(it, dE, [rms1, rms2] if self.singles else [rms2], params):
+                    active[q] = False
+            if not diis:
+                Xn.copy_(X)
+        if any(active):
+            print("Not converged.")
+        dE = E + np.asarray(self.shift)
+        O, V = self.O, self.V
+        dt1 = X[:, :self.OV].view(nb, O, V) if self.singles else None
+        return dE, dt1, X[:, self.OV:].view(nb, O, O, V, V), iters

@@ -212,6 +212,23 @@ size_t apy_cisd_sp_energy_workspace(void);
 int apy_cisd_sp_energy(int64_t o, int64_t v, int singles, const void* g, const void* L, const void* t, void* energy,
                        void* workspace, size_t ws_bytes, void* stream);
 
+/* The same step for nb instances sharing fock, eps, ints, hoo and g (the
+ * reference solves one perturbation at a time, analytic_aats.py:780-890,
+ * :1027-1143; here the first-order amplitudes of all perturbations of a
+ * molecule iterate together):  a->t, a->t_new, a->out and a->resp_b point
+ * to nb vectors of the amplitude layout stored one after another;
+ * a->energy and a->resp_energy to nb device complex scalars; resp_shift to
+ * nb device complex scalars added to the response scalar (may be null);
+ * a->resp_t is one vector shared by all instances.  The ladder and ring
+ * GEMMs stack the instances along M.  nb = 1 is apy_cisd_sp_step. */
+size_t apy_cisd_sp_batch_workspace(int64_t o, int64_t v, int singles, int64_t nb);
+int apy_cisd_sp_step_batch(const apy_cisd_sp_args* a, int64_t nb, const void* resp_shift, void* workspace,
+                           size_t ws_bytes, void* stream);
+/* energy[q] of nb vectors t stored one after another (1 <= nb <= 65535);
+ * workspace nb * apy_cisd_sp_energy_workspace() bytes. */
+int apy_cisd_sp_energy_batch(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L,
+                             const void* t, void* energy, void* workspace, size_t ws_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * CISD / CID in spin orbitals: one iteration of the reference's Jacobi
  * step, or the plain sigma product (for Davidson).

@@ -92,3 +92,93 @@ def test_response_solve_matches_oracle(dev, singles):
     assert np.abs(g2.cpu().numpy() - w2).max() < 1e-9 * max(1.0, np.abs(w2).max())
     if singles:
         assert np.abs(g1.cpu().numpy() - w1).max() < 1e-9 * max(1.0, np.abs(w1).max())
+
+
+def _args(ci, x, xn, out, energy, rb, rt, re):
+    from apyib_amd import _lib
+    import ctypes
+    return _lib.CisdSpArgs(ci.O, ci.V, ci.fock.data_ptr(), ci.eps.data_ptr(), ctypes.pointer(ci.ints.struct),
+                           ci.hoo.data_ptr(), ci.g.data_ptr(), int(ci.singles), energy.data_ptr(), x.data_ptr(),
+                           xn.data_ptr(), out.data_ptr(), rb.data_ptr(), rt.data_ptr(), re.data_ptr(), 0.0, 0.0)
+
+
+@pytest.mark.parametrize("singles", [True, False])
+def test_batched_step_and_energy_match_single_calls(dev, singles):
+    """apy_cisd_sp_step_batch over 3 instances (own vectors, response
+    vectors, energies, response scalars and shifts) against three
+    apy_cisd_sp_step calls; apy_cisd_sp_energy_batch against
+    apy_cisd_sp_energy."""
+    import ctypes
+    from apyib_amd import _lib
+    from apyib_amd.kernels import stream_of, workspace
+    lib = _lib.load()
+    o, v, nb = 4, 11, 3
+    E4, F, eps = _system(o, v, 8)
+    ci = SpatialCI(SPIntegrals.from_mo(_d(E4, dev), o, singles=singles), _d(F, dev), eps, singles=singles)
+    L = ci.L
+
+    def vec(seed):
+        a1, a2 = _amps(o, v, seed)
+        return _d(_vec(a1 if singles else 0 * a1, a2), dev)
+
+    X = torch.stack([vec(40 + q) for q in range(nb)])
+    B = torch.stack([vec(50 + q) for q in range(nb)])
+    rt = vec(60)
+    en = torch.tensor([0.01 - 0.02j, -0.03 + 0.001j, 0.2], dtype=torch.complex128, device=dev)
+    re = torch.tensor([0.004j, -0.01, 0.02 + 0.02j], dtype=torch.complex128, device=dev)
+    sh = torch.tensor([0.5, -0.25j, 0.1 + 0.1j], dtype=torch.complex128, device=dev)
+    Xn, R = torch.zeros_like(X), torch.zeros_like(X)
+    a = _args(ci, X, Xn, R, en, B, rt, re)
+    nbytes = int(lib.apy_cisd_sp_batch_workspace(o, v, int(singles), nb))
+    ws = workspace(nbytes, dev)
+    assert lib.apy_cisd_sp_step_batch(ctypes.byref(a), nb, ctypes.c_void_p(sh.data_ptr()),
+                                      ctypes.c_void_p(ws.data_ptr()), nbytes, stream_of(dev)) == 0
+    for q in range(nb):
+        xn, r = torch.zeros(L, dtype=torch.complex128, device=dev), torch.zeros(L, dtype=torch.complex128, device=dev)
+        req = re[q:q + 1] + sh[q:q + 1]
+        a1 = _args(ci, X[q], xn, r, en[q:q + 1].clone(), B[q].clone(), rt, req)
+        n1 = int(lib.apy_cisd_sp_workspace(o, v, int(singles)))
+        w1 = workspace(n1, dev)
+        assert lib.apy_cisd_sp_step(ctypes.byref(a1), ctypes.c_void_p(w1.data_ptr()), n1, stream_of(dev)) == 0
+        scale = max(1.0, r.abs().max().item())
+        assert (R[q] - r).abs().max().item() < 1e-13 * scale
+        assert (Xn[q] - xn).abs().max().item() < 1e-12 * max(1.0, xn.abs().max().item())
+    energies = torch.zeros(nb, dtype=torch.complex128, device=dev)
+    nb_bytes = nb * int(lib.apy_cisd_sp_energy_workspace())
+    ws = workspace(nb_bytes, dev)
+    assert lib.apy_cisd_sp_energy_batch(o, v, int(singles), nb, ctypes.c_void_p(ci.g.data_ptr()),
+                                        ctypes.c_void_p(ci.ints["l"].data_ptr()), ctypes.c_void_p(X.data_ptr()),
+                                        ctypes.c_void_p(energies.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                                        nb_bytes, stream_of(dev)) == 0
+    for q in range(nb):
+        e1 = ci.energy(X[q], torch.zeros(1, dtype=torch.complex128, device=dev)).item()
+        assert abs(energies[q].item() - e1) < 1e-13 * max(1.0, abs(e1))
+
+
+@pytest.mark.parametrize("singles", [True, False])
+def test_response_set_matches_single_solves(dev, singles):
+    """Three perturbations iterated as one ResponseSet against three
+    ResponseCI solves: same projected derivatives, amplitudes and iteration
+    counts."""
+    from apyib_amd.spatial_ci import ResponseSet
+    o, v, seed = 4, 9, 5
+    E4, F, eps = _system(o, v, seed)
+    params = dict(PARAMS)
+    ints = SPIntegrals.from_mo(_d(E4, dev), o, singles=singles)
+    ci = SpatialCI(ints, _d(F, dev), eps, singles=singles)
+    E, _, _, _ = ci.solve(params)
+    t_ref = ci.t.clone()
+    perts = [_perturbation(o, v, seed + 20 + k, scale=0.02) for k in range(3)]
+    rs = ResponseSet(ints, _d(F, dev), eps, singles, t_ref, E)
+    for dE4, dF in perts:
+        rs.add(SPIntegrals.from_mo(_d(dE4, dev), o, singles=singles), _d(dF, dev))
+    dEs, g1, g2, its = rs.solve(params)
+    for k, (dE4, dF) in enumerate(perts):
+        r = ResponseCI(ints, _d(F, dev), eps, singles, t_ref, E,
+                       SPIntegrals.from_mo(_d(dE4, dev), o, singles=singles), _d(dF, dev))
+        dE, w1, w2, it = r.solve(params)
+        assert its[k] == it
+        assert abs(dEs[k] - dE) < 1e-11 * max(1.0, abs(dE))
+        assert (g2[k] - w2).abs().max().item() < 1e-10 * max(1.0, w2.abs().max().item())
+        if singles:
+            assert (g1[k] - w1).abs().max().item() < 1e-10 * max(1.0, w1.abs().max().item())
