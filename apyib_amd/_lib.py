@@ -177,6 +177,10 @@ SIGNATURES = {
     "apy_zdiffsq_segments": (c_int, [c_vp, c_vp, c_int, _P_I64, c_vp, c_vp, c_size, c_vp]),
     "apy_zcombine": (c_int, [c_int, _P_VP, _P_DBL, c_i64, c_vp, c_vp]),
     "apy_reduce_workspace": (c_size, [c_i64, c_int]),
+    "apy_reduce_seg_workspace": (c_size, [c_i64, c_int]),
+    "apy_zdot_many_seg": (c_int, [c_int, _P_VP, c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_size, c_vp]),
+    "apy_zdiffsq_batch": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_size, c_vp]),
+    "apy_zcombine_seg": (c_int, [c_int, _P_VP, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "apy_mp2_so_batched_workspace": (c_size, [c_int, ctypes.POINTER(Mp2Instance)]),
     "apy_mp2_so_batched": (c_int, [c_int, ctypes.POINTER(Mp2Instance), c_vp, c_vp, c_size, c_vp]),
     "apy_mp2_so": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),

@@ -14,9 +14,14 @@ struct CoefList {
   double re[APY_MAXVEC], im[APY_MAXVEC];
 };
 
+// Segment blockIdx.y covers elements [seg * blockIdx.y, seg * blockIdx.y + len) of every vector; its
+// partial sums go to partial[blockIdx.y][block][m] (one segment: the plain many-vector dot).
 template <int K>
-__global__ __launch_bounds__(256) void dot_many_partial(VecList xs, const double2* __restrict__ y, int64_t len,
-                                                        int conj_x, double2* __restrict__ partial) {
+__global__ __launch_bounds__(256) void dot_many_partial(VecList xs, const double2* __restrict__ yv0, int64_t len,
+                                                        int64_t seg, int conj_x, double2* __restrict__ partial_all) {
+  const int64_t s0 = seg * blockIdx.y;
+  const double2* y = yv0 + s0;
+  double2* partial = partial_all + (int64_t)blockIdx.y * gridDim.x * K;
   double ar[K], ai[K];
 #pragma unroll
   for (int m = 0; m < K; ++m) { ar[m] = 0.0; ai[m] = 0.0; }
@@ -25,7 +30,7 @@ __global__ __launch_bounds__(256) void dot_many_partial(VecList xs, const double
     const double2 yv = y[l];
 #pragma unroll
     for (int m = 0; m < K; ++m) {
-      const double2 x = xs.p[m][l];
+      const double2 x = xs.p[m][s0 + l];
       const double xi = cs * x.y;
       ar[m] += x.x * yv.x - xi * yv.y;
       ai[m] += x.x * yv.y + xi * yv.x;
@@ -46,7 +51,10 @@ __global__ __launch_bounds__(256) void dot_many_partial(VecList xs, const double
   }
 }
 
-__global__ void finalize_many(const double2* __restrict__ partial, int nb, int k, double2* out) {
+// segment blockIdx.x: out[blockIdx.x][m] = sum of its nb partials
+__global__ void finalize_many(const double2* __restrict__ partial_all, int nb, int k, double2* out_all) {
+  const double2* partial = partial_all + (int64_t)blockIdx.x * nb * k;
+  double2* out = out_all + (int64_t)blockIdx.x * k;
   __shared__ double sr[256], si[256];
   for (int m = 0; m < k; ++m) {
     double a = 0.0, b = 0.0;
@@ -63,8 +71,17 @@ __global__ void finalize_many(const double2* __restrict__ partial, int nb, int k
   }
 }
 
+// sum (x - y)^2 (no conjugation) over [beg, end); with seglen > 0, blockIdx.y = 2 q + h selects
+// [q seglen, q seglen + split) (h = 0) or [q seglen + split, (q + 1) seglen) (h = 1) instead
 __global__ __launch_bounds__(256) void diffsq_partial(const double2* __restrict__ x, const double2* __restrict__ y,
-                                                      int64_t beg, int64_t end, double2* __restrict__ partial) {
+                                                      int64_t beg, int64_t end, int64_t seglen, int64_t split,
+                                                      double2* __restrict__ partial_all) {
+  if (seglen > 0) {
+    const int64_t q = blockIdx.y / 2, h = blockIdx.y % 2;
+    beg = q * seglen + (h ? split : 0);
+    end = q * seglen + (h ? seglen : split);
+  }
+  double2* partial = partial_all + (int64_t)blockIdx.y * gridDim.x;
   double a = 0.0, b = 0.0;
   for (int64_t l = beg + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < end; l += (int64_t)gridDim.x * blockDim.x) {
     const double dr = x[l].x - y[l].x, di = x[l].y - y[l].y;
@@ -91,6 +108,24 @@ __global__ __launch_bounds__(256) void combine_kernel(VecList xs, CoefList c, in
       const double2 x = xs.p[m][l];
       a += c.re[m] * x.x - c.im[m] * x.y;
       b += c.re[m] * x.y + c.im[m] * x.x;
+    }
+    y[l] = make_double2(a, b);
+  }
+}
+
+// y[q seglen + l] = sum_m c[q][m] x_m[q seglen + l], coefficients [nseg][K] in device memory
+template <int K>
+__global__ __launch_bounds__(256) void combine_seg_kernel(VecList xs, const double2* __restrict__ c, int64_t seglen,
+                                                          int64_t total, double2* __restrict__ y) {
+  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < total; l += (int64_t)gridDim.x * blockDim.x) {
+    const double2* cq = c + (l / seglen) * K;
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      const double2 x = xs.p[m][l];
+      const double2 cm = cq[m];
+      a += cm.x * x.x - cm.y * x.y;
+      b += cm.x * x.y + cm.y * x.x;
     }
     y[l] = make_double2(a, b);
   }
@@ -136,8 +171,13 @@ void launch_ci_dot(VecList v, const double2* y, int64_t ov, int64_t len, double2
 }
 
 template <int K>
-void launch_dot(VecList v, const double2* y, int64_t len, int conj, double2* part, hipStream_t s) {
-  dot_many_partial<K><<<NB, 256, 0, s>>>(v, y, len, conj, part);
+void launch_dot(VecList v, const double2* y, int64_t len, int64_t nseg, int conj, double2* part, hipStream_t s) {
+  dot_many_partial<K><<<dim3(NB, (unsigned)nseg), 256, 0, s>>>(v, y, len, len, conj, part);
+}
+
+template <int K>
+void launch_comb_seg(VecList v, const double2* c, int64_t seglen, int64_t nseg, double2* y, hipStream_t s) {
+  combine_seg_kernel<K><<<apy_grid1d(seglen * nseg, 8192), 256, 0, s>>>(v, c, seglen, seglen * nseg, y);
 }
 
 template <int K>
@@ -167,7 +207,7 @@ int apy_zdot_many(int k, const void* const* xs, const void* y, int64_t len, int 
   hipStream_t s = (hipStream_t)stream;
   double2* part = (double2*)workspace;
   switch (k) {
-#define C(K) case K: launch_dot<K>(v, (const double2*)y, len, conj_x, part, s); break;
+#define C(K) case K: launch_dot<K>(v, (const double2*)y, len, 1, conj_x, part, s); break;
     C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
 #undef C
   }
@@ -208,7 +248,7 @@ int apy_zdiffsq_segments(const void* x, const void* y, int nseg, const int64_t* 
   double2* part = (double2*)workspace;
   for (int q = 0; q < nseg; ++q) {
     if (off[q + 1] < off[q]) return APY_E_ARG;
-    diffsq_partial<<<NB, 256, 0, s>>>((const double2*)x, (const double2*)y, off[q], off[q + 1], part);
+    diffsq_partial<<<NB, 256, 0, s>>>((const double2*)x, (const double2*)y, off[q], off[q + 1], 0, 0, part);
     APY_LAUNCH_CHECK();
     finalize_many<<<1, 256, 0, s>>>(part, NB, 1, (double2*)out + q);
     APY_LAUNCH_CHECK();
@@ -229,6 +269,67 @@ int apy_zcombine(int k, const void* const* xs, const double* coef, int64_t len, 
   hipStream_t s = (hipStream_t)stream;
   switch (k) {
 #define C(K) case K: launch_comb<K>(v, c, len, (double2*)y, s); break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
+#undef C
+  }
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+size_t apy_reduce_seg_workspace(int64_t nseg, int k) {
+  if (k < 1) k = 1;
+  if (nseg < 1) nseg = 1;
+  return (size_t)nseg * NB * (size_t)k * sizeof(double2);
+}
+
+int apy_zdot_many_seg(int k, const void* const* xs, const void* y, int64_t nseg, int64_t seglen, int conj_x,
+                      void* out, void* workspace, size_t ws_bytes, void* stream) {
+  if (k < 1 || k > APY_MAXVEC || !xs || !y || !out || seglen < 0 || nseg < 1 || nseg > 65535) return APY_E_ARG;
+  if (!workspace || ws_bytes < apy_reduce_seg_workspace(nseg, k)) return APY_E_WORKSPACE;
+  VecList v{};
+  for (int m = 0; m < k; ++m) {
+    if (!xs[m]) return APY_E_ARG;
+    v.p[m] = (const double2*)xs[m];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  double2* part = (double2*)workspace;
+  switch (k) {
+#define C(K) case K: launch_dot<K>(v, (const double2*)y, seglen, nseg, conj_x, part, s); break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
+#undef C
+  }
+  APY_LAUNCH_CHECK();
+  finalize_many<<<(unsigned)nseg, 256, 0, s>>>(part, NB, k, (double2*)out);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_zdiffsq_batch(const void* x, const void* y, int64_t nseg, int64_t seglen, int64_t split, void* out,
+                      void* workspace, size_t ws_bytes, void* stream) {
+  if (!x || !y || !out || nseg < 1 || 2 * nseg > 65535 || seglen < 1 || split < 0 || split > seglen)
+    return APY_E_ARG;
+  if (!workspace || ws_bytes < apy_reduce_seg_workspace(2 * nseg, 1)) return APY_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double2* part = (double2*)workspace;
+  diffsq_partial<<<dim3(NB, (unsigned)(2 * nseg)), 256, 0, s>>>((const double2*)x, (const double2*)y, 0, 0, seglen,
+                                                                split, part);
+  APY_LAUNCH_CHECK();
+  finalize_many<<<(unsigned)(2 * nseg), 256, 0, s>>>(part, NB, 1, (double2*)out);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_zcombine_seg(int k, const void* const* xs, const void* coef, int64_t nseg, int64_t seglen, void* y,
+                     void* stream) {
+  if (k < 1 || k > APY_MAXVEC || !xs || !coef || !y || nseg < 1 || seglen < 1) return APY_E_ARG;
+  VecList v{};
+  for (int m = 0; m < k; ++m) {
+    if (!xs[m]) return APY_E_ARG;
+    v.p[m] = (const double2*)xs[m];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  switch (k) {
+#define C(K) case K: launch_comb_seg<K>(v, (const double2*)coef, seglen, nseg, (double2*)y, s); break;
     C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
 #undef C
   }

@@ -229,9 +229,17 @@ class ResponseSet:
 
     def solve(self, params, print_level=0, max_diis=7):
         """Returns (dE [nb] complex host array, dt1 [nb,o,v] or None,
-        dt2 [nb,o,o,v,v] device views, iterations per perturbation)."""
+        dt2 [nb,o,o,v,v] device views, iterations per perturbation).
+
+        Every perturbation's DIIS history holds the same iterations (they
+        start together and each keeps the newest max_diis+1), so one
+        segmented call forms all new rows of the DIIS matrices, one the
+        extrapolated amplitudes and one the convergence sums; each
+        perturbation's sums are bit-identical to its own solve's."""
         from .so_ci import _converged
         nb, L, dev = len(self.b), self.L, self.device
+        lib = self.lib
+        st = stream_of(dev)
         self.B = torch.stack(self.b)
         self.b = []
         self.shift_dev = torch.tensor(self.shift, dtype=CDT, device=dev)
@@ -250,62 +258,60 @@ class ResponseSet:
         if diis:
             Th = torch.zeros((nvec, nb, L), dtype=CDT, device=dev)
             Eh = torch.zeros((nvec, nb, L), dtype=CDT, device=dev)
-        hist = [[] for _ in range(nb)]
-        Bfull = [dict() for _ in range(nb)]
+        hist = []                                      # (iteration, slot), shared by the perturbations
+        Bs = np.zeros((nb, nvec, nvec), dtype=np.complex128)
         scal = torch.zeros((nb, _lib.APY_MAXVEC), dtype=CDT, device=dev)
         dsq = torch.zeros((nb, 2), dtype=CDT, device=dev)
-        active = [True] * nb
+        active = np.ones(nb, dtype=bool)
         iters = [0] * nb
-        red = int(self.lib.apy_reduce_workspace(L, _lib.APY_MAXVEC))
-        off = (ctypes.c_int64 * 3)(0, self.OV, L)
+        red = int(lib.apy_reduce_seg_workspace(2 * nb, _lib.APY_MAXVEC))
         it = 0
-        while any(active) and it < params['max_iterations']:
+        while active.any() and it < params['max_iterations']:
             it += 1
-            act = [q for q in range(nb) if active[q]]
             if diis:
                 slot = (it - 1) % nvec
                 self._step(X, Th[slot], Eh[slot], s)
-                for q in act:
-                    while len(hist[q]) > max_diis:
-                        hist[q].pop(0)
-                    hist[q].append((it, slot))
-                    xs = [Eh[h, q] for _, h in hist[q]]
-                    arr = (ctypes.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
-                    ws = workspace(red, dev)
-                    _lib.check(self.lib.apy_zdot_many(len(xs), arr, _vp(Eh[slot, q]), L, 1, _vp(scal[q]), _vp(ws),
-                                                      red, stream_of(dev)), "apy_zdot_many")
-                rows = scal.cpu().numpy()
-                for q in act:
-                    for (itm, _), bq in zip(hist[q], rows[q]):
-                        Bfull[q][(itm, it)] = bq
-                        Bfull[q][(it, itm)] = np.conj(bq)
-                    first = hist[q][0][0]
-                    for key in [k for k in Bfull[q] if k[0] < first or k[1] < first]:
-                        del Bfull[q][key]
-                    k = len(hist[q])
-                    Bm = np.zeros((k + 1, k + 1), dtype=np.complex128)
-                    for a_, (ia, _) in enumerate(hist[q]):
-                        for b_, (ib, _) in enumerate(hist[q]):
-                            Bm[a_, b_] = Bfull[q][(ia, ib)]
-                    Bm[-1, :] = -1
-                    Bm[:, -1] = -1
-                    Bm[-1, -1] = 0
-                    rhs = np.zeros(k + 1)
-                    rhs[-1] = -1
-                    c = np.linalg.solve(Bm, rhs)
-                    self.base.combine([Th[h, q] for _, h in hist[q]], c[:-1], Xn[q])
+                while len(hist) > max_diis:
+                    hist.pop(0)
+                hist.append((it, slot))
+                k = len(hist)
+                arr = (ctypes.c_void_p * k)(*[Eh[h].data_ptr() for _, h in hist])
+                ws = workspace(red, dev)
+                _lib.check(lib.apy_zdot_many_seg(k, arr, _vp(Eh[slot]), nb, L, 1, _vp(scal), _vp(ws), red, st),
+                           "apy_zdot_many_seg")
+                rows = scal.view(-1)[:nb * k].cpu().numpy().reshape(nb, k)      # packed [nb][k]
+                slots = [h for _, h in hist]
+                Bs[:, slots, slot] = rows                                         # E^H E by slot (utils.py:121)
+                Bs[:, slot, slots] = np.conj(rows)
+                act = np.flatnonzero(active)
+                Bm = np.full((len(act), k + 1, k + 1), -1.0, dtype=np.complex128)
+                Bm[:, :k, :k] = Bs[act][:, slots][:, :, slots]
+                Bm[:, k, k] = 0
+                rhs = np.zeros((len(act), k + 1, 1), dtype=np.complex128)
+                rhs[:, k] = -1
+                coef = np.zeros((nb, k), dtype=np.complex128)
+                coef[:, -1] = 1.0                      # finished perturbations: unused
+                coef[act] = np.linalg.solve(Bm, rhs)[:, :k, 0]                    # utils.py:135
+                cdev = torch.from_numpy(coef).to(dev)
+                arr = (ctypes.c_void_p * k)(*[Th[h].data_ptr() for _, h in hist])
+                _lib.check(lib.apy_zcombine_seg(k, arr, _vp(cdev), nb, L, _vp(Xn), st), "apy_zcombine_seg")
             else:
                 self._step(X, Xn, R, s)
             self._energies(Xn, s_new)
-            for q in act:
-                ws = workspace(red, dev)
-                _lib.check(self.lib.apy_zdiffsq_segments(_vp(X[q]), _vp(Xn[q]), 2, off, _vp(dsq[q]), _vp(ws), red,
-                                                         stream_of(dev)), "apy_zdiffsq_segments")
+            ws = workspace(red, dev)
+            _lib.check(lib.apy_zdiffsq_batch(_vp(X), _vp(Xn), nb, L, self.OV, _vp(dsq), _vp(ws), red, st),
+                       "apy_zdiffsq_batch")
             E_new = s_new.cpu().numpy()
             d = dsq.cpu().numpy()
+            act = np.flatnonzero(active)
+            if len(act) == nb:
+                X.copy_(Xn)
+                s.copy_(s_new)
+            else:
+                for q in act:
+                    X[q].copy_(Xn[q])
+                    s[q:q + 1].copy_(s_new[q:q + 1])
             for q in act:
-                X[q].copy_(Xn[q])
-                s[q:q + 1].copy_(s_new[q:q + 1])
                 rms1, rms2 = np.sqrt(d[q, 0]), np.sqrt(d[q, 1])
                 dE = E[q] - E_new[q]
                 E[q] = E_new[q]
@@ -315,9 +321,7 @@ class ResponseSet:
                                                                  rms2.real))
                 if _converged(it, dE, [rms1, rms2] if self.singles else [rms2], params):
                     active[q] = False
-            if not diis:
-                Xn.copy_(X)
-        if any(active):
+        if active.any():
             print("Not converged.")
         dE = E + np.asarray(self.shift)
         O, V = self.O, self.V

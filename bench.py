@@ -330,6 +330,103 @@ def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10):
     return res
 
 
+def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=2, h=1e-5):
+    """First-order CISD amplitudes of `nb` perturbations (the 3N+3 field
+    and nuclear perturbations of a 4-atom molecule in the analytic AAT) of
+    one spin-adapted CISD solution at o=10, v=80: seeded chemists' integrals
+    at coupling scale `scale` (small enough for the reference's Jacobi
+    iteration to converge), seeded pair-symmetric derivative integrals and
+    antisymmetric derivative Fock matrices, all generated on the device.
+    Legs: the batched set (ResponseSet), the same solves one at a time
+    (ResponseCI), and central differences (two CISD solves with H +- h dH,
+    integral blocks included) timed on `nfd` perturbations and scaled to
+    `nb`; the differences also check the projected derivatives."""
+    import numpy as np
+    import torch
+    from apyib_amd.spatial_ci import ResponseCI, ResponseSet, SPIntegrals, SpatialCI
+
+    def sync_time(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    n = o + v
+    g = torch.Generator(device=dev)
+
+    def pair_sym(k, amp):
+        g.manual_seed(seed * 1000 + k)
+        x = torch.randn((n,) * 4, generator=g, device=dev, dtype=torch.float64)
+        y = torch.randn((n,) * 4, generator=g, device=dev, dtype=torch.float64)
+        e = amp * torch.complex(x, y)
+        return 0.5 * (e + e.permute(2, 3, 0, 1))
+
+    params = dict(e_convergence=1e-10, d_convergence=1e-10, DIIS=True, max_iterations=100)
+    rng = np.random.default_rng(seed + 77)
+    eps = np.concatenate([np.sort(rng.uniform(-1.5, -0.3, o)), np.sort(rng.uniform(0.2, 3.0, v))])
+    F = torch.from_numpy(np.diag(eps).astype(complex)).to(dev)
+    E4 = pair_sym(0, scale)
+    ints = SPIntegrals.from_mo(E4, o)
+    ci = SpatialCI(ints, F, eps)
+    (E, _, _, it_cisd), t_cisd = sync_time(lambda: ci.solve(params))
+    t_ref = ci.t.clone()
+
+    def pert(k):
+        dE4 = pair_sym(k + 1, scale)
+        g.manual_seed(seed * 1000 + 500 + k)
+        a = 0.02 * torch.randn((n, n), generator=g, device=dev, dtype=torch.float64)
+        return dE4, torch.complex(a - a.T, torch.zeros_like(a))
+
+    def build_set():
+        rs = ResponseSet(ints, F, eps, True, t_ref, E)
+        for k in range(nb):
+            dE4, dF = pert(k)
+            rs.add(SPIntegrals.from_mo(dE4, o), dF)
+        return rs
+    rs, t_setup = sync_time(build_set)
+    (dE_set, _, _, its), t_set = sync_time(lambda: rs.solve(params))
+    del rs
+
+    def one_by_one():
+        out = []
+        for k in range(nb):
+            dE4, dF = pert(k)
+            r = ResponseCI(ints, F, eps, True, t_ref, E, SPIntegrals.from_mo(dE4, o), dF)
+            out.append(r.solve(params)[0])
+        return out
+    dE_seq, t_seq = sync_time(one_by_one)
+
+    def fd():
+        out = []
+        for k in range(nfd):
+            dE4, dF = pert(k)
+            Es = []
+            for sg in (1.0, -1.0):
+                c = SpatialCI(SPIntegrals.from_mo(E4 + sg * h * dE4, o), F + sg * h * dF, eps)
+                Es.append(c.solve(params)[0])
+            out.append((Es[0] - Es[1]) / (2 * h))
+        return out
+    dE_fd, t_fd = sync_time(fd)
+    err_fd = max(abs(a - b) / max(1.0, abs(b)) for a, b in zip(dE_set[:nfd], dE_fd))
+    err_seq = max(abs(a - b) / max(1.0, abs(b)) for a, b in zip(dE_set, dE_seq))
+    fd_scaled = t_fd / nfd * nb
+    res = {"workload": f"first-order CISD amplitudes of {nb} perturbations of one spin-adapted CISD solution "
+                       f"(analytic AAT response, analytic_aats.py:733-890), o={o} v={v}, coupling scale {scale}",
+           "cisd_solve_ms": round(t_cisd * 1e3, 1), "cisd_iterations": it_cisd,
+           "set_setup_ms": round(t_setup * 1e3, 1), "set_solve_ms": round(t_set * 1e3, 1),
+           "set_iterations_max": int(max(its)),
+           "one_by_one_ms": round(t_seq * 1e3, 1),
+           "central_differences_ms": round(fd_scaled * 1e3, 1),
+           "central_differences_measured": f"{nfd} perturbations, scaled to {nb}",
+           "speedup_set_vs_one_by_one": round(t_seq / (t_setup + t_set), 2),
+           "speedup_set_vs_differences": round(fd_scaled / (t_setup + t_set), 2),
+           "max_rel_diff_set_vs_one_by_one": float(err_seq),
+           "max_rel_diff_vs_differences": float(err_fd)}
+    log(f"analytic response: {res}")
+    return res
+
+
 def mp2_adjoint_set_measurement(dev, seed, O=30, V=270, ninst=8, ndir=200, nfd=2, h=1e-5, tol=1e-10):
     """MP2 adjoint set: `ninst` orbital-invariant spin-orbital MP2 instances
     at O=30, V=270 (only <ij||ab>, <ab||ij> and F), each with `ndir` seeded
@@ -440,6 +537,8 @@ def main():
     ap.add_argument("--no-mp2-adjoint-set", action="store_true",
                     help="skip the MP2 adjoint set (8 instances O=30 V=270)")
     ap.add_argument("--no-spatial", action="store_true", help="skip the spin-adapted CISD step timing")
+    ap.add_argument("--no-analytic-response", action="store_true",
+                    help="skip the first-order CISD amplitude set timing")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -524,6 +623,7 @@ def main():
         if not args.no_mp2_adjoint_set:
             aat_set["D"] = mp2_adjoint_set_measurement(dev, args.seed)
     spatial = None if args.no_spatial else spatial_cisd_measurement(dev, args.seed)
+    response = None if args.no_analytic_response else analytic_response_measurement(dev, args.seed)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -568,6 +668,7 @@ def main():
         "cpu_baseline": cpu,
         "aat_set": aat_set,
         "spatial_cisd": spatial,
+        "analytic_response": response,
     }
     print(json.dumps(out), flush=True)
 

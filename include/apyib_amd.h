@@ -477,6 +477,24 @@ int apy_zdiffsq_segments(const void* x, const void* y, int nseg, const int64_t* 
 int apy_zcombine(int k, const void* const* xs, const double* coef, int64_t len, void* y, void* stream);
 size_t apy_reduce_workspace(int64_t len, int k);
 
+/* The same for nseg segments of seglen elements stored one after another
+ * (one DIIS per segment, utils.py:104-140, for sets of problems iterated
+ * together); each segment's sums are bit-identical to the single-vector
+ * calls on it.
+ *   apy_zdot_many_seg:  out[s][m] = sum_l conj?(x_m[s][l]) y[s][l]
+ *   apy_zdiffsq_batch:  out[s][0], out[s][1] = sum (x-y)^2 over [0, split)
+ *                       and [split, seglen) of segment s
+ *   apy_zcombine_seg:   y[s] = sum_m c[s][m] x_m[s], c [nseg][k] complex in
+ *                       device memory
+ * Workspace apy_reduce_seg_workspace(nseg, k) (nseg -> 2 nseg for diffsq). */
+size_t apy_reduce_seg_workspace(int64_t nseg, int k);
+int apy_zdot_many_seg(int k, const void* const* xs, const void* y, int64_t nseg, int64_t seglen, int conj_x,
+                      void* out, void* workspace, size_t ws_bytes, void* stream);
+int apy_zdiffsq_batch(const void* x, const void* y, int64_t nseg, int64_t seglen, int64_t split, void* out,
+                      void* workspace, size_t ws_bytes, void* stream);
+int apy_zcombine_seg(int k, const void* const* xs, const void* coef, int64_t nseg, int64_t seglen, void* y,
+                     void* stream);
+
 /* ------------------------------------------------------------------------
  * Finite-difference atomic axial tensors (replaces AAT.compute_all_dets,
  * aats.py:558-642, and the determinant-amplitude sums of

@@ -182,3 +182,51 @@ def test_response_set_matches_single_solves(dev, singles):
         assert (g2[k] - w2).abs().max().item() < 1e-10 * max(1.0, w2.abs().max().item())
         if singles:
             assert (g1[k] - w1).abs().max().item() < 1e-10 * max(1.0, w1.abs().max().item())
+
+
+def test_segmented_reductions_are_bitwise_the_single_calls(dev):
+    """apy_zdot_many_seg, apy_zdiffsq_batch and apy_zcombine_seg on 5
+    segments against apy_zdot_many, apy_zdiffsq_segments and apy_zcombine
+    on each segment (bit-identical), and the dots against NumPy."""
+    import ctypes
+    from apyib_amd import _lib
+    from apyib_amd.kernels import stream_of, workspace
+    lib = _lib.load()
+    nseg, seglen, split, k = 5, 70001, 1234, 3
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    xs = [torch.randn(nseg * seglen, generator=g, device=dev, dtype=torch.complex128) for _ in range(k)]
+    y = torch.randn(nseg * seglen, generator=g, device=dev, dtype=torch.complex128)
+    st = stream_of(dev)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    red = int(lib.apy_reduce_seg_workspace(2 * nseg, k))
+    ws = workspace(red, dev)
+    arr = (ctypes.c_void_p * k)(*[x.data_ptr() for x in xs])
+    out = torch.zeros((nseg, k), dtype=torch.complex128, device=dev)
+    assert lib.apy_zdot_many_seg(k, arr, P(y), nseg, seglen, 1, P(out), P(ws), red, st) == 0
+    dsq = torch.zeros((nseg, 2), dtype=torch.complex128, device=dev)
+    assert lib.apy_zdiffsq_batch(P(xs[0]), P(y), nseg, seglen, split, P(dsq), P(ws), red, st) == 0
+    coef = torch.randn((nseg, k), generator=g, device=dev, dtype=torch.complex128)
+    comb = torch.zeros(nseg * seglen, dtype=torch.complex128, device=dev)
+    assert lib.apy_zcombine_seg(k, arr, P(coef), nseg, seglen, P(comb), st) == 0
+    out, dsq, comb = out.cpu().numpy(), dsq.cpu().numpy(), comb.cpu()
+    ch = coef.cpu().numpy()
+    one = int(lib.apy_reduce_workspace(seglen, k))
+    for q in range(nseg):
+        sl = slice(q * seglen, (q + 1) * seglen)
+        segs = [x[sl] for x in xs]
+        a1 = (ctypes.c_void_p * k)(*[x.data_ptr() for x in segs])
+        o1 = torch.zeros(k, dtype=torch.complex128, device=dev)
+        w1 = workspace(max(one, red), dev)
+        assert lib.apy_zdot_many(k, a1, P(y[sl]), seglen, 1, P(o1), P(w1), one, st) == 0
+        assert np.array_equal(o1.cpu().numpy(), out[q])
+        want = np.array([np.vdot(x.cpu().numpy(), y[sl].cpu().numpy()) for x in segs])
+        assert np.abs(out[q] - want).max() < 1e-10 * np.abs(want).max()
+        off = (ctypes.c_int64 * 3)(0, split, seglen)
+        d1 = torch.zeros(2, dtype=torch.complex128, device=dev)
+        assert lib.apy_zdiffsq_segments(P(xs[0][sl]), P(y[sl]), 2, off, P(d1), P(w1), one, st) == 0
+        assert np.array_equal(d1.cpu().numpy(), dsq[q])
+        c1 = np.ascontiguousarray(ch[q]).view(np.float64)
+        y1 = torch.zeros(seglen, dtype=torch.complex128, device=dev)
+        assert lib.apy_zcombine(k, a1, c1.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), seglen, P(y1), st) == 0
+        assert torch.equal(y1.cpu(), comb[sl])
