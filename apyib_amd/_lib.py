@@ -67,6 +67,17 @@ class SpInts(ctypes.Structure):
     _fields_ = [("o", c_i64), ("v", c_i64)] + [(b, c_vp) for b in SP_BLOCKS]
 
 
+APY_GTO_MAXPRIM = 6
+
+
+class Gto(ctypes.Structure):
+    _fields_ = [
+        ("atom", ctypes.c_int32), ("nprim", ctypes.c_int32), ("nterm", ctypes.c_int32),
+        ("lmn", (ctypes.c_int32 * 3) * 2), ("pad", ctypes.c_int32),
+        ("exps", c_dbl * APY_GTO_MAXPRIM), ("coefs", (c_dbl * APY_GTO_MAXPRIM) * 2),
+    ]
+
+
 class CisdSpArgs(ctypes.Structure):
     _fields_ = [
         ("o", c_i64), ("v", c_i64), ("fock", c_vp), ("eps", c_vp), ("ints", ctypes.POINTER(SpInts)),
@@ -177,6 +188,8 @@ SIGNATURES = {
     "apy_zdiffsq_segments": (c_int, [c_vp, c_vp, c_int, _P_I64, c_vp, c_vp, c_size, c_vp]),
     "apy_zcombine": (c_int, [c_int, _P_VP, _P_DBL, c_i64, c_vp, c_vp]),
     "apy_reduce_workspace": (c_size, [c_i64, c_int]),
+    "apy_eri_gto": (c_int, [c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
+    "apy_eri_deriv_assemble": (c_int, [c_i64, c_vp, c_int, c_vp, c_vp, c_vp]),
     "apy_reduce_seg_workspace": (c_size, [c_i64, c_int]),
     "apy_zdot_many_seg": (c_int, [c_int, _P_VP, c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_size, c_vp]),
     "apy_zdiffsq_batch": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_size, c_vp]),

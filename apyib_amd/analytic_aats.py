@@ -8,8 +8,10 @@ displacement and magnetic-field component and, for the correlated methods,
 the first-order amplitudes, with no displaced wavefunctions.
 
 Skeleton first-derivative integrals (the reference's psi4 mo_oei_deriv1,
-mo_tei_deriv1 and mo_overlap_half_deriv1) come from the package's host
-integral engine (gaussian.build_deriv1).  On the GPU: the AO -> MO
+mo_tei_deriv1 and mo_overlap_half_deriv1): the one-electron ones from the
+package's host integral engine (gaussian.build_deriv1), the two-electron
+ones on the GPU (integrals.EriDerivatives: apy_eri_gto over the derivative
+functions, apy_eri_deriv_assemble per atom and axis).  On the GPU: the AO -> MO
 transforms of the two-electron integrals and of every derivative integral
 set (apy_eri_mo), the one-electron transforms, the MP2 amplitudes, the
 four U-contractions of the first-order integrals per perturbation (n^5
@@ -26,6 +28,7 @@ import torch
 from . import gaussian
 from . import transforms as tr
 from .hamiltonian import ANGULAR_MOMENTUM_SIGN, Hamiltonian
+from .integrals import EriDerivatives
 from .hf_wfn import hf_wfn
 from .ci_wfn import ci_wfn
 from .kernels import zgemm_raw
@@ -100,7 +103,9 @@ class analytic_derivative(object):
         self.Gm = np.linalg.inv((diag + self.Amag[v, o, v, o]).reshape(nv * no, nv * no))
         charges = list(zip(H.molecule.Z, H.molecule.coords))
         self.natom = len(charges)
-        self.deriv = gaussian.build_deriv1(H.basis_set.functions, H.basis_set.atom_of, charges)
+        self.deriv = gaussian.build_deriv1(H.basis_set.functions, H.basis_set.atom_of, charges,
+                                           eri_derivatives=False)
+        self.dERI = EriDerivatives(H.basis_set, H.molecule.coords, self.dev)
         self.L = ANGULAR_MOMENTUM_SIGN * gaussian.build_angular_momentum(H.basis_set.functions)
         self._ready = True
 
@@ -138,7 +143,7 @@ class analytic_derivative(object):
         o, v, no, nv, nbf = self.o, self.v, self.no, self.nv, self.nbf
         F, A, d = self.F, self.A.swapaxes(1, 2), self.deriv
         T1, V1, S1 = (self._mo1(d[k][N1][a]) for k in ('T', 'V', 'S'))
-        E1d = tr.eri_mo(tr.to_dev(d['ERI'][N1][a], self.dev), self.Cd).permute(0, 2, 1, 3).contiguous()
+        E1d = tr.eri_mo(self.dERI(N1, a), self.Cd).permute(0, 2, 1, 3).contiguous()
         E1 = np.real_if_close(E1d.cpu().numpy())
         F1 = T1 + V1 + np.real_if_close(_c('piqi->pq', 2 * E1[:, o, :, o] - E1.swapaxes(2, 3)[:, o, :, o]))
         B = -F1[v, o] + _c('ai,ii->ai', S1[v, o], F[o, o]) + 0.5 * _c('mn,amin->ai', S1[o, o], A[v, o, o, o])

@@ -341,8 +341,9 @@ def _dsum(fn, terms, *rest):
     return sum(fn(t, *rest) for t in terms)
 
 
-def build_deriv1(basis, atom_of, charges):
-    """First-derivative AO integrals for every atom A and axis x:
+def build_deriv1(basis, atom_of, charges, eri_derivatives=True):
+    """First-derivative AO integrals for every atom A and axis x (the ERI
+    part only with eri_derivatives=True):
         S[A][x], T[A][x], V[A][x]  d/dR_Ax of S, T, V (V with the derivative
                                    of the nuclear attraction operator of A)
         half_S[A][x][m, n] = <d m/dR_Ax | n>  (bra functions only)
@@ -366,8 +367,8 @@ def build_deriv1(basis, atom_of, charges):
                     dVC[C, x, m, k] = _dsum(potential, terms, basis[k], [(Z, R)])
     dV = dVC.sum(0)
     # D[x][m,n,l,s] = (d m/dx n | l s), (l<=s) computed, (l,s) symmetric
-    D = np.zeros((3, n, n, n, n))
-    for x in range(3):
+    D = np.zeros((3, n, n, n, n)) if eri_derivatives else None
+    for x in range(3 if eri_derivatives else 0):
         for m in range(n):
             terms = center_derivative(basis[m], x)
             for q in range(n):
@@ -388,12 +389,13 @@ def build_deriv1(basis, atom_of, charges):
             v = on[:, None] * dV[x]
             op = dVC[A, x] + dVC[A, x].T
             mV.append(v + v.T - op)
-            Dx = D[x]
-            e = (on[:, None, None, None] * Dx
-                 + on[None, :, None, None] * Dx.transpose(1, 0, 2, 3)
-                 + on[None, None, :, None] * Dx.transpose(2, 3, 0, 1)
-                 + on[None, None, None, :] * Dx.transpose(2, 3, 1, 0))
-            mE.append(e)
+            if eri_derivatives:
+                Dx = D[x]
+                e = (on[:, None, None, None] * Dx
+                     + on[None, :, None, None] * Dx.transpose(1, 0, 2, 3)
+                     + on[None, None, :, None] * Dx.transpose(2, 3, 0, 1)
+                     + on[None, None, None, :] * Dx.transpose(2, 3, 1, 0))
+                mE.append(e)
         out['S'].append(mS)
         out['T'].append(mT)
         out['V'].append(mV)

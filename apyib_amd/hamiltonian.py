@@ -30,7 +30,7 @@ class Hamiltonian(object):
         fns = self.basis_set.functions
         charges = list(zip(self.molecule.Z, self.molecule.coords))
         self.S, self.T, self.V = gaussian.build_one_electron(fns, charges)
-        self.ERI = gaussian.build_eri(fns)
+        self._ERI = None
 
         F_el = list(parameters.get('F_el', [0.0, 0.0, 0.0]))
         F_mag = list(parameters.get('F_mag', [0.0, 0.0, 0.0]))
@@ -50,12 +50,21 @@ class Hamiltonian(object):
                 self.V = self.V - F_mag[a] * self.mu_mag[a]
         self._device_cache = {}
 
+    @property
+    def ERI(self):
+        """(mn|ls) on the host (NumPy), built on first use; the GPU path
+        reads device_eri instead."""
+        if self._ERI is None:
+            self._ERI = gaussian.build_eri(self.basis_set.functions)
+        return self._ERI
+
     def device_eri(self, device):
-        """The AO integrals as a complex128 device tensor (uploaded once)."""
-        import torch
+        """The AO integrals as a complex128 device tensor, computed on the
+        device (apy_eri_gto) once per device."""
+        from .integrals import ao_eri
         key = str(device)
         t = self._device_cache.get(key)
         if t is None:
-            t = torch.from_numpy(np.ascontiguousarray(self.ERI, dtype=np.complex128)).to(device)
+            t = ao_eri(self.basis_set, self.molecule.coords, device)
             self._device_cache[key] = t
         return t

@@ -463,6 +463,39 @@ int apy_mp2_nc_batched_energy(int n, int64_t n_items, int use_x_new, double scal
                               const void* workspace, void* stream);
 
 /* ------------------------------------------------------------------------
+ * AO electron-repulsion integrals (replace psi4's MintsHelper.ao_eri of
+ * hamiltonian.py:33 and the two-electron part of mo_tei_deriv1,
+ * analytic_aats.py:94, before its MO transform).
+ * A function is one or two terms sharing a centre (atom) and exponents:
+ *   sum_t sum_p coefs[t][p] x^lmn[t][0] y^lmn[t][1] z^lmn[t][2] e^{-exps[p] r^2}
+ * (coefficients include the primitive normalisation).  Limits: nprim <=
+ * APY_GTO_MAXPRIM, lmn[t] summing to <= 2, quartet total <= 5 (s/p basis
+ * functions and their first derivatives); an entry outside them yields NaN.
+ * ---------------------------------------------------------------------- */
+#define APY_GTO_MAXPRIM 6
+typedef struct apy_gto {
+  int32_t atom;                               /* index into the centres       */
+  int32_t nprim;
+  int32_t nterm;                              /* 1 or 2                       */
+  int32_t lmn[2][3];
+  int32_t pad;                                /* exps at byte offset 40       */
+  double exps[APY_GTO_MAXPRIM];
+  double coefs[2][APY_GTO_MAXPRIM];
+} apy_gto;                                    /* 184 bytes                    */
+
+/* out[g][i][j][k][l] = (fa_i fb_j | fc_k fd_l) at geometry g (chemists'
+ * order, complex128 with zero imaginary part), centres [ngeom][natom][3]
+ * bohr; tables and centres in device memory. */
+int apy_eri_gto(int64_t ngeom, int64_t natom, const double* centers, const apy_gto* fa, int64_t na,
+                const apy_gto* fb, int64_t nb, const apy_gto* fc, int64_t nc, const apy_gto* fd, int64_t nd,
+                void* out, void* stream);
+/* d (mn|ls) / dR_{atom,x} from D[m,n,l,s] = (dm/dx n|l s) (apy_eri_gto with
+ * the x-derivative functions in fa): out = on(m) D[m,n,l,s] + on(n) D[n,m,l,s]
+ * + on(l) D[l,s,m,n] + on(s) D[s,l,m,n], on(p) = (atom_of[p] == atom);
+ * complex128 [n]^4, atom_of int32 [n] in device memory. */
+int apy_eri_deriv_assemble(int64_t n, const int* atom_of, int atom, const void* D, void* out, void* stream);
+
+/* ------------------------------------------------------------------------
  * Vector reductions used by DIIS and the convergence test.
  * ---------------------------------------------------------------------- */
 #define APY_MAXVEC 16

@@ -106,3 +106,18 @@ def test_argument_errors_return_before_any_device_work():
     for rb, rt in ((7, None), (7, 2), (3, 9), (9, 3)):
         args = _lib.CisdSpArgs(3, 5, 1, 1, ctypes.pointer(ints), 1, 1, 1, 1, 1, 2, 3, rb, rt)
         assert lib.apy_cisd_sp_step(ctypes.byref(args), None, 0, None) == E
+
+
+def test_gto_table_layout_and_argument_errors():
+    """apy_gto is 184 bytes with the exponents at byte 40 and the
+    coefficients at 88 (the header's layout); null tables are rejected
+    before any device work."""
+    assert ctypes.sizeof(_lib.Gto) == 184
+    assert _lib.Gto.exps.offset == 40 and _lib.Gto.coefs.offset == 88
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    assert lib.apy_eri_gto(1, 1, None, None, 1, None, 1, None, 1, None, 1, None, None) == _lib.APY_E_ARG
+    assert lib.apy_eri_gto(0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, None) == _lib.APY_E_ARG
+    assert lib.apy_eri_deriv_assemble(4, None, 0, None, None, None) == _lib.APY_E_ARG
+    assert lib.apy_eri_deriv_assemble(4, 1, 0, 8, 8, None) == _lib.APY_E_ARG
