@@ -206,6 +206,7 @@ SIGNATURES = {
     "apy_eri_mo_workspace": (c_size, [c_i64, c_i64]),
     "apy_eri_mo": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_fock_build": (c_int, [c_i64, c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),
+    "apy_fock_build_batch": (c_int, [c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_dbl, c_vp, c_vp]),
     "apy_eri_so_from_mo": (c_int, [c_i64, c_vp, c_vp, c_vp]),
     "apy_mo_overlap_workspace": (c_size, [c_i64, c_i64, c_i64]),
     "apy_mo_overlap": (c_int, [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_size,

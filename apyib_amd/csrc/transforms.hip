@@ -15,9 +15,16 @@ namespace {
 
 // F[m,n] = h[m,n] + sum_{l,s} D[l,s] (2 (mn|ls) - (ml|ns))  (times `scale`
 // on the two-electron part).  One block per (m, n).
-__global__ __launch_bounds__(256) void fock_kernel(int64_t nb, const double2* __restrict__ h,
-                                                   const double2* __restrict__ eri, const double2* __restrict__ D,
-                                                   double scale, double2* __restrict__ F) {
+// Instance blockIdx.y of a batch: h, eri, D and F advance by their strides (eri's may be 0: shared).
+__global__ __launch_bounds__(256) void fock_kernel(int64_t nb, const double2* __restrict__ h0,
+                                                   const double2* __restrict__ eri0, int64_t stride_eri,
+                                                   const double2* __restrict__ D0, double scale,
+                                                   double2* __restrict__ F0) {
+  const int64_t g = blockIdx.y, n2 = nb * nb;
+  const double2* h = h0 ? h0 + g * n2 : nullptr;
+  const double2* eri = eri0 + g * stride_eri;
+  const double2* D = D0 + g * n2;
+  double2* F = F0 + g * n2;
   const int64_t mn = blockIdx.x;
   const int64_t m = mn / nb, n = mn % nb;
   double ar = 0.0, ai = 0.0;
@@ -156,9 +163,14 @@ int apy_eri_so_from_mo(int64_t m, const void* eri_mo, void* eri_so, void* stream
 
 int apy_fock_build(int64_t nbf, const void* h, const void* eri_ao, const void* D, double scale, void* F,
                    void* stream) {
-  if (nbf < 1 || !eri_ao || !D || !F) return APY_E_ARG;
-  fock_kernel<<<(unsigned)(nbf * nbf), 256, 0, (hipStream_t)stream>>>(nbf, (const double2*)h, (const double2*)eri_ao,
-                                                                     (const double2*)D, scale, (double2*)F);
+  return apy_fock_build_batch(nbf, 1, h, eri_ao, 0, D, scale, F, stream);
+}
+
+int apy_fock_build_batch(int64_t nbf, int64_t batch, const void* h, const void* eri_ao, int64_t stride_eri,
+                         const void* D, double scale, void* F, void* stream) {
+  if (nbf < 1 || batch < 1 || batch > 65535 || !eri_ao || !D || !F || stride_eri < 0) return APY_E_ARG;
+  fock_kernel<<<dim3((unsigned)(nbf * nbf), (unsigned)batch), 256, 0, (hipStream_t)stream>>>(
+      nbf, (const double2*)h, (const double2*)eri_ao, stride_eri, (const double2*)D, scale, (double2*)F);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }

@@ -1,9 +1,10 @@
 """Method dispatch with apyib's interface (energy.py:17-150)."""
 from .ci_wfn import ci_wfn
 from .hamiltonian import Hamiltonian
-from .hf_wfn import hf_wfn
+from .hf_wfn import hf_wfn, solve_SCF_batch
+from .integrals import prime_device_eri
 from .mp2_wfn import mp2_wfn
-from .utils import compute_phase
+from .utils import compute_phase, device
 
 
 def _solve(parameters, wfn, print_level):
@@ -68,3 +69,28 @@ def phase_corrected_energy(parameters, unperturbed_basis, unperturbed_C, return_
     if return_H:
         return E_list, T_list, wfn.C, basis, H
     return E_list, T_list, wfn.C, basis
+
+
+def phase_corrected_energies(parameter_list, unperturbed_basis, unperturbed_C, print_level=0):
+    """phase_corrected_energy for a list of parameter sets (the displaced
+    geometries and fields of fin_diff.py:267-372), with the Hartree-Fock
+    part batched: every AO integral set in one launch per basis, the SCFs
+    iterated together (hf_wfn.solve_SCF_batch, one Fock build per iteration
+    for all of them); the phase alignment and the correlated solve then run
+    per instance.  Each result equals phase_corrected_energy's.  Returns
+    [(E_list, T_list, C, basis)]."""
+    Hs = [Hamiltonian(p) for p in parameter_list]
+    prime_device_eri(Hs, device())
+    wfns = [hf_wfn(H) for H in Hs]
+    out = []
+    nbf = {w.nbf for w in wfns}
+    if len(nbf) == 1:
+        scf = solve_SCF_batch(wfns, parameter_list[0], print_level)
+    else:
+        scf = [w.solve_SCF(p, print_level) for w, p in zip(wfns, parameter_list)]
+    for p, H, wfn, (E_SCF, C) in zip(parameter_list, Hs, wfns, scf):
+        basis = H.basis_set
+        wfn.C = compute_phase(wfn.ndocc, wfn.nbf, unperturbed_basis, unperturbed_C, basis, C)
+        E, t1, t2 = _solve(p, wfn, print_level)
+        out.append(([E_SCF, E, H.E_nuc], [1, t1, t2], wfn.C, basis))
+    return out

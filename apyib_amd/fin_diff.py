@@ -4,13 +4,16 @@
 .compute_AAT(nuc_pert_strength, mag_pert_strength)` returns the reference's
 12-tuple (fin_diff.py:267-372): +dR then -dR on every Cartesian coordinate
 (alpha = 3 * atom + xyz, bohr), then +dH then -dH on every component of the
-magnetic field, each a phase_corrected_energy solve on the GPU.
+magnetic field, each a phase_corrected_energy solve on the GPU (the
+Hartree-Fock parts of all of them batched).
 Other finite-difference properties of the reference (Hessian, APT, nuclear
 and field gradients, phase-space AATs) are not part of this path.
 """
+import copy
+
 import numpy as np
 
-from .energy import phase_corrected_energy
+from .energy import phase_corrected_energies, phase_corrected_energy
 from .molecule import Molecule
 
 
@@ -29,34 +32,29 @@ class finite_difference(object):
         return C, T_list, basis
 
     def compute_AAT(self, nuc_pert_strength, mag_pert_strength):
-        """fin_diff.py:267-372."""
-        geom0 = self.parameters['geom']
+        """fin_diff.py:267-372.  The 6N+6 displaced parameter sets are solved
+        as one set (energy.phase_corrected_energies: integrals and SCFs
+        batched on the GPU); every solve equals the reference's one-by-one
+        phase_corrected_energy."""
         self.parameters.setdefault('F_mag', [0.0, 0.0, 0.0])
-        out = {}
-        for sign, key in ((+1, 'nuc_pos'), (-1, 'nuc_neg')):
-            C_l, B_l, T_l = [], [], []
+        plist = []
+        for sign in (+1, -1):
             for alpha in range(3 * self.natom):
                 g = np.copy(self.geom)
                 g[alpha // 3][alpha % 3] += sign * nuc_pert_strength
-                self.parameters['geom'] = self.molecule.with_coords(g).geometry_string()
-                C, T, basis = self._solve()
-                C_l.append(C)
-                T_l.append(T)
-                B_l.append(basis)
-                self.parameters['geom'] = geom0
-            out[key] = (C_l, B_l, T_l)
-        for sign, key in ((+1, 'mag_pos'), (-1, 'mag_neg')):
-            C_l, B_l, T_l = [], [], []
+                q = copy.deepcopy(self.parameters)
+                q['geom'] = self.molecule.with_coords(g).geometry_string()
+                plist.append(q)
+        for sign in (+1, -1):
             for beta in range(3):
-                self.parameters['F_mag'][beta] += sign * mag_pert_strength
-                C, T, basis = self._solve()
-                C_l.append(C)
-                T_l.append(T)
-                B_l.append(basis)
-                self.parameters['F_mag'][beta] -= sign * mag_pert_strength
-            out[key] = (C_l, B_l, T_l)
-        npC, npB, npT = out['nuc_pos']
-        nnC, nnB, nnT = out['nuc_neg']
-        mpC, mpB, mpT = out['mag_pos']
-        mnC, mnB, mnT = out['mag_neg']
+                q = copy.deepcopy(self.parameters)
+                q['F_mag'][beta] += sign * mag_pert_strength
+                plist.append(q)
+        res = phase_corrected_energies(plist, self.unperturbed_basis, self.unperturbed_C)
+        nn = 3 * self.natom
+        parts = [res[:nn], res[nn:2 * nn], res[2 * nn:2 * nn + 3], res[2 * nn + 3:]]
+        out = []
+        for part in parts:
+            out.append(([r[2] for r in part], [r[3] for r in part], [r[1] for r in part]))
+        (npC, npB, npT), (nnC, nnB, nnT), (mpC, mpB, mpT), (mnC, mnB, mnT) = out
         return npC, nnC, npB, nnB, npT, nnT, mpC, mnC, mpB, mnB, mpT, mnT

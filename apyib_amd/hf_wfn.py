@@ -31,50 +31,86 @@ class hf_wfn(object):
 
     def solve_SCF(self, parameters, print_level=0):
         """hf_wfn.py:36-170.  Returns (E_SCF, C)."""
-        H = self.H
-        dev = device()
-        H_core = H.T + H.V
-        X = np.linalg.inv(la.sqrtm(H.S))
-        e, C_p = np.linalg.eigh(X @ H_core @ X)
-        C = X @ C_p
-        D = 2 * C[:, :self.ndocc] @ np.conjugate(C[:, :self.ndocc]).T
-        E_SCF = np.einsum('mn,mn->', D, H_core)
-        eri = H.device_eri(dev)
-        hcore_d = tr.to_dev(H_core, dev)
+        return solve_SCF_batch([self], parameters, print_level)[0]
+
+
+class _SCF:
+    """One instance of the reference's SCF iteration (hf_wfn.py:36-170): the
+    core guess, then per iteration (given the Fock matrix of the current
+    density, hf_wfn.py:81) the DIIS extrapolation, eigh in the orthogonal
+    basis, density, energy and convergence test, on the host."""
+
+    def __init__(self, wfn, parameters, print_level):
+        self.wfn, self.parameters, self.print_level = wfn, parameters, print_level
+        H = wfn.H
+        self.H_core = H.T + H.V
+        self.X = np.linalg.inv(la.sqrtm(H.S))
+        e, C_p = np.linalg.eigh(self.X @ self.H_core @ self.X)
+        C = self.X @ C_p
+        self.D = 2 * C[:, :wfn.ndocc] @ np.conjugate(C[:, :wfn.ndocc]).T
+        self.E = np.einsum('mn,mn->', self.D, self.H_core)
+        self.i = 0
+        self.F_iter = self.e_iter = None
+        self.done = parameters['max_iterations'] < 1
         if print_level > 0:
             print("\n Iter      E_elec(real)       E_elec(imaginary)        E(tot)           Delta_E(real)       "
                   "Delta_E(imaginary)      RMS_D(real)      RMS_D(imaginary)")
-            print(" %02d %20.12f %20.12f %20.12f" % (0, E_SCF.real, E_SCF.imag, E_SCF.real + H.E_nuc))
-        F_iter = e_iter = None
-        for i in range(1, parameters['max_iterations'] + 1):
-            E_old, D_old = E_SCF, D
-            F = tr.fock(hcore_d, eri, tr.to_dev(D, dev), 0.5).cpu().numpy()           # hf_wfn.py:81
-            if not np.iscomplexobj(H_core) and not np.iscomplexobj(D):
-                F = F.real
-            if parameters['DIIS']:
-                SDF = H.S @ D @ F
-                res_vec = np.reshape(X @ (SDF - np.conjugate(SDF.T)) @ X, (-1))
-                F_vec = np.reshape(F, (-1))
-                if i == 1:
-                    F_iter = np.atleast_2d(F_vec).T
-                    e_iter = np.atleast_2d(res_vec).T
-                F_vec, e_iter, F_iter = solve_general_DIIS(parameters, res_vec, F_vec, e_iter, F_iter, i)
-                F = np.reshape(F_vec, (self.nbf, self.nbf))
-            self.eps, C_p = np.linalg.eigh(X @ F @ X)
-            C = self.C = X @ C_p
-            D = 2 * C[:, :self.ndocc] @ np.conjugate(C[:, :self.ndocc]).T
-            E_SCF = 0.5 * np.einsum('nm,mn->', D, H_core + F)
-            delta_E = E_SCF - E_old
-            rms_D = np.sqrt(np.sum((D_old - D) ** 2))
-            if print_level > 0:
-                print(" %02d %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f" % (
-                    i, E_SCF.real, E_SCF.imag, (E_SCF + H.E_nuc).real, delta_E.real, delta_E.imag,
-                    rms_D.real, rms_D.imag))
-            if i > 1 and abs(delta_E) < parameters['e_convergence'] and rms_D < parameters['d_convergence']:
-                break
-            if i == parameters['max_iterations']:
-                if abs(delta_E) > parameters['e_convergence'] or rms_D > parameters['d_convergence']:
-                    if print_level > 0:
-                        print("Not converged.")
-        self.E_SCF = E_SCF
-        return E_SCF, self.C
+            print(" %02d %20.12f %20.12f %20.12f" % (0, self.E.real, self.E.imag, self.E.real + H.E_nuc))
+
+    def update(self, F):
+        wfn, H, X, parameters = self.wfn, self.wfn.H, self.X, self.parameters
+        self.i += 1
+        i = self.i
+        E_old, D_old = self.E, self.D
+        if not np.iscomplexobj(self.H_core) and not np.iscomplexobj(self.D):
+            F = F.real
+        if parameters['DIIS']:
+            SDF = H.S @ self.D @ F
+            res_vec = np.reshape(X @ (SDF - np.conjugate(SDF.T)) @ X, (-1))
+            F_vec = np.reshape(F, (-1))
+            if i == 1:
+                self.F_iter = np.atleast_2d(F_vec).T
+                self.e_iter = np.atleast_2d(res_vec).T
+            F_vec, self.e_iter, self.F_iter = solve_general_DIIS(parameters, res_vec, F_vec, self.e_iter,
+                                                                 self.F_iter, i)
+            F = np.reshape(F_vec, (wfn.nbf, wfn.nbf))
+        wfn.eps, C_p = np.linalg.eigh(X @ F @ X)
+        C = wfn.C = X @ C_p
+        self.D = 2 * C[:, :wfn.ndocc] @ np.conjugate(C[:, :wfn.ndocc]).T
+        self.E = 0.5 * np.einsum('nm,mn->', self.D, self.H_core + F)
+        delta_E = self.E - E_old
+        rms_D = np.sqrt(np.sum((D_old - self.D) ** 2))
+        if self.print_level > 0:
+            print(" %02d %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f" % (
+                i, self.E.real, self.E.imag, (self.E + H.E_nuc).real, delta_E.real, delta_E.imag,
+                rms_D.real, rms_D.imag))
+        if i > 1 and abs(delta_E) < parameters['e_convergence'] and rms_D < parameters['d_convergence']:
+            self.done = True
+        elif i == parameters['max_iterations']:
+            self.done = True
+            if abs(delta_E) > parameters['e_convergence'] or rms_D > parameters['d_convergence']:
+                if self.print_level > 0:
+                    print("Not converged.")
+
+
+def solve_SCF_batch(wfns, parameters, print_level=0):
+    """hf_wfn.solve_SCF for several wavefunctions with the same number of
+    basis functions (the displaced geometries and fields of a
+    finite-difference set), iterated together: each iteration builds the
+    Fock matrices of all of them in one apy_fock_build_batch call
+    (hf_wfn.py:81); DIIS, eigh and the convergence test stay per instance,
+    so every result equals its own solve_SCF.  Returns [(E_SCF, C)]."""
+    dev = device()
+    states = [_SCF(w, parameters, print_level) for w in wfns]
+    eris = [w.H.device_eri(dev) for w in wfns]
+    eri = eris[0] if all(e is eris[0] for e in eris) else torch.stack(eris)
+    hcore = torch.stack([tr.to_dev(st.H_core, dev) for st in states])
+    while not all(st.done for st in states):
+        D = torch.from_numpy(np.stack([np.asarray(st.D, dtype=np.complex128) for st in states])).to(dev)
+        F = tr.fock_batch(hcore, eri, D, 0.5).cpu().numpy()
+        for q, st in enumerate(states):
+            if not st.done:
+                st.update(F[q])
+    for w, st in zip(wfns, states):
+        w.E_SCF = st.E
+    return [(w.E_SCF, w.C) for w in wfns]

@@ -112,3 +112,32 @@ class EriDerivatives:
                                                       ctypes.c_void_p(out.data_ptr()), stream_of(self.device)),
                    "apy_eri_deriv_assemble")
         return out
+
+
+def prime_device_eri(hamiltonians, device):
+    """Fill the device ERI cache of several Hamiltonians with one
+    apy_eri_gto launch per basis (same functions on the same atoms): the
+    distinct geometries are computed together, and Hamiltonians at the same
+    geometry (field displacements) share one tensor."""
+    key = str(device)
+    groups = {}
+    for H in hamiltonians:
+        if key in H._device_cache:
+            continue
+        bs = H.basis_set
+        sig = (bs.name, tuple(int(z) for z in H.molecule.Z))
+        groups.setdefault(sig, []).append(H)
+    for Hs in groups.values():
+        geoms, index = [], []
+        for H in Hs:
+            c = np.asarray(H.molecule.coords, dtype=np.float64)
+            for g, other in enumerate(geoms):
+                if np.array_equal(other, c):
+                    index.append(g)
+                    break
+            else:
+                index.append(len(geoms))
+                geoms.append(c)
+        out = ao_eri(Hs[0].basis_set, np.stack(geoms), device)
+        for H, g in zip(Hs, index):
+            H._device_cache[key] = out[g]

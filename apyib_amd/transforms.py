@@ -47,6 +47,20 @@ def fock(h, eri_ao, D, scale=1.0):
     return F
 
 
+def fock_batch(h, eri_ao, D, scale=1.0):
+    """F[b] = h[b] + scale * sum_ls D[b]_ls (2 (mn|ls)_b - (ml|ns)_b) for a
+    batch: h, D [batch, n, n]; eri_ao [batch, n, n, n, n] or one [n, n, n, n]
+    shared by all."""
+    lib = _lib.load()
+    batch, nbf = D.shape[0], D.shape[1]
+    shared = eri_ao.dim() == 4
+    F = torch.empty((batch, nbf, nbf), dtype=CDT, device=D.device)
+    _lib.check(lib.apy_fock_build_batch(nbf, batch, _vp(h.contiguous()), _vp(eri_ao.contiguous()),
+                                        0 if shared else nbf ** 4, _vp(D.contiguous()), float(scale), _vp(F),
+                                        stream_of(D.device)), "apy_fock_build_batch")
+    return F
+
+
 def density(C_occ):
     """D = C_occ C_occ^H (utils.py:246)."""
     return mm(C_occ, C_occ.t(), conj_b=True)

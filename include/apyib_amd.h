@@ -321,6 +321,13 @@ int apy_eri_so_from_mo(int64_t m, const void* eri_mo, void* eri_so, void* stream
  * scale 1; hf_wfn.py:81 with scale 1/2).  h may be NULL (zero). */
 int apy_fock_build(int64_t nbf, const void* h, const void* eri_ao, const void* D, double scale, void* F,
                    void* stream);
+/* The same for `batch` SCF instances iterated together (the displaced
+ * geometries and fields of fin_diff.py:267-372, each running hf_wfn.py:81):
+ * h, D, F are [batch][nbf][nbf]; eri_ao advances by stride_eri elements per
+ * instance (0: one tensor shared).  Each instance's F is bit-identical to
+ * apy_fock_build on it. */
+int apy_fock_build_batch(int64_t nbf, int64_t batch, const void* h, const void* eri_ao, int64_t stride_eri,
+                         const void* D, double scale, void* F, void* stream);
 /* out_b = C_bra,b^H S_b C_ket,b for b < batch (compute_mo_overlap,
  * utils.py:381); C_* are [nbf][nmo], S [nbf][nbf], out [nmo][nmo]; strides
  * are element counts between batch entries (0 = shared). */
