@@ -427,6 +427,79 @@ def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=
     return res
 
 
+H2O2 = """
+H -1.780954530308296   1.411647335546379   0.872055376436941
+H  1.780954530308296  -1.411647335546379   0.872055376436941
+O -1.371214332646589  -0.115525249760340  -0.054947416764017
+O  1.371214332646589   0.115525249760340  -0.054947416764017
+no_com
+no_reorient
+symmetry c1
+units bohr
+"""
+
+
+def h2o2_aat_measurement(dev, h=1e-4):
+    """End-to-end CISD atomic axial tensor of H2O2/STO-3G (12 basis
+    functions, the molecule of the reference's analytic AAT tests) by the
+    two routes the reference offers: analytic (analytic_aats.py:553-1391:
+    SCF, CISD, CPHF, 15 first-order CI solves as one set) and finite
+    differences (parallel.py:16-48: 30 displaced SCF + CISD solves with the
+    Hartree-Fock parts batched, then the determinant-based AAT).  The 30
+    displaced solves alone are timed batched and one by one.  Both AATs are
+    compared."""
+    import contextlib
+    import copy
+    import numpy as np
+    import torch
+    from apyib_amd.analytic_aats import analytic_derivative
+    from apyib_amd.energy import energy, phase_corrected_energies, phase_corrected_energy
+    from apyib_amd.molecule import Molecule
+    from apyib_amd.parallel import compute_parallel_aats
+
+    params = {'geom': H2O2, 'basis': 'STO-3G', 'method': 'CISD', 'e_convergence': 1e-13, 'd_convergence': 1e-13,
+              'DIIS': True, 'freeze_core': False, 'F_el': [0.0, 0.0, 0.0], 'F_mag': [0.0, 0.0, 0.0],
+              'max_iterations': 120}
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with contextlib.redirect_stdout(sys.stderr):
+            r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    timed(lambda: energy(copy.deepcopy(params)))                      # warm the kernels and plans
+    I_an, t_an = timed(lambda: analytic_derivative(copy.deepcopy(params)).compute_CISD_AATs('full', 'non-canonical'))
+    I_fd, t_fd = timed(lambda: compute_parallel_aats(copy.deepcopy(params), h, h, 'full'))
+
+    (_, _, C0, b0), _ = timed(lambda: energy(copy.deepcopy(params)))
+    mol = Molecule.from_string(H2O2)
+    plist = []
+    for sign in (1, -1):
+        for alpha in range(3 * mol.natom()):
+            g = mol.coords.copy()
+            g[alpha // 3][alpha % 3] += sign * h
+            q = copy.deepcopy(params)
+            q['geom'] = mol.with_coords(g).geometry_string()
+            plist.append(q)
+    for sign in (1, -1):
+        for beta in range(3):
+            q = copy.deepcopy(params)
+            q['F_mag'][beta] += sign * h
+            plist.append(q)
+    _, t_seq = timed(lambda: [phase_corrected_energy(q, b0, C0) for q in plist])
+    _, t_bat = timed(lambda: phase_corrected_energies(plist, b0, C0))
+    diff = float(np.abs(np.asarray(I_an) - np.asarray(I_fd)).max())
+    res = {"workload": "CISD atomic axial tensor of H2O2/STO-3G (12 basis functions), analytic vs finite "
+                       f"differences (h = {h})",
+           "analytic_s": round(t_an, 3), "finite_differences_s": round(t_fd, 3),
+           "displaced_solves_one_by_one_s": round(t_seq, 3), "displaced_solves_batched_s": round(t_bat, 3),
+           "max_abs_diff_analytic_vs_differences": diff, "max_abs_aat": float(np.abs(I_an).max())}
+    log(f"h2o2 aat: {res}")
+    return res
+
+
 def mp2_adjoint_set_measurement(dev, seed, O=30, V=270, ninst=8, ndir=200, nfd=2, h=1e-5, tol=1e-10):
     """MP2 adjoint set: `ninst` orbital-invariant spin-orbital MP2 instances
     at O=30, V=270 (only <ij||ab>, <ab||ij> and F), each with `ndir` seeded
@@ -539,6 +612,8 @@ def main():
     ap.add_argument("--no-spatial", action="store_true", help="skip the spin-adapted CISD step timing")
     ap.add_argument("--no-analytic-response", action="store_true",
                     help="skip the first-order CISD amplitude set timing")
+    ap.add_argument("--no-h2o2-aat", action="store_true",
+                    help="skip the end-to-end H2O2 AAT timing (analytic and finite differences)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -624,6 +699,7 @@ def main():
             aat_set["D"] = mp2_adjoint_set_measurement(dev, args.seed)
     spatial = None if args.no_spatial else spatial_cisd_measurement(dev, args.seed)
     response = None if args.no_analytic_response else analytic_response_measurement(dev, args.seed)
+    h2o2 = None if args.no_h2o2_aat else h2o2_aat_measurement(dev)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -669,6 +745,7 @@ def main():
         "aat_set": aat_set,
         "spatial_cisd": spatial,
         "analytic_response": response,
+        "h2o2_aat": h2o2,
     }
     print(json.dumps(out), flush=True)
 
