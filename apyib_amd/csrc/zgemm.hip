@@ -276,6 +276,47 @@ __global__ __launch_bounds__(256) void zgemm_reduce(const double2* __restrict__ 
   }
 }
 
+// The same sum for small outputs with many K slices (skinny products split thousands of ways): a
+// block takes 32 consecutive outputs and 8 slice lanes; lane r sums slices r, r+8, ... in order (four
+// interleaved accumulators), then the 8 lane sums are added in lane order.  Deterministic for a plan.
+constexpr int RW_E = 32, RW_R = 8;
+__global__ __launch_bounds__(RW_E * RW_R) void zgemm_reduce_wide(const double2* __restrict__ ws, int64_t group,
+                                                                 int64_t ngroups, int64_t m, int64_t n, double2* c,
+                                                                 int64_t ldc, int64_t sc, double ber, double bei,
+                                                                 int use_beta) {
+  const int64_t mn = m * n, total = ngroups * mn;
+  const int j = threadIdx.x % RW_E, r = threadIdx.x / RW_E;
+  const int64_t t = blockIdx.x * (int64_t)RW_E + j;
+  __shared__ double sr[RW_R][RW_E], si[RW_R][RW_E];
+  double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
+  if (t < total) {
+    const int64_t g = t / mn, e = t % mn;
+    const double2* w = ws + g * group * mn + e;
+    int64_t q = r;
+    for (; q + 3 * RW_R < group; q += 4 * RW_R) {
+      const double2 v0 = w[q * mn], v1 = w[(q + RW_R) * mn], v2 = w[(q + 2 * RW_R) * mn], v3 = w[(q + 3 * RW_R) * mn];
+      ar[0] += v0.x; ai[0] += v0.y; ar[1] += v1.x; ai[1] += v1.y;
+      ar[2] += v2.x; ai[2] += v2.y; ar[3] += v3.x; ai[3] += v3.y;
+    }
+    for (int u = 0; q < group; q += RW_R, ++u) { const double2 v = w[q * mn]; ar[u] += v.x; ai[u] += v.y; }
+  }
+  sr[r][j] = (ar[0] + ar[1]) + (ar[2] + ar[3]);
+  si[r][j] = (ai[0] + ai[1]) + (ai[2] + ai[3]);
+  __syncthreads();
+  if (r == 0 && t < total) {
+    double xr = 0.0, xi = 0.0;
+    for (int k = 0; k < RW_R; ++k) { xr += sr[k][j]; xi += si[k][j]; }
+    const int64_t g = t / mn, e = t % mn, row = e / n, col = e % n;
+    double2* cp = c + g * sc + row * ldc + col;
+    if (use_beta) {
+      const double2 old = *cp;
+      xr += ber * old.x - bei * old.y;
+      xi += ber * old.y + bei * old.x;
+    }
+    *cp = make_double2(xr, xi);
+  }
+}
+
 // ------------------------------------------------------------ tile configs
 struct Cfg {
   int bm, bn, wm, wn, bk;
@@ -521,9 +562,15 @@ int run_plan(const apy_gemm_desc* d, const Plan& pl, void* workspace, size_t ws_
     const int64_t group = d->reduce_batch ? pl.npart : pl.split;
     const int64_t ngroups = d->reduce_batch ? 1 : d->batch;
     const int64_t tot = ngroups * d->m * d->n;
-    zgemm_reduce<<<apy_grid1d(tot), 256, 0, s>>>((const double2*)workspace, group, ngroups, d->m, d->n,
-                                                 (double2*)d->c, d->ldc, d->stride_c, d->beta_re, d->beta_im,
-                                                 use_beta);
+    if (tot <= (1 << 18) && group >= 4 * RW_R) {
+      zgemm_reduce_wide<<<(unsigned)((tot + RW_E - 1) / RW_E), RW_E * RW_R, 0, s>>>(
+          (const double2*)workspace, group, ngroups, d->m, d->n, (double2*)d->c, d->ldc, d->stride_c, d->beta_re,
+          d->beta_im, use_beta);
+    } else {
+      zgemm_reduce<<<apy_grid1d(tot), 256, 0, s>>>((const double2*)workspace, group, ngroups, d->m, d->n,
+                                                   (double2*)d->c, d->ldc, d->stride_c, d->beta_re, d->beta_im,
+                                                   use_beta);
+    }
     APY_LAUNCH_CHECK();
   }
   return APY_OK;

@@ -71,3 +71,21 @@ def test_degenerate_shapes(dev):
         Am, Bm = _c(rng, m, k), _c(rng, k, n)
         C = kernels.mm(_d(Am, dev), _d(Bm, dev))
         assert np.abs(C.cpu().numpy() - Am @ Bm).max() < 1e-13 * max(1, k)
+
+
+@pytest.mark.parametrize("split", [32, 57, 200])
+def test_skinny_many_slices(dev, split):
+    """Small outputs split along a long K into many slices (the wide
+    split-K reduction): plain, with beta, and batched with a shared A."""
+    rng = np.random.default_rng(split)
+    m, n, k, nb = 20, 37, 9001, 3
+    Am, Bm, Cm = _c(rng, m, k), _c(rng, nb, k, n), _c(rng, nb, m, n)
+    A, B, C = _d(Am, dev), _d(Bm, dev), _d(Cm, dev)
+    kernels.zgemm_raw(m=m, n=n, k=k, a=A, a_layout=_lib.APY_MK, lda=k, b=B[0], b_layout=_lib.APY_KN, ldb=n,
+                      c=C[0], ldc=n, alpha=0.5 + 0.25j, beta=-0.75j, split_k=split, tile=1)
+    ref = (0.5 + 0.25j) * Am @ Bm[0] - 0.75j * Cm[0]
+    assert np.abs(C[0].cpu().numpy() - ref).max() < 1e-12 * np.sqrt(k) * 4
+    D = torch.zeros((nb, m, n), dtype=torch.complex128, device=dev)
+    kernels.zgemm_raw(m=m, n=n, k=k, a=A, a_layout=_lib.APY_MK, lda=k, b=B, b_layout=_lib.APY_KN, ldb=n, c=D, ldc=n,
+                      batch=nb, stride_a=0, stride_b=k * n, stride_c=m * n, split_k=split, tile=2)
+    assert np.abs(D.cpu().numpy() - Am @ Bm).max() < 1e-12 * np.sqrt(k) * 4
