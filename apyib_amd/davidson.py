@@ -109,8 +109,8 @@ class CISDDavidson:
         """Generator over Davidson iterations: each yields (theta, residual
         norm) after forming the Ritz pair and, unless the caller stops,
         extends the subspace by one preconditioned, orthonormalised vector
-        and its sigma.  The current Ritz vector (unnormalised only by
-        rounding) is self.x."""
+        and its sigma.  The Ritz vector of the last iteration is formed in
+        self.x by finish() (or at a restart)."""
         B, S = self.B, self.S
         B.zero_()
         S.zero_()
@@ -125,12 +125,15 @@ class CISDDavidson:
             Gk = G[:k, :k]
             w, U = np.linalg.eigh(0.5 * (Gk + Gk.conj().T))
             theta, y = float(w[0]), U[:, 0]
-            self.combine([B[j] for j in range(k)], y, x)
-            self.combine([S[j] for j in range(k)] + [x], list(y) + [-theta], r)
+            # residual straight from the basis, r = sum y_j S_j - theta sum y_j B_j; the Ritz vector
+            # itself is formed only at a restart and by finish()
+            self.combine([S[j] for j in range(k)] + [B[j] for j in range(k)], list(y) + list(-theta * y), r)
             rn = float(np.sqrt(abs(self.dots([r], r)[0])))
             self.E = theta
+            self._ritz = (k, y)
             yield theta, rn
             if k == self.max_space:                 # restart from the Ritz vector, with a fresh sigma
+                self._form_ritz()
                 B[0].copy_(x)
                 self._sigma(0)
                 G[0, 0] = self.dots([B[0]], S[0])[0]
@@ -138,19 +141,33 @@ class CISDDavidson:
             d = B[k]
             _lib.check(self.lib.apy_ci_precondition(self.O, self.V, _vp(self.soci.eps), theta, _vp(r), _vp(d),
                                                     stream_of(self.dev)), "apy_ci_precondition")
-            for _ in range(2):                      # Gram-Schmidt against the basis, twice
-                c = self.dots([B[j] for j in range(k)], d)
-                self.combine([B[j] for j in range(k)] + [d], list(-c) + [1.0], d)
-            nd = float(np.sqrt(abs(self.dots([d], d)[0])))
-            self.combine([d], [1.0 / nd], d)
+            # Gram-Schmidt against the basis twice; the second pass also yields <d|d>, so its
+            # projection and the normalisation are one combination (the norm of the twice-projected
+            # vector follows from Pythagoras to a few ulps, its overlaps with the basis being tiny)
+            basis = [B[j] for j in range(k)]
+            c = self.dots(basis, d)
+            self.combine(basis + [d], list(-c) + [1.0], d)
+            cd = self.dots(basis + [d], d)
+            c, dd = cd[:k], float(cd[k].real)
+            nn = dd - float(np.sum(np.abs(c) ** 2))
+            if not nn > 0.0:
+                raise RuntimeError("Davidson: the preconditioned residual lies in the subspace")
+            nd = float(np.sqrt(nn))
+            self.combine(basis + [d], list(-c / nd) + [1.0 / nd], d)
             self._sigma(k)
             col = self.dots([B[j] for j in range(k + 1)], S[k])
             G[:k + 1, k] = col
             G[k, :k + 1] = np.conj(col)
             k += 1
 
+    def _form_ritz(self):
+        """self.x = sum_j y_j B_j for the Ritz pair of the last iteration."""
+        k, y = self._ritz
+        self.combine([self.B[j] for j in range(k)], y, self.x)
+
     def finish(self):
-        """Normalise the current Ritz vector into self.c."""
+        """Form and normalise the current Ritz vector into self.c."""
+        self._form_ritz()
         x = self.x
         nx = float(np.sqrt(abs(self.dots([x], x)[0])))
         self.combine([x], [1.0 / nx], x)
