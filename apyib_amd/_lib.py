@@ -101,7 +101,7 @@ class CisdSoArgs(ctypes.Structure):
         ("energy", c_vp),
         ("c0_re", c_dbl), ("c0_im", c_dbl),
         ("t", c_vp), ("t_new", c_vp), ("out", c_vp),
-        ("ev_begin", c_vp), ("ev_end", c_vp),
+        ("ev_begin", c_vp), ("ev_end", c_vp), ("c0_dev", c_vp),
     ]
 
 
@@ -191,6 +191,7 @@ SIGNATURES = {
     "apy_eri_gto": (c_int, [c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "apy_eri_deriv_assemble": (c_int, [c_i64, c_vp, c_int, c_vp, c_vp, c_vp]),
     "apy_reduce_seg_workspace": (c_size, [c_i64, c_int]),
+    "apy_gs_coefficients": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp]),
     "apy_zdot_many_seg": (c_int, [c_int, _P_VP, c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_size, c_vp]),
     "apy_zdiffsq_batch": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_size, c_vp]),
     "apy_zcombine_seg": (c_int, [c_int, _P_VP, c_vp, c_i64, c_i64, c_vp, c_vp]),

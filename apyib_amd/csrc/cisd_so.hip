@@ -257,6 +257,7 @@ struct EpiArgs {
   const double* eps;
   const double2* E;
   double c0r, c0i;
+  const double2* c0p;      // when set: the driver weight from device memory
   const double2* t2;
   double2* t2n;
   double2* out2;
@@ -280,7 +281,8 @@ __global__ __launch_bounds__(256) void epi_doubles(EpiArgs e) {
     const int64_t ijba = ((i * O + j) * V + b) * V + a;
     const int64_t jiba = ((j * O + i) * V + b) * V + a;
     double2 d = e.drv[ijab];
-    double2 s = make_double2(e.c0r * d.x - e.c0i * d.y, e.c0r * d.y + e.c0i * d.x);
+    const double2 c0 = e.c0p ? *e.c0p : make_double2(e.c0r, e.c0i);
+    double2 s = make_double2(c0.x * d.x - c0.y * d.y, c0.x * d.y + c0.y * d.x);
     acc(s, e.Rp[t]);
     acc(s, e.X[(i * V + a) * OV + j * V + b]);
     dec(s, e.X[(j * V + a) * OV + i * V + b]);
@@ -323,8 +325,9 @@ __global__ __launch_bounds__(256) void epi_doubles(EpiArgs e) {
 __global__ void epi_singles(int64_t O, int64_t V, int64_t n, const double2* __restrict__ F,
                             const double* __restrict__ eps, const double2* E, double c0r, double c0i,
                             const double2* __restrict__ s1, const double2* t1, double2* t1n, double2* out1,
-                            int mode) {
+                            int mode, const double2* c0p) {
   const int64_t total = O * V;
+  if (c0p) { c0r = c0p->x; c0i = c0p->y; }
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t / V, a = t % V;
     double2 f = F[(O + a) * n + i];
@@ -602,6 +605,8 @@ int apy_cisd_so_step(const apy_cisd_so_args* A, void* workspace, size_t ws_bytes
   e.eps = A->eps;
   e.E = (const double2*)A->energy;
   e.c0r = c0r; e.c0i = c0i;
+  const double2* c0p = A->mode == APY_MODE_SIGMA ? (const double2*)A->c0_dev : nullptr;
+  e.c0p = c0p;
   e.t2 = t + OV;
   e.t2n = tn ? tn + OV : nullptr;
   e.out2 = out + OV;
@@ -610,7 +615,7 @@ int apy_cisd_so_step(const apy_cisd_so_args* A, void* workspace, size_t ws_bytes
   APY_LAUNCH_CHECK();
   if (singles) {
     epi_singles<<<apy_grid1d(OV), 256, 0, st>>>(O, V, n, F, A->eps, (const double2*)A->energy, c0r, c0i, w.s1, t, tn,
-                                                out, A->mode);
+                                                out, A->mode, c0p);
     APY_LAUNCH_CHECK();
   }
   return APY_OK;
@@ -806,6 +811,7 @@ int apy_cisd_so_sigma_batch(const apy_cisd_so_sigma_batch_args* A, void* workspa
     e.eps = nullptr;
     e.E = nullptr;
     e.c0r = A->c0[2 * k]; e.c0i = A->c0[2 * k + 1];
+    e.c0p = nullptr;
     e.t2 = T[k] + OV;
     e.t2n = nullptr;
     e.out2 = out + OV;
@@ -814,7 +820,7 @@ int apy_cisd_so_sigma_batch(const apy_cisd_so_sigma_batch_args* A, void* workspa
     APY_LAUNCH_CHECK();
     if (singles) {
       epi_singles<<<apy_grid1d(OV), 256, 0, st>>>(O, V, n, F[k], nullptr, nullptr, e.c0r, e.c0i, w.s1 + k * OV,
-                                                  T[k], nullptr, out, APY_MODE_SIGMA);
+                                                  T[k], nullptr, out, APY_MODE_SIGMA, nullptr);
       APY_LAUNCH_CHECK();
     }
   }

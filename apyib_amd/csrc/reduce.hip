@@ -131,6 +131,22 @@ __global__ __launch_bounds__(256) void combine_seg_kernel(VecList xs, const doub
   }
 }
 
+// Gram-Schmidt coefficients from [<B_j|d> (j < k) | <d|d>] (one thread, fixed order)
+__global__ void gs_coef_kernel(int k, const double2* __restrict__ dots, int normalize, double2* __restrict__ coef,
+                               double2* __restrict__ status) {
+  if (threadIdx.x != 0) return;
+  double scale = 1.0;
+  if (normalize) {
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += dots[j].x * dots[j].x + dots[j].y * dots[j].y;
+    const double dd = dots[k].x, nn = dd - s;
+    if (status) *status = make_double2(nn, dd);
+    scale = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+  }
+  for (int j = 0; j < k; ++j) coef[j] = make_double2(-dots[j].x * scale, -dots[j].y * scale);
+  coef[k] = make_double2(scale, 0.0);
+}
+
 // CI-metric dots over the vector layout [c1 (OV) | c2 (O^2 V^2, weight 1/4) | c0]:
 // sum_l w_l conj(x_m[l]) y[l] in one pass.
 template <int K>
@@ -333,6 +349,14 @@ int apy_zcombine_seg(int k, const void* const* xs, const void* coef, int64_t nse
     C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
 #undef C
   }
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_gs_coefficients(int k, const void* dots, int normalize, void* coef, void* status, void* stream) {
+  if (k < 0 || k >= APY_MAXVEC || !dots || !coef) return APY_E_ARG;
+  gs_coef_kernel<<<1, 64, 0, (hipStream_t)stream>>>(k, (const double2*)dots, normalize, (double2*)coef,
+                                                    (double2*)status);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }

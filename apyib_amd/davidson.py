@@ -49,6 +49,8 @@ class CISDDavidson:
         self.O, self.V = soci.O, soci.V
         self.OV, self.L = soci.OV, soci.L
         self.max_space = int(max_space)
+        if not 2 <= self.max_space <= _lib.APY_MAXVEC:
+            raise ValueError(f"max_space must be in [2, {_lib.APY_MAXVEC}]")
         self.B = torch.zeros((self.max_space, self.L + 1), dtype=CDT, device=self.dev)
         self.S = torch.zeros((self.max_space, self.L + 1), dtype=CDT, device=self.dev)
         self.work = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
@@ -60,12 +62,32 @@ class CISDDavidson:
 
     # -------------------------------------------------------------- primitives
     def _sigma(self, q):
-        """S[q] = H_N B[q] (needs the host value of the reference weight)."""
+        """S[q] = H_N B[q]; the reference weight B[q][L] is read by the
+        kernels from device memory."""
         self.nsigma += 1
         b, s = self.B[q], self.S[q]
-        c0 = complex(b[self.L].cpu())
-        self.soci.sigma(b[:self.L], s[:self.L], c0=c0)
+        self.soci.sigma(b[:self.L], s[:self.L], c0_dev=b[self.L:].data_ptr())
         self.soci.energy(b[:self.L], out=s[self.L:])
+
+    def _dots_dev(self, xs, y, out):
+        """CI-metric <x_m|y> into the device array out (len(xs) <= APY_MAXVEC), no host copy."""
+        arr = (ctypes.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+        nb = int(self.lib.apy_reduce_workspace(0, len(xs)))
+        ws = workspace(nb, self.dev)
+        _lib.check(self.lib.apy_ci_dots(self.O, self.V, len(xs), arr, _vp(y), _vp(out), _vp(ws), nb,
+                                        stream_of(self.dev)), "apy_ci_dots")
+
+    def _gs_pass(self, basis, d, normalize):
+        """d <- d - sum_j <B_j|d> B_j (normalize: then divided by its norm), all on the
+        device: dots, apy_gs_coefficients, apy_zcombine_seg."""
+        k = len(basis)
+        self._dots_dev(basis + [d], d, self._gdots)
+        _lib.check(self.lib.apy_gs_coefficients(k, _vp(self._gdots), int(normalize), _vp(self._gcoef),
+                                                _vp(self._ibuf[1:2]), stream_of(self.dev)), "apy_gs_coefficients")
+        xs = basis + [d]
+        arr = (ctypes.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+        _lib.check(self.lib.apy_zcombine_seg(k + 1, arr, _vp(self._gcoef), 1, d.numel(), _vp(d),
+                                             stream_of(self.dev)), "apy_zcombine_seg")
 
     def dots(self, xs, y):
         """CI-metric <x_m|y> for m < len(xs) (host complex array): one
@@ -106,11 +128,13 @@ class CISDDavidson:
 
     # ------------------------------------------------------------------ solve
     def iterate(self):
-        """Generator over Davidson iterations: each yields (theta, residual
-        norm) after forming the Ritz pair and, unless the caller stops,
-        extends the subspace by one preconditioned, orthonormalised vector
-        and its sigma.  The Ritz vector of the last iteration is formed in
-        self.x by finish() (or at a restart)."""
+        """Generator over Davidson iterations.  Each step forms the Ritz pair
+        of the current subspace, its residual, and extends the subspace by
+        the preconditioned, twice-orthogonalised, normalised residual and its
+        sigma, all on the device; one host read per iteration then brings
+        back the residual norm and the new column of the projected matrix,
+        and the step yields (Ritz value, residual norm) of the pair it
+        started from.  finish() forms that pair's vector."""
         B, S = self.B, self.S
         B.zero_()
         S.zero_()
@@ -121,41 +145,45 @@ class CISDDavidson:
         G[0, 0] = self.dots([B[0]], S[0])[0]
         self.x = x = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
         r = self.work
+        M = _lib.APY_MAXVEC
+        # device scratch: [0] <r|r>, [1] Gram-Schmidt status (nn, <d|d>), [2] restart <B0|S0>,
+        # [4:4+k+1] the new column; Gram-Schmidt dots and coefficients
+        self._ibuf = torch.zeros(4 + M, dtype=CDT, device=self.dev)
+        self._gdots = torch.zeros(M, dtype=CDT, device=self.dev)
+        self._gcoef = torch.zeros(M, dtype=CDT, device=self.dev)
         while True:
             Gk = G[:k, :k]
             w, U = np.linalg.eigh(0.5 * (Gk + Gk.conj().T))
             theta, y = float(w[0]), U[:, 0]
-            # residual straight from the basis, r = sum y_j S_j - theta sum y_j B_j; the Ritz vector
-            # itself is formed only at a restart and by finish()
+            # residual straight from the basis, r = sum y_j S_j - theta sum y_j B_j
             self.combine([S[j] for j in range(k)] + [B[j] for j in range(k)], list(y) + list(-theta * y), r)
-            rn = float(np.sqrt(abs(self.dots([r], r)[0])))
-            self.E = theta
+            self._dots_dev([r], r, self._ibuf[0:1])
             self._ritz = (k, y)
-            yield theta, rn
-            if k == self.max_space:                 # restart from the Ritz vector, with a fresh sigma
+            self.E = theta
+            restart = k == self.max_space
+            if restart:                             # restart from the Ritz vector, with a fresh sigma
                 self._form_ritz()
                 B[0].copy_(x)
+                self._ritz = (1, np.ones(1))
                 self._sigma(0)
-                G[0, 0] = self.dots([B[0]], S[0])[0]
+                self._dots_dev([B[0]], S[0], self._ibuf[2:3])
                 k = 1
             d = B[k]
             _lib.check(self.lib.apy_ci_precondition(self.O, self.V, _vp(self.soci.eps), theta, _vp(r), _vp(d),
                                                     stream_of(self.dev)), "apy_ci_precondition")
-            # Gram-Schmidt against the basis twice; the second pass also yields <d|d>, so its
-            # projection and the normalisation are one combination (the norm of the twice-projected
-            # vector follows from Pythagoras to a few ulps, its overlaps with the basis being tiny)
             basis = [B[j] for j in range(k)]
-            c = self.dots(basis, d)
-            self.combine(basis + [d], list(-c) + [1.0], d)
-            cd = self.dots(basis + [d], d)
-            c, dd = cd[:k], float(cd[k].real)
-            nn = dd - float(np.sum(np.abs(c) ** 2))
-            if not nn > 0.0:
-                raise RuntimeError("Davidson: the preconditioned residual lies in the subspace")
-            nd = float(np.sqrt(nn))
-            self.combine(basis + [d], list(-c / nd) + [1.0 / nd], d)
+            self._gs_pass(basis, d, False)          # Gram-Schmidt against the basis twice, the second
+            self._gs_pass(basis, d, True)           # pass normalising by the Pythagoras norm
             self._sigma(k)
-            col = self.dots([B[j] for j in range(k + 1)], S[k])
+            self._dots_dev([B[j] for j in range(k + 1)], S[k], self._ibuf[4:5 + k])
+            host = self._ibuf[:5 + k].cpu().numpy()          # the one host read of the iteration
+            if not host[1].real > 0.0:
+                raise RuntimeError("Davidson: the preconditioned residual lies in the subspace")
+            if restart:
+                G[0, 0] = host[2]
+            rn = float(np.sqrt(abs(host[0])))
+            col = host[4:5 + k]
+            yield theta, rn
             G[:k + 1, k] = col
             G[k, :k + 1] = np.conj(col)
             k += 1

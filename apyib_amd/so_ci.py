@@ -150,15 +150,16 @@ class SOCI:
         O, V = self.O, self.V
         return t[:self.OV].view(O, V), t[self.OV:].view(O, O, V, V)
 
-    def step(self, t, t_new, out, energy=None, mode=_lib.APY_MODE_RESIDUAL, c0=1.0, ev_dominant=None):
-        """One call of apy_cisd_so_step (residual update or sigma product)."""
+    def step(self, t, t_new, out, energy=None, mode=_lib.APY_MODE_RESIDUAL, c0=1.0, ev_dominant=None, c0_dev=None):
+        """One call of apy_cisd_so_step (residual update or sigma product).
+        c0_dev: optional device address of the sigma-mode driver weight."""
         c0 = complex(c0)
         ev = ev_dominant or self.ev_dominant or (None, None)
         args = _lib.CisdSoArgs(
             self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(), ctypes.pointer(self.ints.struct),
             self.hoo.data_ptr(), int(self.singles), int(mode), int(self.fov),
             (energy if energy is not None else self.energy_dev).data_ptr(), c0.real, c0.imag,
-            t.data_ptr(), t_new.data_ptr() if t_new is not None else None, out.data_ptr(), ev[0], ev[1])
+            t.data_ptr(), t_new.data_ptr() if t_new is not None else None, out.data_ptr(), ev[0], ev[1], c0_dev)
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_so_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
                    "apy_cisd_so_step")
@@ -175,9 +176,10 @@ class SOCI:
         _lib.check(self.lib.apy_cisd_so_tune(ctypes.byref(args), _vp(ws), self.ws_bytes, int(reps),
                                              stream_of(self.device)), "apy_cisd_so_tune")
 
-    def sigma(self, x, out, c0=1.0):
-        """out = H_N (c0, x): the CISD sigma with reference weight c0."""
-        self.step(x, None, out, mode=_lib.APY_MODE_SIGMA, c0=c0)
+    def sigma(self, x, out, c0=1.0, c0_dev=None):
+        """out = H_N (c0, x): the CISD sigma with reference weight c0 (or the
+        device complex scalar at address c0_dev)."""
+        self.step(x, None, out, mode=_lib.APY_MODE_SIGMA, c0=c0, c0_dev=c0_dev)
 
     def energy(self, t, out=None):
         out = self.energy_dev if out is None else out

@@ -59,6 +59,11 @@ class HipEvents:
     def destroy(self, e):
         self.hip.hipEventDestroy(e)
 
+    def record(self, e, stream):
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        if self.hip.hipEventRecord(e, stream) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
 
 def read_traffic():
     """HBM bytes per PPL launch from the committed rocprofv3 PMC summary."""
@@ -662,6 +667,21 @@ def main():
     # vector (plus the restart sigma every max_space iterations), CI-metric
     # projections, Ritz pair, residual, preconditioner, Gram-Schmidt
     dav = CISDDavidson(solver, max_space=args.max_space)
+    # the sigma calls of the timed iterations, bracketed by events on their stream
+    sigma_events = []
+    sigma_plain = dav._sigma
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def sigma_timed(q):
+        if not timing[0]:
+            return sigma_plain(q)
+        e0, e1 = hip.create(), hip.create()
+        hip.record(e0, stream)
+        sigma_plain(q)
+        hip.record(e1, stream)
+        sigma_events.append((e0, e1))
+    timing = [False]
+    dav._sigma = sigma_timed
     gen = dav.iterate()
     next(gen)                       # reference vector, its sigma and the first Ritz pair
     for _ in range(W):
@@ -670,12 +690,18 @@ def main():
     n0 = dav.nsigma
     t_start = time.perf_counter()
     theta = rn = None
+    timing[0] = True
     for k in range(K):
         solver.ev_dominant = evs[k]
         theta, rn = next(gen)
     torch.cuda.synchronize()
+    timing[0] = False
     solver.ev_dominant = None
     elapsed = time.perf_counter() - t_start
+    sigma_ms = [hip.elapsed_ms(a, b) for a, b in sigma_events]
+    for a, b in sigma_events:
+        hip.destroy(a)
+        hip.destroy(b)
     nsig = dav.nsigma - n0
     gen.close()
     ms_step = elapsed * 1e3 / K
@@ -732,6 +758,12 @@ def main():
             "E_corr_after_steps": theta, "residual_after_steps": rn,
         },
         "dense_equivalent_tflops": round(dense_flops(O, V) * nsig / K / (ms_step * 1e-3) / 1e12, 3),
+        "sigma": {"what": "the sigma calls of the timed iterations alone (HIP events on their stream around each "
+                          "apy_cisd_so_step + reference-row energy, host reads inside); executed flops as value",
+                  "calls": len(sigma_ms), "ms": round(sum(sigma_ms) / max(1, len(sigma_ms)), 4),
+                  "tflops": round(flops_sigma / (sum(sigma_ms) / max(1, len(sigma_ms)) * 1e-3) / 1e12, 3),
+                  "frac_of_fp64_mfma_peak": round(flops_sigma / (sum(sigma_ms) / max(1, len(sigma_ms)) * 1e-3)
+                                                  / 1e12 / PEAK_FP64, 4)},
         "roofline": {
             "kernel": "zgemm_mfma (particle-particle ladder, packed pairs)",
             "bound": "mfma",

@@ -262,6 +262,9 @@ typedef struct apy_cisd_so_args {
   void* out;               /* residual (residual mode) or sigma; != t    */
   void* ev_begin;          /* optional hipEvent_t recorded just before and */
   void* ev_end;            /* just after the PPL GEMM kernel (profiling)   */
+  const void* c0_dev;      /* sigma mode, optional: the driver weight read
+                              from this device complex scalar instead of
+                              c0_re/c0_im (no host read before the call)  */
 } apy_cisd_so_args;
 
 /* Pair-space occupied operator: <kl||ij> plus the F_oo terms of the doubles
@@ -528,6 +531,13 @@ size_t apy_reduce_workspace(int64_t len, int k);
  *                       device memory
  * Workspace apy_reduce_seg_workspace(nseg, k) (nseg -> 2 nseg for diffsq). */
 size_t apy_reduce_seg_workspace(int64_t nseg, int k);
+/* Gram-Schmidt coefficients on the device from dots = [<B_j|d> (j < k) |
+ * <d|d>]: normalize = 0 gives coef = [-<B_j|d> | 1]; normalize = 1 gives
+ * coef = [-<B_j|d> | 1] / sqrt(nn), nn = <d|d> - sum_j |<B_j|d>|^2 (the norm
+ * of the projected d), and status = (nn, <d|d>) (coef all 0 if nn <= 0).
+ * One thread; k < APY_MAXVEC.  Lets a Davidson expansion run without host
+ * reads (apply with apy_zcombine_seg, nseg = 1). */
+int apy_gs_coefficients(int k, const void* dots, int normalize, void* coef, void* status, void* stream);
 int apy_zdot_many_seg(int k, const void* const* xs, const void* y, int64_t nseg, int64_t seglen, int conj_x,
                       void* out, void* workspace, size_t ws_bytes, void* stream);
 int apy_zdiffsq_batch(const void* x, const void* y, int64_t nseg, int64_t seglen, int64_t split, void* out,
