@@ -329,9 +329,113 @@ __global__ __launch_bounds__(256) void tpd_full_kernel(int64_t O, int64_t V, apy
   }
 }
 
+// Packed CI vectors [c1 (OV) | c2 over (i<j, a<b) (Po Pv) | c0]: the antisymmetric doubles stored once,
+// so the CI metric (dense c2 weighted 1/4) is the Euclidean inner product.
+__global__ __launch_bounds__(256) void ci_pack_kernel(int64_t O, int64_t V, const double2* __restrict__ dense,
+                                                      double2* __restrict__ packed) {
+  const int64_t OV = O * V, Po = tp_npair(O), Pv = tp_npair(V), L = OV + OV * OV, Lp = OV + Po * Pv + 1;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < Lp; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 v;
+    if (e < OV) {
+      v = dense[e];
+    } else if (e < Lp - 1) {
+      const int64_t q = e - OV;
+      int64_t i, j, a, b;
+      tp_pdec(q / Pv, O, i, j);
+      tp_pdec(q % Pv, V, a, b);
+      v = dense[OV + ((i * O + j) * V + a) * V + b];
+    } else {
+      v = dense[L];
+    }
+    packed[e] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void ci_unpack_kernel(int64_t O, int64_t V, const double2* __restrict__ packed,
+                                                        double2* __restrict__ dense) {
+  const int64_t OV = O * V, Pv = tp_npair(V), L = OV + OV * OV, Lp = OV + tp_npair(O) * Pv + 1;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= L; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 v;
+    if (e < OV) {
+      v = packed[e];
+    } else if (e < L) {
+      int64_t q = e - OV;
+      const int64_t b = q % V; q /= V;
+      const int64_t a = q % V; q /= V;
+      const int64_t j = q % O, i = q / O;
+      if (i == j || a == b) {
+        v = make_double2(0.0, 0.0);
+      } else {
+        const double sg = ((i < j) == (a < b)) ? 1.0 : -1.0;
+        const double2 p = packed[OV + tp_pidx(i < j ? i : j, i < j ? j : i, O) * Pv + tp_pidx(a < b ? a : b, a < b ? b : a, V)];
+        v = make_double2(sg * p.x, sg * p.y);
+      }
+    } else {
+      v = packed[Lp - 1];
+    }
+    dense[e] = v;
+  }
+}
+
+// r / (theta - D) on the packed layout (D_ia, D_ijab; 0 for c0), denominators kept away from zero as in
+// precond_kernel
+__global__ __launch_bounds__(256) void precond_packed_kernel(int64_t O, int64_t V, const double* __restrict__ eps,
+                                                             double theta, const double2* __restrict__ r,
+                                                             double2* __restrict__ out) {
+  const int64_t OV = O * V, Pv = tp_npair(V), Lp = OV + tp_npair(O) * Pv + 1;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < Lp; e += (int64_t)gridDim.x * blockDim.x) {
+    double D;
+    if (e < OV) {
+      D = eps[O + e % V] - eps[e / V];
+    } else if (e < Lp - 1) {
+      const int64_t q = e - OV;
+      int64_t i, j, a, b;
+      tp_pdec(q / Pv, O, i, j);
+      tp_pdec(q % Pv, V, a, b);
+      D = eps[O + a] + eps[O + b] - eps[i] - eps[j];
+    } else {
+      D = 0.0;
+    }
+    double den = theta - D;
+    if (fabs(den) < 1e-8) den = den < 0.0 ? -1e-8 : 1e-8;
+    out[e] = make_double2(r[e].x / den, r[e].y / den);
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int64_t apy_ci_packed_length(int64_t O, int64_t V) {
+  if (O < 2 || V < 2) return 0;
+  return O * V + tp_npair(O) * tp_npair(V) + 1;
+}
+
+int apy_ci_pack(int64_t O, int64_t V, const void* dense, void* packed, void* stream) {
+  if (O < 2 || V < 2 || !dense || !packed || dense == packed) return APY_E_ARG;
+  ci_pack_kernel<<<apy_grid1d(apy_ci_packed_length(O, V)), 256, 0, (hipStream_t)stream>>>(O, V, (const double2*)dense,
+                                                                                          (double2*)packed);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_ci_unpack(int64_t O, int64_t V, const void* packed, void* dense, void* stream) {
+  if (O < 2 || V < 2 || !dense || !packed || dense == packed) return APY_E_ARG;
+  const int64_t L = O * V + O * O * V * V;
+  ci_unpack_kernel<<<apy_grid1d(L + 1), 256, 0, (hipStream_t)stream>>>(O, V, (const double2*)packed,
+                                                                       (double2*)dense);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_ci_precondition_packed(int64_t O, int64_t V, const void* eps, double theta, const void* r, void* out,
+                               void* stream) {
+  if (O < 2 || V < 2 || !eps || !r || !out) return APY_E_ARG;
+  precond_packed_kernel<<<apy_grid1d(apy_ci_packed_length(O, V)), 256, 0, (hipStream_t)stream>>>(
+      O, V, (const double*)eps, theta, (const double2*)r, (double2*)out);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
 
 int apy_ci_precondition(int64_t O, int64_t V, const void* eps, double theta, const void* r, void* out,
                         void* stream) {

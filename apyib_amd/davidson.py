@@ -8,12 +8,14 @@ Hartree-Fock orbitals (F_ov = 0) it is the reference's solution.  The
 reference's Jacobi + DIIS iteration diverges on systems far from
 diagonal dominance; Davidson converges there.
 
-Vectors live on the device with layout [c1 | c2 | c0] (length L + 1);
-inner products count each determinant once (c2 dense, weight 1/4).  Every
-vector operation is a library call: sigma (apy_cisd_so_step, SIGMA mode),
-the reference row (apy_cisd_so_energy), CI-metric dot products
-(apy_ci_dots, one pass with the c2 weight), linear combinations
-(apy_zcombine) and the preconditioner (apy_ci_precondition).  The k x k
+Subspace vectors live on the device as packed [c1 | c2 over (i<j, a<b) |
+c0] (dense [c1 | c2 | c0] for the batched solver); inner products count each
+determinant once.  Every vector operation is a library call: sigma
+(apy_cisd_so_step, SIGMA mode, on the dense image from apy_ci_unpack, packed
+back with apy_ci_pack), the reference row (apy_cisd_so_energy), dot
+products (apy_zdot_many on packed vectors, apy_ci_dots with the c2 weight on
+dense ones), linear combinations (apy_zcombine, apy_zcombine_seg) and the
+preconditioner (apy_ci_precondition[_packed]).  The k x k
 projected eigenproblem is solved on the host.
 
 The gradient (adjoint) needs no iteration history: for the variational
@@ -40,7 +42,13 @@ def _vp(t):
 
 
 class CISDDavidson:
-    def __init__(self, soci, max_space=16):
+    """packed=True keeps the subspace as packed vectors [c1 | c2 over (i<j, a<b) | c0]
+    (apy_ci_pack / apy_ci_unpack around each sigma): the CI metric is then the
+    Euclidean product and every vector pass moves a quarter of the bytes.
+    packed=False keeps dense vectors (the layout MultiDavidson's batched sigma
+    reads)."""
+
+    def __init__(self, soci, max_space=16, packed=True):
         if not soci.fov or not soci.singles:
             raise ValueError("Davidson needs the Hermitian CISD operator: SOCI(..., singles=True, fov_coupling=True)")
         self.soci = soci
@@ -51,9 +59,14 @@ class CISDDavidson:
         self.max_space = int(max_space)
         if not 2 <= self.max_space <= _lib.APY_MAXVEC:
             raise ValueError(f"max_space must be in [2, {_lib.APY_MAXVEC}]")
-        self.B = torch.zeros((self.max_space, self.L + 1), dtype=CDT, device=self.dev)
-        self.S = torch.zeros((self.max_space, self.L + 1), dtype=CDT, device=self.dev)
-        self.work = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
+        self.packed = bool(packed) and self.O >= 2 and self.V >= 2
+        self.Lv = int(self.lib.apy_ci_packed_length(self.O, self.V)) if self.packed else self.L + 1
+        self.B = torch.zeros((self.max_space, self.Lv), dtype=CDT, device=self.dev)
+        self.S = torch.zeros((self.max_space, self.Lv), dtype=CDT, device=self.dev)
+        self.work = torch.zeros(self.Lv, dtype=CDT, device=self.dev)
+        if self.packed:                             # dense images for the sigma
+            self.bd = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
+            self.sd = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
         self.part = torch.zeros(max(_lib.APY_MAXVEC, self.max_space + 1), dtype=CDT, device=self.dev)
         self.c = None
         self.E = None
@@ -62,20 +75,36 @@ class CISDDavidson:
 
     # -------------------------------------------------------------- primitives
     def _sigma(self, q):
-        """S[q] = H_N B[q]; the reference weight B[q][L] is read by the
-        kernels from device memory."""
+        """S[q] = H_N B[q]; the reference weight is read by the kernels from
+        device memory."""
         self.nsigma += 1
         b, s = self.B[q], self.S[q]
+        st = stream_of(self.dev)
+        if self.packed:
+            _lib.check(self.lib.apy_ci_unpack(self.O, self.V, _vp(b), _vp(self.bd), st), "apy_ci_unpack")
+            b, s = self.bd, self.sd
         self.soci.sigma(b[:self.L], s[:self.L], c0_dev=b[self.L:].data_ptr())
         self.soci.energy(b[:self.L], out=s[self.L:])
+        if self.packed:
+            _lib.check(self.lib.apy_ci_pack(self.O, self.V, _vp(self.sd), _vp(self.S[q]), st), "apy_ci_pack")
 
     def _dots_dev(self, xs, y, out):
-        """CI-metric <x_m|y> into the device array out (len(xs) <= APY_MAXVEC), no host copy."""
+        """CI-metric <x_m|y> into the device array out (len(xs) <= APY_MAXVEC), no host copy:
+        apy_zdot_many on packed vectors (the metric is Euclidean there), apy_ci_dots on dense."""
         arr = (ctypes.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
         nb = int(self.lib.apy_reduce_workspace(0, len(xs)))
         ws = workspace(nb, self.dev)
-        _lib.check(self.lib.apy_ci_dots(self.O, self.V, len(xs), arr, _vp(y), _vp(out), _vp(ws), nb,
-                                        stream_of(self.dev)), "apy_ci_dots")
+        if self.packed:
+            _lib.check(self.lib.apy_zdot_many(len(xs), arr, _vp(y), y.numel(), 1, _vp(out), _vp(ws), nb,
+                                              stream_of(self.dev)), "apy_zdot_many")
+        else:
+            _lib.check(self.lib.apy_ci_dots(self.O, self.V, len(xs), arr, _vp(y), _vp(out), _vp(ws), nb,
+                                            stream_of(self.dev)), "apy_ci_dots")
+
+    def _precondition(self, theta, r, d):
+        fn = self.lib.apy_ci_precondition_packed if self.packed else self.lib.apy_ci_precondition
+        _lib.check(fn(self.O, self.V, _vp(self.soci.eps), theta, _vp(r), _vp(d), stream_of(self.dev)),
+                   "apy_ci_precondition")
 
     def _gs_pass(self, basis, d, normalize):
         """d <- d - sum_j <B_j|d> B_j (normalize: then divided by its norm), all on the
@@ -91,16 +120,11 @@ class CISDDavidson:
 
     def dots(self, xs, y):
         """CI-metric <x_m|y> for m < len(xs) (host complex array): one
-        apy_ci_dots pass per APY_MAXVEC vectors, one host copy."""
+        pass per APY_MAXVEC vectors, one host copy."""
         out = np.zeros(len(xs), dtype=np.complex128)
         M = _lib.APY_MAXVEC
         for s0 in range(0, len(xs), M):
-            chunk = xs[s0:s0 + M]
-            arr = (ctypes.c_void_p * len(chunk))(*[x.data_ptr() for x in chunk])
-            nb = int(self.lib.apy_reduce_workspace(0, len(chunk)))
-            ws = workspace(nb, self.dev)
-            _lib.check(self.lib.apy_ci_dots(self.O, self.V, len(chunk), arr, _vp(y), _vp(self.part[s0:]), _vp(ws), nb,
-                                            stream_of(self.dev)), "apy_ci_dots")
+            self._dots_dev(xs[s0:s0 + M], y, self.part[s0:])
         n = len(xs)
         out[:] = self.part[:n].cpu().numpy()
         return out
@@ -138,12 +162,12 @@ class CISDDavidson:
         B, S = self.B, self.S
         B.zero_()
         S.zero_()
-        B[0, self.L] = 1.0                          # the reference determinant
+        B[0, self.Lv - 1] = 1.0                     # the reference determinant
         self._sigma(0)
         k = 1
         G = np.zeros((self.max_space, self.max_space), dtype=np.complex128)
         G[0, 0] = self.dots([B[0]], S[0])[0]
-        self.x = x = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
+        self.x = x = torch.zeros(self.Lv, dtype=CDT, device=self.dev)
         r = self.work
         M = _lib.APY_MAXVEC
         # device scratch: [0] <r|r>, [1] Gram-Schmidt status (nn, <d|d>), [2] restart <B0|S0>,
@@ -169,8 +193,7 @@ class CISDDavidson:
                 self._dots_dev([B[0]], S[0], self._ibuf[2:3])
                 k = 1
             d = B[k]
-            _lib.check(self.lib.apy_ci_precondition(self.O, self.V, _vp(self.soci.eps), theta, _vp(r), _vp(d),
-                                                    stream_of(self.dev)), "apy_ci_precondition")
+            self._precondition(theta, r, d)
             basis = [B[j] for j in range(k)]
             self._gs_pass(basis, d, False)          # Gram-Schmidt against the basis twice, the second
             self._gs_pass(basis, d, True)           # pass normalising by the Pythagoras norm
@@ -194,12 +217,18 @@ class CISDDavidson:
         self.combine([self.B[j] for j in range(k)], y, self.x)
 
     def finish(self):
-        """Form and normalise the current Ritz vector into self.c."""
+        """Form and normalise the current Ritz vector into self.c (dense
+        [c1 | c2 | c0])."""
         self._form_ritz()
         x = self.x
         nx = float(np.sqrt(abs(self.dots([x], x)[0])))
         self.combine([x], [1.0 / nx], x)
-        self.c = x
+        if self.packed:
+            self.c = torch.empty(self.L + 1, dtype=CDT, device=self.dev)
+            _lib.check(self.lib.apy_ci_unpack(self.O, self.V, _vp(x), _vp(self.c), stream_of(self.dev)),
+                       "apy_ci_unpack")
+        else:
+            self.c = x
         return self.c
 
     def solve(self, e_conv=1e-12, r_conv=1e-8, maxit=200):
@@ -332,7 +361,8 @@ class MultiDavidson:
     def __init__(self, ints, focks, eps, max_space=12):
         from .so_ci import SOCI
         self.ints = ints
-        self.dav = [CISDDavidson(SOCI(ints, F, eps, singles=True, fov_coupling=True), max_space) for F in focks]
+        self.dav = [CISDDavidson(SOCI(ints, F, eps, singles=True, fov_coupling=True), max_space, packed=False)
+                    for F in focks]
         self.lib = self.dav[0].lib
         self.dev = self.dav[0].dev
         self.O, self.V, self.L = self.dav[0].O, self.dav[0].V, self.dav[0].L

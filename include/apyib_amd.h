@@ -400,6 +400,19 @@ size_t apy_mp2_nc_density_workspace(int64_t O, int64_t V);
 int apy_mp2_nc_densities(int64_t O, int64_t V, const void* t, const void* lam, void* grad_f, void* workspace,
                          size_t ws_bytes, void* stream);
 
+/* Packed CI vectors [c1 (O*V) | c2 over (i<j, a<b) (Po*Pv) | c0], length
+ * apy_ci_packed_length(O, V): the antisymmetric doubles stored once, so the
+ * CI metric of the dense form (c2 weighted 1/4) is the Euclidean product and
+ * a Davidson subspace takes a quarter of the memory traffic.  apy_ci_pack
+ * reads the (i<j, a<b) entries of a dense [c1 | c2 | c0] vector;
+ * apy_ci_unpack writes all four antisymmetric images (zero diagonals);
+ * apy_ci_precondition_packed is apy_ci_precondition on the packed layout.
+ * O, V >= 2; dense and packed must differ. */
+int64_t apy_ci_packed_length(int64_t O, int64_t V);
+int apy_ci_pack(int64_t O, int64_t V, const void* dense, void* packed, void* stream);
+int apy_ci_unpack(int64_t O, int64_t V, const void* packed, void* dense, void* stream);
+int apy_ci_precondition_packed(int64_t O, int64_t V, const void* eps, double theta, const void* r, void* out,
+                               void* stream);
 /* ------------------------------------------------------------------------
  * Davidson support and the CISD energy gradient with respect to F.
  * Vectors have length L + 1, L = O V + O^2 V^2: [c1 | c2 (dense,

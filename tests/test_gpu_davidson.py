@@ -276,3 +276,49 @@ def test_autograd_full_tensor(dev):
     Ep = float(cisd_energy(_d(F + h * dF, dev), _d(A + h * dA, dev), eps, O))
     Em = float(cisd_energy(_d(F - h * dF, dev), _d(A - h * dA, dev), eps, O))
     assert abs(lin.real - (Ep - Em) / (2 * h)) < 1e-7 * max(1.0, abs(lin))
+
+
+def test_packed_vectors_round_trip_and_metric(dev):
+    """apy_ci_pack / apy_ci_unpack on an antisymmetric [c1 | c2 | c0]: the
+    round trip is exact, the Euclidean product of packed vectors equals the
+    CI-metric product of the dense ones, and the packed preconditioner is
+    the dense one packed."""
+    import ctypes
+    from apyib_amd import _lib
+    from apyib_amd.kernels import stream_of
+    lib = _lib.load()
+    O, V = 5, 7
+    L = O * V + O * O * V * V
+    rng = np.random.default_rng(2)
+
+    def antisym():
+        v = rng.standard_normal(L + 1) + 1j * rng.standard_normal(L + 1)
+        t2 = v[O * V:L].reshape(O, O, V, V)
+        t2 = t2 - t2.transpose(1, 0, 2, 3)
+        t2 = t2 - t2.transpose(0, 1, 3, 2)
+        v[O * V:L] = t2.ravel()
+        return v
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    st = stream_of(dev)
+    Lp = int(lib.apy_ci_packed_length(O, V))
+    assert Lp == O * V + (O * (O - 1) // 2) * (V * (V - 1) // 2) + 1
+    x, y = antisym(), antisym()
+    xd, yd = _d(x, dev), _d(y, dev)
+    xp, yp = (torch.empty(Lp, dtype=torch.complex128, device=dev) for _ in range(2))
+    assert lib.apy_ci_pack(O, V, P(xd), P(xp), st) == 0 and lib.apy_ci_pack(O, V, P(yd), P(yp), st) == 0
+    back = torch.empty(L + 1, dtype=torch.complex128, device=dev)
+    assert lib.apy_ci_unpack(O, V, P(xp), P(back), st) == 0
+    assert torch.equal(back, xd)
+    w = np.ones(L + 1)
+    w[O * V:L] = 0.25
+    ref = np.sum(w * np.conj(x) * y)
+    got = torch.sum(torch.conj(xp) * yp).item()
+    assert abs(got - ref) < 1e-12 * abs(ref)
+    eps = np.sort(rng.uniform(-2, 2, O + V))
+    ed = torch.from_numpy(eps).to(dev)
+    rd, rp = torch.empty_like(xd), torch.empty_like(xp)
+    assert lib.apy_ci_precondition(O, V, P(ed), -0.3, P(xd), P(rd), st) == 0
+    assert lib.apy_ci_precondition_packed(O, V, P(ed), -0.3, P(xp), P(rp), st) == 0
+    packed_ref = torch.empty_like(xp)
+    assert lib.apy_ci_pack(O, V, P(rd), P(packed_ref), st) == 0
+    assert torch.equal(rp, packed_ref)
