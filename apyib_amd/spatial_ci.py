@@ -68,7 +68,7 @@ class SpatialCI(SOCI):
         self.singles = bool(singles)
         self.fov = False
         self.fock = fock.to(self.device, CDT).contiguous()
-        self.eps = torch.as_tensor(np.asarray(eps, dtype=np.float64) if not isinstance(eps, torch.Tensor) else eps,
+        self.eps = torch.as_tensor(np.array(eps, dtype=np.float64) if not isinstance(eps, torch.Tensor) else eps,
                                    dtype=torch.float64).to(self.device).contiguous()
         o, v = self.O, self.V
         self.OV = o * v

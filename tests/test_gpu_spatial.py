@@ -39,7 +39,7 @@ def _amps(o, v, seed):
     return t1, t2
 
 
-@pytest.mark.parametrize("o,v,seed", [(2, 3, 1), (4, 7, 2), (6, 17, 3)])
+@pytest.mark.parametrize("o,v,seed", [(1, 4, 8), (3, 1, 9), (2, 3, 1), (4, 7, 2), (6, 17, 3)])
 @pytest.mark.parametrize("singles", [True, False])
 def test_residual_step_matches_oracle(dev, o, v, seed, singles):
     E4, F, eps = _system(o, v, seed)
