@@ -10,7 +10,6 @@ and magnetic fields enter V as in hamiltonian.py:41-70:
          momentum integrals in the convention of hamiltonian.py:55-57.
 The phase-space extensions (hamiltonian.py:72-538) are not part of this path.
 """
-import numpy as np
 
 from . import gaussian
 from .molecule import BasisSet, Molecule
