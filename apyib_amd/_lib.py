@@ -78,6 +78,18 @@ class Gto(ctypes.Structure):
     ]
 
 
+APY_OEI_OVERLAP, APY_OEI_KINETIC, APY_OEI_POTENTIAL, APY_OEI_DIPOLE, APY_OEI_ANGMOM = 1, 2, 4, 8, 16
+
+
+class OeiArgs(ctypes.Structure):
+    _fields_ = [
+        ("nset", c_i64), ("fa", c_vp), ("na", c_i64), ("ca", c_vp), ("nca", c_i64),
+        ("fb", c_vp), ("nb", c_i64), ("cb", c_vp), ("ncb", c_i64),
+        ("nuclei", c_vp), ("charges", c_vp), ("nnuc", c_i64), ("origin", c_dbl * 3),
+        ("which", ctypes.c_int32), ("pad", ctypes.c_int32), ("out", c_vp),
+    ]
+
+
 class CisdSpArgs(ctypes.Structure):
     _fields_ = [
         ("o", c_i64), ("v", c_i64), ("fock", c_vp), ("eps", c_vp), ("ints", ctypes.POINTER(SpInts)),
@@ -190,6 +202,8 @@ SIGNATURES = {
     "apy_reduce_workspace": (c_size, [c_i64, c_int]),
     "apy_eri_gto": (c_int, [c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "apy_eri_deriv_assemble": (c_int, [c_i64, c_vp, c_int, c_vp, c_vp, c_vp]),
+    "apy_oei_components": (c_int, [c_int]),
+    "apy_oei_gto": (c_int, [ctypes.POINTER(OeiArgs), c_vp]),
     "apy_reduce_seg_workspace": (c_size, [c_i64, c_int]),
     "apy_gs_coefficients": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp]),
     "apy_ci_packed_length": (c_i64, [c_i64, c_i64]),

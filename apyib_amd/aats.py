@@ -37,7 +37,7 @@ import torch
 
 from . import _lib
 from . import transforms as tr
-from .gaussian import build_mixed_overlap
+from .integrals import mixed_overlaps
 from .kernels import stream_of, workspace
 from .utils import device
 
@@ -145,7 +145,7 @@ class AAT(object):
         dev, nbf = self.dev, self.nbf
         nov = len(pairs)
         fn = lambda b: b.functions if hasattr(b, 'functions') else b
-        S_ao = np.stack([build_mixed_overlap(fn(bb), fn(kb)) for _, _, _, (bb, _), (kb, _) in pairs])
+        S_ao = torch.stack(mixed_overlaps([(fn(bb), fn(kb)) for _, _, _, (bb, _), (kb, _) in pairs], dev))
         Cb = np.stack([np.asarray(c, dtype=np.complex128) for _, _, _, (_, c), _ in pairs])
         Ck = np.stack([np.asarray(c, dtype=np.complex128) for _, _, _, _, (_, c) in pairs])
         S_mo = tr.mo_overlap(tr.to_dev(Cb, dev), tr.to_dev(S_ao, dev), tr.to_dev(Ck, dev))

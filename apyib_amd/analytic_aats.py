@@ -8,10 +8,10 @@ displacement and magnetic-field component and, for the correlated methods,
 the first-order amplitudes, with no displaced wavefunctions.
 
 Skeleton first-derivative integrals (the reference's psi4 mo_oei_deriv1,
-mo_tei_deriv1 and mo_overlap_half_deriv1): the one-electron ones from the
-package's host integral engine (gaussian.build_deriv1), the two-electron
-ones on the GPU (integrals.EriDerivatives: apy_eri_gto over the derivative
-functions, apy_eri_deriv_assemble per atom and axis).  On the GPU: the AO -> MO
+mo_tei_deriv1 and mo_overlap_half_deriv1) on the GPU: the one-electron
+ones from apy_oei_gto over the derivative functions
+(integrals.deriv1_one_electron), the two-electron ones from apy_eri_gto over
+them (integrals.EriDerivatives, apy_eri_deriv_assemble per atom and axis).  On the GPU: the AO -> MO
 transforms of the two-electron integrals and of every derivative integral
 set (apy_eri_mo), the one-electron transforms, the MP2 amplitudes, the
 four U-contractions of the first-order integrals per perturbation (n^5
@@ -25,10 +25,9 @@ AAT contractions done on the host, as the reference does.
 import numpy as np
 import torch
 
-from . import gaussian
 from . import transforms as tr
-from .hamiltonian import ANGULAR_MOMENTUM_SIGN, Hamiltonian
-from .integrals import EriDerivatives
+from .hamiltonian import ANGULAR_MOMENTUM_SIGN, build_hamiltonians
+from .integrals import EriDerivatives, deriv1_one_electron, hamiltonian_integrals
 from .hf_wfn import hf_wfn
 from .ci_wfn import ci_wfn
 from .kernels import zgemm_raw
@@ -68,10 +67,10 @@ class analytic_derivative(object):
 
     def __init__(self, parameters):
         self.parameters = parameters
-        self.H = Hamiltonian(self.parameters)
+        self.dev = device()
+        self.H = build_hamiltonians([self.parameters], self.dev)[0]
         self.wfn = hf_wfn(self.H)
         E_SCF, self.C = self.wfn.solve_SCF(self.parameters)
-        self.dev = device()
         self._ready = False
 
     # ------------------------------------------------------------ building blocks
@@ -101,12 +100,12 @@ class analytic_derivative(object):
                   F[v, v].reshape(nv, 1, nv, 1) - F[o, o].reshape(1, no, 1, no))
         self.G = np.linalg.inv((diag + self.A[v, o, v, o]).reshape(nv * no, nv * no))
         self.Gm = np.linalg.inv((diag + self.Amag[v, o, v, o]).reshape(nv * no, nv * no))
-        charges = list(zip(H.molecule.Z, H.molecule.coords))
-        self.natom = len(charges)
-        self.deriv = gaussian.build_deriv1(H.basis_set.functions, H.basis_set.atom_of, charges,
-                                           eri_derivatives=False)
+        self.natom = H.molecule.natom()
+        self.deriv = deriv1_one_electron(H.basis_set, H.molecule.coords, H.molecule.Z, self.dev)
         self.dERI = EriDerivatives(H.basis_set, H.molecule.coords, self.dev)
-        self.L = ANGULAR_MOMENTUM_SIGN * gaussian.build_angular_momentum(H.basis_set.functions)
+        L = hamiltonian_integrals(H.basis_set, np.asarray(H.molecule.coords)[None], H.molecule.Z, self.dev,
+                                  which=_lib.APY_OEI_ANGMOM)[0].cpu().numpy()
+        self.L = ANGULAR_MOMENTUM_SIGN * 0.5 * (L - L.swapaxes(1, 2))
         self._ready = True
 
     def _mo1(self, M):

@@ -66,19 +66,67 @@ __device__ void herm_E(int i, int j, double Q, double a, double b, double* out) 
 
 __device__ inline int lsum(const int* l) { return l[0] + l[1] + l[2]; }
 
+#define kNaN __longlong_as_double(0x7ff8000000000000LL)
+
+// A table entry inside the kernels' limits (centre index below ncen, 1..APY_GTO_MAXPRIM primitives, one or
+// two terms of angular momentum <= LF); entries outside them give NaN, never an out-of-bounds access.
+__device__ bool gto_ok(const apy_gto& g, int64_t ncen) {
+  if (g.atom < 0 || g.atom >= ncen || g.nprim < 1 || g.nprim > APY_GTO_MAXPRIM || g.nterm < 1 || g.nterm > 2)
+    return false;
+  for (int t = 0; t < g.nterm; ++t)
+    if (g.lmn[t][0] < 0 || g.lmn[t][1] < 0 || g.lmn[t][2] < 0 || lsum(g.lmn[t]) > LF) return false;
+  return true;
+}
+
+__host__ __device__ inline int apy_oei_ncomp(int which) {
+  if (which <= 0 || which > 31) return 0;
+  return (which & 1) + ((which >> 1) & 1) + ((which >> 2) & 1) + 3 * ((which >> 3) & 1) + 3 * ((which >> 4) & 1);
+}
+
+// Hermite Coulomb integrals R^0_tuv(alpha, PQ) for t <= Lt[0], u <= Lt[1], v <= Lt[2] (L = the sum, <= LQ;
+// (Lt[0]+1)(Lt[1]+1)(Lt[2]+1) <= NRB), from F = F_n(alpha |PQ|^2), n <= L:
+//   R^n_000 = (-2 alpha)^n F_n,  R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + PQ_x R^{n+1}_{tuv}  (and alike for u, v).
+// Level n is built from level n + 1 in the two alternating buffers Rb; the result is Rb[returned index]
+// at [t sy + u sz + v], sz = Lt[2]+1, sy = (Lt[1]+1) sz.
+__device__ int herm_R(int L, const int* Lt, double alpha, const double* PQ, const double* F, double (*Rb)[NRB]) {
+  const int sz = Lt[2] + 1, sy = (Lt[1] + 1) * sz;
+  int cur = 0;
+  for (int n = L; n >= 0; --n) {
+    const int nxt = cur ^ 1;  // level n + 1 lives in Rb[cur]
+    double* out = Rb[nxt];
+    const double* prev = Rb[cur];
+    const int lim = L - n;
+    for (int t = 0; t <= Lt[0]; ++t)
+      for (int u = 0; u <= Lt[1]; ++u)
+        for (int v = 0; v <= Lt[2]; ++v) {
+          if (t + u + v > lim) continue;
+          double r;
+          if (t > 0) {
+            r = PQ[0] * prev[(t - 1) * sy + u * sz + v];
+            if (t > 1) r += (t - 1) * prev[(t - 2) * sy + u * sz + v];
+          } else if (u > 0) {
+            r = PQ[1] * prev[u * sz - sz + v];
+            if (u > 1) r += (u - 1) * prev[(u - 2) * sz + v];
+          } else if (v > 0) {
+            r = PQ[2] * prev[v - 1];
+            if (v > 1) r += (v - 1) * prev[v - 2];
+          } else {
+            double s = 1.0;
+            for (int k = 0; k < n; ++k) s *= -2.0 * alpha;
+            r = s * F[n];
+          }
+          out[t * sy + u * sz + v] = r;
+        }
+    cur = nxt;
+  }
+  return cur;
+}
+
 // Contracted integral (a b | c d) at one geometry.  Returns NaN for a table entry outside the limits
 // (angular momentum, primitive count, term count, atom index), so a bad table cannot index out of bounds.
 __device__ double eri_quartet(const apy_gto& A, const apy_gto& B, const apy_gto& C, const apy_gto& D,
                               const double* cen, int64_t natom) {
-  const apy_gto* f[4] = {&A, &B, &C, &D};
-  for (int k = 0; k < 4; ++k) {
-    const apy_gto& g = *f[k];
-    if (g.atom < 0 || g.atom >= natom || g.nprim < 1 || g.nprim > APY_GTO_MAXPRIM || g.nterm < 1 || g.nterm > 2)
-      return __longlong_as_double(0x7ff8000000000000LL);
-    for (int t = 0; t < g.nterm; ++t)
-      if (g.lmn[t][0] < 0 || g.lmn[t][1] < 0 || g.lmn[t][2] < 0 || lsum(g.lmn[t]) > LF)
-        return __longlong_as_double(0x7ff8000000000000LL);
-  }
+  if (!gto_ok(A, natom) || !gto_ok(B, natom) || !gto_ok(C, natom) || !gto_ok(D, natom)) return kNaN;
   const double* ca = cen + 3 * A.atom;
   const double* cb = cen + 3 * B.atom;
   const double* cc = cen + 3 * C.atom;
@@ -98,7 +146,7 @@ __device__ double eri_quartet(const apy_gto& A, const apy_gto& B, const apy_gto&
           int L1[3], L2[3], Lt[3];
           for (int x = 0; x < 3; ++x) { L1[x] = la[x] + lb[x]; L2[x] = lc[x] + ld[x]; Lt[x] = L1[x] + L2[x]; }
           const int L = Lt[0] + Lt[1] + Lt[2];
-          if (L > LQ) return __longlong_as_double(0x7ff8000000000000LL);
+          if (L > LQ) return kNaN;
           const int sz = Lt[2] + 1, sy = (Lt[1] + 1) * sz;       // R index (t, u, v) -> t sy + u sz + v
           for (int pa = 0; pa < A.nprim; ++pa)
             for (int pb = 0; pb < B.nprim; ++pb) {
@@ -119,38 +167,8 @@ __device__ double eri_quartet(const apy_gto& A, const apy_gto& B, const apy_gto&
                   const double alpha = p * q / (p + q);
                   double F[LQ + 1];
                   boys_fn(L, alpha * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]), F);
-                  // Hermite Coulomb integrals: level n holds R^(n)_tuv for t + u + v <= L - n
                   double Rb[2][NRB];
-                  int cur = 0;
-                  for (int n = L; n >= 0; --n) {
-                    const int nxt = cur ^ 1;                          // level n + 1 lives in Rb[cur]
-                    double* out = Rb[nxt];
-                    const double* prev = Rb[cur];
-                    const int lim = L - n;
-                    for (int t = 0; t <= Lt[0]; ++t)
-                      for (int u = 0; u <= Lt[1]; ++u)
-                        for (int v = 0; v <= Lt[2]; ++v) {
-                          if (t + u + v > lim) continue;
-                          double r;
-                          if (t > 0) {
-                            r = PQ[0] * prev[(t - 1) * sy + u * sz + v];
-                            if (t > 1) r += (t - 1) * prev[(t - 2) * sy + u * sz + v];
-                          } else if (u > 0) {
-                            r = PQ[1] * prev[u * sz - sz + v];
-                            if (u > 1) r += (u - 1) * prev[(u - 2) * sz + v];
-                          } else if (v > 0) {
-                            r = PQ[2] * prev[v - 1];
-                            if (v > 1) r += (v - 1) * prev[v - 2];
-                          } else {
-                            double s = 1.0;
-                            for (int k = 0; k < n; ++k) s *= -2.0 * alpha;
-                            r = s * F[n];
-                          }
-                          out[t * sy + u * sz + v] = r;
-                        }
-                    cur = nxt;
-                  }
-                  const double* R = Rb[cur];
+                  const double* R = Rb[herm_R(L, Lt, alpha, PQ, F, Rb)];
                   double val = 0.0;
                   for (int t = 0; t <= L1[0]; ++t)
                     for (int u = 0; u <= L1[1]; ++u)
@@ -205,6 +223,122 @@ __global__ __launch_bounds__(256) void eri_deriv_assemble_kernel(int64_t n, cons
   }
 }
 
+
+// ------------------------------------------------------------------------------ one-electron integrals
+constexpr int OJ = LF + 3;  // ket powers j <= LF + 2 (the kinetic operator raises j by two)
+constexpr int NOEI = 9;     // S, T, V, Dx, Dy, Dz, Lx, Ly, Lz
+
+// 1-D overlaps S_ij = int x_A^i x_B^j exp(-a x_A^2 - b x_B^2) dx for i <= imax (<= LF), j <= jmax (< OJ),
+// Obara-Saika: S_00 = sqrt(pi/p) exp(-ab/p Q^2) with Q = A - B, p = a + b, and
+//   S_{i+1,j} = X_PA S_ij + (i S_{i-1,j} + j S_{i,j-1}) / 2p,   X_PA = -b Q / p,
+//   S_{i,j+1} = X_PB S_ij + (i S_{i-1,j} + j S_{i,j-1}) / 2p,   X_PB =  a Q / p.
+__device__ void ovl_1d(int imax, int jmax, double Q, double a, double b, double (*S)[OJ]) {
+  const double p = a + b, h = 0.5 / p, xpa = -b * Q / p, xpb = a * Q / p;
+  S[0][0] = sqrt(M_PI / p) * exp(-a * b / p * Q * Q);
+  for (int j = 0; j < jmax; ++j) S[0][j + 1] = xpb * S[0][j] + (j > 0 ? j * h * S[0][j - 1] : 0.0);
+  for (int i = 0; i < imax; ++i)
+    for (int j = 0; j <= jmax; ++j) {
+      double v = xpa * S[i][j];
+      if (i > 0) v += i * h * S[i - 1][j];
+      if (j > 0) v += j * h * S[i][j - 1];
+      S[i + 1][j] = v;
+    }
+}
+
+// The integrals `which` selects between bra function A at ra and ket function B at rb, accumulated into
+// v[NOEI].  Per axis, from the 1-D overlaps S_ij (i = bra power, j = ket power):
+//   kinetic    t = -1/2 (j(j-1) S_{i,j-2} - 2b(2j+1) S_ij + 4b^2 S_{i,j+2}),
+//   moment     m = S_{i,j+1} + (B - O) S_ij             (x - O = x_B + B - O),
+//   derivative d = j S_{i,j-1} - 2b S_{i,j+1}           (d/dx acting on the ket);
+//   T = t_x s_y s_z + s_x t_y s_z + s_x s_y t_z,  D_x = m_x s_y s_z,  L_x = (m_y d_z - m_z d_y) s_x (cyclic).
+// The nuclear attraction uses the Hermite expansion of the product:
+//   V = -sum_C Z_C 2 pi / p sum_tuv E^x_t E^y_u E^z_v R_tuv(p, P - C).
+__device__ void oei_pair(const apy_gto& A, const double* ra, const apy_gto& B, const double* rb, int which,
+                         const double* origin, const double* nuc, const double* Z, int64_t nnuc, double* v) {
+  const double Q[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
+  for (int ta = 0; ta < A.nterm; ++ta)
+    for (int tb = 0; tb < B.nterm; ++tb) {
+      const int* la = A.lmn[ta];
+      const int* lb = B.lmn[tb];
+      for (int pa = 0; pa < A.nprim; ++pa)
+        for (int pb = 0; pb < B.nprim; ++pb) {
+          const double a = A.exps[pa], b = B.exps[pb], p = a + b;
+          const double c = A.coefs[ta][pa] * B.coefs[tb][pb];
+          double S[3][LF + 1][OJ], s[3], t[3], m[3], d[3];
+          for (int x = 0; x < 3; ++x) {
+            const int i = la[x], j = lb[x];
+            ovl_1d(i, j + 2, Q[x], a, b, S[x]);
+            s[x] = S[x][i][j];
+            t[x] = -0.5 * ((j > 1 ? j * (j - 1) * S[x][i][j - 2] : 0.0) - 2.0 * b * (2 * j + 1) * S[x][i][j] +
+                           4.0 * b * b * S[x][i][j + 2]);
+            m[x] = S[x][i][j + 1] + (rb[x] - origin[x]) * S[x][i][j];
+            d[x] = (j > 0 ? j * S[x][i][j - 1] : 0.0) - 2.0 * b * S[x][i][j + 1];
+          }
+          if (which & APY_OEI_OVERLAP) v[0] += c * s[0] * s[1] * s[2];
+          if (which & APY_OEI_KINETIC)
+            v[1] += c * (t[0] * s[1] * s[2] + s[0] * t[1] * s[2] + s[0] * s[1] * t[2]);
+          if (which & APY_OEI_DIPOLE) {
+            v[3] += c * m[0] * s[1] * s[2];
+            v[4] += c * s[0] * m[1] * s[2];
+            v[5] += c * s[0] * s[1] * m[2];
+          }
+          if (which & APY_OEI_ANGMOM) {
+            v[6] += c * (m[1] * d[2] - m[2] * d[1]) * s[0];
+            v[7] += c * (m[2] * d[0] - m[0] * d[2]) * s[1];
+            v[8] += c * (m[0] * d[1] - m[1] * d[0]) * s[2];
+          }
+          if (which & APY_OEI_POTENTIAL) {
+            double E[3][NE], P[3];
+            int Lt[3];
+            for (int x = 0; x < 3; ++x) {
+              herm_E(la[x], lb[x], Q[x], a, b, E[x]);
+              P[x] = (a * ra[x] + b * rb[x]) / p;
+              Lt[x] = la[x] + lb[x];
+            }
+            const int L = Lt[0] + Lt[1] + Lt[2];
+            const int sz = Lt[2] + 1, sy = (Lt[1] + 1) * sz;
+            double acc = 0.0;
+            for (int64_t C = 0; C < nnuc; ++C) {
+              const double PC[3] = {P[0] - nuc[3 * C], P[1] - nuc[3 * C + 1], P[2] - nuc[3 * C + 2]};
+              double F[LQ + 1], Rb[2][NRB];
+              boys_fn(L, p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), F);
+              const double* R = Rb[herm_R(L, Lt, p, PC, F, Rb)];
+              double sum = 0.0;
+              for (int tt = 0; tt <= Lt[0]; ++tt)
+                for (int u = 0; u <= Lt[1]; ++u)
+                  for (int w = 0; w <= Lt[2]; ++w) sum += E[0][tt] * E[1][u] * E[2][w] * R[tt * sy + u * sz + w];
+              acc -= Z[C] * sum;
+            }
+            v[2] += c * 2.0 * M_PI / p * acc;
+          }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void oei_kernel(apy_oei_args p) {
+  const int64_t per = p.na * p.nb, total = p.nset * per;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = e / per, r = e % per, i = r / p.nb, j = r % p.nb;
+    const apy_gto& A = p.fa[i];
+    const apy_gto& B = p.fb[j];
+    double v[NOEI];
+    for (int k = 0; k < NOEI; ++k) v[k] = 0.0;
+    if (gto_ok(A, p.nca) && gto_ok(B, p.ncb)) {
+      oei_pair(A, p.ca + 3 * (s * p.nca + A.atom), B, p.cb + 3 * (s * p.ncb + B.atom), p.which, p.origin,
+               p.nuclei ? p.nuclei + 3 * s * p.nnuc : nullptr, p.charges, p.nnuc, v);
+    } else {
+      for (int k = 0; k < NOEI; ++k) v[k] = kNaN;
+    }
+    // selected components in the order S, T, V, Dx, Dy, Dz, Lx, Ly, Lz
+    constexpr int bit_of[NOEI] = {APY_OEI_OVERLAP, APY_OEI_KINETIC, APY_OEI_POTENTIAL, APY_OEI_DIPOLE, APY_OEI_DIPOLE,
+                                  APY_OEI_DIPOLE,  APY_OEI_ANGMOM,  APY_OEI_ANGMOM,    APY_OEI_ANGMOM};
+    int64_t c = 0;
+    const int64_t ncomp = apy_oei_ncomp(p.which);
+#pragma unroll
+    for (int k = 0; k < NOEI; ++k)
+      if (p.which & bit_of[k]) p.out[((s * ncomp + c++) * p.na + i) * p.nb + j] = v[k];
+  }
+}
 }  // namespace
 
 extern "C" {
@@ -227,6 +361,19 @@ int apy_eri_deriv_assemble(int64_t n, const int* atom_of, int atom, const void* 
   const int64_t total = n * n * n * n;
   eri_deriv_assemble_kernel<<<apy_grid1d(total), 256, 0, (hipStream_t)stream>>>(n, atom_of, atom, (const double2*)D,
                                                                                  (double2*)out);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_oei_components(int which) { return apy_oei_ncomp(which); }
+
+int apy_oei_gto(const apy_oei_args* a, void* stream) {
+  if (!a || apy_oei_ncomp(a->which) == 0 || a->nset < 1 || !a->fa || !a->fb || a->na < 1 || a->nb < 1 || !a->ca ||
+      !a->cb || a->nca < 1 || a->ncb < 1 || !a->out)
+    return APY_E_ARG;
+  if ((a->which & APY_OEI_POTENTIAL) && (!a->nuclei || !a->charges || a->nnuc < 1)) return APY_E_ARG;
+  const int64_t total = a->nset * a->na * a->nb;
+  oei_kernel<<<apy_grid1d(total), 256, 0, (hipStream_t)stream>>>(*a);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }

@@ -1,6 +1,6 @@
 """Method dispatch with apyib's interface (energy.py:17-150)."""
 from .ci_wfn import ci_wfn
-from .hamiltonian import Hamiltonian
+from .hamiltonian import build_hamiltonians
 from .hf_wfn import hf_wfn, solve_SCF_batch
 from .integrals import prime_device_eri
 from .mp2_wfn import mp2_wfn
@@ -28,7 +28,7 @@ def _solve(parameters, wfn, print_level):
 def energy(parameters, return_H=False, print_level=0):
     """energy.py:17-81.  Returns (E_list, T_list, C, basis[, H]) with
     E_list = [E_SCF, E_corr, E_nuc] and T_list = [1, t1, t2]."""
-    H = Hamiltonian(parameters)
+    H = build_hamiltonians([parameters], device())[0]
     wfn = hf_wfn(H)
     E_SCF, C = wfn.solve_SCF(parameters, print_level)
     basis = H.basis_set
@@ -51,7 +51,7 @@ def energy(parameters, return_H=False, print_level=0):
 def phase_corrected_energy(parameters, unperturbed_basis, unperturbed_C, return_H=False, print_level=0):
     """energy.py:84-150: the MO phases are aligned to the unperturbed
     reference (compute_phase) before the correlated solve."""
-    H = Hamiltonian(parameters)
+    H = build_hamiltonians([parameters], device())[0]
     wfn = hf_wfn(H)
     E_SCF, C = wfn.solve_SCF(parameters, print_level)
     basis = H.basis_set
@@ -74,12 +74,13 @@ def phase_corrected_energy(parameters, unperturbed_basis, unperturbed_C, return_
 def phase_corrected_energies(parameter_list, unperturbed_basis, unperturbed_C, print_level=0):
     """phase_corrected_energy for a list of parameter sets (the displaced
     geometries and fields of fin_diff.py:267-372), with the Hartree-Fock
-    part batched: every AO integral set in one launch per basis, the SCFs
+    part batched: every AO integral set (one- and two-electron) in one
+    launch per basis, the SCFs
     iterated together (hf_wfn.solve_SCF_batch, one Fock build per iteration
     for all of them); the phase alignment and the correlated solve then run
     per instance.  Each result equals phase_corrected_energy's.  Returns
     [(E_list, T_list, C, basis)]."""
-    Hs = [Hamiltonian(p) for p in parameter_list]
+    Hs = build_hamiltonians(parameter_list, device())
     prime_device_eri(Hs, device())
     wfns = [hf_wfn(H) for H in Hs]
     out = []

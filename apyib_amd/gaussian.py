@@ -365,7 +365,6 @@ def build_deriv1(basis, atom_of, charges, eri_derivatives=True):
                 dT[x, m, k] = _dsum(kinetic, terms, basis[k])
                 for C, (Z, R) in enumerate(charges):
                     dVC[C, x, m, k] = _dsum(potential, terms, basis[k], [(Z, R)])
-    dV = dVC.sum(0)
     # D[x][m,n,l,s] = (d m/dx n | l s), (l<=s) computed, (l,s) symmetric
     D = np.zeros((3, n, n, n, n)) if eri_derivatives else None
     for x in range(3 if eri_derivatives else 0):
@@ -376,6 +375,17 @@ def build_deriv1(basis, atom_of, charges, eri_derivatives=True):
                     for s in range(l + 1):
                         v = sum(eri(t, basis[q], basis[l], basis[s]) for t in terms)
                         D[x, m, q, l, s] = D[x, m, q, s, l] = v
+    return assemble_deriv1(dS, dT, dVC, atom_of, D)
+
+
+def assemble_deriv1(dS, dT, dVC, atom_of, D=None):
+    """The per-atom derivative integrals of build_deriv1 from the bra-side
+    derivative integrals dS[x] = <d m/dx | n>, dT[x] = <d m/dx | T | n>,
+    dVC[C, x] = <d m/dx | V_C | n> (V_C the attraction of nucleus C alone)
+    and, when given, D[x] = (d m/dx n | l s)."""
+    natom = dVC.shape[0]
+    atom_of = np.asarray(atom_of)
+    dV = dVC.sum(0)
     out = {'S': [], 'T': [], 'V': [], 'half_S': [], 'ERI': []}
     for A in range(natom):
         on = (atom_of == A).astype(float)
@@ -389,7 +399,7 @@ def build_deriv1(basis, atom_of, charges, eri_derivatives=True):
             v = on[:, None] * dV[x]
             op = dVC[A, x] + dVC[A, x].T
             mV.append(v + v.T - op)
-            if eri_derivatives:
+            if D is not None:
                 Dx = D[x]
                 e = (on[:, None, None, None] * Dx
                      + on[None, :, None, None] * Dx.transpose(1, 0, 2, 3)

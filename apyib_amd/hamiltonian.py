@@ -11,6 +11,8 @@ and magnetic fields enter V as in hamiltonian.py:41-70:
 The phase-space extensions (hamiltonian.py:72-538) are not part of this path.
 """
 
+import numpy as np
+
 from . import gaussian
 from .molecule import BasisSet, Molecule
 
@@ -23,12 +25,19 @@ ANGULAR_MOMENTUM_SIGN = -1.0
 
 
 class Hamiltonian(object):
-    def __init__(self, parameters):
+    def __init__(self, parameters, integrals=None):
+        """integrals: None (built here by the host engine) or the AO arrays
+        (S, T, V, D[3], L[3]) of this molecule and basis, L = <m|r x nabla|n>
+        (as build_hamiltonians passes them from the GPU)."""
         self.molecule = Molecule.from_string(parameters['geom'])
         self.basis_set = BasisSet(self.molecule, parameters['basis'])
         fns = self.basis_set.functions
-        charges = list(zip(self.molecule.Z, self.molecule.coords))
-        self.S, self.T, self.V = gaussian.build_one_electron(fns, charges)
+        if integrals is None:
+            charges = list(zip(self.molecule.Z, self.molecule.coords))
+            self.S, self.T, self.V = gaussian.build_one_electron(fns, charges)
+            D = L = None
+        else:
+            self.S, self.T, self.V, D, L = integrals
         self._ERI = None
 
         F_el = list(parameters.get('F_el', [0.0, 0.0, 0.0]))
@@ -36,10 +45,10 @@ class Hamiltonian(object):
         E_field = any(f != 0.0 for f in F_el)
         M_field = any(f != 0.0 for f in F_mag)
         if E_field:
-            D = gaussian.build_dipole(fns)
+            D = gaussian.build_dipole(fns) if D is None else D
             self.mu_el = [-D[a] for a in range(3)]
         if M_field:
-            L = ANGULAR_MOMENTUM_SIGN * gaussian.build_angular_momentum(fns)
+            L = ANGULAR_MOMENTUM_SIGN * (gaussian.build_angular_momentum(fns) if L is None else L)
             self.mu_mag = [-0.5j * L[a] for a in range(3)]
         self.E_nuc = self.molecule.nuclear_repulsion_energy([-f for f in F_el])
         for a in range(3):
@@ -67,3 +76,25 @@ class Hamiltonian(object):
             t = ao_eri(self.basis_set, self.molecule.coords, device)
             self._device_cache[key] = t
         return t
+
+
+def build_hamiltonians(parameter_list, device):
+    """[Hamiltonian(p) for p in parameter_list] with the one-electron
+    integrals (S, T, V, dipole, angular momentum) of all of them computed on
+    the device, one apy_oei_gto launch per basis (the same functions on the
+    same atoms at different geometries), and one copy to the host."""
+    from .integrals import hamiltonian_integrals
+    mols = [Molecule.from_string(p['geom']) for p in parameter_list]
+    groups = {}
+    for q, (p, m) in enumerate(zip(parameter_list, mols)):
+        groups.setdefault((str(p['basis']).upper(), tuple(int(z) for z in m.Z)), []).append(q)
+    ints = [None] * len(parameter_list)
+    for qs in groups.values():
+        bs = BasisSet(mols[qs[0]], parameter_list[qs[0]]['basis'])
+        coords = np.stack([np.asarray(mols[q].coords, dtype=np.float64) for q in qs])
+        out = hamiltonian_integrals(bs, coords, mols[qs[0]].Z, device).cpu().numpy()
+        for k, q in enumerate(qs):
+            o = out[k]
+            sym = lambda x: 0.5 * (x + x.swapaxes(-1, -2))
+            ints[q] = (sym(o[0]), sym(o[1]), sym(o[2]), sym(o[3:6]), 0.5 * (o[6:9] - o[6:9].swapaxes(-1, -2)))
+    return [Hamiltonian(p, integrals=x) for p, x in zip(parameter_list, ints)]

@@ -1,7 +1,7 @@
 """Complex restricted Hartree-Fock with apyib's interface (hf_wfn.py:9-170).
 
 The upstream producer of C and eps for the correlated path.  The O(nbf^4)
-Fock build runs on the GPU (apy_fock_build); the nbf x nbf algebra
+Fock build runs on the GPU (apy_fock_build_batch); the nbf x nbf algebra
 (orthogonalisation, eigh, DIIS on Fock matrices) is the reference's small
 host-side linear algebra.
 """
@@ -10,7 +10,28 @@ import scipy.linalg as la
 import torch
 
 from . import transforms as tr
-from .utils import device, solve_general_DIIS
+from .utils import device
+
+
+def _fock_diis(res_vec, F_vec, e_iter, F_iter, iteration, max_DIIS=7):
+    """solve_general_DIIS (utils.py:104-140, as hf_wfn.py:91 calls it) on the
+    host: the SCF's vectors hold nbf^2 elements, far too few for a device
+    round trip.  Keeps at most max_DIIS + 1 (error, Fock) pairs (the oldest
+    dropped before appending), solves the bordered system
+    [[E^H E, -1], [-1, 0]] c = [0, ..., 0, -1] and returns sum_j c_j F_j."""
+    while e_iter.shape[1] > max_DIIS:
+        e_iter, F_iter = e_iter[:, 1:], F_iter[:, 1:]
+    if iteration != 1:
+        e_iter = np.hstack((e_iter, np.atleast_2d(res_vec).T))
+        F_iter = np.hstack((F_iter, np.atleast_2d(F_vec).T))
+    nv = e_iter.shape[1]
+    B = -np.ones((nv + 1, nv + 1), dtype=np.result_type(e_iter.dtype, np.float64))
+    B[:nv, :nv] = np.conjugate(e_iter).T @ e_iter
+    B[-1, -1] = 0
+    A = np.zeros(nv + 1)
+    A[-1] = -1
+    c = np.linalg.solve(B, A)
+    return F_iter @ c[:-1], e_iter, F_iter
 
 
 class hf_wfn(object):
@@ -71,8 +92,7 @@ class _SCF:
             if i == 1:
                 self.F_iter = np.atleast_2d(F_vec).T
                 self.e_iter = np.atleast_2d(res_vec).T
-            F_vec, self.e_iter, self.F_iter = solve_general_DIIS(parameters, res_vec, F_vec, self.e_iter,
-                                                                 self.F_iter, i)
+            F_vec, self.e_iter, self.F_iter = _fock_diis(res_vec, F_vec, self.e_iter, self.F_iter, i)
             F = np.reshape(F_vec, (wfn.nbf, wfn.nbf))
         wfn.eps, C_p = np.linalg.eigh(X @ F @ X)
         C = wfn.C = X @ C_p

@@ -1,4 +1,4 @@
-"""AO electron-repulsion integrals on the GPU for the package's basis sets.
+"""AO integrals on the GPU for the package's basis sets.
 
 Replaces psi4's MintsHelper.ao_eri (hamiltonian.py:33) and the two-electron
 part of mo_tei_deriv1 (analytic_aats.py:94) on the device: the basis (and
@@ -7,6 +7,11 @@ its nuclear derivatives) become apy_gto tables and apy_eri_gto evaluates
 forms d(mn|ls)/dR for one atom and axis from the derivative-function
 integrals.  Results are complex128 device tensors, the form every consumer
 on the GPU (Fock build, AO -> MO transforms) reads.
+
+The one-electron integrals of hamiltonian.py:29-35, 49, 55 (overlap,
+kinetic, nuclear attraction, dipole, angular momentum) and the
+mixed-geometry overlaps of utils.py:371-376 come from apy_oei_gto, again
+for many geometries (or bra/ket geometry pairs) per launch.
 """
 import ctypes
 
@@ -14,7 +19,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .gaussian import center_derivative
+from .gaussian import assemble_deriv1, center_derivative
 from .kernels import stream_of
 
 
@@ -57,6 +62,14 @@ def derivative_table(functions, atom_of, axis, device):
     return _upload(out, device)
 
 
+def function_table(functions, device):
+    """Device apy_gto table in which every function is its own centre
+    (entry i at centre i), and the [n, 3] host array of those centres: the
+    form for function lists that carry their centres (mixed overlaps)."""
+    tab = _upload([_entry(i, [(f.lmn, f.coefs, f.exps)]) for i, f in enumerate(functions)], device)
+    return tab, np.stack([np.asarray(f.center, dtype=np.float64) for f in functions])
+
+
 def _centers(coords, device):
     c = np.ascontiguousarray(np.asarray(coords, dtype=np.float64))
     return torch.from_numpy(c).to(device)
@@ -88,6 +101,101 @@ def ao_eri(basis_set, coords, device):
     one = c.ndim == 2
     out = eri(tab, tab, tab, tab, (n, n, n, n), c[None] if one else c, device)
     return out[0] if one else out
+
+
+OEI_ALL = (_lib.APY_OEI_OVERLAP | _lib.APY_OEI_KINETIC | _lib.APY_OEI_POTENTIAL | _lib.APY_OEI_DIPOLE
+           | _lib.APY_OEI_ANGMOM)
+
+
+def one_electron(fa, ca, fb, cb, which, device, nuclei=None, charges=None, origin=(0.0, 0.0, 0.0)):
+    """apy_oei_gto over the device tables fa (bra) and fb (ket) with host
+    centres ca [nset, nca, 3] and cb [nset, ncb, 3] (bohr); nuclei
+    [nset, nnuc, 3] and charges [nnuc] for the nuclear attraction.  Returns a
+    float64 device tensor [nset, ncomp, na, nb], the components `which`
+    selects in the order S, T, V, Dx, Dy, Dz, Lx, Ly, Lz."""
+    lib = _lib.load()
+    ncomp = int(lib.apy_oei_components(int(which)))
+    if ncomp == 0:
+        raise ValueError(f"invalid one-electron component selection {which}")
+    na, nb = fa.numel() // ctypes.sizeof(_lib.Gto), fb.numel() // ctypes.sizeof(_lib.Gto)
+    ca, cb = np.asarray(ca, dtype=np.float64), np.asarray(cb, dtype=np.float64)
+    if ca.ndim != 3 or cb.ndim != 3 or ca.shape[0] != cb.shape[0] or ca.shape[2] != 3 or cb.shape[2] != 3:
+        raise ValueError("centres must be [nset, ncentre, 3] with the same nset")
+    nset = ca.shape[0]
+    dca, dcb = _centers(ca, device), _centers(cb, device)
+    dnuc = dchg = None
+    nnuc = 0
+    if which & _lib.APY_OEI_POTENTIAL:
+        nuc = np.asarray(nuclei, dtype=np.float64)
+        if nuc.ndim != 3 or nuc.shape[0] != nset or nuc.shape[2] != 3 or len(charges) != nuc.shape[1]:
+            raise ValueError("nuclei must be [nset, nnuc, 3] with one charge per nucleus")
+        dnuc, dchg, nnuc = _centers(nuc, device), _centers(charges, device), nuc.shape[1]
+    out = torch.empty((nset, ncomp, na, nb), dtype=torch.float64, device=device)
+    P = lambda t: t.data_ptr() if t is not None else None
+    args = _lib.OeiArgs(nset, P(fa), na, P(dca), ca.shape[1], P(fb), nb, P(dcb), cb.shape[1], P(dnuc), P(dchg), nnuc,
+                        (ctypes.c_double * 3)(*[float(x) for x in origin]), int(which), 0, P(out))
+    _lib.check(lib.apy_oei_gto(ctypes.byref(args), stream_of(device)), "apy_oei_gto")
+    return out
+
+
+def hamiltonian_integrals(basis_set, coords, charges, device, which=OEI_ALL):
+    """One-electron integrals of a BasisSet at several geometries
+    ([ngeom, natom, 3], bohr; functions follow their atoms, nuclei of the
+    given charges at the same positions): float64 device tensor
+    [ngeom, ncomp, n, n] (components as in one_electron)."""
+    tab = basis_table(basis_set.functions, basis_set.atom_of, device)
+    c = np.asarray(coords, dtype=np.float64)
+    return one_electron(tab, c, tab, c, which, device, nuclei=c, charges=np.asarray(charges, dtype=np.float64))
+
+
+def deriv1_one_electron(basis_set, coords, charges, device):
+    """The one-electron part of gaussian.build_deriv1 (per atom A and axis x:
+    d/dR_Ax of S, T, V, and half_S = <d m/dR_Ax | n>; the AO integrals behind
+    mo_oei_deriv1 and mo_overlap_half_deriv1 of analytic_aats.py:84-96) from
+    two apy_oei_gto launches: the three derivative tables stacked as one bra
+    table of 3n entries against the basis, giving <d m/dx | n> and
+    <d m/dx | T | n>, then <d m/dx | V_C | n> for every nucleus C alone (one
+    set per nucleus at unit charge, scaled by Z_C); assembled on the host."""
+    fns, atom_of = basis_set.functions, basis_set.atom_of
+    n = len(fns)
+    c = np.asarray(coords, dtype=np.float64)
+    natom = c.shape[0]
+    Z = np.asarray(charges, dtype=np.float64)
+    dtab = torch.cat([derivative_table(fns, atom_of, x, device) for x in range(3)])
+    tab = basis_table(fns, atom_of, device)
+    ST = one_electron(dtab, c[None], tab, c[None], _lib.APY_OEI_OVERLAP | _lib.APY_OEI_KINETIC, device)
+    crep = np.ascontiguousarray(np.broadcast_to(c, (natom, natom, 3)))
+    VC = one_electron(dtab, crep, tab, crep, _lib.APY_OEI_POTENTIAL, device, nuclei=c[:, None, :],
+                      charges=np.ones(1))
+    ST, VC = ST[0].cpu().numpy(), VC[:, 0].cpu().numpy()
+    dVC = VC.reshape(natom, 3, n, n) * Z[:, None, None, None]
+    return assemble_deriv1(ST[0].reshape(3, n, n), ST[1].reshape(3, n, n), dVC, atom_of)
+
+
+def _signature(functions):
+    return tuple((tuple(int(x) for x in f.lmn), np.asarray(f.exps).tobytes(), np.asarray(f.coefs).tobytes())
+                 for f in functions)
+
+
+def mixed_overlaps(pairs, device):
+    """<bra_i|ket_j> for a list of (bra functions, ket functions) pairs, each
+    function at its own centre (utils.py:371-376 for every pair): pairs with
+    the same function shapes share one apy_oei_gto launch.  Returns a list of
+    float64 device tensors [na, nb] in the order of `pairs`."""
+    groups = {}
+    for q, (bra, ket) in enumerate(pairs):
+        groups.setdefault((_signature(bra), _signature(ket)), []).append(q)
+    out = [None] * len(pairs)
+    for qs in groups.values():
+        bra0, ket0 = pairs[qs[0]]
+        fa, _ = function_table(bra0, device)
+        fb, _ = function_table(ket0, device)
+        ca = np.stack([[np.asarray(f.center, dtype=np.float64) for f in pairs[q][0]] for q in qs])
+        cb = np.stack([[np.asarray(f.center, dtype=np.float64) for f in pairs[q][1]] for q in qs])
+        S = one_electron(fa, ca, fb, cb, _lib.APY_OEI_OVERLAP, device)
+        for k, q in enumerate(qs):
+            out[q] = S[k, 0]
+    return out
 
 
 class EriDerivatives:

@@ -9,15 +9,13 @@ accepted for signature compatibility and not used.
 from .aats import AAT
 from .energy import energy
 from .fin_diff import finite_difference
-from .hamiltonian import Hamiltonian
 from .hf_wfn import hf_wfn
 
 
 def compute_parallel_aats(parameters, nuc_pert_strength, mag_pert_strength, normalization='full', num_processes=4):
-    E_list, T_list, C, basis = energy(parameters)
+    E_list, T_list, C, basis, H = energy(parameters, return_H=True)
     print("Total Energy: ", E_list[0] + E_list[1] + E_list[2])
-    H = Hamiltonian(parameters)
-    wfn = hf_wfn(H)
+    wfn = hf_wfn(H)    # parallel.py:19-20 builds the same Hamiltonian a second time
     fd = finite_difference(parameters, basis, C)
     (nuc_pos_C, nuc_neg_C, nuc_pos_basis, nuc_neg_basis, nuc_pos_T, nuc_neg_T, mag_pos_C, mag_neg_C, mag_pos_basis,
      mag_neg_basis, mag_pos_T, mag_neg_T) = fd.compute_AAT(nuc_pert_strength, mag_pert_strength)

@@ -11,7 +11,7 @@ import torch
 
 from . import _lib
 from . import transforms as tr
-from .gaussian import build_mixed_overlap
+from .integrals import mixed_overlaps
 from .kernels import stream_of, workspace
 
 CDT = torch.complex128
@@ -139,10 +139,9 @@ def solve_general_DIIS(parameters, res_vec, t_vec, e_iter, t_iter, iteration, mi
 
 
 def ao_overlap(bra_basis, ket_basis):
-    """AO overlap <bra|ket> (mixed geometries allowed; utils.py:371-376)."""
-    if bra_basis == ket_basis:
-        return build_mixed_overlap(bra_basis.functions, bra_basis.functions)
-    return build_mixed_overlap(bra_basis.functions, ket_basis.functions)
+    """AO overlap <bra|ket> (mixed geometries allowed; utils.py:371-376),
+    computed on the device (apy_oei_gto): float64 device tensor."""
+    return mixed_overlaps([(bra_basis.functions, ket_basis.functions)], device())[0]
 
 
 def compute_mo_overlap(ndocc, nbf, bra_basis, bra_wfn, ket_basis, ket_wfn):

@@ -518,6 +518,48 @@ int apy_eri_gto(int64_t ngeom, int64_t natom, const double* centers, const apy_g
  * complex128 [n]^4, atom_of int32 [n] in device memory. */
 int apy_eri_deriv_assemble(int64_t n, const int* atom_of, int atom, const void* D, void* out, void* stream);
 
+/* One-electron integrals over two apy_gto tables (replace psi4's
+ * MintsHelper ao_overlap, ao_kinetic, ao_potential, ao_dipole and
+ * ao_angular_momentum of hamiltonian.py:29-35, 49, 55 and the mixed-geometry
+ * ao_overlap(bra_basis, ket_basis) of utils.py:371-376), for nset sets at
+ * once; every array in device memory:
+ *   out[s][c][i][j], c over the selected components in the order
+ *   S, T, V, Dx, Dy, Dz, Lx, Ly, Lz:
+ *   S  = <a_i|b_j>,  T = <a_i| -1/2 nabla^2 |b_j>,
+ *   V  = <a_i| -sum_C Z_C / |r - C| |b_j>   (nuclei[s][C], charges[C]),
+ *   D  = <a_i| r - origin |b_j>,
+ *   L  = <a_i| (r - origin) x nabla |b_j>   (real, antisymmetric for a = b).
+ * Bra function i sits at ca[s][fa[i].atom], ket function j at
+ * cb[s][fb[j].atom] (bohr), so bra and ket may belong to different
+ * geometries.  Limits as for apy_eri_gto (lmn summing to <= 2); an entry
+ * outside them yields NaN. */
+#define APY_OEI_OVERLAP 1
+#define APY_OEI_KINETIC 2
+#define APY_OEI_POTENTIAL 4
+#define APY_OEI_DIPOLE 8
+#define APY_OEI_ANGMOM 16
+typedef struct apy_oei_args {
+  int64_t nset;
+  const apy_gto* fa;                          /* bra table [na]               */
+  int64_t na;
+  const double* ca;                           /* bra centres [nset][nca][3]   */
+  int64_t nca;
+  const apy_gto* fb;                          /* ket table [nb]               */
+  int64_t nb;
+  const double* cb;                           /* ket centres [nset][ncb][3]   */
+  int64_t ncb;
+  const double* nuclei;                       /* [nset][nnuc][3] (V only)     */
+  const double* charges;                      /* [nnuc]          (V only)     */
+  int64_t nnuc;
+  double origin[3];                           /* for D and L                  */
+  int32_t which;                              /* OR of APY_OEI_* bits         */
+  int32_t pad;
+  double* out;                                /* [nset][ncomp][na][nb]        */
+} apy_oei_args;                               /* 136 bytes                    */
+/* Number of output components that `which` selects (0 if invalid). */
+int apy_oei_components(int which);
+int apy_oei_gto(const apy_oei_args* args, void* stream);
+
 /* ------------------------------------------------------------------------
  * Vector reductions used by DIIS and the convergence test.
  * ---------------------------------------------------------------------- */

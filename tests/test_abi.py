@@ -121,3 +121,22 @@ def test_gto_table_layout_and_argument_errors():
     assert lib.apy_eri_gto(0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, None) == _lib.APY_E_ARG
     assert lib.apy_eri_deriv_assemble(4, None, 0, None, None, None) == _lib.APY_E_ARG
     assert lib.apy_eri_deriv_assemble(4, 1, 0, 8, 8, None) == _lib.APY_E_ARG
+
+
+def test_one_electron_args_layout_and_errors():
+    """apy_oei_args is 136 bytes (header); the component count follows the
+    selection bits; invalid selections, empty sets, null tables and a
+    potential request without nuclei are rejected before any device work."""
+    assert ctypes.sizeof(_lib.OeiArgs) == 136
+    assert _lib.OeiArgs.origin.offset == 96 and _lib.OeiArgs.out.offset == 128
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    assert [lib.apy_oei_components(w) for w in (0, 1, 2, 4, 8, 16, 31, 32, -1)] == [0, 1, 1, 1, 3, 3, 9, 0, 0]
+    good = dict(nset=1, fa=8, na=1, ca=8, nca=1, fb=8, nb=1, cb=8, ncb=1, nuclei=8, charges=8, nnuc=1, which=31,
+                out=8)
+    for bad in (dict(which=0), dict(which=32), dict(nset=0), dict(fa=None), dict(na=0), dict(cb=None),
+                dict(ncb=0), dict(out=None), dict(nuclei=None), dict(charges=None), dict(nnuc=0)):
+        a = _lib.OeiArgs(**{**good, **bad})
+        assert lib.apy_oei_gto(ctypes.byref(a), None) == _lib.APY_E_ARG, bad
+    assert lib.apy_oei_gto(None, None) == _lib.APY_E_ARG
