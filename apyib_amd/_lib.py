@@ -203,6 +203,9 @@ SIGNATURES = {
     "apy_eri_gto": (c_int, [c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "apy_eri_deriv_assemble": (c_int, [c_i64, c_vp, c_int, c_vp, c_vp, c_vp]),
     "apy_oei_components": (c_int, [c_int]),
+    "apy_diis_slot": (c_int, [c_int, c_int, ctypes.c_uint32, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "apy_zcombine_guarded": (c_int, [c_int, _P_VP, c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "apy_iter_check": (c_int, [c_vp, c_vp, c_int, c_dbl, c_dbl, c_int, c_vp, c_vp, c_int, c_vp, c_vp]),
     "apy_oei_gto": (c_int, [ctypes.POINTER(OeiArgs), c_vp]),
     "apy_reduce_seg_workspace": (c_size, [c_i64, c_int]),
     "apy_gs_coefficients": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp]),

@@ -116,7 +116,9 @@ __global__ __launch_bounds__(256) void combine_kernel(VecList xs, CoefList c, in
 // y[q seglen + l] = sum_m c[q][m] x_m[q seglen + l], coefficients [nseg][K] in device memory
 template <int K>
 __global__ __launch_bounds__(256) void combine_seg_kernel(VecList xs, const double2* __restrict__ c, int64_t seglen,
-                                                          int64_t total, double2* __restrict__ y) {
+                                                          int64_t total, double2* __restrict__ y,
+                                                          const int* __restrict__ skip) {
+  if (skip && skip[1] != 0) return;  // an iteration control block whose loop has stopped
   for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < total; l += (int64_t)gridDim.x * blockDim.x) {
     const double2* cq = c + (l / seglen) * K;
     double a = 0.0, b = 0.0;
@@ -192,8 +194,136 @@ void launch_dot(VecList v, const double2* y, int64_t len, int64_t nseg, int conj
 }
 
 template <int K>
-void launch_comb_seg(VecList v, const double2* c, int64_t seglen, int64_t nseg, double2* y, hipStream_t s) {
-  combine_seg_kernel<K><<<apy_grid1d(seglen * nseg, 8192), 256, 0, s>>>(v, c, seglen, seglen * nseg, y);
+void launch_comb_seg(VecList v, const double2* c, int64_t seglen, int64_t nseg, double2* y, const int* skip,
+                     hipStream_t s) {
+  combine_seg_kernel<K><<<apy_grid1d(seglen * nseg, 8192), 256, 0, s>>>(v, c, seglen, seglen * nseg, y, skip);
+}
+
+// ------------------------------------------------------------------ Jacobi + DIIS control on the device
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cdiv(double2 a, double2 b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ inline double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
+
+// DIIS of utils.py:104-140 over a ring of nslot stored (amplitude, residual) pairs.  row[q] = <r_q | r_slot>
+// (conjugated dots of every slot's residual with the newest one, at slot `slot`); `valid` marks the slots of
+// the window.  B [nslot][nslot] keeps the dots of earlier iterations (B[q][slot] = row[q], B[slot][q] =
+// conj(row[q])); the bordered system [[B_w, -1], [-1, 0]] c = [0, .., 0, -1] over the window's slots (in
+// slot order) is solved by Gaussian elimination with partial pivoting (largest |re| + |im|, as LAPACK's
+// getrf), threads over rows; coef[q] = c of slot q, zero outside the window.  Nothing is done once the
+// control block's status ctl[1] is set; a zero pivot sets it to 3 (singular DIIS system).
+__global__ __launch_bounds__(64) void diis_slot_kernel(int nslot, int slot, unsigned valid,
+                                                       const double2* __restrict__ row, double2* __restrict__ B,
+                                                       double2* __restrict__ coef, int* __restrict__ ctl) {
+  constexpr int NM = APY_MAXVEC + 1;
+  __shared__ double2 M[NM][NM + 1];
+  __shared__ int idx[APY_MAXVEC];
+  __shared__ int nv_s, bad_s;
+  if (ctl[1] != 0) return;
+  const int tid = threadIdx.x;
+  if (tid < nslot && ((valid >> tid) & 1u)) {
+    B[tid * nslot + slot] = row[tid];
+    B[slot * nslot + tid] = make_double2(row[tid].x, -row[tid].y);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int nv = 0;
+    for (int q = 0; q < nslot; ++q)
+      if ((valid >> q) & 1u) idx[nv++] = q;
+    nv_s = nv;
+    bad_s = 0;
+  }
+  __syncthreads();
+  const int nv = nv_s, n = nv + 1;
+  if (tid < n) {
+    for (int b = 0; b < nv; ++b)
+      M[tid][b] = tid < nv ? B[idx[tid] * nslot + idx[b]] : make_double2(-1.0, 0.0);
+    M[tid][nv] = tid < nv ? make_double2(-1.0, 0.0) : make_double2(0.0, 0.0);
+    M[tid][n] = tid < nv ? make_double2(0.0, 0.0) : make_double2(-1.0, 0.0);
+  }
+  __syncthreads();
+  for (int c = 0; c < n; ++c) {
+    if (tid == 0) {
+      int p = c;
+      for (int r = c + 1; r < n; ++r)
+        if (cabs1(M[r][c]) > cabs1(M[p][c])) p = r;
+      if (cabs1(M[p][c]) == 0.0) bad_s = 1;
+      if (p != c)
+        for (int j = c; j <= n; ++j) {
+          const double2 t = M[p][j];
+          M[p][j] = M[c][j];
+          M[c][j] = t;
+        }
+    }
+    __syncthreads();
+    if (bad_s) break;
+    if (tid > c && tid < n) {
+      const double2 f = cdiv(M[tid][c], M[c][c]);
+      for (int j = c; j <= n; ++j) {
+        const double2 m = cmul(f, M[c][j]);
+        M[tid][j] = make_double2(M[tid][j].x - m.x, M[tid][j].y - m.y);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  if (bad_s) {
+    ctl[1] = 3;
+    return;
+  }
+  double2 x[NM];
+  for (int r = n - 1; r >= 0; --r) {
+    double2 acc = M[r][n];
+    for (int j = r + 1; j < n; ++j) {
+      const double2 m = cmul(M[r][j], x[j]);
+      acc = make_double2(acc.x - m.x, acc.y - m.y);
+    }
+    x[r] = cdiv(acc, M[r][r]);
+  }
+  for (int q = 0; q < nslot; ++q) coef[q] = make_double2(0.0, 0.0);
+  for (int a = 0; a < nv; ++a) coef[idx[a]] = x[a];
+}
+
+// Principal square root (the branch of numpy.sqrt on complex numbers), without cancellation.
+__device__ inline double2 csqrt_principal(double2 z) {
+  if (z.x == 0.0 && z.y == 0.0) return make_double2(0.0, z.y);
+  const double r = hypot(z.x, z.y);
+  if (z.x >= 0.0) {
+    const double re = sqrt(0.5 * (r + z.x));
+    return make_double2(re, z.y / (2.0 * re));
+  }
+  const double im = copysign(sqrt(0.5 * (r - z.x)), z.y);
+  return make_double2(z.y / (2.0 * im), im);
+}
+
+// a < d for complex a and real d in NumPy's ordering of complex numbers (real part first).
+__device__ inline bool np_less(double2 a, double d) { return a.x < d || (a.x == d && a.y < 0.0); }
+
+// The convergence test of ci_wfn.py:361-378 after the next iteration has committed its amplitudes:
+// it = ctl[0] + 1, E = energy[0], dE = E_prev - E, rms_k = sqrt(sums[k]) (the principal root of the
+// un-conjugated sums of squares of the change); log[it] = (E, dE, rms1, rms2); status ctl[1] = 1 when
+// it > 1, |dE| < e_conv and rms2 (and rms1 if check_rms1) < d_conv, else 2 when it reaches max_it.
+__global__ void iter_check_kernel(const double2* __restrict__ energy, const double2* __restrict__ sums, int check_rms1,
+                                  double e_conv, double d_conv, int max_it, double2* __restrict__ E_prev,
+                                  double2* __restrict__ log, int cap, int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || ctl[1] != 0) return;
+  const int it = ctl[0] + 1;
+  ctl[0] = it;
+  const double2 E = energy[0], Ep = *E_prev;
+  const double2 dE = make_double2(Ep.x - E.x, Ep.y - E.y);
+  *E_prev = E;
+  const double2 r1 = csqrt_principal(sums[0]), r2 = csqrt_principal(sums[1]);
+  if (it < cap) {
+    log[4 * it] = E;
+    log[4 * it + 1] = dE;
+    log[4 * it + 2] = r1;
+    log[4 * it + 3] = r2;
+  }
+  const bool conv = it > 1 && hypot(dE.x, dE.y) < e_conv && np_less(r2, d_conv) && (!check_rms1 || np_less(r1, d_conv));
+  if (conv) ctl[1] = 1;
+  else if (it >= max_it) ctl[1] = 2;
 }
 
 template <int K>
@@ -345,10 +475,48 @@ int apy_zcombine_seg(int k, const void* const* xs, const void* coef, int64_t nse
   }
   hipStream_t s = (hipStream_t)stream;
   switch (k) {
-#define C(K) case K: launch_comb_seg<K>(v, (const double2*)coef, seglen, nseg, (double2*)y, s); break;
+#define C(K) case K: launch_comb_seg<K>(v, (const double2*)coef, seglen, nseg, (double2*)y, nullptr, s); break;
     C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
 #undef C
   }
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_zcombine_guarded(int k, const void* const* xs, const void* coef, int64_t len, void* y, const int* ctl,
+                         void* stream) {
+  if (k < 1 || k > APY_MAXVEC || !xs || !coef || !y || len < 1 || !ctl) return APY_E_ARG;
+  VecList v{};
+  for (int m = 0; m < k; ++m) {
+    if (!xs[m]) return APY_E_ARG;
+    v.p[m] = (const double2*)xs[m];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  switch (k) {
+#define C(K) case K: launch_comb_seg<K>(v, (const double2*)coef, len, 1, (double2*)y, ctl, s); break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16)
+#undef C
+  }
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_diis_slot(int nslot, int slot, uint32_t valid, const void* row, void* B, void* coef, int* ctl, void* stream) {
+  if (nslot < 1 || nslot > APY_MAXVEC || slot < 0 || slot >= nslot || !((valid >> slot) & 1u) ||
+      (nslot < 32 && (valid >> nslot) != 0u) || !row || !B || !coef || !ctl)
+    return APY_E_ARG;
+  diis_slot_kernel<<<1, 64, 0, (hipStream_t)stream>>>(nslot, slot, valid, (const double2*)row, (double2*)B,
+                                                      (double2*)coef, ctl);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_iter_check(const void* energy, const void* sums, int check_rms1, double e_conv, double d_conv, int max_it,
+                   void* E_prev, void* log, int cap, int* ctl, void* stream) {
+  if (!energy || !sums || !E_prev || !log || cap < 1 || !ctl) return APY_E_ARG;
+  iter_check_kernel<<<1, 64, 0, (hipStream_t)stream>>>((const double2*)energy, (const double2*)sums, check_rms1,
+                                                       e_conv, d_conv, max_it, (double2*)E_prev, (double2*)log, cap,
+                                                       ctl);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }

@@ -636,10 +636,18 @@ int apy_zgemm_tune_impl(const apy_gemm_desc* d, void* workspace, size_t ws_bytes
       Plan p = plan_for(d, cfg, q);
       if (p.cost < m.cost) m = p;
     }
-    const int cands[3] = {m.split, std::max(1, m.split / 2), std::min(smax, m.split * 2)};
-    for (int ci = 0; ci < 3; ++ci) {
-      if (ci > 0 && cands[ci] == cands[0]) continue;
-      if (ci == 2 && cands[2] == cands[1]) continue;
+    // the model's split, its neighbours, half and double, and the small splits that fill a skinny grid
+    const int raw[8] = {m.split, m.split - 1, m.split + 1, m.split / 2, m.split * 2, 1, 2, 4};
+    int cands[8], nc = 0;
+    for (int q : raw) {
+      if (q < 1 || q > smax) continue;
+      if (q > 1 && (d->k + q - 1) / q < 4 * CFGS[cfg].bk) continue;
+      const int eff = plan_for(d, cfg, q).split;   // splits that round to the same K slices time once
+      bool dup = false;
+      for (int c = 0; c < nc; ++c) dup = dup || cands[c] == eff;
+      if (!dup) cands[nc++] = eff;
+    }
+    for (int ci = 0; ci < nc; ++ci) {
       Plan p = plan_for(d, cfg, cands[ci]);
       if (ws_need(d, p) > ws_bytes) continue;
       st = run_plan(d, p, workspace, ws_bytes, s);  // warm-up

@@ -1,9 +1,10 @@
 """Spin-orbital CISD / CID on the GPU: device integrals and the reference's
 Jacobi + DIIS iteration (ci_wfn.py:171-416, utils.py:104-140).
 
-All tensor work runs in libapyib_amd.so; Python only sequences the calls,
-solves the (at most 9x9) bordered DIIS system and applies the reference's
-convergence test to a handful of scalars copied back per iteration.
+All tensor work runs in libapyib_amd.so.  With DIIS the bordered system and
+the convergence test are solved on the device as well (solve_device), so
+Python only queues iterations and reads a control block; solve_host keeps
+them on the host with a handful of scalars copied back per iteration.
 """
 import ctypes
 
@@ -286,13 +287,140 @@ class SOCI:
             self.t = t_cur
             yield it, E, E_old - E, rms1, rms2
 
-    def solve(self, params, print_level=0, max_diis=7, e_offset=0.0):
+    def solve(self, params, print_level=0, max_diis=7, e_offset=0.0, control="device", graph=None):
         """The reference's iteration (ci_wfn.py:263-378 for CISD, :171-257
         for CID) until its convergence test passes or max_iterations.
-        Returns (E, t1, t2, iterations performed), E a Python complex."""
+        control="device" (with DIIS) keeps the DIIS solve and the
+        convergence test on the device (solve_device); "host" reads the
+        scalars back every iteration (solve_host).  Returns (E, t1, t2,
+        iterations performed), E a Python complex."""
+        if control == "device" and params['DIIS'] and max_diis + 1 <= _lib.APY_MAXVEC:
+            return self.solve_device(params, print_level, max_diis, e_offset, graph=graph)
+        return self.solve_host(params, print_level, max_diis, e_offset)
+
+    def _print_header(self):
+        print("\n Iter      E_elec(real)       E_elec(imaginary)        E(tot)           Delta_E(real)       "
+              "Delta_E(imaginary)      RMS_T1(real)      RMS_T1(imaginary)      RMS_T2(real)      RMS_T2(imaginary)")
+
+    @staticmethod
+    def _print_row(it, E, dE, rms1, rms2, e_offset):
+        print(" %02d %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f" % (
+            it, E.real, E.imag, (E + e_offset).real, dE.real, dE.imag, rms1.real, rms1.imag, rms2.real, rms2.imag))
+
+    # heavier iterations than this (seconds, measured on the first one) read the control block after each
+    # iteration instead of queueing a chunk: queued iterations after convergence would cost real time
+    CHUNK_MAX_ITERATION_S = 5.0e-4
+
+    def solve_device(self, params, print_level=0, max_diis=7, e_offset=0.0, graph=None):
+        """solve() with the DIIS extrapolation (apy_diis_slot), the guarded
+        combination (apy_zcombine_guarded) and the convergence test
+        (apy_iter_check) on the device: the (amplitude, residual) pairs live
+        in a ring of max_diis + 1 slots, so each iteration's launches depend
+        only on its position modulo lcm(2, slots).  Light problems queue that
+        many iterations per host read (iterations queued past convergence do
+        nothing that is kept); with graph=True they replay the steady-state
+        chunk from a HIP graph captured in this call, which pays off only for
+        long solves (at 15-35 iterations the capture costs more than the
+        launches it saves, so it is off by default).  Heavy problems read the
+        control block after every iteration.  The iterates are the
+        reference's: same window, same solve, same test."""
+        import math
+        import time
+        dev, lib = self.device, self.lib
+        nvec = max_diis + 1
+        max_it = int(params['max_iterations'])
         if print_level > 0:
-            print("\n Iter      E_elec(real)       E_elec(imaginary)        E(tot)           Delta_E(real)       "
-                  "Delta_E(imaginary)      RMS_T1(real)      RMS_T1(imaginary)      RMS_T2(real)      RMS_T2(imaginary)")
+            self._print_header()
+        bufs = [self.initial_guess(self.new_vector()), self.new_vector()]
+        self.energy(bufs[0])
+        E0 = complex(self.energy_dev[0].cpu())
+        if max_it < 1:
+            self.t = bufs[0]
+            t1, t2 = self.split(self.t)
+            return E0, t1, t2, 0
+        Eh = torch.zeros((nvec, self.L), dtype=CDT, device=dev)
+        Th = torch.zeros((nvec, self.L), dtype=CDT, device=dev)
+        B = torch.zeros((nvec, nvec), dtype=CDT, device=dev)
+        coef = torch.zeros(nvec, dtype=CDT, device=dev)
+        row = torch.zeros(nvec, dtype=CDT, device=dev)
+        sums = torch.zeros(2, dtype=CDT, device=dev)
+        ctl = torch.zeros(2, dtype=torch.int32, device=dev)
+        E_prev = self.energy_dev[:1].clone()
+        cap = max_it + 1
+        log = torch.zeros((cap, 4), dtype=CDT, device=dev)
+        e_conv, d_conv = float(params['e_convergence']), float(params['d_convergence'])
+        th_ptrs = (ctypes.c_void_p * nvec)(*[Th[q].data_ptr() for q in range(nvec)])
+        eh_list = [Eh[q] for q in range(nvec)]
+        full = (1 << nvec) - 1
+
+        def iteration(it):
+            slot = (it - 1) % nvec
+            t_cur, t_nxt = bufs[(it - 1) % 2], bufs[it % 2]
+            valid = full if it >= nvec else (1 << it) - 1
+            st = stream_of(dev)
+            self.step(t_cur, Th[slot], Eh[slot])
+            self.dots(eh_list, Eh[slot], row)                                    # utils.py:121
+            _lib.check(lib.apy_diis_slot(nvec, slot, valid, _vp(row), _vp(B), _vp(coef), _vp(ctl), st),
+                       "apy_diis_slot")                                        # utils.py:122-135
+            _lib.check(lib.apy_zcombine_guarded(nvec, th_ptrs, _vp(coef), self.L, _vp(t_nxt), _vp(ctl), st),
+                       "apy_zcombine_guarded")                                 # utils.py:138
+            self.energy(t_nxt)
+            self.diffsq(t_cur, t_nxt, sums)
+            _lib.check(lib.apy_iter_check(_vp(self.energy_dev), _vp(sums), int(self.singles), e_conv, d_conv,
+                                          max_it, _vp(E_prev), _vp(log), cap, _vp(ctl), st), "apy_iter_check")
+
+        period = 2 * nvec // math.gcd(2, nvec)
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        iteration(1)
+        state = ctl.cpu().numpy()
+        light = time.perf_counter() - t0 < self.CHUNK_MAX_ITERATION_S
+        use_graph = bool(graph) and light
+        queued, printed, g = 1, 0, None
+        keep = workspace(0, dev)        # the scratch buffer a captured graph points into stays alive
+        while True:
+            done, status = int(state[0]), int(state[1])
+            if print_level > 0 and done > printed:
+                rows = log[printed + 1:done + 1].cpu().numpy()
+                for k, (E, dE, r1, r2) in enumerate(rows):
+                    self._print_row(printed + 1 + k, complex(E), complex(dE), complex(r1), complex(r2), e_offset)
+                printed = done
+            if status != 0:
+                break
+            if not light:
+                iteration(queued + 1)
+                queued += 1
+            elif use_graph and queued % period == 0 and queued >= period:
+                if g is None:
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g):
+                        for q in range(period):
+                            iteration(queued + 1 + q)
+                g.replay()
+                queued += period
+            else:
+                n = period - queued % period
+                for q in range(n):
+                    iteration(queued + 1 + q)
+                queued += n
+            state = ctl.cpu().numpy()
+        del keep
+        it = int(state[0])
+        if int(state[1]) == 3:
+            raise np.linalg.LinAlgError("singular DIIS system")
+        E, dE, r1, r2 = (complex(x) for x in log[it].cpu().numpy())
+        if int(state[1]) == 2 and (abs(dE) > params['e_convergence'] or r2 > params['d_convergence']):
+            print("Not converged.")
+        self.t = bufs[it % 2]
+        self.energy(self.t)                 # queued iterations past the stop left another energy here
+        t1, t2 = self.split(self.t)
+        return E, t1, t2, it
+
+    def solve_host(self, params, print_level=0, max_diis=7, e_offset=0.0):
+        """solve() with the scalars of every iteration read back and the
+        DIIS system solved on the host (also the form without DIIS)."""
+        if print_level > 0:
+            self._print_header()
         gen = self.iterations(diis=bool(params['DIIS']), max_diis=max_diis)
         _, E, _, _, _ = next(gen)
         it = 0
@@ -300,9 +428,7 @@ class SOCI:
         if params['max_iterations'] >= 1:
             for it, E, dE, rms1, rms2 in gen:
                 if print_level > 0:
-                    print(" %02d %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f %20.12f" % (
-                        it, E.real, E.imag, (E + e_offset).real, dE.real, dE.imag, rms1.real, rms1.imag,
-                        rms2.real, rms2.imag))
+                    self._print_row(it, E, dE, rms1, rms2, e_offset)
                 checks = [rms1, rms2] if self.singles else [rms2]
                 if _converged(it, dE, checks, params):
                     break

@@ -162,10 +162,10 @@ class ResponseCI(SpatialCI):
         self.step(self.new_vector(), t, scratch, energy=self.zero_energy)
         return t
 
-    def solve(self, params, print_level=0, max_diis=7, e_offset=0.0):
+    def solve(self, params, print_level=0, max_diis=7, e_offset=0.0, **control):
         """Returns (dE, dt1, dt2, iterations): dE the converged projected
         energy derivative, dt1/dt2 views of the device amplitude vector."""
-        E, dt1, dt2, it = super().solve(params, print_level, max_diis, e_offset)
+        E, dt1, dt2, it = super().solve(params, print_level, max_diis, e_offset, **control)
         return E + self.shift, dt1, dt2, it
 
 

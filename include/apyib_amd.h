@@ -600,6 +600,33 @@ int apy_zdiffsq_batch(const void* x, const void* y, int64_t nseg, int64_t seglen
 int apy_zcombine_seg(int k, const void* const* xs, const void* coef, int64_t nseg, int64_t seglen, void* y,
                      void* stream);
 
+/* The Jacobi + DIIS iteration of ci_wfn.py:302-378 controlled from the
+ * device, so that several iterations can be queued (or replayed from a
+ * captured HIP graph) between host reads.  The control block ctl (int32[2],
+ * device) holds the iterations performed and the status: 0 running,
+ * 1 converged, 2 max_iterations reached, 3 singular DIIS system; once the
+ * status is set, apy_diis_slot, apy_zcombine_guarded and apy_iter_check do
+ * nothing, so the amplitudes of the last iteration stay where it put them.
+ *
+ * apy_diis_slot: DIIS of utils.py:104-140 over a ring of nslot stored
+ * (amplitude, residual) pairs.  row[q] = <r_q | r_slot> (conjugated dots with
+ * the newest residual, apy_zdot_many over every slot); valid = bit mask of
+ * the window's slots (slot included).  B [nslot][nslot] (device) keeps the
+ * dots of earlier iterations; coef[q] (device, zero outside the window)
+ * solves the bordered system [[B_w, -1], [-1, 0]] c = [0, .., 0, -1].
+ * apy_zcombine_guarded: y = sum_q coef[q] x_q (coef in device memory).
+ * apy_iter_check: iteration it = ctl[0] + 1 done: E = energy[0],
+ * dE = E_prev - E, rms_k = sqrt(sums[k]) (principal root; sums are the
+ * un-conjugated sums of squares of the change of t1 and t2); log[4 it + j]
+ * = (E, dE, rms1, rms2) while it < cap; converged (ci_wfn.py:371-374) when
+ * it > 1, |dE| < e_conv and rms2 (and rms1 if check_rms1) < d_conv in
+ * NumPy's complex order; status 2 when it reaches max_it.  E_prev <- E. */
+int apy_diis_slot(int nslot, int slot, uint32_t valid, const void* row, void* B, void* coef, int* ctl, void* stream);
+int apy_zcombine_guarded(int k, const void* const* xs, const void* coef, int64_t len, void* y, const int* ctl,
+                         void* stream);
+int apy_iter_check(const void* energy, const void* sums, int check_rms1, double e_conv, double d_conv, int max_it,
+                   void* E_prev, void* log, int cap, int* ctl, void* stream);
+
 /* ------------------------------------------------------------------------
  * Finite-difference atomic axial tensors (replaces AAT.compute_all_dets,
  * aats.py:558-642, and the determinant-amplitude sums of

@@ -140,3 +140,27 @@ def test_one_electron_args_layout_and_errors():
         a = _lib.OeiArgs(**{**good, **bad})
         assert lib.apy_oei_gto(ctypes.byref(a), None) == _lib.APY_E_ARG, bad
     assert lib.apy_oei_gto(None, None) == _lib.APY_E_ARG
+
+
+def test_device_iteration_control_argument_errors():
+    """apy_diis_slot, apy_zcombine_guarded and apy_iter_check reject bad
+    sizes, a slot outside its window and null pointers before any device
+    work."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    ok = 8
+    assert lib.apy_diis_slot(0, 0, 1, ok, ok, ok, ok, None) == _lib.APY_E_ARG
+    assert lib.apy_diis_slot(17, 0, 1, ok, ok, ok, ok, None) == _lib.APY_E_ARG
+    assert lib.apy_diis_slot(8, 8, 0xFF, ok, ok, ok, ok, None) == _lib.APY_E_ARG
+    assert lib.apy_diis_slot(8, 3, 0x07, ok, ok, ok, ok, None) == _lib.APY_E_ARG      # slot not in the window
+    assert lib.apy_diis_slot(8, 0, 0x1FF, ok, ok, ok, ok, None) == _lib.APY_E_ARG     # bit beyond nslot
+    assert lib.apy_diis_slot(8, 0, 0x01, None, ok, ok, ok, None) == _lib.APY_E_ARG
+    assert lib.apy_diis_slot(8, 0, 0x01, ok, ok, ok, None, None) == _lib.APY_E_ARG
+    ptrs = (ctypes.c_void_p * 2)(8, 8)
+    assert lib.apy_zcombine_guarded(0, ptrs, ok, 4, ok, ok, None) == _lib.APY_E_ARG
+    assert lib.apy_zcombine_guarded(2, ptrs, ok, 4, ok, None, None) == _lib.APY_E_ARG
+    assert lib.apy_zcombine_guarded(2, ptrs, ok, 0, ok, ok, None) == _lib.APY_E_ARG
+    assert lib.apy_iter_check(ok, ok, 1, 1e-12, 1e-12, 10, ok, ok, 0, ok, None) == _lib.APY_E_ARG
+    assert lib.apy_iter_check(ok, None, 1, 1e-12, 1e-12, 10, ok, ok, 4, ok, None) == _lib.APY_E_ARG
+    assert lib.apy_iter_check(ok, ok, 1, 1e-12, 1e-12, 10, ok, ok, 4, None, None) == _lib.APY_E_ARG
