@@ -252,6 +252,9 @@ SIGNATURES = {
     "apy_cisd_sp_batch_workspace": (c_size, [c_i64, c_i64, c_int, c_i64]),
     "apy_cisd_sp_step_batch": (c_int, [ctypes.POINTER(CisdSpArgs), c_i64, c_vp, c_vp, c_size, c_vp]),
     "apy_cisd_sp_energy_batch": (c_int, [c_i64, c_i64, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_cisd_sp_multi_workspace": (c_size, [c_i64, c_i64, c_int, c_i64]),
+    "apy_cisd_sp_step_multi": (c_int, [ctypes.POINTER(CisdSpArgs), c_i64, c_vp, c_size, c_vp]),
+    "apy_cisd_sp_energy_multi": (c_int, [c_i64, c_i64, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_cisd_so_tpd_workspace": (c_size, [c_i64, c_i64]),
     "apy_cisd_so_tpd": (c_int, [c_i64, c_i64, c_vp, ctypes.POINTER(SoInts), c_vp, c_size, c_vp]),
     "apy_so_ints_grad_to_full": (c_int, [c_i64, ctypes.POINTER(SoInts), c_vp, c_vp]),

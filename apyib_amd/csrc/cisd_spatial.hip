@@ -256,6 +256,7 @@ struct SpEpi {
   const double2* re;
   double2 rs;
   const double2* es;
+  int64_t s_eps, s_vvoo;  // per-instance strides of eps and vvoo (0: shared by the instances)
 };
 
 // scalar of the response term of instance q: re[q] + rs + es[q] (null arrays read as zero)
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
     const int64_t q = u / per, t = u % per, qd = q * per, qv = q * e.L;
     const int64_t b = t % v, a = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
     const double2 E = e.energy[q];
-    double2 r = e.vvoo[t];
+    double2 r = e.vvoo[q * e.s_vvoo + t];
     const double2 p = e.R[u];
     r.x += p.x; r.y += p.y;
     {  // ladder: P+ + s_ij s_ab P-
@@ -309,7 +310,8 @@ __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
       r.x += bb.x - (s.x * y.x - s.y * y.y);
       r.y += bb.y - (s.x * y.y + s.y * y.x);
     }
-    const double D = e.eps[i] + e.eps[j] - e.eps[o + a] - e.eps[o + b];
+    const double* ep = e.eps + q * e.s_eps;
+    const double D = ep[i] + ep[j] - ep[o + a] - ep[o + b];
     e.r2[qv + t] = r;
     e.t2_new[qv + t] = make_double2(x.x + r.x / D, x.y + r.y / D);
   }
@@ -323,20 +325,21 @@ __global__ __launch_bounds__(256) void sp_epi_singles(int64_t o, int64_t v, int6
                                                       const double2* __restrict__ energy, double2* __restrict__ t1_new,
                                                       double2* __restrict__ r1, const double2* __restrict__ rb,
                                                       const double2* __restrict__ rt, const double2* __restrict__ re,
-                                                      double2 rs, const double2* __restrict__ es) {
+                                                      double2 rs, const double2* __restrict__ es, int64_t s_eps,
+                                                      int64_t s_fock) {
   const int64_t n = o + v, ov = o * v, total = nb * ov;
   for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total; u += (int64_t)gridDim.x * blockDim.x) {
     const int64_t q = u / ov, t = u % ov, qv = q * L;
     const int64_t i = t / v, a = t % v;
     const double2 E = energy[q];
-    const double2 f = F[(o + a) * n + i], s = acc[u], x = t1[qv + t];
+    const double2 f = F[q * s_fock + (o + a) * n + i], s = acc[u], x = t1[qv + t];
     double2 r = make_double2(f.x + s.x - (E.x * x.x - E.y * x.y), f.y + s.y - (E.x * x.y + E.y * x.x));
     if (rb) {
       const double2 c = resp_scalar(re, rs, es, q), bb = rb[qv + t], y = rt[t];
       r.x += bb.x - (c.x * y.x - c.y * y.y);
       r.y += bb.y - (c.x * y.y + c.y * y.x);
     }
-    const double D = eps[i] - eps[o + a];
+    const double D = eps[q * s_eps + i] - eps[q * s_eps + o + a];
     r1[qv + t] = r;
     t1_new[qv + t] = make_double2(x.x + r.x / D, x.y + r.y / D);
   }
@@ -356,11 +359,14 @@ constexpr int EN_BLOCKS = 256;
 
 // E = sum g[0:ov] t1 + sum L t2 (no conjugation; g[0:ov] = 2 F_ov); instance blockIdx.y of vectors
 // `stride` apart, partial sums [instance][EN_BLOCKS]
-__global__ __launch_bounds__(256) void sp_energy_partial(int64_t ov, int singles, const double2* __restrict__ gvec,
-                                                         const double2* __restrict__ L, const double2* __restrict__ tv,
-                                                         int64_t stride, double2* __restrict__ partial) {
+__global__ __launch_bounds__(256) void sp_energy_partial(int64_t ov, int singles, const double2* __restrict__ gv,
+                                                         const double2* __restrict__ Lv, const double2* __restrict__ tv,
+                                                         int64_t stride, int64_t s_g, int64_t s_l,
+                                                         double2* __restrict__ partial) {
   const int64_t n2 = ov * ov;
   const double2* t = tv + blockIdx.y * stride;
+  const double2* gvec = gv ? gv + blockIdx.y * s_g : nullptr;
+  const double2* L = Lv + blockIdx.y * s_l;
   double sr = 0.0, si = 0.0;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ov + n2; q += (int64_t)gridDim.x * blockDim.x) {
     double2 w, x = t[q];
@@ -422,11 +428,32 @@ struct SpWS {
 
 constexpr int NSPG = 14;
 
+// Element strides between the instances of a step over independent systems (0 everywhere: the instances
+// share fock, eps, hoo, g and the integral blocks).
+struct SpInst {
+  int64_t fock, eps, hoo, g, blk[S_COUNT];
+};
+
+SpInst sp_inst_strides(int64_t o, int64_t v) {
+  SpInst S{};
+  const int64_t n = o + v;
+  S.fock = n * n;
+  S.eps = n;
+  S.hoo = o * o * o * o;
+  S.g = 2 * o * v;
+  for (int k = 0; k < S_COUNT; ++k) S.blk[k] = sp_size(k, o, v);
+  return S;
+}
+
 // The GEMMs of one step over nb vectors L apart, in order (pointers null when only sizing).  The
 // ladder and ring GEMMs stack the instances along M; the ones reading the vectors directly run as
 // batched GEMMs with the vector stride.
 int sp_gemms(int64_t o, int64_t v, int singles, int64_t nb, int64_t L, const apy_cisd_sp_args* A, char* base,
-             const SpWS* w, apy_gemm_desc g[NSPG]) {
+             const SpWS* w, apy_gemm_desc g[NSPG], const SpInst* S) {
+  // S null: shared integrals, the instances stacked along M of the ladder and ring products; S set: every
+  // product batched over the instances with the operands' own strides.
+  const SpInst Z{};
+  const SpInst& st = S ? *S : Z;
   const int64_t n = o + v, ov = o * v, oo = o * o, vv = v * v, d2 = ov * ov;
   auto P = [&](size_t off) -> double2* { return base ? (double2*)(base + off) : nullptr; };
   const double2* t1 = A ? (const double2*)A->t : nullptr;
@@ -439,28 +466,48 @@ int sp_gemms(int64_t o, int64_t v, int singles, int64_t nb, int64_t L, const apy
   double2* acc = P(w->acc);
   int q = 0;
   const int64_t so = nsym(o), ao = nanti(o), sv = nsym(v), av = nanti(v);
-  g[q++] = gd(nb * so, sv, sv, APY_MK, P(w->Ts), sv, APY_NK, I ? I->vvvs : nullptr, sv, P(w->Ps), sv, 1.0, 0.0);  // P+
-  if (ao > 0 && av > 0)
-    g[q++] = gd(nb * ao, av, av, APY_MK, P(w->Ta), av, APY_NK, I ? I->vvva : nullptr, av, P(w->Pa), av, 1.0, 0.0);  // P-
-  g[q++] = gd(oo, vv, oo, APY_MK, A ? A->hoo : nullptr, oo, APY_KN, t2, vv, R, vv, 1.0, 0.0, nb, 0, L, d2);  // HHL+F_oo
-  g[q++] = gd(oo * v, v, v, APY_MK, t2, v, APY_NK, Fvv, n, P(w->Y), v, 1.0, 0.0, nb, L, 0, d2);             // F_vv
-  g[q++] = gd(nb * ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, I ? I->w21 : nullptr, ov, P(w->X1), ov, 1.0, 0.0);
-  g[q++] = gd(nb * ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2), ov,
-              1.0, 0.0);
+  if (S) {  // ladder (P+, P-) and ring (X1, X2) per instance
+    g[q++] = gd(so, sv, sv, APY_MK, P(w->Ts), sv, APY_NK, I ? I->vvvs : nullptr, sv, P(w->Ps), sv, 1.0, 0.0, nb,
+                so * sv, st.blk[S_VVVS], so * sv);
+    if (ao > 0 && av > 0)
+      g[q++] = gd(ao, av, av, APY_MK, P(w->Ta), av, APY_NK, I ? I->vvva : nullptr, av, P(w->Pa), av, 1.0, 0.0, nb,
+                  ao * av, st.blk[S_VVVA], ao * av);
+  } else {
+    g[q++] = gd(nb * so, sv, sv, APY_MK, P(w->Ts), sv, APY_NK, I ? I->vvvs : nullptr, sv, P(w->Ps), sv, 1.0, 0.0);  // P+
+    if (ao > 0 && av > 0)
+      g[q++] = gd(nb * ao, av, av, APY_MK, P(w->Ta), av, APY_NK, I ? I->vvva : nullptr, av, P(w->Pa), av, 1.0, 0.0);  // P-
+  }
+  g[q++] = gd(oo, vv, oo, APY_MK, A ? A->hoo : nullptr, oo, APY_KN, t2, vv, R, vv, 1.0, 0.0, nb, st.hoo, L, d2);  // HHL
+  g[q++] = gd(oo * v, v, v, APY_MK, t2, v, APY_NK, Fvv, n, P(w->Y), v, 1.0, 0.0, nb, L, st.fock, d2);          // F_vv
+  if (S) {
+    g[q++] = gd(ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, I ? I->w21 : nullptr, ov, P(w->X1), ov, 1.0, 0.0, nb,
+                2 * d2, st.blk[S_W21], d2);
+    g[q++] = gd(ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2), ov,
+                1.0, 0.0, nb, 2 * d2, st.blk[S_W3], d2);
+  } else {
+    g[q++] = gd(nb * ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, I ? I->w21 : nullptr, ov, P(w->X1), ov, 1.0, 0.0);
+    g[q++] = gd(nb * ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2),
+                ov, 1.0, 0.0);
+  }
   if (singles) {
-    g[q++] = gd(o, vv * o, v, APY_MK, t1, v, APY_NK, I ? I->vvvo : nullptr, v, P(w->Q1), vv * o, 1.0, 0.0, nb, L, 0, d2);
-    g[q++] = gd(v * oo, v, o, APY_MK, I ? I->vooo : nullptr, o, APY_KN, t1, v, P(w->Q2), v, 1.0, 0.0, nb, 0, L, d2);
-    g[q++] = gd(ov, 1, ov, APY_MK, I ? I->s1 : nullptr, ov, APY_KN, t1, 1, acc, 1, 1.0, 0.0, nb, 0, L, ov);
-    g[q++] = gd(ov, 1, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, A ? A->g : nullptr, 1, acc, 1, 1.0, 1.0, nb, 2 * d2, 0, ov);
-    g[q++] = gd(o, v, ov * v, APY_MK, t2, ov * v, APY_NK, I ? I->w4 : nullptr, ov * v, acc, v, 1.0, 1.0, nb, L, 0, ov);
-    g[q++] = gd(o, v, oo * v, APY_MK, I ? I->w5 : nullptr, oo * v, APY_KN, t2, v, acc, v, -1.0, 1.0, nb, 0, L, ov);
-    g[q++] = gd(o, v, o, APY_KM, F, n, APY_KN, t1, v, acc, v, -1.0, 1.0, nb, 0, L, ov);   // -F_oo^T t1
-    g[q++] = gd(o, v, v, APY_MK, t1, v, APY_NK, Fvv, n, acc, v, 1.0, 1.0, nb, L, 0, ov);  // t1 F_vv^T
+    g[q++] = gd(o, vv * o, v, APY_MK, t1, v, APY_NK, I ? I->vvvo : nullptr, v, P(w->Q1), vv * o, 1.0, 0.0, nb, L,
+                st.blk[S_VVVO], d2);
+    g[q++] = gd(v * oo, v, o, APY_MK, I ? I->vooo : nullptr, o, APY_KN, t1, v, P(w->Q2), v, 1.0, 0.0, nb,
+                st.blk[S_VOOO], L, d2);
+    g[q++] = gd(ov, 1, ov, APY_MK, I ? I->s1 : nullptr, ov, APY_KN, t1, 1, acc, 1, 1.0, 0.0, nb, st.blk[S_S1], L, ov);
+    g[q++] = gd(ov, 1, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, A ? A->g : nullptr, 1, acc, 1, 1.0, 1.0, nb, 2 * d2, st.g,
+                ov);
+    g[q++] = gd(o, v, ov * v, APY_MK, t2, ov * v, APY_NK, I ? I->w4 : nullptr, ov * v, acc, v, 1.0, 1.0, nb, L,
+                st.blk[S_W4], ov);
+    g[q++] = gd(o, v, oo * v, APY_MK, I ? I->w5 : nullptr, oo * v, APY_KN, t2, v, acc, v, -1.0, 1.0, nb, st.blk[S_W5],
+                L, ov);
+    g[q++] = gd(o, v, o, APY_KM, F, n, APY_KN, t1, v, acc, v, -1.0, 1.0, nb, st.fock, L, ov);   // -F_oo^T t1
+    g[q++] = gd(o, v, v, APY_MK, t1, v, APY_NK, Fvv, n, acc, v, 1.0, 1.0, nb, L, st.fock, ov);  // t1 F_vv^T
   }
   return q;
 }
 
-SpWS sp_carve(int64_t o, int64_t v, int singles, int64_t nb) {
+SpWS sp_carve(int64_t o, int64_t v, int singles, int64_t nb, bool multi = false) {
   const size_t z = 16, ov = (size_t)(o * v), d2 = ov * ov * z, B = (size_t)nb;
   auto rnd = [](size_t b) { return (b + 255) / 256 * 256; };
   SpWS w{};
@@ -482,7 +529,8 @@ SpWS sp_carve(int64_t o, int64_t v, int singles, int64_t nb) {
   }
   w.gemm = off;
   apy_gemm_desc g[NSPG];
-  const int q = sp_gemms(o, v, singles, nb, (int64_t)(ov + ov * ov), nullptr, nullptr, &w, g);
+  const SpInst S = sp_inst_strides(o, v);
+  const int q = sp_gemms(o, v, singles, nb, (int64_t)(ov + ov * ov), nullptr, nullptr, &w, g, multi ? &S : nullptr);
   size_t m = 0;
   for (int k = 0; k < q; ++k) m = std::max(m, apy_zgemm_ws_bytes(&g[k]));
   w.total = off + std::max(m, (size_t)EN_BLOCKS * z);
@@ -496,7 +544,7 @@ bool overlap(const void* p, size_t n, const void* q, size_t m) {
 }
 
 int sp_step_impl(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, void* workspace, size_t ws_bytes,
-                 void* stream) {
+                 void* stream, bool multi = false) {
   if (!A || !A->ints || !A->fock || !A->eps || !A->hoo || !A->g || !A->t || !A->t_new || !A->out || !A->energy ||
       nb < 1)
     return APY_E_ARG;
@@ -512,10 +560,12 @@ int sp_step_impl(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, 
                     overlap(A->resp_t, one, A->out, vb) || overlap(A->resp_t, one, A->t_new, vb)))
     return APY_E_ARG;
   const int singles = A->singles ? 1 : 0;
+  if (multi && (A->resp_b || resp_shift)) return APY_E_ARG;  // the response term needs shared integrals
   if (!I->vvvs || (v > 1 && !I->vvva) || !I->w21 || !I->w3 || !I->vvoo) return APY_E_ARG;
   if (singles && (!I->vvvo || !I->vooo || !I->s1 || !I->w4 || !I->w5)) return APY_E_ARG;
-  const SpWS w = sp_carve(o, v, singles, nb);
+  const SpWS w = sp_carve(o, v, singles, nb, multi);
   if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
+  const SpInst S = sp_inst_strides(o, v);
   hipStream_t st = (hipStream_t)stream;
   char* base = (char*)workspace;
   const double2* t = (const double2*)A->t;
@@ -525,7 +575,7 @@ int sp_step_impl(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, 
                                                             (double2*)(base + w.Ta));
   APY_LAUNCH_CHECK();
   apy_gemm_desc g[NSPG];
-  const int q = sp_gemms(o, v, singles, nb, L, A, base, &w, g);
+  const int q = sp_gemms(o, v, singles, nb, L, A, base, &w, g, multi ? &S : nullptr);
   for (int k = 0; k < q; ++k) {
     const int s = apy_zgemm_launch(&g[k], base + w.gemm, ws_bytes - w.gemm, st);
     if (s != APY_OK) return s;
@@ -540,16 +590,33 @@ int sp_step_impl(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, 
           (const double2*)(base + w.X1), (const double2*)(base + w.X2),
           singles ? (const double2*)(base + w.Q1) : nullptr, singles ? (const double2*)(base + w.Q2) : nullptr,
           t + ov, (const double2*)A->energy, (double2*)A->t_new + ov, (double2*)A->out + ov,
-          rb ? rb + ov : nullptr, rb ? rt + ov : nullptr, re, rs, es};
+          rb ? rb + ov : nullptr, rb ? rt + ov : nullptr, re, rs, es, multi ? S.eps : 0,
+          multi ? S.blk[S_VVOO] : 0};
   sp_epi_doubles<<<apy_grid1d(nb * ov * ov), 256, 0, st>>>(e);
   APY_LAUNCH_CHECK();
   if (singles) {
     sp_epi_singles<<<apy_grid1d(nb * ov), 256, 0, st>>>(o, v, nb, L, A->eps, (const double2*)A->fock,
                                                         (const double2*)(base + w.acc), t, (const double2*)A->energy,
-                                                        (double2*)A->t_new, (double2*)A->out, rb, rt, re, rs, es);
+                                                        (double2*)A->t_new, (double2*)A->out, rb, rt, re, rs, es,
+                                                        multi ? S.eps : 0, multi ? S.fock : 0);
   } else {  // CID keeps the singles block of the vectors at zero
     sp_zero_kernel<<<apy_grid1d(nb * ov), 256, 0, st>>>(ov, nb, L, (double2*)A->t_new, (double2*)A->out);
   }
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int sp_energy_impl(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L, const void* t,
+                   void* energy, void* workspace, size_t ws_bytes, void* stream, int64_t s_g, int64_t s_l) {
+  if (o < 1 || v < 1 || nb < 1 || nb > 65535 || !L || !t || !energy || (singles && !g)) return APY_E_ARG;
+  if (!workspace || ws_bytes < (size_t)nb * apy_cisd_sp_energy_workspace()) return APY_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t ov = o * v;
+  sp_energy_partial<<<dim3(EN_BLOCKS, (unsigned)nb), 256, 0, st>>>(ov, singles ? 1 : 0, (const double2*)g,
+                                                                    (const double2*)L, (const double2*)t,
+                                                                    ov + ov * ov, s_g, s_l, (double2*)workspace);
+  APY_LAUNCH_CHECK();
+  sp_energy_final<<<(unsigned)nb, 256, 0, st>>>((const double2*)workspace, EN_BLOCKS, (double2*)energy);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }
@@ -623,17 +690,22 @@ int apy_cisd_sp_energy(int64_t o, int64_t v, int singles, const void* g, const v
 
 int apy_cisd_sp_energy_batch(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L,
                              const void* t, void* energy, void* workspace, size_t ws_bytes, void* stream) {
-  if (o < 1 || v < 1 || nb < 1 || nb > 65535 || !L || !t || !energy || (singles && !g)) return APY_E_ARG;
-  if (!workspace || ws_bytes < (size_t)nb * apy_cisd_sp_energy_workspace()) return APY_E_WORKSPACE;
-  hipStream_t st = (hipStream_t)stream;
-  const int64_t ov = o * v;
-  sp_energy_partial<<<dim3(EN_BLOCKS, (unsigned)nb), 256, 0, st>>>(ov, singles ? 1 : 0, (const double2*)g,
-                                                                    (const double2*)L, (const double2*)t,
-                                                                    ov + ov * ov, (double2*)workspace);
-  APY_LAUNCH_CHECK();
-  sp_energy_final<<<(unsigned)nb, 256, 0, st>>>((const double2*)workspace, EN_BLOCKS, (double2*)energy);
-  APY_LAUNCH_CHECK();
-  return APY_OK;
+  return sp_energy_impl(o, v, singles, nb, g, L, t, energy, workspace, ws_bytes, stream, 0, 0);
+}
+
+int apy_cisd_sp_energy_multi(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L,
+                             const void* t, void* energy, void* workspace, size_t ws_bytes, void* stream) {
+  return sp_energy_impl(o, v, singles, nb, g, L, t, energy, workspace, ws_bytes, stream, 2 * o * v,
+                        sp_size(S_L, o, v));
+}
+
+size_t apy_cisd_sp_multi_workspace(int64_t o, int64_t v, int singles, int64_t nb) {
+  if (o < 1 || v < 1 || nb < 1) return 0;
+  return sp_carve(o, v, singles ? 1 : 0, nb, true).total;
+}
+
+int apy_cisd_sp_step_multi(const apy_cisd_sp_args* A, int64_t nb, void* workspace, size_t ws_bytes, void* stream) {
+  return sp_step_impl(A, nb, nullptr, workspace, ws_bytes, stream, true);
 }
 
 }  // extern "C"

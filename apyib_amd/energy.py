@@ -89,9 +89,51 @@ def phase_corrected_energies(parameter_list, unperturbed_basis, unperturbed_C, p
         scf = solve_SCF_batch(wfns, parameter_list[0], print_level)
     else:
         scf = [w.solve_SCF(p, print_level) for w, p in zip(wfns, parameter_list)]
-    for p, H, wfn, (E_SCF, C) in zip(parameter_list, Hs, wfns, scf):
-        basis = H.basis_set
-        wfn.C = compute_phase(wfn.ndocc, wfn.nbf, unperturbed_basis, unperturbed_C, basis, C)
-        E, t1, t2 = _solve(p, wfn, print_level)
-        out.append(([E_SCF, E, H.E_nuc], [1, t1, t2], wfn.C, basis))
+    for H, wfn, (E_SCF, C) in zip(Hs, wfns, scf):
+        wfn.C = compute_phase(wfn.ndocc, wfn.nbf, unperturbed_basis, unperturbed_C, H.basis_set, C)
+    corr = _solve_spatial_set(parameter_list, wfns) if print_level == 0 else None
+    for q, (p, H, wfn, (E_SCF, C)) in enumerate(zip(parameter_list, Hs, wfns, scf)):
+        E, t1, t2 = corr[q] if corr is not None else _solve(p, wfn, print_level)
+        out.append(([E_SCF, E, H.E_nuc], [1, t1, t2], wfn.C, H.basis_set))
+    return out
+
+
+_SOLVER_KEYS = ('method', 'e_convergence', 'd_convergence', 'DIIS', 'max_iterations', 'freeze_core')
+
+
+def _solve_spatial_set(parameter_list, wfns):
+    """The spin-adapted CISD / CID solves (ci_wfn.py:420-574, :51-167) of a
+    finite-difference set as one spatial_ci.SpatialCISet: every instance's
+    MO integrals and Fock matrix from its own ci_wfn, the Jacobi + DIIS
+    iterations of all of them in one batched step per iteration.  Each
+    result equals its own ci_wfn solve (printing the same amplitude lines).
+    None when the set does not qualify (other methods, one instance,
+    differing solver settings or sizes)."""
+    import numpy as np
+    from .spatial_ci import SpatialCISet
+    p0 = parameter_list[0]
+    if p0['method'] not in ('CISD', 'CID') or len(parameter_list) < 2:
+        return None
+    if any(any(p.get(k) != p0.get(k) for k in _SOLVER_KEYS) for p in parameter_list):
+        return None
+    cws = [ci_wfn(p, w) for p, w in zip(parameter_list, wfns)]
+    shapes = {(len(c.eps_o), len(c.eps_v)) for c in cws}
+    if len(shapes) != 1:
+        return None
+    singles = p0['method'] == 'CISD'
+    cset = SpatialCISet([c._ERI_MO for c in cws], [c._F_MO for c in cws],
+                        [np.concatenate([c.eps_o, c.eps_v]) for c in cws], len(cws[0].eps_o), singles=singles)
+    E, t1, t2, _ = cset.solve(p0)
+    out = []
+    for q in range(len(cws)):
+        t2s = t2[q].cpu().numpy()
+        print("t-Amplitude Data:")
+        if singles:
+            t1s = t1[q].cpu().numpy()
+            print("Maximum t1: ", np.max(t1s))
+            print("Maximum t2: ", np.max(t2s))
+            out.append((np.complex128(E[q]), t1s, t2s))
+        else:
+            print("Maximum t2: ", np.max(t2s))
+            out.append((np.complex128(E[q]), 0, t2s))
     return out

@@ -229,101 +229,187 @@ class ResponseSet:
 
     def solve(self, params, print_level=0, max_diis=7):
         """Returns (dE [nb] complex host array, dt1 [nb,o,v] or None,
-        dt2 [nb,o,o,v,v] device views, iterations per perturbation).
-
-        Every perturbation's DIIS history holds the same iterations (they
-        start together and each keeps the newest max_diis+1), so one
-        segmented call forms all new rows of the DIIS matrices, one the
-        extrapolated amplitudes and one the convergence sums; each
-        perturbation's sums are bit-identical to its own solve's."""
-        from .so_ci import _converged
-        nb, L, dev = len(self.b), self.L, self.device
-        lib = self.lib
-        st = stream_of(dev)
+        dt2 [nb,o,o,v,v] device views, iterations per perturbation)."""
+        nb, dev = len(self.b), self.device
         self.B = torch.stack(self.b)
         self.b = []
         self.shift_dev = torch.tensor(self.shift, dtype=CDT, device=dev)
         self.E_arr = torch.full((nb,), self.E, dtype=CDT, device=dev)
-        diis = bool(params['DIIS'])
-        nvec = max_diis + 1
-        X = torch.zeros((nb, L), dtype=CDT, device=dev)
-        Xn = torch.zeros((nb, L), dtype=CDT, device=dev)
-        s = torch.zeros(nb, dtype=CDT, device=dev)
-        s_new = torch.zeros(nb, dtype=CDT, device=dev)
-        R = torch.zeros((nb, L), dtype=CDT, device=dev)
-        self._step(Xn, X, R, s)                        # guess: one step from dt = 0 (s = 0)
-        Xn.copy_(X)
-        self._energies(X, s)
-        E = s.cpu().numpy()
-        if diis:
-            Th = torch.zeros((nvec, nb, L), dtype=CDT, device=dev)
-            Eh = torch.zeros((nvec, nb, L), dtype=CDT, device=dev)
-        hist = []                                      # (iteration, slot), shared by the perturbations
-        Bs = np.zeros((nb, nvec, nvec), dtype=np.complex128)
-        scal = torch.zeros((nb, _lib.APY_MAXVEC), dtype=CDT, device=dev)
-        dsq = torch.zeros((nb, 2), dtype=CDT, device=dev)
-        active = np.ones(nb, dtype=bool)
-        iters = [0] * nb
-        red = int(lib.apy_reduce_seg_workspace(2 * nb, _lib.APY_MAXVEC))
-        it = 0
-        while active.any() and it < params['max_iterations']:
-            it += 1
-            if diis:
-                slot = (it - 1) % nvec
-                self._step(X, Th[slot], Eh[slot], s)
-                while len(hist) > max_diis:
-                    hist.pop(0)
-                hist.append((it, slot))
-                k = len(hist)
-                arr = (ctypes.c_void_p * k)(*[Eh[h].data_ptr() for _, h in hist])
-                ws = workspace(red, dev)
-                _lib.check(lib.apy_zdot_many_seg(k, arr, _vp(Eh[slot]), nb, L, 1, _vp(scal), _vp(ws), red, st),
-                           "apy_zdot_many_seg")
-                rows = scal.view(-1)[:nb * k].cpu().numpy().reshape(nb, k)      # packed [nb][k]
-                slots = [h for _, h in hist]
-                Bs[:, slots, slot] = rows                                         # E^H E by slot (utils.py:121)
-                Bs[:, slot, slots] = np.conj(rows)
-                act = np.flatnonzero(active)
-                Bm = np.full((len(act), k + 1, k + 1), -1.0, dtype=np.complex128)
-                Bm[:, :k, :k] = Bs[act][:, slots][:, :, slots]
-                Bm[:, k, k] = 0
-                rhs = np.zeros((len(act), k + 1, 1), dtype=np.complex128)
-                rhs[:, k] = -1
-                coef = np.zeros((nb, k), dtype=np.complex128)
-                coef[:, -1] = 1.0                      # finished perturbations: unused
-                coef[act] = np.linalg.solve(Bm, rhs)[:, :k, 0]                    # utils.py:135
-                cdev = torch.from_numpy(coef).to(dev)
-                arr = (ctypes.c_void_p * k)(*[Th[h].data_ptr() for _, h in hist])
-                _lib.check(lib.apy_zcombine_seg(k, arr, _vp(cdev), nb, L, _vp(Xn), st), "apy_zcombine_seg")
-            else:
-                self._step(X, Xn, R, s)
-            self._energies(Xn, s_new)
-            ws = workspace(red, dev)
-            _lib.check(lib.apy_zdiffsq_batch(_vp(X), _vp(Xn), nb, L, self.OV, _vp(dsq), _vp(ws), red, st),
-                       "apy_zdiffsq_batch")
-            E_new = s_new.cpu().numpy()
-            d = dsq.cpu().numpy()
-            act = np.flatnonzero(active)
-            if len(act) == nb:
-                X.copy_(Xn)
-                s.copy_(s_new)
-            else:
-                for q in act:
-                    X[q].copy_(Xn[q])
-                    s[q:q + 1].copy_(s_new[q:q + 1])
-            for q in act:
-                rms1, rms2 = np.sqrt(d[q, 0]), np.sqrt(d[q, 1])
-                dE = E[q] - E_new[q]
-                E[q] = E_new[q]
-                iters[q] = it
-                if print_level > 0:
-                    print(" %02d %3d %20.12f %20.12f %20.12f" % (it, q, (E[q] + self.shift[q]).real, dE.real,
-                                                                 rms2.real))
-                if _converged(it, dE, [rms1, rms2] if self.singles else [rms2], params):
-                    active[q] = False
-        if active.any():
-            print("Not converged.")
+        E, X, iters = _solve_set(self._step, self._energies, nb, self.L, self.OV, self.singles, dev, params,
+                                 print_level, max_diis, shift=self.shift)
         dE = E + np.asarray(self.shift)
         O, V = self.O, self.V
         dt1 = X[:, :self.OV].view(nb, O, V) if self.singles else None
         return dE, dt1, X[:, self.OV:].view(nb, O, O, V, V), iters
+
+
+def _solve_set(step, energies, nb, L, OV, singles, dev, params, print_level=0, max_diis=7, shift=None):
+    """The reference's Jacobi + DIIS iteration (ci_wfn.py:302-378,
+    utils.py:104-140) for nb problems iterated together.  step(x, x_new,
+    out, s): one step of all problems (vectors [nb, L]) with the
+    per-problem energies s ([nb] device); energies(x, out): their energies.
+    The guess is one step from zero amplitudes with zero energies.  Every
+    problem's DIIS history holds the same iterations (they start together
+    and each keeps the newest max_diis+1), so one segmented call forms all
+    new rows of the DIIS matrices, one the extrapolated amplitudes and one
+    the convergence sums; each problem's sums are bit-identical to its own
+    solve's, and a problem whose test has passed keeps its amplitudes from
+    then on, as one solve per problem would leave them.  Returns (E [nb]
+    host, X [nb, L] device, iterations per problem)."""
+    from .so_ci import _converged
+    lib = _lib.load()
+    st = stream_of(dev)
+    diis = bool(params['DIIS'])
+    nvec = max_diis + 1
+    X = torch.zeros((nb, L), dtype=CDT, device=dev)
+    Xn = torch.zeros((nb, L), dtype=CDT, device=dev)
+    s = torch.zeros(nb, dtype=CDT, device=dev)
+    s_new = torch.zeros(nb, dtype=CDT, device=dev)
+    R = torch.zeros((nb, L), dtype=CDT, device=dev)
+    step(Xn, X, R, s)                              # guess: one step from zero amplitudes (s = 0)
+    Xn.copy_(X)
+    energies(X, s)
+    E = s.cpu().numpy()
+    if diis:
+        Th = torch.zeros((nvec, nb, L), dtype=CDT, device=dev)
+        Eh = torch.zeros((nvec, nb, L), dtype=CDT, device=dev)
+    hist = []                                      # (iteration, slot), shared by the problems
+    Bs = np.zeros((nb, nvec, nvec), dtype=np.complex128)
+    scal = torch.zeros((nb, _lib.APY_MAXVEC), dtype=CDT, device=dev)
+    dsq = torch.zeros((nb, 2), dtype=CDT, device=dev)
+    active = np.ones(nb, dtype=bool)
+    iters = [0] * nb
+    red = int(lib.apy_reduce_seg_workspace(2 * nb, _lib.APY_MAXVEC))
+    shift = np.zeros(nb) if shift is None else np.asarray(shift)
+    it = 0
+    while active.any() and it < params['max_iterations']:
+        it += 1
+        if diis:
+            slot = (it - 1) % nvec
+            step(X, Th[slot], Eh[slot], s)
+            while len(hist) > max_diis:
+                hist.pop(0)
+            hist.append((it, slot))
+            k = len(hist)
+            arr = (ctypes.c_void_p * k)(*[Eh[h].data_ptr() for _, h in hist])
+            ws = workspace(red, dev)
+            _lib.check(lib.apy_zdot_many_seg(k, arr, _vp(Eh[slot]), nb, L, 1, _vp(scal), _vp(ws), red, st),
+                       "apy_zdot_many_seg")
+            rows = scal.view(-1)[:nb * k].cpu().numpy().reshape(nb, k)      # packed [nb][k]
+            slots = [h for _, h in hist]
+            Bs[:, slots, slot] = rows                                         # E^H E by slot (utils.py:121)
+            Bs[:, slot, slots] = np.conj(rows)
+            act = np.flatnonzero(active)
+            Bm = np.full((len(act), k + 1, k + 1), -1.0, dtype=np.complex128)
+            Bm[:, :k, :k] = Bs[act][:, slots][:, :, slots]
+            Bm[:, k, k] = 0
+            rhs = np.zeros((len(act), k + 1, 1), dtype=np.complex128)
+            rhs[:, k] = -1
+            coef = np.zeros((nb, k), dtype=np.complex128)
+            coef[:, -1] = 1.0                      # finished problems: unused
+            coef[act] = np.linalg.solve(Bm, rhs)[:, :k, 0]                    # utils.py:135
+            cdev = torch.from_numpy(coef).to(dev)
+            arr = (ctypes.c_void_p * k)(*[Th[h].data_ptr() for _, h in hist])
+            _lib.check(lib.apy_zcombine_seg(k, arr, _vp(cdev), nb, L, _vp(Xn), st), "apy_zcombine_seg")
+        else:
+            step(X, Xn, R, s)
+        energies(Xn, s_new)
+        ws = workspace(red, dev)
+        _lib.check(lib.apy_zdiffsq_batch(_vp(X), _vp(Xn), nb, L, OV, _vp(dsq), _vp(ws), red, st),
+                   "apy_zdiffsq_batch")
+        E_new = s_new.cpu().numpy()
+        d = dsq.cpu().numpy()
+        act = np.flatnonzero(active)
+        if len(act) == nb:
+            X.copy_(Xn)
+            s.copy_(s_new)
+        else:
+            for q in act:
+                X[q].copy_(Xn[q])
+                s[q:q + 1].copy_(s_new[q:q + 1])
+        for q in act:
+            rms1, rms2 = np.sqrt(d[q, 0]), np.sqrt(d[q, 1])
+            dE = E[q] - E_new[q]
+            E[q] = E_new[q]
+            iters[q] = it
+            if print_level > 0:
+                print(" %02d %3d %20.12f %20.12f %20.12f" % (it, q, (E[q] + shift[q]).real, dE.real, rms2.real))
+            if _converged(it, dE, [rms1, rms2] if singles else [rms2], params):
+                active[q] = False
+    if active.any():
+        print("Not converged.")
+    return E, X, iters
+
+
+class SpatialCISet:
+    """Spin-adapted CISD (singles=True) or CID of nb independent systems of
+    one (o, v) — the displaced geometries and fields of a finite-difference
+    set, which fin_diff.py:267-372 solves one after another (ci_wfn.py:420-574
+    or :51-167 each) — iterated together: every iteration is one
+    apy_cisd_sp_step_multi over all systems (each with its own integral
+    blocks, Fock matrix and orbital energies, every product batched), then
+    per system the reference's DIIS extrapolation and convergence test
+    (_solve_set).  Each system's iterates are those of its own SpatialCI
+    solve.
+
+    eri_mos: nb chemists' MO tensors (pq|rs) [m]^4 on one device; focks: nb
+    [m, m] MO Fock matrices; eps_list: nb orbital-energy arrays [m]; o: the
+    occupied count (the same for all)."""
+
+    def __init__(self, eri_mos, focks, eps_list, o, singles=True):
+        self.lib = lib = _lib.load()
+        self.nb = nb = len(eri_mos)
+        if nb < 1 or not (len(focks) == len(eps_list) == nb):
+            raise ValueError("one Fock matrix and one orbital-energy array per system")
+        self.device = dev = eri_mos[0].device
+        m = eri_mos[0].shape[0]
+        if any(tuple(e.shape) != (m,) * 4 for e in eri_mos):
+            raise ValueError("all systems must have the same number of orbitals")
+        self.O, self.V = o, v = int(o), m - int(o)
+        self.singles = bool(singles)
+        self.OV, self.L = o * v, o * v + o * o * v * v
+        st = stream_of(dev)
+        sizes = (ctypes.c_int64 * len(_lib.SP_BLOCKS))()
+        _lib.check(lib.apy_sp_ints_sizes(o, v, sizes), "apy_sp_ints_sizes")
+        names = _lib.SP_BLOCKS if self.singles else CID_SP_BLOCKS
+        self.blocks = {b: torch.empty((nb, int(sizes[k])), dtype=CDT, device=dev)
+                       for k, b in enumerate(_lib.SP_BLOCKS) if b in names}
+        self.structs = [_lib.SpInts(o, v, *[self.blocks[b][q].data_ptr() if b in self.blocks else None
+                                            for b in _lib.SP_BLOCKS]) for q in range(nb)]
+        self.fock = torch.stack([f.to(dev, CDT) for f in focks]).contiguous()
+        self.eps = torch.stack([torch.as_tensor(np.array(e, dtype=np.float64) if not isinstance(e, torch.Tensor)
+                                                else e, dtype=torch.float64).to(dev) for e in eps_list]).contiguous()
+        self.hoo = torch.empty((nb, o ** 4), dtype=CDT, device=dev)
+        self.g = torch.empty((nb, 2 * o * v), dtype=CDT, device=dev)
+        for q in range(nb):
+            eri = eri_mos[q].to(CDT).contiguous()
+            _lib.check(lib.apy_sp_ints_from_mo(m, _vp(eri), ctypes.byref(self.structs[q]), st),
+                       "apy_sp_ints_from_mo")
+            _lib.check(lib.apy_cisd_sp_prepare(o, v, _vp(self.fock[q]), ctypes.byref(self.structs[q]),
+                                               _vp(self.hoo[q]), _vp(self.g[q]), st), "apy_cisd_sp_prepare")
+        self.ws_bytes = int(lib.apy_cisd_sp_multi_workspace(o, v, int(self.singles), nb))
+        self.en_bytes = nb * int(lib.apy_cisd_sp_energy_workspace())
+
+    def _step(self, x, x_new, out, s):
+        args = _lib.CisdSpArgs(self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(),
+                               ctypes.pointer(self.structs[0]), self.hoo.data_ptr(), self.g.data_ptr(),
+                               int(self.singles), s.data_ptr(), x.data_ptr(), x_new.data_ptr(), out.data_ptr())
+        ws = workspace(self.ws_bytes, self.device)
+        _lib.check(self.lib.apy_cisd_sp_step_multi(ctypes.byref(args), self.nb, _vp(ws), self.ws_bytes,
+                                                   stream_of(self.device)), "apy_cisd_sp_step_multi")
+
+    def _energies(self, x, out):
+        ws = workspace(self.en_bytes, self.device)
+        _lib.check(self.lib.apy_cisd_sp_energy_multi(self.O, self.V, int(self.singles), self.nb, _vp(self.g),
+                                                     _vp(self.blocks["l"]), _vp(x), _vp(out), _vp(ws),
+                                                     self.en_bytes, stream_of(self.device)),
+                   "apy_cisd_sp_energy_multi")
+
+    def solve(self, params, print_level=0, max_diis=7):
+        """Returns (E [nb] complex host array, t1 [nb,o,v] or None,
+        t2 [nb,o,o,v,v] device views, iterations per system)."""
+        E, X, iters = _solve_set(self._step, self._energies, self.nb, self.L, self.OV, self.singles,
+                                 self.device, params, print_level, max_diis)
+        nb, O, V = self.nb, self.O, self.V
+        t1 = X[:, :self.OV].view(nb, O, V) if self.singles else None
+        return E, t1, X[:, self.OV:].view(nb, O, O, V, V), iters

@@ -229,6 +229,21 @@ int apy_cisd_sp_step_batch(const apy_cisd_sp_args* a, int64_t nb, const void* re
 int apy_cisd_sp_energy_batch(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L,
                              const void* t, void* energy, void* workspace, size_t ws_bytes, void* stream);
 
+/* The same step for nb independent systems of one (o, v) — the displaced
+ * geometries and fields of a finite-difference set, which fin_diff.py:267-372
+ * solves one after another (ci_wfn.py:420-574 each) — every instance with
+ * its own operands, stored one instance after another: fock + q n^2,
+ * eps + q n, hoo + q o^4, g + q 2ov, each block of ints + q times its
+ * apy_sp_ints_sizes entry (a->ints points at instance 0's blocks), t, t_new,
+ * out + q (ov + o^2 v^2), energy + q.  Every product is batched over the
+ * instances.  The response term is not available (resp_b must be null).
+ * apy_cisd_sp_energy_multi: energy[q] with g and L per instance (strides 2ov
+ * and o^2 v^2). */
+size_t apy_cisd_sp_multi_workspace(int64_t o, int64_t v, int singles, int64_t nb);
+int apy_cisd_sp_step_multi(const apy_cisd_sp_args* a, int64_t nb, void* workspace, size_t ws_bytes, void* stream);
+int apy_cisd_sp_energy_multi(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L,
+                             const void* t, void* energy, void* workspace, size_t ws_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * CISD / CID in spin orbitals: one iteration of the reference's Jacobi
  * step, or the plain sigma product (for Davidson).

@@ -1,7 +1,9 @@
 """Batched displaced Hartree-Fock (hf_wfn.solve_SCF_batch: one
-apy_fock_build_batch per iteration for every instance) and the batched
-phase-corrected energies behind the finite-difference driver, against the
-one-by-one solves they replace (hf_wfn.py:36-170, energy.py:84-150)."""
+apy_fock_build_batch per iteration for every instance), the batched
+phase-corrected energies behind the finite-difference driver (CISD/CID
+through spatial_ci.SpatialCISet) and the set solver itself, against the
+one-by-one solves they replace (hf_wfn.py:36-170, energy.py:84-150,
+ci_wfn.py:420-574)."""
 import copy
 
 import numpy as np
@@ -51,7 +53,7 @@ def test_batched_scf_equals_single_solves(dev):
         assert np.array_equal(w.eps, single.eps)
 
 
-@pytest.mark.parametrize("method", ["MP2", "CISD"])
+@pytest.mark.parametrize("method", ["MP2", "CISD", "CID"])
 def test_phase_corrected_energies_equal_one_by_one(dev, method):
     p0, plist = _displaced(method)
     _, _, C0, basis0 = energy(p0)
@@ -61,3 +63,41 @@ def test_phase_corrected_energies_equal_one_by_one(dev, method):
         assert np.abs(np.asarray(E_list, dtype=complex) - np.asarray(E1, dtype=complex)).max() < 1e-13
         assert np.abs(C - Cs).max() < 1e-12
         assert np.abs(np.asarray(T_list[2]) - np.asarray(T1[2])).max() < 1e-12
+
+
+def _spatial_system(o, v, seed, std=0.05, noise=1e-3):
+    rng = np.random.default_rng(seed)
+    n = o + v
+    E = std * (rng.standard_normal((n,) * 4) + 1j * rng.standard_normal((n,) * 4))
+    E = 0.5 * (E + E.transpose(2, 3, 0, 1))
+    eps = np.concatenate([np.sort(rng.uniform(-2.0, -0.5, o)), np.sort(rng.uniform(0.3, 2.0, v))])
+    F = np.diag(eps).astype(complex) + noise * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    return E, F, eps
+
+
+@pytest.mark.parametrize("singles", [True, False], ids=["CISD", "CID"])
+@pytest.mark.parametrize("o,v,nb", [(3, 5, 4), (4, 9, 7), (1, 4, 3)])
+def test_spatial_set_equals_single_solves(dev, singles, o, v, nb):
+    """nb systems with their own integrals, Fock matrices and orbital
+    energies iterated together (apy_cisd_sp_step_multi): every system's
+    energy, amplitudes and iteration count equal its own SpatialCI solve
+    (host-controlled, the same DIIS arithmetic); systems converge at
+    different iterations."""
+    import torch
+    from apyib_amd.spatial_ci import SPIntegrals, SpatialCI, SpatialCISet
+    from tests.helpers import PARAMS as P
+    systems = [_spatial_system(o, v, 40 + q, std=0.03 + 0.01 * q) for q in range(nb)]
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    cset = SpatialCISet([d(E4) for E4, _, _ in systems], [d(F) for _, F, _ in systems],
+                        [eps for _, _, eps in systems], o, singles=singles)
+    E, t1, t2, iters = cset.solve(P)
+    if o > 1:
+        assert len(set(iters)) > 1         # systems of different coupling stop at different iterations
+    for q, (E4, F, eps) in enumerate(systems):
+        ci = SpatialCI(SPIntegrals.from_mo(d(E4), o, singles=singles), d(F), eps, singles=singles)
+        Eq, t1q, t2q, itq = ci.solve(P, control="host")
+        assert iters[q] == itq
+        assert abs(E[q] - Eq) < 1e-13 * max(1.0, abs(Eq))
+        assert (t2[q] - t2q).abs().max().item() < 1e-12
+        if singles:
+            assert (t1[q] - t1q).abs().max().item() < 1e-12
