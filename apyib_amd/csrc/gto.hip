@@ -122,86 +122,111 @@ __device__ int herm_R(int L, const int* Lt, double alpha, const double* PQ, cons
   return cur;
 }
 
-// Contracted integral (a b | c d) at one geometry.  Returns NaN for a table entry outside the limits
-// (angular momentum, primitive count, term count, atom index), so a bad table cannot index out of bounds.
-__device__ double eri_quartet(const apy_gto& A, const apy_gto& B, const apy_gto& C, const apy_gto& D,
-                              const double* cen, int64_t natom) {
-  if (!gto_ok(A, natom) || !gto_ok(B, natom) || !gto_ok(C, natom) || !gto_ok(D, natom)) return kNaN;
-  const double* ca = cen + 3 * A.atom;
-  const double* cb = cen + 3 * B.atom;
-  const double* cc = cen + 3 * C.atom;
-  const double* cd = cen + 3 * D.atom;
-  const double qab[3] = {ca[0] - cb[0], ca[1] - cb[1], ca[2] - cb[2]};
-  const double qcd[3] = {cc[0] - cd[0], cc[1] - cd[1], cc[2] - cd[2]};
+// One primitive quartet of (a b | c d): terms (ta, tb, tc, td), primitives (pa, pb, pc, pd), centres ca..cd,
+// qab = A - B, qcd = C - D.  The caller has checked the table entries and that the quartet's total angular
+// momentum is at most LQ.
+__device__ double eri_prim(const apy_gto& A, const apy_gto& B, const apy_gto& C, const apy_gto& D, int ta, int tb,
+                           int tc, int td, int pa, int pb, int pc, int pd, const double* ca, const double* cb,
+                           const double* cc, const double* cd, const double* qab, const double* qcd) {
+  const int* la = A.lmn[ta];
+  const int* lb = B.lmn[tb];
+  const int* lc = C.lmn[tc];
+  const int* ld = D.lmn[td];
+  int L1[3], L2[3], Lt[3];
+  for (int x = 0; x < 3; ++x) { L1[x] = la[x] + lb[x]; L2[x] = lc[x] + ld[x]; Lt[x] = L1[x] + L2[x]; }
+  const int L = Lt[0] + Lt[1] + Lt[2];
+  const int sz = Lt[2] + 1, sy = (Lt[1] + 1) * sz;  // R index (t, u, v) -> t sy + u sz + v
+  const double a = A.exps[pa], b = B.exps[pb], p = a + b;
+  const double c = C.exps[pc], d = D.exps[pd], q = c + d;
+  double Eab[3][NE], Ecd[3][NE], PQ[3];
+  for (int x = 0; x < 3; ++x) {
+    herm_E(la[x], lb[x], qab[x], a, b, Eab[x]);
+    herm_E(lc[x], ld[x], qcd[x], c, d, Ecd[x]);
+    PQ[x] = (a * ca[x] + b * cb[x]) / p - (c * cc[x] + d * cd[x]) / q;
+  }
+  const double alpha = p * q / (p + q);
+  double F[LQ + 1];
+  boys_fn(L, alpha * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]), F);
+  double Rb[2][NRB];
+  const double* R = Rb[herm_R(L, Lt, alpha, PQ, F, Rb)];
+  double val = 0.0;
+  for (int t = 0; t <= L1[0]; ++t)
+    for (int u = 0; u <= L1[1]; ++u)
+      for (int v = 0; v <= L1[2]; ++v) {
+        const double e1 = Eab[0][t] * Eab[1][u] * Eab[2][v];
+        double s2 = 0.0;
+        for (int tt = 0; tt <= L2[0]; ++tt)
+          for (int uu = 0; uu <= L2[1]; ++uu)
+            for (int vv = 0; vv <= L2[2]; ++vv) {
+              const double e2 = Ecd[0][tt] * Ecd[1][uu] * Ecd[2][vv];
+              const double sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
+              s2 += sg * e2 * R[(t + tt) * sy + (u + uu) * sz + (v + vv)];
+            }
+        val += e1 * s2;
+      }
   const double pref0 = 2.0 * M_PI * M_PI * sqrt(M_PI);
-  double total = 0.0;
+  return A.coefs[ta][pa] * B.coefs[tb][pb] * C.coefs[tc][pc] * D.coefs[td][pd] * pref0 / (p * q * sqrt(p + q)) * val;
+}
+
+// Whether every term combination of the quartet stays within the total angular momentum LQ.
+__device__ bool quartet_ok(const apy_gto& A, const apy_gto& B, const apy_gto& C, const apy_gto& D) {
   for (int ta = 0; ta < A.nterm; ++ta)
     for (int tb = 0; tb < B.nterm; ++tb)
       for (int tc = 0; tc < C.nterm; ++tc)
-        for (int td = 0; td < D.nterm; ++td) {
-          const int* la = A.lmn[ta];
-          const int* lb = B.lmn[tb];
-          const int* lc = C.lmn[tc];
-          const int* ld = D.lmn[td];
-          int L1[3], L2[3], Lt[3];
-          for (int x = 0; x < 3; ++x) { L1[x] = la[x] + lb[x]; L2[x] = lc[x] + ld[x]; Lt[x] = L1[x] + L2[x]; }
-          const int L = Lt[0] + Lt[1] + Lt[2];
-          if (L > LQ) return kNaN;
-          const int sz = Lt[2] + 1, sy = (Lt[1] + 1) * sz;       // R index (t, u, v) -> t sy + u sz + v
-          for (int pa = 0; pa < A.nprim; ++pa)
-            for (int pb = 0; pb < B.nprim; ++pb) {
-              const double a = A.exps[pa], b = B.exps[pb], p = a + b;
-              const double cab = A.coefs[ta][pa] * B.coefs[tb][pb];
-              double Eab[3][NE];
-              for (int x = 0; x < 3; ++x) herm_E(la[x], lb[x], qab[x], a, b, Eab[x]);
-              double P[3];
-              for (int x = 0; x < 3; ++x) P[x] = (a * ca[x] + b * cb[x]) / p;
-              for (int pc = 0; pc < C.nprim; ++pc)
-                for (int pd = 0; pd < D.nprim; ++pd) {
-                  const double c = C.exps[pc], d = D.exps[pd], q = c + d;
-                  const double ccd = C.coefs[tc][pc] * D.coefs[td][pd];
-                  double Ecd[3][NE];
-                  for (int x = 0; x < 3; ++x) herm_E(lc[x], ld[x], qcd[x], c, d, Ecd[x]);
-                  double PQ[3];
-                  for (int x = 0; x < 3; ++x) PQ[x] = P[x] - (c * cc[x] + d * cd[x]) / q;
-                  const double alpha = p * q / (p + q);
-                  double F[LQ + 1];
-                  boys_fn(L, alpha * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]), F);
-                  double Rb[2][NRB];
-                  const double* R = Rb[herm_R(L, Lt, alpha, PQ, F, Rb)];
-                  double val = 0.0;
-                  for (int t = 0; t <= L1[0]; ++t)
-                    for (int u = 0; u <= L1[1]; ++u)
-                      for (int v = 0; v <= L1[2]; ++v) {
-                        const double e1 = Eab[0][t] * Eab[1][u] * Eab[2][v];
-                        double s2 = 0.0;
-                        for (int tt = 0; tt <= L2[0]; ++tt)
-                          for (int uu = 0; uu <= L2[1]; ++uu)
-                            for (int vv = 0; vv <= L2[2]; ++vv) {
-                              const double e2 = Ecd[0][tt] * Ecd[1][uu] * Ecd[2][vv];
-                              const double sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
-                              s2 += sg * e2 * R[(t + tt) * sy + (u + uu) * sz + (v + vv)];
-                            }
-                        val += e1 * s2;
-                      }
-                  total += cab * ccd * pref0 / (p * q * sqrt(p + q)) * val;
-                }
-            }
-        }
-  return total;
+        for (int td = 0; td < D.nterm; ++td)
+          if (lsum(A.lmn[ta]) + lsum(B.lmn[tb]) + lsum(C.lmn[tc]) + lsum(D.lmn[td]) > LQ) return false;
+  return true;
 }
 
+// One wavefront per output element (wave-uniform control flow): the lanes split the element's
+// (term, primitive) quartets and a fixed-order butterfly sums them.  A table entry outside the limits
+// (angular momentum, primitive count, term count, atom index) gives NaN before any indexing.
 __global__ __launch_bounds__(256) void eri_kernel(int64_t ngeom, int64_t natom, const double* __restrict__ centers,
                                                   const apy_gto* __restrict__ fa, int64_t na,
                                                   const apy_gto* __restrict__ fb, int64_t nb,
                                                   const apy_gto* __restrict__ fc, int64_t nc,
                                                   const apy_gto* __restrict__ fd, int64_t nd,
                                                   double2* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
   const int64_t per = na * nb * nc * nd, total = ngeom * per;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t e = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; e < total; e += nwave) {
     const int64_t g = e / per, r = e % per;
     const int64_t l = r % nd, k = (r / nd) % nc, j = (r / (nd * nc)) % nb, i = r / (nd * nc * nb);
-    out[e] = make_double2(eri_quartet(fa[i], fb[j], fc[k], fd[l], centers + 3 * natom * g, natom), 0.0);
+    const apy_gto& A = fa[i];
+    const apy_gto& B = fb[j];
+    const apy_gto& C = fc[k];
+    const apy_gto& D = fd[l];
+    if (!gto_ok(A, natom) || !gto_ok(B, natom) || !gto_ok(C, natom) || !gto_ok(D, natom) ||
+        !quartet_ok(A, B, C, D)) {
+      if (lane == 0) out[e] = make_double2(kNaN, 0.0);
+      continue;
+    }
+    const double* cen = centers + 3 * natom * g;
+    const double* ca = cen + 3 * A.atom;
+    const double* cb = cen + 3 * B.atom;
+    const double* cc = cen + 3 * C.atom;
+    const double* cd = cen + 3 * D.atom;
+    const double qab[3] = {ca[0] - cb[0], ca[1] - cb[1], ca[2] - cb[2]};
+    const double qcd[3] = {cc[0] - cd[0], cc[1] - cd[1], cc[2] - cd[2]};
+    // quartet w = ((((((ta nbt + tb) nct + tc) ndt + td) npa + pa) npb + pb) npc + pc) npd + pd
+    const int nt = A.nterm * B.nterm * C.nterm * D.nterm;
+    const int np = A.nprim * B.nprim * C.nprim * D.nprim;
+    double acc = 0.0;
+    for (int w = lane; w < nt * np; w += 64) {
+      int x = w;
+      const int pd = x % D.nprim; x /= D.nprim;
+      const int pc = x % C.nprim; x /= C.nprim;
+      const int pb = x % B.nprim; x /= B.nprim;
+      const int pa = x % A.nprim; x /= A.nprim;
+      const int td = x % D.nterm; x /= D.nterm;
+      const int tc = x % C.nterm; x /= C.nterm;
+      const int tb = x % B.nterm; x /= B.nterm;
+      const int ta = x;
+      acc += eri_prim(A, B, C, D, ta, tb, tc, td, pa, pb, pc, pd, ca, cb, cc, cd, qab, qcd);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) out[e] = make_double2(acc, 0.0);
   }
 }
 
@@ -253,71 +278,76 @@ __device__ void ovl_1d(int imax, int jmax, double Q, double a, double b, double 
 //   T = t_x s_y s_z + s_x t_y s_z + s_x s_y t_z,  D_x = m_x s_y s_z,  L_x = (m_y d_z - m_z d_y) s_x (cyclic).
 // The nuclear attraction uses the Hermite expansion of the product:
 //   V = -sum_C Z_C 2 pi / p sum_tuv E^x_t E^y_u E^z_v R_tuv(p, P - C).
+// The (term, primitive) pairs w = ((ta nbt + tb) npa + pa) npb + pb from `first` in steps of `step` (the
+// lanes of a wavefront split them).
 __device__ void oei_pair(const apy_gto& A, const double* ra, const apy_gto& B, const double* rb, int which,
-                         const double* origin, const double* nuc, const double* Z, int64_t nnuc, double* v) {
+                         const double* origin, const double* nuc, const double* Z, int64_t nnuc, int first, int step,
+                         double* v) {
   const double Q[3] = {ra[0] - rb[0], ra[1] - rb[1], ra[2] - rb[2]};
-  for (int ta = 0; ta < A.nterm; ++ta)
-    for (int tb = 0; tb < B.nterm; ++tb) {
-      const int* la = A.lmn[ta];
-      const int* lb = B.lmn[tb];
-      for (int pa = 0; pa < A.nprim; ++pa)
-        for (int pb = 0; pb < B.nprim; ++pb) {
-          const double a = A.exps[pa], b = B.exps[pb], p = a + b;
-          const double c = A.coefs[ta][pa] * B.coefs[tb][pb];
-          double S[3][LF + 1][OJ], s[3], t[3], m[3], d[3];
-          for (int x = 0; x < 3; ++x) {
-            const int i = la[x], j = lb[x];
-            ovl_1d(i, j + 2, Q[x], a, b, S[x]);
-            s[x] = S[x][i][j];
-            t[x] = -0.5 * ((j > 1 ? j * (j - 1) * S[x][i][j - 2] : 0.0) - 2.0 * b * (2 * j + 1) * S[x][i][j] +
-                           4.0 * b * b * S[x][i][j + 2]);
-            m[x] = S[x][i][j + 1] + (rb[x] - origin[x]) * S[x][i][j];
-            d[x] = (j > 0 ? j * S[x][i][j - 1] : 0.0) - 2.0 * b * S[x][i][j + 1];
-          }
-          if (which & APY_OEI_OVERLAP) v[0] += c * s[0] * s[1] * s[2];
-          if (which & APY_OEI_KINETIC)
-            v[1] += c * (t[0] * s[1] * s[2] + s[0] * t[1] * s[2] + s[0] * s[1] * t[2]);
-          if (which & APY_OEI_DIPOLE) {
-            v[3] += c * m[0] * s[1] * s[2];
-            v[4] += c * s[0] * m[1] * s[2];
-            v[5] += c * s[0] * s[1] * m[2];
-          }
-          if (which & APY_OEI_ANGMOM) {
-            v[6] += c * (m[1] * d[2] - m[2] * d[1]) * s[0];
-            v[7] += c * (m[2] * d[0] - m[0] * d[2]) * s[1];
-            v[8] += c * (m[0] * d[1] - m[1] * d[0]) * s[2];
-          }
-          if (which & APY_OEI_POTENTIAL) {
-            double E[3][NE], P[3];
-            int Lt[3];
-            for (int x = 0; x < 3; ++x) {
-              herm_E(la[x], lb[x], Q[x], a, b, E[x]);
-              P[x] = (a * ra[x] + b * rb[x]) / p;
-              Lt[x] = la[x] + lb[x];
-            }
-            const int L = Lt[0] + Lt[1] + Lt[2];
-            const int sz = Lt[2] + 1, sy = (Lt[1] + 1) * sz;
-            double acc = 0.0;
-            for (int64_t C = 0; C < nnuc; ++C) {
-              const double PC[3] = {P[0] - nuc[3 * C], P[1] - nuc[3 * C + 1], P[2] - nuc[3 * C + 2]};
-              double F[LQ + 1], Rb[2][NRB];
-              boys_fn(L, p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), F);
-              const double* R = Rb[herm_R(L, Lt, p, PC, F, Rb)];
-              double sum = 0.0;
-              for (int tt = 0; tt <= Lt[0]; ++tt)
-                for (int u = 0; u <= Lt[1]; ++u)
-                  for (int w = 0; w <= Lt[2]; ++w) sum += E[0][tt] * E[1][u] * E[2][w] * R[tt * sy + u * sz + w];
-              acc -= Z[C] * sum;
-            }
-            v[2] += c * 2.0 * M_PI / p * acc;
-          }
-        }
+  const int nw = A.nterm * B.nterm * A.nprim * B.nprim;
+  for (int w = first; w < nw; w += step) {
+    const int pb = w % B.nprim, pa = (w / B.nprim) % A.nprim, tb = (w / (B.nprim * A.nprim)) % B.nterm,
+              ta = w / (B.nprim * A.nprim * B.nterm);
+    const int* la = A.lmn[ta];
+    const int* lb = B.lmn[tb];
+    const double a = A.exps[pa], b = B.exps[pb], p = a + b;
+    const double c = A.coefs[ta][pa] * B.coefs[tb][pb];
+    double S[3][LF + 1][OJ], s[3], t[3], m[3], d[3];
+    for (int x = 0; x < 3; ++x) {
+      const int i = la[x], j = lb[x];
+      ovl_1d(i, j + 2, Q[x], a, b, S[x]);
+      s[x] = S[x][i][j];
+      t[x] = -0.5 * ((j > 1 ? j * (j - 1) * S[x][i][j - 2] : 0.0) - 2.0 * b * (2 * j + 1) * S[x][i][j] +
+                     4.0 * b * b * S[x][i][j + 2]);
+      m[x] = S[x][i][j + 1] + (rb[x] - origin[x]) * S[x][i][j];
+      d[x] = (j > 0 ? j * S[x][i][j - 1] : 0.0) - 2.0 * b * S[x][i][j + 1];
     }
+    if (which & APY_OEI_OVERLAP) v[0] += c * s[0] * s[1] * s[2];
+    if (which & APY_OEI_KINETIC) v[1] += c * (t[0] * s[1] * s[2] + s[0] * t[1] * s[2] + s[0] * s[1] * t[2]);
+    if (which & APY_OEI_DIPOLE) {
+      v[3] += c * m[0] * s[1] * s[2];
+      v[4] += c * s[0] * m[1] * s[2];
+      v[5] += c * s[0] * s[1] * m[2];
+    }
+    if (which & APY_OEI_ANGMOM) {
+      v[6] += c * (m[1] * d[2] - m[2] * d[1]) * s[0];
+      v[7] += c * (m[2] * d[0] - m[0] * d[2]) * s[1];
+      v[8] += c * (m[0] * d[1] - m[1] * d[0]) * s[2];
+    }
+    if (which & APY_OEI_POTENTIAL) {
+      double E[3][NE], P[3];
+      int Lt[3];
+      for (int x = 0; x < 3; ++x) {
+        herm_E(la[x], lb[x], Q[x], a, b, E[x]);
+        P[x] = (a * ra[x] + b * rb[x]) / p;
+        Lt[x] = la[x] + lb[x];
+      }
+      const int L = Lt[0] + Lt[1] + Lt[2];
+      const int sz = Lt[2] + 1, sy = (Lt[1] + 1) * sz;
+      double acc = 0.0;
+      for (int64_t C = 0; C < nnuc; ++C) {
+        const double PC[3] = {P[0] - nuc[3 * C], P[1] - nuc[3 * C + 1], P[2] - nuc[3 * C + 2]};
+        double F[LQ + 1], Rb[2][NRB];
+        boys_fn(L, p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), F);
+        const double* R = Rb[herm_R(L, Lt, p, PC, F, Rb)];
+        double sum = 0.0;
+        for (int tt = 0; tt <= Lt[0]; ++tt)
+          for (int u = 0; u <= Lt[1]; ++u)
+            for (int vv = 0; vv <= Lt[2]; ++vv) sum += E[0][tt] * E[1][u] * E[2][vv] * R[tt * sy + u * sz + vv];
+        acc -= Z[C] * sum;
+      }
+      v[2] += c * 2.0 * M_PI / p * acc;
+    }
+  }
 }
 
+// One wavefront per output element (wave-uniform control flow): the lanes split the (term, primitive)
+// pairs and a fixed-order butterfly sums each component.
 __global__ __launch_bounds__(256) void oei_kernel(apy_oei_args p) {
+  const int lane = threadIdx.x & 63;
   const int64_t per = p.na * p.nb, total = p.nset * per;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t e = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; e < total; e += nwave) {
     const int64_t s = e / per, r = e % per, i = r / p.nb, j = r % p.nb;
     const apy_gto& A = p.fa[i];
     const apy_gto& B = p.fb[j];
@@ -325,10 +355,14 @@ __global__ __launch_bounds__(256) void oei_kernel(apy_oei_args p) {
     for (int k = 0; k < NOEI; ++k) v[k] = 0.0;
     if (gto_ok(A, p.nca) && gto_ok(B, p.ncb)) {
       oei_pair(A, p.ca + 3 * (s * p.nca + A.atom), B, p.cb + 3 * (s * p.ncb + B.atom), p.which, p.origin,
-               p.nuclei ? p.nuclei + 3 * s * p.nnuc : nullptr, p.charges, p.nnuc, v);
+               p.nuclei ? p.nuclei + 3 * s * p.nnuc : nullptr, p.charges, p.nnuc, lane, 64, v);
+#pragma unroll
+      for (int k = 0; k < NOEI; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
     } else {
       for (int k = 0; k < NOEI; ++k) v[k] = kNaN;
     }
+    if (lane != 0) continue;
     // selected components in the order S, T, V, Dx, Dy, Dz, Lx, Ly, Lz
     constexpr int bit_of[NOEI] = {APY_OEI_OVERLAP, APY_OEI_KINETIC, APY_OEI_POTENTIAL, APY_OEI_DIPOLE, APY_OEI_DIPOLE,
                                   APY_OEI_DIPOLE,  APY_OEI_ANGMOM,  APY_OEI_ANGMOM,    APY_OEI_ANGMOM};
@@ -350,7 +384,7 @@ int apy_eri_gto(int64_t ngeom, int64_t natom, const double* centers, const apy_g
       nd < 1)
     return APY_E_ARG;
   const int64_t total = ngeom * na * nb * nc * nd;
-  eri_kernel<<<apy_grid1d(total), 256, 0, (hipStream_t)stream>>>(ngeom, natom, centers, fa, na, fb, nb, fc, nc, fd,
+  eri_kernel<<<apy_grid1d(64 * total), 256, 0, (hipStream_t)stream>>>(ngeom, natom, centers, fa, na, fb, nb, fc, nc, fd,
                                                                   nd, (double2*)out);
   APY_LAUNCH_CHECK();
   return APY_OK;
@@ -373,7 +407,7 @@ int apy_oei_gto(const apy_oei_args* a, void* stream) {
     return APY_E_ARG;
   if ((a->which & APY_OEI_POTENTIAL) && (!a->nuclei || !a->charges || a->nnuc < 1)) return APY_E_ARG;
   const int64_t total = a->nset * a->na * a->nb;
-  oei_kernel<<<apy_grid1d(total), 256, 0, (hipStream_t)stream>>>(*a);
+  oei_kernel<<<apy_grid1d(64 * total), 256, 0, (hipStream_t)stream>>>(*a);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }
