@@ -68,6 +68,7 @@ class SpInts(ctypes.Structure):
 
 
 APY_GTO_MAXPRIM = 6
+APY_GTO_MAXL = 3
 
 
 class Gto(ctypes.Structure):

@@ -11,10 +11,10 @@
 
 namespace {
 
-constexpr int LF = 2;                     // largest angular momentum of one term along all axes together
-constexpr int LQ = 5;                     // largest total angular momentum of a quartet
+constexpr int LF = APY_GTO_MAXL;          // largest angular momentum of one term along all axes together
+constexpr int LQ = 3 * 2 + LF;            // largest total of a quartet: three d functions and one f term
 constexpr int NE = 2 * LF + 1;            // E_t^{ij}, t <= i + j
-constexpr int NRB = 18;                   // (Lx+1)(Ly+1)(Lz+1) <= 18 for Lx+Ly+Lz <= 5
+constexpr int NRB = 64;                   // (Lx+1)(Ly+1)(Lz+1) <= 64 for Lx+Ly+Lz <= 9
 
 // F_n(T), n = 0..nmax (nmax <= LQ): the series e^-T sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)) for the top
 // order and downward recursion F_n = (2T F_{n+1} + e^-T) / (2n+1) below T = 30; above, F_0 from erf and

@@ -20,7 +20,8 @@ from itertools import product
 import numpy as np
 from scipy.special import gammainc, gamma
 
-ANG = {0: [(0, 0, 0)], 1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)]}
+ANG = {0: [(0, 0, 0)], 1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
+       2: [(2, 0, 0), (1, 1, 0), (1, 0, 1), (0, 2, 0), (0, 1, 1), (0, 0, 2)]}   # Cartesian d, psi4's order
 
 
 def boys(nmax, T):

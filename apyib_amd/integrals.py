@@ -31,7 +31,7 @@ def _entry(atom, terms):
         raise ValueError("function outside the integral kernel's limits")
     g.atom, g.nprim, g.nterm = int(atom), len(exps), len(terms)
     for k, (lmn, coefs, e) in enumerate(terms):
-        if sum(lmn) > 2 or not np.array_equal(e, exps):
+        if sum(lmn) > _lib.APY_GTO_MAXL or not np.array_equal(e, exps):
             raise ValueError("function outside the integral kernel's limits")
         for x in range(3):
             g.lmn[k][x] = int(lmn[x])

@@ -4,7 +4,8 @@ Parses the subset of the psi4 geometry format used by the reference's
 apyib/data/molecules.py (symbol x y z lines, `units bohr|angstrom`,
 `symmetry c1`, `no_com`/`nocom`, `no_reorient`/`noreorient`, optional
 charge/multiplicity line), and builds STO-3G (H, C, N, O) and STO-6G (H)
-basis sets from the published Hehre-Stewart-Pople contraction tables.
+basis sets from the published Hehre-Stewart-Pople contraction tables, and
+6-31G / 6-31G* (H, O; Cartesian d functions) from the Pople tables.
 Coordinates are used as given (the reference's molecules all disable
 re-centering and re-orientation).
 """
@@ -21,6 +22,16 @@ _S3 = (0.15432897, 0.53532814, 0.44463454)
 _SP3_S = (-0.09996723, 0.39951283, 0.70011547)
 _SP3_P = (0.15591627, 0.60768372, 0.39195739)
 
+# 6-31G (Hehre-Ditchfield-Pople; split valence, sp shells sharing exponents) for H and O, and the d
+# polarisation exponent 0.8 of 6-31G* on O (Hariharan-Pople), as the reference's psi4 uses them.
+_H_631G = [((0,), (18.7311370, 2.8253937, 0.6401217), ((0.03349460, 0.23472695, 0.81375733),)),
+           ((0,), (0.1612778,), ((1.0,),))]
+_O_631G = [((0,), (5484.67166, 825.234946, 188.046958, 52.9645000, 16.8975704, 5.79963534),
+            ((0.00183107, 0.0139501, 0.0684451, 0.232714, 0.470193, 0.358521),)),
+           ((0, 1), (15.5396162, 3.59993359, 1.01376175),
+            ((-0.110777549, -0.148026262, 1.13076701), (0.0708742682, 0.339752839, 0.727158577))),
+           ((0, 1), (0.270005823,), ((1.0,), (1.0,)))]
+
 # element -> list of (angular momentum list, exponents, coefficients per l)
 BASIS = {
     'STO-3G': {
@@ -36,6 +47,8 @@ BASIS = {
         1: [((0,), (35.52322122, 6.513143725, 1.822142904, 0.6259552659, 0.2430767471, 0.1001124280),
              ((0.00916359628, 0.04936149294, 0.16853830490, 0.37056279970, 0.41649152980, 0.13033408410),))],
     },
+    '6-31G': {1: _H_631G, 8: _O_631G},
+    '6-31G*': {1: _H_631G, 8: _O_631G + [((2,), (0.8,), ((1.0,),))]},   # Cartesian d (six functions)
 }
 
 

@@ -507,10 +507,11 @@ int apy_mp2_nc_batched_energy(int n, int64_t n_items, int use_x_new, double scal
  * A function is one or two terms sharing a centre (atom) and exponents:
  *   sum_t sum_p coefs[t][p] x^lmn[t][0] y^lmn[t][1] z^lmn[t][2] e^{-exps[p] r^2}
  * (coefficients include the primitive normalisation).  Limits: nprim <=
- * APY_GTO_MAXPRIM, lmn[t] summing to <= 2, quartet total <= 5 (s/p basis
+ * APY_GTO_MAXPRIM, lmn[t] summing to <= APY_GTO_MAXL, quartet total <= 9 (s/p/d basis
  * functions and their first derivatives); an entry outside them yields NaN.
  * ---------------------------------------------------------------------- */
 #define APY_GTO_MAXPRIM 6
+#define APY_GTO_MAXL 3
 typedef struct apy_gto {
   int32_t atom;                               /* index into the centres       */
   int32_t nprim;
@@ -546,7 +547,7 @@ int apy_eri_deriv_assemble(int64_t n, const int* atom_of, int atom, const void* 
  *   L  = <a_i| (r - origin) x nabla |b_j>   (real, antisymmetric for a = b).
  * Bra function i sits at ca[s][fa[i].atom], ket function j at
  * cb[s][fb[j].atom] (bohr), so bra and ket may belong to different
- * geometries.  Limits as for apy_eri_gto (lmn summing to <= 2); an entry
+ * geometries.  Limits as for apy_eri_gto (lmn summing to <= 3); an entry
  * outside them yields NaN. */
 #define APY_OEI_OVERLAP 1
 #define APY_OEI_KINETIC 2

@@ -1,17 +1,17 @@
 """Analytic Hartree-Fock, MP2, CID and CISD AATs through the drop-in
 apyib_amd.analytic_aats.analytic_derivative (GPU transforms of the MO and
 derivative integrals, first-order CI amplitudes) against the reference
-tests' arrays."""
+tests' arrays: all 28, H2O2/STO-3G and H2O/6-31G* (d functions)."""
 import numpy as np
 import pytest
 
 from apyib_amd.analytic_aats import analytic_derivative
-from tests.test_oracle_analytic import GOLDEN, golden_id, golden_params, runnable
+from tests.test_oracle_analytic import GOLDEN, golden_id, golden_params, gpu_runnable
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("e", [e for e in GOLDEN if runnable(e)], ids=golden_id)
+@pytest.mark.parametrize("e", [e for e in GOLDEN if gpu_runnable(e)], ids=golden_id)
 def test_analytic_aat_golden(dev, e):
     ad = analytic_derivative(golden_params(e))
     c = e["call"]

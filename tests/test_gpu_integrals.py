@@ -63,14 +63,14 @@ def test_eri_derivatives_match_host(dev):
 
 
 def test_entry_outside_limits_gives_nan(dev):
-    """A table entry with an f-type term (l = 3) is rejected inside the
-    kernel before any indexing: that element comes back NaN, the others
-    are unaffected."""
+    """A table entry with a g-type term (l = 4, beyond APY_GTO_MAXL) is
+    rejected inside the kernel before any indexing: that element comes back
+    NaN, the others are unaffected."""
     mol = Molecule.from_string(H2O)
     bs = BasisSet(mol, "STO-3G")
     good = basis_table(bs.functions, bs.atom_of, dev)
     g = _lib.Gto.from_buffer_copy(bytes(good[:ctypes.sizeof(_lib.Gto)].cpu().numpy()))
-    g.lmn[0][0] = 3
+    g.lmn[0][0] = _lib.APY_GTO_MAXL + 1
     bad = torch.from_numpy(np.frombuffer(bytes(g), dtype=np.uint8).copy()).to(dev)
     out = eri(bad, good, good, good, (1, 1, 1, 1), mol.coords[None], dev).cpu().numpy()
     assert np.isnan(out.real).all()
@@ -183,3 +183,32 @@ def test_deriv1_one_electron_matches_host(dev, geom):
             for x in range(3):
                 w = want[key][A][x]
                 assert np.abs(got[key][A][x] - w).max() < 1e-12 * max(1.0, np.abs(w).max()), (key, A, x)
+
+
+def test_d_functions_match_host(dev):
+    """6-31G* on H2O (Cartesian d on O): every one-electron integral, and
+    300 seeded ERI elements plus 99 derivative-function ERI elements (f
+    terms) against the host engine's single-quartet routine."""
+    mol = Molecule.from_string(H2O)
+    bs = BasisSet(mol, "6-31G*")
+    fns = bs.functions
+    n = len(fns)
+    got = hamiltonian_integrals(bs, mol.coords[None], mol.Z, dev).cpu().numpy()[0]
+    S, T, V, D, L = _host_one_electron(bs, mol)
+    for k, want in enumerate([S, T, V, D[0], D[1], D[2], L[0], L[1], L[2]]):
+        assert np.abs(got[k] - want).max() < 1e-12 * max(1.0, np.abs(want).max()), k
+    G = ao_eri(bs, mol.coords, dev).cpu().numpy().real
+    rng = np.random.default_rng(3)
+    dd = [i for i, f in enumerate(fns) if sum(f.lmn) == 2]
+    picks = [tuple(rng.integers(0, n, 4)) for _ in range(200)] + [tuple(rng.choice(dd, 4)) for _ in range(100)]
+    for (i, j, k, l) in picks:
+        want = gaussian.eri(fns[i], fns[j], fns[k], fns[l])
+        assert abs(G[i, j, k, l] - want) < 1e-12 * max(1.0, abs(want)), (i, j, k, l)
+    der = EriDerivatives(bs, mol.coords, dev)
+    for x in range(3):
+        Dx = der.D[x].cpu().numpy().real
+        for _ in range(33):
+            i = int(rng.choice(dd))
+            j, k, l = (int(q) for q in rng.integers(0, n, 3))
+            want = sum(gaussian.eri(t, fns[j], fns[k], fns[l]) for t in gaussian.center_derivative(fns[i], x))
+            assert abs(Dx[i, j, k, l] - want) < 1e-12 * max(1.0, abs(want)), (x, i, j, k, l)

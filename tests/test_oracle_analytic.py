@@ -2,9 +2,9 @@
 
 Goldens: tests/golden/analytic_aat_goldens.json (written by
 make_analytic_goldens.py from apyib/tests/test_018..021_*_Analytic_AAT.py).
-The H2O2/STO-3G entries run; the 6-31G* ones need d functions and that
-basis, which the package's basis table does not hold, and are reported as
-skipped."""
+The H2O2/STO-3G entries run here; the H2O/6-31G* ones (d functions) would
+take the host integral engine minutes and run on the GPU drop-in instead
+(tests/test_gpu_analytic.py)."""
 import copy
 import json
 import os
@@ -25,7 +25,13 @@ def golden_id(e):
 
 
 def runnable(e):
+    """Goldens the CPU oracle runs (the host integral engine at STO-3G)."""
     return e["parameters"]["basis"] == "STO-3G"
+
+
+def gpu_runnable(e):
+    """Goldens the GPU drop-in runs: every basis in the package's table."""
+    return e["parameters"]["basis"] in ("STO-3G", "6-31G*")
 
 
 def golden_params(e):
@@ -144,5 +150,18 @@ def test_density_gradient_equals_projected_derivative(method):
 
 
 @pytest.mark.parametrize("e", [e for e in GOLDEN if not runnable(e)], ids=golden_id)
-def test_analytic_golden_needs_basis(e):
-    pytest.skip("needs the 6-31G* basis (d functions), not in the package's basis table")
+def test_analytic_golden_on_the_host(e):
+    pytest.skip("19 functions with d shells: the host integral engine takes minutes; this golden runs on the "
+                "GPU drop-in (tests/test_gpu_analytic.py)")
+
+
+def test_631gs_basis_water():
+    """6-31G* on H2O: 19 Cartesian functions (O 1s, 2sp, 3sp, six d; H two
+    s each), every contracted function normalised, one frozen core orbital."""
+    mol = Molecule.from_string(H2O)
+    bs = BasisSet(mol, '6-31G*')
+    assert bs.nbf() == 19 and bs.n_frozen_core() == 1
+    assert sum(1 for f in bs.functions if sum(f.lmn) == 2) == 6
+    S = gaussian.build_mixed_overlap(bs.functions, bs.functions)
+    assert np.abs(np.diag(S) - 1.0).max() < 1e-12
+    assert np.linalg.eigvalsh(S).min() > 1e-3
