@@ -9,6 +9,8 @@
 //             R_(t+t')(u+u')(v+v')(alpha, P - Q).
 #include "common.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int LF = APY_GTO_MAXL;          // largest angular momentum of one term along all axes together
@@ -16,20 +18,55 @@ constexpr int LQ = 3 * 2 + LF;            // largest total of a quartet: three d
 constexpr int NE = 2 * LF + 1;            // E_t^{ij}, t <= i + j
 constexpr int NRB = 64;                   // (Lx+1)(Ly+1)(Lz+1) <= 64 for Lx+Ly+Lz <= 9
 
-// F_n(T), n = 0..nmax (nmax <= LQ): the series e^-T sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)) for the top
-// order and downward recursion F_n = (2T F_{n+1} + e^-T) / (2n+1) below T = 30; above, F_0 from erf and
-// upward recursion F_{n+1} = ((2n+1) F_n - e^-T) / (2T) (no cancellation there for n <= 5).
+// F_n(T), n = 0..nmax, by the series e^-T sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)) for the top order and
+// downward recursion F_n = (2T F_{n+1} + e^-T) / (2n+1) (all terms positive: no cancellation).
+__device__ void boys_series(int nmax, double T, double* F) {
+  double term = 1.0 / (2 * nmax + 1), sum = term;
+  for (int k = 1; k < 400; ++k) {
+    term *= 2.0 * T / (2 * nmax + 2 * k + 1);
+    sum += term;
+    if (term < 1e-17 * sum) break;
+  }
+  const double e = exp(-T);
+  F[nmax] = e * sum;
+  for (int n = nmax - 1; n >= 0; --n) F[n] = (2.0 * T * F[n + 1] + e) / (2 * n + 1);
+}
+
+// F_n(T_k), T_k = k / BOYS_DENS below BOYS_TMAX, orders 0..BOYS_NORD-1: filled once per device by
+// boys_table_kernel (boys_series at every grid point), read by boys_fn.
+constexpr int BOYS_NORD = LQ + 7;
+constexpr int BOYS_DENS = 20;
+constexpr double BOYS_TMAX = 36.0;
+constexpr int BOYS_NGRID = 36 * BOYS_DENS + 2;
+__device__ double g_boys[BOYS_NGRID][BOYS_NORD];
+
+__global__ __launch_bounds__(256) void boys_table_kernel() {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= BOYS_NGRID) return;
+  double F[BOYS_NORD];
+  boys_series(BOYS_NORD - 1, (double)k / BOYS_DENS, F);
+  for (int n = 0; n < BOYS_NORD; ++n) g_boys[k][n] = F[n];
+}
+
+// F_n(T), n = 0..nmax (nmax <= LQ).  Below BOYS_TMAX the seven-term Taylor expansion about the nearest grid
+// point (dF_n/dT = -F_{n+1}):  F_n(T) = sum_j F_{n+j}(T_k) (T_k - T)^j / j!,  |T_k - T| <= 0.025, remainder
+// below 1e-15 F_n.  Above, F_0 from erf and upward recursion F_{n+1} = ((2n+1) F_n - e^-T) / (2T) (e^-T is
+// then far below every (2n+1) F_n: no cancellation).
 __device__ void boys_fn(int nmax, double T, double* F) {
-  if (T < 30.0) {
-    double term = 1.0 / (2 * nmax + 1), sum = term;
-    for (int k = 1; k < 400; ++k) {
-      term *= 2.0 * T / (2 * nmax + 2 * k + 1);
-      sum += term;
-      if (term < 1e-17 * sum) break;
+  if (T < BOYS_TMAX) {
+    const double x = T * BOYS_DENS;
+    const int k = (int)(x + 0.5);
+    const double d = ((double)k - x) / BOYS_DENS;
+    const double* g = g_boys[k];
+    for (int n = 0; n <= nmax; ++n) {
+      double f = g[n + 6];
+      f = g[n + 5] + d * f * (1.0 / 6.0);
+      f = g[n + 4] + d * f * (1.0 / 5.0);
+      f = g[n + 3] + d * f * (1.0 / 4.0);
+      f = g[n + 2] + d * f * (1.0 / 3.0);
+      f = g[n + 1] + d * f * (1.0 / 2.0);
+      F[n] = g[n] + d * f;
     }
-    const double e = exp(-T);
-    F[nmax] = e * sum;
-    for (int n = nmax - 1; n >= 0; --n) F[n] = (2.0 * T * F[n + 1] + e) / (2 * n + 1);
   } else {
     const double e = exp(-T);
     F[0] = 0.5 * sqrt(M_PI / T) * erf(sqrt(T));
@@ -373,6 +410,22 @@ __global__ __launch_bounds__(256) void oei_kernel(apy_oei_args p) {
       if (p.which & bit_of[k]) p.out[((s * ncomp + c++) * p.na + i) * p.nb + j] = v[k];
   }
 }
+// Fill g_boys on the current device before its first integral launch; the stream is synchronised once so
+// that launches on other streams see the table.
+int ensure_boys_table(hipStream_t st) {
+  static std::mutex mu;
+  static bool ready[256] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 256) return APY_E_HIP;
+  std::lock_guard<std::mutex> lk(mu);
+  if (ready[dev]) return APY_OK;
+  boys_table_kernel<<<(BOYS_NGRID + 255) / 256, 256, 0, st>>>();
+  APY_LAUNCH_CHECK();
+  if (hipStreamSynchronize(st) != hipSuccess) return APY_E_HIP;
+  ready[dev] = true;
+  return APY_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -384,6 +437,7 @@ int apy_eri_gto(int64_t ngeom, int64_t natom, const double* centers, const apy_g
       nd < 1)
     return APY_E_ARG;
   const int64_t total = ngeom * na * nb * nc * nd;
+  if (const int rc = ensure_boys_table((hipStream_t)stream); rc != APY_OK) return rc;
   eri_kernel<<<apy_grid1d(64 * total), 256, 0, (hipStream_t)stream>>>(ngeom, natom, centers, fa, na, fb, nb, fc, nc, fd,
                                                                   nd, (double2*)out);
   APY_LAUNCH_CHECK();
@@ -407,6 +461,8 @@ int apy_oei_gto(const apy_oei_args* a, void* stream) {
     return APY_E_ARG;
   if ((a->which & APY_OEI_POTENTIAL) && (!a->nuclei || !a->charges || a->nnuc < 1)) return APY_E_ARG;
   const int64_t total = a->nset * a->na * a->nb;
+  if ((a->which & APY_OEI_POTENTIAL) != 0)
+    if (const int rc = ensure_boys_table((hipStream_t)stream); rc != APY_OK) return rc;
   oei_kernel<<<apy_grid1d(64 * total), 256, 0, (hipStream_t)stream>>>(*a);
   APY_LAUNCH_CHECK();
   return APY_OK;
