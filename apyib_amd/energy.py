@@ -1,9 +1,12 @@
 """Method dispatch with apyib's interface (energy.py:17-150)."""
+import numpy as np
+
 from .ci_wfn import ci_wfn
 from .hamiltonian import build_hamiltonians
 from .hf_wfn import hf_wfn, solve_SCF_batch
 from .integrals import prime_device_eri
 from .mp2_wfn import mp2_wfn
+from .spatial_ci import SpatialCISet
 from .utils import compute_phase, device
 
 
@@ -109,8 +112,6 @@ def _solve_spatial_set(parameter_list, wfns):
     result equals its own ci_wfn solve (printing the same amplitude lines).
     None when the set does not qualify (other methods, one instance,
     differing solver settings or sizes)."""
-    import numpy as np
-    from .spatial_ci import SpatialCISet
     p0 = parameter_list[0]
     if p0['method'] not in ('CISD', 'CID') or len(parameter_list) < 2:
         return None

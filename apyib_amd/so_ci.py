@@ -7,6 +7,8 @@ Python only queues iterations and reads a control block; solve_host keeps
 them on the host with a handful of scalars copied back per iteration.
 """
 import ctypes
+import math
+import time
 
 import numpy as np
 import torch
@@ -324,8 +326,6 @@ class SOCI:
         launches it saves, so it is off by default).  Heavy problems read the
         control block after every iteration.  The iterates are the
         reference's: same window, same solve, same test."""
-        import math
-        import time
         dev, lib = self.device, self.lib
         nvec = max_diis + 1
         max_it = int(params['max_iterations'])
