@@ -23,14 +23,15 @@ _SP3_S = (-0.09996723, 0.39951283, 0.70011547)
 _SP3_P = (0.15591627, 0.60768372, 0.39195739)
 
 # 6-31G (Hehre-Ditchfield-Pople; split valence, sp shells sharing exponents) for H and O, and the d
-# polarisation exponent 0.8 of 6-31G* on O (Hariharan-Pople), as the reference's psi4 uses them.
+# polarisation exponent 0.8 of 6-31G* on O (Hariharan-Pople), to the digits of the basis-set library files
+# the reference's psi4 reads (the RHF/6-31G(d) golden of test_002_RHF.py:73 fixes them to 1e-12 Eh).
 _H_631G = [((0,), (18.7311370, 2.8253937, 0.6401217), ((0.03349460, 0.23472695, 0.81375733),)),
            ((0,), (0.1612778,), ((1.0,),))]
-_O_631G = [((0,), (5484.67166, 825.234946, 188.046958, 52.9645000, 16.8975704, 5.79963534),
-            ((0.00183107, 0.0139501, 0.0684451, 0.232714, 0.470193, 0.358521),)),
-           ((0, 1), (15.5396162, 3.59993359, 1.01376175),
-            ((-0.110777549, -0.148026262, 1.13076701), (0.0708742682, 0.339752839, 0.727158577))),
-           ((0, 1), (0.270005823,), ((1.0,), (1.0,)))]
+_O_631G = [((0,), (5484.6717, 825.23495, 188.04696, 52.9645, 16.89757, 5.7996353),
+            ((0.0018311, 0.0139501, 0.0684451, 0.2327143, 0.4701930, 0.3585209),)),
+           ((0, 1), (15.539616, 3.5999336, 1.0137618),
+            ((-0.1107775, -0.1480263, 1.1307670), (0.0708743, 0.3397528, 0.7271586))),
+           ((0, 1), (0.2700058,), ((1.0,), (1.0,)))]
 
 # element -> list of (angular momentum list, exponents, coefficients per l)
 BASIS = {
@@ -50,6 +51,7 @@ BASIS = {
     '6-31G': {1: _H_631G, 8: _O_631G},
     '6-31G*': {1: _H_631G, 8: _O_631G + [((2,), (0.8,), ((1.0,),))]},   # Cartesian d (six functions)
 }
+BASIS['6-31G(D)'] = BASIS['6-31G*']      # psi4 reads both names as the same basis
 
 
 class Molecule:

@@ -31,6 +31,23 @@ def test_energy_h2o_sto3g_golden(dev, method, gold):
     assert abs((E_list[0] + E_list[1] + E_list[2]) - gold) < 1e-10
 
 
+# H2O (apyib/data/molecules.py:2-10) in the split-valence bases: test_002_RHF.py:73 and :98 (RHF, 6-31G(d)),
+# test_003_MP2.py:60 and :99 (MP2, 6-31G(d)), test_005_CISD.py:21 and test_008_CISD_SO.py:21 (CISD, 6-31G).
+GOLD_RHF_H2O_631GD = -75.97474825544526
+GOLD_MP2_H2O_631GD = -76.17533681889488
+GOLD_CISD_H2O_631G = -76.09503651362023
+
+
+@pytest.mark.parametrize("method,basis,gold", [("RHF", "6-31G(d)", GOLD_RHF_H2O_631GD),
+                                               ("MP2", "6-31G(d)", GOLD_MP2_H2O_631GD),
+                                               ("MP2_SO", "6-31G(d)", GOLD_MP2_H2O_631GD),
+                                               ("CISD", "6-31G", GOLD_CISD_H2O_631G),
+                                               ("CISD_SO", "6-31G", GOLD_CISD_H2O_631G)])
+def test_energy_h2o_split_valence_golden(dev, method, basis, gold):
+    E_list, T_list, C, b = energy_mod.energy(params(H2O, method, basis))
+    assert abs((E_list[0] + E_list[1] + E_list[2]) - gold) < 1e-10
+
+
 @pytest.mark.parametrize("method", ["CISD_SO", "CISD"])
 def test_energy_h2_2_cisd_golden(dev, method):
     E_list, T_list, C, basis = energy_mod.energy(params(H2_2, method))
