@@ -1,13 +1,15 @@
 """AO->MO->spin-orbital transforms, overlaps and phase of apyib's utils.py
 restated in NumPy (oracle only).
 
-Pinned by: tests/test_oracle_transforms.py -- the loop forms of
-compute_F_SO / compute_ERI_SO / compute_so_overlap (statements of
-utils.py:283-365, 393-422 with explicit spin integration) against the
-vectorized forms used elsewhere; compute_ERI_MO against a direct four-index
-sum on small inputs and against invariants (unitary C leaves the
-full-space trace invariant; real C gives the real transform); compute_phase
-against its defining property (diagonal of the overlap made real positive).
+Pinned by (tests/test_oracle_golden.py unless noted): the loop forms of
+compute_F_SO / compute_ERI_SO (statements of utils.py:283-365 with explicit
+spin integration) against the vectorized forms used elsewhere;
+compute_ERI_MO against a direct four-index sum on small inputs;
+compute_phase against its defining property (diagonal of the overlap made
+real positive); compute_so_overlap through the spin-orbital determinants,
+which must factor into products of the spatial alpha and beta ones
+(tests/test_oracle_aat.py); all of them through the reference's golden
+energies and AAT arrays they feed.
 """
 import numpy as np
 
