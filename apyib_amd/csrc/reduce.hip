@@ -214,7 +214,7 @@ __device__ inline double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
 // slot order) is solved by Gaussian elimination with partial pivoting (largest |re| + |im|, as LAPACK's
 // getrf), threads over rows; coef[q] = c of slot q, zero outside the window.  Nothing is done once the
 // control block's status ctl[1] is set; a zero pivot sets it to 3 (singular DIIS system).
-__global__ __launch_bounds__(64) void diis_slot_kernel(int nslot, int slot, unsigned valid,
+__global__ __launch_bounds__(256) void diis_slot_kernel(int nslot, int slot, unsigned valid,
                                                        const double2* __restrict__ row, double2* __restrict__ B,
                                                        double2* __restrict__ coef, int* __restrict__ ctl) {
   constexpr int NM = APY_MAXVEC + 1;
@@ -244,27 +244,33 @@ __global__ __launch_bounds__(64) void diis_slot_kernel(int nslot, int slot, unsi
     M[tid][n] = tid < nv ? make_double2(0.0, 0.0) : make_double2(-1.0, 0.0);
   }
   __syncthreads();
+  __shared__ double2 fac[NM], xs[NM];
+  __shared__ int piv_s;
   for (int c = 0; c < n; ++c) {
     if (tid == 0) {
       int p = c;
       for (int r = c + 1; r < n; ++r)
         if (cabs1(M[r][c]) > cabs1(M[p][c])) p = r;
       if (cabs1(M[p][c]) == 0.0) bad_s = 1;
-      if (p != c)
-        for (int j = c; j <= n; ++j) {
-          const double2 t = M[p][j];
-          M[p][j] = M[c][j];
-          M[c][j] = t;
-        }
+      piv_s = p;
     }
     __syncthreads();
     if (bad_s) break;
-    if (tid > c && tid < n) {
-      const double2 f = cdiv(M[tid][c], M[c][c]);
-      for (int j = c; j <= n; ++j) {
-        const double2 m = cmul(f, M[c][j]);
-        M[tid][j] = make_double2(M[tid][j].x - m.x, M[tid][j].y - m.y);
-      }
+    const int p = piv_s;
+    if (p != c && tid <= n - c) {                   // swap rows p and c, one column per thread
+      const int j = c + tid;
+      const double2 t = M[p][j];
+      M[p][j] = M[c][j];
+      M[c][j] = t;
+    }
+    __syncthreads();
+    if (tid > c && tid < n) fac[tid] = cdiv(M[tid][c], M[c][c]);
+    __syncthreads();
+    const int w = n - c;                             // columns c+1..n of rows c+1..n-1, one element per thread
+    for (int e = tid; e < (n - 1 - c) * w; e += blockDim.x) {
+      const int r = c + 1 + e / w, j = c + 1 + e % w;
+      const double2 m = cmul(fac[r], M[c][j]);
+      M[r][j] = make_double2(M[r][j].x - m.x, M[r][j].y - m.y);
     }
     __syncthreads();
   }
@@ -273,17 +279,16 @@ __global__ __launch_bounds__(64) void diis_slot_kernel(int nslot, int slot, unsi
     ctl[1] = 3;
     return;
   }
-  double2 x[NM];
   for (int r = n - 1; r >= 0; --r) {
     double2 acc = M[r][n];
     for (int j = r + 1; j < n; ++j) {
-      const double2 m = cmul(M[r][j], x[j]);
+      const double2 m = cmul(M[r][j], xs[j]);
       acc = make_double2(acc.x - m.x, acc.y - m.y);
     }
-    x[r] = cdiv(acc, M[r][r]);
+    xs[r] = cdiv(acc, M[r][r]);
   }
   for (int q = 0; q < nslot; ++q) coef[q] = make_double2(0.0, 0.0);
-  for (int a = 0; a < nv; ++a) coef[idx[a]] = x[a];
+  for (int a = 0; a < nv; ++a) coef[idx[a]] = xs[a];
 }
 
 // Principal square root (the branch of numpy.sqrt on complex numbers), without cancellation.
@@ -505,7 +510,7 @@ int apy_diis_slot(int nslot, int slot, uint32_t valid, const void* row, void* B,
   if (nslot < 1 || nslot > APY_MAXVEC || slot < 0 || slot >= nslot || !((valid >> slot) & 1u) ||
       (nslot < 32 && (valid >> nslot) != 0u) || !row || !B || !coef || !ctl)
     return APY_E_ARG;
-  diis_slot_kernel<<<1, 64, 0, (hipStream_t)stream>>>(nslot, slot, valid, (const double2*)row, (double2*)B,
+  diis_slot_kernel<<<1, 256, 0, (hipStream_t)stream>>>(nslot, slot, valid, (const double2*)row, (double2*)B,
                                                       (double2*)coef, ctl);
   APY_LAUNCH_CHECK();
   return APY_OK;
