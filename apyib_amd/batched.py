@@ -149,9 +149,9 @@ def mp2_adjoint_directional(base, dF, dW, dV, tol=1e-12, maxit=200):
     dev = base.F.device
     O, V = base.O, base.V
     tb = MP2NCBatch([base])
-    tb.solve(tol, maxit)
     lb = MP2NCBatch([Instance(O, V, base.F, base.eps, base.w, None)], transpose_f=True, drv_scale=0.25)
-    lb.solve(tol, maxit)
+    tb.solve(tol, maxit, chunk=8)
+    lb.solve(tol, maxit, chunk=8)
     t, lam = tb.x[0], lb.x[0]
     n = O + V
     G = torch.empty(n * n, dtype=CDT, device=dev)
@@ -161,20 +161,21 @@ def mp2_adjoint_directional(base, dF, dW, dV, tol=1e-12, maxit=200):
                                         ctypes.c_void_p(G.data_ptr()), ctypes.c_void_p(ws.data_ptr()), nb,
                                         stream_of(dev)), "apy_mp2_nc_densities")
     K = len(dF)
-    out = np.zeros(K, dtype=np.complex128)
-    part = torch.empty(_lib.APY_MAXVEC, dtype=CDT, device=dev)
-    for xs, y, scale in ((dF, G, 1.0), (dW, t, 0.25), (dV, lam, 1.0)):
-        if xs is None:
-            continue
+    terms = [(xs, y, scale) for xs, y, scale in ((dF, G, 1.0), (dW, t, 0.25), (dV, lam, 1.0)) if xs is not None]
+    part = torch.zeros((len(terms), K), dtype=CDT, device=dev)      # every directional sum, read back once
+    for q, (xs, y, _) in enumerate(terms):
         for s0 in range(0, K, _lib.APY_MAXVEC):
             chunk = xs[s0:s0 + _lib.APY_MAXVEC]
             arr = (ctypes.c_void_p * len(chunk))(*[x.data_ptr() for x in chunk])
             nb = int(lib.apy_reduce_workspace(y.numel(), len(chunk)))
             ws = workspace(nb, dev)
             _lib.check(lib.apy_zdot_many(len(chunk), arr, ctypes.c_void_p(y.data_ptr()), y.numel(), 0,
-                                         ctypes.c_void_p(part.data_ptr()), ctypes.c_void_p(ws.data_ptr()), nb,
-                                         stream_of(dev)), "apy_zdot_many")
-            out[s0:s0 + len(chunk)] += scale * part[:len(chunk)].cpu().numpy()
+                                         ctypes.c_void_p(part[q, s0:].data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                                         nb, stream_of(dev)), "apy_zdot_many")
+    sums = part.cpu().numpy()
+    out = np.zeros(K, dtype=np.complex128)
+    for q, (_, _, scale) in enumerate(terms):
+        out += scale * sums[q]
     return out, (G, t, lam)
 
 

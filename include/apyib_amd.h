@@ -478,8 +478,8 @@ int apy_so_ints_grad_to_full(int64_t n, const apy_so_ints* grad, void* full, voi
  * One workgroup per (instance, i, j) pair; every step covers all instances.
  * setup uploads the instance and pair tables into the workspace (host
  * arrays; synchronises the stream) and returns the pair count; step and
- * energy read only device memory, so a sequence of them can be captured in
- * a HIP graph.  reverse = 1 iterates x_new -> x (ping-pong without
+ * energy touch only device memory (they write scratch regions of the
+ * workspace), so a sequence of them can be captured in a HIP graph.  reverse = 1 iterates x_new -> x (ping-pong without
  * re-setup).  rnorm[k] = sum |r_k|^2 (double); energy[k] = scale *
  * sum w_k x_k (complex; w may be NULL).
  * ---------------------------------------------------------------------- */
@@ -496,9 +496,9 @@ size_t apy_mp2_nc_batched_workspace(int n, const apy_mp2_nc_instance* inst);
 int apy_mp2_nc_batched_setup(int n, const apy_mp2_nc_instance* inst, void* workspace, size_t ws_bytes,
                              int64_t* n_items, void* stream);
 int apy_mp2_nc_batched_step(int n, int64_t n_items, int transpose_f, double drv_scale, int reverse, void* rnorm,
-                            const void* workspace, void* stream);
+                            void* workspace, void* stream);
 int apy_mp2_nc_batched_energy(int n, int64_t n_items, int use_x_new, double scale, void* energy,
-                              const void* workspace, void* stream);
+                              void* workspace, void* stream);
 
 /* ------------------------------------------------------------------------
  * AO electron-repulsion integrals (replace psi4's MintsHelper.ao_eri of
