@@ -116,13 +116,12 @@ __global__ __launch_bounds__(BLK) void nc_z_kernel(const apy_mp2_nc_instance* __
 
 // W_ij[a][b] = Y_ij[a][b] - Y_ij[b][a], Y_ij = x_ij Fvv^T, for every pair i != j: one (instance, i, j) pair per
 // workgroup.  Y is formed in 2 x 2 blocks: a thread owns the block pair {(A,B), (B,A)} (A <= B), so it holds
-// Y[a][b] and Y[b][a] for its elements in registers and writes both antisymmetrised elements; x_ij and Fvv are
-// staged in LDS when V <= LDS_V.
+// Y[a][b] and Y[b][a] for its elements in registers and writes both antisymmetrised elements; Fvv is staged in
+// LDS when V <= LDS_V, x_ij is read through the caches.
 __global__ __launch_bounds__(BLK) void nc_w_kernel(const apy_mp2_nc_instance* __restrict__ insts,
                                                    const Item* __restrict__ items, int transpose_f, int reverse,
                                                    const int64_t* __restrict__ zoff, int ninst,
                                                    double2* __restrict__ zbase) {
-  __shared__ double2 xs[LDS_V * LDS_V];
   __shared__ double2 fs[LDS_V * LDS_V];
   const Item it = items[blockIdx.x];
   if (it.i == it.j) return;  // block-uniform: the diagonal pair keeps its amplitudes
@@ -136,12 +135,9 @@ __global__ __launch_bounds__(BLK) void nc_w_kernel(const apy_mp2_nc_instance* __
   // Fvv(b, q): F[O+b][O+q] (or F[O+q][O+b] for F^T), staged as fs[b V + q]
   const int64_t fr0 = transpose_f ? 1 : n, fc0 = transpose_f ? n : 1;
   if (lds)
-    for (int64_t e = threadIdx.x; e < VV; e += BLK) {
-      xs[e] = xij[e];
-      fs[e] = F[(O + e / V) * fr0 + (O + e % V) * fc0];
-    }
+    for (int64_t e = threadIdx.x; e < VV; e += BLK) fs[e] = F[(O + e / V) * fr0 + (O + e % V) * fc0];
   __syncthreads();
-  const double2* X = lds ? (const double2*)xs : xij;
+  const double2* X = xij;   // rows of x_ij stay in L1/L2: with Fvv alone in LDS two workgroups fit a CU
   const int64_t fr = lds ? V : fr0, fc = lds ? 1 : fc0;
   const double2* Fv = lds ? (const double2*)fs : F + O * n + O;
   const int64_t nb = (V + 1) / 2;
