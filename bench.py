@@ -623,7 +623,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} (WORLD_SIZE={world}) is not supported: "
-              "the sharded design is described in DESIGN.md and is not built", file=sys.stderr)
+              "sharding the instance sets and the ladder product over GPUs is designed but not built",
+              file=sys.stderr)
         sys.exit(2)
 
     sys.path.insert(0, ROOT)
