@@ -22,6 +22,8 @@
 // can be captured in a HIP graph.
 #include "common.h"
 
+#include <map>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -29,6 +31,7 @@ namespace {
 constexpr int BLK = 256;
 constexpr int LDS_V = 66;          // x_ij and Fvv staged in LDS when V <= LDS_V
 constexpr int ZMAX_O = 32;         // Z kernel keeps an element's O amplitudes in registers up to this O
+constexpr int GEMM_MIN_V = 48;     // from this V on, Y = x Fvv^T of an instance is one MFMA GEMM (M = O^2 V)
 
 struct Item {
   int32_t inst, i, j, pad;
@@ -114,10 +117,9 @@ __global__ __launch_bounds__(BLK) void nc_z_kernel(const apy_mp2_nc_instance* __
   }
 }
 
-// W_ij[a][b] = Y_ij[a][b] - Y_ij[b][a], Y_ij = x_ij Fvv^T, for every pair i != j: one (instance, i, j) pair per
-// workgroup.  Y is formed in 2 x 2 blocks: a thread owns the block pair {(A,B), (B,A)} (A <= B), so it holds
-// Y[a][b] and Y[b][a] for its elements in registers and writes both antisymmetrised elements; Fvv is staged in
-// LDS when V <= LDS_V, x_ij is read through the caches.
+// Y_ij = x_ij Fvv^T for every pair i != j of the instances whose F_vv term is not a GEMM (V > GEMM_MIN_V does
+// not hold): one (instance, i, j) pair per workgroup, Y formed in 2 x 2 blocks (a thread owns the block pair
+// {(A,B), (B,A)}, A <= B); Fvv is staged in LDS when V <= LDS_V, x_ij is read through the caches.
 __global__ __launch_bounds__(BLK) void nc_w_kernel(const apy_mp2_nc_instance* __restrict__ insts,
                                                    const Item* __restrict__ items, int transpose_f, int reverse,
                                                    const int64_t* __restrict__ zoff, int ninst,
@@ -127,6 +129,7 @@ __global__ __launch_bounds__(BLK) void nc_w_kernel(const apy_mp2_nc_instance* __
   if (it.i == it.j) return;  // block-uniform: the diagonal pair keeps its amplitudes
   const apy_mp2_nc_instance I = insts[it.inst];
   const int64_t O = I.O, V = I.V, n = O + V, VV = V * V;
+  if (V >= GEMM_MIN_V) return;  // formed by the instance's GEMM
   const int64_t i = it.i, j = it.j;
   const double2* F = (const double2*)I.fock;
   const double2* xij = (const double2*)(reverse ? I.x_new : I.x) + (i * O + j) * VV;
@@ -163,19 +166,12 @@ __global__ __launch_bounds__(BLK) void nc_w_kernel(const apy_mp2_nc_instance* __
       cfma(yba[0][0], xb0, fa0); cfma(yba[0][1], xb0, fa1);
       cfma(yba[1][0], xb1, fa0); cfma(yba[1][1], xb1, fa1);
     }
-    // yab[u][w] = Y[a0+u][b0+w], yba[u][w] = Y[b0+u][a0+w]: W[a][b] for the elements of block (a0.., b0..)
-    // and of its mirror; on the diagonal block (A == B) the mirror is the same block.
-    for (int side = 0; side < 2; ++side) {
-      if (side == 1 && a0 == b0) break;
-      for (int u = 0; u < 2; ++u)
-        for (int w = 0; w < 2; ++w) {
-          const int64_t a = side == 0 ? a0 + u : b0 + w;
-          const int64_t b = side == 0 ? b0 + w : a0 + u;
-          if (a >= V || b >= V) continue;
-          W[a * V + b] = side == 0 ? make_double2(yab[u][w].x - yba[w][u].x, yab[u][w].y - yba[w][u].y)
-                                   : make_double2(yba[w][u].x - yab[u][w].x, yba[w][u].y - yab[u][w].y);
-        }
-    }
+    // yab[u][w] = Y[a0+u][b0+w], yba[u][w] = Y[b0+u][a0+w]
+    for (int u = 0; u < 2; ++u)
+      for (int w = 0; w < 2; ++w) {
+        if (a0 + u < V && b0 + w < V) W[(a0 + u) * V + b0 + w] = yab[u][w];
+        if (b0 + u < V && a0 + w < V) W[(b0 + u) * V + a0 + w] = yba[u][w];
+      }
   }
 }
 
@@ -200,7 +196,7 @@ __global__ __launch_bounds__(BLK) void nc_update_kernel(const apy_mp2_nc_instanc
     for (int64_t e = threadIdx.x; e < VV; e += BLK) xn[e] = x[e];
   } else {
     const double2* drv = (const double2*)I.drv + ij;
-    const double2* Wij = zbase + zoff[ninst] + zoff[it.inst] + ij;
+    const double2* Yij = zbase + zoff[ninst] + zoff[it.inst] + ij;   // Y_ij = x_ij Fvv^T
     const double2* Zij = zbase + zoff[it.inst] + ij;
     const double2* Zji = zbase + zoff[it.inst] + (j * O + i) * VV;
     const double eij = eps[i] + eps[j];
@@ -211,7 +207,8 @@ __global__ __launch_bounds__(BLK) void nc_update_kernel(const apy_mp2_nc_instanc
         xn[e] = xv;
         continue;
       }
-      const double2 d = drv[e], w = Wij[e], zij = Zij[e], zji = Zji[e];
+      const double2 d = drv[e], y1 = Yij[e], y2 = Yij[b * V + a], zij = Zij[e], zji = Zji[e];
+      const double2 w = make_double2(y1.x - y2.x, y1.y - y2.y);
       const double rr = s * d.x + w.x - zij.x + zji.x;
       const double ri = s * d.y + w.y - zij.y + zji.y;
       const double D = eij - eps[O + a] - eps[O + b];
@@ -282,6 +279,11 @@ __global__ __launch_bounds__(EN_B) void nc_energy_final(const double2* __restric
   if (threadIdx.x == 0) out[blockIdx.x] = make_double2(scale * red[0].x, scale * red[0].y);
 }
 
+// Host copy of each workspace's instance table (set up by apy_mp2_nc_batched_setup), from which the step
+// launches the per-instance F_vv GEMMs.
+std::mutex g_tab_mu;
+std::map<const void*, std::vector<apy_mp2_nc_instance>> g_tabs;
+
 int64_t count_items(int n, const apy_mp2_nc_instance* inst) {
   int64_t items = 0;
   for (int k = 0; k < n; ++k) items += inst[k].O * inst[k].O;
@@ -330,6 +332,10 @@ int apy_mp2_nc_batched_setup(int n, const apy_mp2_nc_instance* inst, void* works
   APY_HIP_TRY(hipMemcpyAsync(w + L.off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   APY_HIP_TRY(hipMemcpyAsync(w + L.zoff, zoff.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   APY_HIP_TRY(hipStreamSynchronize(st));
+  {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    g_tabs[workspace] = std::vector<apy_mp2_nc_instance>(inst, inst + n);
+  }
   *n_items = items;
   return APY_OK;
 }
@@ -345,11 +351,41 @@ int apy_mp2_nc_batched_step(int n, int64_t n_items, int transpose_f, double drv_
   double2* zbase = (double2*)(w + L.z);
   const auto* insts = (const apy_mp2_nc_instance*)(w + L.inst);
   const auto* items = (const Item*)(w + L.item);
-  // the W blocks start zoff[n] elements after the Z blocks; the kernels read that entry themselves
+  std::vector<apy_mp2_nc_instance> tab;
+  {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto f = g_tabs.find(workspace);
+    if (f == g_tabs.end() || (int)f->second.size() != n) return APY_E_ARG;  // not set up on this workspace
+    tab = f->second;
+  }
+  // the Y blocks start zoff[n] elements after the Z blocks (the kernels read that entry themselves)
   nc_z_kernel<<<(unsigned)n_items, BLK, 0, st>>>(insts, items, zoff, transpose_f, reverse, zbase);
   APY_LAUNCH_CHECK();
   nc_w_kernel<<<(unsigned)n_items, BLK, 0, st>>>(insts, items, transpose_f, reverse, zoff, n, zbase);
   APY_LAUNCH_CHECK();
+  // Y_k[(i,j,a)][b] = sum_q x_k[(i,j,a)][q] Fvv(b, q) on the matrix cores, one GEMM per instance with V >=
+  // GEMM_MIN_V (M = O^2 V, N = K = V; Fvv(b, q) = F[O+b][O+q], or F[O+q][O+b] for F^T)
+  double2* ybase = zbase;
+  int64_t yoff = 0;
+  for (int k = 0; k < n; ++k) yoff += tab[k].O * tab[k].O * tab[k].V * tab[k].V;
+  ybase += yoff;
+  int64_t off = 0;
+  for (int k = 0; k < n; ++k) {
+    const apy_mp2_nc_instance& I = tab[k];
+    const int64_t O = I.O, V = I.V, nn = O + V;
+    if (V >= GEMM_MIN_V) {
+      apy_gemm_desc d{};
+      d.m = O * O * V; d.n = V; d.k = V;
+      d.a_layout = APY_MK; d.a = reverse ? I.x_new : I.x; d.lda = V;
+      d.b_layout = transpose_f ? APY_KN : APY_NK;
+      d.b = (const double2*)I.fock + O * nn + O; d.ldb = nn;
+      d.c = ybase + off; d.ldc = V;
+      d.alpha_re = 1.0; d.batch = 1; d.split_k = 1;
+      const int rc = apy_zgemm_launch(&d, nullptr, 0, st);
+      if (rc != APY_OK) return rc;
+    }
+    off += O * O * V * V;
+  }
   nc_update_kernel<<<(unsigned)n_items, BLK, 0, st>>>(insts, items, drv_scale, reverse, zoff, n, zbase, part);
   APY_LAUNCH_CHECK();
   per_instance_sum<<<(unsigned)n, BLK, 0, st>>>((const int64_t*)(w + L.off), part, (double*)rnorm);

@@ -479,7 +479,10 @@ int apy_so_ints_grad_to_full(int64_t n, const apy_so_ints* grad, void* full, voi
  * setup uploads the instance and pair tables into the workspace (host
  * arrays; synchronises the stream) and returns the pair count; step and
  * energy touch only device memory (they write scratch regions of the
- * workspace), so a sequence of them can be captured in a HIP graph.  reverse = 1 iterates x_new -> x (ping-pong without
+ * workspace), so a sequence of them can be captured in a HIP graph.  setup
+ * also keeps a host copy of the instance table under the workspace's
+ * address, from which step launches one F_vv GEMM per instance with
+ * V >= 48 (step on a workspace that was never set up returns APY_E_ARG).  reverse = 1 iterates x_new -> x (ping-pong without
  * re-setup).  rnorm[k] = sum |r_k|^2 (double); energy[k] = scale *
  * sum w_k x_k (complex; w may be NULL).
  * ---------------------------------------------------------------------- */

@@ -129,3 +129,32 @@ def test_adjoint_directional_matches_fd(dev):
         scale = float((torch.sum((G * dF[k]).abs()) + 0.25 * torch.sum((t * dW[k]).abs())
                        + torch.sum((lam * dV[k]).abs())).cpu())
         assert abs(an[k] - fd) < 1e-7 * scale
+
+
+@pytest.mark.parametrize("transpose", [False, True], ids=["amplitudes", "lambda"])
+def test_batched_gemm_path_for_wide_instances(dev, transpose):
+    """Instances with V >= 48 form their F_vv term as one MFMA GEMM, the
+    others in the blocked kernel: a mixed batch converges to the oracle's
+    amplitudes (and lambda with F^T), and a graph-replayed chunk equals
+    direct launches bit for bit."""
+    sizes = [(3, 50), (2, 6), (2, 49)]
+    data = [_host_instance(O, V, 40 + k, dev) for k, (O, V) in enumerate(sizes)]
+    if transpose:
+        insts = [Instance(I.O, I.V, I.F, I.eps, I.w, None) for I, _ in data]
+        b = MP2NCBatch(insts, transpose_f=True, drv_scale=0.25)
+    else:
+        b = MP2NCBatch([d[0] for d in data])
+    b.solve(tol=1e-13, maxit=400, chunk=8)
+    for k, (_, (F, W, Vd, eps)) in enumerate(data):
+        if transpose:
+            _, _, want, _, _, _ = adjoint.gradients(F, W, Vd, eps, sizes[k][0])
+        else:
+            _, want = adjoint.energy(F, W, Vd, eps, sizes[k][0])
+        got = b.x[k].cpu().numpy().reshape(want.shape)
+        assert np.abs(got - want).max() < 1e-11 * max(1.0, np.abs(want).max()), k
+    g = MP2NCBatch([d[0] for d in data])
+    h = MP2NCBatch([d[0] for d in data])
+    g.solve(tol=0.0, maxit=8, chunk=4, graph=True)
+    h.solve(tol=0.0, maxit=8, chunk=4, graph=False)
+    for k in range(len(sizes)):
+        assert torch.equal(g.x[k], h.x[k])
