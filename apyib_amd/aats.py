@@ -39,7 +39,7 @@ from . import _lib
 from . import transforms as tr
 from .integrals import mixed_overlaps
 from .kernels import stream_of, workspace
-from .utils import device
+from .utils import device, mixed_to_basis
 
 CDT = torch.complex128
 NT = _lib.APY_AAT_NTERMS
@@ -145,7 +145,11 @@ class AAT(object):
         dev, nbf = self.dev, self.nbf
         nov = len(pairs)
         fn = lambda b: b.functions if hasattr(b, 'functions') else b
-        S_ao = torch.stack(mixed_overlaps([(fn(bb), fn(kb)) for _, _, _, (bb, _), (kb, _) in pairs], dev))
+        S_ao = mixed_overlaps([(fn(bb), fn(kb)) for _, _, _, (bb, _), (kb, _) in pairs], dev)
+        if any(getattr(b, 'transform', None) is not None for _, _, _, (bb, _), (kb, _) in pairs for b in (bb, kb)):
+            S_ao = [torch.from_numpy(mixed_to_basis(bb, kb, S.cpu().numpy())).to(dev)
+                    for S, (_, _, _, (bb, _), (kb, _)) in zip(S_ao, pairs)]
+        S_ao = torch.stack(S_ao)
         Cb = np.stack([np.asarray(c, dtype=np.complex128) for _, _, _, (_, c), _ in pairs])
         Ck = np.stack([np.asarray(c, dtype=np.complex128) for _, _, _, _, (_, c) in pairs])
         S_mo = tr.mo_overlap(tr.to_dev(Cb, dev), tr.to_dev(S_ao, dev), tr.to_dev(Ck, dev))

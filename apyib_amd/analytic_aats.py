@@ -101,6 +101,9 @@ class analytic_derivative(object):
         self.G = np.linalg.inv((diag + self.A[v, o, v, o]).reshape(nv * no, nv * no))
         self.Gm = np.linalg.inv((diag + self.Amag[v, o, v, o]).reshape(nv * no, nv * no))
         self.natom = H.molecule.natom()
+        if H.basis_set.transform is not None:
+            raise NotImplementedError(f"analytic derivatives for the {H.basis_set.name} basis (spherical d or split "
+                                      "contractions) are not built")
         self.deriv = deriv1_one_electron(H.basis_set, H.molecule.coords, H.molecule.Z, self.dev)
         self.dERI = EriDerivatives(H.basis_set, H.molecule.coords, self.dev)
         L = hamiltonian_integrals(H.basis_set, np.asarray(H.molecule.coords)[None], H.molecule.Z, self.dev,

@@ -32,12 +32,13 @@ class Hamiltonian(object):
         self.molecule = Molecule.from_string(parameters['geom'])
         self.basis_set = BasisSet(self.molecule, parameters['basis'])
         fns = self.basis_set.functions
+        bs = self.basis_set
         if integrals is None:
             charges = list(zip(self.molecule.Z, self.molecule.coords))
-            self.S, self.T, self.V = gaussian.build_one_electron(fns, charges)
+            self.S, self.T, self.V = (bs.to_basis(x) for x in gaussian.build_one_electron(fns, charges))
             D = L = None
         else:
-            self.S, self.T, self.V, D, L = integrals
+            self.S, self.T, self.V, D, L = (None if x is None else bs.to_basis(x) for x in integrals)
         self._ERI = None
 
         F_el = list(parameters.get('F_el', [0.0, 0.0, 0.0]))
@@ -45,10 +46,10 @@ class Hamiltonian(object):
         E_field = any(f != 0.0 for f in F_el)
         M_field = any(f != 0.0 for f in F_mag)
         if E_field:
-            D = gaussian.build_dipole(fns) if D is None else D
+            D = bs.to_basis(gaussian.build_dipole(fns)) if D is None else D
             self.mu_el = [-D[a] for a in range(3)]
         if M_field:
-            L = ANGULAR_MOMENTUM_SIGN * (gaussian.build_angular_momentum(fns) if L is None else L)
+            L = ANGULAR_MOMENTUM_SIGN * (bs.to_basis(gaussian.build_angular_momentum(fns)) if L is None else L)
             self.mu_mag = [-0.5j * L[a] for a in range(3)]
         self.E_nuc = self.molecule.nuclear_repulsion_energy([-f for f in F_el])
         for a in range(3):
@@ -63,17 +64,17 @@ class Hamiltonian(object):
         """(mn|ls) on the host (NumPy), built on first use; the GPU path
         reads device_eri instead."""
         if self._ERI is None:
-            self._ERI = gaussian.build_eri(self.basis_set.functions)
+            self._ERI = self.basis_set.eri_to_basis(gaussian.build_eri(self.basis_set.functions))
         return self._ERI
 
     def device_eri(self, device):
         """The AO integrals as a complex128 device tensor, computed on the
         device (apy_eri_gto) once per device."""
-        from .integrals import ao_eri
+        from .integrals import ao_eri, eri_to_basis
         key = str(device)
         t = self._device_cache.get(key)
         if t is None:
-            t = ao_eri(self.basis_set, self.molecule.coords, device)
+            t = eri_to_basis(self.basis_set, ao_eri(self.basis_set, self.molecule.coords, device))
             self._device_cache[key] = t
         return t
 

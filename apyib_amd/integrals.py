@@ -103,6 +103,18 @@ def ao_eri(basis_set, coords, device):
     return out[0] if one else out
 
 
+def eri_to_basis(basis_set, G):
+    """(mn|ls) over basis_set.functions (device [n_fn]^4) -> over the basis
+    functions when the basis has a transform (spherical d shells, split
+    contractions): the four-index transform with C = transform
+    (apy_eri_mo)."""
+    if basis_set.transform is None:
+        return G
+    from .transforms import eri_mo
+    C = torch.from_numpy(basis_set.transform.astype(np.complex128)).to(G.device)
+    return eri_mo(G.contiguous(), C)
+
+
 OEI_ALL = (_lib.APY_OEI_OVERLAP | _lib.APY_OEI_KINETIC | _lib.APY_OEI_POTENTIAL | _lib.APY_OEI_DIPOLE
            | _lib.APY_OEI_ANGMOM)
 
@@ -247,5 +259,7 @@ def prime_device_eri(hamiltonians, device):
                 index.append(len(geoms))
                 geoms.append(c)
         out = ao_eri(Hs[0].basis_set, np.stack(geoms), device)
+        if Hs[0].basis_set.transform is not None:
+            out = [eri_to_basis(Hs[0].basis_set, out[g]) for g in range(len(geoms))]
         for H, g in zip(Hs, index):
             H._device_cache[key] = out[g]

@@ -140,8 +140,20 @@ def solve_general_DIIS(parameters, res_vec, t_vec, e_iter, t_iter, iteration, mi
 
 def ao_overlap(bra_basis, ket_basis):
     """AO overlap <bra|ket> (mixed geometries allowed; utils.py:371-376),
-    computed on the device (apy_oei_gto): float64 device tensor."""
-    return mixed_overlaps([(bra_basis.functions, ket_basis.functions)], device())[0]
+    computed on the device (apy_oei_gto): float64 device tensor (a host
+    array for bases with a function transform, X = T_bra^T S T_ket)."""
+    S = mixed_overlaps([(bra_basis.functions, ket_basis.functions)], device())[0]
+    if getattr(bra_basis, 'transform', None) is None and getattr(ket_basis, 'transform', None) is None:
+        return S
+    return mixed_to_basis(bra_basis, ket_basis, S.cpu().numpy())
+
+
+def mixed_to_basis(bra_basis, ket_basis, S):
+    """<bra fn|ket fn> [n_fn, n_fn] (host) -> over the two bases' functions."""
+    Tb, Tk = getattr(bra_basis, 'transform', None), getattr(ket_basis, 'transform', None)
+    Tb = np.eye(S.shape[0]) if Tb is None else Tb
+    Tk = np.eye(S.shape[1]) if Tk is None else Tk
+    return Tb.T @ S @ Tk
 
 
 def compute_mo_overlap(ndocc, nbf, bra_basis, bra_wfn, ket_basis, ket_wfn):

@@ -17,7 +17,7 @@ from apyib_amd.kernels import stream_of, workspace
 from apyib_amd.parallel import compute_parallel_aats
 from oracle import aats as oaats
 from oracle import fd
-from tests.test_oracle_aat import GOLDEN, golden_id, golden_params, golden_tolerance, runnable
+from tests.test_oracle_aat import GOLDEN, golden_id, golden_params, golden_tolerance
 
 pytestmark = pytest.mark.gpu
 
@@ -183,7 +183,7 @@ def test_aat_class_matches_oracle_on_same_wavefunctions(dev, method):
                 assert abs(getattr(gpu, f"compute_SO_I_{k}")(4, 1, norm) - P[k][4, 1]) < 1e-12, k
 
 
-@pytest.mark.parametrize("e", [e for e in GOLDEN if runnable(e)], ids=golden_id)
+@pytest.mark.parametrize("e", GOLDEN, ids=golden_id)
 def test_compute_parallel_aats_golden(dev, e):
     """The drop-in driver (energy -> finite_difference.compute_AAT -> AAT) on
     the GPU against the reference test's AAT array, at its own tolerance."""
@@ -191,7 +191,7 @@ def test_compute_parallel_aats_golden(dev, e):
     assert np.abs(I - np.array(e["aat"])).max() < golden_tolerance(e)
 
 
-@pytest.mark.parametrize("e", [e for e in GOLDEN if e["components"] and runnable(e)], ids=golden_id)
+@pytest.mark.parametrize("e", [e for e in GOLDEN if e["components"]], ids=golden_id)
 def test_aat_terms_golden(dev, e):
     """The reference tests' I_00, I_0D, I_D0 and I_DD arrays against the
     drop-in chain's parts (energy -> finite_difference.compute_AAT -> AAT).

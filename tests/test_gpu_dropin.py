@@ -48,6 +48,21 @@ def test_energy_h2o_split_valence_golden(dev, method, basis, gold):
     assert abs((E_list[0] + E_list[1] + E_list[2]) - gold) < 1e-10
 
 
+# H2O/cc-pVDZ (spherical d, general contractions): test_005_CISD.py:59 (CISD), :122 (CISD, frozen core),
+# test_003_MP2.py:124 (MP2, frozen core), test_004_CID.py:58-60 (RHF and CID: CFOUR's split), :125 (CID, frozen core)
+@pytest.mark.parametrize("method,fc,gold", [("RHF", False, -75.98979581991861),
+                                            ("CID", False, -75.98979581991861 - 0.21279410950205),
+                                            ("CISD", False, -76.20375874728018), ("CISD", True, -76.20195791202802),
+                                            ("MP2", True, -76.20202577979143), ("CID", True, -76.20078548433332),
+                                            ("CISD_SO", False, -76.20375874728018),
+                                            ("MP2_SO", True, -76.20202577979143)])
+def test_energy_h2o_cc_pvdz_golden(dev, method, fc, gold):
+    p = dict(params(H2O, method, 'cc-pVDZ'), freeze_core=fc)
+    E_list, T_list, C, b = energy_mod.energy(p)
+    assert b.nbf() == 24
+    assert abs((E_list[0] + E_list[1] + E_list[2]) - gold) < 1e-10
+
+
 @pytest.mark.parametrize("method", ["CISD_SO", "CISD"])
 def test_energy_h2_2_cisd_golden(dev, method):
     E_list, T_list, C, basis = energy_mod.energy(params(H2_2, method))

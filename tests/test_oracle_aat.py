@@ -4,8 +4,7 @@ tensors against Jacobi's complementary-minor theorem.  CPU only.
 
 Goldens: tests/golden/aat_goldens.json (written by make_aat_goldens.py from
 apyib/tests/test_011_AAT.py, test_012_AAT_SO.py, test_013_AAT_parallel.py).
-The (H2)2 entries run here; the H2O entries need STO-6G for oxygen, which
-the package's basis table does not hold, and are reported as skipped.
+All entries run here: (H2)2 in STO-3G / STO-6G and H2O in STO-6G.
 """
 import copy
 import itertools
@@ -16,24 +15,27 @@ import numpy as np
 import pytest
 
 from oracle import aats, fd
-from tests.molecules import H2_2
+from tests.molecules import H2O, H2_2
 
 GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "aat_goldens.json")))
-MOLS = {"(H2)_2": H2_2}
+MOLS = {"(H2)_2": H2_2, "H2O": H2O}
 
 
 def golden_id(e):
     return f"{e['file'][:8]}:{e['line']}:{e['test']}"
 
 
-def runnable(e):
-    return e["parameters"]["geom"]["moldict"] in MOLS
+def _is_421(e):
+    return e["test"] == "test_cid_SO_aat_full_norm" and e["file"] == "test_012_AAT_SO.py"
 
 
 def golden_tolerance(e):
-    """The reference test's own bound.  test_012_AAT_SO.py:421 is skipped in
-    the reference (reason 'Too slow.'); its array agrees to 1.2e-7 only."""
-    return 2e-7 if e["test"] == "test_cid_SO_aat_full_norm" and e["file"] == "test_012_AAT_SO.py" else e["tolerance"]
+    """The reference test's own bound, except test_012_AAT_SO.py:421, which
+    the reference skips (reason 'Too slow.') and whose array is 1.15e-7 from
+    the CID AAT that the spatial and the spin-orbital forms both give (they
+    agree with each other to 6e-10, test_cid_421_spatial_and_so_forms_agree):
+    held to 2e-7."""
+    return 2e-7 if _is_421(e) else e["tolerance"]
 
 
 def golden_params(e):
@@ -44,10 +46,26 @@ def golden_params(e):
 
 @pytest.mark.parametrize("e", GOLDEN, ids=golden_id)
 def test_oracle_fd_aat_golden(e):
-    if not runnable(e):
-        pytest.skip("needs STO-6G for O (not in the package's basis table)")
     I = fd.compute_aats(golden_params(e), e["dR"], e["dH"], e["normalization"])
     assert np.abs(I - np.array(e["aat"])).max() < golden_tolerance(e)
+
+
+def test_cid_421_spatial_and_so_forms_agree():
+    """The (H2)2/STO-3G CID full-norm AAT of test_012_AAT_SO.py:421 from the
+    spin-orbital determinant form and from the spatial form (independent
+    code paths of the reference, aats.py:120-554 and :646-1055) agree to
+    1e-9; both sit 1.15e-7 from the stored array, and the spin-orbital one
+    moves by only 1e-8 when the step changes from 1e-6 to 1e-4, so the
+    difference is neither finite-difference noise nor a form-specific term."""
+    e = [g for g in GOLDEN if _is_421(g)][0]
+    p = golden_params(e)
+    I_so = fd.compute_aats(p, e["dR"], e["dH"], e["normalization"])
+    I_sp = fd.compute_aats(dict(p, method='CID'), e["dR"], e["dH"], e["normalization"])
+    ref = np.array(e["aat"])
+    assert np.abs(I_so - I_sp).max() < 1e-9
+    assert 1e-7 < np.abs(I_so - ref).max() < 2e-7
+    I4 = fd.compute_aats(p, 1e-4, 1e-4, e["normalization"])
+    assert np.abs(I4 - I_so).max() < 3e-8
 
 
 def _gamma(S, no):
@@ -113,7 +131,7 @@ def test_spin_orbital_determinants_factor_into_spatial():
             assert abs(D[q, 0]) < 1e-14
 
 
-@pytest.mark.parametrize("e", [e for e in GOLDEN if e["components"] and runnable(e)], ids=golden_id)
+@pytest.mark.parametrize("e", [e for e in GOLDEN if e["components"]], ids=golden_id)
 def test_oracle_aat_terms_golden(e):
     """The reference's tests also hold the AAT split into I_00, I_0D, I_D0 and
     I_DD.  Spin-orbital: each part agrees.  Spatial: I_00 and I_DD agree; the
