@@ -54,11 +54,11 @@ class SoInts(ctypes.Structure):
     _fields_ = [
         ("O", c_i64), ("V", c_i64),
         ("vvvv_p", c_vp), ("ring", c_vp), ("oooo_p", c_vp), ("ovoo_p", c_vp), ("vvvo_p", c_vp),
-        ("vovv", c_vp), ("ooov", c_vp), ("vvoo_t", c_vp), ("oovv", c_vp),
+        ("vovv_p", c_vp), ("ooov", c_vp), ("vvoo_t", c_vp), ("oovv", c_vp),
     ]
 
 
-SO_BLOCKS = ("vvvv_p", "ring", "oooo_p", "ovoo_p", "vvvo_p", "vovv", "ooov", "vvoo_t", "oovv")
+SO_BLOCKS = ("vvvv_p", "ring", "oooo_p", "ovoo_p", "vvvo_p", "vovv_p", "ooov", "vvoo_t", "oovv")
 
 SP_BLOCKS = ("vvvs", "vvva", "oooo", "w21", "w3", "vvoo", "l", "vvvo", "vooo", "s1", "w4", "w5")
 
@@ -115,6 +115,7 @@ class CisdSoArgs(ctypes.Structure):
         ("c0_re", c_dbl), ("c0_im", c_dbl),
         ("t", c_vp), ("t_new", c_vp), ("out", c_vp),
         ("ev_begin", c_vp), ("ev_end", c_vp), ("c0_dev", c_vp),
+        ("ring_f", c_vp), ("packed", c_int),
     ]
 
 
@@ -191,12 +192,14 @@ SIGNATURES = {
                                   ctypes.POINTER(SoInts), c_vp]),
     "apy_so_fock_from_mo": (c_int, [c_i64, c_vp, c_vp, c_vp]),
     "apy_cisd_so_pair_operator": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "apy_cisd_so_ring_fock": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "apy_cisd_so_workspace": (c_size, [c_i64, c_i64, c_int]),
     "apy_cisd_so_step": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_vp]),
     "apy_cisd_so_tune": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_int, c_vp]),
     "apy_cisd_so_sigma_batch_workspace": (c_size, [c_i64, c_i64, c_int, c_int]),
     "apy_cisd_so_sigma_batch": (c_int, [ctypes.POINTER(CisdSoSigmaBatchArgs), c_vp, c_size, c_vp]),
     "apy_cisd_so_energy": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_cisd_so_energy_packed": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_zdot_many": (c_int, [c_int, _P_VP, c_vp, c_i64, c_int, c_vp, c_vp, c_size, c_vp]),
     "apy_zdiffsq_segments": (c_int, [c_vp, c_vp, c_int, _P_I64, c_vp, c_vp, c_size, c_vp]),
     "apy_zcombine": (c_int, [c_int, _P_VP, _P_DBL, c_i64, c_vp, c_vp]),

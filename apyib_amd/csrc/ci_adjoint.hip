@@ -23,7 +23,7 @@
 //   oooo_p[(i<j),(k<l)] = sum_{a<b} conj(c2[i,j,a,b]) c2[k,l,a,b]            GEMM Po x Po x Pv
 //   ring[(j,b),(k,c)]   = conj(c1[j,b]) c1[k,c]
 //                         + sum_ia conj(c2[i,j,a,b]) c2[i,k,a,c]               GEMM OV x OV x (OV+1)
-//   vovv[a,j,c,b]       = 1/2 sum_i conj(c1[i,a]) c2[i,j,c,b]                 GEMM V x OV^2 x O
+//   vovv_p[a,j,(c<b)]   = sum_i conj(c1[i,a]) c2[i,j,c,b]                     GEMM V x O Pv x O
 //   ooov[k,j,i,b]       = -1/2 sum_a conj(c1[i,a]) c2[k,j,a,b]
 //   ovoo_p[k,b,(i<j)]   = -sum_a conj(c2[i,j,a,b]) c1[k,a]
 //   vvvo_p[(a<b),j,c]   = sum_i conj(c2[i,j,a,b]) c1[i,c]
@@ -229,9 +229,21 @@ __global__ __launch_bounds__(256) void tpd_small_kernel(int64_t O, int64_t V, co
   }
 }
 
+// Th[i][j][(c<b)] = c2[i][j][c][b] (the dense c2 packed over its virtual pair).
+__global__ __launch_bounds__(256) void tpd_half_pack_kernel(int64_t O, int64_t V, const double2* __restrict__ c2,
+                                                            double2* __restrict__ th) {
+  const int64_t Pv = tp_npair(V), total = O * O * Pv;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t P = t % Pv, ij = t / Pv;
+    int64_t c, b;
+    tp_pdec(P, V, c, b);
+    th[t] = c2[(ij * V + c) * V + b];
+  }
+}
+
 // The four GEMM blocks (pointers null when only sizing).
-void tpd_gemms(int64_t O, int64_t V, const double2* c, const double2* Cp, const double2* Xr, const apy_so_ints* g,
-               apy_gemm_desc d[4], int* used) {
+void tpd_gemms(int64_t O, int64_t V, const double2* c, const double2* Cp, const double2* Xr, const double2* Th,
+               const apy_so_ints* g, apy_gemm_desc d[4], int* used) {
   const int64_t OV = O * V, Po = tp_npair(O), Pv = tp_npair(V);
   const double2* c1 = c;
   const double2* c2 = c ? c + OV : nullptr;
@@ -242,14 +254,13 @@ void tpd_gemms(int64_t O, int64_t V, const double2* c, const double2* Cp, const 
     d[q++] = desc(Po, Po, Pv, APY_MK, Cp, Pv, APY_NK, Cp, Pv, g ? g->oooo_p : nullptr, Po, 1.0, 0.0, 1, 0);
   if (!g || g->ring)
     d[q++] = desc(OV, OV, OV + 1, APY_KM, Xr, OV, APY_KN, Xr, OV, g ? g->ring : nullptr, OV, 1.0, 0.0, 1, 0);
-  if (!g || g->vovv)
-    d[q++] = desc(V, O * V * V, O, APY_KM, c1, V, APY_KN, c2, O * V * V, g ? g->vovv : nullptr, O * V * V, 0.5, 0.0,
-                  1, 0);
+  if (!g || g->vovv_p)
+    d[q++] = desc(V, O * Pv, O, APY_KM, c1, V, APY_KN, Th, O * Pv, g ? g->vovv_p : nullptr, O * Pv, 1.0, 0.0, 1, 0);
   *used = q;
 }
 
 struct TpdWS {
-  size_t cp, xr, gemm, total;
+  size_t cp, xr, th, gemm, total;
 };
 
 TpdWS tpd_carve(int64_t O, int64_t V) {
@@ -257,10 +268,11 @@ TpdWS tpd_carve(int64_t O, int64_t V) {
   TpdWS w{};
   w.cp = 0;
   w.xr = ((size_t)(Po * Pv) * 16 + 255) / 256 * 256;
-  w.gemm = w.xr + ((size_t)((OV + 1) * OV) * 16 + 255) / 256 * 256;
+  w.th = w.xr + ((size_t)((OV + 1) * OV) * 16 + 255) / 256 * 256;
+  w.gemm = w.th + ((size_t)(O * O * Pv) * 16 + 255) / 256 * 256;
   apy_gemm_desc d[4];
   int used = 0;
-  tpd_gemms(O, V, nullptr, nullptr, nullptr, nullptr, d, &used);
+  tpd_gemms(O, V, nullptr, nullptr, nullptr, nullptr, nullptr, d, &used);
   size_t m = 0;
   for (int q = 0; q < used; ++q) m = std::max(m, apy_zgemm_ws_bytes(&d[q]));
   w.total = w.gemm + m;
@@ -278,7 +290,7 @@ __global__ __launch_bounds__(256) void tpd_full_kernel(int64_t O, int64_t V, apy
   const double2* oooo_p = (const double2*)g.oooo_p;
   const double2* ovoo_p = (const double2*)g.ovoo_p;
   const double2* vvvo_p = (const double2*)g.vvvo_p;
-  const double2* vovv = (const double2*)g.vovv;
+  const double2* vovv_p = (const double2*)g.vovv_p;
   const double2* ooov = (const double2*)g.ooov;
   const double2* vvoo_t = (const double2*)g.vvoo_t;
   const double2* oovv = (const double2*)g.oovv;
@@ -318,8 +330,13 @@ __global__ __launch_bounds__(256) void tpd_full_kernel(int64_t O, int64_t V, apy
         srt(a, b);
         v = vvvo_p[(tp_pidx(a, b, V) * O + j) * V + c]; break;
       }
-      case 0xB: v = vovv[((p - O) * O + q) * V * V + (r - O) * V + (s - O)]; f = 0.5; break;    // <aj||cb>
-      case 0x7: v = vovv[((q - O) * O + p) * V * V + (r - O) * V + (s - O)]; f = -0.5; break;   // <ja||cb>
+      case 0xB: case 0x7: {  // <aj||cb>, <ja||cb> -> vovv_p[a][j][(c<b)]
+        const int64_t a = (code == 0xB ? p : q) - O, j = code == 0xB ? q : p;
+        int64_t c = r - O, b = s - O;
+        f = code == 0xB ? 0.25 : -0.25;
+        srt(c, b);
+        v = vovv_p[(a * O + j) * Pv + tp_pidx(c, b, V)]; break;
+      }
       case 0x1: v = ooov[((p * O + q) * O + r) * V + (s - O)]; f = 0.5; break;                  // <kj||ib>
       case 0x2: v = ooov[((p * O + q) * O + s) * V + (r - O)]; f = -0.5; break;                 // <kj||bi>
       case 0xC: v = vvoo_t[((r * O + s) * V + (p - O)) * V + (q - O)]; break;                     // <ab||ij>
@@ -496,7 +513,12 @@ int apy_cisd_so_tpd(int64_t O, int64_t V, const void* c, const apy_so_ints* grad
   }
   apy_gemm_desc d[4];
   int used = 0;
-  tpd_gemms(O, V, cv, Cp, Xr, grad, d, &used);
+  double2* Th = (double2*)(base + w.th);
+  if (grad->vovv_p) {
+    tpd_half_pack_kernel<<<apy_grid1d(O * O * tp_npair(V)), 256, 0, st>>>(O, V, cv + OV, Th);
+    APY_LAUNCH_CHECK();
+  }
+  tpd_gemms(O, V, cv, Cp, Xr, Th, grad, d, &used);
   for (int q = 0; q < used; ++q) {
     const int s = apy_zgemm_launch(&d[q], base + w.gemm, ws_bytes - w.gemm, st);
     if (s != APY_OK) return s;
@@ -515,7 +537,7 @@ int apy_cisd_so_tpd(int64_t O, int64_t V, const void* c, const apy_so_ints* grad
 
 int apy_so_ints_grad_to_full(int64_t n, const apy_so_ints* grad, void* full, void* stream) {
   if (!grad || !full || grad->O < 1 || grad->V < 1 || grad->O + grad->V != n) return APY_E_ARG;
-  if (!grad->vvvv_p || !grad->ring || !grad->oooo_p || !grad->ovoo_p || !grad->vvvo_p || !grad->vovv ||
+  if (!grad->vvvv_p || !grad->ring || !grad->oooo_p || !grad->ovoo_p || !grad->vvvo_p || !grad->vovv_p ||
       !grad->ooov || !grad->vvoo_t || !grad->oovv)
     return APY_E_ARG;
   const int64_t total = n * n * n * n;

@@ -52,5 +52,11 @@ inline unsigned apy_grid1d(int64_t n, int64_t cap = 65536) {
 int apy_zgemm_launch(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, hipStream_t s,
                      hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 size_t apy_zgemm_ws_bytes(const apy_gemm_desc* d);
+// Workspace of the plan apy_zgemm_launch would run for d now (cached or model plan).
+size_t apy_zgemm_plan_ws_bytes(const apy_gemm_desc* d);
+// apy_cisd_so_step on a caller's workspace; exact = 1: the GEMM region sized
+// for the plans that run now (apy_cisd_so_ws_exact bytes suffice).
+int apy_cisd_so_step_ws(const apy_cisd_so_args* A, void* workspace, size_t ws_bytes, void* stream, int exact);
+size_t apy_cisd_so_ws_exact(const apy_cisd_so_args* A);
 int apy_zgemm_tune_impl(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, int reps, hipStream_t s,
                         int* best_cfg, int* best_split, double* best_ms);

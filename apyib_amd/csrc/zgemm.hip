@@ -597,6 +597,11 @@ size_t apy_zgemm_ws_bytes(const apy_gemm_desc* d) {
   return (size_t)smax * (size_t)d->batch * (size_t)d->m * (size_t)d->n * sizeof(double2);
 }
 
+size_t apy_zgemm_plan_ws_bytes(const apy_gemm_desc* d) {
+  if (!d || d->m <= 0 || d->n <= 0 || d->k <= 0 || d->batch <= 0) return 0;
+  return ws_need(d, make_plan(d));
+}
+
 int apy_zgemm_launch(const apy_gemm_desc* d, void* workspace, size_t ws_bytes, hipStream_t s, hipEvent_t ev0,
                      hipEvent_t ev1) {
   int st = check_desc(d);

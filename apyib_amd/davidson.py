@@ -43,8 +43,8 @@ def _vp(t):
 
 class CISDDavidson:
     """packed=True keeps the subspace as packed vectors [c1 | c2 over (i<j, a<b) | c0]
-    (apy_ci_pack / apy_ci_unpack around each sigma): the CI metric is then the
-    Euclidean product and every vector pass moves a quarter of the bytes.
+    whose sigma apy_cisd_so_step forms directly in that layout: the CI metric is
+    then the Euclidean product and every vector pass moves a quarter of the bytes.
     packed=False keeps dense vectors (the layout MultiDavidson's batched sigma
     reads)."""
 
@@ -64,9 +64,6 @@ class CISDDavidson:
         self.B = torch.zeros((self.max_space, self.Lv), dtype=CDT, device=self.dev)
         self.S = torch.zeros((self.max_space, self.Lv), dtype=CDT, device=self.dev)
         self.work = torch.zeros(self.Lv, dtype=CDT, device=self.dev)
-        if self.packed:                             # dense images for the sigma
-            self.bd = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
-            self.sd = torch.zeros(self.L + 1, dtype=CDT, device=self.dev)
         self.part = torch.zeros(max(_lib.APY_MAXVEC, self.max_space + 1), dtype=CDT, device=self.dev)
         self.c = None
         self.E = None
@@ -79,14 +76,9 @@ class CISDDavidson:
         device memory."""
         self.nsigma += 1
         b, s = self.B[q], self.S[q]
-        st = stream_of(self.dev)
-        if self.packed:
-            _lib.check(self.lib.apy_ci_unpack(self.O, self.V, _vp(b), _vp(self.bd), st), "apy_ci_unpack")
-            b, s = self.bd, self.sd
-        self.soci.sigma(b[:self.L], s[:self.L], c0_dev=b[self.L:].data_ptr())
-        self.soci.energy(b[:self.L], out=s[self.L:])
-        if self.packed:
-            _lib.check(self.lib.apy_ci_pack(self.O, self.V, _vp(self.sd), _vp(self.S[q]), st), "apy_ci_pack")
+        n = self.Lv - 1
+        self.soci.sigma(b[:n], s[:n], c0_dev=b[n:].data_ptr(), packed=self.packed)
+        self.soci.energy(b[:n], out=s[n:], packed=self.packed)
 
     def _dots_dev(self, xs, y, out):
         """CI-metric <x_m|y> into the device array out (len(xs) <= APY_MAXVEC), no host copy:
@@ -361,8 +353,8 @@ class MultiDavidson:
     def __init__(self, ints, focks, eps, max_space=12):
         from .so_ci import SOCI
         self.ints = ints
-        self.dav = [CISDDavidson(SOCI(ints, F, eps, singles=True, fov_coupling=True), max_space, packed=False)
-                    for F in focks]
+        self.dav = [CISDDavidson(SOCI(ints, F, eps, singles=True, fov_coupling=True, fold_fvv=False), max_space,
+                                 packed=False) for F in focks]
         self.lib = self.dav[0].lib
         self.dev = self.dav[0].dev
         self.O, self.V, self.L = self.dav[0].O, self.dav[0].V, self.dav[0].L

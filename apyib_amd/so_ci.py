@@ -118,9 +118,12 @@ class SOCI:
     ints: SOIntegrals with at least the blocks the method reads
     fov_coupling: add the P(ij)P(ab) f_bj t_ia term the reference omits
         (Hermitian operator; identical results for canonical HF orbitals).
+    fold_fvv: keep a copy of the ring block with the F_vv terms folded in
+        (apy_cisd_so_ring_fock, O^2 V^2 elements) so each step runs one GEMM
+        fewer; off for instances whose sigma is only formed in batches.
     """
 
-    def __init__(self, ints, fock, eps, singles=True, fov_coupling=False):
+    def __init__(self, ints, fock, eps, singles=True, fov_coupling=False, fold_fvv=True):
         self.lib = _lib.load()
         self.ints = ints
         self.O, self.V = ints.O, ints.V
@@ -137,6 +140,12 @@ class SOCI:
         oooo = ints.blocks.get("oooo_p")
         _lib.check(self.lib.apy_cisd_so_pair_operator(O, V, _vp(self.fock), _vp(oooo), _vp(self.hoo),
                                                       stream_of(self.device)), "apy_cisd_so_pair_operator")
+        self.ring_f = None
+        if fold_fvv and "ring" in ints.blocks:
+            self.ring_f = torch.empty(O * V * O * V, dtype=CDT, device=self.device)
+            _lib.check(self.lib.apy_cisd_so_ring_fock(O, V, _vp(self.fock), _vp(ints["ring"]), _vp(self.ring_f),
+                                                      stream_of(self.device)), "apy_cisd_so_ring_fock")
+        self.Lp = O * V + npair(O) * npair(V)
         self.ws_bytes = int(self.lib.apy_cisd_so_workspace(O, V, int(self.singles)))
         self.red_bytes = int(self.lib.apy_reduce_workspace(self.L, _lib.APY_MAXVEC))
         self.energy_dev = torch.zeros(1, dtype=CDT, device=self.device)
@@ -153,16 +162,19 @@ class SOCI:
         O, V = self.O, self.V
         return t[:self.OV].view(O, V), t[self.OV:].view(O, O, V, V)
 
-    def step(self, t, t_new, out, energy=None, mode=_lib.APY_MODE_RESIDUAL, c0=1.0, ev_dominant=None, c0_dev=None):
+    def step(self, t, t_new, out, energy=None, mode=_lib.APY_MODE_RESIDUAL, c0=1.0, ev_dominant=None, c0_dev=None,
+             packed=False):
         """One call of apy_cisd_so_step (residual update or sigma product).
-        c0_dev: optional device address of the sigma-mode driver weight."""
+        c0_dev: optional device address of the sigma-mode driver weight;
+        packed: t and out are packed vectors [t1 | t2 over (i<j, a<b)]."""
         c0 = complex(c0)
         ev = ev_dominant or self.ev_dominant or (None, None)
         args = _lib.CisdSoArgs(
             self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(), ctypes.pointer(self.ints.struct),
             self.hoo.data_ptr(), int(self.singles), int(mode), int(self.fov),
             (energy if energy is not None else self.energy_dev).data_ptr(), c0.real, c0.imag,
-            t.data_ptr(), t_new.data_ptr() if t_new is not None else None, out.data_ptr(), ev[0], ev[1], c0_dev)
+            t.data_ptr(), t_new.data_ptr() if t_new is not None else None, out.data_ptr(), ev[0], ev[1], c0_dev,
+            self.ring_f.data_ptr() if self.ring_f is not None else None, int(bool(packed)))
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_so_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
                    "apy_cisd_so_step")
@@ -174,22 +186,24 @@ class SOCI:
         args = _lib.CisdSoArgs(
             self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(), ctypes.pointer(self.ints.struct),
             self.hoo.data_ptr(), int(self.singles), _lib.APY_MODE_SIGMA, int(self.fov),
-            self.energy_dev.data_ptr(), 1.0, 0.0, t.data_ptr(), None, t.data_ptr(), None, None)
+            self.energy_dev.data_ptr(), 1.0, 0.0, t.data_ptr(), None, t.data_ptr(), None, None, None,
+            self.ring_f.data_ptr() if self.ring_f is not None else None, 0)
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_so_tune(ctypes.byref(args), _vp(ws), self.ws_bytes, int(reps),
                                              stream_of(self.device)), "apy_cisd_so_tune")
 
-    def sigma(self, x, out, c0=1.0, c0_dev=None):
+    def sigma(self, x, out, c0=1.0, c0_dev=None, packed=False):
         """out = H_N (c0, x): the CISD sigma with reference weight c0 (or the
-        device complex scalar at address c0_dev)."""
-        self.step(x, None, out, mode=_lib.APY_MODE_SIGMA, c0=c0, c0_dev=c0_dev)
+        device complex scalar at address c0_dev); packed vectors with
+        packed=True."""
+        self.step(x, None, out, mode=_lib.APY_MODE_SIGMA, c0=c0, c0_dev=c0_dev, packed=packed)
 
-    def energy(self, t, out=None):
+    def energy(self, t, out=None, packed=False):
         out = self.energy_dev if out is None else out
         ws = workspace(self.red_bytes, self.device)
-        _lib.check(self.lib.apy_cisd_so_energy(self.O, self.V, int(self.singles), _vp(self.fock),
-                                               _vp(self.ints["oovv"]), _vp(t), _vp(out), _vp(ws), self.red_bytes,
-                                               stream_of(self.device)), "apy_cisd_so_energy")
+        fn = self.lib.apy_cisd_so_energy_packed if packed else self.lib.apy_cisd_so_energy
+        _lib.check(fn(self.O, self.V, int(self.singles), _vp(self.fock), _vp(self.ints["oovv"]), _vp(t), _vp(out),
+                      _vp(ws), self.red_bytes, stream_of(self.device)), "apy_cisd_so_energy")
         return out
 
     def dots(self, xs, y, out, conj=True):
@@ -441,25 +455,28 @@ class SOCI:
         return E, t1, t2, it
 
 
-def step_flops(O, V, singles=True):
-    """Real flops executed by one apy_cisd_so_step (8 per complex
-    multiply-add), per GEMM of the step; 'ppl' is the dominant kernel."""
+def step_flops(O, V, singles=True, fold_fvv=True):
+    """Real flops executed by the GEMMs of one apy_cisd_so_step (8 per
+    complex multiply-add), per GEMM; 'ppl' is the dominant kernel.  With
+    the F_vv terms folded into the ring block (fold_fvv) they cost no
+    separate product.  The T1 -> T2 term through <kb||ij> (2 O flops per
+    packed doubles element) is computed in the epilogue and not counted."""
     Po, Pv, OV = npair(O), npair(V), O * V
     f = {
         'ppl': 8.0 * Po * Pv * Pv,
         'hhl': 8.0 * Po * Pv * Po,
         'ring': 8.0 * OV * OV * OV,
-        'fvv': 8.0 * O * O * V * V * V,
     }
+    if not fold_fvv:
+        f['fvv'] = 8.0 * O * O * V * V * V
     if singles:
         f.update({
-            'u': 8.0 * V * V * Po * O,
             'q': 8.0 * Pv * O * O * V,
             's1_fvv': 8.0 * O * V * V,
             's1_foo': 8.0 * O * V * O,
             's1_ring': 8.0 * OV * OV,
             's1_fov': 8.0 * O * OV * V,
-            's1_vovv': 8.0 * O * V * OV * V,
+            's1_vovv': 8.0 * O * V * O * Pv,
             's1_ooov': 8.0 * O * O * O * V * V,
         })
     return f

@@ -74,6 +74,13 @@ def eri_block(n, seed, ps, qs, rs, ss, std=0.05, tag=TAG_ERI):
     return 0.125 * (X1 + np.conj(X2))
 
 
+def eri_elements(n, seed, p, q, r, s, std=0.05, tag=TAG_ERI):
+    """A[p_k,q_k,r_k,s_k] = <pq||rs> for matching index arrays (scattered
+    elements, no grid)."""
+    p, q, r, s = (np.asarray(x, dtype=np.uint64) for x in (p, q, r, s))
+    return 0.125 * (_xsum(seed, tag, n, p, q, r, s, std) + np.conj(_xsum(seed, tag, n, r, s, p, q, std)))
+
+
 def _xsum(seed, tag, n, p, q, r, s, std):
     """(G(pqrs) + G(qpsr)) - (G(qprs) + G(pqsr)): grouped so that pair
     antisymmetry, zero diagonals and Hermiticity hold exactly in floating

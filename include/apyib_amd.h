@@ -108,14 +108,14 @@ typedef struct apy_so_ints {
   void* oooo_p;  /* [Po][Po]      <kl||ij>, (i<j) rows, (k<l) cols  (ci_wfn.py:328)      */
   void* ovoo_p;  /* [O][V][Po]    <kb||ij>, i<j                     (ci_wfn.py:320-321)  */
   void* vvvo_p;  /* [Pv][O][V]    <ab||cj> stored at [(a<b)][j][c]  (ci_wfn.py:322-323)  */
-  void* vovv;    /* [V][O][V][V]  <aj||cb>                          (ci_wfn.py:314)      */
+  void* vovv_p;  /* [V][O][Pv]    <aj||cb> stored at [a][j][(c<b)]  (ci_wfn.py:314)      */
   void* ooov;    /* [O][O][O][V]  <kj||ib>                          (ci_wfn.py:315)      */
   void* vvoo_t;  /* [O][O][V][V]  <ab||ij> stored at [i][j][a][b]   (ci_wfn.py:319)      */
   void* oovv;    /* [O][O][V][V]  <ij||ab>                          (ci_wfn.py:355)      */
 } apy_so_ints;
 
 /* Number of complex elements of each block, in the order of the struct
- * (vvvv_p, ring, oooo_p, ovoo_p, vvvo_p, vovv, ooov, vvoo_t, oovv). Host. */
+ * (vvvv_p, ring, oooo_p, ovoo_p, vvvo_p, vovv_p, ooov, vvoo_t, oovv). Host. */
 int apy_so_ints_sizes(int64_t O, int64_t V, int64_t sizes_out[9]);
 
 /* Blocks from a full antisymmetrized tensor A[n][n][n][n] = <pq||rs>. */
@@ -280,12 +280,24 @@ typedef struct apy_cisd_so_args {
   const void* c0_dev;      /* sigma mode, optional: the driver weight read
                               from this device complex scalar instead of
                               c0_re/c0_im (no host read before the call)  */
+  const void* ring_f;      /* optional [OV][OV] from apy_cisd_so_ring_fock:
+                              the ring block with the F_vv term folded in,
+                              which removes one GEMM from the step         */
+  int packed;              /* 0: t, t_new, out are [t1 | t2 dense] (length
+                              O*V + O^2*V^2); 1: [t1 | t2 over (i<j, a<b)]
+                              (length O*V + Po*Pv), sigma mode only        */
 } apy_cisd_so_args;
 
 /* Pair-space occupied operator: <kl||ij> plus the F_oo terms of the doubles
  * residual (ci_wfn.py:326-328), [(i<j)][(k<l)]; built once per solve. */
 int apy_cisd_so_pair_operator(int64_t O, int64_t V, const void* fock, const void* oooo_p, void* hoo,
                               void* stream);
+/* ring_f[(j,b)][(k,c)] = <kb||cj> + 1/2 delta_jk F[O+b][O+c]: the ring block
+ * with the F_vv terms of the doubles residual (ci_wfn.py:324-325) folded in.
+ * With it the ring product X = T' ring_f^T carries 1/2 sum_c t_ijac F_bc in
+ * X[(i,a),(j,b)], whose four P(ij)P(ab) images sum to P(ab) sum_c t_ijac F_bc.
+ * Built once per Fock matrix; ring_f must not alias ring. */
+int apy_cisd_so_ring_fock(int64_t O, int64_t V, const void* fock, const void* ring, void* ring_f, void* stream);
 size_t apy_cisd_so_workspace(int64_t O, int64_t V, int singles);
 int apy_cisd_so_step(const apy_cisd_so_args* a, void* workspace, size_t ws_bytes, void* stream);
 /* Autotune every GEMM of a step for this (O, V) on the operands in `a`
@@ -319,6 +331,10 @@ int apy_cisd_so_sigma_batch(const apy_cisd_so_sigma_batch_args* a, void* workspa
  * singles = 0), written to the device complex scalar `energy`. */
 int apy_cisd_so_energy(int64_t O, int64_t V, int singles, const void* fock, const void* oovv, const void* t,
                        void* energy, void* workspace, size_t ws_bytes, void* stream);
+/* The same energy of a packed vector [t1 | t2 over (i<j, a<b)]: the four
+ * antisymmetric images of each packed pair give 1/4 sum = sum over i<j, a<b. */
+int apy_cisd_so_energy_packed(int64_t O, int64_t V, int singles, const void* fock, const void* oovv,
+                              const void* t, void* energy, void* workspace, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * AO -> MO transforms, Fock build, overlaps, phase (utils.py:217-279, 370-446)
@@ -461,7 +477,7 @@ int apy_cisd_so_opd(int64_t O, int64_t V, const void* c, void* D, void* workspac
  *   dE = sum_blocks sum_entries grad_blk * d(ints_blk)
  * for every perturbation antisymmetric in (p,q) and (r,s); with the OPD,
  * E = sum F D + sum_blocks ints_blk . grad_blk.  vvvv_p, oooo_p, ring and
- * vovv are GEMMs on the matrix cores.  O, V >= 2. */
+ * vovv_p are GEMMs on the matrix cores.  O, V >= 2. */
 size_t apy_cisd_so_tpd_workspace(int64_t O, int64_t V);
 int apy_cisd_so_tpd(int64_t O, int64_t V, const void* c, const apy_so_ints* grad, void* workspace, size_t ws_bytes,
                     void* stream);

@@ -147,9 +147,9 @@ def _pairs(N):
 
 def tpd_blocks(G, O):
     """The density in the packed block layouts of the device integrals
-    (vvvv_p, ring, oooo_p, ovoo_p, vvvo_p, vovv, ooov, vvoo_t, oovv): each
+    (vvvv_p, ring, oooo_p, ovoo_p, vvvo_p, vovv_p, ooov, vvoo_t, oovv): each
     stored entry carries the sum over the full-tensor positions it stands
-    for (4 for the packed blocks and ring, 2 for vovv and ooov, 1 for
+    for (4 for the packed blocks and ring, 2 for ooov, 1 for
     vvoo_t and oovv), so that dE = sum_blocks sum Gblk * dblk."""
     n = G.shape[0]
     V = n - O
@@ -167,7 +167,8 @@ def tpd_blocks(G, O):
     out['ovoo_p'] = 4 * G[o, v][:, :, oi[0], oi[1]]
     # vvvo_p[(a<b)][j][c] = <ab||cj>
     out['vvvo_p'] = 4 * G[vi[0], vi[1]][:, v, o].transpose(0, 2, 1)
-    out['vovv'] = 2 * G[v, o, v, v]
+    # vovv_p[a][j][(c<b)] = <aj||cb>
+    out['vovv_p'] = 4 * G[v, o][:, :, vi[0], vi[1]]
     out['ooov'] = 2 * G[o, o, o, v]
     out['vvoo_t'] = G[v, v, o, o].transpose(2, 3, 0, 1)
     out['oovv'] = G[o, o, v, v].copy()
@@ -188,7 +189,7 @@ def integral_blocks(A, O):
         'oooo_p': A[oi[0][None, :], oi[1][None, :], oi[0][:, None], oi[1][:, None]],
         'ovoo_p': A[o, v][:, :, oi[0], oi[1]],
         'vvvo_p': A[vi[0], vi[1]][:, v, o].transpose(0, 2, 1),
-        'vovv': A[v, o, v, v].copy(),
+        'vovv_p': A[v, o][:, :, vi[0], vi[1]],
         'ooov': A[o, o, o, v].copy(),
         'vvoo_t': A[v, v, o, o].transpose(2, 3, 0, 1),
         'oovv': A[o, o, v, v].copy(),

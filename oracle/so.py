@@ -269,3 +269,59 @@ def cisd_so_sigma_blocks(F, W, t1, t2, O):
     s2 += contract('kacj,ikcb->ijab', W['ovvo'], t2)
     s2 += contract('kaci,kjcb->ijab', W['ovvo'], t2)
     return s1, s2
+
+
+def blocks_from_packed(host, O, V):
+    """Dense antisymmetrized blocks W (keys of BLOCK_NAMES) from the packed
+    device layouts of apy_so_ints copied to the host (flat arrays keyed by
+    vvvv_p, ring, oooo_p, ovoo_p, vvvo_p, vovv_p, ooov, vvoo_t, oovv).  Each
+    packed pair (p<q) fills the (p,q) entry and its negated (q,p) image; the
+    diagonals stay zero.  Used where the full n^4 tensor is too large to
+    build (O=20, V=160: the dense vvvv block alone is 10.5 GB).
+
+    Pinned by tests/test_oracle_so.py::test_blocks_from_packed_inverts_layouts
+    (the host image of the device layouts of a small full tensor maps back
+    to blocks_from_eri of that tensor, exactly)."""
+    Po, Pv = O * (O - 1) // 2, V * (V - 1) // 2
+    io, jo = np.triu_indices(O, 1)
+    ia, ib = np.triu_indices(V, 1)
+    W = {}
+    W['oovv'] = host['oovv'].reshape(O, O, V, V)
+    W['vvoo'] = host['vvoo_t'].reshape(O, O, V, V).transpose(2, 3, 0, 1).copy()
+    W['ovvo'] = host['ring'].reshape(O, V, O, V).transpose(2, 1, 3, 0).copy()
+    Pw = host['vovv_p'].reshape(V, O, Pv)                # [a][j][(c<b)] = <aj||cb>
+    vovv = np.zeros((V, O, V, V), dtype=np.complex128)
+    vovv[:, :, ia, ib] = Pw
+    vovv[:, :, ib, ia] = -Pw
+    W['vovv'] = vovv
+    W['ooov'] = host['ooov'].reshape(O, O, O, V)
+    P = host['oooo_p'].reshape(Po, Po)                  # [(i<j)][(k<l)] = <kl||ij>
+    X = np.zeros((O, O, O, O), dtype=np.complex128)      # X[i,j,k,l] = <kl||ij>
+    Xv = X.reshape(O * O, O * O)
+    r1, r2 = io * O + jo, jo * O + io
+    Xv[np.ix_(r1, r1)] = P
+    Xv[np.ix_(r1, r2)] = -P
+    Xv[np.ix_(r2, r1)] = -P
+    Xv[np.ix_(r2, r2)] = P
+    W['oooo'] = X.transpose(2, 3, 0, 1).copy()
+    Pk = host['ovoo_p'].reshape(O, V, Po)                # [k][b][(i<j)] = <kb||ij>
+    ovoo = np.zeros((O, V, O, O), dtype=np.complex128)
+    ovoo[:, :, io, jo] = Pk
+    ovoo[:, :, jo, io] = -Pk
+    W['ovoo'] = ovoo
+    W['vooo'] = -ovoo.transpose(1, 0, 2, 3).copy()
+    Pq = host['vvvo_p'].reshape(Pv, O, V)                # [(a<b)][j][c] = <ab||cj>
+    vvvo = np.zeros((V, V, V, O), dtype=np.complex128)
+    vvvo[ia, ib] = Pq.transpose(0, 2, 1)
+    vvvo[ib, ia] = -Pq.transpose(0, 2, 1)
+    W['vvvo'] = vvvo
+    W['vvov'] = -vvvo.transpose(0, 1, 3, 2).copy()
+    Pvv = host['vvvv_p'].reshape(Pv, Pv)                 # [(a<b)][(c<d)] = <ab||cd>
+    vvvv = np.zeros((V * V, V * V), dtype=np.complex128)
+    ra, rb = ia * V + ib, ib * V + ia
+    vvvv[np.ix_(ra, ra)] = Pvv
+    vvvv[np.ix_(ra, rb)] = -Pvv
+    vvvv[np.ix_(rb, ra)] = -Pvv
+    vvvv[np.ix_(rb, rb)] = Pvv
+    W['vvvv'] = vvvv.reshape(V, V, V, V)
+    return W

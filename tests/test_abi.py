@@ -46,7 +46,7 @@ def test_load_binds_signatures():
     assert lib.apy_so_ints_sizes(20, 160, sizes) == 0
     Po, Pv = 190, 12720
     assert list(sizes) == [Pv * Pv, 20 * 160 * 20 * 160, Po * Po, 20 * 160 * Po, Pv * 20 * 160,
-                           160 * 20 * 160 * 160, 20 * 20 * 20 * 160, 20 * 20 * 160 * 160, 20 * 20 * 160 * 160]
+                           160 * 20 * Pv, 20 * 20 * 20 * 160, 20 * 20 * 160 * 160, 20 * 20 * 160 * 160]
     assert lib.apy_so_ints_sizes(-1, 3, sizes) == _lib.APY_E_ARG
 
 

@@ -149,3 +149,89 @@ def test_max_iterations_reached_returns_last_iterate(dev, capsys):
     assert it == 2
     assert abs(E - E_ref) < 1e-12
     assert "Not converged." in capsys.readouterr().out
+
+
+# virtual spaces wider than one 32-wide epilogue tile (several tile pairs, a
+# ragged last tile) and the narrowest case
+WIDE = [(4, 40, 3), (3, 45, 7), (2, 33, 2), (2, 2, 4)]
+
+
+def _packed(t1, t2, O, V, extra=()):
+    io, jo = np.triu_indices(O, 1)
+    ia, ib = np.triu_indices(V, 1)
+    return np.concatenate([t1.ravel(), t2[io, jo][:, ia, ib].ravel(), np.asarray(extra, dtype=complex)])
+
+
+@pytest.mark.parametrize("O,V,seed", WIDE)
+@pytest.mark.parametrize("fold", [True, False])
+def test_residual_step_wide_virtual_space(dev, O, V, seed, fold):
+    """Residual step (reference form) with several epilogue tiles, with the
+    F_vv terms folded into the ring block and as a separate product."""
+    F, A, eps, ints = _system(O, V, seed, dev)
+    t1, t2 = rand_amplitudes(O, V, seed + 1)
+    E = 0.021 + 0.004j
+    solver = SOCI(ints, _dev(F, dev), eps, singles=True, fold_fvv=fold)
+    assert (solver.ring_f is not None) == fold
+    t = _dev(np.concatenate([t1.ravel(), t2.ravel()]), dev)
+    tn, r = solver.new_vector(), solver.new_vector()
+    solver.step(t, tn, r, energy=torch.tensor([E], dtype=torch.complex128, device=dev))
+    r1, r2 = so.cisd_so_residual(F, A / 2, t1, t2, E, O)
+    D1, D2 = so.denominators(eps[:O], eps[O:])
+    g1, g2 = (x.cpu().numpy() for x in solver.split(r))
+    n1, n2 = (x.cpu().numpy() for x in solver.split(tn))
+    scale = max(1.0, np.abs(r2).max())
+    assert np.abs(g1 - r1).max() < 1e-13 * scale and np.abs(g2 - r2).max() < 1e-13 * scale
+    assert np.abs(n1 - (t1 + r1 / D1)).max() < 1e-12 and np.abs(n2 - (t2 + r2 / D2)).max() < 1e-12
+
+
+@pytest.mark.parametrize("O,V,seed", WIDE)
+def test_residual_step_in_place_update(dev, O, V, seed):
+    """t_new may alias t (the header allows it): the same update as into a
+    separate vector."""
+    F, A, eps, ints = _system(O, V, seed, dev)
+    t1, t2 = rand_amplitudes(O, V, seed + 5)
+    solver = SOCI(ints, _dev(F, dev), eps, singles=True)
+    Ed = torch.tensor([0.01 - 0.003j], dtype=torch.complex128, device=dev)
+    t = _dev(np.concatenate([t1.ravel(), t2.ravel()]), dev)
+    t_sep, r1, r2 = solver.new_vector(), solver.new_vector(), solver.new_vector()
+    solver.step(t, t_sep, r1, energy=Ed)
+    solver.step(t, t, r2, energy=Ed)
+    assert torch.equal(t, t_sep) and torch.equal(r1, r2)
+
+
+@pytest.mark.parametrize("O,V,seed", WIDE + [(6, 8, 5)])
+@pytest.mark.parametrize("singles", [True, False])
+def test_packed_sigma_matches_dense_and_oracle(dev, O, V, seed, singles):
+    """sigma mode on packed vectors [t1 | t2 over (i<j, a<b)] (the Davidson
+    layout) equals the packed image of the dense sigma and the oracle's
+    Hermitian CISD matrix product (singles) or the reference's CID doubles
+    sigma; the packed energy equals the dense one."""
+    F, A, eps, ints = _system(O, V, seed, dev)
+    t1, t2 = rand_amplitudes(O, V, seed + 2)
+    if not singles:
+        t1 = np.zeros_like(t1)
+    c0 = 0.9 + 0.15j
+    solver = SOCI(ints, _dev(F, dev), eps, singles=singles, fov_coupling=singles)
+    dense_in = _dev(np.concatenate([t1.ravel(), t2.ravel()]), dev)
+    dense_out = solver.new_vector()
+    solver.sigma(dense_in, dense_out, c0=c0)
+    p_in = _dev(_packed(t1, t2, O, V), dev)
+    p_out = torch.zeros_like(p_in)
+    solver.sigma(p_in, p_out, c0=c0, packed=True)
+    d1, d2 = (x.cpu().numpy() for x in solver.split(dense_out))
+    want_p = _packed(d1, d2, O, V)
+    assert np.array_equal(p_out.cpu().numpy()[:O * V], want_p[:O * V]) or not singles
+    scale = max(1.0, np.abs(want_p).max())
+    assert np.abs(p_out.cpu().numpy() - want_p).max() < 1e-14 * scale
+    W = so.blocks_from_eri(A / 2, O)
+    if singles:
+        from oracle import davidson
+        s0, s1, s2 = davidson.sigma(F, W, c0, t1, t2, O)
+    else:
+        s1 = np.zeros_like(t1)
+        s2 = so.cid_so_residual(F, A / 2, t2, 0.0, O) + (c0 - 1.0) * W['vvoo'].transpose(2, 3, 0, 1)
+    assert np.abs(p_out.cpu().numpy() - _packed(s1, s2, O, V)).max() < 1e-13 * scale
+    e_dense = complex(solver.energy(dense_in, out=torch.zeros(1, dtype=torch.complex128, device=dev)).cpu()[0])
+    e_pack = complex(solver.energy(p_in, out=torch.zeros(1, dtype=torch.complex128, device=dev), packed=True)
+                     .cpu()[0])
+    assert abs(e_dense - e_pack) < 1e-14 * max(1.0, abs(e_dense))

@@ -176,7 +176,7 @@ def test_field_set_batched_fd_matches_adjoint(dev):
         assert abs(fd - an.real) < 1e-7 * scale
 
 
-BLOCKS = ("vvvv_p", "ring", "oooo_p", "ovoo_p", "vvvo_p", "vovv", "ooov", "vvoo_t", "oovv")
+BLOCKS = ("vvvv_p", "ring", "oooo_p", "ovoo_p", "vvvo_p", "vovv_p", "ooov", "vvoo_t", "oovv")
 
 
 def _gpu_vector(c, dev):

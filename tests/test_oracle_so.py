@@ -116,6 +116,33 @@ def test_block_form_equals_full_form():
     assert np.abs(b1 - r1).max() < 1e-15 and np.abs(b2 - r2).max() < 1e-15
 
 
+@pytest.mark.parametrize("O,V,seed", [(4, 6, 8), (3, 7, 2), (2, 2, 5)])
+def test_blocks_from_packed_inverts_layouts(O, V, seed):
+    """The host image of the packed device layouts of a full tensor expands
+    back to the dense blocks of that tensor, bit for bit."""
+    from tests.gpu_layouts import expected_blocks
+    _, A, _ = synthetic.system(O, V, seed)
+    want = so.blocks_from_eri(A / 2, O)
+    got = so.blocks_from_packed(expected_blocks(A, O), O, V)
+    assert set(got) == set(want)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+
+
+@pytest.mark.parametrize("O,V,seed", [(4, 6, 3), (3, 5, 1)])
+def test_block_element_indices_match_layouts(O, V, seed):
+    """The flat-position -> (p,q,r,s) maps used to sample full-size device
+    blocks address the same entries as the layouts, and the scattered
+    generator gives the same values as the full tensor."""
+    from tests.gpu_layouts import block_element_indices, expected_blocks
+    n = O + V
+    A = synthetic.eri_full(n, seed)
+    for name, arr in expected_blocks(A, O).items():
+        p, q, r, s = block_element_indices(name, np.arange(arr.size), O, V)
+        assert np.array_equal(A[p, q, r, s], arr), name
+        assert np.array_equal(synthetic.eri_elements(n, seed, p, q, r, s), arr), name
+
+
 def test_convergence_compares_complex_rms_by_real_part():
     """The reference compares sqrt(sum dt^2), a complex number, to a float;
     NumPy orders complex values by real part first (ci_wfn.py:371-372)."""
