@@ -151,6 +151,7 @@ class Mp2NcInstance(ctypes.Structure):
     _fields_ = [
         ("O", c_i64), ("V", c_i64),
         ("fock", c_vp), ("eps", c_vp), ("drv", c_vp), ("x", c_vp), ("x_new", c_vp), ("w", c_vp),
+        ("transpose_f", c_int), ("drv_scale", c_dbl),
     ]
 
 
@@ -238,9 +239,12 @@ SIGNATURES = {
     "apy_mo_overlap": (c_int, [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_size,
                                c_vp]),
     "apy_phase_align": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "apy_t2_packed_length": (c_i64, [c_i64, c_i64]),
+    "apy_t2_pack": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp]),
+    "apy_t2_unpack": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp]),
     "apy_mp2_nc_batched_workspace": (c_size, [c_int, ctypes.POINTER(Mp2NcInstance)]),
     "apy_mp2_nc_batched_setup": (c_int, [c_int, ctypes.POINTER(Mp2NcInstance), c_vp, c_size, _P_I64, c_vp]),
-    "apy_mp2_nc_batched_step": (c_int, [c_int, c_i64, c_int, c_dbl, c_int, c_vp, c_vp, c_vp]),
+    "apy_mp2_nc_batched_step": (c_int, [c_int, c_i64, c_int, c_vp, c_vp, c_vp]),
     "apy_mp2_nc_batched_energy": (c_int, [c_int, c_i64, c_int, c_dbl, c_vp, c_vp, c_vp]),
     "apy_ci_precondition": (c_int, [c_i64, c_i64, c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "apy_cisd_so_opd_workspace": (c_size, [c_i64, c_i64]),

@@ -6,9 +6,10 @@ energy()/phase_corrected_energy()).  Here the instances of a set are
 solved together: `MP2NCBatch` runs the orbital-invariant spin-orbital MP2
 iteration (ci_wfn.py:210-212 driven by <ab||ij>; solve_MP2_SO at diagonal
 F, mp2_wfn.py:64-87) for all instances, of possibly different (O, V), in
-one launch per iteration (apy_mp2_nc_batched_step), with chunks of
-iterations captured in a HIP graph so the host checks convergence once per
-chunk.  `plain_solve` runs the same kernels one instance at a time with a
+one fused launch per iteration (apy_mp2_nc_batched_step) on amplitudes
+stored packed over (i<j, a<b), optionally with chunks of iterations
+captured in a HIP graph, the host checking convergence once per chunk.
+`plain_solve` runs the same kernels one instance at a time with a
 convergence check every iteration: the per-displacement loop.
 
 `fd_mp2_instances` builds a seeded finite-difference set: a base system
@@ -29,25 +30,30 @@ CDT = torch.complex128
 
 class Instance:
     """One MP2 system on the device: F [n,n], eps [n], drv = <ab||ij> at
-    [i,j,a,b], w = <ij||ab>."""
+    [i,j,a,b], w = <ij||ab>; transpose_f / drv_scale select the lambda
+    equations (F^T, driver <ij||ab> / 4)."""
 
-    def __init__(self, O, V, F, eps, drv, w):
+    def __init__(self, O, V, F, eps, drv, w, transpose_f=False, drv_scale=1.0):
         self.O, self.V = int(O), int(V)
         self.F = F.contiguous()
         self.eps = eps.contiguous()
         self.drv = drv.contiguous()
         self.w = w.contiguous() if w is not None else None
+        self.transpose_f = bool(transpose_f)
+        self.drv_scale = float(drv_scale)
 
 
 class MP2NCBatch:
-    def __init__(self, instances, transpose_f=False, drv_scale=1.0):
+    """transpose_f / drv_scale, when given, apply to every instance;
+    otherwise each instance's own settings are used (amplitude and lambda
+    solves may share one batch)."""
+
+    def __init__(self, instances, transpose_f=None, drv_scale=None):
         self.lib = _lib.load()
         self.inst = list(instances)
         self.n = len(self.inst)
         self.dev = self.inst[0].F.device
-        self.transpose_f = int(bool(transpose_f))
-        self.drv_scale = float(drv_scale)
-        sizes = [I.O * I.O * I.V * I.V for I in self.inst]
+        sizes = [int(self.lib.apy_t2_packed_length(I.O, I.V)) for I in self.inst]   # x[(i<j)][(a<b)]
         offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
         self._xbuf = torch.zeros(int(offs[-1]), dtype=CDT, device=self.dev)
         self._xnbuf = torch.empty(int(offs[-1]), dtype=CDT, device=self.dev)
@@ -55,9 +61,11 @@ class MP2NCBatch:
         self.x_new = [self._xnbuf[offs[k]:offs[k + 1]] for k in range(self.n)]
         tab = (_lib.Mp2NcInstance * self.n)()
         for k, I in enumerate(self.inst):
+            tr = I.transpose_f if transpose_f is None else bool(transpose_f)
+            sc = I.drv_scale if drv_scale is None else float(drv_scale)
             tab[k] = _lib.Mp2NcInstance(I.O, I.V, I.F.data_ptr(), I.eps.data_ptr(), I.drv.data_ptr(),
                                         self.x[k].data_ptr(), self.x_new[k].data_ptr(),
-                                        I.w.data_ptr() if I.w is not None else None)
+                                        I.w.data_ptr() if I.w is not None else None, int(tr), sc)
         self.table = tab
         nbytes = int(self.lib.apy_mp2_nc_batched_workspace(self.n, tab))
         # a private workspace: the tables stay valid while graphs replay
@@ -73,8 +81,8 @@ class MP2NCBatch:
         self._graph = None
 
     def step(self, reverse):
-        _lib.check(self.lib.apy_mp2_nc_batched_step(self.n, self.items, self.transpose_f, self.drv_scale,
-                                                    int(reverse), ctypes.c_void_p(self.rnorm.data_ptr()),
+        _lib.check(self.lib.apy_mp2_nc_batched_step(self.n, self.items, int(reverse),
+                                                    ctypes.c_void_p(self.rnorm.data_ptr()),
                                                     ctypes.c_void_p(self.ws.data_ptr()), stream_of(self.dev)),
                    "apy_mp2_nc_batched_step")
 
@@ -91,6 +99,22 @@ class MP2NCBatch:
         self._xbuf.zero_()
         self.iterations = 0
 
+    def amplitudes(self, k):
+        """Instance k's amplitudes as the dense [O, O, V, V] tensor."""
+        I = self.inst[k]
+        out = torch.empty((I.O, I.O, I.V, I.V), dtype=CDT, device=self.dev)
+        _lib.check(self.lib.apy_t2_unpack(I.O, I.V, ctypes.c_void_p(self.x[k].data_ptr()),
+                                          ctypes.c_void_p(out.data_ptr()), stream_of(self.dev)), "apy_t2_unpack")
+        return out
+
+    def start_from(self, k, dense):
+        """Set instance k's amplitudes from a dense [O, O, V, V] tensor (a warm
+        start, e.g. the converged amplitudes of the undisplaced system)."""
+        I = self.inst[k]
+        dense = dense.contiguous()
+        _lib.check(self.lib.apy_t2_pack(I.O, I.V, ctypes.c_void_p(dense.data_ptr()),
+                                        ctypes.c_void_p(self.x[k].data_ptr()), stream_of(self.dev)), "apy_t2_pack")
+
     def solve(self, tol=1e-12, maxit=200, chunk=4, graph=False):
         """Iterate all instances until every residual norm is below tol,
         checked once per chunk of iterations (chunk even; the amplitudes end
@@ -99,8 +123,7 @@ class MP2NCBatch:
         count."""
         chunk = max(2, chunk + (chunk % 2))
         if graph and (self._graph is None or self._graph[0] != chunk):
-            self._graph = (chunk, self._capture(chunk))
-            self.reset()
+            self._graph = (chunk, self._capture(chunk))        # recorded, not run: x is untouched
         it = 0
         while it < maxit:
             if graph:
@@ -138,45 +161,67 @@ class MP2NCBatch:
         return self.energy_out
 
 
-def mp2_adjoint_directional(base, dF, dW, dV, tol=1e-12, maxit=200):
-    """Directional derivatives of the MP2 energy of `base` along directions
-    (dF_k [n,n], dW_k = d<ij||ab>, dV_k = d<ab||ij>; flattened device
-    tensors, dW/dV may be None) from one amplitude solve, one lambda solve
-    (the same batched kernels with F^T and the driver <ij||ab>/4) and the
-    gradient densities (apy_mp2_nc_densities):
-        dE = sum dE/dF . dF + sum t/4 . dW + sum lambda . dV."""
+def mp2_adjoint_set(bases, directions, tol=1e-12, maxit=200, graph=False):
+    """Directional derivatives of the MP2 energies of several base systems:
+    for base b the directions directions[b] = (dF list [n*n], dW list or
+    None, dV list or None; flattened device tensors).  The amplitude and the
+    lambda equations (F^T, driver <ij||ab>/4) of every base are one batch
+    (one fused launch per iteration for all of them); then per base the
+    gradient densities (apy_mp2_nc_densities) and the directional sums
+        dE = sum dE/dF . dF + sum t/4 . dW + sum lambda . dV,
+    all read back with one host copy.  Returns ([array of dE per base],
+    [(G, t, lambda) per base])."""
     lib = _lib.load()
-    dev = base.F.device
-    O, V = base.O, base.V
-    tb = MP2NCBatch([base])
-    lb = MP2NCBatch([Instance(O, V, base.F, base.eps, base.w, None)], transpose_f=True, drv_scale=0.25)
-    tb.solve(tol, maxit, chunk=8)
-    lb.solve(tol, maxit, chunk=8)
-    t, lam = tb.x[0], lb.x[0]
-    n = O + V
-    G = torch.empty(n * n, dtype=CDT, device=dev)
-    nb = int(lib.apy_mp2_nc_density_workspace(O, V))
-    ws = workspace(nb, dev)
-    _lib.check(lib.apy_mp2_nc_densities(O, V, ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(lam.data_ptr()),
-                                        ctypes.c_void_p(G.data_ptr()), ctypes.c_void_p(ws.data_ptr()), nb,
-                                        stream_of(dev)), "apy_mp2_nc_densities")
-    K = len(dF)
-    terms = [(xs, y, scale) for xs, y, scale in ((dF, G, 1.0), (dW, t, 0.25), (dV, lam, 1.0)) if xs is not None]
-    part = torch.zeros((len(terms), K), dtype=CDT, device=dev)      # every directional sum, read back once
-    for q, (xs, y, _) in enumerate(terms):
-        for s0 in range(0, K, _lib.APY_MAXVEC):
-            chunk = xs[s0:s0 + _lib.APY_MAXVEC]
-            arr = (ctypes.c_void_p * len(chunk))(*[x.data_ptr() for x in chunk])
-            nb = int(lib.apy_reduce_workspace(y.numel(), len(chunk)))
-            ws = workspace(nb, dev)
-            _lib.check(lib.apy_zdot_many(len(chunk), arr, ctypes.c_void_p(y.data_ptr()), y.numel(), 0,
-                                         ctypes.c_void_p(part[q, s0:].data_ptr()), ctypes.c_void_p(ws.data_ptr()),
-                                         nb, stream_of(dev)), "apy_zdot_many")
+    dev = bases[0].F.device
+    nbase = len(bases)
+    lam_inst = [Instance(B.O, B.V, B.F, B.eps, B.w, None, transpose_f=True, drv_scale=0.25) for B in bases]
+    batch = MP2NCBatch(list(bases) + lam_inst)
+    batch.solve(tol, maxit, chunk=8, graph=graph)
+    out, dens = [], []
+    jobs = []
+    for b, B in enumerate(bases):
+        O, V, n = B.O, B.V, B.O + B.V
+        t, lam = batch.amplitudes(b).view(-1), batch.amplitudes(nbase + b).view(-1)
+        G = torch.empty(n * n, dtype=CDT, device=dev)
+        nb = int(lib.apy_mp2_nc_density_workspace(O, V))
+        ws = workspace(nb, dev)
+        _lib.check(lib.apy_mp2_nc_densities(O, V, ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(lam.data_ptr()),
+                                            ctypes.c_void_p(G.data_ptr()), ctypes.c_void_p(ws.data_ptr()), nb,
+                                            stream_of(dev)), "apy_mp2_nc_densities")
+        dens.append((G, t, lam))
+        dF, dW, dV = directions[b]
+        terms = [(xs, y, scale) for xs, y, scale in ((dF, G, 1.0), (dW, t, 0.25), (dV, lam, 1.0)) if xs is not None]
+        jobs.append((len(dF), terms))
+    total = sum(len(terms) * K for K, terms in jobs)
+    part = torch.zeros(max(total, 1), dtype=CDT, device=dev)      # every directional sum, read back once
+    off = 0
+    for K, terms in jobs:
+        for xs, y, _ in terms:
+            for s0 in range(0, K, _lib.APY_MAXVEC):
+                chunk = xs[s0:s0 + _lib.APY_MAXVEC]
+                arr = (ctypes.c_void_p * len(chunk))(*[x.data_ptr() for x in chunk])
+                nb = int(lib.apy_reduce_workspace(y.numel(), len(chunk)))
+                ws = workspace(nb, dev)
+                _lib.check(lib.apy_zdot_many(len(chunk), arr, ctypes.c_void_p(y.data_ptr()), y.numel(), 0,
+                                             ctypes.c_void_p(part[off + s0:].data_ptr()),
+                                             ctypes.c_void_p(ws.data_ptr()), nb, stream_of(dev)), "apy_zdot_many")
+            off += K
     sums = part.cpu().numpy()
-    out = np.zeros(K, dtype=np.complex128)
-    for q, (_, _, scale) in enumerate(terms):
-        out += scale * sums[q]
-    return out, (G, t, lam)
+    off = 0
+    for K, terms in jobs:
+        v = np.zeros(K, dtype=np.complex128)
+        for _, _, scale in terms:
+            v += scale * sums[off:off + K]
+            off += K
+        out.append(v)
+    return out, dens
+
+
+def mp2_adjoint_directional(base, dF, dW, dV, tol=1e-12, maxit=200):
+    """Directional derivatives of the MP2 energy of `base` (mp2_adjoint_set
+    with one base).  Returns (array of dE, (G, t, lambda))."""
+    out, dens = mp2_adjoint_set([base], [(dF, dW, dV)], tol, maxit)
+    return out[0], dens[0]
 
 
 def plain_solve(instances, tol=1e-12, maxit=200):

@@ -491,31 +491,37 @@ int apy_so_ints_grad_to_full(int64_t n, const apy_so_ints* grad, void* full, voi
  * The same non-canonical MP2 (or lambda) iteration batched over independent
  * instances of possibly different (O, V): the +/- displaced systems of a
  * finite-difference set (fin_diff.py:267-372 solves them one at a time).
- * One workgroup per (instance, i, j) pair; every step covers all instances.
- * setup uploads the instance and pair tables into the workspace (host
- * arrays; synchronises the stream) and returns the pair count; step and
- * energy touch only device memory (they write scratch regions of the
- * workspace), so a sequence of them can be captured in a HIP graph.  setup
- * also keeps a host copy of the instance table under the workspace's
- * address, from which step launches one F_vv GEMM per instance with
- * V >= 48 (step on a workspace that was never set up returns APY_E_ARG).  reverse = 1 iterates x_new -> x (ping-pong without
- * re-setup).  rnorm[k] = sum |r_k|^2 (double); energy[k] = scale *
- * sum w_k x_k (complex; w may be NULL).
+ * The amplitudes are antisymmetric in (i,j) and (a,b) and stored packed,
+ * x[(i<j)][(a<b)] (apy_t2_packed_length elements; apy_t2_pack / apy_t2_unpack
+ * convert from / to the dense [O][O][V][V] tensor).  One fused kernel per
+ * step covers every instance (a workgroup per instance, pair i<j and 32 x 32
+ * tile pair of the virtual plane).  setup uploads the instance and work
+ * tables into the workspace (host arrays; synchronises the stream) and
+ * returns the item count; step and energy touch only device memory (they
+ * write scratch regions of the workspace), so a sequence of them can be
+ * captured in a HIP graph.  reverse = 1 iterates x_new -> x (ping-pong
+ * without re-setup).  rnorm[k] = sum over (i<j, a<b) |r_k|^2 (double);
+ * energy[k] = scale * sum_dense w_k x_k (complex, w antisymmetric; w may be
+ * NULL).  Instances with O < 2 or V < 2 have no pairs and keep x.
  * ---------------------------------------------------------------------- */
+int64_t apy_t2_packed_length(int64_t O, int64_t V);
+int apy_t2_pack(int64_t O, int64_t V, const void* dense, void* packed, void* stream);
+int apy_t2_unpack(int64_t O, int64_t V, const void* packed, void* dense, void* stream);
 typedef struct {
   int64_t O, V;
   const void* fock;  /* [(O+V)][(O+V)] complex */
   const void* eps;   /* [O+V] double: Jacobi denominators */
-  const void* drv;   /* [O][O][V][V] driver (<ab||ij> at [i,j,a,b], or <ij||ab>) */
-  void* x;           /* [O][O][V][V] amplitudes */
-  void* x_new;       /* [O][O][V][V] (distinct from x) */
+  const void* drv;   /* [O][O][V][V] driver (<ab||ij> at [i,j,a,b], or <ij||ab>), antisymmetric */
+  void* x;           /* [Po][Pv] packed amplitudes */
+  void* x_new;       /* [Po][Pv] (distinct from x) */
   const void* w;     /* [O][O][V][V] <ij||ab> for the energy, or NULL */
+  int transpose_f;   /* 1: the lambda equations (F^T) */
+  double drv_scale;  /* s: the driver enters as s * drv */
 } apy_mp2_nc_instance;
 size_t apy_mp2_nc_batched_workspace(int n, const apy_mp2_nc_instance* inst);
 int apy_mp2_nc_batched_setup(int n, const apy_mp2_nc_instance* inst, void* workspace, size_t ws_bytes,
                              int64_t* n_items, void* stream);
-int apy_mp2_nc_batched_step(int n, int64_t n_items, int transpose_f, double drv_scale, int reverse, void* rnorm,
-                            void* workspace, void* stream);
+int apy_mp2_nc_batched_step(int n, int64_t n_items, int reverse, void* rnorm, void* workspace, void* stream);
 int apy_mp2_nc_batched_energy(int n, int64_t n_items, int use_x_new, double scale, void* energy,
                               void* workspace, void* stream);
 

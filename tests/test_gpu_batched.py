@@ -47,7 +47,7 @@ def test_batched_converges_to_oracle(dev):
     for k, (_, (F, W, Vd, eps)) in enumerate(data):
         E_ref, t_ref = adjoint.energy(F, W, Vd, eps, sizes[k][0])
         assert abs(E[k] - E_ref) < 1e-12
-        assert np.abs(b.x[k].cpu().numpy().reshape(t_ref.shape) - t_ref).max() < 1e-11
+        assert np.abs(b.amplitudes(k).cpu().numpy() - t_ref).max() < 1e-11
 
 
 def test_lambda_batch_matches_adjoint(dev):
@@ -56,7 +56,7 @@ def test_lambda_batch_matches_adjoint(dev):
     b = MP2NCBatch([lam_inst], transpose_f=True, drv_scale=0.25)
     b.solve(tol=1e-14, maxit=400)
     _, _, lam_ref, _, _, _ = adjoint.gradients(F, W, Vd, eps, 5)
-    assert np.abs(b.x[0].cpu().numpy().reshape(lam_ref.shape) - lam_ref).max() < 1e-12
+    assert np.abs(b.amplitudes(0).cpu().numpy() - lam_ref).max() < 1e-12
 
 
 def test_plain_loop_energies_match_batched(dev):
@@ -132,12 +132,12 @@ def test_adjoint_directional_matches_fd(dev):
 
 
 @pytest.mark.parametrize("transpose", [False, True], ids=["amplitudes", "lambda"])
-def test_batched_gemm_path_for_wide_instances(dev, transpose):
-    """Instances with V >= 48 form their F_vv term as one MFMA GEMM, the
-    others in the blocked kernel: a mixed batch converges to the oracle's
-    amplitudes (and lambda with F^T), and a graph-replayed chunk equals
-    direct launches bit for bit."""
-    sizes = [(3, 50), (2, 6), (2, 49)]
+def test_batched_wide_instances(dev, transpose):
+    """Instances spanning one to three 32-wide virtual tiles (ragged last
+    tiles) and single-pair occupied spaces in one batch converge to the
+    oracle's amplitudes (and lambda with F^T), and a graph-replayed chunk
+    equals direct launches bit for bit."""
+    sizes = [(3, 50), (2, 6), (2, 49), (4, 70), (2, 33)]
     data = [_host_instance(O, V, 40 + k, dev) for k, (O, V) in enumerate(sizes)]
     if transpose:
         insts = [Instance(I.O, I.V, I.F, I.eps, I.w, None) for I, _ in data]
@@ -150,7 +150,7 @@ def test_batched_gemm_path_for_wide_instances(dev, transpose):
             _, _, want, _, _, _ = adjoint.gradients(F, W, Vd, eps, sizes[k][0])
         else:
             _, want = adjoint.energy(F, W, Vd, eps, sizes[k][0])
-        got = b.x[k].cpu().numpy().reshape(want.shape)
+        got = b.amplitudes(k).cpu().numpy()
         assert np.abs(got - want).max() < 1e-11 * max(1.0, np.abs(want).max()), k
     g = MP2NCBatch([d[0] for d in data])
     h = MP2NCBatch([d[0] for d in data])
@@ -158,3 +158,21 @@ def test_batched_gemm_path_for_wide_instances(dev, transpose):
     h.solve(tol=0.0, maxit=8, chunk=4, graph=False)
     for k in range(len(sizes)):
         assert torch.equal(g.x[k], h.x[k])
+
+
+def test_packed_storage_round_trip_and_warm_start(dev):
+    """The packed amplitudes expand to the antisymmetric dense tensor and
+    pack back exactly; a batch started from converged amplitudes stays at
+    them (residual at rounding level after one chunk)."""
+    sizes = [(4, 9), (3, 40)]
+    data = [_host_instance(O, V, 60 + k, dev) for k, (O, V) in enumerate(sizes)]
+    b = MP2NCBatch([d[0] for d in data])
+    b.solve(tol=1e-13, maxit=400, chunk=8)
+    for k in range(len(sizes)):
+        t = b.amplitudes(k)
+        assert torch.equal(t, -t.transpose(0, 1)) and torch.equal(t, -t.transpose(2, 3))
+        w = MP2NCBatch([data[k][0]])
+        w.start_from(0, t)
+        assert torch.equal(w.x[0], b.x[k])
+        w.solve(tol=0.0, maxit=2, chunk=2)
+        assert float(w.rnorm.max().cpu()) ** 0.5 < 1e-12
