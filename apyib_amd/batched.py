@@ -242,14 +242,16 @@ def base_system(O, V, seed, device, std=0.05):
     return (torch.from_numpy(F).to(device), torch.from_numpy(np.asarray(eps, dtype=np.float64)).to(device), ints)
 
 
-def fd_mp2_instances(O, V, seed, natom, h=1e-4, device="cuda", sizes=None):
+def fd_mp2_instances(O, V, seed, natom, h=1e-4, device="cuda", sizes=None, with_bases=False):
     """A finite-difference set of 2 (3 natom + 3) instances: for each
     nuclear-like direction k (real symmetric dF_k, real <pq||rs> draw with
     tag 100 + k) and each field-like direction (i x antisymmetric dF), the
     base system displaced by +h and -h.  With `sizes` (a list of (O, V) per
     direction), each direction uses its own base system of that size (the
     uneven-size stress case).  Returns (instances, directions) where
-    directions[d] = (kind, seed_d, (O, V)) and instances 2d, 2d+1 are +h, -h."""
+    directions[d] = (kind, seed_d, (O, V)) and instances 2d, 2d+1 are +h, -h;
+    with_bases=True also returns the undisplaced systems, one Instance per
+    distinct size, and base_of[d], the index of direction d's system."""
     ndir = 3 * natom + 3
     dirs, out = [], []
     cache = {}
@@ -269,4 +271,27 @@ def fd_mp2_instances(O, V, seed, natom, h=1e-4, device="cuda", sizes=None):
             n = Od * Od * Vd * Vd
             out.append(Instance(Od, Vd, F0 + sgn * h * dF, eps, ints.blocks['vvoo_t'].view(n),
                                 ints.blocks['oovv'].view(n)))
-    return out, dirs
+    if not with_bases:
+        return out, dirs
+    keys = list(cache)
+    bases = []
+    for (Od, Vd) in keys:
+        F0, eps, ints = cache[(Od, Vd)]
+        n = Od * Od * Vd * Vd
+        bases.append(Instance(Od, Vd, F0, eps, ints.blocks['vvoo_t'].view(n), ints.blocks['oovv'].view(n)))
+    base_of = [keys.index(d[2]) for d in dirs]
+    return out, dirs, bases, base_of
+
+
+def warm_batch_solve(instances, bases, base_of_instance, tol=1e-12, maxit=400, graph=False):
+    """The finite-difference set solved batched with warm starts: the
+    undisplaced systems first (one batch), then every displaced instance
+    started from its system's converged amplitudes (one batch).  Returns
+    (energies of the instances, base iterations, instance iterations)."""
+    bb = MP2NCBatch(bases)
+    itb = bb.solve(tol, maxit, chunk=4, graph=graph)
+    b = MP2NCBatch(instances)
+    for k, q in enumerate(base_of_instance):
+        b.x[k].copy_(bb.x[q])                   # packed amplitudes of the same (O, V) layout
+    it = b.solve(tol, maxit, chunk=2, graph=graph)
+    return b.energies().clone(), itb, it

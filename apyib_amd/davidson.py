@@ -380,15 +380,23 @@ class MultiDavidson:
         for d, q in entries:
             d.soci.energy(d.B[q][:L], out=d.S[q][L:])
 
-    def solve(self, e_conv=1e-12, r_conv=1e-8, maxit=200):
+    def solve(self, e_conv=1e-12, r_conv=1e-8, maxit=200, guess=None):
         """Returns the array of correlation energies; each instance's
-        normalised vector is in dav[k].c (iteration counts in dav[k].iterations)."""
+        normalised vector is in dav[k].c (iteration counts in dav[k].iterations).
+        guess: optional start vectors [c1 | c2 | c0] per instance (e.g. the
+        root of the undisplaced system for its displaced copies) in place of
+        the reference determinant."""
         L = self.L
         st = []
-        for d in self.dav:
+        for k, d in enumerate(self.dav):
             d.B.zero_()
             d.S.zero_()
-            d.B[0, L] = 1.0
+            if guess is not None and guess[k] is not None:
+                d.B[0].copy_(guess[k])
+                nrm = float(np.sqrt(abs(d.dots([d.B[0]], d.B[0])[0])))
+                d.combine([d.B[0]], [1.0 / nrm], d.B[0])
+            else:
+                d.B[0, L] = 1.0
             d.history = []
             st.append({'k': 1, 'G': np.zeros((d.max_space, d.max_space), np.complex128), 'theta_old': None,
                        'x': torch.zeros(L + 1, dtype=CDT, device=self.dev), 'done': False})

@@ -148,16 +148,23 @@ def cpu_baseline(solver, O, V, flops_step, seed, reps=3):
 
 def aat_set_measurement(dev, seed, tol=1e-12):
     """Second metric: a finite-difference tensor set solved batched (one
-    launch per iteration for all instances, graph-captured chunks) plus the
-    adjoint gradient, against the plain per-displacement loop (the same
-    kernels, one instance at a time, host convergence check per iteration).
+    fused launch per iteration for all instances) and by the adjoint,
+    against the plain per-displacement loop (the same kernels, one instance
+    at a time from a cold start, host convergence check per iteration: the
+    shape of fin_diff.py:285-370).
 
     Set A: 24 instances at O=10, V=38 (3 atoms: 9 nuclear + 3 field
     directions, +/- 1e-4).  Set B: 36 instances at O~16, V~60 with seeded
-    uneven sizes (18 directions).  MP2 (orbital-invariant spin-orbital)."""
+    uneven sizes (18 directions, 14 distinct systems).  MP2 (orbital-
+    invariant spin-orbital).  Batched legs: cold starts with direct
+    launches, the same replayed from a captured graph (capture included),
+    and warm starts (the undisplaced systems solved first, one batch, then
+    every displaced instance from its system's amplitudes).  Adjoint: the
+    amplitude and lambda equations of every undisplaced system in one batch,
+    then the densities and all directional sums."""
     import numpy as np
     import torch
-    from apyib_amd.batched import Instance, MP2NCBatch, fd_mp2_instances, mp2_adjoint_directional, plain_solve
+    from apyib_amd.batched import MP2NCBatch, fd_mp2_instances, mp2_adjoint_set, plain_solve, warm_batch_solve
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -173,41 +180,52 @@ def aat_set_measurement(dev, seed, tol=1e-12):
         "B": dict(O=16, V=60, natom=5, sizes=[(int(rng.integers(14, 19)), int(rng.integers(54, 67)))
                                               for _ in range(18)]),
     }
+    h = 1e-4
     for name, cfg in sets.items():
-        insts, dirs = fd_mp2_instances(cfg["O"], cfg["V"], seed, cfg["natom"], h=1e-4, device=dev,
-                                       sizes=cfg["sizes"])
-        plain_solve(insts[:2], tol)                                   # warm both paths once
+        insts, dirs, bases, base_of = fd_mp2_instances(cfg["O"], cfg["V"], seed, cfg["natom"], h=h, device=dev,
+                                                       sizes=cfg["sizes"], with_bases=True)
+        base_of_inst = [base_of[k // 2] for k in range(len(insts))]
+        plain_solve(insts[:2], tol)                                   # warm every path once
         MP2NCBatch(insts[:2]).solve(tol)
         (Ep, its), t_plain = timed(lambda: plain_solve(insts, tol))
 
-        def batched():
+        def batched(graph):
             b = MP2NCBatch(insts)
-            b.solve(tol, maxit=400)
+            b.solve(tol, maxit=400, graph=graph)
             return b.energies().clone(), b.iterations
-        (Eb, itb), t_batch = timed(batched)
-        entry = {"instances": len(insts), "directions": len(dirs),
-                 "sizes": sorted({(I.O, I.V) for I in insts}) if cfg["sizes"] else [(cfg["O"], cfg["V"])],
+        (Eb, itb), t_batch = timed(lambda: batched(False))
+        (Eg, itg), t_graph = timed(lambda: batched(True))
+        (Ew, itwb, itw), t_warm = timed(lambda: warm_batch_solve(insts, bases, base_of_inst, tol))
+        nd = len(dirs)
+        dF = [((insts[2 * k].F - insts[2 * k + 1].F) / (2 * h)).reshape(-1) for k in range(nd)]
+        dW = [(insts[2 * k].w - insts[2 * k + 1].w) / (2 * h) for k in range(nd)]
+        dV = [(insts[2 * k].drv - insts[2 * k + 1].drv) / (2 * h) for k in range(nd)]
+        groups = [[d for d in range(nd) if base_of[d] == b] for b in range(len(bases))]
+        directions = [([dF[d] for d in g], [dW[d] for d in g], [dV[d] for d in g]) for g in groups]
+        mp2_adjoint_set(bases[:1], directions[:1], tol)               # warm
+        (an, dens), t_adj = timed(lambda: mp2_adjoint_set(bases, directions, tol))
+        fd = ((Ep[0::2] - Ep[1::2]) / (2 * h)).cpu().numpy()
+        errs = []
+        for b, g in enumerate(groups):
+            G, t, lam = dens[b]
+            for q, d in enumerate(g):
+                # error relative to the natural scale of each directional sum, sum |dE/dx| |dx|
+                scale = float((torch.sum((G * dF[d]).abs()) + 0.25 * torch.sum((t * dW[d]).abs())
+                               + torch.sum((lam * dV[d]).abs())).cpu())
+                errs.append(abs(an[b][q] - fd[d]) / scale)
+        dE = lambda E: float(torch.abs(E - Ep).max().cpu())
+        entry = {"instances": len(insts), "directions": nd, "systems": len(bases),
+                 "sizes": sorted({(I.O, I.V) for I in insts}),
                  "plain_loop_s": round(t_plain, 5), "batched_s": round(t_batch, 5),
-                 "speedup_batched": round(t_plain / t_batch, 2),
+                 "batched_graph_s": round(t_graph, 5), "batched_warm_s": round(t_warm, 5),
+                 "speedup_batched": round(t_plain / t_batch, 2), "speedup_batched_graph": round(t_plain / t_graph, 2),
+                 "speedup_batched_warm": round(t_plain / t_warm, 2),
                  "iterations_plain_max": int(max(its)), "iterations_batched": int(itb),
-                 "max_abs_dE_batched_vs_plain": float(torch.abs(Eb - Ep).max().cpu())}
-        if cfg["sizes"] is None:
-            h = 1e-4
-            fd = ((Ep[0::2] - Ep[1::2]) / (2 * h)).cpu().numpy()
-            P, M = insts[0], insts[1]
-            base = Instance(cfg["O"], cfg["V"], 0.5 * (P.F + M.F), P.eps, 0.5 * (P.drv + M.drv), 0.5 * (P.w + M.w))
-            dF = [((insts[2 * k].F - insts[2 * k + 1].F) / (2 * h)).reshape(-1) for k in range(len(dirs))]
-            dW = [(insts[2 * k].w - insts[2 * k + 1].w) / (2 * h) for k in range(len(dirs))]
-            dV = [(insts[2 * k].drv - insts[2 * k + 1].drv) / (2 * h) for k in range(len(dirs))]
-            mp2_adjoint_directional(base, dF[:1], dW[:1], dV[:1], tol)        # warm
-            (an, (G, t, lam)), t_adj = timed(lambda: mp2_adjoint_directional(base, dF, dW, dV, tol))
-            # error relative to the natural scale of each directional sum, sum |dE/dx| |dx|
-            scale = np.array([float((torch.sum((G * dF[k]).abs()) + 0.25 * torch.sum((t * dW[k]).abs())
-                                     + torch.sum((lam * dV[k]).abs())).cpu()) for k in range(len(dirs))])
-            err = np.abs(an - fd) / scale
-            entry.update({"adjoint_s": round(t_adj, 5), "speedup_adjoint_vs_plain_fd": round(t_plain / t_adj, 2),
-                          "max_err_adjoint_vs_fd_rel_to_scale": float(err.max()),
-                          "fd_step": h})
+                 "iterations_warm": [int(itwb), int(itw)],
+                 "max_abs_dE_batched_vs_plain": dE(Eb), "max_abs_dE_graph_vs_plain": dE(Eg),
+                 "max_abs_dE_warm_vs_plain": dE(Ew),
+                 "adjoint_s": round(t_adj, 5), "speedup_adjoint_vs_plain_fd": round(t_plain / t_adj, 2),
+                 "max_err_adjoint_vs_fd_rel_to_scale": float(max(errs)), "fd_step": h}
         res[name] = entry
         log(f"aat set {name}: {entry}")
     return res
@@ -265,6 +283,32 @@ def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
     (Eb, its_b), t_batch = timed(batched)
     log(f"cisd field set: batched {t_batch:.2f} s, iterations {its_b}")
 
+    def warm(batch):
+        """The undisplaced root first, then the displaced instances from it
+        (batched, or one at a time)."""
+        md = MultiDavidson(ints, [F0], eps, max_space)
+        md.solve(**conv)
+        c0 = md.dav[0].c.clone()
+        base_its = md.dav[0].iterations
+        del md
+        if batch:
+            md = MultiDavidson(ints, focks, eps, max_space)
+            E = md.solve(**conv, guess=[c0] * len(focks))
+            its = [d.iterations for d in md.dav]
+            del md
+            return np.asarray(E), [base_its] + its
+        E, its = [], [base_its]
+        for Fk in focks:
+            md = MultiDavidson(ints, [Fk], eps, max_space)
+            E.append(md.solve(**conv, guess=[c0])[0])
+            its.append(md.dav[0].iterations)
+            del md
+        return np.array(E), its
+    (Ew, its_w), t_warm = timed(lambda: warm(True))
+    log(f"cisd field set: batched warm {t_warm:.2f} s, iterations {its_w}")
+    (Epw, its_pw), t_plain_warm = timed(lambda: warm(False))
+    log(f"cisd field set: one at a time warm {t_plain_warm:.2f} s, iterations {its_pw}")
+
     def adjoint():
         md = MultiDavidson(ints, [F0], eps, max_space)
         md.solve(**conv)
@@ -279,8 +323,13 @@ def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
     res = {"workload": f"Hermitian CISD lowest root, O={O} V={V}, 6 instances (3 field components x +/-{h}) "
                        f"sharing one seeded <pq||rs>", "instances": 6,
            "plain_loop_s": round(t_plain, 3), "batched_s": round(t_batch, 3), "adjoint_s": round(t_adj, 3),
-           "speedup_batched": round(t_plain / t_batch, 2), "speedup_adjoint_vs_plain_fd": round(t_plain / t_adj, 2),
+           "batched_warm_s": round(t_warm, 3), "plain_warm_s": round(t_plain_warm, 3),
+           "speedup_batched": round(t_plain / t_batch, 2), "speedup_batched_warm": round(t_plain / t_warm, 2),
+           "speedup_plain_warm": round(t_plain / t_plain_warm, 2),
+           "speedup_adjoint_vs_plain_fd": round(t_plain / t_adj, 2),
            "iterations_plain": its_p, "iterations_batched": its_b,
+           "iterations_warm_base_then_instances": its_w,
+           "max_abs_dE_warm_vs_plain": float(np.abs(Ew - Ep).max()),
            "max_abs_dE_batched_vs_plain": float(np.abs(Eb - Ep).max()),
            "max_err_adjoint_vs_fd_rel_to_scale": float((np.abs(g.real - fd_b) / scale).max()),
            "max_err_fd_batched_vs_plain_rel_to_scale": float((np.abs(fd_b - fd_p) / scale).max())}
