@@ -15,12 +15,14 @@ APY_E_ARG = -1
 APY_E_HIP = -2
 APY_E_WORKSPACE = -3
 APY_E_SINGULAR = -4
+APY_E_TOO_LARGE = -5
 
 _ERR = {
     APY_E_ARG: "invalid argument",
     APY_E_HIP: "HIP runtime error",
     APY_E_WORKSPACE: "workspace too small",
     APY_E_SINGULAR: "singular linear system",
+    APY_E_TOO_LARGE: "problem larger than the library's static scratch (APY_NOALLOC_SCRATCH_BYTES)",
 }
 
 APY_MK, APY_KM = 0, 1
@@ -98,6 +100,8 @@ class CisdSpArgs(ctypes.Structure):
         ("out", c_vp), ("resp_b", c_vp), ("resp_t", c_vp), ("resp_energy", c_vp),
         ("resp_shift_re", c_dbl), ("resp_shift_im", c_dbl),
     ]
+
+APY_NOALLOC_SCRATCH_BYTES = 256 << 20
 
 APY_MODE_RESIDUAL = 0
 APY_MODE_SIGMA = 1
@@ -196,6 +200,8 @@ SIGNATURES = {
     "apy_cisd_so_ring_fock": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "apy_cisd_so_workspace": (c_size, [c_i64, c_i64, c_int]),
     "apy_cisd_so_step": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_vp]),
+    "apy_cisd_so_step_noalloc": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp]),
+    "apy_cisd_so_noalloc_bytes": (c_size, [ctypes.POINTER(CisdSoArgs)]),
     "apy_cisd_so_tune": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_int, c_vp]),
     "apy_cisd_so_sigma_batch_workspace": (c_size, [c_i64, c_i64, c_int, c_int]),
     "apy_cisd_so_sigma_batch": (c_int, [ctypes.POINTER(CisdSoSigmaBatchArgs), c_vp, c_size, c_vp]),

@@ -163,10 +163,12 @@ class SOCI:
         return t[:self.OV].view(O, V), t[self.OV:].view(O, O, V, V)
 
     def step(self, t, t_new, out, energy=None, mode=_lib.APY_MODE_RESIDUAL, c0=1.0, ev_dominant=None, c0_dev=None,
-             packed=False):
+             packed=False, noalloc=False):
         """One call of apy_cisd_so_step (residual update or sigma product).
         c0_dev: optional device address of the sigma-mode driver weight;
-        packed: t and out are packed vectors [t1 | t2 over (i<j, a<b)]."""
+        packed: t and out are packed vectors [t1 | t2 over (i<j, a<b)];
+        noalloc: apy_cisd_so_step_noalloc (the library's static scratch, no
+        workspace allocated here)."""
         c0 = complex(c0)
         ev = ev_dominant or self.ev_dominant or (None, None)
         args = _lib.CisdSoArgs(
@@ -175,6 +177,10 @@ class SOCI:
             (energy if energy is not None else self.energy_dev).data_ptr(), c0.real, c0.imag,
             t.data_ptr(), t_new.data_ptr() if t_new is not None else None, out.data_ptr(), ev[0], ev[1], c0_dev,
             self.ring_f.data_ptr() if self.ring_f is not None else None, int(bool(packed)))
+        if noalloc:
+            _lib.check(self.lib.apy_cisd_so_step_noalloc(ctypes.byref(args), stream_of(self.device)),
+                       "apy_cisd_so_step_noalloc")
+            return
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_so_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
                    "apy_cisd_so_step")

@@ -34,6 +34,8 @@ extern "C" {
 #define APY_E_HIP (-2)       /* a HIP runtime call or launch failed       */
 #define APY_E_WORKSPACE (-3) /* caller workspace smaller than required    */
 #define APY_E_SINGULAR (-4)  /* singular linear system (DIIS)             */
+#define APY_E_TOO_LARGE (-5) /* the problem needs more than the library's
+                                static scratch (no-allocation entry point) */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int apy_version(void);
@@ -287,6 +289,19 @@ typedef struct apy_cisd_so_args {
                               O*V + O^2*V^2); 1: [t1 | t2 over (i<j, a<b)]
                               (length O*V + Po*Pv), sigma mode only        */
 } apy_cisd_so_args;
+
+/* apy_cisd_so_step without a workspace argument and without any device
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc): no counterpart
+ * in the reference.  The workspace is the library's static per-device array
+ * of APY_NOALLOC_SCRATCH_BYTES; problems that need more (see
+ * apy_cisd_so_noalloc_bytes, the bytes this call would use with the GEMM
+ * plans in effect) return APY_E_TOO_LARGE.  All calls on one device share
+ * the array: calls that overlap on one GPU race.  Same arguments and
+ * results as apy_cisd_so_step. */
+#define APY_NOALLOC_SCRATCH_BYTES (256u << 20)
+size_t apy_cisd_so_noalloc_bytes(const apy_cisd_so_args* a);
+int apy_cisd_so_step_noalloc(const apy_cisd_so_args* a, void* stream);
 
 /* Pair-space occupied operator: <kl||ij> plus the F_oo terms of the doubles
  * residual (ci_wfn.py:326-328), [(i<j)][(k<l)]; built once per solve. */
