@@ -15,15 +15,18 @@
 // packed, x[(i<j)][(a<b)], a quarter of the dense tensor; the residual is
 // formed only for i<j, a<b.
 //
-// One fused kernel per step covers every instance: a workgroup owns one
-// (instance, pair i<j, 32 x 32 tile pair A <= B of the virtual plane).  It
+// Two kernels per step cover every instance.  nc_z_kernel forms
+// Z_ij[(ab)] = sum_m Foo(m,j) x_im[(ab)] for all ordered pairs (i,j): a
+// workgroup takes one (instance, i, slice of the (a<b) pairs), holds the O
+// amplitudes x_im of each element in registers (read once, coalesced) and
+// writes Z_ij for every j.  ncp_step_kernel then gives a workgroup one
+// (instance, pair i<j, 32 x 32 tile pair A <= B of the virtual plane): it
 // forms Y over the tile as a small product on the vector units with K = e
 // staged in 16-wide chunks through LDS (the tile's rows of x_ij expanded from
 // the packed row with the antisymmetry signs, and the matching columns of
-// Fvv), then adds the driver, the Foo part read from the 2(O-1) packed rows
-// x_im, x_jm of the same (a<b) positions (coalesced), applies the update and
-// writes its partial residual norm; per_instance_sum reduces those per
-// instance in a fixed order.  All tables live in the workspace, written once
+// Fvv), adds the driver and -Z_ij + Z_ji (two coalesced reads), applies the
+// update and writes its partial residual norm; per_instance_sum reduces
+// those per instance in a fixed order.  All tables live in the workspace, written once
 // by apy_mp2_nc_batched_setup, so a sequence of steps can be captured in a
 // HIP graph.
 #include "common.h"
@@ -41,8 +44,16 @@ struct Item {
   int32_t inst, pij, ta, tb;
 };
 
+constexpr int ZS = 256;      // (a<b) elements per Z work item
+constexpr int ZMAX_O = 32;   // occupied orbitals held in registers by the Z kernel
+
+struct ZItem {
+  int32_t inst, i, p0, pad;
+};
+
 struct Header {
-  int64_t n, items;
+  int64_t n, items, zitems;
+  int64_t zitem_off, z_off;   // byte offsets of the Z work items and of the Z blocks
 };
 
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -65,20 +76,38 @@ __device__ inline void pdec(int64_t P, int64_t N, int64_t& p, int64_t& q) {
 }
 
 // Workspace: header, instance table, item table, per-instance item offsets,
-// item partial norms, energy partial sums.
+// item partial norms, energy partial sums, per-instance offsets of the Z
+// blocks, then (at byte offsets the header holds, so the step needs only
+// (n, items) on the host) the Z work items and the Z blocks themselves
+// (sum_k O_k^2 Pv_k complex).
 struct WsLayout {
-  size_t inst, item, off, part, epart, total;
+  size_t inst, item, off, part, epart, zoff, zitem, z, total;
 };
 
-inline WsLayout layout(int n, int64_t items) {
+inline WsLayout layout(int n, int64_t items, int64_t zitems = 0, int64_t zelems = 0) {
   WsLayout L;
   L.inst = al256(sizeof(Header));
   L.item = L.inst + al256((size_t)n * sizeof(apy_mp2_nc_instance));
   L.off = L.item + al256((size_t)items * sizeof(Item));
   L.part = L.off + al256((size_t)(n + 1) * sizeof(int64_t));
   L.epart = L.part + al256((size_t)items * sizeof(double));
-  L.total = L.epart + al256((size_t)n * EN_B * sizeof(double2));
+  L.zoff = L.epart + al256((size_t)n * EN_B * sizeof(double2));
+  L.zitem = L.zoff + al256((size_t)(n + 1) * sizeof(int64_t));
+  L.z = L.zitem + al256((size_t)zitems * sizeof(ZItem));
+  L.total = L.z + al256((size_t)zelems * sizeof(double2));
   return L;
+}
+
+int64_t count_zitems(int n, const apy_mp2_nc_instance* inst, int64_t* zelems) {
+  int64_t z = 0, e = 0;
+  for (int k = 0; k < n; ++k) {
+    if (inst[k].O < 2 || inst[k].V < 2) continue;
+    const int64_t Pv = npair(inst[k].V);
+    z += inst[k].O * ((Pv + ZS - 1) / ZS);
+    e += inst[k].O * inst[k].O * Pv;
+  }
+  if (zelems) *zelems = e;
+  return z;
 }
 
 int64_t count_items(int n, const apy_mp2_nc_instance* inst) {
@@ -103,8 +132,60 @@ __device__ inline double2 xget(const double2* x, int64_t O, int64_t Pv, int64_t 
   return ((p < q) == (c < d)) ? v : make_double2(-v.x, -v.y);
 }
 
+// Z_k[i][j][(ab)] = sum_m Foo(m,j) x_im[(ab)] for every instance, row i and slice of
+// ZS pairs (ab) (grid-stride over the Z work list, whose length is in the header).
+__global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const apy_mp2_nc_instance* __restrict__ insts,
+                                                   const int64_t* __restrict__ zoff, int reverse) {
+  __shared__ double2 fo[ZMAX_O * ZMAX_O];
+  const Header* hdr = (const Header*)ws;
+  const int64_t nz = hdr->zitems;
+  const ZItem* zitems = (const ZItem*)(ws + hdr->zitem_off);
+  double2* zbase = (double2*)(ws + hdr->z_off);
+  for (int64_t w = blockIdx.x; w < nz; w += gridDim.x) {
+    const ZItem it = zitems[w];
+    const apy_mp2_nc_instance I = insts[it.inst];
+    const int64_t O = I.O, V = I.V, n = O + V, Pv = npair(V), i = it.i;
+    const int transpose_f = I.transpose_f;
+    const double2* F = (const double2*)I.fock;
+    const double2* x = (const double2*)(reverse ? I.x_new : I.x);
+    double2* Z = zbase + zoff[it.inst] + i * O * Pv;
+    const int64_t P = it.p0 + threadIdx.x;
+    auto foo = [&](int64_t m, int64_t j) { return transpose_f ? F[j * n + m] : F[m * n + j]; };
+    // x_im[P]: +row (i,m) for i < m, -row (m,i) for m < i, zero for m == i
+    auto xim = [&](int64_t m) -> double2 {
+      if (m == i) return make_double2(0.0, 0.0);
+      const double2 v = x[pidx(i < m ? i : m, i < m ? m : i, O) * Pv + P];
+      return i < m ? v : make_double2(-v.x, -v.y);
+    };
+    if (O <= ZMAX_O) {
+      __syncthreads();                                   // fo of the previous item is no longer read
+      for (int64_t q = threadIdx.x; q < O * O; q += BLK) fo[q] = foo(q / O, q % O);   // fo[m O + j]
+      __syncthreads();
+      if (P < Pv) {
+        double2 xv[ZMAX_O];
+#pragma unroll
+        for (int m = 0; m < ZMAX_O; ++m) xv[m] = m < O ? xim(m) : make_double2(0.0, 0.0);
+        for (int64_t j = 0; j < O; ++j) {
+          double2 z = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int m = 0; m < ZMAX_O; ++m)
+            if (m < O) cfma(z, fo[m * O + j], xv[m]);
+          Z[j * Pv + P] = z;
+        }
+      }
+    } else if (P < Pv) {
+      for (int64_t j = 0; j < O; ++j) {
+        double2 z = make_double2(0.0, 0.0);
+        for (int64_t m = 0; m < O; ++m) cfma(z, foo(m, j), xim(m));
+        Z[j * Pv + P] = z;
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(BLK) void ncp_step_kernel(const apy_mp2_nc_instance* __restrict__ insts,
                                                        const Item* __restrict__ items, int reverse,
+                                                       const char* __restrict__ ws, const int64_t* __restrict__ zoff,
                                                        double* __restrict__ part) {
   __shared__ double2 XA[MT][MKC + 1], XB[MT][MKC + 1], GA[MKC][MT + 1], GB[MKC][MT + 1];
   __shared__ double red[BLK];
@@ -159,6 +240,7 @@ __global__ __launch_bounds__(BLK) void ncp_step_kernel(const apy_mp2_nc_instance
 
   const double* eps = (const double*)I.eps;
   const double2* drv = (const double2*)I.drv;
+  const double2* Z = (const double2*)(ws + ((const Header*)ws)->z_off) + zoff[it.inst];
   const double eij = eps[i] + eps[j];
   double acc = 0.0;
   for (int q = 0; q < MT / 8; ++q) {
@@ -166,16 +248,10 @@ __global__ __launch_bounds__(BLK) void ncp_step_kernel(const apy_mp2_nc_instance
     if (b >= V || a >= b) continue;
     const int64_t Pab = pidx(a, b, V);
     const double2 d = drv[((i * O + j) * V + a) * V + b];
-    double2 r = make_double2(s * d.x + yab[q].x - yba[q].x, s * d.y + yab[q].y - yba[q].y);
-    for (int64_t m = 0; m < O; ++m) {
-      // -Foo(m,j) x_im[a,b] + Foo(m,i) x_jm[a,b]
-      const double2 fj = transpose_f ? F[j * n + m] : F[m * n + j];
-      const double2 fi = transpose_f ? F[i * n + m] : F[m * n + i];
-      const double2 xim = xget(x, O, Pv, V, i, m, a, b), xjm = xget(x, O, Pv, V, j, m, a, b);
-      r.x -= fj.x * xim.x - fj.y * xim.y;
-      r.y -= fj.x * xim.y + fj.y * xim.x;
-      cfma(r, fi, xjm);
-    }
+    // -Z_ij + Z_ji: -sum_m Foo(m,j) x_im[a,b] + sum_m Foo(m,i) x_jm[a,b]
+    const double2 zij = Z[(i * O + j) * Pv + Pab], zji = Z[(j * O + i) * Pv + Pab];
+    const double2 r = make_double2(s * d.x + yab[q].x - yba[q].x - zij.x + zji.x,
+                                   s * d.y + yab[q].y - yba[q].y - zij.y + zji.y);
     const double D = eij - eps[O + a] - eps[O + b];
     const double2 xv = row[Pab];
     xn[(int64_t)it.pij * Pv + Pab] = make_double2(xv.x + r.x / D, xv.y + r.y / D);
@@ -305,7 +381,9 @@ int apy_t2_unpack(int64_t O, int64_t V, const void* packed, void* dense, void* s
 
 size_t apy_mp2_nc_batched_workspace(int n, const apy_mp2_nc_instance* inst) {
   if (n < 1 || !inst) return 0;
-  return layout(n, count_items(n, inst)).total;
+  int64_t ze = 0;
+  const int64_t zi = count_zitems(n, inst, &ze);
+  return layout(n, count_items(n, inst), zi, ze).total;
 }
 
 int apy_mp2_nc_batched_setup(int n, const apy_mp2_nc_instance* inst, void* workspace, size_t ws_bytes,
@@ -318,22 +396,31 @@ int apy_mp2_nc_batched_setup(int n, const apy_mp2_nc_instance* inst, void* works
       return APY_E_ARG;
   }
   const int64_t items = count_items(n, inst);
-  if (items < 1 || items > 2147483647) return APY_E_ARG;
-  const WsLayout L = layout(n, items);
+  int64_t zelems = 0;
+  const int64_t zitems = count_zitems(n, inst, &zelems);
+  if (items < 1 || items > 2147483647 || zitems > 2147483647) return APY_E_ARG;
+  const WsLayout L = layout(n, items, zitems, zelems);
   if (!workspace || ws_bytes < L.total) return APY_E_WORKSPACE;
   std::vector<Item> it;
   it.reserve(items);
-  std::vector<int64_t> off(n + 1, 0);
+  std::vector<ZItem> zit;
+  zit.reserve(zitems);
+  std::vector<int64_t> off(n + 1, 0), zoff(n + 1, 0);
   for (int k = 0; k < n; ++k) {
     off[k] = (int64_t)it.size();
+    zoff[k + 1] = zoff[k];
     if (inst[k].O < 2 || inst[k].V < 2) continue;
     const int32_t nt = (int32_t)ntiles(inst[k].V);
+    const int64_t Pv = npair(inst[k].V);
     for (int32_t p = 0; p < (int32_t)npair(inst[k].O); ++p)
       for (int32_t ta = 0; ta < nt; ++ta)
         for (int32_t tb = ta; tb < nt; ++tb) it.push_back(Item{k, p, ta, tb});
+    for (int32_t i = 0; i < (int32_t)inst[k].O; ++i)
+      for (int64_t p0 = 0; p0 < Pv; p0 += ZS) zit.push_back(ZItem{k, i, (int32_t)p0, 0});
+    zoff[k + 1] = zoff[k] + inst[k].O * inst[k].O * Pv;
   }
   off[n] = (int64_t)it.size();
-  Header h{n, items};
+  Header h{n, items, zitems, (int64_t)L.zitem, (int64_t)L.z};
   char* w = (char*)workspace;
   hipStream_t st = (hipStream_t)stream;
   // pageable sources: each copy returns once the host buffer has been read
@@ -341,6 +428,9 @@ int apy_mp2_nc_batched_setup(int n, const apy_mp2_nc_instance* inst, void* works
   APY_HIP_TRY(hipMemcpyAsync(w + L.inst, inst, n * sizeof(apy_mp2_nc_instance), hipMemcpyHostToDevice, st));
   APY_HIP_TRY(hipMemcpyAsync(w + L.item, it.data(), items * sizeof(Item), hipMemcpyHostToDevice, st));
   APY_HIP_TRY(hipMemcpyAsync(w + L.off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  APY_HIP_TRY(hipMemcpyAsync(w + L.zoff, zoff.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  if (zitems > 0)
+    APY_HIP_TRY(hipMemcpyAsync(w + L.zitem, zit.data(), zitems * sizeof(ZItem), hipMemcpyHostToDevice, st));
   APY_HIP_TRY(hipStreamSynchronize(st));
   *n_items = items;
   return APY_OK;
@@ -352,8 +442,12 @@ int apy_mp2_nc_batched_step(int n, int64_t n_items, int reverse, void* rnorm, vo
   char* w = (char*)workspace;
   hipStream_t st = (hipStream_t)stream;
   double* part = (double*)(w + L.part);
-  ncp_step_kernel<<<(unsigned)n_items, BLK, 0, st>>>((const apy_mp2_nc_instance*)(w + L.inst),
-                                                     (const Item*)(w + L.item), reverse, part);
+  const auto* insts = (const apy_mp2_nc_instance*)(w + L.inst);
+  const int64_t* zoff = (const int64_t*)(w + L.zoff);
+  // the Z work list is grid-strided over as many workgroups as the step has items
+  nc_z_kernel<<<(unsigned)n_items, BLK, 0, st>>>(w, insts, zoff, reverse);
+  APY_LAUNCH_CHECK();
+  ncp_step_kernel<<<(unsigned)n_items, BLK, 0, st>>>(insts, (const Item*)(w + L.item), reverse, w, zoff, part);
   APY_LAUNCH_CHECK();
   per_instance_sum<<<(unsigned)n, BLK, 0, st>>>((const int64_t*)(w + L.off), part, (double*)rnorm);
   APY_LAUNCH_CHECK();
