@@ -268,6 +268,14 @@ SIGNATURES = {
     "apy_cisd_sp_energy_batch": (c_int, [c_i64, c_i64, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_cisd_sp_multi_workspace": (c_size, [c_i64, c_i64, c_int, c_i64]),
     "apy_cisd_sp_step_multi": (c_int, [ctypes.POINTER(CisdSpArgs), c_i64, c_vp, c_size, c_vp]),
+    "apy_cisd_sp_tune": (c_int, [ctypes.POINTER(CisdSpArgs), c_i64, c_int, c_vp, c_size, c_int, c_vp]),
+    "apy_mp2_sp_response_workspace": (c_size, [c_i64, c_i64, c_i64]),
+    "apy_mp2_sp_response": (c_int, [c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_mp2_sp_densities_workspace": (c_size, [c_i64, c_i64]),
+    "apy_mp2_sp_densities": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_mp2_sp_directional_workspace": (c_size, [c_i64]),
+    "apy_mp2_sp_directional": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_size,
+                                       c_vp]),
     "apy_cisd_sp_energy_multi": (c_int, [c_i64, c_i64, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_cisd_so_tpd_workspace": (c_size, [c_i64, c_i64]),
     "apy_cisd_so_tpd": (c_int, [c_i64, c_i64, c_vp, ctypes.POINTER(SoInts), c_vp, c_size, c_vp]),

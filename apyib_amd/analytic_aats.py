@@ -32,6 +32,7 @@ from .hf_wfn import hf_wfn
 from .ci_wfn import ci_wfn
 from .kernels import zgemm_raw
 from .mp2_wfn import mp2_wfn
+from . import mp2_grad
 from .spatial_ci import ResponseSet, SPIntegrals
 from .utils import device, get_slices
 from . import _lib
@@ -366,31 +367,39 @@ class analytic_derivative(object):
         o, v, eps = self.o, self.v, self.eps
         f_, o_, v_, t_ = self.C_list
         eo, ev = eps[o_], eps[v_]
-        D = eo.reshape(-1, 1, 1, 1) + eo.reshape(-1, 1, 1) - ev.reshape(-1, 1) - ev
         tt = 2 * t2 - t2.swapaxes(2, 3)
         N = 1.0 if normalization == 'intermediate' else 1 / np.sqrt(1 + _c('ijab,ijab', t2, tt))
+        # first-order amplitudes on the device (apy_mp2_sp_response): the Fock terms in the
+        # magnetic order (analytic_aats.py:346-350, form 1) and the nuclear order (:446-451, form 0)
+        act = np.r_[np.arange(eps.shape[0])[o_], np.arange(eps.shape[0])[v_]]
+        eps_act = np.concatenate([eo, ev])
+        t2_dev = tr.to_dev(np.asarray(t2, dtype=np.complex128), self.dev)
+        act_dev = torch.as_tensor(act, device=self.dev)
+        no = eo.shape[0]
+
+        def first_order(dE_oovv, df, form):
+            f_act = tr.to_dev(np.ascontiguousarray(df[np.ix_(act, act)]).astype(np.complex128), self.dev)
+            dt = mp2_grad.t2_response(t2_dev, f_act, dE_oovv, eps_act, form=form)
+            return np.real_if_close(dt.cpu().numpy())
+
+        def oovv(X):   # X[p,q,r,s] over all orbitals -> [o_, o_, v_, v_] of the active ones
+            Xa = X.index_select(0, act_dev).index_select(1, act_dev).index_select(2, act_dev)
+            return Xa.index_select(3, act_dev)[:no, :no, no:, no:]
+
         U_H, dT2_dH = [], []
         for b in range(3):
             U, h_core = self._magnetic(b, orbitals)
             df = self._df_magnetic(U, h_core)
-            dE = np.real_if_close(_dERI(tr.to_dev(U.astype(np.complex128), self.dev), self.Ed,
-                                        (-1.0, -1.0, 1.0, 1.0)).cpu().numpy())
-            dt = dE.swapaxes(0, 2).swapaxes(1, 3)[o_, o_, v_, v_].astype(np.result_type(dE, t2))
-            dt += _c('ac,ijcb->ijab', df[v_, v_], t2) + _c('bc,ijac->ijab', df[v_, v_], t2)
-            dt -= _c('ki,kjab->ijab', df[o_, o_], t2) + _c('kj,ikab->ijab', df[o_, o_], t2)
+            dE = _dERI(tr.to_dev(U.astype(np.complex128), self.dev), self.Ed, (-1.0, -1.0, 1.0, 1.0))
             U_H.append(U)
-            dT2_dH.append(dt / D)
+            dT2_dH.append(first_order(oovv(dE.permute(2, 3, 0, 1)), df, 1))
         AAT = np.zeros((self.natom * 3, 3))
         for N1 in range(self.natom):
             for a in range(3):
                 U_R, F1, S1, E1d, half = self._nuclear(N1, a, orbitals)
                 df = self._df_nuclear(U_R, F1, S1)
-                dE = np.real_if_close((E1d + _dERI(tr.to_dev(U_R.astype(np.complex128), self.dev), self.Ed,
-                                                   (1.0, 1.0, 1.0, 1.0))).cpu().numpy())
-                dt = dE[o_, o_, v_, v_].astype(np.result_type(dE, t2))
-                dt -= _c('kjab,ik->ijab', t2, df[o_, o_]) + _c('ikab,kj->ijab', t2, df[o_, o_])
-                dt += _c('ijcb,ac->ijab', t2, df[v_, v_]) + _c('ijac,cb->ijab', t2, df[v_, v_])
-                dt = dt / D
+                dE = E1d + _dERI(tr.to_dev(U_R.astype(np.complex128), self.dev), self.Ed, (1.0, 1.0, 1.0, 1.0))
+                dt = first_order(oovv(dE), df, 0)
                 N_R = -(1 / np.sqrt((1 + _c('ijab,ijab', np.conj(t2), tt)) ** 3))
                 N_R *= 0.5 * (_c('ijab,ijab', np.conj(dt), tt) + _c('ijab,ijab', dt, np.conj(tt)))
                 R = U_R + half.T

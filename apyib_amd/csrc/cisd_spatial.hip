@@ -19,8 +19,11 @@
 //     X1 = T W2 - U W1,  W2[(k,c),(j,b)] = 2<kb|cj> - <kb|jc>, W1 = <kb|cj>,
 //     X2 = U W3,         W3[(k,c),(j,a)] = <ka|jc>,
 //     r += X1[(ia),(jb)] + X1[(jb),(ia)] - X2[(ib),(ja)] - X2[(ja),(ib)];
-//   singles into doubles: Q1[i][(a,b,j)] = sum_c <ab|cj> t_ic,
-//     Q2[(a,i,j)][b] = sum_k <ak|ij> t_kb,  r += Q1 - Q2 + images.
+//   singles into doubles: Q1[i][(j,a,b)] = sum_c <ab|cj> t_ic,
+//     Q2[(i,j,a)][b] = sum_k <ak|ij> t_kb,  r += Q1 - Q2 + images.
+// The epilogue (sp_epi_doubles) reads every operand along b, the contiguous
+// index of the output, except the pair images, which it sums along a into an
+// LDS tile and reads back transposed.
 // Each product is one complex GEMM on the FP64 matrix cores (zgemm.hip);
 // the two ladder GEMMs dominate.  The singles residual (ci_wfn.py:
 // 457-464) is six GEMMs into one accumulator; the energy (:495) one
@@ -129,13 +132,13 @@ __global__ __launch_bounds__(256) void sp_build_kernel(Dirac g, int kind, int64_
         val = d2lin(2.0, g(i, j, o + a, o + b), -1.0, g(i, j, o + b, o + a));
         break;
       }
-      case S_VVVO: {  // [(a,b,j)][c] = <ab|cj>
-        const int64_t c = t % v, j = (t / v) % o, b = (t / ov) % v, a = t / (ov * v);
+      case S_VVVO: {  // [(j,a,b)][c] = <ab|cj>
+        const int64_t c = t % v, b = (t / v) % v, a = (t / (v * v)) % v, j = t / (v * v * v);
         val = g(o + a, o + b, o + c, j);
         break;
       }
-      case S_VOOO: {  // [(a,i,j)][k] = <ak|ij>
-        const int64_t k = t % o, j = (t / o) % o, i = (t / (o * o)) % o, a = t / (o * o * o);
+      case S_VOOO: {  // [(i,j,a)][k] = <ak|ij>
+        const int64_t k = t % o, a = (t / o) % v, j = (t / ov) % o, i = t / (ov * o);
         val = g(o + a, k, i, j);
         break;
       }
@@ -219,7 +222,7 @@ __global__ __launch_bounds__(256) void sp_prepare_kernel(int64_t o, int64_t v, c
   }
 }
 
-// TU[q][(i,a)][(k,c)] = t_ikac, TU[q][(i,a)][ov + (k,c)] = t_ikca for the nb vectors (stride L)
+// TU[q][(i,a)][(k,c)] = t_ikac, TU[q][(i,a)][ov + (k,c)] = t_ikca = t_kiac for the nb vectors (stride L)
 __global__ __launch_bounds__(256) void sp_tu_kernel(int64_t o, int64_t v, int64_t nb, int64_t L,
                                                     const double2* __restrict__ tv, double2* __restrict__ TU) {
   const int64_t ov = o * v, per = 2 * ov * ov, total = nb * per;
@@ -229,7 +232,8 @@ __global__ __launch_bounds__(256) void sp_tu_kernel(int64_t o, int64_t v, int64_
     const int64_t i = row / v, a = row % v;
     const int64_t cc = col % ov, k = cc / v, c = cc % v;
     const double2* t2 = tv + q * L + ov;
-    TU[u] = col < ov ? t2[((i * o + k) * v + a) * v + c] : t2[((i * o + k) * v + c) * v + a];
+    // t_ikca = t_kiac by the pair symmetry: both halves read along c
+    TU[u] = col < ov ? t2[((i * o + k) * v + a) * v + c] : t2[((k * o + i) * v + a) * v + c];
   }
 }
 
@@ -266,54 +270,108 @@ __device__ __forceinline__ double2 resp_scalar(const double2* re, double2 rs, co
   return rs;
 }
 
+constexpr int SET = 32;  // edge of the (a, b) tile of the doubles epilogue
+
+// The doubles epilogue over 32 x 32 tiles of (a, b) for one (instance, i, j): the terms whose
+// operands are stored (b, a)-transposed (the pair images of Y, X1, Q1, Q2, the X2 term and the
+// a > b half of the ladder) are summed while reading them along a, into one LDS tile, and read
+// back along b; every other operand is read along b directly.  All global reads and writes are
+// along the contiguous index.
 __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
-  const int64_t o = e.o, v = e.v, ov = o * v, per = ov * ov, total = e.nb * per;
-  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < total; u += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t q = u / per, t = u % per, qd = q * per, qv = q * e.L;
-    const int64_t b = t % v, a = (t / v) % v, j = (t / (v * v)) % o, i = t / (v * v * o);
-    const double2 E = e.energy[q];
-    double2 r = e.vvoo[q * e.s_vvoo + t];
-    const double2 p = e.R[u];
-    r.x += p.x; r.y += p.y;
-    {  // ladder: P+ + s_ij s_ab P-
-      const int64_t i0 = i < j ? i : j, j0 = i < j ? j : i, a0 = a < b ? a : b, b0 = a < b ? b : a;
-      const double2 ps = e.Ps[q * nsym(o) * nsym(v) + isym(i0, j0, o) * nsym(v) + isym(a0, b0, v)];
-      r.x += ps.x; r.y += ps.y;
-      if (i != j && a != b) {
-        const double2 pa = e.Pa[q * nanti(o) * nanti(v) + ianti(i0, j0, o) * nanti(v) + ianti(a0, b0, v)];
-        const double sg = ((i < j) == (a < b)) ? 1.0 : -1.0;
-        r.x += sg * pa.x; r.y += sg * pa.y;
-      }
-    }
+  __shared__ double2 T[SET][SET + 1];  // T[b - b0][a - a0]: the transposed terms of output (a, b)
+  const int64_t o = e.o, v = e.v, ov = o * v, vv = v * v, per = ov * ov;
+  const int64_t nt = (v + SET - 1) / SET, tiles = e.nb * o * o * nt * nt;
+  const int64_t so = nsym(o), sv = nsym(v), ao = nanti(o), av = nanti(v);
+  const int tx = threadIdx.x % SET, ty = threadIdx.x / SET;
+  for (int64_t w = blockIdx.x; w < tiles; w += gridDim.x) {
+    int64_t r = w;
+    const int64_t tb = r % nt; r /= nt;
+    const int64_t ta = r % nt; r /= nt;
+    const int64_t j = r % o; r /= o;
+    const int64_t i = r % o;
+    const int64_t q = r / o, qd = q * per, qv = q * e.L;
+    const int64_t a0 = ta * SET, b0 = tb * SET;
+    const int64_t i0 = i < j ? i : j, j0 = i < j ? j : i;
+    const double2* Ps = e.Ps + q * so * sv + isym(i0, j0, o) * sv;
+    const double2* Pa = (i != j) ? e.Pa + q * ao * av + ianti(i0, j0, o) * av : nullptr;
+    const double sgn_lower = i < j ? -1.0 : 1.0;  // sign of P- at a > b
     const double2* Y = e.Y + qd;
-    const double2 y1 = Y[t], y2 = Y[((j * o + i) * v + b) * v + a];
-    r.x += y1.x + y2.x; r.y += y1.y + y2.y;
     const double2* X1 = e.X1 + qd;
     const double2* X2 = e.X2 + qd;
-    const double2 x1 = X1[(i * v + a) * ov + j * v + b], x1t = X1[(j * v + b) * ov + i * v + a];
-    const double2 x2 = X2[(i * v + b) * ov + j * v + a], x2t = X2[(j * v + a) * ov + i * v + b];
-    r.x += x1.x + x1t.x - x2.x - x2t.x;
-    r.y += x1.y + x1t.y - x2.y - x2t.y;
-    if (e.Q1) {
-      const double2* Q1 = e.Q1 + qd;
-      const double2* Q2 = e.Q2 + qd;
-      const double2 q1 = Q1[i * (v * ov) + (a * v + b) * o + j], q1t = Q1[j * (v * ov) + (b * v + a) * o + i];
-      const double2 q2 = Q2[((a * o + i) * o + j) * v + b], q2t = Q2[((b * o + j) * o + i) * v + a];
-      r.x += q1.x + q1t.x - q2.x - q2t.x;
-      r.y += q1.y + q1t.y - q2.y - q2t.y;
+    const double2* Q1 = e.Q1 ? e.Q1 + qd : nullptr;
+    const double2* Q2 = e.Q2 ? e.Q2 + qd : nullptr;
+    __syncthreads();  // T of the previous tile is no longer read
+#pragma unroll
+    for (int k = 0; k < SET / 8; ++k) {  // element (b, a) = (b0 + ty + 8k, a0 + tx), read along a
+      const int64_t b = b0 + ty + 8 * k, a = a0 + tx;
+      double2 s = make_double2(0.0, 0.0);
+      if (a < v && b < v) {
+        const double2 y2 = Y[((j * o + i) * v + b) * v + a];
+        const double2 x1t = X1[(j * v + b) * ov + i * v + a];
+        const double2 x2 = X2[(i * v + b) * ov + j * v + a];
+        s.x = y2.x + x1t.x - x2.x;
+        s.y = y2.y + x1t.y - x2.y;
+        if (Q1) {
+          const double2 q1t = Q1[j * (o * vv) + (i * v + b) * v + a];
+          const double2 q2t = Q2[((j * o + i) * v + b) * v + a];
+          s.x += q1t.x - q2t.x;
+          s.y += q1t.y - q2t.y;
+        }
+        if (b < a) {
+          const double2 ps = Ps[isym(b, a, v)];
+          s.x += ps.x; s.y += ps.y;
+          if (Pa) {
+            const double2 pa = Pa[ianti(b, a, v)];
+            s.x += sgn_lower * pa.x; s.y += sgn_lower * pa.y;
+          }
+        }
+      }
+      T[ty + 8 * k][tx] = s;
     }
-    const double2 x = e.t2[qv + t];
-    r.x -= E.x * x.x - E.y * x.y;
-    r.y -= E.x * x.y + E.y * x.x;
-    if (e.rb) {
-      const double2 s = resp_scalar(e.re, e.rs, e.es, q), bb = e.rb[qv + t], y = e.rt[t];
-      r.x += bb.x - (s.x * y.x - s.y * y.y);
-      r.y += bb.y - (s.x * y.y + s.y * y.x);
-    }
+    __syncthreads();
+    const double2 E = e.energy[q];
     const double* ep = e.eps + q * e.s_eps;
-    const double D = ep[i] + ep[j] - ep[o + a] - ep[o + b];
-    e.r2[qv + t] = r;
-    e.t2_new[qv + t] = make_double2(x.x + r.x / D, x.y + r.y / D);
+    const double eij = ep[i] + ep[j];
+    const double2 rsc = e.rb ? resp_scalar(e.re, e.rs, e.es, q) : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < SET / 8; ++k) {  // output (a, b) = (a0 + ty + 8k, b0 + tx), read along b
+      const int64_t a = a0 + ty + 8 * k, b = b0 + tx;
+      if (a >= v || b >= v) continue;
+      const int64_t t = ((i * o + j) * v + a) * v + b;
+      double2 rr = e.vvoo[q * e.s_vvoo + t];
+      const double2 tt = T[tx][ty + 8 * k];
+      const double2 p = e.R[qd + t];
+      const double2 y1 = Y[t];
+      const double2 x1 = X1[(i * v + a) * ov + j * v + b];
+      const double2 x2t = X2[(j * v + a) * ov + i * v + b];
+      rr.x += tt.x + p.x + y1.x + x1.x - x2t.x;
+      rr.y += tt.y + p.y + y1.y + x1.y - x2t.y;
+      if (Q1) {
+        const double2 q1 = Q1[i * (o * vv) + (j * v + a) * v + b];
+        const double2 q2 = Q2[t];
+        rr.x += q1.x - q2.x;
+        rr.y += q1.y - q2.y;
+      }
+      if (a <= b) {
+        const double2 ps = Ps[isym(a, b, v)];
+        rr.x += ps.x; rr.y += ps.y;
+        if (Pa && a < b) {
+          const double2 pa = Pa[ianti(a, b, v)];
+          rr.x -= sgn_lower * pa.x; rr.y -= sgn_lower * pa.y;
+        }
+      }
+      const double2 x = e.t2[qv + t];
+      rr.x -= E.x * x.x - E.y * x.y;
+      rr.y -= E.x * x.y + E.y * x.x;
+      if (e.rb) {
+        const double2 bb = e.rb[qv + t], y = e.rt[t];
+        rr.x += bb.x - (rsc.x * y.x - rsc.y * y.y);
+        rr.y += bb.y - (rsc.x * y.y + rsc.y * y.x);
+      }
+      const double D = eij - ep[o + a] - ep[o + b];
+      e.r2[qv + t] = rr;
+      e.t2_new[qv + t] = make_double2(x.x + rr.x / D, x.y + rr.y / D);
+    }
   }
 }
 
@@ -592,7 +650,8 @@ int sp_step_impl(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, 
           t + ov, (const double2*)A->energy, (double2*)A->t_new + ov, (double2*)A->out + ov,
           rb ? rb + ov : nullptr, rb ? rt + ov : nullptr, re, rs, es, multi ? S.eps : 0,
           multi ? S.blk[S_VVOO] : 0};
-  sp_epi_doubles<<<apy_grid1d(nb * ov * ov), 256, 0, st>>>(e);
+  const int64_t nt = (v + SET - 1) / SET;
+  sp_epi_doubles<<<(unsigned)std::min<int64_t>(nb * o * o * nt * nt, 1 << 20), 256, 0, st>>>(e);
   APY_LAUNCH_CHECK();
   if (singles) {
     sp_epi_singles<<<apy_grid1d(nb * ov), 256, 0, st>>>(o, v, nb, L, A->eps, (const double2*)A->fock,
@@ -691,6 +750,30 @@ int apy_cisd_sp_energy(int64_t o, int64_t v, int singles, const void* g, const v
 int apy_cisd_sp_energy_batch(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L,
                              const void* t, void* energy, void* workspace, size_t ws_bytes, void* stream) {
   return sp_energy_impl(o, v, singles, nb, g, L, t, energy, workspace, ws_bytes, stream, 0, 0);
+}
+
+int apy_cisd_sp_tune(const apy_cisd_sp_args* A, int64_t nb, int multi, void* workspace, size_t ws_bytes, int reps,
+                     void* stream) {
+  if (!A || !A->ints || !A->fock || !A->hoo || !A->g || !A->t || nb < 1) return APY_E_ARG;
+  const int64_t o = A->o, v = A->v;
+  const apy_sp_ints* I = A->ints;
+  if (o < 1 || v < 1 || I->o != o || I->v != v) return APY_E_ARG;
+  const int singles = A->singles ? 1 : 0;
+  if (!I->vvvs || (v > 1 && !I->vvva) || !I->w21 || !I->w3) return APY_E_ARG;
+  if (singles && (!I->vvvo || !I->vooo || !I->s1 || !I->w4 || !I->w5)) return APY_E_ARG;
+  const SpWS w = sp_carve(o, v, singles, nb, multi != 0);
+  if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
+  const SpInst S = sp_inst_strides(o, v);
+  const int64_t ov = o * v, L = ov + ov * ov;
+  char* base = (char*)workspace;
+  apy_gemm_desc g[NSPG];
+  const int q = sp_gemms(o, v, singles, nb, L, A, base, &w, g, multi ? &S : nullptr);
+  for (int k = 0; k < q; ++k) {
+    const int s = apy_zgemm_tune_impl(&g[k], base + w.gemm, ws_bytes - w.gemm, reps, (hipStream_t)stream, nullptr,
+                                      nullptr, nullptr);
+    if (s != APY_OK) return s;
+  }
+  return APY_OK;
 }
 
 int apy_cisd_sp_energy_multi(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L,

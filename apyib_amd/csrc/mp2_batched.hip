@@ -31,20 +31,24 @@
 // HIP graph.
 #include "common.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace {
 
 constexpr int BLK = 256;
 constexpr int MT = 32;       // virtual tile edge
-constexpr int MKC = 16;      // e chunk of the Y product
+constexpr int MKC = 8;       // e chunk of the Y product
+constexpr int RM = 4, RN = 2;                      // rows a x columns b per thread in the step kernel
+constexpr int SBX = MT / RN, SBY = MT / RM;        // thread grid of the step kernel (b fastest)
+constexpr int SBLK = SBX * SBY;                    // step kernel workgroup size
 constexpr int EN_B = 64;     // energy partial sums per instance
 
 struct Item {
   int32_t inst, pij, ta, tb;
 };
 
-constexpr int ZS = 256;      // (a<b) elements per Z work item
+constexpr int ZS = 2 * BLK;  // (a<b) elements per Z work item
 constexpr int ZMAX_O = 32;   // occupied orbitals held in registers by the Z kernel
 
 struct ZItem {
@@ -132,6 +136,44 @@ __device__ inline double2 xget(const double2* x, int64_t O, int64_t Pv, int64_t 
   return ((p < q) == (c < d)) ? v : make_double2(-v.x, -v.y);
 }
 
+// Z of one work item: NP elements P per thread (p0 + tid + BLK q), the O <= MO amplitudes
+// x_im of each in registers, so every Foo(m, j) read from LDS feeds NP multiply-adds.
+template <int MO, int NP>
+__device__ __forceinline__ void z_item(const double2* __restrict__ fo, const double2* __restrict__ x, double2* Z,
+                                       int64_t O, int64_t Pv, int64_t i, int64_t p0) {
+  double2 xv[NP][MO];
+  bool ok[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int64_t P = p0 + threadIdx.x + (int64_t)BLK * q;
+    ok[q] = P < Pv;
+#pragma unroll
+    for (int m = 0; m < MO; ++m) {
+      double2 v = make_double2(0.0, 0.0);
+      if (ok[q] && m < O && m != i) {   // x_im: +row (i,m) for i < m, -row (m,i) for m < i
+        v = x[pidx(i < m ? i : m, i < m ? m : i, O) * Pv + P];
+        if (m < i) v = make_double2(-v.x, -v.y);
+      }
+      xv[q][m] = v;
+    }
+  }
+  for (int64_t j = 0; j < O; ++j) {
+    double2 z[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) z[q] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int m = 0; m < MO; ++m)
+      if (m < O) {
+        const double2 f = fo[m * O + j];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) cfma(z[q], f, xv[q][m]);
+      }
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+      if (ok[q]) Z[j * Pv + p0 + threadIdx.x + (int64_t)BLK * q] = z[q];
+  }
+}
+
 // Z_k[i][j][(ab)] = sum_m Foo(m,j) x_im[(ab)] for every instance, row i and slice of
 // ZS pairs (ab) (grid-stride over the Z work list, whose length is in the header).
 __global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const apy_mp2_nc_instance* __restrict__ insts,
@@ -149,52 +191,53 @@ __global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const 
     const double2* F = (const double2*)I.fock;
     const double2* x = (const double2*)(reverse ? I.x_new : I.x);
     double2* Z = zbase + zoff[it.inst] + i * O * Pv;
-    const int64_t P = it.p0 + threadIdx.x;
     auto foo = [&](int64_t m, int64_t j) { return transpose_f ? F[j * n + m] : F[m * n + j]; };
-    // x_im[P]: +row (i,m) for i < m, -row (m,i) for m < i, zero for m == i
-    auto xim = [&](int64_t m) -> double2 {
-      if (m == i) return make_double2(0.0, 0.0);
-      const double2 v = x[pidx(i < m ? i : m, i < m ? m : i, O) * Pv + P];
-      return i < m ? v : make_double2(-v.x, -v.y);
-    };
     if (O <= ZMAX_O) {
       __syncthreads();                                   // fo of the previous item is no longer read
       for (int64_t q = threadIdx.x; q < O * O; q += BLK) fo[q] = foo(q / O, q % O);   // fo[m O + j]
       __syncthreads();
-      if (P < Pv) {
-        double2 xv[ZMAX_O];
-#pragma unroll
-        for (int m = 0; m < ZMAX_O; ++m) xv[m] = m < O ? xim(m) : make_double2(0.0, 0.0);
+      if (O <= 16)
+        z_item<16, 2>(fo, x, Z, O, Pv, i, it.p0);
+      else if (O <= 24)
+        z_item<24, 2>(fo, x, Z, O, Pv, i, it.p0);
+      else {
+        z_item<ZMAX_O, 1>(fo, x, Z, O, Pv, i, it.p0);
+        z_item<ZMAX_O, 1>(fo, x, Z, O, Pv, i, it.p0 + BLK);
+      }
+    } else {
+      for (int q = 0; q < ZS / BLK; ++q) {
+        const int64_t P = it.p0 + threadIdx.x + (int64_t)BLK * q;
+        if (P >= Pv) continue;
         for (int64_t j = 0; j < O; ++j) {
           double2 z = make_double2(0.0, 0.0);
-#pragma unroll
-          for (int m = 0; m < ZMAX_O; ++m)
-            if (m < O) cfma(z, fo[m * O + j], xv[m]);
+          for (int64_t m = 0; m < O; ++m) {
+            if (m == i) continue;
+            double2 v = x[pidx(i < m ? i : m, i < m ? m : i, O) * Pv + P];
+            if (m < i) v = make_double2(-v.x, -v.y);
+            cfma(z, foo(m, j), v);
+          }
           Z[j * Pv + P] = z;
         }
-      }
-    } else if (P < Pv) {
-      for (int64_t j = 0; j < O; ++j) {
-        double2 z = make_double2(0.0, 0.0);
-        for (int64_t m = 0; m < O; ++m) cfma(z, foo(m, j), xim(m));
-        Z[j * Pv + P] = z;
       }
     }
   }
 }
 
-__global__ __launch_bounds__(BLK) void ncp_step_kernel(const apy_mp2_nc_instance* __restrict__ insts,
-                                                       const Item* __restrict__ items, int reverse,
-                                                       const char* __restrict__ ws, const int64_t* __restrict__ zoff,
-                                                       double* __restrict__ part) {
+// One work item per workgroup of SBLK threads: a 32 x 32 tile of the residual, each thread a
+// 4 x 2 block of (a, b) for both products, so every LDS value a thread reads feeds 2 or 4
+// complex multiply-adds.
+__global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instance* __restrict__ insts,
+                                                        const Item* __restrict__ items, int reverse,
+                                                        const char* __restrict__ ws, const int64_t* __restrict__ zoff,
+                                                        double* __restrict__ part) {
   __shared__ double2 XA[MT][MKC + 1], XB[MT][MKC + 1], GA[MKC][MT + 1], GB[MKC][MT + 1];
-  __shared__ double red[BLK];
+  __shared__ double red[SBLK];
   const Item it = items[blockIdx.x];
   const apy_mp2_nc_instance I = insts[it.inst];
   const int64_t O = I.O, V = I.V, n = O + V, Pv = npair(V);
   const int transpose_f = I.transpose_f;
   const double s = I.drv_scale;
-  const int tid = threadIdx.x, tx = tid & (MT - 1), ty = tid >> 5;
+  const int tid = threadIdx.x, tx = tid % SBX, ty = tid / SBX;
   int64_t i, j;
   pdec(it.pij, O, i, j);
   const double2* F = (const double2*)I.fock;
@@ -202,39 +245,71 @@ __global__ __launch_bounds__(BLK) void ncp_step_kernel(const apy_mp2_nc_instance
   double2* xn = (double2*)(reverse ? I.x : I.x_new);
   const double2* row = x + (int64_t)it.pij * Pv;
   const int64_t a0 = (int64_t)it.ta * MT, b0 = (int64_t)it.tb * MT;
-  // x_ij[p, q] from the pair's packed row
-  auto xr = [&](int64_t p, int64_t q) -> double2 {
-    if (p == q) return make_double2(0.0, 0.0);
-    const double2 v = row[pidx(p < q ? p : q, p < q ? q : p, V)];
-    return p < q ? v : make_double2(-v.x, -v.y);
-  };
   // Fvv(b, e) (F^T: F[O+e][O+b])
   auto fvv = [&](int64_t b, int64_t e) { return transpose_f ? F[(O + e) * n + O + b] : F[(O + b) * n + O + e]; };
 
-  double2 yab[MT / 8], yba[MT / 8];
-  for (int q = 0; q < MT / 8; ++q) { yab[q] = make_double2(0.0, 0.0); yba[q] = make_double2(0.0, 0.0); }
-  for (int64_t e0 = 0; e0 < V; e0 += MKC) {
-    __syncthreads();
-    for (int q = tid; q < MT * MKC; q += BLK) {
+  double2 yab[RM][RN], yba[RM][RN];
+#pragma unroll
+  for (int r = 0; r < RM; ++r)
+#pragma unroll
+    for (int c = 0; c < RN; ++c) { yab[r][c] = make_double2(0.0, 0.0); yba[r][c] = make_double2(0.0, 0.0); }
+  // one e chunk of x_ij rows a0.., b0.. and of Fvv columns, fetched into registers while the previous
+  // chunk is multiplied (NQ elements of each of the four tiles per thread), then stored to LDS
+  constexpr int NQ = MT * MKC / SBLK;
+  const int Vi = (int)V;
+  const uint32_t V2 = 2u * (uint32_t)Vi;
+  // x_ij[p, q] (p, q < V) from the pair's packed row, 32-bit index arithmetic
+  auto xr32 = [&](int p, int q) -> double2 {
+    if (p == q) return make_double2(0.0, 0.0);
+    const int lo = p < q ? p : q, hi = p < q ? q : p;
+    const double2 v = row[(uint32_t)lo * (V2 - (uint32_t)lo - 1u) / 2u + (uint32_t)(hi - lo - 1)];
+    return p < q ? v : make_double2(-v.x, -v.y);
+  };
+  double2 sxa[NQ], sxb[NQ], sga[NQ], sgb[NQ];
+  auto fetch = [&](int e0) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const int q = tid + SBLK * k;
       const int r = q / MKC, c = q % MKC;            // XA[r][c] = x_ij[a0+r, e0+c]
-      const int64_t e = e0 + c;
-      XA[r][c] = (a0 + r < V && e < V) ? xr(a0 + r, e) : make_double2(0.0, 0.0);
-      XB[r][c] = (b0 + r < V && e < V) ? xr(b0 + r, e) : make_double2(0.0, 0.0);
-      const int rr = q / MT, cc = q % MT;            // GA[rr][cc] = Fvv(a0+cc, e0+rr)
-      const int64_t e2 = e0 + rr;
-      GA[rr][cc] = (e2 < V && a0 + cc < V) ? fvv(a0 + cc, e2) : make_double2(0.0, 0.0);
-      GB[rr][cc] = (e2 < V && b0 + cc < V) ? fvv(b0 + cc, e2) : make_double2(0.0, 0.0);
+      const int e = e0 + c, ra = (int)a0 + r, rb = (int)b0 + r;
+      sxa[k] = (ra < Vi && e < Vi) ? xr32(ra, e) : make_double2(0.0, 0.0);
+      sxb[k] = (rb < Vi && e < Vi) ? xr32(rb, e) : make_double2(0.0, 0.0);
+      // GA[rr][cc] = Fvv(a0+cc, e0+rr), read along the contiguous index of F (e, or b for F^T)
+      const int rr = transpose_f ? q / MT : q % MKC, cc = transpose_f ? q % MT : q / MKC;
+      const int e2 = e0 + rr, ca = (int)a0 + cc, cb = (int)b0 + cc;
+      sga[k] = (e2 < Vi && ca < Vi) ? fvv(ca, e2) : make_double2(0.0, 0.0);
+      sgb[k] = (e2 < Vi && cb < Vi) ? fvv(cb, e2) : make_double2(0.0, 0.0);
+    }
+  };
+  fetch(0);
+  for (int e0 = 0; e0 < Vi; e0 += MKC) {
+    __syncthreads();                                 // the previous chunk is no longer read
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const int q = tid + SBLK * k;
+      const int r = q / MKC, c = q % MKC;
+      XA[r][c] = sxa[k];
+      XB[r][c] = sxb[k];
+      const int rr = transpose_f ? q / MT : q % MKC, cc = transpose_f ? q % MT : q / MKC;
+      GA[rr][cc] = sga[k];
+      GB[rr][cc] = sgb[k];
     }
     __syncthreads();
-#pragma unroll 4
+    if (e0 + MKC < Vi) fetch(e0 + MKC);              // in flight during the products below
+#pragma unroll 2
     for (int le = 0; le < MKC; ++le) {
-      const double2 gb = GB[le][tx], xb = XB[tx][le];
+      double2 xa[RM], ga[RM], xb[RN], gb[RN];
 #pragma unroll
-      for (int q = 0; q < MT / 8; ++q) {
-        const int la = ty + 8 * q;
-        cfma(yab[q], XA[la][le], gb);                // sum_e x[a,e] Fvv(b,e)
-        cfma(yba[q], xb, GA[le][la]);                // sum_e x[b,e] Fvv(a,e)
-      }
+      for (int r = 0; r < RM; ++r) { xa[r] = XA[ty + SBY * r][le]; ga[r] = GA[le][ty + SBY * r]; }
+#pragma unroll
+      for (int c = 0; c < RN; ++c) { xb[c] = XB[tx + SBX * c][le]; gb[c] = GB[le][tx + SBX * c]; }
+#pragma unroll
+      for (int r = 0; r < RM; ++r)
+#pragma unroll
+        for (int c = 0; c < RN; ++c) {
+          cfma(yab[r][c], xa[r], gb[c]);             // sum_e x[a,e] Fvv(b,e)
+          cfma(yba[r][c], xb[c], ga[r]);             // sum_e x[b,e] Fvv(a,e)
+        }
     }
   }
 
@@ -243,23 +318,26 @@ __global__ __launch_bounds__(BLK) void ncp_step_kernel(const apy_mp2_nc_instance
   const double2* Z = (const double2*)(ws + ((const Header*)ws)->z_off) + zoff[it.inst];
   const double eij = eps[i] + eps[j];
   double acc = 0.0;
-  for (int q = 0; q < MT / 8; ++q) {
-    const int64_t a = a0 + ty + 8 * q, b = b0 + tx;
-    if (b >= V || a >= b) continue;
-    const int64_t Pab = pidx(a, b, V);
-    const double2 d = drv[((i * O + j) * V + a) * V + b];
-    // -Z_ij + Z_ji: -sum_m Foo(m,j) x_im[a,b] + sum_m Foo(m,i) x_jm[a,b]
-    const double2 zij = Z[(i * O + j) * Pv + Pab], zji = Z[(j * O + i) * Pv + Pab];
-    const double2 r = make_double2(s * d.x + yab[q].x - yba[q].x - zij.x + zji.x,
-                                   s * d.y + yab[q].y - yba[q].y - zij.y + zji.y);
-    const double D = eij - eps[O + a] - eps[O + b];
-    const double2 xv = row[Pab];
-    xn[(int64_t)it.pij * Pv + Pab] = make_double2(xv.x + r.x / D, xv.y + r.y / D);
-    acc += r.x * r.x + r.y * r.y;
-  }
+#pragma unroll
+  for (int r = 0; r < RM; ++r)
+#pragma unroll
+    for (int c = 0; c < RN; ++c) {
+      const int64_t a = a0 + ty + SBY * r, b = b0 + tx + SBX * c;
+      if (b >= V || a >= b) continue;
+      const int64_t Pab = pidx(a, b, V);
+      const double2 d = drv[((i * O + j) * V + a) * V + b];
+      // -Z_ij + Z_ji: -sum_m Foo(m,j) x_im[a,b] + sum_m Foo(m,i) x_jm[a,b]
+      const double2 zij = Z[(i * O + j) * Pv + Pab], zji = Z[(j * O + i) * Pv + Pab];
+      const double2 rr = make_double2(s * d.x + yab[r][c].x - yba[r][c].x - zij.x + zji.x,
+                                      s * d.y + yab[r][c].y - yba[r][c].y - zij.y + zji.y);
+      const double D = eij - eps[O + a] - eps[O + b];
+      const double2 xv = row[Pab];
+      xn[(int64_t)it.pij * Pv + Pab] = make_double2(xv.x + rr.x / D, xv.y + rr.y / D);
+      acc += rr.x * rr.x + rr.y * rr.y;
+    }
   red[tid] = acc;
   __syncthreads();
-  for (int w = BLK / 2; w > 0; w >>= 1) {
+  for (int w = SBLK / 2; w > 0; w >>= 1) {
     if (tid < w) red[tid] += red[tid + w];
     __syncthreads();
   }
@@ -444,10 +522,10 @@ int apy_mp2_nc_batched_step(int n, int64_t n_items, int reverse, void* rnorm, vo
   double* part = (double*)(w + L.part);
   const auto* insts = (const apy_mp2_nc_instance*)(w + L.inst);
   const int64_t* zoff = (const int64_t*)(w + L.zoff);
-  // the Z work list is grid-strided over as many workgroups as the step has items
-  nc_z_kernel<<<(unsigned)n_items, BLK, 0, st>>>(w, insts, zoff, reverse);
+  // the Z work list is grid-strided over at most 4096 workgroups
+  nc_z_kernel<<<(unsigned)std::min<int64_t>(n_items, 4096), BLK, 0, st>>>(w, insts, zoff, reverse);
   APY_LAUNCH_CHECK();
-  ncp_step_kernel<<<(unsigned)n_items, BLK, 0, st>>>(insts, (const Item*)(w + L.item), reverse, w, zoff, part);
+  ncp_step_kernel<<<(unsigned)n_items, SBLK, 0, st>>>(insts, (const Item*)(w + L.item), reverse, w, zoff, part);
   APY_LAUNCH_CHECK();
   per_instance_sum<<<(unsigned)n, BLK, 0, st>>>((const int64_t*)(w + L.off), part, (double*)rnorm);
   APY_LAUNCH_CHECK();

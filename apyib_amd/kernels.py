@@ -5,6 +5,8 @@ pointers, element strides and the current HIP stream of the tensor's
 device.  No computation happens in Python.
 """
 import ctypes
+import json
+import os
 
 import numpy as np
 import torch
@@ -152,6 +154,33 @@ def plans_export():
     _lib.check(lib.apy_zgemm_tune_export(rows, n.value, ctypes.byref(n)), "apy_zgemm_tune_export")
     names = ("m", "n", "k", "a_layout", "b_layout", "batch", "reduce_batch", "tile", "split")
     return [dict(zip(names, rows[9 * q:9 * q + 9])) for q in range(n.value)]
+
+
+_DEFAULT_PLANS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "plans_gfx950.json")
+_DEFAULT_DONE = set()
+
+
+def ensure_default_plans(device):
+    """Install the saved gfx950 GEMM plans (plans_gfx950.json beside this
+    file, or the file named by APYIB_AMD_PLANS; "none" disables) for
+    ``device``, once per process and device.  Shapes without a saved plan use
+    the cost model."""
+    path = os.environ.get("APYIB_AMD_PLANS", _DEFAULT_PLANS)
+    dev = torch.device(device)
+    if dev.type != "cuda" or path.lower() == "none" or not os.path.exists(path):
+        return
+    idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    if (path, idx) in _DEFAULT_DONE:
+        return
+    with torch.cuda.device(idx):
+        plans_load(json.load(open(path)))
+    _DEFAULT_DONE.add((path, idx))
+
+
+def plans_clear():
+    """Forget every cached GEMM plan (saved or tuned) on every device."""
+    _lib.check(_lib.load().apy_zgemm_tune_clear(), "apy_zgemm_tune_clear")
+    _DEFAULT_DONE.clear()
 
 
 def plans_load(entries):

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .kernels import stream_of, workspace
+from .kernels import ensure_default_plans, stream_of, workspace
 
 CDT = torch.complex128
 
@@ -131,6 +131,7 @@ class SOCI:
         self.device = ints.device
         self.singles = bool(singles)
         self.fov = bool(fov_coupling)
+        ensure_default_plans(self.device)
         self.fock = fock.to(self.device, CDT).contiguous()
         self.eps = torch.as_tensor(eps, dtype=torch.float64).to(self.device).contiguous()
         O, V = self.O, self.V

@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .kernels import stream_of, workspace
+from .kernels import ensure_default_plans, stream_of, workspace
 from .so_ci import SOCI
 
 CDT = torch.complex128
@@ -67,6 +67,7 @@ class SpatialCI(SOCI):
         self.device = ints.device
         self.singles = bool(singles)
         self.fov = False
+        ensure_default_plans(self.device)
         self.fock = fock.to(self.device, CDT).contiguous()
         self.eps = torch.as_tensor(np.array(eps, dtype=np.float64) if not isinstance(eps, torch.Tensor) else eps,
                                    dtype=torch.float64).to(self.device).contiguous()
@@ -106,8 +107,19 @@ class SpatialCI(SOCI):
     def sigma(self, *a, **k):
         raise NotImplementedError("the spin-adapted solver runs the reference's Jacobi iteration only")
 
-    def tune(self, *a, **k):
-        raise NotImplementedError("the spin-adapted solver uses the cost-model GEMM plans")
+    def tune(self, nb=1, reps=3):
+        """Time the candidate GEMM plans of one step over nb vectors
+        (apy_cisd_sp_tune; nb > 1 is the shared-integral batch of
+        ResponseSet) and cache the fastest per shape, process-wide."""
+        t = torch.zeros((nb, self.L), dtype=CDT, device=self.device)
+        e = torch.zeros(nb, dtype=CDT, device=self.device)
+        args = _lib.CisdSpArgs(self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(),
+                               ctypes.pointer(self.ints.struct), self.hoo.data_ptr(), self.g.data_ptr(),
+                               int(self.singles), e.data_ptr(), t.data_ptr(), t.data_ptr(), t.data_ptr())
+        nbytes = int(self.lib.apy_cisd_sp_batch_workspace(self.O, self.V, int(self.singles), nb))
+        ws = workspace(nbytes, self.device)
+        _lib.check(self.lib.apy_cisd_sp_tune(ctypes.byref(args), nb, 0, _vp(ws), nbytes, int(reps),
+                                             stream_of(self.device)), "apy_cisd_sp_tune")
 
 
 class ResponseCI(SpatialCI):
@@ -363,6 +375,7 @@ class SpatialCISet:
             raise ValueError("one Fock matrix and one orbital-energy array per system")
         self.device = dev = eri_mos[0].device
         m = eri_mos[0].shape[0]
+        ensure_default_plans(dev)
         if any(tuple(e.shape) != (m,) * 4 for e in eri_mos):
             raise ValueError("all systems must have the same number of orbitals")
         self.O, self.V = o, v = int(o), m - int(o)

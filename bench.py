@@ -337,7 +337,7 @@ def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
     return res
 
 
-def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10):
+def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10, tune=False):
     """Spin-adapted closed-shell CISD residual steps (apy_cisd_sp_step) at
     o=10, v=80 spatial orbitals (the closed-shell counterpart of the
     O=20, V=160 spin-orbital workload), seeded chemists' integrals with the
@@ -354,6 +354,8 @@ def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10):
     F = torch.from_numpy(np.diag(eps).astype(complex)).to(dev)
     ci = SpatialCI(SPIntegrals.from_mo(E4, o), F, eps)
     del E4
+    if tune:
+        ci.tune()
     t = ci.initial_guess(ci.new_vector())
     tn, r = ci.new_vector(), ci.new_vector()
     ci.energy(t)
@@ -378,7 +380,7 @@ def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10):
     return res
 
 
-def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=2, h=1e-5):
+def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=2, h=1e-5, tune=False, steps=10):
     """First-order CISD amplitudes of `nb` perturbations (the 3N+3 field
     and nuclear perturbations of a 4-atom molecule in the analytic AAT) of
     one spin-adapted CISD solution at o=10, v=80: seeded chemists' integrals
@@ -417,6 +419,9 @@ def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=
     E4 = pair_sym(0, scale)
     ints = SPIntegrals.from_mo(E4, o)
     ci = SpatialCI(ints, F, eps)
+    if tune:
+        ci.tune()
+        ci.tune(nb)
     (E, _, _, it_cisd), t_cisd = sync_time(lambda: ci.solve(params))
     t_ref = ci.t.clone()
 
@@ -434,7 +439,16 @@ def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=
         return rs
     rs, t_setup = sync_time(build_set)
     (dE_set, _, _, its), t_set = sync_time(lambda: rs.solve(params))
-    del rs
+    # one batched response step (apy_cisd_sp_step_batch over the nb vectors), timed alone
+    X = torch.zeros((nb, rs.L), dtype=torch.complex128, device=dev)
+    Xn, R = torch.zeros_like(X), torch.zeros_like(X)
+    sv = torch.zeros(nb, dtype=torch.complex128, device=dev)
+    rs._step(X, Xn, R, sv)
+    X.copy_(Xn)
+    for _ in range(2):
+        rs._step(X, Xn, R, sv)
+    _, t_steps = sync_time(lambda: [rs._step(X, Xn, R, sv) for _ in range(steps)])
+    del rs, X, Xn, R
 
     def one_by_one():
         out = []
@@ -464,6 +478,7 @@ def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=
            "cisd_solve_ms": round(t_cisd * 1e3, 1), "cisd_iterations": it_cisd,
            "set_setup_ms": round(t_setup * 1e3, 1), "set_solve_ms": round(t_set * 1e3, 1),
            "set_iterations_max": int(max(its)),
+           "response_step_ms": round(t_steps / steps * 1e3, 3),
            "one_by_one_ms": round(t_seq * 1e3, 1),
            "central_differences_ms": round(fd_scaled * 1e3, 1),
            "central_differences_measured": f"{nfd} perturbations, scaled to {nb}",
@@ -685,8 +700,11 @@ def main():
     solver = SOCI(ints, torch.from_numpy(F).to(dev), eps, singles=True, fov_coupling=True)
     torch.cuda.synchronize()
     log(f"setup: integrals {ints.nbytes() / 1e9:.2f} GB generated on device in {time.perf_counter() - t0:.2f} s")
-    from apyib_amd.kernels import plans_export, plans_load
+    from apyib_amd.kernels import plans_clear, plans_export, plans_load
     plans = "model"
+    plans_clear()   # the solver installed the saved plans; --no-tune and --tune start from none
+    if args.no_tune or args.tune:
+        os.environ["APYIB_AMD_PLANS"] = "none"   # and later solvers install none either
     if not args.no_tune:
         if not args.tune and os.path.exists(args.plans):
             plans_load(json.load(open(args.plans)))
@@ -767,8 +785,9 @@ def main():
             aat_set["C"] = cisd_field_set_measurement(dev, args.seed)
         if not args.no_mp2_adjoint_set:
             aat_set["D"] = mp2_adjoint_set_measurement(dev, args.seed)
-    spatial = None if args.no_spatial else spatial_cisd_measurement(dev, args.seed)
-    response = None if args.no_analytic_response else analytic_response_measurement(dev, args.seed)
+    spatial = None if args.no_spatial else spatial_cisd_measurement(dev, args.seed, tune=args.tune)
+    response = None if args.no_analytic_response else analytic_response_measurement(dev, args.seed,
+                                                                                     tune=args.tune)
     h2o2 = None if args.no_h2o2_aat else h2o2_aat_measurement(dev)
 
     cpu = None

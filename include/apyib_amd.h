@@ -160,8 +160,8 @@ typedef struct apy_sp_ints {
   void* w3;    /* [(k,c)][(j,a)]      <ka|jc>                             (ci_wfn.py:478, 482) */
   void* vvoo;  /* [(i,j)][(a,b)]      <ab|ij>  (doubles driver)            (ci_wfn.py:466) */
   void* l;     /* [(i,j)][(a,b)]      2<ij|ab> - <ij|ba>  (energy)          (ci_wfn.py:495) */
-  void* vvvo;  /* [(a,b,j)][c]        <ab|cj>                             (ci_wfn.py:467-468) */
-  void* vooo;  /* [(a,i,j)][k]        <ak|ij>                             (ci_wfn.py:469-470) */
+  void* vvvo;  /* [(j,a,b)][c]        <ab|cj>                             (ci_wfn.py:467-468) */
+  void* vooo;  /* [(i,j,a)][k]        <ak|ij>                             (ci_wfn.py:469-470) */
   void* s1;    /* [(i,a)][(j,b)]      2<ja|bi> - <ja|ib>                  (ci_wfn.py:460) */
   void* w4;    /* [a][(j,b,c)]        2<aj|bc> - <aj|cb>                  (ci_wfn.py:462) */
   void* w5;    /* [i][(j,k,b)]        2<kj|ib> - <kj|bi>                  (ci_wfn.py:463) */
@@ -245,6 +245,16 @@ size_t apy_cisd_sp_multi_workspace(int64_t o, int64_t v, int singles, int64_t nb
 int apy_cisd_sp_step_multi(const apy_cisd_sp_args* a, int64_t nb, void* workspace, size_t ws_bytes, void* stream);
 int apy_cisd_sp_energy_multi(int64_t o, int64_t v, int singles, int64_t nb, const void* g, const void* L,
                              const void* t, void* energy, void* workspace, size_t ws_bytes, void* stream);
+
+/* apy_cisd_sp_tune: time the candidate tile/split plans of every GEMM of one
+ * spin-adapted step (nb instances; multi = 0 for apy_cisd_sp_step_batch's
+ * shared-integral form, 1 for apy_cisd_sp_step_multi) and keep the fastest
+ * in the process-wide plan cache that later steps of the same shapes use.
+ * The arguments are those of the step; workspace as for that step.  Writes
+ * only the workspace.  No counterpart in the reference (its einsums have no
+ * launch plans). */
+int apy_cisd_sp_tune(const apy_cisd_sp_args* a, int64_t nb, int multi, void* workspace, size_t ws_bytes, int reps,
+                     void* stream);
 
 /* ------------------------------------------------------------------------
  * CISD / CID in spin orbitals: one iteration of the reference's Jacobi
@@ -445,6 +455,40 @@ int apy_mp2_nc_step(const apy_mp2_nc_args* a, void* workspace, size_t ws_bytes, 
 size_t apy_mp2_nc_density_workspace(int64_t O, int64_t V);
 int apy_mp2_nc_densities(int64_t O, int64_t V, const void* t, const void* lam, void* grad_f, void* workspace,
                          size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Spin-adapted (closed-shell) MP2 energy derivatives in the reference's own
+ * forms (compute_MP2_Hessian, analytic.py:307-335).  t2 [o][o][v][v] are the
+ * closed-shell amplitudes of apy_mp2_spatial (mp2_wfn.py:54), pair symmetric
+ * (t_ijab = t_jiba); f-like matrices [n][n] row-major, n = o + v; a
+ * direction q holds f_grad + q n^2 and eri_grad + q o^2 v^2 (<ij|ab> at
+ * [i][j][a][b]).  Plain products throughout, no conjugation.
+ *   apy_mp2_sp_response: the t2 response of nb directions, t2_grad =
+ *     (eri_grad + Fock terms) / D with the Fock terms in the reference's two
+ *     index orders: form 0 (analytic.py:326-331, analytic_aats.py:447-452)
+ *     - f_ik t_kjab - t_ikab f_kj + t_ijcb f_ac + t_ijac f_cb; form 1
+ *     (analytic_aats.py:346-350) - f_ki t_kjab - f_kj t_ikab + f_ac t_ijcb
+ *     + f_bc t_ijac.  eri_grad may be null (zero).
+ *   apy_mp2_sp_densities: the coefficients of E_grad (analytic.py:320-324):
+ *     d_eri = 2 (2 t - t^T) [o][o][v][v] (may be null), d_oo [o][o] =
+ *     -2 sum_jab t_ijab u_kjab, d_vv [v][v] = 2 sum_ijb t_ijab u_ijcb with
+ *     u = 2 t - t^T (each f term of the reference counted twice by the pair
+ *     symmetry).
+ *   apy_mp2_sp_directional: out[q] = sum d_eri eri_grad[q] + sum d_oo
+ *     f_grad[q]_oo + sum d_vv f_grad[q]_vv, the E_grad of nd directions
+ *     (either of f_grad, eri_grad may be null; out: nd complex).
+ * ---------------------------------------------------------------------- */
+size_t apy_mp2_sp_response_workspace(int64_t o, int64_t v, int64_t nb);
+int apy_mp2_sp_response(int64_t o, int64_t v, int64_t nb, int form, const void* t2, const void* f_grad,
+                        const void* eri_grad, const double* eps, void* t2_grad, void* workspace, size_t ws_bytes,
+                        void* stream);
+size_t apy_mp2_sp_densities_workspace(int64_t o, int64_t v);
+int apy_mp2_sp_densities(int64_t o, int64_t v, const void* t2, void* d_eri, void* d_oo, void* d_vv, void* workspace,
+                         size_t ws_bytes, void* stream);
+size_t apy_mp2_sp_directional_workspace(int64_t nd);
+int apy_mp2_sp_directional(int64_t o, int64_t v, int64_t nd, const void* d_eri, const void* d_oo, const void* d_vv,
+                           const void* f_grad, const void* eri_grad, void* out, void* workspace, size_t ws_bytes,
+                           void* stream);
 
 /* Packed CI vectors [c1 (O*V) | c2 over (i<j, a<b) (Po*Pv) | c0], length
  * apy_ci_packed_length(O, V): the antisymmetric doubles stored once, so the
