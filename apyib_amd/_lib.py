@@ -62,7 +62,7 @@ class SoInts(ctypes.Structure):
 
 SO_BLOCKS = ("vvvv_p", "ring", "oooo_p", "ovoo_p", "vvvo_p", "vovv_p", "ooov", "vvoo_t", "oovv")
 
-SP_BLOCKS = ("vvvs", "vvva", "oooo", "w21", "w3", "vvoo", "l", "vvvo", "vooo", "s1", "w4", "w5")
+SP_BLOCKS = ("vvvs", "vvva", "oooo", "w21", "w3", "vvoo", "l", "vvvo", "vooo", "s1", "w4", "w5", "w4t")
 
 
 class SpInts(ctypes.Structure):
@@ -98,7 +98,7 @@ class CisdSpArgs(ctypes.Structure):
         ("o", c_i64), ("v", c_i64), ("fock", c_vp), ("eps", c_vp), ("ints", ctypes.POINTER(SpInts)),
         ("hoo", c_vp), ("g", c_vp), ("singles", c_int), ("energy", c_vp), ("t", c_vp), ("t_new", c_vp),
         ("out", c_vp), ("resp_b", c_vp), ("resp_t", c_vp), ("resp_energy", c_vp),
-        ("resp_shift_re", c_dbl), ("resp_shift_im", c_dbl),
+        ("resp_shift_re", c_dbl), ("resp_shift_im", c_dbl), ("w21_f", c_vp),
     ]
 
 APY_NOALLOC_SCRATCH_BYTES = 256 << 20
@@ -269,6 +269,7 @@ SIGNATURES = {
     "apy_cisd_sp_multi_workspace": (c_size, [c_i64, c_i64, c_int, c_i64]),
     "apy_cisd_sp_step_multi": (c_int, [ctypes.POINTER(CisdSpArgs), c_i64, c_vp, c_size, c_vp]),
     "apy_cisd_sp_tune": (c_int, [ctypes.POINTER(CisdSpArgs), c_i64, c_int, c_vp, c_size, c_int, c_vp]),
+    "apy_cisd_sp_fold_fock": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "apy_mp2_sp_response_workspace": (c_size, [c_i64, c_i64, c_i64]),
     "apy_mp2_sp_response": (c_int, [c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_mp2_sp_densities_workspace": (c_size, [c_i64, c_i64]),

@@ -36,7 +36,7 @@
 
 namespace {
 
-enum SpKind { S_VVVS = 0, S_VVVA, S_OOOO, S_W21, S_W3, S_VVOO, S_L, S_VVVO, S_VOOO, S_S1, S_W4, S_W5, S_COUNT };
+enum SpKind { S_VVVS = 0, S_VVVA, S_OOOO, S_W21, S_W3, S_VVOO, S_L, S_VVVO, S_VOOO, S_S1, S_W4, S_W5, S_W4T, S_COUNT };
 
 // pairs x <= y (symmetric) and x < y (antisymmetric) among N, row-major
 __host__ __device__ inline int64_t nsym(int64_t N) { return N * (N + 1) / 2; }
@@ -59,6 +59,7 @@ __host__ __device__ inline int64_t sp_size(int kind, int64_t o, int64_t v) {
     case S_S1: return ov * ov;
     case S_W4: return v * o * v * v;
     case S_W5: return o * o * o * v;
+    case S_W4T: return v * v * o * v;
   }
   return 0;
 }
@@ -77,6 +78,7 @@ void* sp_ptr(const apy_sp_ints* I, int kind) {
     case S_S1: return I->s1;
     case S_W4: return I->w4;
     case S_W5: return I->w5;
+    case S_W4T: return I->w4t;
   }
   return nullptr;
 }
@@ -150,6 +152,11 @@ __global__ __launch_bounds__(256) void sp_build_kernel(Dirac g, int kind, int64_
       }
       case S_W4: {  // [a][(j,b,c)] = 2<aj|bc> - <aj|cb>
         const int64_t c = t % v, b = (t / v) % v, j = (t / (v * v)) % o, a = t / (v * v * o);
+        val = d2lin(2.0, g(o + a, j, o + b, o + c), -1.0, g(o + a, j, o + c, o + b));
+        break;
+      }
+      case S_W4T: {  // [b][a][(j,c)] = 2<aj|bc> - <aj|cb>
+        const int64_t c = t % v, j = (t / v) % o, a = (t / ov) % v, b = t / (ov * v);
         val = d2lin(2.0, g(o + a, j, o + b, o + c), -1.0, g(o + a, j, o + c, o + b));
         break;
       }
@@ -237,6 +244,31 @@ __global__ __launch_bounds__(256) void sp_tu_kernel(int64_t o, int64_t v, int64_
   }
 }
 
+// w21_f[q] = w21[q] + delta_kj F_q[o+b][o+c] on the rows (k,c) < ov (the W2 half) of nb ring operators:
+// X1 = T w21_f then carries the F_vv term Y[(i,a),(j,b)] = sum_c t_ijac F_bc, and its pair image
+__global__ __launch_bounds__(256) void sp_fold_kernel(int64_t o, int64_t v, int64_t nb, const double2* __restrict__ F,
+                                                      int64_t sF, const double2* __restrict__ w21,
+                                                      double2* __restrict__ w21_f) {
+  const int64_t n = o + v, ov = o * v, per = 2 * ov * ov;
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nb * per; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = u / per, t = u % per, row = t / ov, col = t % ov;
+    double2 x = w21[u];
+    if (row < ov && row / v == col / v) {
+      const double2 f = F[q * sF + (o + col % v) * n + o + row % v];
+      x.x += f.x;
+      x.y += f.y;
+    }
+    w21_f[u] = x;
+  }
+}
+
+// T1[q][(i,a)] = t1 of vector q (stride L): the singles of nb vectors stacked for one product
+__global__ __launch_bounds__(256) void sp_t1_stack(int64_t ov, int64_t nb, int64_t L, const double2* __restrict__ tv,
+                                                   double2* __restrict__ T1) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nb * ov; u += (int64_t)gridDim.x * blockDim.x)
+    T1[u] = tv[(u / ov) * L + u % ov];
+}
+
 // Epilogue operands; every per-instance array is instance 0's, the others follow at the region
 // strides (so*sv for Ps, ao*av for Pa, (ov)^2 for R, Y, X1, X2, Q1, Q2; L for the vectors).
 struct SpEpi {
@@ -295,7 +327,7 @@ __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
     const double2* Ps = e.Ps + q * so * sv + isym(i0, j0, o) * sv;
     const double2* Pa = (i != j) ? e.Pa + q * ao * av + ianti(i0, j0, o) * av : nullptr;
     const double sgn_lower = i < j ? -1.0 : 1.0;  // sign of P- at a > b
-    const double2* Y = e.Y + qd;
+    const double2* Y = e.Y ? e.Y + qd : nullptr;
     const double2* X1 = e.X1 + qd;
     const double2* X2 = e.X2 + qd;
     const double2* Q1 = e.Q1 ? e.Q1 + qd : nullptr;
@@ -306,11 +338,15 @@ __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
       const int64_t b = b0 + ty + 8 * k, a = a0 + tx;
       double2 s = make_double2(0.0, 0.0);
       if (a < v && b < v) {
-        const double2 y2 = Y[((j * o + i) * v + b) * v + a];
         const double2 x1t = X1[(j * v + b) * ov + i * v + a];
         const double2 x2 = X2[(i * v + b) * ov + j * v + a];
-        s.x = y2.x + x1t.x - x2.x;
-        s.y = y2.y + x1t.y - x2.y;
+        s.x = x1t.x - x2.x;
+        s.y = x1t.y - x2.y;
+        if (Y) {
+          const double2 y2 = Y[((j * o + i) * v + b) * v + a];
+          s.x += y2.x;
+          s.y += y2.y;
+        }
         if (Q1) {
           const double2 q1t = Q1[j * (o * vv) + (i * v + b) * v + a];
           const double2 q2t = Q2[((j * o + i) * v + b) * v + a];
@@ -341,11 +377,15 @@ __global__ __launch_bounds__(256) void sp_epi_doubles(SpEpi e) {
       double2 rr = e.vvoo[q * e.s_vvoo + t];
       const double2 tt = T[tx][ty + 8 * k];
       const double2 p = e.R[qd + t];
-      const double2 y1 = Y[t];
       const double2 x1 = X1[(i * v + a) * ov + j * v + b];
       const double2 x2t = X2[(j * v + a) * ov + i * v + b];
-      rr.x += tt.x + p.x + y1.x + x1.x - x2t.x;
-      rr.y += tt.y + p.y + y1.y + x1.y - x2t.y;
+      rr.x += tt.x + p.x + x1.x - x2t.x;
+      rr.y += tt.y + p.y + x1.y - x2t.y;
+      if (Y) {
+        const double2 y1 = Y[t];
+        rr.x += y1.x;
+        rr.y += y1.y;
+      }
       if (Q1) {
         const double2 q1 = Q1[i * (o * vv) + (j * v + a) * v + b];
         const double2 q2 = Q2[t];
@@ -467,21 +507,22 @@ __global__ __launch_bounds__(256) void sp_energy_final(const double2* __restrict
 
 apy_gemm_desc gd(int64_t m, int64_t nn, int64_t k, int al, const void* a, int64_t lda, int bl, const void* b,
                  int64_t ldb, void* c, int64_t ldc, double alpha, double beta, int64_t batch = 1, int64_t sa = 0,
-                 int64_t sb = 0, int64_t sc = 0) {
+                 int64_t sb = 0, int64_t sc = 0, int reduce = 0) {
   apy_gemm_desc d{};
   d.m = m; d.n = nn; d.k = k;
   d.a_layout = al; d.b_layout = bl;
   d.alpha_re = alpha; d.beta_re = beta;
   d.a = a; d.lda = lda; d.b = b; d.ldb = ldb; d.c = c; d.ldc = ldc;
   d.batch = batch;
+  d.reduce_batch = reduce;
   if (batch > 1) { d.stride_a = sa; d.stride_b = sb; d.stride_c = sc; }
   return d;
 }
 
 // Workspace regions: each holds nb instances one after another (R, Y, X1, X2, Q1, Q2: (ov)^2 each;
-// TU: 2 (ov)^2; Ts/Ps: so*sv; Ta/Pa: ao*av; acc: ov), then the GEMM scratch.
+// TU: 2 (ov)^2; Ts/Ps: so*sv; Ta/Pa: ao*av; acc, T1: ov), then the GEMM scratch.
 struct SpWS {
-  size_t R, Y, X1, X2, TU, Ts, Ta, Ps, Pa, Q1, Q2, acc, gemm, total;
+  size_t R, Y, X1, X2, TU, Ts, Ta, Ps, Pa, Q1, Q2, acc, t1, gemm, total;
 };
 
 constexpr int NSPG = 14;
@@ -507,7 +548,7 @@ SpInst sp_inst_strides(int64_t o, int64_t v) {
 // ladder and ring GEMMs stack the instances along M; the ones reading the vectors directly run as
 // batched GEMMs with the vector stride.
 int sp_gemms(int64_t o, int64_t v, int singles, int64_t nb, int64_t L, const apy_cisd_sp_args* A, char* base,
-             const SpWS* w, apy_gemm_desc g[NSPG], const SpInst* S) {
+             const SpWS* w, apy_gemm_desc g[NSPG], const SpInst* S, bool use_w4t = false) {
   // S null: shared integrals, the instances stacked along M of the ladder and ring products; S set: every
   // product batched over the instances with the operands' own strides.
   const SpInst Z{};
@@ -536,27 +577,43 @@ int sp_gemms(int64_t o, int64_t v, int singles, int64_t nb, int64_t L, const apy
       g[q++] = gd(nb * ao, av, av, APY_MK, P(w->Ta), av, APY_NK, I ? I->vvva : nullptr, av, P(w->Pa), av, 1.0, 0.0);  // P-
   }
   g[q++] = gd(oo, vv, oo, APY_MK, A ? A->hoo : nullptr, oo, APY_KN, t2, vv, R, vv, 1.0, 0.0, nb, st.hoo, L, d2);  // HHL
-  g[q++] = gd(oo * v, v, v, APY_MK, t2, v, APY_NK, Fvv, n, P(w->Y), v, 1.0, 0.0, nb, L, st.fock, d2);          // F_vv
+  // F_vv t2 (Y), unless folded into the ring operator (w21_f = w21 + delta_kj F_bc, apy_cisd_sp_fold_fock)
+  const bool fold = A && A->w21_f;
+  const void* w21 = fold ? A->w21_f : (I ? I->w21 : nullptr);
+  if (!fold)
+    g[q++] = gd(oo * v, v, v, APY_MK, t2, v, APY_NK, Fvv, n, P(w->Y), v, 1.0, 0.0, nb, L, st.fock, d2);
   if (S) {
-    g[q++] = gd(ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, I ? I->w21 : nullptr, ov, P(w->X1), ov, 1.0, 0.0, nb,
-                2 * d2, st.blk[S_W21], d2);
+    g[q++] = gd(ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, w21, ov, P(w->X1), ov, 1.0, 0.0, nb, 2 * d2,
+                st.blk[S_W21], d2);
     g[q++] = gd(ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2), ov,
                 1.0, 0.0, nb, 2 * d2, st.blk[S_W3], d2);
   } else {
-    g[q++] = gd(nb * ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, I ? I->w21 : nullptr, ov, P(w->X1), ov, 1.0, 0.0);
+    g[q++] = gd(nb * ov, ov, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, w21, ov, P(w->X1), ov, 1.0, 0.0);
     g[q++] = gd(nb * ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2),
                 ov, 1.0, 0.0);
   }
+  // shared integrals: the t1 of the nb vectors stacked (sp_t1_stack) into one Q1 product of nb*o rows, and
+  // the w4 product taken from the T half of TU, rows (q,i), with w4t [b][a][(j,c)] as a sum over b
+  const bool stack = !S && nb > 1;
+  const bool w4t = !S && use_w4t;
   if (singles) {
-    g[q++] = gd(o, vv * o, v, APY_MK, t1, v, APY_NK, I ? I->vvvo : nullptr, v, P(w->Q1), vv * o, 1.0, 0.0, nb, L,
-                st.blk[S_VVVO], d2);
+    if (stack)
+      g[q++] = gd(nb * o, vv * o, v, APY_MK, P(w->t1), v, APY_NK, I ? I->vvvo : nullptr, v, P(w->Q1), vv * o, 1.0,
+                  0.0);
+    else
+      g[q++] = gd(o, vv * o, v, APY_MK, t1, v, APY_NK, I ? I->vvvo : nullptr, v, P(w->Q1), vv * o, 1.0, 0.0, nb, L,
+                  st.blk[S_VVVO], d2);
     g[q++] = gd(v * oo, v, o, APY_MK, I ? I->vooo : nullptr, o, APY_KN, t1, v, P(w->Q2), v, 1.0, 0.0, nb,
                 st.blk[S_VOOO], L, d2);
     g[q++] = gd(ov, 1, ov, APY_MK, I ? I->s1 : nullptr, ov, APY_KN, t1, 1, acc, 1, 1.0, 0.0, nb, st.blk[S_S1], L, ov);
     g[q++] = gd(ov, 1, 2 * ov, APY_MK, TU, 2 * ov, APY_KN, A ? A->g : nullptr, 1, acc, 1, 1.0, 1.0, nb, 2 * d2, st.g,
                 ov);
-    g[q++] = gd(o, v, ov * v, APY_MK, t2, ov * v, APY_NK, I ? I->w4 : nullptr, ov * v, acc, v, 1.0, 1.0, nb, L,
-                st.blk[S_W4], ov);
+    if (w4t)
+      g[q++] = gd(nb * o, v, ov, APY_MK, TU, v * 2 * ov, APY_NK, I ? I->w4t : nullptr, ov, acc, v, 1.0, 1.0, v, 2 * ov,
+                  v * ov, 0, 1);
+    else
+      g[q++] = gd(o, v, ov * v, APY_MK, t2, ov * v, APY_NK, I ? I->w4 : nullptr, ov * v, acc, v, 1.0, 1.0, nb, L,
+                  st.blk[S_W4], ov);
     g[q++] = gd(o, v, oo * v, APY_MK, I ? I->w5 : nullptr, oo * v, APY_KN, t2, v, acc, v, -1.0, 1.0, nb, st.blk[S_W5],
                 L, ov);
     g[q++] = gd(o, v, o, APY_KM, F, n, APY_KN, t1, v, acc, v, -1.0, 1.0, nb, st.fock, L, ov);   // -F_oo^T t1
@@ -584,13 +641,17 @@ SpWS sp_carve(int64_t o, int64_t v, int singles, int64_t nb, bool multi = false)
     w.Q1 = off; off += rnd(B * d2);
     w.Q2 = off; off += rnd(B * d2);
     w.acc = off; off += rnd(B * ov * z);
+    w.t1 = off; off += rnd(B * ov * z);
   }
   w.gemm = off;
   apy_gemm_desc g[NSPG];
   const SpInst S = sp_inst_strides(o, v);
-  const int q = sp_gemms(o, v, singles, nb, (int64_t)(ov + ov * ov), nullptr, nullptr, &w, g, multi ? &S : nullptr);
   size_t m = 0;
-  for (int k = 0; k < q; ++k) m = std::max(m, apy_zgemm_ws_bytes(&g[k]));
+  for (int form = 0; form < (multi ? 1 : 2); ++form) {   // with and without the w4t form of the w4 product
+    const int q = sp_gemms(o, v, singles, nb, (int64_t)(ov + ov * ov), nullptr, nullptr, &w, g, multi ? &S : nullptr,
+                           form == 1);
+    for (int k = 0; k < q; ++k) m = std::max(m, apy_zgemm_ws_bytes(&g[k]));
+  }
   w.total = off + std::max(m, (size_t)EN_BLOCKS * z);
   return w;
 }
@@ -632,8 +693,12 @@ int sp_step_impl(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, 
   sp_pack_kernel<<<apy_grid1d(nb * ov * ov), 256, 0, st>>>(o, v, nb, L, t, (double2*)(base + w.Ts),
                                                             (double2*)(base + w.Ta));
   APY_LAUNCH_CHECK();
+  if (singles && !multi && nb > 1) {
+    sp_t1_stack<<<apy_grid1d(nb * ov), 256, 0, st>>>(ov, nb, L, t, (double2*)(base + w.t1));
+    APY_LAUNCH_CHECK();
+  }
   apy_gemm_desc g[NSPG];
-  const int q = sp_gemms(o, v, singles, nb, L, A, base, &w, g, multi ? &S : nullptr);
+  const int q = sp_gemms(o, v, singles, nb, L, A, base, &w, g, multi ? &S : nullptr, !multi && I->w4t);
   for (int k = 0; k < q; ++k) {
     const int s = apy_zgemm_launch(&g[k], base + w.gemm, ws_bytes - w.gemm, st);
     if (s != APY_OK) return s;
@@ -644,7 +709,7 @@ int sp_step_impl(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, 
   const double2* es = (const double2*)resp_shift;
   const double2 rs = make_double2(A->resp_shift_re, A->resp_shift_im);
   SpEpi e{o, v, nb, L, A->eps, (const double2*)I->vvoo, (const double2*)(base + w.Ps), (const double2*)(base + w.Pa),
-          (const double2*)(base + w.R), (const double2*)(base + w.Y),
+          (const double2*)(base + w.R), A->w21_f ? nullptr : (const double2*)(base + w.Y),
           (const double2*)(base + w.X1), (const double2*)(base + w.X2),
           singles ? (const double2*)(base + w.Q1) : nullptr, singles ? (const double2*)(base + w.Q2) : nullptr,
           t + ov, (const double2*)A->energy, (double2*)A->t_new + ov, (double2*)A->out + ov,
@@ -684,7 +749,7 @@ int sp_energy_impl(int64_t o, int64_t v, int singles, int64_t nb, const void* g,
 
 extern "C" {
 
-int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[12]) {
+int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[13]) {
   if (o < 1 || v < 1 || !sizes_out) return APY_E_ARG;
   for (int k = 0; k < S_COUNT; ++k) sizes_out[k] = sp_size(k, o, v);
   return APY_OK;
@@ -707,6 +772,16 @@ int apy_sp_ints_from_mo(int64_t m, const void* eri_mo, const apy_sp_ints* ints, 
                                                                                  total);
     APY_LAUNCH_CHECK();
   }
+  return APY_OK;
+}
+
+int apy_cisd_sp_fold_fock(int64_t o, int64_t v, int64_t nb, const void* fock, const void* w21, void* w21_f,
+                          void* stream) {
+  if (o < 1 || v < 1 || nb < 1 || !fock || !w21 || !w21_f || w21 == w21_f) return APY_E_ARG;
+  const int64_t n = o + v, total = nb * 2 * (o * v) * (o * v);
+  sp_fold_kernel<<<apy_grid1d(total, 1 << 20), 256, 0, (hipStream_t)stream>>>(
+      o, v, nb, (const double2*)fock, n * n, (const double2*)w21, (double2*)w21_f);
+  APY_LAUNCH_CHECK();
   return APY_OK;
 }
 
@@ -767,7 +842,7 @@ int apy_cisd_sp_tune(const apy_cisd_sp_args* A, int64_t nb, int multi, void* wor
   const int64_t ov = o * v, L = ov + ov * ov;
   char* base = (char*)workspace;
   apy_gemm_desc g[NSPG];
-  const int q = sp_gemms(o, v, singles, nb, L, A, base, &w, g, multi ? &S : nullptr);
+  const int q = sp_gemms(o, v, singles, nb, L, A, base, &w, g, multi ? &S : nullptr, !multi && I->w4t);
   for (int k = 0; k < q; ++k) {
     const int s = apy_zgemm_tune_impl(&g[k], base + w.gemm, ws_bytes - w.gemm, reps, (hipStream_t)stream, nullptr,
                                       nullptr, nullptr);

@@ -123,9 +123,12 @@ int64_t count_items(int n, const apy_mp2_nc_instance* inst) {
   return items;
 }
 
+// acc += a b as four fused multiply-adds (written out: the compiler may not reassociate acc + (ab - cd))
 __device__ inline void cfma(double2& acc, double2 a, double2 b) {
-  acc.x += a.x * b.x - a.y * b.y;
-  acc.y += a.x * b.y + a.y * b.x;
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
 }
 
 // x_(pq)[c, d] of the packed amplitudes for any p != q, c != d (zero otherwise)
@@ -258,22 +261,29 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
   constexpr int NQ = MT * MKC / SBLK;
   const int Vi = (int)V;
   const uint32_t V2 = 2u * (uint32_t)Vi;
-  // x_ij[p, q] (p, q < V) from the pair's packed row, 32-bit index arithmetic
-  auto xr32 = [&](int p, int q) -> double2 {
-    if (p == q) return make_double2(0.0, 0.0);
-    const int lo = p < q ? p : q, hi = p < q ? q : p;
-    const double2 v = row[(uint32_t)lo * (V2 - (uint32_t)lo - 1u) / 2u + (uint32_t)(hi - lo - 1)];
-    return p < q ? v : make_double2(-v.x, -v.y);
+  // x_ij[p][e] is row[pidx(p, e)] above the diagonal (contiguous along e) and -row[pidx(e, p)] below it
+  // (contiguous along p): each tile element is fetched by one of two passes, the upper one with e fastest
+  // across the threads and the lower one with p fastest, so that both read along the packed row.
+  double2 sxa_u[NQ], sxa_l[NQ], sxb_u[NQ], sxb_l[NQ], sga[NQ], sgb[NQ];
+  auto upper = [&](int p, int e) -> double2 {        // p <= e or out of range (zero there)
+    if (p >= Vi || e >= Vi || p >= e) return make_double2(0.0, 0.0);
+    return row[(uint32_t)p * (V2 - (uint32_t)p - 1u) / 2u + (uint32_t)(e - p - 1)];
   };
-  double2 sxa[NQ], sxb[NQ], sga[NQ], sgb[NQ];
+  auto lower = [&](int p, int e) -> double2 {        // p > e in range
+    if (p >= Vi || e >= Vi || p <= e) return make_double2(0.0, 0.0);
+    const double2 v = row[(uint32_t)e * (V2 - (uint32_t)e - 1u) / 2u + (uint32_t)(p - e - 1)];
+    return make_double2(-v.x, -v.y);
+  };
   auto fetch = [&](int e0) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       const int q = tid + SBLK * k;
-      const int r = q / MKC, c = q % MKC;            // XA[r][c] = x_ij[a0+r, e0+c]
-      const int e = e0 + c, ra = (int)a0 + r, rb = (int)b0 + r;
-      sxa[k] = (ra < Vi && e < Vi) ? xr32(ra, e) : make_double2(0.0, 0.0);
-      sxb[k] = (rb < Vi && e < Vi) ? xr32(rb, e) : make_double2(0.0, 0.0);
+      const int r = q / MKC, c = q % MKC;            // upper pass: XA[r][c] = x_ij[a0+r, e0+c], c fastest
+      sxa_u[k] = upper((int)a0 + r, e0 + c);
+      sxb_u[k] = upper((int)b0 + r, e0 + c);
+      const int rl = q % MT, cl = q / MT;            // lower pass: r fastest
+      sxa_l[k] = lower((int)a0 + rl, e0 + cl);
+      sxb_l[k] = lower((int)b0 + rl, e0 + cl);
       // GA[rr][cc] = Fvv(a0+cc, e0+rr), read along the contiguous index of F (e, or b for F^T)
       const int rr = transpose_f ? q / MT : q % MKC, cc = transpose_f ? q % MT : q / MKC;
       const int e2 = e0 + rr, ca = (int)a0 + cc, cb = (int)b0 + cc;
@@ -287,9 +297,13 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       const int q = tid + SBLK * k;
-      const int r = q / MKC, c = q % MKC;
-      XA[r][c] = sxa[k];
-      XB[r][c] = sxb[k];
+      const int r = q / MKC, c = q % MKC, rl = q % MT, cl = q / MT;
+      const int pa = (int)a0 + r, pb = (int)b0 + r, e = e0 + c, el = e0 + cl;
+      // each element is stored once: by the lower pass where p > e is in range, else by the upper pass
+      if (!(pa > e && pa < Vi && e < Vi)) XA[r][c] = sxa_u[k];
+      if (!(pb > e && pb < Vi && e < Vi)) XB[r][c] = sxb_u[k];
+      if ((int)a0 + rl > el && (int)a0 + rl < Vi && el < Vi) XA[rl][cl] = sxa_l[k];
+      if ((int)b0 + rl > el && (int)b0 + rl < Vi && el < Vi) XB[rl][cl] = sxb_l[k];
       const int rr = transpose_f ? q / MT : q % MKC, cc = transpose_f ? q % MT : q / MKC;
       GA[rr][cc] = sga[k];
       GB[rr][cc] = sgb[k];

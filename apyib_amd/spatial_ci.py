@@ -78,6 +78,10 @@ class SpatialCI(SOCI):
         self.g = torch.empty(2 * o * v, dtype=CDT, device=self.device)
         _lib.check(self.lib.apy_cisd_sp_prepare(o, v, _vp(self.fock), ctypes.byref(ints.struct), _vp(self.hoo),
                                                 _vp(self.g), stream_of(self.device)), "apy_cisd_sp_prepare")
+        # the ring operator with F_vv folded in: one GEMM fewer per step
+        self.w21_f = torch.empty_like(ints["w21"])
+        _lib.check(self.lib.apy_cisd_sp_fold_fock(o, v, 1, _vp(self.fock), _vp(ints["w21"]), _vp(self.w21_f),
+                                                  stream_of(self.device)), "apy_cisd_sp_fold_fock")
         self.ws_bytes = int(self.lib.apy_cisd_sp_workspace(o, v, int(self.singles)))
         self.red_bytes = int(self.lib.apy_reduce_workspace(self.L, _lib.APY_MAXVEC))
         self.en_bytes = int(self.lib.apy_cisd_sp_energy_workspace())
@@ -92,6 +96,7 @@ class SpatialCI(SOCI):
                                ctypes.pointer(self.ints.struct), self.hoo.data_ptr(), self.g.data_ptr(),
                                int(self.singles), (energy if energy is not None else self.energy_dev).data_ptr(),
                                t.data_ptr(), t_new.data_ptr(), out.data_ptr())
+        args.w21_f = self.w21_f.data_ptr()
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_sp_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
                    "apy_cisd_sp_step")
@@ -116,6 +121,7 @@ class SpatialCI(SOCI):
         args = _lib.CisdSpArgs(self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(),
                                ctypes.pointer(self.ints.struct), self.hoo.data_ptr(), self.g.data_ptr(),
                                int(self.singles), e.data_ptr(), t.data_ptr(), t.data_ptr(), t.data_ptr())
+        args.w21_f = self.w21_f.data_ptr()
         nbytes = int(self.lib.apy_cisd_sp_batch_workspace(self.O, self.V, int(self.singles), nb))
         ws = workspace(nbytes, self.device)
         _lib.check(self.lib.apy_cisd_sp_tune(ctypes.byref(args), nb, 0, _vp(ws), nbytes, int(reps),
@@ -165,6 +171,7 @@ class ResponseCI(SpatialCI):
                                out.data_ptr(), self.b.data_ptr(), self.t_ref.data_ptr(),
                                (energy if energy is not None else self.energy_dev).data_ptr(),
                                self.shift.real, self.shift.imag)
+        args.w21_f = self.w21_f.data_ptr()
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_sp_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
                    "apy_cisd_sp_step")
@@ -226,6 +233,7 @@ class ResponseSet:
                                ctypes.pointer(base.ints.struct), base.hoo.data_ptr(), base.g.data_ptr(),
                                int(self.singles), self.E_arr.data_ptr(), x.data_ptr(), x_new.data_ptr(),
                                out.data_ptr(), self.B.data_ptr(), self.t_ref.data_ptr(), s.data_ptr(), 0.0, 0.0)
+        args.w21_f = base.w21_f.data_ptr()
         nbytes = int(self.lib.apy_cisd_sp_batch_workspace(self.O, self.V, int(self.singles), nb))
         ws = workspace(nbytes, self.device)
         _lib.check(self.lib.apy_cisd_sp_step_batch(ctypes.byref(args), nb, _vp(self.shift_dev), _vp(ws), nbytes,
@@ -384,7 +392,8 @@ class SpatialCISet:
         st = stream_of(dev)
         sizes = (ctypes.c_int64 * len(_lib.SP_BLOCKS))()
         _lib.check(lib.apy_sp_ints_sizes(o, v, sizes), "apy_sp_ints_sizes")
-        names = _lib.SP_BLOCKS if self.singles else CID_SP_BLOCKS
+        # independent systems: every product runs per system, the stacked-w4 block is not used
+        names = tuple(b for b in _lib.SP_BLOCKS if b != "w4t") if self.singles else CID_SP_BLOCKS
         self.blocks = {b: torch.empty((nb, int(sizes[k])), dtype=CDT, device=dev)
                        for k, b in enumerate(_lib.SP_BLOCKS) if b in names}
         self.structs = [_lib.SpInts(o, v, *[self.blocks[b][q].data_ptr() if b in self.blocks else None
@@ -400,6 +409,9 @@ class SpatialCISet:
                        "apy_sp_ints_from_mo")
             _lib.check(lib.apy_cisd_sp_prepare(o, v, _vp(self.fock[q]), ctypes.byref(self.structs[q]),
                                                _vp(self.hoo[q]), _vp(self.g[q]), st), "apy_cisd_sp_prepare")
+        self.w21_f = torch.empty_like(self.blocks["w21"])
+        _lib.check(lib.apy_cisd_sp_fold_fock(o, v, nb, _vp(self.fock), _vp(self.blocks["w21"]), _vp(self.w21_f), st),
+                   "apy_cisd_sp_fold_fock")
         self.ws_bytes = int(lib.apy_cisd_sp_multi_workspace(o, v, int(self.singles), nb))
         self.en_bytes = nb * int(lib.apy_cisd_sp_energy_workspace())
 
@@ -407,6 +419,7 @@ class SpatialCISet:
         args = _lib.CisdSpArgs(self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(),
                                ctypes.pointer(self.structs[0]), self.hoo.data_ptr(), self.g.data_ptr(),
                                int(self.singles), s.data_ptr(), x.data_ptr(), x_new.data_ptr(), out.data_ptr())
+        args.w21_f = self.w21_f.data_ptr()
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_sp_step_multi(ctypes.byref(args), self.nb, _vp(ws), self.ws_bytes,
                                                    stream_of(self.device)), "apy_cisd_sp_step_multi")

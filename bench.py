@@ -337,6 +337,17 @@ def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
     return res
 
 
+def spatial_step_flops(o, v):
+    """Executed flops of one spin-adapted CISD step (apy_cisd_sp_step, F_vv folded into the ring
+    operator): the two ladder halves, HHL, the ring X1 (K = 2ov) and X2, singles->doubles and the six
+    singles products."""
+    ov, oo, vv = o * v, o * o, v * v
+    so, ao, sv, av = o * (o + 1) // 2, o * (o - 1) // 2, v * (v + 1) // 2, v * (v - 1) // 2
+    ladder = 8.0 * (so * sv * sv + ao * av * av)
+    return ladder, ladder + 8.0 * (oo * vv * oo + 3 * ov ** 3 + o * vv * o * v + v * oo * v * o + ov * ov
+                                   + 2 * ov * ov + o * v * ov * v + o * v * oo * v + o * v * o + o * v * v)
+
+
 def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10, tune=False):
     """Spin-adapted closed-shell CISD residual steps (apy_cisd_sp_step) at
     o=10, v=80 spatial orbitals (the closed-shell counterpart of the
@@ -368,10 +379,7 @@ def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10, tune=False):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / steps * 1e3
     ov, oo, vv = o * v, o * o, v * v
-    so, ao, sv, av = o * (o + 1) // 2, o * (o - 1) // 2, v * (v + 1) // 2, v * (v - 1) // 2
-    ladder = 8.0 * (so * sv * sv + ao * av * av)              # the symmetric and antisymmetric ladder GEMMs
-    flops = ladder + 8.0 * (oo * vv * oo + oo * v * v * v + 3 * ov ** 3 + o * vv * o * v + v * oo * v * o
-                            + ov * ov + 2 * ov * ov + o * v * ov * v + o * v * oo * v + o * v * o + o * v * v)
+    ladder, flops = spatial_step_flops(o, v)
     res = {"workload": f"spin-adapted CISD residual step (ci_wfn.py:457-486), o={o} v={v} spatial orbitals",
            "ms_per_step": round(ms, 3), "executed_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
            "ladder_flops": ladder, "step_flops": flops,
@@ -479,6 +487,7 @@ def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=
            "set_setup_ms": round(t_setup * 1e3, 1), "set_solve_ms": round(t_set * 1e3, 1),
            "set_iterations_max": int(max(its)),
            "response_step_ms": round(t_steps / steps * 1e3, 3),
+           "response_step_executed_tflops": round(nb * spatial_step_flops(o, v)[1] / (t_steps / steps) / 1e12, 2),
            "one_by_one_ms": round(t_seq * 1e3, 1),
            "central_differences_ms": round(fd_scaled * 1e3, 1),
            "central_differences_measured": f"{nfd} perturbations, scaled to {nb}",

@@ -165,12 +165,15 @@ typedef struct apy_sp_ints {
   void* s1;    /* [(i,a)][(j,b)]      2<ja|bi> - <ja|ib>                  (ci_wfn.py:460) */
   void* w4;    /* [a][(j,b,c)]        2<aj|bc> - <aj|cb>                  (ci_wfn.py:462) */
   void* w5;    /* [i][(j,k,b)]        2<kj|ib> - <kj|bi>                  (ci_wfn.py:463) */
+  void* w4t;   /* [b][a][(j,c)]       2<aj|bc> - <aj|cb>, optional: w4 reordered so that a step over
+                  several vectors sharing these integrals forms the w4 term of all of them in one
+                  product (rows (vector, i), summed over b); NULL: the per-vector w4 product */
 } apy_sp_ints;
 
 /* Complex elements of each block, in struct order.  Pairs are packed row
  * by row: (x<=y) at x(2N-x+1)/2 + y - x, (x<y) at x(2N-x-1)/2 + y - x - 1.
  * Host. */
-int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[12]);
+int apy_sp_ints_sizes(int64_t o, int64_t v, int64_t sizes_out[13]);
 /* Blocks from eri_mo[m][m][m][m] = (pq|rs), m = o + v (NULL blocks skipped). */
 int apy_sp_ints_from_mo(int64_t m, const void* eri_mo, const apy_sp_ints* ints, void* stream);
 /* Per-solve operands that depend on the Fock matrix fock[(o+v)][(o+v)]:
@@ -202,7 +205,17 @@ typedef struct apy_cisd_sp_args {
   const void* resp_t;
   const void* resp_energy;
   double resp_shift_re, resp_shift_im;
+  /* Optional ring operator with the F_vv term folded in (apy_cisd_sp_fold_fock
+   * of this fock and ints->w21; for apy_cisd_sp_step_multi one per instance,
+   * 2 (ov)^2 apart): the step then runs one GEMM fewer.  NULL: off. */
+  const void* w21_f;
 } apy_cisd_sp_args;
+
+/* w21_f[q] = w21[q] + delta_kj F_q[o+b][o+c] on the rows (k,c) < ov, for nb
+ * instances (fock + q n^2, w21 and w21_f + q 2 (ov)^2): the ring operator with
+ * the F_vv term of the doubles residual (ci_wfn.py:471-472) folded in. */
+int apy_cisd_sp_fold_fock(int64_t o, int64_t v, int64_t nb, const void* fock, const void* w21, void* w21_f,
+                          void* stream);
 
 /* One Jacobi step of ci_wfn.py:457-486 (CID :83-96):  r = sigma(t) - E t
  * (+ the response term above), t_new = t + r / D. */

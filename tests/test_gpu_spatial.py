@@ -95,3 +95,31 @@ def test_cid_solve_matches_oracle(dev, o, v, seed):
     assert it == it_ref
     assert abs(E - Er) < 1e-10
     assert np.abs(t2.cpu().numpy() - r2).max() < 1e-9
+
+
+@pytest.mark.parametrize("singles", [True, False])
+def test_folded_ring_operator_matches_separate_fvv_product(dev, singles):
+    """apy_cisd_sp_step with w21_f (F_vv folded into the ring operator) and
+    with the separate F_vv product (w21_f null) agree, with a non-diagonal F."""
+    import ctypes
+    from apyib_amd import _lib
+    from apyib_amd.kernels import stream_of, workspace
+    o, v = 5, 11
+    E4, F, eps = _system(o, v, 21, noise=2e-2)
+    ints = SPIntegrals.from_mo(_d(E4, dev), o, singles=singles)
+    ci = SpatialCI(ints, _d(F, dev), eps, singles=singles)
+    t1, t2 = _amps(o, v, 22)
+    if not singles:
+        t1 = np.zeros_like(t1)
+    t = _d(np.concatenate([t1.ravel(), t2.ravel()]), dev)
+    E = torch.tensor([0.01 + 0.002j], dtype=torch.complex128, device=dev)
+    tn_f, r_f = ci.new_vector(), ci.new_vector()
+    ci.step(t, tn_f, r_f, energy=E)
+    tn, r = ci.new_vector(), ci.new_vector()
+    a = _lib.CisdSpArgs(o, v, ci.fock.data_ptr(), ci.eps.data_ptr(), ctypes.pointer(ints.struct), ci.hoo.data_ptr(),
+                        ci.g.data_ptr(), int(singles), E.data_ptr(), t.data_ptr(), tn.data_ptr(), r.data_ptr())
+    ws = workspace(ci.ws_bytes, dev)
+    assert ci.lib.apy_cisd_sp_step(ctypes.byref(a), ctypes.c_void_p(ws.data_ptr()), ci.ws_bytes, stream_of(dev)) == 0
+    scale = float(torch.abs(r).max().cpu())
+    assert float(torch.abs(r_f - r).max().cpu()) < 1e-13 * max(1.0, scale)
+    assert float(torch.abs(tn_f - tn).max().cpu()) < 1e-12
