@@ -116,24 +116,33 @@ class MP2NCBatch:
                                         ctypes.c_void_p(self.x[k].data_ptr()), stream_of(self.dev)), "apy_t2_pack")
 
     def solve(self, tol=1e-12, maxit=200, chunk=4, graph=False):
-        """Iterate all instances until every residual norm is below tol,
-        checked once per chunk of iterations (chunk even; the amplitudes end
-        in x).  graph=True replays the chunk as a captured HIP graph (worth
-        it when the same batch is solved repeatedly).  Returns the iteration
-        count."""
-        chunk = max(2, chunk + (chunk % 2))
-        if graph and (self._graph is None or self._graph[0] != chunk):
-            self._graph = (chunk, self._capture(chunk))        # recorded, not run: x is untouched
+        """Iterate all instances until every residual norm is below tol; the
+        amplitudes end in x.  Direct launches check the norms after every
+        step (one host read each; an odd final step's x_new is copied back
+        in one pass over the packed buffer).  graph=True replays chunks of
+        `chunk` (even) steps from a captured HIP graph, checked once per
+        chunk (worth it when the same batch is solved repeatedly).  Returns
+        the iteration count."""
         it = 0
-        while it < maxit:
-            if graph:
+        if graph:
+            chunk = max(2, chunk + (chunk % 2))
+            if self._graph is None or self._graph[0] != chunk:
+                self._graph = (chunk, self._capture(chunk))    # recorded, not run: x is untouched
+            while it < maxit:
                 self._graph[1].replay()
-            else:
-                for q in range(chunk):
-                    self.step(q % 2)
-            it += chunk
-            if float(self.rnorm.max().cpu()) ** 0.5 < tol:
-                break
+                it += chunk
+                if float(self.rnorm.max().cpu()) ** 0.5 < tol:
+                    break
+        else:
+            rev = 0
+            while it < maxit:
+                self.step(rev)
+                rev ^= 1
+                it += 1
+                if float(self.rnorm.max().cpu()) ** 0.5 < tol:
+                    break
+            if rev:                                             # the last step wrote x_new
+                self._xbuf.copy_(self._xnbuf)
         self.iterations = it
         return it
 

@@ -21,10 +21,10 @@
 // amplitudes x_im of each element in registers (read once, coalesced) and
 // writes Z_ij for every j.  ncp_step_kernel then gives a workgroup one
 // (instance, pair i<j, 32 x 32 tile pair A <= B of the virtual plane): it
-// forms Y over the tile as a small product on the vector units with K = e
-// staged in 16-wide chunks through LDS (the tile's rows of x_ij expanded from
-// the packed row with the antisymmetry signs, and the matching columns of
-// Fvv), adds the driver and -Z_ij + Z_ji (two coalesced reads), applies the
+// forms Y_ij[a,b] - Y_ij[b,a] over the tile on the FP64 matrix cores with
+// K = e staged in 8-wide chunks through LDS (the tile's rows of x_ij expanded
+// from the packed row with the antisymmetry signs, and the matching columns
+// of Fvv), adds the driver and -Z_ij + Z_ji (two coalesced reads), applies the
 // update and writes its partial residual norm; per_instance_sum reduces
 // those per instance in a fixed order.  All tables live in the workspace, written once
 // by apy_mp2_nc_batched_setup, so a sequence of steps can be captured in a
@@ -39,9 +39,7 @@ namespace {
 constexpr int BLK = 256;
 constexpr int MT = 32;       // virtual tile edge
 constexpr int MKC = 8;       // e chunk of the Y product
-constexpr int RM = 4, RN = 2;                      // rows a x columns b per thread in the step kernel
-constexpr int SBX = MT / RN, SBY = MT / RM;        // thread grid of the step kernel (b fastest)
-constexpr int SBLK = SBX * SBY;                    // step kernel workgroup size
+constexpr int SBLK = 256;                          // step kernel: 4 waves, 16 x 16 of the 32 x 32 tile each
 constexpr int EN_B = 64;     // energy partial sums per instance
 
 struct Item {
@@ -49,7 +47,7 @@ struct Item {
 };
 
 constexpr int ZS = 2 * BLK;  // (a<b) elements per Z work item
-constexpr int ZMAX_O = 32;   // occupied orbitals held in registers by the Z kernel
+constexpr int ZMAX_O = 32;   // largest O of the matrix-core Z path (two 16-row j blocks, Foo in LDS)
 
 struct ZItem {
   int32_t inst, i, p0, pad;
@@ -139,41 +137,57 @@ __device__ inline double2 xget(const double2* x, int64_t O, int64_t Pv, int64_t 
   return ((p < q) == (c < d)) ? v : make_double2(-v.x, -v.y);
 }
 
-// Z of one work item: NP elements P per thread (p0 + tid + BLK q), the O <= MO amplitudes
-// x_im of each in registers, so every Foo(m, j) read from LDS feeds NP multiply-adds.
-template <int MO, int NP>
-__device__ __forceinline__ void z_item(const double2* __restrict__ fo, const double2* __restrict__ x, double2* Z,
-                                       int64_t O, int64_t Pv, int64_t i, int64_t p0) {
-  double2 xv[NP][MO];
-  bool ok[NP];
+// Z of one work item on the FP64 matrix cores: Z_ij[P] = sum_m Foo(m,j) x_im[P] as 16 x 16 blocks
+// D[j][P] = sum_m A[j][m] B[m][P] with A = Foo^T (from LDS) and B the rows x_im (read along P, 16
+// consecutive P per 16 lanes); each wave takes ZS / 64 blocks of 16 P, every j block (O <= 32) and
+// K = m in steps of 4.  MFMA f64 16x16x4 lanes: A[l%16][l/16], B[l/16][l%16], D[l/16 + 4r][l%16].
+__device__ __forceinline__ void z_item_mfma(const double2* __restrict__ fo, const double2* __restrict__ x,
+                                            double2* Z, int O, int64_t Pv, int i, int64_t p0) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nj = (O + 15) / 16, nk = (O + 3) / 4;
+  for (int pb = wave; pb < ZS / 16; pb += BLK / 64) {
+    const int64_t P = p0 + 16 * pb + (lane & 15);
+    v4d re[2], im[2];
 #pragma unroll
-  for (int q = 0; q < NP; ++q) {
-    const int64_t P = p0 + threadIdx.x + (int64_t)BLK * q;
-    ok[q] = P < Pv;
+    for (int jb = 0; jb < 2; ++jb) { re[jb] = (v4d){0, 0, 0, 0}; im[jb] = (v4d){0, 0, 0, 0}; }
+    double2 bvs[ZMAX_O / 4];                            // B[m][P] = x_im[P] for every k step, loaded up front
 #pragma unroll
-    for (int m = 0; m < MO; ++m) {
-      double2 v = make_double2(0.0, 0.0);
-      if (ok[q] && m < O && m != i) {   // x_im: +row (i,m) for i < m, -row (m,i) for m < i
-        v = x[pidx(i < m ? i : m, i < m ? m : i, O) * Pv + P];
-        if (m < i) v = make_double2(-v.x, -v.y);
+    for (int kb = 0; kb < ZMAX_O / 4; ++kb) {
+      const int m = 4 * kb + (lane >> 4);
+      double2 bv = make_double2(0.0, 0.0);
+      if (kb < nk && m < O && m != i && P < Pv) {
+        bv = x[pidx(i < m ? i : m, i < m ? m : i, O) * Pv + P];
+        if (m < i) bv = make_double2(-bv.x, -bv.y);
       }
-      xv[q][m] = v;
+      bvs[kb] = bv;
     }
-  }
-  for (int64_t j = 0; j < O; ++j) {
-    double2 z[NP];
 #pragma unroll
-    for (int q = 0; q < NP; ++q) z[q] = make_double2(0.0, 0.0);
+    for (int kb = 0; kb < ZMAX_O / 4; ++kb) {
+      if (kb >= nk) break;
+      const int m = 4 * kb + (lane >> 4);
+      const double2 bv = bvs[kb];
 #pragma unroll
-    for (int m = 0; m < MO; ++m)
-      if (m < O) {
-        const double2 f = fo[m * O + j];
-#pragma unroll
-        for (int q = 0; q < NP; ++q) cfma(z[q], f, xv[q][m]);
+      for (int jb = 0; jb < 2; ++jb) {
+        if (jb >= nj) break;
+        const int j = 16 * jb + (lane & 15);
+        const double2 av = (m < O && j < O) ? fo[m * O + j] : make_double2(0.0, 0.0);   // A[j][m] = Foo(m, j)
+        re[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, re[jb], 0, 0, 0);
+        im[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.y, im[jb], 0, 0, 0);
+        re[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.y, bv.y, re[jb], 0, 0, 0);
+        im[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.x, im[jb], 0, 0, 0);
       }
+    }
+    if (P < Pv) {
 #pragma unroll
-    for (int q = 0; q < NP; ++q)
-      if (ok[q]) Z[j * Pv + p0 + threadIdx.x + (int64_t)BLK * q] = z[q];
+      for (int jb = 0; jb < 2; ++jb) {
+        if (jb >= nj) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = 16 * jb + (lane >> 4) + 4 * r;
+          if (j < O) Z[j * Pv + P] = make_double2(re[jb][r], im[jb][r]);
+        }
+      }
+    }
   }
 }
 
@@ -199,14 +213,7 @@ __global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const 
       __syncthreads();                                   // fo of the previous item is no longer read
       for (int64_t q = threadIdx.x; q < O * O; q += BLK) fo[q] = foo(q / O, q % O);   // fo[m O + j]
       __syncthreads();
-      if (O <= 16)
-        z_item<16, 2>(fo, x, Z, O, Pv, i, it.p0);
-      else if (O <= 24)
-        z_item<24, 2>(fo, x, Z, O, Pv, i, it.p0);
-      else {
-        z_item<ZMAX_O, 1>(fo, x, Z, O, Pv, i, it.p0);
-        z_item<ZMAX_O, 1>(fo, x, Z, O, Pv, i, it.p0 + BLK);
-      }
+      z_item_mfma(fo, x, Z, (int)O, Pv, (int)i, it.p0);
     } else {
       for (int q = 0; q < ZS / BLK; ++q) {
         const int64_t P = it.p0 + threadIdx.x + (int64_t)BLK * q;
@@ -226,9 +233,12 @@ __global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const 
   }
 }
 
-// One work item per workgroup of SBLK threads: a 32 x 32 tile of the residual, each thread a
-// 4 x 2 block of (a, b) for both products, so every LDS value a thread reads feeds 2 or 4
-// complex multiply-adds.
+// One work item per workgroup of four waves: a 32 x 32 tile of the residual, each wave a 16 x 16
+// quarter of it on the FP64 matrix cores.  Both products go into one complex accumulator,
+//   r[a][b] += sum_e XA[a][e] GB[e][b] - sum_e GA[e][a] XB[b][e],
+// as eight v_mfma_f64_16x16x4f64 per four values of e (real and imaginary planes; the second
+// product with its A operand negated).  MFMA f64 16x16x4 lanes: A[l%16][l/16], B[l/16][l%16],
+// D[l/16 + 4r][l%16].
 __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instance* __restrict__ insts,
                                                         const Item* __restrict__ items, int reverse,
                                                         const char* __restrict__ ws, const int64_t* __restrict__ zoff,
@@ -240,7 +250,7 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
   const int64_t O = I.O, V = I.V, n = O + V, Pv = npair(V);
   const int transpose_f = I.transpose_f;
   const double s = I.drv_scale;
-  const int tid = threadIdx.x, tx = tid % SBX, ty = tid / SBX;
+  const int tid = threadIdx.x, lane = tid & 63, wm = (tid >> 6) & 1, wn = tid >> 7;
   int64_t i, j;
   pdec(it.pij, O, i, j);
   const double2* F = (const double2*)I.fock;
@@ -251,11 +261,8 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
   // Fvv(b, e) (F^T: F[O+e][O+b])
   auto fvv = [&](int64_t b, int64_t e) { return transpose_f ? F[(O + e) * n + O + b] : F[(O + b) * n + O + e]; };
 
-  double2 yab[RM][RN], yba[RM][RN];
-#pragma unroll
-  for (int r = 0; r < RM; ++r)
-#pragma unroll
-    for (int c = 0; c < RN; ++c) { yab[r][c] = make_double2(0.0, 0.0); yba[r][c] = make_double2(0.0, 0.0); }
+  v4d acc_re = (v4d){0, 0, 0, 0}, acc_im = (v4d){0, 0, 0, 0};
+  const bool lower_quarter = it.ta == it.tb && wm > wn;   // uniform across the wave
   // one e chunk of x_ij rows a0.., b0.. and of Fvv columns, fetched into registers while the previous
   // chunk is multiplied (NQ elements of each of the four tiles per thread), then stored to LDS
   constexpr int NQ = MT * MKC / SBLK;
@@ -310,20 +317,21 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
     }
     __syncthreads();
     if (e0 + MKC < Vi) fetch(e0 + MKC);              // in flight during the products below
-#pragma unroll 2
-    for (int le = 0; le < MKC; ++le) {
-      double2 xa[RM], ga[RM], xb[RN], gb[RN];
+    // on a diagonal tile pair the lower-left quarter holds only a > b (not stored): its wave skips the products
 #pragma unroll
-      for (int r = 0; r < RM; ++r) { xa[r] = XA[ty + SBY * r][le]; ga[r] = GA[le][ty + SBY * r]; }
-#pragma unroll
-      for (int c = 0; c < RN; ++c) { xb[c] = XB[tx + SBX * c][le]; gb[c] = GB[le][tx + SBX * c]; }
-#pragma unroll
-      for (int r = 0; r < RM; ++r)
-#pragma unroll
-        for (int c = 0; c < RN; ++c) {
-          cfma(yab[r][c], xa[r], gb[c]);             // sum_e x[a,e] Fvv(b,e)
-          cfma(yba[r][c], xb[c], ga[r]);             // sum_e x[b,e] Fvv(a,e)
-        }
+    for (int ks = 0; ks < MKC / 4; ++ks) {
+      if (lower_quarter) break;
+      const int kk = ks * 4 + (lane >> 4), ra = wm * 16 + (lane & 15), cb = wn * 16 + (lane & 15);
+      const double2 xa = XA[ra][kk], gb = GB[kk][cb];   // sum_e x[a,e] Fvv(b,e)
+      const double2 ga = GA[kk][ra], xb = XB[cb][kk];   // - sum_e Fvv(a,e) x[b,e]
+      acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.x, gb.x, acc_re, 0, 0, 0);
+      acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.x, gb.y, acc_im, 0, 0, 0);
+      acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(-xa.y, gb.y, acc_re, 0, 0, 0);
+      acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.y, gb.x, acc_im, 0, 0, 0);
+      acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(-ga.x, xb.x, acc_re, 0, 0, 0);
+      acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(-ga.x, xb.y, acc_im, 0, 0, 0);
+      acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(ga.y, xb.y, acc_re, 0, 0, 0);
+      acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(-ga.y, xb.x, acc_im, 0, 0, 0);
     }
   }
 
@@ -333,17 +341,14 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
   const double eij = eps[i] + eps[j];
   double acc = 0.0;
 #pragma unroll
-  for (int r = 0; r < RM; ++r)
-#pragma unroll
-    for (int c = 0; c < RN; ++c) {
-      const int64_t a = a0 + ty + SBY * r, b = b0 + tx + SBX * c;
+  for (int r = 0; r < 4; ++r) {
+      const int64_t a = a0 + wm * 16 + (lane >> 4) + 4 * r, b = b0 + wn * 16 + (lane & 15);
       if (b >= V || a >= b) continue;
       const int64_t Pab = pidx(a, b, V);
       const double2 d = drv[((i * O + j) * V + a) * V + b];
       // -Z_ij + Z_ji: -sum_m Foo(m,j) x_im[a,b] + sum_m Foo(m,i) x_jm[a,b]
       const double2 zij = Z[(i * O + j) * Pv + Pab], zji = Z[(j * O + i) * Pv + Pab];
-      const double2 rr = make_double2(s * d.x + yab[r][c].x - yba[r][c].x - zij.x + zji.x,
-                                      s * d.y + yab[r][c].y - yba[r][c].y - zij.y + zji.y);
+      const double2 rr = make_double2(s * d.x + acc_re[r] - zij.x + zji.x, s * d.y + acc_im[r] - zij.y + zji.y);
       const double D = eij - eps[O + a] - eps[O + b];
       const double2 xv = row[Pab];
       xn[(int64_t)it.pij * Pv + Pab] = make_double2(xv.x + rr.x / D, xv.y + rr.y / D);
