@@ -270,6 +270,7 @@ SIGNATURES = {
     "apy_cisd_sp_step_multi": (c_int, [ctypes.POINTER(CisdSpArgs), c_i64, c_vp, c_size, c_vp]),
     "apy_cisd_sp_tune": (c_int, [ctypes.POINTER(CisdSpArgs), c_i64, c_int, c_vp, c_size, c_int, c_vp]),
     "apy_cisd_sp_fold_fock": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "apy_sp_amplitudes_to_so": (c_int, [c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp]),
     "apy_mp2_sp_response_workspace": (c_size, [c_i64, c_i64, c_i64]),
     "apy_mp2_sp_response": (c_int, [c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_mp2_sp_densities_workspace": (c_size, [c_i64, c_i64]),

@@ -262,6 +262,40 @@ __global__ __launch_bounds__(256) void sp_fold_kernel(int64_t o, int64_t v, int6
   }
 }
 
+// Spin-orbital amplitudes of nb closed-shell solutions (spin orbital P = 2p + spin, spin 0 = alpha, as
+// utils.py:283-365 orders them), with t2 taken pair-symmetrised (t_ijab + t_jiba)/2:
+// T1[I][A] = t1[i][a] for equal spins; T2[I][J][A][B] = t_ijab where
+// (I, J) and (A, B) carry the spins (s, s') with s != s', -t_ijba for (s, s') against (s', s), and
+// t_ijab - t_ijba when all four spins agree; zero where spin is not conserved.
+__global__ __launch_bounds__(256) void sp_to_so_kernel(int64_t o, int64_t v, int64_t nb, int singles,
+                                                       const double2* __restrict__ t, double2* __restrict__ T1,
+                                                       double2* __restrict__ T2) {
+  const int64_t O = 2 * o, V = 2 * v, ov = o * v, L = ov + ov * ov, n1 = O * V, n2 = n1 * n1, per = n1 + n2;
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nb * per; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = u / per, w = u % per;
+    const double2* tq = t + q * L;
+    if (w < n1) {
+      const int64_t I = w / V, A = w % V;
+      T1[q * n1 + w] = (singles && (I & 1) == (A & 1)) ? tq[(I >> 1) * v + (A >> 1)] : make_double2(0.0, 0.0);
+      continue;
+    }
+    const int64_t e = w - n1;
+    const int64_t B = e % V, A = (e / V) % V, J = (e / (V * V)) % O, I = e / (V * V * O);
+    const int64_t i = I >> 1, j = J >> 1, a = A >> 1, b = B >> 1;
+    const int si = I & 1, sj = J & 1, sa = A & 1, sb = B & 1;
+    const double2* t2 = tq + ov;
+    // the pair-symmetrised t_ijab and t_ijba: T comes out exactly antisymmetric in (I,J) and in (A,B)
+    const double2 p0 = t2[((i * o + j) * v + a) * v + b], p1 = t2[((j * o + i) * v + b) * v + a];
+    const double2 q0 = t2[((i * o + j) * v + b) * v + a], q1 = t2[((j * o + i) * v + a) * v + b];
+    const double2 x = make_double2(0.5 * (p0.x + p1.x), 0.5 * (p0.y + p1.y));
+    const double2 y = make_double2(0.5 * (q0.x + q1.x), 0.5 * (q0.y + q1.y));
+    double2 r = make_double2(0.0, 0.0);
+    if (si == sa && sj == sb) { r.x += x.x; r.y += x.y; }
+    if (si == sb && sj == sa) { r.x -= y.x; r.y -= y.y; }
+    T2[q * n2 + e] = r;
+  }
+}
+
 // T1[q][(i,a)] = t1 of vector q (stride L): the singles of nb vectors stacked for one product
 __global__ __launch_bounds__(256) void sp_t1_stack(int64_t ov, int64_t nb, int64_t L, const double2* __restrict__ tv,
                                                    double2* __restrict__ T1) {
@@ -781,6 +815,16 @@ int apy_cisd_sp_fold_fock(int64_t o, int64_t v, int64_t nb, const void* fock, co
   const int64_t n = o + v, total = nb * 2 * (o * v) * (o * v);
   sp_fold_kernel<<<apy_grid1d(total, 1 << 20), 256, 0, (hipStream_t)stream>>>(
       o, v, nb, (const double2*)fock, n * n, (const double2*)w21, (double2*)w21_f);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_sp_amplitudes_to_so(int64_t o, int64_t v, int64_t nb, int singles, const void* t, void* t1_so, void* t2_so,
+                            void* stream) {
+  if (o < 1 || v < 1 || nb < 1 || !t || !t1_so || !t2_so) return APY_E_ARG;
+  const int64_t n1 = 4 * o * v, total = nb * (n1 + n1 * n1);
+  sp_to_so_kernel<<<apy_grid1d(total, 1 << 20), 256, 0, (hipStream_t)stream>>>(
+      o, v, nb, singles ? 1 : 0, (const double2*)t, (double2*)t1_so, (double2*)t2_so);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }

@@ -105,15 +105,21 @@ _SOLVER_KEYS = ('method', 'e_convergence', 'd_convergence', 'DIIS', 'max_iterati
 
 
 def _solve_spatial_set(parameter_list, wfns):
-    """The spin-adapted CISD / CID solves (ci_wfn.py:420-574, :51-167) of a
-    finite-difference set as one spatial_ci.SpatialCISet: every instance's
-    MO integrals and Fock matrix from its own ci_wfn, the Jacobi + DIIS
-    iterations of all of them in one batched step per iteration.  Each
-    result equals its own ci_wfn solve (printing the same amplitude lines).
-    None when the set does not qualify (other methods, one instance,
-    differing solver settings or sizes)."""
+    """The CISD / CID solves of a finite-difference set as one
+    spatial_ci.SpatialCISet: every instance's MO integrals and Fock matrix
+    from its own ci_wfn, the Jacobi + DIIS iterations of all of them in one
+    batched spin-adapted step per iteration.  Spin-adapted methods
+    (ci_wfn.py:420-574, :51-167): each result equals its own ci_wfn solve
+    (printing the same amplitude lines).  Spin-orbital methods (CISD_SO,
+    CID_SO, ci_wfn.py:171-416): for the RHF reference the spin-orbital
+    equations are solved by the spin-adapted amplitudes expanded to spin
+    orbitals (apy_sp_amplitudes_to_so), so the set is solved spin-adapted
+    and returned in the spin-orbital layout, equal to the per-instance
+    spin-orbital solves to their convergence threshold.  None when the set
+    does not qualify (other methods, one instance, differing solver settings
+    or sizes)."""
     p0 = parameter_list[0]
-    if p0['method'] not in ('CISD', 'CID') or len(parameter_list) < 2:
+    if p0['method'] not in ('CISD', 'CID', 'CISD_SO', 'CID_SO') or len(parameter_list) < 2:
         return None
     if any(any(p.get(k) != p0.get(k) for k in _SOLVER_KEYS) for p in parameter_list):
         return None
@@ -121,11 +127,17 @@ def _solve_spatial_set(parameter_list, wfns):
     shapes = {(len(c.eps_o), len(c.eps_v)) for c in cws}
     if len(shapes) != 1:
         return None
-    singles = p0['method'] == 'CISD'
+    singles = p0['method'] in ('CISD', 'CISD_SO')
     cset = SpatialCISet([c._ERI_MO for c in cws], [c._F_MO for c in cws],
                         [np.concatenate([c.eps_o, c.eps_v]) for c in cws], len(cws[0].eps_o), singles=singles)
     E, t1, t2, _ = cset.solve(p0)
     out = []
+    if p0['method'].endswith('_SO'):                  # the spin-orbital solves print nothing at print_level 0
+        T1, T2 = cset.spin_orbital_amplitudes()
+        for q in range(len(cws)):
+            t1q = T1[q].cpu().numpy() if singles else 0
+            out.append((np.complex128(E[q]), t1q, T2[q].cpu().numpy()))
+        return out
     for q in range(len(cws)):
         t2s = t2[q].cpu().numpy()
         print("t-Amplitude Data:")

@@ -436,6 +436,20 @@ class SpatialCISet:
         t2 [nb,o,o,v,v] device views, iterations per system)."""
         E, X, iters = _solve_set(self._step, self._energies, self.nb, self.L, self.OV, self.singles,
                                  self.device, params, print_level, max_diis)
+        self.X = X
         nb, O, V = self.nb, self.O, self.V
         t1 = X[:, :self.OV].view(nb, O, V) if self.singles else None
         return E, t1, X[:, self.OV:].view(nb, O, O, V, V), iters
+
+    def spin_orbital_amplitudes(self):
+        """The solved amplitudes in the spin-orbital layout of solve_CISD_SO /
+        solve_CID_SO (ci_wfn.py:171-416; spin orbital 2p + spin, utils.py:283-365):
+        (t1 [nb, 2o, 2v], t2 [nb, 2o, 2o, 2v, 2v]) device tensors
+        (apy_sp_amplitudes_to_so).  For an RHF reference the spin-orbital
+        equations are solved by exactly these amplitudes."""
+        nb, o, v = self.nb, self.O, self.V
+        t1 = torch.empty((nb, 2 * o, 2 * v), dtype=CDT, device=self.device)
+        t2 = torch.empty((nb, 2 * o, 2 * o, 2 * v, 2 * v), dtype=CDT, device=self.device)
+        _lib.check(self.lib.apy_sp_amplitudes_to_so(o, v, nb, int(self.singles), _vp(self.X), _vp(t1), _vp(t2),
+                                                    stream_of(self.device)), "apy_sp_amplitudes_to_so")
+        return t1, t2

@@ -211,6 +211,15 @@ typedef struct apy_cisd_sp_args {
   const void* w21_f;
 } apy_cisd_sp_args;
 
+/* Spin-orbital amplitudes of nb closed-shell (spin-adapted) solutions t [nb][o v + o^2 v^2] in the
+ * spin-orbital layout of ci_wfn.py:263-416 (spin orbital 2p + spin, alpha first, utils.py:283-365):
+ * t1_so [nb][2o][2v] (zero when singles = 0), t2_so [nb][2o][2o][2v][2v] with
+ * T_{IJAB} = t_ijab for opposite-spin pairs matched (I,A),(J,B), -t_ijba matched crosswise, and
+ * t_ijab - t_ijba for four equal spins.  For an RHF reference these are the solutions of the
+ * spin-orbital CISD/CID equations (the spin-orbital Hamiltonian is spin-free). */
+int apy_sp_amplitudes_to_so(int64_t o, int64_t v, int64_t nb, int singles, const void* t, void* t1_so, void* t2_so,
+                            void* stream);
+
 /* w21_f[q] = w21[q] + delta_kj F_q[o+b][o+c] on the rows (k,c) < ov, for nb
  * instances (fock + q n^2, w21 and w21_f + q 2 (ov)^2): the ring operator with
  * the F_vv term of the doubles residual (ci_wfn.py:471-472) folded in. */

@@ -65,6 +65,27 @@ def test_phase_corrected_energies_equal_one_by_one(dev, method):
         assert np.abs(np.asarray(T_list[2]) - np.asarray(T1[2])).max() < 1e-12
 
 
+@pytest.mark.parametrize("method", ["CISD_SO", "CID_SO"])
+def test_spin_orbital_set_equals_one_by_one(dev, method):
+    """A spin-orbital finite-difference set solved as one spin-adapted set and
+    expanded to spin orbitals against the per-instance spin-orbital solves
+    (ci_wfn.py:171-416 on the SO blocks): energies and amplitudes equal to
+    the convergence threshold (1e-12 on energy and amplitude change)."""
+    p0, plist = _displaced(method)
+    _, _, C0, basis0 = energy(p0)
+    got = phase_corrected_energies(plist, basis0, C0)
+    for p, (E_list, T_list, C, basis) in zip(plist, got):
+        E1, T1, Cs, b1 = phase_corrected_energy(p, basis0, C0)
+        assert abs(complex(E_list[1]) - complex(E1[1])) < 1e-10
+        t2, t2s = np.asarray(T_list[2]), np.asarray(T1[2])
+        assert t2.shape == t2s.shape and np.abs(t2 - t2s).max() < 1e-9
+        assert np.abs(t2 + t2.transpose(1, 0, 2, 3)).max() == 0     # antisymmetric, as the SO solver's
+        if method == "CISD_SO":
+            assert np.abs(np.asarray(T_list[1]) - np.asarray(T1[1])).max() < 1e-9
+        else:
+            assert T_list[1] == 0 and T1[1] == 0
+
+
 def _spatial_system(o, v, seed, std=0.05, noise=1e-3):
     rng = np.random.default_rng(seed)
     n = o + v
