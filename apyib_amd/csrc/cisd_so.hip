@@ -220,22 +220,47 @@ __device__ inline int64_t pidx(int64_t p, int64_t q, int64_t N) { return prow(p,
 
 // T'[(i,a),(k,c)] = t2[i,k,a,c] from T_p: +T_p[(ik),(ac)] when i<k and a<c
 // (or both reversed), minus when one pair is reversed, zero on i == k or a == c.
+// One workgroup per (i, k, 32 x 32 tile of (a, c)): the packed row (ik) is read in two passes, the
+// a < c half with c fastest and the a > c half with a fastest, so that both read along the packed
+// row; the tile goes through LDS and out along c.
+constexpr int RT = 32;
 __global__ __launch_bounds__(256) void ring_perm_packed(const double2* __restrict__ tp, double2* __restrict__ tr,
                                                         int64_t O, int64_t V) {
-  const int64_t Pv = npair(V), total = O * V * O * V;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = t;
-    const int64_t c = r % V; r /= V;
-    const int64_t k = r % O; r /= O;
-    const int64_t a = r % V;
-    const int64_t i = r / V;
-    double2 v = make_double2(0.0, 0.0);
-    if (i != k && a != c) {
-      const int64_t p = i < k ? i : k, q = i < k ? k : i, x = a < c ? a : c, y = a < c ? c : a;
-      const double2 w = tp[pidx(p, q, O) * Pv + pidx(x, y, V)];
-      v = ((i < k) == (a < c)) ? w : make_double2(-w.x, -w.y);
+  __shared__ double2 T[RT][RT + 1];
+  const int64_t Pv = npair(V), OV = O * V, nt = (V + RT - 1) / RT, tiles = O * O * nt * nt;
+  const int tx = threadIdx.x % RT, ty = threadIdx.x / RT;
+  for (int64_t w = blockIdx.x; w < tiles; w += gridDim.x) {
+    const int64_t tc = w % nt, ta = (w / nt) % nt, k = (w / (nt * nt)) % O, i = w / (nt * nt * O);
+    const int64_t a0 = ta * RT, c0 = tc * RT;
+    const double2* row = i != k ? tp + pidx(i < k ? i : k, i < k ? k : i, O) * Pv : nullptr;
+    const double su = i < k ? 1.0 : -1.0;     // sign of the a < c half; the a > c half has -su
+    __syncthreads();                            // the previous tile has been written out
+#pragma unroll
+    for (int r = 0; r < RT / 8; ++r) {
+      const int ra = ty + 8 * r;
+      // upper pass (c fastest): a < c, and every slot the lower pass does not cover (zeros)
+      const int64_t a = a0 + ra, c = c0 + tx;
+      if (!(row && a < V && c < V && a > c)) {
+        double2 x = make_double2(0.0, 0.0);
+        if (row && a < V && c < V && a < c) {
+          const double2 y = row[pidx(a, c, V)];
+          x = make_double2(su * y.x, su * y.y);
+        }
+        T[ra][tx] = x;
+      }
+      // lower pass (a fastest): a > c
+      const int64_t al = a0 + tx, cl = c0 + ra;
+      if (row && al < V && cl < V && al > cl) {
+        const double2 y = row[pidx(cl, al, V)];
+        T[tx][ra] = make_double2(-su * y.x, -su * y.y);
+      }
     }
-    tr[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RT / 8; ++r) {
+      const int64_t a = a0 + ty + 8 * r, c = c0 + tx;
+      if (a < V && c < V) tr[(i * V + a) * OV + k * V + c] = T[ty + 8 * r][tx];
+    }
   }
 }
 
@@ -747,7 +772,8 @@ int sigma_body(int64_t O, int64_t V, int nb, int singles, const apy_so_ints* I, 
       pack_t2<<<apy_grid1d(PP, 1 << 20), 256, 0, st>>>(t[k] + OV, tpk, O, V);
       APY_LAUNCH_CHECK();
     }
-    ring_perm_packed<<<apy_grid1d(OV * OV, 1 << 20), 256, 0, st>>>(tpk, w.tr + k * OV * OV, O, V);
+    ring_perm_packed<<<(unsigned)std::min<int64_t>(O * O * ((V + RT - 1) / RT) * ((V + RT - 1) / RT), 1 << 20), 256, 0, st>>>(
+        tpk, w.tr + k * OV * OV, O, V);
     APY_LAUNCH_CHECK();
     if (singles) {
       half_pack_packed<<<apy_grid1d(O * O * Pv, 1 << 20), 256, 0, st>>>(tpk, w.th + k * O * O * Pv, O, Pv);
