@@ -135,6 +135,15 @@ class CisdSoSigmaBatchArgs(ctypes.Structure):
     ]
 
 
+class CisdSoFockSigmaArgs(ctypes.Structure):
+    _fields_ = [
+        ("O", c_i64), ("V", c_i64),
+        ("nf", c_int),
+        ("fock", ctypes.POINTER(c_vp)), ("hoo", ctypes.POINTER(c_vp)),
+        ("t", c_vp), ("out", ctypes.POINTER(c_vp)),
+    ]
+
+
 class Mp2Instance(ctypes.Structure):
     _fields_ = [
         ("O", c_i64), ("V", c_i64),
@@ -205,6 +214,8 @@ SIGNATURES = {
     "apy_cisd_so_tune": (c_int, [ctypes.POINTER(CisdSoArgs), c_vp, c_size, c_int, c_vp]),
     "apy_cisd_so_sigma_batch_workspace": (c_size, [c_i64, c_i64, c_int, c_int]),
     "apy_cisd_so_sigma_batch": (c_int, [ctypes.POINTER(CisdSoSigmaBatchArgs), c_vp, c_size, c_vp]),
+    "apy_cisd_so_fock_sigma_workspace": (c_size, [c_i64, c_i64]),
+    "apy_cisd_so_fock_sigma": (c_int, [ctypes.POINTER(CisdSoFockSigmaArgs), c_vp, c_size, c_vp]),
     "apy_cisd_so_energy": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_cisd_so_energy_packed": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_zdot_many": (c_int, [c_int, _P_VP, c_vp, c_i64, c_int, c_vp, c_vp, c_size, c_vp]),

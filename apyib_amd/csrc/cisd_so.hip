@@ -362,7 +362,7 @@ constexpr int EKC = 8;   // occupied chunk of the in-kernel U contraction
 struct Epi2Args {
   int64_t O, V, Po, Pv, n;
   int ntile;
-  const double2* drv;      // vvoo_t [i][j][a][b] = <ab||ij>
+  const double2* drv;      // vvoo_t [i][j][a][b] = <ab||ij> (NULL: no driver term)
   const double2* Rp;       // [(ij)][(ab)]
   const double2* X;        // [(i,a)][(j,b)]
   const double2* QT;       // Q_T[j][i][(ab)] at QT + j*qj + i*Pv (NULL: no T1 -> T2 terms)
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(256) void epi2_doubles(Epi2Args e) {
     const int64_t a = a0 + la, b = b0 + tx;
     double2 v = make_double2(0.0, 0.0);
     if (b < V && a < b) {
-      v = cm(c0, e.drv[((i * O + j) * V + a) * V + b]);
+      if (e.drv) v = cm(c0, e.drv[((i * O + j) * V + a) * V + b]);
       const int64_t Pab = pidx(a, b, V);
       acc(v, e.Rp[Pij * Pv + Pab]);
       acc(v, e.X[(i * V + a) * OV + j * V + b]);
@@ -582,7 +582,9 @@ __global__ __launch_bounds__(256) void energy_partial(int64_t O, int64_t V, int6
     }
   }
   double qr = 0.0, qi = 0.0;
-  if (packed) {
+  if (!oovv) {
+    // one-electron part only: no doubles term
+  } else if (packed) {
     const int64_t Pv = npair(V), L2 = npair(O) * Pv;
     for (int64_t q = l; q < L2; q += stride) {
       int64_t i, j, a, b;
@@ -1052,3 +1054,137 @@ int apy_cisd_so_sigma_batch(const apy_cisd_so_sigma_batch_args* A, void* workspa
 
 }  // extern "C"
 
+// ------------------------------------------------------------ one-electron sigma
+// The Fock-linear part L(F) of the Hermitian CISD sigma on a packed vector
+// [c1 | c2 over (i<j, a<b) | c0]: the terms of ci_wfn.py:309-334 that carry
+// F_SO (r_T1: F_vo c0, -F_oo^T c1, c1 F_vv^T, F_ov.c2; r_T2: the two F_vv and
+// the two F_oo contractions) plus the Hermitian f_ov couplings (c0 row
+// F_ov.c1, doubles P(ij)P(ab) f_bj c_ia).  For a one-electron perturbation
+// dF of a Hamiltonian whose two-electron integrals stay fixed (the field-
+// displaced copies of fin_diff.py:335-370), H(F0 + dF) = H(F0) + L(dF)
+// exactly, so the sigma of every perturbed instance follows from one full
+// sigma and this O^2 V^3 product.  The products are the step's own GEMMs
+// (ring permutation, 1/2 T' F_vv^T into X, pair operator of F alone) and its
+// epilogues with the two-electron pieces left out.
+namespace {
+
+struct FockWS {
+  double2 *tr, *X, *rp, *s1, *part;
+  void* gemm;
+  size_t gemm_bytes;
+  size_t total;
+};
+
+void fock_gemms(int64_t O, int64_t V, const double2* F, const void* hoo, const double2* t, const FockWS& w,
+                std::vector<apy_gemm_desc>& G) {
+  const int64_t Po = npair(O), Pv = npair(V), OV = O * V, n = O + V;
+  G.clear();
+  apy_gemm_desc d;
+  const double2* tp = t ? t + OV : nullptr;
+  // X[(i,a,j)][b] = 1/2 sum_c T'[(i,a,j)][c] F[O+b][O+c]
+  fill(d, O * V * O, V, V, APY_MK, w.tr, V, APY_NK, F ? F + O * n + O : nullptr, n, w.X, V, 0.5, 0.0);
+  G.push_back(d);
+  // R_p = Hoo(F) . T_p  (the F_oo terms over occupied pairs)
+  fill(d, Po, Pv, Po, APY_MK, hoo, Po, APY_KN, tp, Pv, w.rp, Pv, 1.0, 0.0);
+  G.push_back(d);
+  // s1 = c1 F_vv^T - F_oo^T c1 + sum_jb T'[(ia),(jb)] F[j][O+b]
+  fill(d, O, V, V, APY_MK, t, V, APY_NK, F ? F + O * n + O : nullptr, n, w.s1, V, 1.0, 0.0);
+  G.push_back(d);
+  fill(d, O, V, O, APY_KM, F, n, APY_KN, t, V, w.s1, V, -1.0, 1.0);
+  G.push_back(d);
+  fill(d, OV, 1, V, APY_MK, w.tr, OV, APY_KN, F ? F + O : nullptr, 1, w.s1, 1, 1.0, 1.0);
+  d.batch = O; d.stride_a = V; d.stride_b = n; d.stride_c = 0; d.reduce_batch = 1;
+  G.push_back(d);
+}
+
+FockWS carve_fock(int64_t O, int64_t V, void* base) {
+  const int64_t Po = npair(O), Pv = npair(V), OV = O * V;
+  FockWS w{};
+  size_t off = 0;
+  char* b = (char*)base;
+  auto take = [&](size_t elems) -> double2* {
+    double2* p = b ? (double2*)(b + off) : nullptr;
+    off += al256(elems * sizeof(double2));
+    return p;
+  };
+  w.tr = take((size_t)OV * OV);
+  w.X = take((size_t)OV * OV);
+  w.rp = take((size_t)Po * Pv);
+  w.s1 = take((size_t)OV);
+  w.part = take((size_t)ENERGY_BLOCKS);
+  std::vector<apy_gemm_desc> G;
+  fock_gemms(O, V, nullptr, nullptr, nullptr, w, G);
+  size_t g = 0;
+  for (const apy_gemm_desc& d : G) g = std::max(g, apy_zgemm_ws_bytes(&d));
+  w.gemm = b ? (void*)(b + off) : nullptr;
+  w.gemm_bytes = g;
+  off += al256(g);
+  w.total = off;
+  return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t apy_cisd_so_fock_sigma_workspace(int64_t O, int64_t V) {
+  if (O < 2 || V < 2) return 0;
+  return carve_fock(O, V, nullptr).total;
+}
+
+int apy_cisd_so_fock_sigma(const apy_cisd_so_fock_sigma_args* A, void* workspace, size_t ws_bytes, void* stream) {
+  if (!A || !A->fock || !A->hoo || !A->t || !A->out) return APY_E_ARG;
+  const int64_t O = A->O, V = A->V;
+  const int nf = A->nf;
+  if (O < 2 || V < 2 || nf < 1 || nf > 4096) return APY_E_ARG;
+  for (int f = 0; f < nf; ++f)
+    if (!A->fock[f] || !A->hoo[f] || !A->out[f] || A->out[f] == A->t) return APY_E_ARG;
+  FockWS w = carve_fock(O, V, workspace);
+  if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t Po = npair(O), Pv = npair(V), OV = O * V, n = O + V, PP = Po * Pv;
+  const double2* t = (const double2*)A->t;
+  const double2* c0p = t + OV + PP;
+  ring_perm_packed<<<(unsigned)std::min<int64_t>(O * O * ((V + RT - 1) / RT) * ((V + RT - 1) / RT), 1 << 20), 256, 0, st>>>(
+      t + OV, w.tr, O, V);
+  APY_LAUNCH_CHECK();
+  std::vector<apy_gemm_desc> G;
+  for (int f = 0; f < nf; ++f) {
+    const double2* F = (const double2*)A->fock[f];
+    double2* out = (double2*)A->out[f];
+    fock_gemms(O, V, F, A->hoo[f], t, w, G);
+    for (const apy_gemm_desc& d : G) {
+      const int s = apy_zgemm_launch(&d, w.gemm, w.gemm_bytes, st);
+      if (s != APY_OK) return s;
+    }
+    Epi2Args e{};
+    e.O = O; e.V = V; e.Po = Po; e.Pv = Pv; e.n = n;
+    e.ntile = (int)((V + ET - 1) / ET);
+    e.drv = nullptr;
+    e.Rp = w.rp;
+    e.X = w.X;
+    e.QT = nullptr;
+    e.qj = 0;
+    e.ovT = nullptr;
+    e.t1 = t;
+    e.F = F;
+    e.fov = 1;
+    e.c0p = c0p;
+    e.Tp = t + OV;
+    e.out = out + OV;
+    e.mode = APY_MODE_SIGMA;
+    e.packed = 1;
+    epi2_doubles<<<epi_grid(O, V), 256, 0, st>>>(e);
+    APY_LAUNCH_CHECK();
+    epi_singles<<<apy_grid1d(OV), 256, 0, st>>>(O, V, n, F, nullptr, nullptr, 0.0, 0.0, w.s1, t, nullptr, out,
+                                                APY_MODE_SIGMA, c0p);
+    APY_LAUNCH_CHECK();
+    energy_partial<<<ENERGY_BLOCKS, 256, 0, st>>>(O, V, n, 1, 1, F, nullptr, t, w.part);
+    APY_LAUNCH_CHECK();
+    finalize_sum<<<1, 256, 0, st>>>(w.part, ENERGY_BLOCKS, out + OV + PP);
+    APY_LAUNCH_CHECK();
+  }
+  return APY_OK;
+}
+
+}  // extern "C"

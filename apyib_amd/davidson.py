@@ -455,3 +455,260 @@ class MultiDavidson:
             d.combine([x], [1.0 / nx], x)
             d.c = x
         return np.array([d.E for d in self.dav])
+
+
+class FieldSetDavidson:
+    """Lowest roots of a set of Hermitian CISD problems that differ only in a
+    one-electron term, H_k = H(F0 + s_k dF_{d_k}) with shared <pq||rs> (the
+    +/- field- or one-electron-displaced instances of fin_diff.py:335-370),
+    solved in ONE common subspace.
+
+    The sigma is affine in F, so H_k b = H(F0) b + s_k L(dF_d) b with L the
+    Fock-linear part (apy_cisd_so_fock_sigma, O^2 V^3).  Every subspace vector
+    b_j therefore carries one full sigma S_j = H(F0) b_j and one cheap
+    product P_dj = L(dF_d) b_j per direction, and every instance's projected
+    matrix is G0 + s_k G_d with G0 = B^H S, G_d = B^H P_d.  The root of F0 is
+    solved first (CISDDavidson on packed vectors); the shared phase starts
+    from it and, each iteration, adds the preconditioned residuals of the
+    unconverged instances after Gram-Schmidt against the subspace and each
+    other, dropping a candidate whose part outside the span is below `drop`
+    of its norm: the residuals of a +/- pair are equal and opposite to first
+    order in the step, so a pair adds one direction (one full sigma) where
+    two separate solves would each need their own.  When the subspace is
+    full it collapses onto the span of the instances' Ritz vectors (their
+    sigmas and L-products by linear combination, no new sigma).  Each
+    instance stops on the test of CISDDavidson.solve (Ritz value change below
+    e_conv and residual norm below r_conv)."""
+
+    def __init__(self, ints, F0, eps, directions, instances, max_space=16, base_space=12, drop=1e-2):
+        from .so_ci import SOCI, npair
+        self.soci = SOCI(ints, F0, eps, singles=True, fov_coupling=True)
+        self.base = CISDDavidson(self.soci, base_space, packed=True)
+        if not self.base.packed:
+            raise ValueError("FieldSetDavidson needs O >= 2 and V >= 2 (packed vectors)")
+        self.lib, self.dev = self.base.lib, self.base.dev
+        self.O, self.V, self.Lv = self.base.O, self.base.V, self.base.Lv
+        self.dirs = [torch.as_tensor(d, dtype=CDT, device=self.dev).contiguous() for d in directions]
+        n = self.O + self.V
+        for d in self.dirs:
+            if d.shape != (n, n):
+                raise ValueError(f"direction matrices must be [{n}][{n}]")
+        self.inst = [(int(d), float(s)) for d, s in instances]
+        if any(not 0 <= d < len(self.dirs) for d, _ in self.inst):
+            raise ValueError("instance direction index out of range")
+        self.M = int(max_space)
+        if self.M < len(self.inst) + 2:
+            raise ValueError("max_space must exceed the number of instances by at least 2")
+        self.drop = float(drop)
+        Po = npair(self.O)
+        self.hoo_d = []
+        for d in self.dirs:
+            h = torch.zeros(Po * Po, dtype=CDT, device=self.dev)
+            _lib.check(self.lib.apy_cisd_so_pair_operator(self.O, self.V, _vp(d), None, _vp(h), stream_of(self.dev)),
+                       "apy_cisd_so_pair_operator")
+            self.hoo_d.append(h)
+        self.fs_bytes = int(self.lib.apy_cisd_so_fock_sigma_workspace(self.O, self.V))
+        nd, K, M, Lv = len(self.dirs), len(self.inst), self.M, self.Lv
+        # two copies of [B | S | P_0 .. P_nd-1], swapped at a collapse
+        self._bufs = [torch.zeros((2 + nd, M, Lv), dtype=CDT, device=self.dev) for _ in range(2)]
+        self._cur = 0
+        self.R = torch.zeros((K, Lv), dtype=CDT, device=self.dev)        # residuals
+        self.C = torch.zeros((K, Lv), dtype=CDT, device=self.dev)        # corrections (candidates)
+        self.X = torch.zeros((K, Lv), dtype=CDT, device=self.dev)        # Ritz vectors (final)
+        self._dbuf = torch.zeros(_lib.APY_MAXVEC, dtype=CDT, device=self.dev)
+        self.nsigma = 0
+        self.nfock = 0
+
+    # -------------------------------------------------------------- primitives
+    def _arrays(self, which=None):
+        buf = self._bufs[self._cur if which is None else which]
+        return buf[0], buf[1], [buf[2 + q] for q in range(len(self.dirs))]
+
+    def _dots(self, xs, y):
+        """<x_m|y> (packed vectors: Euclidean), host array; one read per 16 vectors."""
+        out = np.zeros(len(xs), dtype=np.complex128)
+        M = _lib.APY_MAXVEC
+        for s0 in range(0, len(xs), M):
+            self.base._dots_dev(xs[s0:s0 + M], y, self._dbuf)
+            out[s0:s0 + M] = self._dbuf[:len(xs[s0:s0 + M])].cpu().numpy()
+        return out
+
+    def _subtract(self, xs, coef, y):
+        """y -= sum_m coef_m x_m, in calls of at most APY_MAXVEC - 1 inputs plus y."""
+        M = _lib.APY_MAXVEC - 1
+        for s0 in range(0, len(xs), M):
+            chunk = list(xs[s0:s0 + M])
+            self.base.combine(chunk + [y], list(-np.asarray(coef[s0:s0 + M])) + [1.0], y)
+
+    def _products(self, q):
+        """S[q] = H(F0) B[q] (full sigma) and P_d[q] = L(dF_d) B[q] for every direction."""
+        B, S, P = self._arrays()
+        n = self.Lv - 1
+        self.soci.sigma(B[q][:n], S[q][:n], c0_dev=B[q][n:].data_ptr(), packed=True)
+        self.soci.energy(B[q][:n], out=S[q][n:], packed=True)
+        self.nsigma += 1
+        self._fock(q)
+
+    def _fock(self, q):
+        """P_d[q] = L(dF_d) B[q] for every direction (one apy_cisd_so_fock_sigma call)."""
+        B, S, P = self._arrays()
+        nd = len(self.dirs)
+        f = (ctypes.c_void_p * nd)(*[d.data_ptr() for d in self.dirs])
+        h = (ctypes.c_void_p * nd)(*[x.data_ptr() for x in self.hoo_d])
+        o = (ctypes.c_void_p * nd)(*[Pd[q].data_ptr() for Pd in P])
+        args = _lib.CisdSoFockSigmaArgs(self.O, self.V, nd, f, h, B[q].data_ptr(), o)
+        ws = workspace(self.fs_bytes, self.dev)
+        _lib.check(self.lib.apy_cisd_so_fock_sigma(ctypes.byref(args), _vp(ws), self.fs_bytes, stream_of(self.dev)),
+                   "apy_cisd_so_fock_sigma")
+        self.nfock += 1
+
+    def _column(self, q):
+        """Row/column q of G0 and every G_d from dots of the new vector."""
+        B, S, P = self._arrays()
+        bs = [B[j] for j in range(q + 1)]
+        for G, Y in [(self.G0, S)] + list(zip(self.Gd, P)):
+            col = self._dots(bs, Y[q])
+            G[:q + 1, q] = col
+            G[q, :q + 1] = np.conj(col)
+
+    def _collapse(self, Ys):
+        """Replace the m-vector subspace by the span of the coefficient vectors
+        Ys (m x K), orthonormalised on the host; B, S, P follow by linear
+        combination into the other buffer."""
+        m = self.m
+        U, sv, _ = np.linalg.svd(Ys, full_matrices=False)
+        keep = sv > 1e-12 * sv[0]
+        U = U[:, keep]
+        src, dst = self._cur, 1 - self._cur
+        Bs, Ss, Ps = self._arrays(src)
+        Bd, Sd, Pd = self._arrays(dst)
+        for c in range(U.shape[1]):
+            coef = list(U[:, c])
+            self.base.combine([Bs[j] for j in range(m)], coef, Bd[c])
+            self.base.combine([Ss[j] for j in range(m)], coef, Sd[c])
+            for Pq_s, Pq_d in zip(Ps, Pd):
+                self.base.combine([Pq_s[j] for j in range(m)], coef, Pq_d[c])
+        self._cur = dst
+        Uh = U.conj().T
+        k = U.shape[1]
+        G0 = np.zeros_like(self.G0)
+        G0[:k, :k] = Uh @ self.G0[:m, :m] @ U
+        self.G0 = G0
+        for q in range(len(self.Gd)):
+            Gq = np.zeros_like(self.Gd[q])
+            Gq[:k, :k] = Uh @ self.Gd[q][:m, :m] @ U
+            self.Gd[q] = Gq
+        self.m = k
+        return U
+
+    # ------------------------------------------------------------------ solve
+    def solve(self, e_conv=1e-12, r_conv=1e-8, maxit=200, base_maxit=200, seed_subspace=False):
+        """Returns the array of the instances' correlation energies; self.E0
+        holds the root of F0, self.iterations the shared-phase iterations,
+        self.history[k] the (Ritz value, residual norm) of instance k per
+        iteration, and vector(k) its normalised packed vector.
+        seed_subspace: start the shared phase from the base solve's whole
+        final subspace (its sigmas are known; only the L-products are new)
+        instead of its root alone."""
+        self.E0 = self.base.solve(e_conv=e_conv, r_conv=r_conv, maxit=base_maxit)
+        self.base_iterations = self.base.iterations
+        self.nsigma = self.base.nsigma
+        K, nd, M = len(self.inst), len(self.dirs), self.M
+        self.G0 = np.zeros((M, M), np.complex128)
+        self.Gd = [np.zeros((M, M), np.complex128) for _ in range(nd)]
+        B, S, P = self._arrays()
+        kb, yb = self.base._ritz
+        if seed_subspace and kb + K <= M:
+            # the base subspace (orthonormal, sigmas known); its span holds the root of F0
+            for j in range(kb):
+                B[j].copy_(self.base.B[j])
+                S[j].copy_(self.base.S[j])
+        else:
+            # the root of F0 and its sigma (combination of the base subspace's sigmas)
+            self.base.combine([self.base.B[j] for j in range(kb)], list(yb), B[0])
+            self.base.combine([self.base.S[j] for j in range(kb)], list(yb), S[0])
+            nx = float(np.sqrt(abs(self._dots([B[0]], B[0])[0])))
+            self.base.combine([B[0]], [1.0 / nx], B[0])
+            self.base.combine([S[0]], [1.0 / nx], S[0])
+            kb = 1
+        for j in range(kb):
+            self._fock(j)
+            self._column(j)
+        self.m = kb
+        done = [False] * K
+        theta_old = [None] * K
+        self.E = np.zeros(K)
+        self.y = [None] * K
+        self.history = [[] for _ in range(K)]
+        self.iterations = 0
+        for it in range(1, maxit + 1):
+            self.iterations = it
+            m = self.m
+            B, S, P = self._arrays()
+            cand = []
+            for k, (d, s) in enumerate(self.inst):
+                Gk = self.G0[:m, :m] + s * self.Gd[d][:m, :m]
+                w, U = np.linalg.eigh(0.5 * (Gk + Gk.conj().T))
+                theta, y = float(w[0]), U[:, 0]
+                self.y[k] = y
+                self.E[k] = theta
+                if done[k]:
+                    continue
+                r = self.R[k]
+                self.base.combine([S[j] for j in range(m)] + [P[d][j] for j in range(m)] + [B[j] for j in range(m)],
+                                  list(y) + list(s * y) + list(-theta * y), r)
+                rn = float(np.sqrt(abs(self._dots([r], r)[0])))
+                self.history[k].append((theta, rn))
+                if theta_old[k] is not None and abs(theta - theta_old[k]) < e_conv and rn < r_conv:
+                    done[k] = True
+                    continue
+                theta_old[k] = theta
+                self.base._precondition(theta, r, self.C[k])
+                cand.append((rn, k))
+            if not cand:
+                break
+            cand = [k for _, k in sorted(cand, reverse=True)]      # largest residuals first
+            # Gram-Schmidt of the candidates against the subspace and each other
+            accepted = []
+            for k in cand:
+                dv = self.C[k]
+                n0 = float(np.sqrt(abs(self._dots([dv], dv)[0])))
+                if not n0 > 0.0:
+                    continue
+                basis = [B[j] for j in range(m)] + [self.C[a] for a in accepted]
+                for _ in range(2):
+                    self._subtract(basis, self._dots(basis, dv), dv)
+                n1 = float(np.sqrt(abs(self._dots([dv], dv)[0])))
+                if n1 > self.drop * n0:
+                    self.base.combine([dv], [1.0 / n1], dv)
+                    accepted.append(k)
+            if not accepted:
+                raise RuntimeError("FieldSetDavidson: every correction lies in the subspace")
+            if m + len(accepted) > M:
+                Ys = np.stack(self.y, axis=1)
+                self._collapse(Ys)
+                m = self.m
+                B, S, P = self._arrays()
+                accepted = accepted[:M - m]      # the collapsed span holds at most K <= M - 2 vectors
+            for a in accepted:
+                B[m].copy_(self.C[a])
+                self._products(m)
+                self.m = m + 1
+                self._column(m)
+                m += 1
+        self.converged = all(done)
+        # final Ritz vectors
+        B, S, P = self._arrays()
+        for k in range(K):
+            y = self.y[k]
+            self.base.combine([B[j] for j in range(len(y))], list(y), self.X[k])
+            nx = float(np.sqrt(abs(self._dots([self.X[k]], self.X[k])[0])))
+            self.base.combine([self.X[k]], [1.0 / nx], self.X[k])
+        return self.E.copy()
+
+    def vector(self, k):
+        """Instance k's normalised vector as dense [c1 | c2 | c0]."""
+        c = torch.empty(self.soci.L + 1, dtype=CDT, device=self.dev)
+        _lib.check(self.lib.apy_ci_unpack(self.O, self.V, _vp(self.X[k]), _vp(c), stream_of(self.dev)),
+                   "apy_ci_unpack")
+        return c

@@ -237,13 +237,17 @@ def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
     perturbations) of the Hermitian CISD lowest root.  Plain loop: one
     Davidson solve per instance on the same kernels (batch of one);
     batched: all 6 in one multi-instance Davidson (stacked PPL/ring GEMMs);
-    adjoint: one solve at the base point plus the one-particle density
-    (dE/dF), giving all three derivatives."""
+    field set: the root of the undisplaced F first, then all 6 in one
+    shared subspace (one full sigma per new direction, the +/- pairs
+    sharing theirs; L(dF) products for the rest), against the same loop on
+    the packed-vector Davidson that it runs on; adjoint: one solve at the
+    base point plus the one-particle density (dE/dF), giving all three
+    derivatives."""
     import numpy as np
     import torch
     from apyib_amd import synthetic
-    from apyib_amd.davidson import MultiDavidson
-    from apyib_amd.so_ci import SOIntegrals
+    from apyib_amd.davidson import CISDDavidson, FieldSetDavidson, MultiDavidson
+    from apyib_amd.so_ci import SOCI, SOIntegrals
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -309,6 +313,29 @@ def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
     (Epw, its_pw), t_plain_warm = timed(lambda: warm(False))
     log(f"cisd field set: one at a time warm {t_plain_warm:.2f} s, iterations {its_pw}")
 
+    def plain_packed():
+        E, its, ns = [], [], 0
+        for Fk in focks:
+            dav = CISDDavidson(SOCI(ints, Fk, eps, singles=True, fov_coupling=True), max_space)
+            E.append(dav.solve(**conv))
+            its.append(dav.iterations)
+            ns += dav.nsigma
+            del dav
+        return np.array(E), its, ns
+    (Epp, its_pp, ns_pp), t_plain_packed = timed(plain_packed)
+    log(f"cisd field set: plain loop (packed vectors) {t_plain_packed:.2f} s, iterations {its_pp}, sigmas {ns_pp}")
+
+    def field_set():
+        inst = [(d, sgn * h) for d in range(len(dirs)) for sgn in (+1, -1)]
+        fs = FieldSetDavidson(ints, F0, eps, dirs, inst, max_space=16, base_space=max_space)
+        E = fs.solve(**conv)
+        out = (E, fs.base_iterations, fs.iterations, fs.nsigma, fs.nfock, fs.converged)
+        del fs
+        return out
+    (Efs, its_fb, its_fs, ns_fs, nf_fs, conv_fs), t_field = timed(field_set)
+    log(f"cisd field set: shared subspace {t_field:.2f} s, base iterations {its_fb}, shared {its_fs}, "
+        f"sigmas {ns_fs}, L(dF) products {nf_fs}")
+
     def adjoint():
         md = MultiDavidson(ints, [F0], eps, max_space)
         md.solve(**conv)
@@ -327,6 +354,13 @@ def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
            "speedup_batched": round(t_plain / t_batch, 2), "speedup_batched_warm": round(t_plain / t_warm, 2),
            "speedup_plain_warm": round(t_plain / t_plain_warm, 2),
            "speedup_adjoint_vs_plain_fd": round(t_plain / t_adj, 2),
+           "plain_packed_s": round(t_plain_packed, 3), "sigmas_plain_packed": int(ns_pp),
+           "field_set_s": round(t_field, 3), "sigmas_field_set": int(ns_fs), "fock_products_field_set": int(nf_fs),
+           "iterations_field_set_base_then_shared": [int(its_fb), int(its_fs)], "field_set_converged": bool(conv_fs),
+           "speedup_field_set_vs_plain_packed": round(t_plain_packed / t_field, 2),
+           "speedup_field_set_vs_plain": round(t_plain / t_field, 2),
+           "max_abs_dE_field_set_vs_plain": float(np.abs(Efs - Ep).max()),
+           "max_abs_dE_plain_packed_vs_plain": float(np.abs(Epp - Ep).max()),
            "iterations_plain": its_p, "iterations_batched": its_b,
            "iterations_warm_base_then_instances": its_w,
            "max_abs_dE_warm_vs_plain": float(np.abs(Ew - Ep).max()),

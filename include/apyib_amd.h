@@ -374,6 +374,28 @@ typedef struct apy_cisd_so_sigma_batch_args {
 size_t apy_cisd_so_sigma_batch_workspace(int64_t O, int64_t V, int nb, int singles);
 int apy_cisd_so_sigma_batch(const apy_cisd_so_sigma_batch_args* a, void* workspace, size_t ws_bytes, void* stream);
 
+/* The Fock-linear part L(F) of the Hermitian CISD sigma (SIGMA mode with
+ * fov_coupling) on one packed vector [c1 | c2 over (i<j, a<b) | c0], for nf
+ * one-electron matrices at once: out_f = L(fock[f]) t.  L collects the F_SO
+ * terms of ci_wfn.py:309-334 (r_T1: F_vo c0, F_oo, F_vv, F_ov.t2; r_T2: the
+ * two F_vv and the two F_oo contractions) and the Hermitian f_ov couplings,
+ * so for a perturbation dF that leaves <pq||rs> unchanged (the field-
+ * displaced instances of fin_diff.py:335-370) sigma(F0 + dF) = sigma(F0) +
+ * L(dF) exactly.  hoo[f] is apy_cisd_so_pair_operator(O, V, fock[f], NULL)
+ * (the F_oo part alone).  No counterpart in the reference: it lets the
+ * instances of a field set share one subspace and one full sigma per
+ * direction. */
+typedef struct apy_cisd_so_fock_sigma_args {
+  int64_t O, V;
+  int nf;
+  const void* const* fock;  /* nf device pointers, [n][n] each            */
+  const void* const* hoo;   /* nf device pointers, [Po][Po] each          */
+  const void* t;            /* packed input, apy_ci_packed_length(O, V)   */
+  void* const* out;         /* nf packed outputs, none equal to t         */
+} apy_cisd_so_fock_sigma_args;
+size_t apy_cisd_so_fock_sigma_workspace(int64_t O, int64_t V);
+int apy_cisd_so_fock_sigma(const apy_cisd_so_fock_sigma_args* a, void* workspace, size_t ws_bytes, void* stream);
+
 /* E = sum_ia t1 f_ia + 1/4 sum t2 <ij||ab> (ci_wfn.py:355; CID :237 with
  * singles = 0), written to the device complex scalar `energy`. */
 int apy_cisd_so_energy(int64_t O, int64_t V, int singles, const void* fock, const void* oovv, const void* t,

@@ -2,7 +2,8 @@
 spin orbitals sharing one seeded <pq||rs> (6.3 GB packed vvvv), 6 instances
 = 3 imaginary-antisymmetric one-electron directions x +/- 1e-4 on F.  The
 batched multi-instance Davidson against per-instance plain solves (1e-10 in
-E), the adjoint dE/dF (one-particle density of the base root, the variational
+E), the shared-subspace field set against the batched solves, the
+adjoint dE/dF (one-particle density of the base root, the variational
 lambda = conj(c)) against central differences of the set (1e-7 relative to
 sum |D| |dF|), and one displaced instance's root checked as an eigenpair of
 the oracle's Hermitian CISD matrix (ci_wfn.py:309-334 + f_vo coupling).
@@ -13,7 +14,7 @@ import pytest
 import torch
 
 from apyib_amd import synthetic
-from apyib_amd.davidson import MultiDavidson
+from apyib_amd.davidson import FieldSetDavidson, MultiDavidson
 from apyib_amd.so_ci import SOIntegrals
 from oracle import davidson as odav
 from oracle import so
@@ -55,6 +56,25 @@ def test_c4_batched_equals_plain(c4):
         Ep = md.solve(**CONV)[0]
         del md
         assert abs(Eb[k] - Ep) < 1e-10, k
+
+
+def test_c4_field_set_equals_batched(c4):
+    """The shared-subspace field set (root of F0 first, then the six
+    instances in one subspace) against the batched solves: 1e-10 in E, and
+    fewer full sigmas than six separate solves take (6 x 11 iterations)."""
+    ints, F, eps, F0, dirs, focks, Eb, _ = c4
+    inst = [(d, s * H) for d in range(3) for s in (+1, -1)]
+    fs = FieldSetDavidson(ints, F0, eps, dirs, inst, max_space=16, base_space=8)
+    E = fs.solve(**CONV)
+    assert fs.converged
+    print("field set: base", fs.base_iterations, "shared", fs.iterations, "sigmas", fs.nsigma)
+    assert np.abs(E - Eb).max() < 1e-10
+    assert fs.nsigma < 6 * 11
+    fd_fs = (E[0::2] - E[1::2]) / (2 * H)
+    fd_b = (Eb[0::2] - Eb[1::2]) / (2 * H)
+    assert np.abs(fd_fs - fd_b).max() < 1e-6 * np.abs(fd_b).max()
+    del fs
+    torch.cuda.empty_cache()
 
 
 def test_c4_adjoint_matches_central_differences(c4):
