@@ -38,7 +38,11 @@ namespace {
 
 constexpr int BLK = 256;
 constexpr int MT = 32;       // virtual tile edge
-constexpr int MKC = 8;       // e chunk of the Y product
+#ifndef APY_MP2_MKC
+#define APY_MP2_MKC 8
+#endif
+constexpr int MKC = APY_MP2_MKC;   // e chunk of the Y product (a multiple of 8)
+static_assert(MKC % 8 == 0 && MKC <= 32, "e chunk");
 constexpr int SBLK = 256;                          // step kernel: 4 waves, 16 x 16 of the 32 x 32 tile each
 constexpr int EN_B = 64;     // energy partial sums per instance
 
@@ -234,17 +238,27 @@ __global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const 
 }
 
 // One work item per workgroup of four waves: a 32 x 32 tile of the residual, each wave a 16 x 16
-// quarter of it on the FP64 matrix cores.  Both products go into one complex accumulator,
+// quarter of it on the FP64 matrix cores.  Off the diagonal (A < B) both products go into one
+// complex accumulator,
 //   r[a][b] += sum_e XA[a][e] GB[e][b] - sum_e GA[e][a] XB[b][e],
 // as eight v_mfma_f64_16x16x4f64 per four values of e (real and imaginary planes; the second
-// product with its A operand negated).  MFMA f64 16x16x4 lanes: A[l%16][l/16], B[l/16][l%16],
-// D[l/16 + 4r][l%16].
+// product with its A operand negated).  On a diagonal tile pair (A == B) the two products are one
+// matrix and its transpose, so each wave forms only Y = XA GA over its quarter (four MFMAs per
+// four e) and r = Y - Y^T is assembled through LDS.  MFMA f64 16x16x4 lanes: A[l%16][l/16],
+// B[l/16][l%16], D[l/16 + 4r][l%16].
 __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instance* __restrict__ insts,
                                                         const Item* __restrict__ items, int reverse,
                                                         const char* __restrict__ ws, const int64_t* __restrict__ zoff,
                                                         double* __restrict__ part) {
-  __shared__ double2 XA[MT][MKC + 1], XB[MT][MKC + 1], GA[MKC][MT + 1], GB[MKC][MT + 1];
+  constexpr int TILE_ELEMS = 2 * MT * (MKC + 1) + 2 * MKC * (MT + 1);
+  constexpr int SH_ELEMS = TILE_ELEMS > MT * (MT + 1) ? TILE_ELEMS : MT * (MT + 1);
+  __shared__ double2 sh[SH_ELEMS];
   __shared__ double red[SBLK];
+  auto XA = reinterpret_cast<double2 (*)[MKC + 1]>(sh);
+  auto XB = reinterpret_cast<double2 (*)[MKC + 1]>(sh + MT * (MKC + 1));
+  auto GA = reinterpret_cast<double2 (*)[MT + 1]>(sh + 2 * MT * (MKC + 1));
+  auto GB = reinterpret_cast<double2 (*)[MT + 1]>(sh + 2 * MT * (MKC + 1) + MKC * (MT + 1));
+  auto Ysh = reinterpret_cast<double2 (*)[MT + 1]>(sh);    // diagonal tiles, after the products
   const Item it = items[blockIdx.x];
   const apy_mp2_nc_instance I = insts[it.inst];
   const int64_t O = I.O, V = I.V, n = O + V, Pv = npair(V);
@@ -262,7 +276,9 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
   auto fvv = [&](int64_t b, int64_t e) { return transpose_f ? F[(O + e) * n + O + b] : F[(O + b) * n + O + e]; };
 
   v4d acc_re = (v4d){0, 0, 0, 0}, acc_im = (v4d){0, 0, 0, 0};
-  const bool lower_quarter = it.ta == it.tb && wm > wn;   // uniform across the wave
+  // diagonal tile pair (A == B): XB = XA and GB = GA, so every wave forms only Y = x Fvv^T over its
+  // quarter and the residual Y[a][b] - Y[b][a] is assembled through LDS after the products
+  const bool diag = it.ta == it.tb;                          // uniform across the workgroup
   // one e chunk of x_ij rows a0.., b0.. and of Fvv columns, fetched into registers while the previous
   // chunk is multiplied (NQ elements of each of the four tiles per thread), then stored to LDS
   constexpr int NQ = MT * MKC / SBLK;
@@ -287,15 +303,17 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
       const int q = tid + SBLK * k;
       const int r = q / MKC, c = q % MKC;            // upper pass: XA[r][c] = x_ij[a0+r, e0+c], c fastest
       sxa_u[k] = upper((int)a0 + r, e0 + c);
-      sxb_u[k] = upper((int)b0 + r, e0 + c);
       const int rl = q % MT, cl = q / MT;            // lower pass: r fastest
       sxa_l[k] = lower((int)a0 + rl, e0 + cl);
-      sxb_l[k] = lower((int)b0 + rl, e0 + cl);
       // GA[rr][cc] = Fvv(a0+cc, e0+rr), read along the contiguous index of F (e, or b for F^T)
       const int rr = transpose_f ? q / MT : q % MKC, cc = transpose_f ? q % MT : q / MKC;
       const int e2 = e0 + rr, ca = (int)a0 + cc, cb = (int)b0 + cc;
       sga[k] = (e2 < Vi && ca < Vi) ? fvv(ca, e2) : make_double2(0.0, 0.0);
-      sgb[k] = (e2 < Vi && cb < Vi) ? fvv(cb, e2) : make_double2(0.0, 0.0);
+      if (!diag) {
+        sxb_u[k] = upper((int)b0 + r, e0 + c);
+        sxb_l[k] = lower((int)b0 + rl, e0 + cl);
+        sgb[k] = (e2 < Vi && cb < Vi) ? fvv(cb, e2) : make_double2(0.0, 0.0);
+      }
     }
   };
   fetch(0);
@@ -308,30 +326,46 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
       const int pa = (int)a0 + r, pb = (int)b0 + r, e = e0 + c, el = e0 + cl;
       // each element is stored once: by the lower pass where p > e is in range, else by the upper pass
       if (!(pa > e && pa < Vi && e < Vi)) XA[r][c] = sxa_u[k];
-      if (!(pb > e && pb < Vi && e < Vi)) XB[r][c] = sxb_u[k];
       if ((int)a0 + rl > el && (int)a0 + rl < Vi && el < Vi) XA[rl][cl] = sxa_l[k];
-      if ((int)b0 + rl > el && (int)b0 + rl < Vi && el < Vi) XB[rl][cl] = sxb_l[k];
       const int rr = transpose_f ? q / MT : q % MKC, cc = transpose_f ? q % MT : q / MKC;
       GA[rr][cc] = sga[k];
-      GB[rr][cc] = sgb[k];
+      if (!diag) {
+        if (!(pb > e && pb < Vi && e < Vi)) XB[r][c] = sxb_u[k];
+        if ((int)b0 + rl > el && (int)b0 + rl < Vi && el < Vi) XB[rl][cl] = sxb_l[k];
+        GB[rr][cc] = sgb[k];
+      }
     }
     __syncthreads();
     if (e0 + MKC < Vi) fetch(e0 + MKC);              // in flight during the products below
-    // on a diagonal tile pair the lower-left quarter holds only a > b (not stored): its wave skips the products
 #pragma unroll
     for (int ks = 0; ks < MKC / 4; ++ks) {
-      if (lower_quarter) break;
       const int kk = ks * 4 + (lane >> 4), ra = wm * 16 + (lane & 15), cb = wn * 16 + (lane & 15);
-      const double2 xa = XA[ra][kk], gb = GB[kk][cb];   // sum_e x[a,e] Fvv(b,e)
-      const double2 ga = GA[kk][ra], xb = XB[cb][kk];   // - sum_e Fvv(a,e) x[b,e]
+      const double2 xa = XA[ra][kk], gb = diag ? GA[kk][cb] : GB[kk][cb];   // sum_e x[a,e] Fvv(b,e)
       acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.x, gb.x, acc_re, 0, 0, 0);
       acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.x, gb.y, acc_im, 0, 0, 0);
       acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(-xa.y, gb.y, acc_re, 0, 0, 0);
       acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(xa.y, gb.x, acc_im, 0, 0, 0);
-      acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(-ga.x, xb.x, acc_re, 0, 0, 0);
-      acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(-ga.x, xb.y, acc_im, 0, 0, 0);
-      acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(ga.y, xb.y, acc_re, 0, 0, 0);
-      acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(-ga.y, xb.x, acc_im, 0, 0, 0);
+      if (!diag) {
+        const double2 ga = GA[kk][ra], xb = XB[cb][kk];   // - sum_e Fvv(a,e) x[b,e]
+        acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(-ga.x, xb.x, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(-ga.x, xb.y, acc_im, 0, 0, 0);
+        acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(ga.y, xb.y, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(-ga.y, xb.x, acc_im, 0, 0, 0);
+      }
+    }
+  }
+  if (diag) {
+    // every wave holds Y over its quarter of the tile (the lower-left one too): r = Y[a][b] - Y[b][a]
+    __syncthreads();                                 // the last chunk's tiles (aliased by Ysh) are read
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      Ysh[wm * 16 + (lane >> 4) + 4 * r][wn * 16 + (lane & 15)] = make_double2(acc_re[r], acc_im[r]);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double2 yt = Ysh[wn * 16 + (lane & 15)][wm * 16 + (lane >> 4) + 4 * r];
+      acc_re[r] -= yt.x;
+      acc_im[r] -= yt.y;
     }
   }
 

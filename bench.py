@@ -154,7 +154,8 @@ def aat_set_measurement(dev, seed, tol=1e-12):
     shape of fin_diff.py:285-370).
 
     Set A: 24 instances at O=10, V=38 (3 atoms: 9 nuclear + 3 field
-    directions, +/- 1e-4).  Set B: 36 instances at O~16, V~60 with seeded
+    directions, +/- 1e-4).  Every leg is the median of 3 runs after a
+    warm-up of each path.  Set B: 36 instances at O~16, V~60 with seeded
     uneven sizes (18 directions, 14 distinct systems).  MP2 (orbital-
     invariant spin-orbital).  Batched legs: cold starts with direct
     launches, the same replayed from a captured graph (capture included),
@@ -166,12 +167,16 @@ def aat_set_measurement(dev, seed, tol=1e-12):
     import torch
     from apyib_amd.batched import MP2NCBatch, fd_mp2_instances, mp2_adjoint_set, plain_solve, warm_batch_solve
 
-    def timed(fn):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        out = fn()
-        torch.cuda.synchronize()
-        return out, time.perf_counter() - t0
+    def timed(fn, reps=3):
+        """(result of the last run, median wall time of reps runs)"""
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return out, float(np.median(ts))
 
     res = {}
     rng = np.random.default_rng(seed + 17)
@@ -195,6 +200,16 @@ def aat_set_measurement(dev, seed, tol=1e-12):
             return b.energies().clone(), b.iterations
         (Eb, itb), t_batch = timed(lambda: batched(False))
         (Eg, itg), t_graph = timed(lambda: batched(True))
+        # the same batch solved again from zero amplitudes, replaying the graph captured above
+        rb = MP2NCBatch(insts)
+        rb.solve(tol, maxit=400, graph=True)
+
+        def replay():
+            rb.reset()
+            rb.solve(tol, maxit=400, graph=True)
+            return rb.energies().clone(), rb.iterations
+        (Er, itr), t_replay = timed(replay)
+        del rb
         (Ew, itwb, itw), t_warm = timed(lambda: warm_batch_solve(insts, bases, base_of_inst, tol))
         nd = len(dirs)
         dF = [((insts[2 * k].F - insts[2 * k + 1].F) / (2 * h)).reshape(-1) for k in range(nd)]
@@ -218,6 +233,9 @@ def aat_set_measurement(dev, seed, tol=1e-12):
                  "sizes": sorted({(I.O, I.V) for I in insts}),
                  "plain_loop_s": round(t_plain, 5), "batched_s": round(t_batch, 5),
                  "batched_graph_s": round(t_graph, 5), "batched_warm_s": round(t_warm, 5),
+                 "batched_graph_replay_s": round(t_replay, 5), "iterations_graph_replay": int(itr),
+                 "speedup_batched_graph_replay": round(t_plain / t_replay, 2),
+                 "max_abs_dE_graph_replay_vs_plain": dE(Er),
                  "speedup_batched": round(t_plain / t_batch, 2), "speedup_batched_graph": round(t_plain / t_graph, 2),
                  "speedup_batched_warm": round(t_plain / t_warm, 2),
                  "iterations_plain_max": int(max(its)), "iterations_batched": int(itb),
