@@ -398,7 +398,9 @@ __global__ __launch_bounds__(256) void ci_unpack_kernel(int64_t O, int64_t V, co
 // precond_kernel
 __global__ __launch_bounds__(256) void precond_packed_kernel(int64_t O, int64_t V, const double* __restrict__ eps,
                                                              double theta, const double2* __restrict__ r,
-                                                             double2* __restrict__ out) {
+                                                             double2* __restrict__ out,
+                                                             const double* __restrict__ theta_dev) {
+  if (theta_dev) theta = *theta_dev;
   const int64_t OV = O * V, Pv = tp_npair(V), Lp = OV + tp_npair(O) * Pv + 1;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < Lp; e += (int64_t)gridDim.x * blockDim.x) {
     double D;
@@ -449,7 +451,16 @@ int apy_ci_precondition_packed(int64_t O, int64_t V, const void* eps, double the
                                void* stream) {
   if (O < 2 || V < 2 || !eps || !r || !out) return APY_E_ARG;
   precond_packed_kernel<<<apy_grid1d(apy_ci_packed_length(O, V)), 256, 0, (hipStream_t)stream>>>(
-      O, V, (const double*)eps, theta, (const double2*)r, (double2*)out);
+      O, V, (const double*)eps, theta, (const double2*)r, (double2*)out, nullptr);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_ci_precondition_packed_dev(int64_t O, int64_t V, const void* eps, const void* theta, const void* r, void* out,
+                                   void* stream) {
+  if (O < 2 || V < 2 || !eps || !theta || !r || !out) return APY_E_ARG;
+  precond_packed_kernel<<<apy_grid1d(apy_ci_packed_length(O, V)), 256, 0, (hipStream_t)stream>>>(
+      O, V, (const double*)eps, 0.0, (const double2*)r, (double2*)out, (const double*)theta);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }

@@ -1,0 +1,197 @@
+// Device-side control of the Davidson iteration for the lowest CISD root: the
+// projected eigenproblem, the residual coefficients and the convergence test
+// run on the GPU, so an iteration needs no host read and a sequence of them
+// can be queued ahead of the GPU or replayed from a captured HIP graph.
+//
+// The projected matrix is kept column by column, g[j * ld + i] = <B_i|S_j>
+// for i <= j (each expansion step writes one contiguous column with
+// apy_zdot_many); the Hermitian matrix is its upper triangle mirrored.
+#include "common.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int JM = APY_MAXVEC;      // largest subspace
+constexpr int EIG_NEG = 1 + JM;     // offset of -theta y in the eigen block
+
+// Lowest eigenpair of the Hermitian k x k matrix by cyclic parallel Jacobi in
+// one wavefront.  Each round of the circle-method tournament rotates n/2
+// disjoint pairs (p, q); the rotation of a pair depends only on its own 2 x 2
+// block, so all row rotations of a round and then all column rotations apply
+// at once.  For h = A[p][q] = |h| e, the unitary Q = diag(1, conj e) R,
+// R = [[c, s], [-s, c]] with t = s / c the root of t^2 + 2 tau t - 1 = 0,
+// tau = (A[q][q] - A[p][p]) / (2 |h|), zeroes A[p][q] in Q^H A Q.  Sweeps stop
+// when the squared off-diagonal norm is below 1e-30 of the squared Frobenius
+// norm (5-6 sweeps at k = 16), or after 40.
+__global__ __launch_bounds__(64) void heev_lowest_kernel(int k, const double2* __restrict__ g, int ld,
+                                                         double2* __restrict__ out) {
+  __shared__ double2 A[JM][JM + 1];
+  __shared__ double2 Q[JM][JM + 1];
+  __shared__ double rc[JM / 2], rs[JM / 2];
+  __shared__ double2 re[JM / 2];
+  __shared__ int rp[JM / 2], rq[JM / 2];
+  __shared__ double red[64];
+  const int t = threadIdx.x;
+  const int n = k + (k & 1);
+  for (int e = t; e < JM * JM; e += 64) {
+    const int i = e / JM, j = e % JM;
+    double2 v = make_double2(0.0, 0.0);
+    if (i < k && j < k) {
+      if (i <= j) {
+        v = g[j * ld + i];
+      } else {
+        const double2 w = g[i * ld + j];
+        v = make_double2(w.x, -w.y);
+      }
+      if (i == j) v.y = 0.0;
+    }
+    A[i][j] = v;
+    Q[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 40 && k > 1; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int e = t; e < k * k; e += 64) {
+      const int i = e / k, j = e % k;
+      const double2 v = A[i][j];
+      if (i == j) dia += v.x * v.x;
+      else off += v.x * v.x + v.y * v.y;
+    }
+    red[t] = off;
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+      if (t < w) red[t] += red[t + w];
+      __syncthreads();
+    }
+    const double offs = red[0];
+    __syncthreads();
+    red[t] = dia;
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+      if (t < w) red[t] += red[t + w];
+      __syncthreads();
+    }
+    const double dias = red[0];
+    __syncthreads();
+    if (!(offs > 1e-30 * (dias + offs))) break;
+    for (int round = 0; round < n - 1; ++round) {
+      if (t < n / 2) {
+        int p, q;
+        if (t == 0) {
+          p = round;
+          q = n - 1;
+        } else {
+          p = (round + t) % (n - 1);
+          q = (round - t + n - 1) % (n - 1);
+        }
+        if (p > q) { const int x = p; p = q; q = x; }
+        double c = 1.0, s = 0.0;
+        double2 ph = make_double2(1.0, 0.0);
+        if (q < k) {
+          const double2 h = A[p][q];
+          const double ah = sqrt(h.x * h.x + h.y * h.y);
+          if (ah > 0.0) {
+            const double tau = (A[q][q].x - A[p][p].x) / (2.0 * ah);
+            const double tt = fabs(tau) > 1e150 ? 0.5 / tau
+                                                : (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + tt * tt);
+            s = tt * c;
+            ph = make_double2(h.x / ah, h.y / ah);
+          }
+        }
+        rp[t] = p;
+        rq[t] = q < k ? q : p;                 // a pair with the padding index is the identity
+        rc[t] = c;
+        rs[t] = s;
+        re[t] = ph;
+      }
+      __syncthreads();
+      // rows: A[p][j] = c A[p][j] - s e A[q][j], A[q][j] = s A[p][j] + c e A[q][j]
+      for (int e = t; e < (n / 2) * k; e += 64) {
+        const int m = e / k, j = e % k;
+        const int p = rp[m], q = rq[m];
+        if (p == q) continue;
+        const double c = rc[m], s = rs[m];
+        const double2 ph = re[m];
+        const double2 ap = A[p][j], aq = A[q][j];
+        const double2 eaq = make_double2(ph.x * aq.x - ph.y * aq.y, ph.x * aq.y + ph.y * aq.x);
+        A[p][j] = make_double2(c * ap.x - s * eaq.x, c * ap.y - s * eaq.y);
+        A[q][j] = make_double2(s * ap.x + c * eaq.x, s * ap.y + c * eaq.y);
+      }
+      __syncthreads();
+      // columns of A and Q: X[i][p] = c X[i][p] - s conj(e) X[i][q], X[i][q] = s X[i][p] + c conj(e) X[i][q]
+      for (int e = t; e < 2 * (n / 2) * k; e += 64) {
+        const int which = e / ((n / 2) * k), r = e % ((n / 2) * k);
+        const int m = r / k, i = r % k;
+        const int p = rp[m], q = rq[m];
+        if (p == q) continue;
+        const double c = rc[m], s = rs[m];
+        const double2 ph = re[m];
+        double2 (*X)[JM + 1] = which ? Q : A;
+        const double2 xp = X[i][p], xq = X[i][q];
+        const double2 cxq = make_double2(ph.x * xq.x + ph.y * xq.y, ph.x * xq.y - ph.y * xq.x);
+        X[i][p] = make_double2(c * xp.x - s * cxq.x, c * xp.y - s * cxq.y);
+        X[i][q] = make_double2(s * xp.x + c * cxq.x, s * xp.y + c * cxq.y);
+      }
+      __syncthreads();
+    }
+  }
+  if (t == 0) {
+    int m = 0;
+    for (int i = 1; i < k; ++i)
+      if (A[i][i].x < A[m][m].x) m = i;
+    red[0] = A[m][m].x;
+    red[1] = (double)m;
+  }
+  __syncthreads();
+  const double theta = red[0];
+  const int m = (int)red[1];
+  if (t == 0) out[0] = make_double2(theta, 0.0);
+  if (t < JM) {
+    const double2 y = t < k ? Q[t][m] : make_double2(0.0, 0.0);
+    out[1 + t] = y;
+    out[EIG_NEG + t] = t < k ? make_double2(-theta * y.x, -theta * y.y) : make_double2(t == k ? 1.0 : 0.0, 0.0);
+  }
+  if (t == JM && k == JM) out[EIG_NEG + JM] = make_double2(1.0, 0.0);
+}
+
+// Convergence test of one iteration: ctl = [iterations, theta of the previous
+// iteration, status (0 running, 1 converged), residual norm]; log[2 (it - 1)]
+// = (theta, residual norm) while it <= cap.
+__global__ void davidson_check_kernel(const double2* __restrict__ eig, const double2* __restrict__ rr, double e_conv,
+                                      double r_conv, double* __restrict__ ctl, double* __restrict__ log, int cap) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int it = (int)ctl[0] + 1;
+  const double theta = eig[0].x, rn = sqrt(fabs(rr[0].x));
+  if (it <= cap) {
+    log[2 * (it - 1)] = theta;
+    log[2 * (it - 1) + 1] = rn;
+  }
+  if (ctl[2] == 0.0 && it > 1 && fabs(theta - ctl[1]) < e_conv && rn < r_conv) ctl[2] = 1.0;
+  ctl[1] = theta;
+  ctl[3] = rn;
+  ctl[0] = (double)it;
+}
+
+}  // namespace
+
+extern "C" {
+
+int apy_heev_lowest(int k, const void* g, int ld, void* eig, void* stream) {
+  if (k < 1 || k > APY_MAXVEC || ld < k || !g || !eig) return APY_E_ARG;
+  heev_lowest_kernel<<<1, 64, 0, (hipStream_t)stream>>>(k, (const double2*)g, ld, (double2*)eig);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_davidson_check(const void* eig, const void* rr, double e_conv, double r_conv, void* ctl, void* log, int cap,
+                       void* stream) {
+  if (!eig || !rr || !ctl || (cap > 0 && !log) || cap < 0) return APY_E_ARG;
+  davidson_check_kernel<<<1, 64, 0, (hipStream_t)stream>>>((const double2*)eig, (const double2*)rr, e_conv, r_conv,
+                                                           (double*)ctl, (double*)log, cap);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+}  // extern "C"

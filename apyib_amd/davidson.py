@@ -151,6 +151,7 @@ class CISDDavidson:
         back the residual norm and the new column of the projected matrix,
         and the step yields (Ritz value, residual norm) of the pair it
         started from.  finish() forms that pair's vector."""
+        self._device_mode = False
         B, S = self.B, self.S
         B.zero_()
         S.zero_()
@@ -203,8 +204,145 @@ class CISDDavidson:
             G[k, :k + 1] = np.conj(col)
             k += 1
 
+    # ------------------------------------------------- device-controlled iteration
+    def _comb_dev(self, xs, coef, y):
+        """y = sum_m coef[m] x_m with the coefficients in device memory (len(xs) <= APY_MAXVEC)."""
+        arr = (ctypes.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+        _lib.check(self.lib.apy_zcombine_seg(len(xs), arr, _vp(coef), 1, y.numel(), _vp(y), stream_of(self.dev)),
+                   "apy_zcombine_seg")
+
+    def _device_init(self, e_conv, r_conv, cap):
+        if not self.packed:
+            raise ValueError("device-controlled Davidson runs on packed vectors")
+        if self.max_space > _lib.APY_MAXVEC - 1:
+            raise ValueError(f"device-controlled Davidson needs max_space <= {_lib.APY_MAXVEC - 1}")
+        M = self.max_space
+        self.G_dev = torch.zeros((M, M), dtype=CDT, device=self.dev)      # row j: column j of the projected matrix
+        self.eig = torch.zeros(2 * _lib.APY_MAXVEC + 2, dtype=CDT, device=self.dev)
+        self.rr = torch.zeros(1, dtype=CDT, device=self.dev)
+        self.ctl = torch.tensor([0.0, float("nan"), 0.0, 0.0], dtype=torch.float64, device=self.dev)
+        self.cap = int(cap)
+        self.log = torch.zeros(2 * max(self.cap, 1), dtype=torch.float64, device=self.dev)
+        self.one = torch.ones(1, dtype=CDT, device=self.dev)
+        self._conv = (float(e_conv), float(r_conv))
+        M2 = _lib.APY_MAXVEC
+        self._ibuf = torch.zeros(4 + M2, dtype=CDT, device=self.dev)
+        self._gdots = torch.zeros(M2, dtype=CDT, device=self.dev)
+        self._gcoef = torch.zeros(M2, dtype=CDT, device=self.dev)
+        self.x = torch.zeros(self.Lv, dtype=CDT, device=self.dev)
+        self._graphs = {}
+
+    def _device_iteration(self, k):
+        """One Davidson iteration at subspace size k with no host read: the
+        Ritz pair from apy_heev_lowest, residual, convergence test, restart
+        (k == max_space), preconditioner with theta from device memory,
+        Gram-Schmidt, sigma and the new column.  Returns the new size."""
+        B, S, r = self.B, self.S, self.work
+        st = stream_of(self.dev)
+        M, MV = self.max_space, _lib.APY_MAXVEC
+        _lib.check(self.lib.apy_heev_lowest(k, _vp(self.G_dev), M, _vp(self.eig), st), "apy_heev_lowest")
+        self._comb_dev([S[j] for j in range(k)], self.eig[1:], r)
+        self._comb_dev([B[j] for j in range(k)] + [r], self.eig[1 + MV:], r)
+        self._dots_dev([r], r, self.rr)
+        e_conv, r_conv = self._conv
+        _lib.check(self.lib.apy_davidson_check(_vp(self.eig), _vp(self.rr), e_conv, r_conv, _vp(self.ctl),
+                                               _vp(self.log), self.cap, st), "apy_davidson_check")
+        if k == M:                                  # restart from the Ritz vector, with a fresh sigma
+            self._comb_dev([B[j] for j in range(k)], self.eig[1:], B[0])
+            self._sigma(0)
+            self._dots_dev([B[0]], S[0], self.G_dev[0][0:1])
+            k = 1
+            self._ritz_dev = (1, self.one)
+        else:
+            self._ritz_dev = (k, self.eig[1:])
+        d = B[k]
+        _lib.check(self.lib.apy_ci_precondition_packed_dev(self.O, self.V, _vp(self.soci.eps), _vp(self.eig), _vp(r),
+                                                           _vp(d), st), "apy_ci_precondition_packed_dev")
+        basis = [B[j] for j in range(k)]
+        self._gs_pass(basis, d, False)
+        self._gs_pass(basis, d, True)
+        self._sigma(k)
+        self._dots_dev([B[j] for j in range(k + 1)], S[k], self.G_dev[k][0:k + 1])
+        return k + 1
+
+    def prepare_graphs(self):
+        """Capture one HIP graph per subspace size (1 .. max_space): each holds
+        the whole device-controlled iteration at that size, so a solve
+        replays a fixed sequence of graphs.  Capture records without running."""
+        for k in range(1, self.max_space + 1):
+            if k in self._graphs:
+                continue
+            torch.cuda.synchronize(self.dev)
+            g = torch.cuda.CUDAGraph()
+            n0 = self.nsigma
+            with torch.cuda.graph(g):
+                nk = self._device_iteration(k)
+            ns, self.nsigma = self.nsigma - n0, n0          # sigmas per replay; capture ran none
+            self._graphs[k] = (g, nk, self._ritz_dev, ns)
+
+    def iterate_device(self, e_conv=0.0, r_conv=0.0, graph=False, cap=0):
+        """Generator over device-controlled Davidson iterations: the same
+        iteration as iterate() with the projected eigenproblem (Jacobi,
+        apy_heev_lowest), the residual coefficients and the convergence test
+        (apy_davidson_check) on the GPU, so no iteration waits for the host;
+        graph=True replays one captured HIP graph per subspace size.  Yields
+        nothing per step (read_state() brings back the Ritz value, residual
+        norm and status); finish() forms the vector of the last Ritz pair."""
+        self._device_init(e_conv, r_conv, cap)
+        self._device_mode = True
+        B, S = self.B, self.S
+        B.zero_()
+        S.zero_()
+        self.G_dev.zero_()
+        B[0, self.Lv - 1] = 1.0                     # the reference determinant
+        self._sigma(0)
+        self._dots_dev([B[0]], S[0], self.G_dev[0][0:1])
+        if graph:
+            self.prepare_graphs()
+        k = 1
+        while True:
+            if graph:
+                g, nk, ritz, ns = self._graphs[k]
+                g.replay()
+                k, self._ritz_dev = nk, ritz
+                self.nsigma += ns
+            else:
+                k = self._device_iteration(k)
+            yield None
+
+    def read_state(self):
+        """(iterations, Ritz value, residual norm, converged) of the device
+        iteration: one host read."""
+        c = self.ctl.cpu().numpy()
+        return int(c[0]), float(c[1]), float(c[3]), bool(c[2] == 1.0)
+
+    def solve_device(self, e_conv=1e-12, r_conv=1e-8, maxit=200, chunk=4, graph=False):
+        """solve() with device-controlled iterations, checked once per chunk of
+        iterations (one host read each); iterations after convergence inside
+        a chunk only extend the subspace.  Returns the correlation energy."""
+        gen = self.iterate_device(e_conv, r_conv, graph=graph, cap=maxit)
+        it = 0
+        while it < maxit:
+            for _ in range(min(chunk, maxit - it)):
+                next(gen)
+                it += 1
+            if self.read_state()[3]:
+                break
+        gen.close()
+        n, theta, rn, _ = self.read_state()
+        lg = self.log[:2 * min(n, self.cap)].cpu().numpy().reshape(-1, 2)
+        self.history = [(float(a), float(b)) for a, b in lg]
+        self.E = theta
+        self.iterations = n
+        self.finish()
+        return theta
+
     def _form_ritz(self):
         """self.x = sum_j y_j B_j for the Ritz pair of the last iteration."""
+        if getattr(self, "_device_mode", False):
+            k, coef = self._ritz_dev
+            self._comb_dev([self.B[j] for j in range(k)], coef, self.x)
+            return
         k, y = self._ritz
         self.combine([self.B[j] for j in range(k)], y, self.x)
 

@@ -728,6 +728,10 @@ def main():
                     help="saved GEMM plans (loaded unless --tune or --no-tune)")
     ap.add_argument("--save-plans", default=None, help="write the tuned plans here (with --tune)")
     ap.add_argument("--max-space", type=int, default=12, help="Davidson subspace size before restart")
+    ap.add_argument("--host-control", action="store_true",
+                    help="Davidson with the projected eigenproblem on the host (numpy eigh, one host read per "
+                         "iteration) instead of on the device")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replayed Davidson timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-aat-set", action="store_true", help="skip the batched finite-difference set timing")
     ap.add_argument("--no-cisd-set", action="store_true", help="skip the CISD field-derivative set timing")
@@ -805,7 +809,10 @@ def main():
         sigma_events.append((e0, e1))
     timing = [False]
     dav._sigma = sigma_timed
-    gen = dav.iterate()
+    device_control = not args.host_control
+    # device control: Ritz pair (apy_heev_lowest), residual and convergence test on the GPU, no host
+    # read inside an iteration; host control: numpy eigh and one host read per iteration
+    gen = dav.iterate_device() if device_control else dav.iterate()
     next(gen)                       # reference vector, its sigma and the first Ritz pair
     for _ in range(W):
         next(gen)
@@ -816,11 +823,15 @@ def main():
     timing[0] = True
     for k in range(K):
         solver.ev_dominant = evs[k]
-        theta, rn = next(gen)
+        out = next(gen)
+        if not device_control:
+            theta, rn = out
     torch.cuda.synchronize()
     timing[0] = False
     solver.ev_dominant = None
     elapsed = time.perf_counter() - t_start
+    if device_control:
+        _, theta, rn, _ = dav.read_state()
     sigma_ms = [hip.elapsed_ms(a, b) for a, b in sigma_events]
     for a, b in sigma_events:
         hip.destroy(a)
@@ -838,6 +849,34 @@ def main():
     value = flops_step / (ms_step * 1e-3) / 1e12
     log(f"step {ms_step:.3f} ms ({nsig} sigma in {K} iterations), PPL kernel {ppl_avg:.3f} ms "
         f"({achieved:.1f} TFLOP/s), E_corr = {theta:.12f}, residual {rn:.2e}")
+
+    graph_leg = None
+    if device_control and not args.no_graph:
+        # the same iterations replayed from HIP graphs (one per subspace size, captured before timing)
+        dav._sigma = sigma_plain
+        dg = CISDDavidson(solver, max_space=args.max_space)
+        gg = dg.iterate_device(graph=True)
+        next(gg)
+        for _ in range(W):
+            next(gg)
+        torch.cuda.synchronize()
+        g0 = dg.nsigma
+        tg = time.perf_counter()
+        for _ in range(K):
+            next(gg)
+        torch.cuda.synchronize()
+        eg = time.perf_counter() - tg
+        gsig = dg.nsigma - g0
+        _, theta_g, rn_g, _ = dg.read_state()
+        gg.close()
+        ms_g = eg * 1e3 / K
+        graph_leg = {"what": "the same Davidson iterations replayed from captured HIP graphs (one per subspace "
+                             "size, captured before the timed region)",
+                     "ms_per_step": round(ms_g, 4), "sigma_per_step": round(gsig / K, 3),
+                     "tflops": round(flops_sigma * gsig / K / (ms_g * 1e-3) / 1e12, 3),
+                     "E_corr_after_steps": theta_g, "residual_after_steps": rn_g}
+        log(f"graph-replayed step {ms_g:.3f} ms ({gsig} sigma in {K} iterations), E_corr = {theta_g:.12f}")
+        del dg
 
     aat_set = None
     if not args.no_aat_set:
@@ -875,6 +914,8 @@ def main():
             "workload": (f"CISD lowest root (spin-orbital, Hermitian) by Davidson, seeded complex128 Hamiltonian, "
                          f"O={O} V={V}; one step = one Davidson iteration"),
             "O": O, "V": V, "seed": args.seed, "solver": f"davidson (max_space {args.max_space})",
+            "control": ("device: Ritz pair (apy_heev_lowest), residual and convergence test on the GPU, no host read "
+                        "inside an iteration" if device_control else "host: numpy eigh, one host read per iteration"),
             "gemm_plans": plans,
             "sigma_per_step": nsig / K,
             "step_flops_executed": flops_step,
@@ -882,8 +923,9 @@ def main():
             "E_corr_after_steps": theta, "residual_after_steps": rn,
         },
         "dense_equivalent_tflops": round(dense_flops(O, V) * nsig / K / (ms_step * 1e-3) / 1e12, 3),
+        "graph": graph_leg,
         "sigma": {"what": "the sigma calls of the timed iterations alone (HIP events on their stream around each "
-                          "apy_cisd_so_step + reference-row energy, host reads inside); executed flops as value",
+                          "apy_cisd_so_step + reference-row energy); executed flops as value",
                   "calls": len(sigma_ms), "ms": round(sum(sigma_ms) / max(1, len(sigma_ms)), 4),
                   "tflops": round(flops_sigma / (sum(sigma_ms) / max(1, len(sigma_ms)) * 1e-3) / 1e12, 3),
                   "frac_of_fp64_mfma_peak": round(flops_sigma / (sum(sigma_ms) / max(1, len(sigma_ms)) * 1e-3)

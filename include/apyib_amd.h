@@ -547,6 +547,11 @@ int apy_ci_pack(int64_t O, int64_t V, const void* dense, void* packed, void* str
 int apy_ci_unpack(int64_t O, int64_t V, const void* packed, void* dense, void* stream);
 int apy_ci_precondition_packed(int64_t O, int64_t V, const void* eps, double theta, const void* r, void* out,
                                void* stream);
+/* The same with theta read from device memory (a double, e.g. the first
+ * entry of an apy_heev_lowest block), so a device-controlled Davidson step
+ * needs no host value. */
+int apy_ci_precondition_packed_dev(int64_t O, int64_t V, const void* eps, const void* theta, const void* r, void* out,
+                                   void* stream);
 /* ------------------------------------------------------------------------
  * Davidson support and the CISD energy gradient with respect to F.
  * Vectors have length L + 1, L = O V + O^2 V^2: [c1 | c2 (dense,
@@ -737,6 +742,26 @@ size_t apy_reduce_seg_workspace(int64_t nseg, int k);
  * One thread; k < APY_MAXVEC.  Lets a Davidson expansion run without host
  * reads (apply with apy_zcombine_seg, nseg = 1). */
 int apy_gs_coefficients(int k, const void* dots, int normalize, void* coef, void* status, void* stream);
+/* The projected eigenproblem of the Davidson iteration on the device (the
+ * host numpy.linalg.eigh of the lowest root in davidson.py; the reference
+ * has no Davidson solver, its CISD is the Jacobi iteration of
+ * ci_wfn.py:302-378).  g holds the k x k Hermitian projected matrix by its
+ * upper triangle, column-major: g[j * ld + i] = <B_i|S_j> for i <= j
+ * (k <= APY_MAXVEC).  eig (device, 2 APY_MAXVEC + 2 complex) receives
+ * [theta | y (APY_MAXVEC, zero past k) | -theta y (k) | 1 | 0 ...]: the
+ * lowest eigenvalue (real part of entry 0), its unit eigenvector, and the
+ * coefficients that form the residual sum_j y_j S_j - theta sum_j y_j B_j
+ * with two apy_zcombine_seg calls (the second appending the partial sum
+ * with weight 1).  Cyclic parallel Jacobi rotations in one wavefront. */
+int apy_heev_lowest(int k, const void* g, int ld, void* eig, void* stream);
+/* Convergence test of a device-controlled Davidson iteration: ctl (device,
+ * double[4]) = [iterations, previous theta, status (0 running, 1
+ * converged), residual norm]; with theta = eig[0] and the residual norm
+ * sqrt(rr[0]) the status becomes 1 when |theta - previous| < e_conv and the
+ * norm < r_conv (not on the first iteration); log[2 (it - 1) + {0, 1}] =
+ * (theta, norm) for it <= cap. */
+int apy_davidson_check(const void* eig, const void* rr, double e_conv, double r_conv, void* ctl, void* log, int cap,
+                       void* stream);
 int apy_zdot_many_seg(int k, const void* const* xs, const void* y, int64_t nseg, int64_t seglen, int conj_x,
                       void* out, void* workspace, size_t ws_bytes, void* stream);
 int apy_zdiffsq_batch(const void* x, const void* y, int64_t nseg, int64_t seglen, int64_t split, void* out,
