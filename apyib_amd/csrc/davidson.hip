@@ -174,9 +174,206 @@ __global__ void davidson_check_kernel(const double2* __restrict__ eig, const dou
   ctl[0] = (double)it;
 }
 
+constexpr int FNB = 512;   // partial-sum blocks of the fused passes
+
+struct DvList {
+  const double2* p[JM];
+};
+
+__device__ inline int64_t dv_prow(int64_t x, int64_t N) { return x * (2 * N - x - 1) / 2; }
+
+// (x < y) of the packed pair index P among N
+__device__ inline void dv_pdec(int64_t P, int64_t N, int64_t& x, int64_t& y) {
+  const double b = 2.0 * (double)N - 1.0, disc = b * b - 8.0 * (double)P;
+  int64_t r = (int64_t)((b - sqrt(disc > 0.0 ? disc : 0.0)) * 0.5);
+  if (r < 0) r = 0;
+  if (r > N - 2) r = N - 2;
+  while (r > 0 && dv_prow(r, N) > P) --r;
+  while (r + 1 <= N - 2 && dv_prow(r + 1, N) <= P) ++r;
+  x = r;
+  y = P - dv_prow(r, N) + r + 1;
+}
+
+// Fixed-order block sum of NV complex accumulators into part[blockIdx.x][NV].
+template <int NV>
+__device__ inline void block_sums(double (&ar)[NV], double (&ai)[NV], double2* __restrict__ part) {
+  __shared__ double sr[256], si[256];
+#pragma unroll 1
+  for (int m = 0; m < NV; ++m) {
+    sr[threadIdx.x] = ar[m];
+    si[threadIdx.x] = ai[m];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) { sr[threadIdx.x] += sr[threadIdx.x + w]; si[threadIdx.x] += si[threadIdx.x + w]; }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * NV + m] = make_double2(sr[0], si[0]);
+    __syncthreads();
+  }
+}
+
+// One pass over packed vectors [c1 (O V) | c2 (i<j, a<b) | c0] of length len: the residual
+// r = sum_j y_j S_j - theta sum_j y_j B_j of the Ritz pair in eig (apy_heev_lowest layout), the
+// preconditioned correction d = r / (theta - D) written to d, and the block partials of
+// [<B_j|d> (j < K) | <d|d> | <r|r>] (the first Gram-Schmidt dots and the residual norm).
+template <int K>
+__global__ __launch_bounds__(256) void resid_precond_partial(DvList S, DvList B, const double2* __restrict__ eig,
+                                                             const double* __restrict__ eps, int64_t O, int64_t V,
+                                                             int64_t len, double2* __restrict__ d,
+                                                             double2* __restrict__ part) {
+  const double theta = eig[0].x;
+  double2 y[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) y[j] = eig[1 + j];
+  double ar[K + 2], ai[K + 2];
+#pragma unroll
+  for (int m = 0; m < K + 2; ++m) { ar[m] = 0.0; ai[m] = 0.0; }
+  const int64_t OV = O * V, Pv = V * (V - 1) / 2;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < len; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 bv[K];
+    double rr = 0.0, ri = 0.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const double2 sv = S.p[j][e];
+      bv[j] = B.p[j][e];
+      const double2 u = make_double2(sv.x - theta * bv[j].x, sv.y - theta * bv[j].y);
+      rr += y[j].x * u.x - y[j].y * u.y;
+      ri += y[j].x * u.y + y[j].y * u.x;
+    }
+    double D;
+    if (e < OV) {
+      D = eps[O + e % V] - eps[e / V];
+    } else if (e < len - 1) {
+      const int64_t q = e - OV;
+      int64_t i, j, a, b;
+      dv_pdec(q / Pv, O, i, j);
+      dv_pdec(q % Pv, V, a, b);
+      D = eps[O + a] + eps[O + b] - eps[i] - eps[j];
+    } else {
+      D = 0.0;
+    }
+    double den = theta - D;
+    if (fabs(den) < 1e-8) den = den < 0.0 ? -1e-8 : 1e-8;
+    const double2 dv = make_double2(rr / den, ri / den);
+    d[e] = dv;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      ar[j] += bv[j].x * dv.x + bv[j].y * dv.y;
+      ai[j] += bv[j].x * dv.y - bv[j].y * dv.x;
+    }
+    ar[K] += dv.x * dv.x + dv.y * dv.y;
+    ar[K + 1] += rr * rr + ri * ri;
+  }
+  block_sums<K + 2>(ar, ai, part);
+}
+
+// d <- sum_j c_j B_j + c_K d (coefficients in device memory; d read before it is written at each
+// index) and the block partials of [<B_j|d> (j < K) | <d|d>] of the result.
+template <int K>
+__global__ __launch_bounds__(256) void combine_dots_partial(DvList B, const double2* __restrict__ c, int64_t len,
+                                                            double2* __restrict__ d, double2* __restrict__ part) {
+  double2 cc[K + 1];
+#pragma unroll
+  for (int j = 0; j <= K; ++j) cc[j] = c[j];
+  double ar[K + 1], ai[K + 1];
+#pragma unroll
+  for (int m = 0; m <= K; ++m) { ar[m] = 0.0; ai[m] = 0.0; }
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < len; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 bv[K];
+    const double2 x = d[e];
+    double vr = cc[K].x * x.x - cc[K].y * x.y, vi = cc[K].x * x.y + cc[K].y * x.x;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      bv[j] = B.p[j][e];
+      vr += cc[j].x * bv[j].x - cc[j].y * bv[j].y;
+      vi += cc[j].x * bv[j].y + cc[j].y * bv[j].x;
+    }
+    d[e] = make_double2(vr, vi);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      ar[j] += bv[j].x * vr + bv[j].y * vi;
+      ai[j] += bv[j].x * vi - bv[j].y * vr;
+    }
+    ar[K] += vr * vr + vi * vi;
+  }
+  block_sums<K + 1>(ar, ai, part);
+}
+
+__global__ void dv_finalize(const double2* __restrict__ part, int nb, int nv, double2* __restrict__ out) {
+  __shared__ double sr[256], si[256];
+  for (int m = 0; m < nv; ++m) {
+    double a = 0.0, b = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) { a += part[(int64_t)q * nv + m].x; b += part[(int64_t)q * nv + m].y; }
+    sr[threadIdx.x] = a;
+    si[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) { sr[threadIdx.x] += sr[threadIdx.x + w]; si[threadIdx.x] += si[threadIdx.x + w]; }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[m] = make_double2(sr[0], si[0]);
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t apy_davidson_fused_workspace(void) { return (size_t)FNB * (JM + 2) * sizeof(double2); }
+
+int apy_davidson_residual(int k, const void* const* S, const void* const* B, const void* eig, const void* eps,
+                          int64_t O, int64_t V, void* d, void* dots, void* workspace, size_t ws_bytes, void* stream) {
+  if (k < 1 || k > JM - 1 || !S || !B || !eig || !eps || !d || !dots || O < 2 || V < 2) return APY_E_ARG;
+  if (!workspace || ws_bytes < apy_davidson_fused_workspace()) return APY_E_WORKSPACE;
+  DvList sl{}, bl{};
+  for (int j = 0; j < k; ++j) {
+    if (!S[j] || !B[j] || B[j] == d || S[j] == d) return APY_E_ARG;
+    sl.p[j] = (const double2*)S[j];
+    bl.p[j] = (const double2*)B[j];
+  }
+  const int64_t len = O * V + (O * (O - 1) / 2) * (V * (V - 1) / 2) + 1;
+  hipStream_t st = (hipStream_t)stream;
+  double2* part = (double2*)workspace;
+  switch (k) {
+#define C(K)                                                                                                     \
+  case K:                                                                                                        \
+    resid_precond_partial<K><<<FNB, 256, 0, st>>>(sl, bl, (const double2*)eig, (const double*)eps, O, V, len,  \
+                                                  (double2*)d, part);                                            \
+    break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15)
+#undef C
+  }
+  APY_LAUNCH_CHECK();
+  dv_finalize<<<1, 256, 0, st>>>(part, FNB, k + 2, (double2*)dots);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_davidson_combine_dots(int k, const void* const* B, const void* coef, int64_t len, void* d, void* dots,
+                              void* workspace, size_t ws_bytes, void* stream) {
+  if (k < 1 || k > JM - 1 || !B || !coef || !d || !dots || len < 1) return APY_E_ARG;
+  if (!workspace || ws_bytes < apy_davidson_fused_workspace()) return APY_E_WORKSPACE;
+  DvList bl{};
+  for (int j = 0; j < k; ++j) {
+    if (!B[j] || B[j] == d) return APY_E_ARG;
+    bl.p[j] = (const double2*)B[j];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  double2* part = (double2*)workspace;
+  switch (k) {
+#define C(K)                                                                                              \
+  case K:                                                                                                 \
+    combine_dots_partial<K><<<FNB, 256, 0, st>>>(bl, (const double2*)coef, len, (double2*)d, part);      \
+    break;
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15)
+#undef C
+  }
+  APY_LAUNCH_CHECK();
+  dv_finalize<<<1, 256, 0, st>>>(part, FNB, k + 1, (double2*)dots);
+  APY_LAUNCH_CHECK();
+  return APY_OK;
+}
 
 int apy_heev_lowest(int k, const void* g, int ld, void* eig, void* stream) {
   if (k < 1 || k > APY_MAXVEC || ld < k || !g || !eig) return APY_E_ARG;

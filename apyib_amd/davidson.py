@@ -231,16 +231,43 @@ class CISDDavidson:
         self._gcoef = torch.zeros(M2, dtype=CDT, device=self.dev)
         self.x = torch.zeros(self.Lv, dtype=CDT, device=self.dev)
         self._graphs = {}
+        self._fws_bytes = int(self.lib.apy_davidson_fused_workspace())
+        self._fws = torch.empty(self._fws_bytes, dtype=torch.uint8, device=self.dev)
+        self._fdots = torch.zeros(M2 + 2, dtype=CDT, device=self.dev)
 
     def _device_iteration(self, k):
         """One Davidson iteration at subspace size k with no host read: the
         Ritz pair from apy_heev_lowest, residual, convergence test, restart
         (k == max_space), preconditioner with theta from device memory,
-        Gram-Schmidt, sigma and the new column.  Returns the new size."""
+        Gram-Schmidt, sigma and the new column.  Returns the new size.
+        Below max_space (fused=True) the residual, preconditioner and first
+        Gram-Schmidt dots are one pass over the subspace (apy_davidson_residual)
+        and the first Gram-Schmidt combination carries the second pass's dots
+        (apy_davidson_combine_dots)."""
         B, S, r = self.B, self.S, self.work
         st = stream_of(self.dev)
         M, MV = self.max_space, _lib.APY_MAXVEC
         _lib.check(self.lib.apy_heev_lowest(k, _vp(self.G_dev), M, _vp(self.eig), st), "apy_heev_lowest")
+        if self.fused and k < M:
+            d = B[k]
+            sl = (ctypes.c_void_p * k)(*[S[j].data_ptr() for j in range(k)])
+            bl = (ctypes.c_void_p * k)(*[B[j].data_ptr() for j in range(k)])
+            fd, ws, nb = self._fdots, self._fws, self._fws_bytes
+            _lib.check(self.lib.apy_davidson_residual(k, sl, bl, _vp(self.eig), _vp(self.soci.eps), self.O, self.V,
+                                                      _vp(d), _vp(fd), _vp(ws), nb, st), "apy_davidson_residual")
+            e_conv, r_conv = self._conv
+            _lib.check(self.lib.apy_davidson_check(_vp(self.eig), _vp(fd[k + 1:k + 2]), e_conv, r_conv,
+                                                   _vp(self.ctl), _vp(self.log), self.cap, st), "apy_davidson_check")
+            self._ritz_dev = (k, self.eig[1:])
+            _lib.check(self.lib.apy_gs_coefficients(k, _vp(fd), 0, _vp(self._gcoef), None, st), "apy_gs_coefficients")
+            _lib.check(self.lib.apy_davidson_combine_dots(k, bl, _vp(self._gcoef), d.numel(), _vp(d), _vp(fd), _vp(ws),
+                                                          nb, st), "apy_davidson_combine_dots")
+            _lib.check(self.lib.apy_gs_coefficients(k, _vp(fd), 1, _vp(self._gcoef), _vp(self._ibuf[1:2]), st),
+                       "apy_gs_coefficients")
+            self._comb_dev([B[j] for j in range(k)] + [d], self._gcoef, d)
+            self._sigma(k)
+            self._dots_dev([B[j] for j in range(k + 1)], S[k], self.G_dev[k][0:k + 1])
+            return k + 1
         self._comb_dev([S[j] for j in range(k)], self.eig[1:], r)
         self._comb_dev([B[j] for j in range(k)] + [r], self.eig[1 + MV:], r)
         self._dots_dev([r], r, self.rr)
@@ -280,7 +307,7 @@ class CISDDavidson:
             ns, self.nsigma = self.nsigma - n0, n0          # sigmas per replay; capture ran none
             self._graphs[k] = (g, nk, self._ritz_dev, ns)
 
-    def iterate_device(self, e_conv=0.0, r_conv=0.0, graph=False, cap=0):
+    def iterate_device(self, e_conv=0.0, r_conv=0.0, graph=False, cap=0, fused=True):
         """Generator over device-controlled Davidson iterations: the same
         iteration as iterate() with the projected eigenproblem (Jacobi,
         apy_heev_lowest), the residual coefficients and the convergence test
@@ -290,6 +317,7 @@ class CISDDavidson:
         norm and status); finish() forms the vector of the last Ritz pair."""
         self._device_init(e_conv, r_conv, cap)
         self._device_mode = True
+        self.fused = bool(fused)
         B, S = self.B, self.S
         B.zero_()
         S.zero_()
@@ -316,11 +344,11 @@ class CISDDavidson:
         c = self.ctl.cpu().numpy()
         return int(c[0]), float(c[1]), float(c[3]), bool(c[2] == 1.0)
 
-    def solve_device(self, e_conv=1e-12, r_conv=1e-8, maxit=200, chunk=4, graph=False):
+    def solve_device(self, e_conv=1e-12, r_conv=1e-8, maxit=200, chunk=4, graph=False, fused=True):
         """solve() with device-controlled iterations, checked once per chunk of
         iterations (one host read each); iterations after convergence inside
         a chunk only extend the subspace.  Returns the correlation energy."""
-        gen = self.iterate_device(e_conv, r_conv, graph=graph, cap=maxit)
+        gen = self.iterate_device(e_conv, r_conv, graph=graph, cap=maxit, fused=fused)
         it = 0
         while it < maxit:
             for _ in range(min(chunk, maxit - it)):

@@ -762,6 +762,24 @@ int apy_heev_lowest(int k, const void* g, int ld, void* eig, void* stream);
  * (theta, norm) for it <= cap. */
 int apy_davidson_check(const void* eig, const void* rr, double e_conv, double r_conv, void* ctl, void* log, int cap,
                        void* stream);
+/* Fused passes of the device-controlled Davidson step over packed vectors
+ * (apy_ci_packed_length(O, V)), k <= APY_MAXVEC - 1 subspace vectors, each
+ * one read of the operands instead of three or four calls:
+ *   apy_davidson_residual: d = r / (theta - D) with r = sum_j y_j S_j - theta
+ *     sum_j y_j B_j for the Ritz pair in eig (apy_heev_lowest) and D the
+ *     packed diagonal of apy_ci_precondition_packed; dots (device, k + 2
+ *     complex) = [<B_j|d> (j < k) | <d|d> | <r|r>] (the input of
+ *     apy_gs_coefficients, then the residual norm for apy_davidson_check).
+ *   apy_davidson_combine_dots: d <- sum_j coef_j B_j + coef_k d (coef in
+ *     device memory, e.g. apy_gs_coefficients) and dots = [<B_j|d> (j < k) |
+ *     <d|d>] of the result (the second Gram-Schmidt pass's input).
+ * d must differ from every S_j and B_j.  Workspace
+ * apy_davidson_fused_workspace() bytes; deterministic fixed-order sums. */
+size_t apy_davidson_fused_workspace(void);
+int apy_davidson_residual(int k, const void* const* S, const void* const* B, const void* eig, const void* eps,
+                          int64_t O, int64_t V, void* d, void* dots, void* workspace, size_t ws_bytes, void* stream);
+int apy_davidson_combine_dots(int k, const void* const* B, const void* coef, int64_t len, void* d, void* dots,
+                              void* workspace, size_t ws_bytes, void* stream);
 int apy_zdot_many_seg(int k, const void* const* xs, const void* y, int64_t nseg, int64_t seglen, int conj_x,
                       void* out, void* workspace, size_t ws_bytes, void* stream);
 int apy_zdiffsq_batch(const void* x, const void* y, int64_t nseg, int64_t seglen, int64_t split, void* out,

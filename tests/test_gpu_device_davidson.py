@@ -92,6 +92,51 @@ def test_davidson_check_semantics(dev):
     assert np.allclose(log.cpu().numpy().reshape(3, 2), np.array(seq[:3]), rtol=1e-15, atol=0)
 
 
+@pytest.mark.parametrize("O,V,k", [(2, 3, 1), (4, 6, 5), (6, 14, 15)])
+def test_fused_residual_and_gram_schmidt_passes(dev, O, V, k):
+    """apy_davidson_residual / apy_davidson_combine_dots against the unfused
+    calls (apy_zcombine_seg, apy_ci_precondition_packed, torch dots)."""
+    lib = _lib.load()
+    st = stream_of(dev)
+    Lv = int(lib.apy_ci_packed_length(O, V))
+    g = torch.Generator(device="cpu").manual_seed(O * 100 + k)
+    rnd = lambda: torch.complex(torch.randn(Lv, generator=g, dtype=torch.float64),
+                                torch.randn(Lv, generator=g, dtype=torch.float64)).to(dev)
+    S = [rnd() for _ in range(k)]
+    B = [rnd() for _ in range(k)]
+    eps = torch.from_numpy(synthetic.fock(O, V, 3)[1]).to(dev)
+    theta = -0.37
+    y = torch.complex(torch.randn(k, generator=g, dtype=torch.float64), torch.randn(k, generator=g, dtype=torch.float64))
+    eig = torch.zeros(2 * MV + 2, dtype=CDT)
+    eig[0] = theta
+    eig[1:1 + k] = y
+    eig = eig.to(dev)
+    r_ref = sum(y[j].item() * (S[j] - theta * B[j]) for j in range(k))
+    d_ref = torch.zeros_like(r_ref)
+    _lib.check(lib.apy_ci_precondition_packed(O, V, _vp(eps), theta, _vp(r_ref), _vp(d_ref), st), "precondition")
+    d = torch.full_like(r_ref, complex(np.nan, np.nan))
+    dots = torch.zeros(MV + 2, dtype=CDT, device=dev)
+    nb = int(lib.apy_davidson_fused_workspace())
+    ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+    arr = lambda xs: (ctypes.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+    _lib.check(lib.apy_davidson_residual(k, arr(S), arr(B), _vp(eig), _vp(eps), O, V, _vp(d), _vp(dots), _vp(ws), nb,
+                                         st), "apy_davidson_residual")
+    scale = float(d_ref.abs().max().cpu())
+    assert float((d - d_ref).abs().max().cpu()) <= 1e-13 * scale
+    got = dots.cpu().numpy()
+    want = [complex(torch.vdot(B[j], d_ref).cpu()) for j in range(k)] + [complex(torch.vdot(d_ref, d_ref).cpu()),
+                                                                          complex(torch.vdot(r_ref, r_ref).cpu())]
+    assert np.allclose(got[:k + 2], want, rtol=1e-12, atol=1e-12 * abs(want[k]))
+    coef = torch.complex(torch.randn(k + 1, generator=g, dtype=torch.float64),
+                         torch.randn(k + 1, generator=g, dtype=torch.float64)).to(dev)
+    v_ref = sum(coef[j].item() * B[j] for j in range(k)) + coef[k].item() * d
+    _lib.check(lib.apy_davidson_combine_dots(k, arr(B), _vp(coef), Lv, _vp(d), _vp(dots), _vp(ws), nb, st),
+               "apy_davidson_combine_dots")
+    assert float((d - v_ref).abs().max().cpu()) <= 1e-13 * float(v_ref.abs().max().cpu())
+    want = [complex(torch.vdot(B[j], v_ref).cpu()) for j in range(k)] + [complex(torch.vdot(v_ref, v_ref).cpu())]
+    assert np.allclose(dots.cpu().numpy()[:k + 1], want, rtol=1e-12, atol=1e-12 * abs(want[k]))
+
+
 def _system(O, V, seed, dev):
     F, A, eps = synthetic.system(O, V, seed)
     ints = SOIntegrals.synthetic(O, V, seed, device=dev)
@@ -103,15 +148,15 @@ def _dav(F, eps, ints, dev, max_space):
                         max_space=max_space)
 
 
-@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("graph,fused", [(False, True), (True, True), (False, False), (True, False)])
 @pytest.mark.parametrize("O,V,seed,max_space", [(6, 14, 4, 15), (8, 20, 5, 6), (4, 33, 6, 4)])
-def test_device_davidson_matches_host_control(dev, O, V, seed, max_space, graph):
+def test_device_davidson_matches_host_control(dev, O, V, seed, max_space, graph, fused):
     F, A, eps, ints = _system(O, V, seed, dev)
     host = _dav(F, eps, ints, dev, max_space)
     Eh = host.solve(e_conv=1e-13, r_conv=1e-9)
     t1h, t2h = host.amplitudes()
     d = _dav(F, eps, ints, dev, max_space)
-    Ed = d.solve_device(e_conv=1e-13, r_conv=1e-9, chunk=3, graph=graph)
+    Ed = d.solve_device(e_conv=1e-13, r_conv=1e-9, chunk=3, graph=graph, fused=fused)
     t1d, t2d = d.amplitudes()                            # intermediate normalisation: phase-free
     print("host", host.iterations, "device", d.iterations, Eh, Ed)
     assert abs(Ed - Eh) < 1e-11
