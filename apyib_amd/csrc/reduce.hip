@@ -51,24 +51,23 @@ __global__ __launch_bounds__(256) void dot_many_partial(VecList xs, const double
   }
 }
 
-// segment blockIdx.x: out[blockIdx.x][m] = sum of its nb partials
+// segment blockIdx.x, value m = blockIdx.y: out[blockIdx.x][m] = sum of its nb partials (one block
+// per value, each in the same fixed order)
 __global__ void finalize_many(const double2* __restrict__ partial_all, int nb, int k, double2* out_all) {
   const double2* partial = partial_all + (int64_t)blockIdx.x * nb * k;
   double2* out = out_all + (int64_t)blockIdx.x * k;
   __shared__ double sr[256], si[256];
-  for (int m = 0; m < k; ++m) {
-    double a = 0.0, b = 0.0;
-    for (int q = threadIdx.x; q < nb; q += blockDim.x) { a += partial[(int64_t)q * k + m].x; b += partial[(int64_t)q * k + m].y; }
-    sr[threadIdx.x] = a;
-    si[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) { sr[threadIdx.x] += sr[threadIdx.x + w]; si[threadIdx.x] += si[threadIdx.x + w]; }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[m] = make_double2(sr[0], si[0]);
+  const int m = blockIdx.y;
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) { a += partial[(int64_t)q * k + m].x; b += partial[(int64_t)q * k + m].y; }
+  sr[threadIdx.x] = a;
+  si[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { sr[threadIdx.x] += sr[threadIdx.x + w]; si[threadIdx.x] += si[threadIdx.x + w]; }
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[m] = make_double2(sr[0], si[0]);
 }
 
 // sum (x - y)^2 (no conjugation) over [beg, end); with seglen > 0, blockIdx.y = 2 q + h selects
@@ -363,7 +362,7 @@ int apy_zdot_many(int k, const void* const* xs, const void* y, int64_t len, int 
 #undef C
   }
   APY_LAUNCH_CHECK();
-  finalize_many<<<1, 256, 0, s>>>(part, NB, k, (double2*)out);
+  finalize_many<<<dim3(1, (unsigned)k), 256, 0, s>>>(part, NB, k, (double2*)out);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }
@@ -386,7 +385,7 @@ int apy_ci_dots(int64_t O, int64_t V, int k, const void* const* xs, const void* 
 #undef C
   }
   APY_LAUNCH_CHECK();
-  finalize_many<<<1, 256, 0, s>>>(part, NB, k, (double2*)out);
+  finalize_many<<<dim3(1, (unsigned)k), 256, 0, s>>>(part, NB, k, (double2*)out);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }
@@ -401,7 +400,7 @@ int apy_zdiffsq_segments(const void* x, const void* y, int nseg, const int64_t* 
     if (off[q + 1] < off[q]) return APY_E_ARG;
     diffsq_partial<<<NB, 256, 0, s>>>((const double2*)x, (const double2*)y, off[q], off[q + 1], 0, 0, part);
     APY_LAUNCH_CHECK();
-    finalize_many<<<1, 256, 0, s>>>(part, NB, 1, (double2*)out + q);
+    finalize_many<<<dim3(1, 1), 256, 0, s>>>(part, NB, 1, (double2*)out + q);
     APY_LAUNCH_CHECK();
   }
   return APY_OK;
@@ -450,7 +449,7 @@ int apy_zdot_many_seg(int k, const void* const* xs, const void* y, int64_t nseg,
 #undef C
   }
   APY_LAUNCH_CHECK();
-  finalize_many<<<(unsigned)nseg, 256, 0, s>>>(part, NB, k, (double2*)out);
+  finalize_many<<<dim3((unsigned)nseg, (unsigned)k), 256, 0, s>>>(part, NB, k, (double2*)out);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }
@@ -465,7 +464,7 @@ int apy_zdiffsq_batch(const void* x, const void* y, int64_t nseg, int64_t seglen
   diffsq_partial<<<dim3(NB, (unsigned)(2 * nseg)), 256, 0, s>>>((const double2*)x, (const double2*)y, 0, 0, seglen,
                                                                 split, part);
   APY_LAUNCH_CHECK();
-  finalize_many<<<(unsigned)(2 * nseg), 256, 0, s>>>(part, NB, 1, (double2*)out);
+  finalize_many<<<dim3((unsigned)(2 * nseg), 1), 256, 0, s>>>(part, NB, 1, (double2*)out);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }
