@@ -33,6 +33,8 @@
 #include "common.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 namespace {
 
@@ -556,8 +558,26 @@ apy_gemm_desc gd(int64_t m, int64_t nn, int64_t k, int al, const void* a, int64_
 // Workspace regions: each holds nb instances one after another (R, Y, X1, X2, Q1, Q2: (ov)^2 each;
 // TU: 2 (ov)^2; Ts/Ps: so*sv; Ta/Pa: ao*av; acc, T1: ov), then the GEMM scratch.
 struct SpWS {
-  size_t R, Y, X1, X2, TU, Ts, Ta, Ps, Pa, Q1, Q2, acc, t1, gemm, total;
+  size_t R, Y, X1, X2, TU, Ts, Ta, Ps, Pa, Q1, Q2, acc, t1, gemm, gemm2, total;
 };
+
+// One side stream per device (default priority).  A step's products fall into two chains with no
+// data between them once TU and T+/- exist (below): the second runs on this stream, so the small
+// products of a single vector (ladders with M = o(o+1)/2 rows, rings with M = ov) fill the chip
+// together instead of one after another.
+hipStream_t sp_side_stream() {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> streams;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = streams.find(dev);
+  if (it != streams.end()) return it->second;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  streams[dev] = s;
+  return s;
+}
 
 constexpr int NSPG = 14;
 
@@ -686,7 +706,8 @@ SpWS sp_carve(int64_t o, int64_t v, int singles, int64_t nb, bool multi = false)
                            form == 1);
     for (int k = 0; k < q; ++k) m = std::max(m, apy_zgemm_ws_bytes(&g[k]));
   }
-  w.total = off + std::max(m, (size_t)EN_BLOCKS * z);
+  w.gemm2 = off + rnd(std::max(m, (size_t)EN_BLOCKS * z));   // split-K region of the side chain
+  w.total = w.gemm2 + std::max(m, (size_t)256);
   return w;
 }
 
@@ -733,10 +754,42 @@ int sp_step_impl(const apy_cisd_sp_args* A, int64_t nb, const void* resp_shift, 
   }
   apy_gemm_desc g[NSPG];
   const int q = sp_gemms(o, v, singles, nb, L, A, base, &w, g, multi ? &S : nullptr, !multi && I->w4t);
-  for (int k = 0; k < q; ++k) {
-    const int s = apy_zgemm_launch(&g[k], base + w.gemm, ws_bytes - w.gemm, st);
-    if (s != APY_OK) return s;
+  // Two chains with no data between them: P+, HHL (+ Y), X1, Q1 on the caller's stream; P-, X2, Q2 and
+  // the singles accumulator products (one after another into acc) forked onto the side stream and
+  // joined before the epilogues.  Under stream capture everything stays on the caller's stream.
+  auto on_side = [&](const apy_gemm_desc& d) {
+    const char* c = (const char*)d.c;
+    return c == base + w.Pa || c == base + w.X2 || (singles && (c == base + w.Q2 || c == base + w.acc));
+  };
+  hipStream_t ss = nullptr;
+  {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) ss = sp_side_stream();
   }
+  hipEvent_t evf = nullptr, evj = nullptr;
+  if (ss) {
+    if (hipEventCreateWithFlags(&evf, hipEventDisableTiming) != hipSuccess) return APY_E_HIP;
+    if (hipEventCreateWithFlags(&evj, hipEventDisableTiming) != hipSuccess) {
+      (void)hipEventDestroy(evf);
+      return APY_E_HIP;
+    }
+    APY_HIP_TRY(hipEventRecord(evf, st));
+    APY_HIP_TRY(hipStreamWaitEvent(ss, evf, 0));
+  }
+  int status = APY_OK;
+  const size_t gb = w.gemm2 - w.gemm;
+  for (int k = 0; k < q && status == APY_OK; ++k)
+    if (ss && on_side(g[k])) status = apy_zgemm_launch(&g[k], base + w.gemm2, ws_bytes - w.gemm2, ss);
+  if (ss && hipEventRecord(evj, ss) != hipSuccess && status == APY_OK) status = APY_E_HIP;
+  for (int k = 0; k < q && status == APY_OK; ++k)
+    if (!ss || !on_side(g[k])) status = apy_zgemm_launch(&g[k], base + w.gemm, gb, st);
+  if (ss) {
+    // join even after a failed launch, so no later work on the caller's stream races the side stream
+    if (hipStreamWaitEvent(st, evj, 0) != hipSuccess && status == APY_OK) status = APY_E_HIP;
+    (void)hipEventDestroy(evf);
+    (void)hipEventDestroy(evj);
+  }
+  if (status != APY_OK) return status;
   const double2* rb = (const double2*)A->resp_b;
   const double2* rt = (const double2*)A->resp_t;
   const double2* re = (const double2*)A->resp_energy;
