@@ -268,6 +268,7 @@ SIGNATURES = {
     "apy_mp2_nc_batched_workspace": (c_size, [c_int, ctypes.POINTER(Mp2NcInstance)]),
     "apy_mp2_nc_batched_setup": (c_int, [c_int, ctypes.POINTER(Mp2NcInstance), c_vp, c_size, _P_I64, c_vp]),
     "apy_mp2_nc_batched_step": (c_int, [c_int, c_i64, c_int, c_vp, c_vp, c_vp]),
+    "apy_mp2_nc_batched_solve": (c_int, [c_int, c_i64, c_dbl, c_int, c_vp, c_vp, c_vp, c_vp]),
     "apy_mp2_nc_batched_energy": (c_int, [c_int, c_i64, c_int, c_dbl, c_vp, c_vp, c_vp]),
     "apy_ci_precondition": (c_int, [c_i64, c_i64, c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "apy_cisd_so_opd_workspace": (c_size, [c_i64, c_i64]),

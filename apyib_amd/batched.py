@@ -115,15 +115,27 @@ class MP2NCBatch:
         _lib.check(self.lib.apy_t2_pack(I.O, I.V, ctypes.c_void_p(dense.data_ptr()),
                                         ctypes.c_void_p(self.x[k].data_ptr()), stream_of(self.dev)), "apy_t2_pack")
 
-    def solve(self, tol=1e-12, maxit=200, chunk=4, graph=False):
+    def solve(self, tol=1e-12, maxit=200, chunk=4, graph=False, persistent=False):
         """Iterate all instances until every residual norm is below tol; the
         amplitudes end in x.  Direct launches check the norms after every
         step (one host read each; an odd final step's x_new is copied back
         in one pass over the packed buffer).  graph=True replays chunks of
         `chunk` (even) steps from a captured HIP graph, checked once per
-        chunk (worth it when the same batch is solved repeatedly).  Returns
-        the iteration count."""
+        chunk (worth it when the same batch is solved repeatedly).
+        persistent=True runs the whole solve as one cooperative launch that
+        tests convergence on the device (apy_mp2_nc_batched_solve; the same
+        iterates and stopping step as the direct launches, one host read at
+        the end).  Returns the iteration count."""
         it = 0
+        if persistent:
+            its = torch.zeros(1, dtype=torch.int64, device=self.dev)
+            _lib.check(self.lib.apy_mp2_nc_batched_solve(self.n, self.items, float(tol), int(maxit),
+                                                         ctypes.c_void_p(self.rnorm.data_ptr()),
+                                                         ctypes.c_void_p(its.data_ptr()),
+                                                         ctypes.c_void_p(self.ws.data_ptr()), stream_of(self.dev)),
+                       "apy_mp2_nc_batched_solve")
+            self.iterations = int(its.cpu())
+            return self.iterations
         if graph:
             chunk = max(2, chunk + (chunk % 2))
             if self._graph is None or self._graph[0] != chunk:

@@ -31,6 +31,8 @@
 // HIP graph.
 #include "common.h"
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
 #include <vector>
 
@@ -197,8 +199,8 @@ __device__ __forceinline__ void z_item_mfma(const double2* __restrict__ fo, cons
 
 // Z_k[i][j][(ab)] = sum_m Foo(m,j) x_im[(ab)] for every instance, row i and slice of
 // ZS pairs (ab) (grid-stride over the Z work list, whose length is in the header).
-__global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const apy_mp2_nc_instance* __restrict__ insts,
-                                                   const int64_t* __restrict__ zoff, int reverse) {
+__device__ __forceinline__ void nc_z_phase(char* __restrict__ ws, const apy_mp2_nc_instance* __restrict__ insts,
+                                           const int64_t* __restrict__ zoff, int reverse) {
   __shared__ double2 fo[ZMAX_O * ZMAX_O];
   const Header* hdr = (const Header*)ws;
   const int64_t nz = hdr->zitems;
@@ -237,6 +239,11 @@ __global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const 
   }
 }
 
+__global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const apy_mp2_nc_instance* __restrict__ insts,
+                                                   const int64_t* __restrict__ zoff, int reverse) {
+  nc_z_phase(ws, insts, zoff, reverse);
+}
+
 // One work item per workgroup of four waves: a 32 x 32 tile of the residual, each wave a 16 x 16
 // quarter of it on the FP64 matrix cores.  Off the diagonal (A < B) both products go into one
 // complex accumulator,
@@ -246,10 +253,10 @@ __global__ __launch_bounds__(BLK) void nc_z_kernel(char* __restrict__ ws, const 
 // matrix and its transpose, so each wave forms only Y = XA GA over its quarter (four MFMAs per
 // four e) and r = Y - Y^T is assembled through LDS.  MFMA f64 16x16x4 lanes: A[l%16][l/16],
 // B[l/16][l%16], D[l/16 + 4r][l%16].
-__global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instance* __restrict__ insts,
-                                                        const Item* __restrict__ items, int reverse,
-                                                        const char* __restrict__ ws, const int64_t* __restrict__ zoff,
-                                                        double* __restrict__ part) {
+__device__ __forceinline__ void ncp_step_item(int64_t b, const apy_mp2_nc_instance* __restrict__ insts,
+                                              const Item* __restrict__ items, int reverse,
+                                              const char* __restrict__ ws, const int64_t* __restrict__ zoff,
+                                              double* __restrict__ part) {
   constexpr int TILE_ELEMS = 2 * MT * (MKC + 1) + 2 * MKC * (MT + 1);
   constexpr int SH_ELEMS = TILE_ELEMS > MT * (MT + 1) ? TILE_ELEMS : MT * (MT + 1);
   __shared__ double2 sh[SH_ELEMS];
@@ -259,7 +266,8 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
   auto GA = reinterpret_cast<double2 (*)[MT + 1]>(sh + 2 * MT * (MKC + 1));
   auto GB = reinterpret_cast<double2 (*)[MT + 1]>(sh + 2 * MT * (MKC + 1) + MKC * (MT + 1));
   auto Ysh = reinterpret_cast<double2 (*)[MT + 1]>(sh);    // diagonal tiles, after the products
-  const Item it = items[blockIdx.x];
+  __syncthreads();                                   // LDS of the previous item is no longer read
+  const Item it = items[b];
   const apy_mp2_nc_instance I = insts[it.inst];
   const int64_t O = I.O, V = I.V, n = O + V, Pv = npair(V);
   const int transpose_f = I.transpose_f;
@@ -394,13 +402,21 @@ __global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instanc
     if (tid < w) red[tid] += red[tid + w];
     __syncthreads();
   }
-  if (tid == 0) part[blockIdx.x] = red[0];
+  if (tid == 0) part[b] = red[0];
 }
 
-__global__ __launch_bounds__(BLK) void per_instance_sum(const int64_t* __restrict__ off,
-                                                        const double* __restrict__ part, double* __restrict__ out) {
+__global__ __launch_bounds__(SBLK) void ncp_step_kernel(const apy_mp2_nc_instance* __restrict__ insts,
+                                                        const Item* __restrict__ items, int reverse,
+                                                        const char* __restrict__ ws, const int64_t* __restrict__ zoff,
+                                                        double* __restrict__ part) {
+  ncp_step_item(blockIdx.x, insts, items, reverse, ws, zoff, part);
+}
+
+__device__ __forceinline__ void instance_sum(int64_t k, const int64_t* __restrict__ off,
+                                             const double* __restrict__ part, double* __restrict__ out) {
   __shared__ double red[BLK];
-  const int64_t b = off[blockIdx.x], e = off[blockIdx.x + 1];
+  __syncthreads();
+  const int64_t b = off[k], e = off[k + 1];
   double acc = 0.0;
   for (int64_t q = b + threadIdx.x; q < e; q += BLK) acc += part[q];
   red[threadIdx.x] = acc;
@@ -409,7 +425,54 @@ __global__ __launch_bounds__(BLK) void per_instance_sum(const int64_t* __restric
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) out[k] = red[0];
+}
+
+__global__ __launch_bounds__(BLK) void per_instance_sum(const int64_t* __restrict__ off,
+                                                        const double* __restrict__ part, double* __restrict__ out) {
+  instance_sum(blockIdx.x, off, part, out);
+}
+
+// The whole solve in one cooperative launch: every resident workgroup loops over the Z work items,
+// the step items and the per-instance sums of an iteration with a grid barrier after each phase, then
+// reads the n residual norms and stops when sqrt(max) < tol (the test of MP2NCBatch.solve after every
+// step) or after maxit steps.  Each phase is the same per-item code as the three kernels of
+// apy_mp2_nc_batched_step, so the iterates are bit-identical to the step-by-step solve.  An odd step
+// count leaves the amplitudes in x_new; they are copied back to x before the kernel ends.
+__global__ __launch_bounds__(SBLK) void nc_persistent_kernel(char* __restrict__ ws,
+                                                             const apy_mp2_nc_instance* __restrict__ insts,
+                                                             const Item* __restrict__ items,
+                                                             const int64_t* __restrict__ off,
+                                                             const int64_t* __restrict__ zoff,
+                                                             double* __restrict__ part, double* __restrict__ rnorm,
+                                                             int n, int64_t n_items, double tol, int maxit,
+                                                             int64_t* __restrict__ iters) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  int rev = 0, it = 0;
+  while (it < maxit) {
+    nc_z_phase(ws, insts, zoff, rev);
+    grid.sync();
+    for (int64_t b = blockIdx.x; b < n_items; b += gridDim.x) ncp_step_item(b, insts, items, rev, ws, zoff, part);
+    grid.sync();
+    for (int64_t k = blockIdx.x; k < n; k += gridDim.x) instance_sum(k, off, part, rnorm);
+    grid.sync();
+    ++it;
+    rev ^= 1;
+    double mx = 0.0;
+    for (int k = 0; k < n; ++k) mx = fmax(mx, rnorm[k]);
+    if (sqrt(mx) < tol) break;
+  }
+  if (rev) {
+    for (int k = 0; k < n; ++k) {
+      const apy_mp2_nc_instance I = insts[k];
+      const int64_t len = npair(I.O) * npair(I.V);
+      const double2* src = (const double2*)I.x_new;
+      double2* dst = (double2*)I.x;
+      for (int64_t e = blockIdx.x * (int64_t)SBLK + threadIdx.x; e < len; e += (int64_t)gridDim.x * SBLK)
+        dst[e] = src[e];
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *iters = it;
 }
 
 // energy[k] = scale * sum_dense w_k x_k = 4 scale * sum_(i<j, a<b) w_k x_k (w and x
@@ -582,6 +645,33 @@ int apy_mp2_nc_batched_step(int n, int64_t n_items, int reverse, void* rnorm, vo
   APY_LAUNCH_CHECK();
   per_instance_sum<<<(unsigned)n, BLK, 0, st>>>((const int64_t*)(w + L.off), part, (double*)rnorm);
   APY_LAUNCH_CHECK();
+  return APY_OK;
+}
+
+int apy_mp2_nc_batched_solve(int n, int64_t n_items, double tol, int maxit, void* rnorm, void* iterations,
+                             void* workspace, void* stream) {
+  if (n < 1 || n > 65535 || n_items < 1 || n_items > 2147483647 || maxit < 1 || !(tol >= 0.0) || !rnorm ||
+      !iterations || !workspace)
+    return APY_E_ARG;
+  const WsLayout L = layout(n, n_items);
+  char* w = (char*)workspace;
+  int dev = 0, ncu = 0, per_cu = 0;
+  APY_HIP_TRY(hipGetDevice(&dev));
+  APY_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  APY_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, nc_persistent_kernel, SBLK, 0));
+  if (per_cu < 1 || ncu < 1) return APY_E_HIP;
+  // every workgroup resident at once (a cooperative launch refuses a larger grid)
+  const int64_t grid = std::min<int64_t>((int64_t)per_cu * ncu, n_items);
+  const apy_mp2_nc_instance* insts = (const apy_mp2_nc_instance*)(w + L.inst);
+  const Item* items = (const Item*)(w + L.item);
+  const int64_t* off = (const int64_t*)(w + L.off);
+  const int64_t* zoff = (const int64_t*)(w + L.zoff);
+  double* part = (double*)(w + L.part);
+  double* rn = (double*)rnorm;
+  int64_t* it = (int64_t*)iterations;
+  void* args[] = {&w, &insts, &items, &off, &zoff, &part, &rn, &n, &n_items, &tol, &maxit, &it};
+  APY_HIP_TRY(hipLaunchCooperativeKernel((const void*)nc_persistent_kernel, dim3((unsigned)grid), dim3(SBLK), args, 0,
+                                         (hipStream_t)stream));
   return APY_OK;
 }
 

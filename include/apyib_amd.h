@@ -630,6 +630,17 @@ size_t apy_mp2_nc_batched_workspace(int n, const apy_mp2_nc_instance* inst);
 int apy_mp2_nc_batched_setup(int n, const apy_mp2_nc_instance* inst, void* workspace, size_t ws_bytes,
                              int64_t* n_items, void* stream);
 int apy_mp2_nc_batched_step(int n, int64_t n_items, int reverse, void* rnorm, void* workspace, void* stream);
+/* The whole batched solve in ONE persistent cooperative launch (the
+ * per-displacement loop of fin_diff.py:285-370 as one kernel): the resident
+ * workgroups iterate apy_mp2_nc_batched_step's three phases with a grid
+ * barrier after each, test sqrt(max_k rnorm[k]) < tol on the device after
+ * every step, and stop there or after maxit steps, leaving the amplitudes in
+ * x (copied back from x_new after an odd step count).  iterations (device
+ * int64) receives the step count; the iterates are bit-identical to stepping
+ * from x with apy_mp2_nc_batched_step and testing after each step.  The grid
+ * is the number of workgroups that fit on the device at once. */
+int apy_mp2_nc_batched_solve(int n, int64_t n_items, double tol, int maxit, void* rnorm, void* iterations,
+                             void* workspace, void* stream);
 int apy_mp2_nc_batched_energy(int n, int64_t n_items, int use_x_new, double scale, void* energy,
                               void* workspace, void* stream);
 

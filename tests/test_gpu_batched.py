@@ -176,3 +176,44 @@ def test_packed_storage_round_trip_and_warm_start(dev):
         assert torch.equal(w.x[0], b.x[k])
         w.solve(tol=0.0, maxit=2, chunk=2)
         assert float(w.rnorm.max().cpu()) ** 0.5 < 1e-12
+
+
+@pytest.mark.parametrize("tol,maxit", [(1e-12, 400), (0.0, 7), (0.0, 8), (1e-3, 400)])
+def test_persistent_solve_equals_direct_launches(dev, tol, maxit):
+    """One cooperative launch for the whole solve (apy_mp2_nc_batched_solve)
+    against the step-by-step solve with a host test after each step: the
+    same stopping step and bit-identical amplitudes and energies, for uneven
+    sizes with amplitude and lambda instances in one batch, odd and even
+    step counts (x_new copied back) and the maxit stop."""
+    sizes = [(4, 6), (3, 9), (6, 5), (2, 11), (5, 7), (7, 40), (3, 66)]
+    insts = []
+    for k, (O, V) in enumerate(sizes):
+        inst, _ = _host_instance(O, V, 40 + k, dev)
+        if k % 3 == 2:                                   # a lambda instance: F^T and the driver <ij||ab>/4
+            inst = Instance(O, V, inst.F, inst.eps, inst.w, inst.w, transpose_f=True, drv_scale=0.25)
+        insts.append(inst)
+    d = MP2NCBatch(insts)
+    itd = d.solve(tol=tol, maxit=maxit)
+    p = MP2NCBatch(insts)
+    itp = p.solve(tol=tol, maxit=maxit, persistent=True)
+    assert itd == itp
+    for k in range(len(insts)):
+        assert torch.equal(d.x[k], p.x[k])
+    assert torch.equal(d.energies(), p.energies())
+    assert torch.equal(d.rnorm, p.rnorm)
+
+
+def test_persistent_solve_set_b_sizes(dev):
+    """The 36-instance uneven set at its own sizes (O 14-18, V 54-66): the
+    persistent solve against the direct launches, bit for bit."""
+    rng = np.random.default_rng(18)
+    sizes = [(int(rng.integers(14, 19)), int(rng.integers(54, 67))) for _ in range(18)]
+    insts = fd_mp2_instances(16, 60, 1, 5, h=1e-4, device=dev, sizes=sizes)[0]
+    d = MP2NCBatch(insts)
+    itd = d.solve(tol=1e-12, maxit=400)
+    p = MP2NCBatch(insts)
+    itp = p.solve(tol=1e-12, maxit=400, persistent=True)
+    assert itd == itp and len(insts) == 36
+    assert torch.equal(d.energies(), p.energies())
+    for k in range(len(insts)):
+        assert torch.equal(d.x[k], p.x[k])
