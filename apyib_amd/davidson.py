@@ -295,7 +295,12 @@ class CISDDavidson:
     def prepare_graphs(self):
         """Capture one HIP graph per subspace size (1 .. max_space): each holds
         the whole device-controlled iteration at that size, so a solve
-        replays a fixed sequence of graphs.  Capture records without running."""
+        replays a fixed sequence of graphs.  Capture records without running.
+        The graphs point into the per-device scratch buffer that is current
+        now (already sized by the sigma of the first vector); this object
+        keeps a reference to it, so a later, larger request elsewhere (which
+        replaces the cached buffer) cannot free memory the graphs write."""
+        self._graph_ws = workspace(max(self.soci.ws_bytes, self.soci.red_bytes), self.dev)
         for k in range(1, self.max_space + 1):
             if k in self._graphs:
                 continue
@@ -304,6 +309,8 @@ class CISDDavidson:
             n0 = self.nsigma
             with torch.cuda.graph(g):
                 nk = self._device_iteration(k)
+            if workspace(0, self.dev) is not self._graph_ws:
+                raise RuntimeError("the scratch buffer grew during graph capture")
             ns, self.nsigma = self.nsigma - n0, n0          # sigmas per replay; capture ran none
             self._graphs[k] = (g, nk, self._ritz_dev, ns)
 
