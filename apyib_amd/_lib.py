@@ -239,6 +239,10 @@ SIGNATURES = {
     "apy_heev_lowest": (c_int, [c_int, c_vp, c_int, c_vp, c_vp]),
     "apy_davidson_check": (c_int, [c_vp, c_vp, c_dbl, c_dbl, c_vp, c_vp, c_int, c_vp]),
     "apy_davidson_fused_workspace": (c_size, []),
+    "apy_peak_mfma_f64_flops": (c_dbl, [c_int, c_int]),
+    "apy_peak_mfma_f64": (c_int, [c_int, c_int, c_vp, c_vp]),
+    "apy_peak_copy": (c_int, [c_i64, c_vp, c_vp, c_int, c_vp]),
+    "apy_peak_read": (c_int, [c_i64, c_vp, c_vp, c_int, c_vp]),
     "apy_davidson_residual": (c_int, [c_int, _P_VP, _P_VP, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_davidson_combine_dots": (c_int, [c_int, _P_VP, c_vp, c_i64, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_zdot_many_seg": (c_int, [c_int, _P_VP, c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_size, c_vp]),

@@ -65,6 +65,53 @@ class HipEvents:
             raise RuntimeError("hipEventRecord failed")
 
 
+def measured_peaks(dev, hip, reps=4):
+    """The two roofline denominators measured on this device with the
+    library's microbenchmarks: FP64 matrix-core rate (apy_peak_mfma_f64,
+    MFMA chains from registers) and HBM copy bandwidth (apy_peak_copy, 1 GiB
+    read + 1 GiB written).  Best of reps launches each, timed with HIP events
+    on the launch stream."""
+    import torch
+    from apyib_amd import _lib
+    lib = _lib.load()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    ncu = torch.cuda.get_device_properties(dev).multi_processor_count
+    blocks, iters = 8 * ncu, 20000
+    out = torch.empty(blocks * 256, dtype=torch.float64, device=dev)
+    flops = float(lib.apy_peak_mfma_f64_flops(blocks, iters))
+    n = 1 << 26                                      # complex128 words: 1 GiB per buffer
+    src = torch.ones(n, dtype=torch.complex128, device=dev)
+    dst = torch.empty_like(src)
+
+    def best(launch, amount):
+        rate = 0.0
+        for _ in range(reps + 1):                    # the first launch warms up
+            e0, e1 = hip.create(), hip.create()
+            hip.record(e0, stream)
+            _lib.check(launch(), "peak microbenchmark")
+            hip.record(e1, stream)
+            torch.cuda.synchronize(dev)
+            rate = max(rate, amount / (hip.elapsed_ms(e0, e1) * 1e-3))
+            hip.destroy(e0)
+            hip.destroy(e1)
+        return rate
+    mf = best(lambda: lib.apy_peak_mfma_f64(blocks, iters, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)),
+              flops) / 1e12
+    bw = best(lambda: lib.apy_peak_copy(n, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                        8 * ncu, ctypes.c_void_p(stream)), 32.0 * n) / 1e12
+    rd_out = torch.empty(8 * ncu * 256, dtype=torch.float64, device=dev)
+    rd = best(lambda: lib.apy_peak_read(n, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(rd_out.data_ptr()),
+                                        8 * ncu, ctypes.c_void_p(stream)), 16.0 * n) / 1e12
+    del out, src, dst, rd_out
+    res = {"what": "build-owned microbenchmarks on this device: v_mfma_f64_16x16x4f64 chains from registers "
+                   f"({blocks} workgroups x {iters} iterations), a 1 GiB complex128 copy (bytes read + written) "
+                   "and a 1 GiB read stream",
+           "fp64_mfma_tflops": round(mf, 2), "hbm_copy_tbps": round(bw, 3), "hbm_read_tbps": round(rd, 3),
+           "published": {"fp64_mfma_tflops": PEAK_FP64, "hbm_tbps": 8.0}}
+    log(f"measured peaks: {res}")
+    return res
+
+
 def read_traffic():
     """HBM bytes per PPL launch from the committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", "ppl_o20_v160_pmc.txt")
@@ -797,6 +844,7 @@ def main():
     fl = step_flops(O, V, True)
     flops_sigma = sum(fl.values())
     hip = HipEvents()
+    peaks = measured_peaks(dev, hip)
     K, W = args.steps, args.warmup
     evs = [(hip.create(), hip.create()) for _ in range(K)]
 
@@ -948,7 +996,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64, 4),
             "traffic": read_traffic(),
+            "frac_of_measured_peak": round(achieved / peaks["fp64_mfma_tflops"], 4),
         },
+        "measured_peaks": peaks,
         "cpu_baseline": cpu,
         "aat_set": aat_set,
         "spatial_cisd": spatial,

@@ -892,6 +892,24 @@ size_t apy_aat_norms_workspace(int64_t nset);
 int apy_aat_norms(int64_t oa, int64_t nv, int64_t nset, const void* const* t1, const void* const* t2, void* out,
                   void* workspace, size_t ws_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Roofline denominators measured on the device (no counterpart in the
+ * reference; they give the measured side of the FP64 matrix-core and HBM
+ * peaks the throughput figures are quoted against).
+ *   apy_peak_mfma_f64: blocks x 256 threads, each wave iterating 4
+ *     independent v_mfma_f64_16x16x4f64 accumulator chains `iters` times from
+ *     registers; apy_peak_mfma_f64_flops gives its flop count (host); out
+ *     receives blocks x 256 doubles (keeps the chains live).
+ *   apy_peak_copy: dst[e] = src[e] for n complex128 words (16 n bytes read
+ *     and 16 n written), grid-stride over `blocks` x 256 threads.
+ *   apy_peak_read: the n words of src read (16 n bytes), each thread's sum
+ *     written to out (blocks x 256 doubles).
+ * The caller times a launch with events on the stream. */
+double apy_peak_mfma_f64_flops(int blocks, int iters);
+int apy_peak_mfma_f64(int blocks, int iters, void* out, void* stream);
+int apy_peak_copy(int64_t n, const void* src, void* dst, int blocks, void* stream);
+int apy_peak_read(int64_t n, const void* src, void* out, int blocks, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
