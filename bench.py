@@ -446,6 +446,59 @@ def cisd_field_set_measurement(dev, seed, O=24, V=200, h=1e-4, max_space=8):
     return res
 
 
+def eri_so_measurement(dev, seed, m=90):
+    """compute_ERI_SO (utils.py:317-365) at a C1-sized molecule, m = 90
+    spatial orbitals -> 180 spin orbitals (1.05 GB in, 16.8 GB out): the GPU
+    expansion (apy_eri_so_from_mo) at full size, the reference's four-index
+    Python loop on a subsample of one outer index (extrapolated by 2m), and
+    the vectorized NumPy form at m/2 (scaled by the element count); the GPU
+    and loop results agree bit for bit on the subsample."""
+    import numpy as np
+    import torch
+    from apyib_amd import transforms as tr
+    from oracle import transforms as otr
+    g = torch.Generator(device=dev).manual_seed(seed + 90)
+    x = torch.complex(torch.randn((m,) * 4, generator=g, dtype=torch.float64, device=dev),
+                      torch.randn((m,) * 4, generator=g, dtype=torch.float64, device=dev))
+    out = tr.eri_so(x)                                    # warm
+    del out
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ts = []
+    for _ in range(3):
+        e0.record()
+        out = tr.eri_so(x)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+        if len(ts) < 3:
+            del out
+    gpu_ms = min(ts)
+    n = 2 * m
+    host = x.cpu().numpy()
+    t0 = time.perf_counter()
+    loop = otr.compute_ERI_SO_loop(host, p_range=[0])     # p = 0: n^3 iterations of the loop body
+    t_loop = time.perf_counter() - t0
+    exact = bool(np.array_equal(loop[0], out[0].cpu().numpy()))
+    del out, loop
+    mh = m // 2
+    t0 = time.perf_counter()
+    vec = otr.compute_ERI_SO(host[:mh, :mh, :mh, :mh].copy())
+    t_vec = time.perf_counter() - t0
+    del vec, host, x
+    torch.cuda.empty_cache()
+    bytes_moved = 16.0 * (m ** 4 + n ** 4)
+    res = {"workload": f"compute_ERI_SO at m={m} spatial -> {n} spin orbitals (16.8 GB complex128 out)",
+           "gpu_ms": round(gpu_ms, 3), "gpu_tbps": round(bytes_moved / (gpu_ms * 1e-3) / 1e12, 3),
+           "cpu_loop_s_sample": round(t_loop, 2), "cpu_loop_sample": f"outer index p = 0 of {n} ({n ** 3} iterations)",
+           "cpu_loop_s_extrapolated": round(t_loop * n, 1),
+           "cpu_vectorized_s_sample": round(t_vec, 2), "cpu_vectorized_sample": f"m = {mh} (1/16 of the elements)",
+           "cpu_vectorized_s_extrapolated": round(t_vec * (m / mh) ** 4, 1),
+           "gpu_equals_loop_on_sample": exact, "cores": 1}
+    log(f"eri_so: {res}")
+    return res
+
+
 def spatial_step_flops(o, v):
     """Executed flops of one spin-adapted CISD step (apy_cisd_sp_step, F_vv folded into the ring
     operator): the two ladder halves, HHL, the ring X1 (K = 2ov) and X2, singles->doubles and the six
@@ -795,6 +848,7 @@ def main():
     ap.add_argument("--no-mp2-adjoint-set", action="store_true",
                     help="skip the MP2 adjoint set (8 instances O=30 V=270)")
     ap.add_argument("--no-spatial", action="store_true", help="skip the spin-adapted CISD step timing")
+    ap.add_argument("--no-eri-so", action="store_true", help="skip the compute_ERI_SO timing (GPU and CPU sample)")
     ap.add_argument("--no-analytic-response", action="store_true",
                     help="skip the first-order CISD amplitude set timing")
     ap.add_argument("--no-h2o2-aat", action="store_true",
@@ -944,6 +998,7 @@ def main():
         if not args.no_mp2_adjoint_set:
             aat_set["D"] = mp2_adjoint_set_measurement(dev, args.seed)
     spatial = None if args.no_spatial else spatial_cisd_measurement(dev, args.seed, tune=args.tune)
+    eri_so = None if args.no_eri_so else eri_so_measurement(dev, args.seed)
     response = None if args.no_analytic_response else analytic_response_measurement(dev, args.seed,
                                                                                      tune=args.tune)
     h2o2 = None if args.no_h2o2_aat else h2o2_aat_measurement(dev)
@@ -1002,6 +1057,7 @@ def main():
         "cpu_baseline": cpu,
         "aat_set": aat_set,
         "spatial_cisd": spatial,
+        "eri_so": eri_so,
         "analytic_response": response,
         "h2o2_aat": h2o2,
     }

@@ -70,11 +70,13 @@ def compute_F_SO_loop(F_MO):
     return F
 
 
-def compute_ERI_SO_loop(ERI_MO):
-    """utils.py:317-365 as written (pure-Python four-index loop)."""
+def compute_ERI_SO_loop(ERI_MO, p_range=None):
+    """utils.py:317-365 as written (pure-Python four-index loop).  p_range
+    (optional) restricts the outer index to a subsample; the other rows of
+    the result stay zero."""
     n = [2 * s for s in ERI_MO.shape]
     out = np.zeros(n, dtype=ERI_MO.dtype)
-    for p in range(n[0]):
+    for p in (range(n[0]) if p_range is None else p_range):
         for q in range(n[1]):
             spq = max(_spin(p) * _spin(q), 0)
             for r in range(n[2]):
