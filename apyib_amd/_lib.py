@@ -49,6 +49,7 @@ class GemmDesc(ctypes.Structure):
         ("reduce_batch", c_int),
         ("split_k", c_int),
         ("tile", c_int),
+        ("gauss3m", c_int),
     ]
 
 
@@ -119,7 +120,7 @@ class CisdSoArgs(ctypes.Structure):
         ("c0_re", c_dbl), ("c0_im", c_dbl),
         ("t", c_vp), ("t_new", c_vp), ("out", c_vp),
         ("ev_begin", c_vp), ("ev_end", c_vp), ("c0_dev", c_vp),
-        ("ring_f", c_vp), ("packed", c_int),
+        ("ring_f", c_vp), ("packed", c_int), ("gauss3m", c_int),
     ]
 
 
@@ -131,7 +132,7 @@ class CisdSoSigmaBatchArgs(ctypes.Structure):
         ("fock", ctypes.POINTER(c_vp)), ("hoo", ctypes.POINTER(c_vp)),
         ("c0", ctypes.POINTER(c_dbl)),
         ("t", ctypes.POINTER(c_vp)), ("out", ctypes.POINTER(c_vp)),
-        ("ev_begin", c_vp), ("ev_end", c_vp),
+        ("ev_begin", c_vp), ("ev_end", c_vp), ("gauss3m", c_int),
     ]
 
 

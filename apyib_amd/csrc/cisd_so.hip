@@ -659,9 +659,10 @@ struct SigmaWS {
 //   F[k]      its Fock matrix, hoo[k] its pair operator
 //   ring_f    (nb == 1 only, optional) the ring block with F_vv folded in
 // With null pointers the list is built only to size the workspace.
+//   g3        the PPL and ring products in the three-product complex form
 void sigma_gemms(int64_t O, int64_t V, int nb, int singles, const apy_so_ints* I, const double2* const* F,
                  const void* const* hoo, const double2* const* t, const void* ring_f, const SigmaWS& w,
-                 std::vector<apy_gemm_desc>& G) {
+                 std::vector<apy_gemm_desc>& G, int g3 = 0) {
   const int64_t Po = npair(O), Pv = npair(V), OV = O * V, n = O + V;
   const int64_t PP = Po * Pv, R2 = OV * OV;
   G.clear();
@@ -670,9 +671,11 @@ void sigma_gemms(int64_t O, int64_t V, int nb, int singles, const apy_so_ints* I
   auto offw = [](double2* p, int64_t o) -> double2* { return p ? p + o : nullptr; };
   // [0] PPL over all instances: R_p = T_p . vvvv_p^T
   fill(d, nb * Po, Pv, Pv, APY_MK, w.tp, Pv, APY_NK, I ? I->vvvv_p : nullptr, Pv, w.rp, Pv, 1.0, 0.0);
+  d.gauss3m = g3;
   G.push_back(d);
   // [1] ring over all instances: X = T' . ring^T (ring_f: F_vv folded in)
   fill(d, nb * OV, OV, OV, APY_MK, w.tr, OV, APY_NK, ring_f ? ring_f : (I ? I->ring : nullptr), OV, w.X, OV, 1.0, 0.0);
+  d.gauss3m = g3;
   G.push_back(d);
   if (singles) {
     // Q_T[j][(k,i)][(ab)] = sum_c t1_k[i,c] <ab||cj>: batched over j, all instances' t1 stacked along M
@@ -761,7 +764,7 @@ SigmaWS carve(int64_t O, int64_t V, int nb, int singles, int dense_in, const voi
 // images, the transposed integral pieces and all GEMMs.
 int sigma_body(int64_t O, int64_t V, int nb, int singles, const apy_so_ints* I, const double2* const* F,
                const void* const* hoo, const double2* const* t, int packed_in, const void* ring_f, SigmaWS& w,
-               hipEvent_t ev0, hipEvent_t ev1, hipStream_t st) {
+               hipEvent_t ev0, hipEvent_t ev1, hipStream_t st, int g3) {
   const int64_t Po = npair(O), Pv = npair(V), OV = O * V, PP = Po * Pv;
   if (packed_in) {
     if (nb != 1) return APY_E_ARG;
@@ -791,7 +794,7 @@ int sigma_body(int64_t O, int64_t V, int nb, int singles, const apy_so_ints* I, 
     APY_LAUNCH_CHECK();
   }
   std::vector<apy_gemm_desc> G;
-  sigma_gemms(O, V, nb, singles, I, F, hoo, t, ring_f, w, G);
+  sigma_gemms(O, V, nb, singles, I, F, hoo, t, ring_f, w, G, g3);
   for (size_t q = 0; q < G.size(); ++q) {
     const int s = q == 0 ? apy_zgemm_launch(&G[q], w.gemm, w.gemm_bytes, st, ev0, ev1)
                          : apy_zgemm_launch(&G[q], w.gemm, w.gemm_bytes, st);
@@ -917,7 +920,7 @@ int apy_cisd_so_step_ws(const apy_cisd_so_args* A, void* workspace, size_t ws_by
   const void* Hk[1] = {A->hoo};
   const double2* Tk[1] = {t};
   int s = sigma_body(O, V, 1, singles, I, Fk, Hk, Tk, packed, A->ring_f, w, (hipEvent_t)A->ev_begin,
-                     (hipEvent_t)A->ev_end, st);
+                     (hipEvent_t)A->ev_end, st, A->gauss3m ? 1 : 0);
   if (s != APY_OK) return s;
 
   const double c0r = A->mode == APY_MODE_RESIDUAL ? 1.0 : A->c0_re;
@@ -965,7 +968,7 @@ int apy_cisd_so_tune(const apy_cisd_so_args* A, void* workspace, size_t ws_bytes
   const void* Hk[1] = {A->hoo};
   const double2* Tk[1] = {(const double2*)A->t};
   std::vector<apy_gemm_desc> G;
-  sigma_gemms(O, V, 1, singles, A->ints, Fk, Hk, Tk, A->ring_f, w, G);
+  sigma_gemms(O, V, 1, singles, A->ints, Fk, Hk, Tk, A->ring_f, w, G, A->gauss3m ? 1 : 0);
   for (const apy_gemm_desc& d : G) {
     if (!d.a || !d.b) continue;
     int s = apy_zgemm_tune_impl(&d, w.gemm, w.gemm_bytes, reps, (hipStream_t)stream, nullptr, nullptr, nullptr);
@@ -1032,7 +1035,7 @@ int apy_cisd_so_sigma_batch(const apy_cisd_so_sigma_batch_args* A, void* workspa
     T[k] = (const double2*)A->t[k];
   }
   int s = sigma_body(O, V, nb, singles, I, F.data(), H.data(), T.data(), 0, nullptr, w, (hipEvent_t)A->ev_begin,
-                     (hipEvent_t)A->ev_end, st);
+                     (hipEvent_t)A->ev_end, st, A->gauss3m ? 1 : 0);
   if (s != APY_OK) return s;
   for (int k = 0; k < nb; ++k) {
     double2* out = (double2*)A->out[k];

@@ -59,6 +59,7 @@ struct GemmArgs {
   int64_t mtiles, ntiles;
   int64_t total;
   double2* ws;      // partial sums [npart][m][n] or nullptr
+  int g3;           // Gauss three-product form (apy_gemm_desc.gauss3m)
 };
 
 __device__ inline int64_t xcd_remap(int64_t bid, int64_t total) {
@@ -74,7 +75,10 @@ __device__ inline double2 masked(double2 v, bool in, unsigned long long sg) {
                       __longlong_as_double((__double_as_longlong(v.y) ^ sg) & m));
 }
 
-template <int BM, int BN, int WM, int WN, int BK, bool A_MK, bool B_NK>
+// G3: the Gauss form of the complex product, three real products per complex multiply-add
+// (T1 = ar br, T2 = ai bi, T3 = (ar + ai)(br + bi); re = T1 - T2, im = T3 - T1 - T2): three MFMAs
+// instead of four, one more accumulator plane, the operand sums formed in registers.
+template <int BM, int BN, int WM, int WN, int BK, bool A_MK, bool B_NK, bool G3>
 __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
   constexpr int RB = WM / 16, CB = WN / 16;
   constexpr int BKP = BK + 1;  // padded row (complex elements) of k-contiguous LDS tiles
@@ -109,11 +113,17 @@ __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
   const double2* A = p.a + bidx * p.sa;
   const double2* B = p.b + bidx * p.sb;
 
-  v4d acc_re[RB][CB], acc_im[RB][CB];
+  v4d acc_re[RB][CB], acc_im[RB][CB], acc_3[G3 ? RB : 1][G3 ? CB : 1];
 #pragma unroll
   for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int j = 0; j < CB; ++j) { acc_re[i][j] = (v4d){0, 0, 0, 0}; acc_im[i][j] = (v4d){0, 0, 0, 0}; }
+  if (G3) {
+#pragma unroll
+    for (int i = 0; i < (G3 ? RB : 1); ++i)
+#pragma unroll
+      for (int j = 0; j < (G3 ? CB : 1); ++j) acc_3[i][j] = (v4d){0, 0, 0, 0};
+  }
 
   // Two register stages: stage s+2 is in flight while stage s+1 waits in
   // registers and stage s is consumed from LDS.
@@ -200,20 +210,38 @@ __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
         const int col = wn * WN + j * 16 + (lane & 15);
         bv[j] = Bb[B_NK ? col * BKP + kk : kk * BN + col];
       }
+      if (G3) {
+        // acc_re = T1, acc_im = T2, acc_3 = T3
+        double sa[RB], sb[CB];
 #pragma unroll
-      for (int i = 0; i < RB; ++i)
+        for (int i = 0; i < RB; ++i) sa[i] = av[i].x + av[i].y;
 #pragma unroll
-        for (int j = 0; j < CB; ++j) {
-          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].x, acc_re[i][j], 0, 0, 0);
-          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].y, acc_im[i][j], 0, 0, 0);
-        }
+        for (int j = 0; j < CB; ++j) sb[j] = bv[j].x + bv[j].y;
 #pragma unroll
-      for (int i = 0; i < RB; ++i)
+        for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < CB; ++j) {
-          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[i].y, bv[j].y, acc_re[i][j], 0, 0, 0);
-          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].y, bv[j].x, acc_im[i][j], 0, 0, 0);
-        }
+          for (int j = 0; j < CB; ++j) {
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].x, acc_re[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].y, bv[j].y, acc_im[i][j], 0, 0, 0);
+            acc_3[G3 ? i : 0][G3 ? j : 0] =
+                __builtin_amdgcn_mfma_f64_16x16x4f64(sa[i], sb[j], acc_3[G3 ? i : 0][G3 ? j : 0], 0, 0, 0);
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+          for (int j = 0; j < CB; ++j) {
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].x, acc_re[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].x, bv[j].y, acc_im[i][j], 0, 0, 0);
+          }
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+          for (int j = 0; j < CB; ++j) {
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[i].y, bv[j].y, acc_re[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].y, bv[j].x, acc_im[i][j], 0, 0, 0);
+          }
+      }
     }
     store(Other{}, kbeg + (st + 1) * BK);  // past the end: writes zeros
     __syncthreads();
@@ -235,7 +263,12 @@ __global__ __launch_bounds__(NTHR) void zgemm_mfma(GemmArgs p) {
         const int64_t row = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
         const int64_t col = n0 + wn * WN + j * 16 + (lane & 15);
         if (row < p.m && col < p.n) {
-          const double vr = acc_re[i][j][r], vi = acc_im[i][j][r];
+          double vr = acc_re[i][j][r], vi = acc_im[i][j][r];
+          if (G3) {
+            const double t1 = vr, t2 = vi, t3 = acc_3[G3 ? i : 0][G3 ? j : 0][r];
+            vr = t1 - t2;
+            vi = t3 - t1 - t2;
+          }
           double o_r = p.alr * vr - p.ali * vi, o_i = p.alr * vi + p.ali * vr;
           if (W) {
             W[row * p.n + col] = make_double2(o_r, o_i);
@@ -340,9 +373,12 @@ constexpr Cfg CFGS[NCFG + 1] = {
 template <int CI, bool AM, bool BN_>
 struct KernelOf {
   static constexpr Cfg c = CFGS[CI];
-  static void* ptr() { return (void*)zgemm_mfma<c.bm, c.bn, c.wm, c.wn, c.bk, AM, BN_>; }
+  static void* ptr() { return (void*)zgemm_mfma<c.bm, c.bn, c.wm, c.wn, c.bk, AM, BN_, false>; }
   static void launch(const GemmArgs& a, hipStream_t s) {
-    zgemm_mfma<c.bm, c.bn, c.wm, c.wn, c.bk, AM, BN_><<<dim3((unsigned)a.total), dim3(NTHR), 0, s>>>(a);
+    if (a.g3)
+      zgemm_mfma<c.bm, c.bn, c.wm, c.wn, c.bk, AM, BN_, true><<<dim3((unsigned)a.total), dim3(NTHR), 0, s>>>(a);
+    else
+      zgemm_mfma<c.bm, c.bn, c.wm, c.wn, c.bk, AM, BN_, false><<<dim3((unsigned)a.total), dim3(NTHR), 0, s>>>(a);
   }
 };
 
@@ -554,6 +590,7 @@ int run_plan(const apy_gemm_desc* d, const Plan& pl, void* workspace, size_t ws_
   a.batch = d->batch; a.split = pl.split; a.kchunk = pl.kchunk;
   a.mtiles = pl.mtiles; a.ntiles = pl.ntiles; a.total = pl.total;
   a.ws = pl.use_ws ? (double2*)workspace : nullptr;
+  a.g3 = d->gauss3m ? 1 : 0;
   if (ev0) APY_HIP_TRY(hipEventRecord(ev0, s));
   launch_cfg(pl.cfg, a, d->a_layout == APY_MK, d->b_layout == APY_NK, s);
   APY_LAUNCH_CHECK();

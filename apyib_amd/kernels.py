@@ -43,10 +43,11 @@ def ptr(t):
 
 def zgemm_raw(*, m, n, k, a, a_layout, lda, b, b_layout, ldb, c, ldc, alpha=1.0, beta=0.0,
               conj_a=False, conj_b=False, batch=1, stride_a=0, stride_b=0, stride_c=0,
-              reduce_batch=False, split_k=0, tile=0, tune=False, reps=3):
+              reduce_batch=False, split_k=0, tile=0, tune=False, reps=3, gauss3m=False):
     """C = alpha * op(A) op(B) + beta * C on raw strides (see apy_zgemm).
     With tune=True the candidate plans are timed first (apy_zgemm_tune) and
-    the fastest is cached for the shape; returns (tile, split, ms) then."""
+    the fastest is cached for the shape; returns (tile, split, ms) then.
+    gauss3m: three real products per complex multiply-add instead of four."""
     lib = _lib.load()
     alpha = complex(alpha)
     beta = complex(beta)
@@ -56,7 +57,7 @@ def zgemm_raw(*, m, n, k, a, a_layout, lda, b, b_layout, ldb, c, ldc, alpha=1.0,
         a.data_ptr(), lda, stride_a,
         b.data_ptr(), ldb, stride_b,
         c.data_ptr(), ldc, stride_c,
-        batch, int(reduce_batch), int(split_k), int(tile))
+        batch, int(reduce_batch), int(split_k), int(tile), int(gauss3m))
     nbytes = lib.apy_zgemm_workspace(ctypes.byref(d))
     ws = workspace(nbytes, c.device) if nbytes else None
     wsp = ctypes.c_void_p(ws.data_ptr() if ws is not None else 0)

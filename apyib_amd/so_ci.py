@@ -121,9 +121,15 @@ class SOCI:
     fold_fvv: keep a copy of the ring block with the F_vv terms folded in
         (apy_cisd_so_ring_fock, O^2 V^2 elements) so each step runs one GEMM
         fewer; off for instances whose sigma is only formed in batches.
+    gauss3m: run the PPL and ring products in the three-product complex
+        form (3 real products instead of 4, 3/4 of their matrix-core work).
+        Its rounding error in the imaginary part scales with |re| + |im|
+        rather than |im|, so it is for operands whose imaginary parts are
+        not small against their real parts; real-orbital molecules in a
+        weak field keep the four-product form (the default).
     """
 
-    def __init__(self, ints, fock, eps, singles=True, fov_coupling=False, fold_fvv=True):
+    def __init__(self, ints, fock, eps, singles=True, fov_coupling=False, fold_fvv=True, gauss3m=False):
         self.lib = _lib.load()
         self.ints = ints
         self.O, self.V = ints.O, ints.V
@@ -131,6 +137,7 @@ class SOCI:
         self.device = ints.device
         self.singles = bool(singles)
         self.fov = bool(fov_coupling)
+        self.gauss3m = bool(gauss3m)
         ensure_default_plans(self.device)
         self.fock = fock.to(self.device, CDT).contiguous()
         self.eps = torch.as_tensor(eps, dtype=torch.float64).to(self.device).contiguous()
@@ -177,7 +184,7 @@ class SOCI:
             self.hoo.data_ptr(), int(self.singles), int(mode), int(self.fov),
             (energy if energy is not None else self.energy_dev).data_ptr(), c0.real, c0.imag,
             t.data_ptr(), t_new.data_ptr() if t_new is not None else None, out.data_ptr(), ev[0], ev[1], c0_dev,
-            self.ring_f.data_ptr() if self.ring_f is not None else None, int(bool(packed)))
+            self.ring_f.data_ptr() if self.ring_f is not None else None, int(bool(packed)), int(self.gauss3m))
         if noalloc:
             _lib.check(self.lib.apy_cisd_so_step_noalloc(ctypes.byref(args), stream_of(self.device)),
                        "apy_cisd_so_step_noalloc")
@@ -194,7 +201,7 @@ class SOCI:
             self.O, self.V, self.fock.data_ptr(), self.eps.data_ptr(), ctypes.pointer(self.ints.struct),
             self.hoo.data_ptr(), int(self.singles), _lib.APY_MODE_SIGMA, int(self.fov),
             self.energy_dev.data_ptr(), 1.0, 0.0, t.data_ptr(), None, t.data_ptr(), None, None, None,
-            self.ring_f.data_ptr() if self.ring_f is not None else None, 0)
+            self.ring_f.data_ptr() if self.ring_f is not None else None, 0, int(self.gauss3m))
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_so_tune(ctypes.byref(args), _vp(ws), self.ws_bytes, int(reps),
                                              stream_of(self.device)), "apy_cisd_so_tune")
@@ -463,11 +470,13 @@ class SOCI:
 
 
 def step_flops(O, V, singles=True, fold_fvv=True):
-    """Real flops executed by the GEMMs of one apy_cisd_so_step (8 per
-    complex multiply-add), per GEMM; 'ppl' is the dominant kernel.  With
-    the F_vv terms folded into the ring block (fold_fvv) they cost no
-    separate product.  The T1 -> T2 term through <kb||ij> (2 O flops per
-    packed doubles element) is computed in the epilogue and not counted."""
+    """Real flops of the GEMMs of one apy_cisd_so_step (8 per complex
+    multiply-add), per GEMM; 'ppl' is the dominant kernel.  With the F_vv
+    terms folded into the ring block (fold_fvv) they cost no separate
+    product.  The T1 -> T2 term through <kb||ij> (2 O flops per packed
+    doubles element) is computed in the epilogue and not counted.  In the
+    three-product form (SOCI gauss3m) 'ppl' and 'ring' execute 6 flops per
+    complex multiply-add: see executed_flops."""
     Po, Pv, OV = npair(O), npair(V), O * V
     f = {
         'ppl': 8.0 * Po * Pv * Pv,
@@ -487,6 +496,16 @@ def step_flops(O, V, singles=True, fold_fvv=True):
             's1_ooov': 8.0 * O * O * O * V * V,
         })
     return f
+
+
+def executed_flops(f, gauss3m=False):
+    """Matrix-core flops one step executes for the per-GEMM counts f of
+    step_flops: the PPL and ring products run 3 real products instead of 4
+    in the three-product form."""
+    tot = sum(f.values())
+    if gauss3m:
+        tot -= 0.25 * (f['ppl'] + f['ring'])
+    return tot
 
 
 def dense_flops(O, V):

@@ -71,6 +71,12 @@ typedef struct apy_gemm_desc {
   int reduce_batch;   /* 0 or 1 */
   int split_k;        /* 0 = choose automatically, else number of K slices */
   int tile;           /* 0 = choose automatically, else tile config 1..9 */
+  int gauss3m;        /* 1: three real products per complex multiply-add
+                         (re = ar br - ai bi, im = (ar + ai)(br + bi) - ar br
+                         - ai bi): 3/4 of the matrix-core work, with the
+                         imaginary part's rounding relative to |a||b| rather
+                         than to its own size (for operands whose imaginary
+                         parts are not small next to their real parts) */
 } apy_gemm_desc;
 
 /* Workspace (bytes) apy_zgemm needs for this descriptor (0 if none); it
@@ -320,6 +326,8 @@ typedef struct apy_cisd_so_args {
   int packed;              /* 0: t, t_new, out are [t1 | t2 dense] (length
                               O*V + O^2*V^2); 1: [t1 | t2 over (i<j, a<b)]
                               (length O*V + Po*Pv), sigma mode only        */
+  int gauss3m;             /* 1: the PPL and ring products in the three-
+                              product complex form (apy_gemm_desc.gauss3m) */
 } apy_cisd_so_args;
 
 /* apy_cisd_so_step without a workspace argument and without any device
@@ -370,6 +378,7 @@ typedef struct apy_cisd_so_sigma_batch_args {
   void* const* out;
   void* ev_begin;
   void* ev_end;
+  int gauss3m;             /* as in apy_cisd_so_args */
 } apy_cisd_so_sigma_batch_args;
 size_t apy_cisd_so_sigma_batch_workspace(int64_t O, int64_t V, int nb, int singles);
 int apy_cisd_so_sigma_batch(const apy_cisd_so_sigma_batch_args* a, void* workspace, size_t ws_bytes, void* stream);

@@ -149,3 +149,31 @@ def test_c1_davidson_root_is_an_eigenpair_of_the_oracle(c1, dev):
     assert abs(rq - E) <= 1e-10 * max(1.0, abs(E))
     res = tuple(s - E * x for s, x in zip(sv, cv))
     assert np.sqrt(abs(odav.dot(res, res))) < 1e-7
+
+
+def test_c1_gauss3m_sigma_and_davidson(c1, dev):
+    """The three-product form of the PPL and ring products (SOCI gauss3m) at
+    O=20, V=160: the Hermitian sigma against the oracle at the same 1e-12
+    relative bound as the four-product form, and the Davidson lowest root
+    within 1e-10 Eh of the four-product solve."""
+    ints, F, eps, W = c1
+    t1, t2 = rand_amplitudes(O, V, 13, scale=0.01)
+    c0 = 0.91 + 0.07j
+    g3 = SOCI(ints, torch.from_numpy(F).to(dev), eps, singles=True, fov_coupling=True, gauss3m=True)
+    x = torch.from_numpy(np.concatenate([t1.ravel(), t2.ravel()])).to(dev)
+    out = torch.zeros_like(x)
+    g3.sigma(x, out, c0=c0)
+    _, s1, s2 = odav.sigma(F, W, c0, t1, t2, O)
+    g = out.cpu().numpy()
+    scale = max(np.abs(s1).max(), np.abs(s2).max())
+    assert np.abs(g[:O * V] - s1.ravel()).max() <= 1e-12 * scale
+    assert np.abs(g[O * V:] - s2.ravel()).max() <= 1e-12 * scale
+    _clear_plans()
+    plans_load(json.load(open(PLANS)))
+    try:
+        E3 = CISDDavidson(g3, max_space=12).solve_device(e_conv=1e-12, r_conv=1e-7, maxit=60, graph=True)
+        herm = SOCI(ints, torch.from_numpy(F).to(dev), eps, singles=True, fov_coupling=True)
+        E4 = CISDDavidson(herm, max_space=12).solve_device(e_conv=1e-12, r_conv=1e-7, maxit=60, graph=True)
+    finally:
+        _clear_plans()
+    assert abs(E3 - E4) <= 1e-10

@@ -235,3 +235,35 @@ def test_packed_sigma_matches_dense_and_oracle(dev, O, V, seed, singles):
     e_pack = complex(solver.energy(p_in, out=torch.zeros(1, dtype=torch.complex128, device=dev), packed=True)
                      .cpu()[0])
     assert abs(e_dense - e_pack) < 1e-14 * max(1.0, abs(e_dense))
+
+
+@pytest.mark.parametrize("O,V,seed", WIDE + [(6, 8, 5)])
+@pytest.mark.parametrize("fold", [True, False])
+def test_gauss3m_sigma_and_residual_match_oracle(dev, O, V, seed, fold):
+    """SOCI(gauss3m=True), the PPL and ring products in the three-product
+    complex form: the packed Hermitian sigma against the oracle's CISD
+    matrix product and the dense residual step against the reference's
+    residual, at the bounds the four-product form meets."""
+    F, A, eps, ints = _system(O, V, seed, dev)
+    t1, t2 = rand_amplitudes(O, V, seed + 4)
+    c0 = 0.8 - 0.2j
+    herm = SOCI(ints, _dev(F, dev), eps, singles=True, fov_coupling=True, fold_fvv=fold, gauss3m=True)
+    p_in = _dev(_packed(t1, t2, O, V), dev)
+    p_out = torch.zeros_like(p_in)
+    herm.sigma(p_in, p_out, c0=c0, packed=True)
+    W = so.blocks_from_eri(A / 2, O)
+    from oracle import davidson
+    _, s1, s2 = davidson.sigma(F, W, c0, t1, t2, O)
+    want = _packed(s1, s2, O, V)
+    scale = max(1.0, np.abs(want).max())
+    assert np.abs(p_out.cpu().numpy() - want).max() < 1e-13 * scale
+    solver = SOCI(ints, _dev(F, dev), eps, singles=True, fold_fvv=fold, gauss3m=True)
+    E = -0.21 + 0.003j
+    t = _dev(np.concatenate([t1.ravel(), t2.ravel()]), dev)
+    tn, r = solver.new_vector(), solver.new_vector()
+    solver.step(t, tn, r, energy=torch.tensor([E], dtype=torch.complex128, device=dev))
+    g1, g2 = (x.cpu().numpy() for x in solver.split(r))
+    r1, r2 = so.cisd_so_residual(F, A / 2, t1, t2, E, O)
+    scale = max(1.0, np.abs(r1).max(), np.abs(r2).max())
+    assert np.abs(g1 - r1).max() < 1e-13 * scale
+    assert np.abs(g2 - r2).max() < 1e-13 * scale
