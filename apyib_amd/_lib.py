@@ -99,7 +99,7 @@ class CisdSpArgs(ctypes.Structure):
         ("o", c_i64), ("v", c_i64), ("fock", c_vp), ("eps", c_vp), ("ints", ctypes.POINTER(SpInts)),
         ("hoo", c_vp), ("g", c_vp), ("singles", c_int), ("energy", c_vp), ("t", c_vp), ("t_new", c_vp),
         ("out", c_vp), ("resp_b", c_vp), ("resp_t", c_vp), ("resp_energy", c_vp),
-        ("resp_shift_re", c_dbl), ("resp_shift_im", c_dbl), ("w21_f", c_vp),
+        ("resp_shift_re", c_dbl), ("resp_shift_im", c_dbl), ("w21_f", c_vp), ("gauss3m", c_int),
     ]
 
 APY_NOALLOC_SCRATCH_BYTES = 256 << 20

@@ -619,6 +619,8 @@ int sp_gemms(int64_t o, int64_t v, int singles, int64_t nb, int64_t L, const apy
   double2* acc = P(w->acc);
   int q = 0;
   const int64_t so = nsym(o), ao = nanti(o), sv = nsym(v), av = nanti(v);
+  // the three-product complex form (apy_cisd_sp_args.gauss3m) for the ladder and ring products
+  const int g3 = A && A->gauss3m ? 1 : 0;
   if (S) {  // ladder (P+, P-) and ring (X1, X2) per instance
     g[q++] = gd(so, sv, sv, APY_MK, P(w->Ts), sv, APY_NK, I ? I->vvvs : nullptr, sv, P(w->Ps), sv, 1.0, 0.0, nb,
                 so * sv, st.blk[S_VVVS], so * sv);
@@ -630,6 +632,7 @@ int sp_gemms(int64_t o, int64_t v, int singles, int64_t nb, int64_t L, const apy
     if (ao > 0 && av > 0)
       g[q++] = gd(nb * ao, av, av, APY_MK, P(w->Ta), av, APY_NK, I ? I->vvva : nullptr, av, P(w->Pa), av, 1.0, 0.0);  // P-
   }
+  for (int e = 0; e < q; ++e) g[e].gauss3m = g3;
   g[q++] = gd(oo, vv, oo, APY_MK, A ? A->hoo : nullptr, oo, APY_KN, t2, vv, R, vv, 1.0, 0.0, nb, st.hoo, L, d2);  // HHL
   // F_vv t2 (Y), unless folded into the ring operator (w21_f = w21 + delta_kj F_bc, apy_cisd_sp_fold_fock)
   const bool fold = A && A->w21_f;
@@ -646,6 +649,7 @@ int sp_gemms(int64_t o, int64_t v, int singles, int64_t nb, int64_t L, const apy
     g[q++] = gd(nb * ov, ov, ov, APY_MK, TU ? TU + ov : nullptr, 2 * ov, APY_KN, I ? I->w3 : nullptr, ov, P(w->X2),
                 ov, 1.0, 0.0);
   }
+  g[q - 1].gauss3m = g[q - 2].gauss3m = g3;
   // shared integrals: the t1 of the nb vectors stacked (sp_t1_stack) into one Q1 product of nb*o rows, and
   // the w4 product taken from the T half of TU, rows (q,i), with w4t [b][a][(j,c)] as a sum over b
   const bool stack = !S && nb > 1;

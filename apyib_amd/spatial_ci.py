@@ -57,9 +57,13 @@ class SpatialCI(SOCI):
     """One spin-adapted CISD (singles=True) or CID problem on one device.
 
     ints: SPIntegrals; fock: [n,n] MO Fock matrix (device); eps: [n] orbital
-    energies of the denominators (ci_wfn.py:41-43)."""
+    energies of the denominators (ci_wfn.py:41-43).  gauss3m: the ladder and
+    ring products in the three-product complex form (3/4 of their
+    matrix-core work; for integrals and amplitudes whose imaginary parts are
+    not small against their real parts, or one of the two purely real or
+    purely imaginary, where the form is as accurate as four products)."""
 
-    def __init__(self, ints, fock, eps, singles=True):
+    def __init__(self, ints, fock, eps, singles=True, gauss3m=False):
         self.lib = _lib.load()
         self.ints = ints
         self.O, self.V = ints.o, ints.v
@@ -67,6 +71,7 @@ class SpatialCI(SOCI):
         self.device = ints.device
         self.singles = bool(singles)
         self.fov = False
+        self.gauss3m = bool(gauss3m)
         ensure_default_plans(self.device)
         self.fock = fock.to(self.device, CDT).contiguous()
         self.eps = torch.as_tensor(np.array(eps, dtype=np.float64) if not isinstance(eps, torch.Tensor) else eps,
@@ -97,6 +102,7 @@ class SpatialCI(SOCI):
                                int(self.singles), (energy if energy is not None else self.energy_dev).data_ptr(),
                                t.data_ptr(), t_new.data_ptr(), out.data_ptr())
         args.w21_f = self.w21_f.data_ptr()
+        args.gauss3m = int(self.gauss3m)
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_sp_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
                    "apy_cisd_sp_step")
@@ -122,6 +128,7 @@ class SpatialCI(SOCI):
                                ctypes.pointer(self.ints.struct), self.hoo.data_ptr(), self.g.data_ptr(),
                                int(self.singles), e.data_ptr(), t.data_ptr(), t.data_ptr(), t.data_ptr())
         args.w21_f = self.w21_f.data_ptr()
+        args.gauss3m = int(self.gauss3m)
         nbytes = int(self.lib.apy_cisd_sp_batch_workspace(self.O, self.V, int(self.singles), nb))
         ws = workspace(nbytes, self.device)
         _lib.check(self.lib.apy_cisd_sp_tune(ctypes.byref(args), nb, 0, _vp(ws), nbytes, int(reps),
@@ -149,8 +156,8 @@ class ResponseCI(SpatialCI):
     converged amplitude vector on the device; dints: SPIntegrals of the
     perturbed <pq|rs>; dfock: [n, n] derivative Fock matrix (device)."""
 
-    def __init__(self, ints, fock, eps, singles, t, E, dints, dfock):
-        super().__init__(ints, fock, eps, singles)
+    def __init__(self, ints, fock, eps, singles, t, E, dints, dfock, gauss3m=False):
+        super().__init__(ints, fock, eps, singles, gauss3m=gauss3m)
         self.t_ref = t.contiguous()
         self.E_dev = torch.full((1,), complex(E), dtype=CDT, device=self.device)
         pert = SpatialCI(dints, dfock, self.eps, singles)
@@ -172,6 +179,7 @@ class ResponseCI(SpatialCI):
                                (energy if energy is not None else self.energy_dev).data_ptr(),
                                self.shift.real, self.shift.imag)
         args.w21_f = self.w21_f.data_ptr()
+        args.gauss3m = int(self.gauss3m)
         ws = workspace(self.ws_bytes, self.device)
         _lib.check(self.lib.apy_cisd_sp_step(ctypes.byref(args), _vp(ws), self.ws_bytes, stream_of(self.device)),
                    "apy_cisd_sp_step")
@@ -201,10 +209,13 @@ class ResponseSet:
 
     ints, fock, eps, singles, t, E: the unperturbed problem and solution;
     add(dints, dfock) registers one perturbation (its residual of t is
-    formed at once, so dints can be released afterwards)."""
+    formed at once, so dints can be released afterwards).  gauss3m: as for
+    SpatialCI, in the batched steps (each perturbation's right-hand side is
+    formed with four products)."""
 
-    def __init__(self, ints, fock, eps, singles, t, E):
+    def __init__(self, ints, fock, eps, singles, t, E, gauss3m=False):
         self.base = SpatialCI(ints, fock, eps, singles)
+        self.gauss3m = bool(gauss3m)
         self.lib = self.base.lib
         self.singles = bool(singles)
         self.device = self.base.device
@@ -234,6 +245,7 @@ class ResponseSet:
                                int(self.singles), self.E_arr.data_ptr(), x.data_ptr(), x_new.data_ptr(),
                                out.data_ptr(), self.B.data_ptr(), self.t_ref.data_ptr(), s.data_ptr(), 0.0, 0.0)
         args.w21_f = base.w21_f.data_ptr()
+        args.gauss3m = int(self.gauss3m)
         nbytes = int(self.lib.apy_cisd_sp_batch_workspace(self.O, self.V, int(self.singles), nb))
         ws = workspace(nbytes, self.device)
         _lib.check(self.lib.apy_cisd_sp_step_batch(ctypes.byref(args), nb, _vp(self.shift_dev), _vp(ws), nbytes,

@@ -561,6 +561,14 @@ def spatial_step_flops(o, v):
                                    + 2 * ov * ov + o * v * ov * v + o * v * oo * v + o * v * o + o * v * v)
 
 
+def spatial_step_flops_gauss3m(o, v):
+    """Matrix-core flops one spin-adapted step issues with the ladder and ring
+    products in the three-product form (6 instead of 8 per complex
+    multiply-add in those products)."""
+    ladder, total = spatial_step_flops(o, v)
+    return total - 0.25 * (ladder + 8.0 * 3 * (o * v) ** 3)
+
+
 def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10, tune=False):
     """Spin-adapted closed-shell CISD residual steps (apy_cisd_sp_step) at
     o=10, v=80 spatial orbitals (the closed-shell counterpart of the
@@ -591,12 +599,32 @@ def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10, tune=False):
         ci.step(t, tn, r)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / steps * 1e3
+    # the same steps with the ladder and ring products in the three-product form
+    ci3 = SpatialCI(ci.ints, F, eps, gauss3m=True)
+    tn3, r3 = ci3.new_vector(), ci3.new_vector()
+    ci.step(t, tn, r)
+    ci3.step(t, tn3, r3, energy=ci.energy_dev)     # the same E in both residuals
+    diff3 = float((r3 - r).abs().max().cpu()) / max(1e-300, float(r.abs().max().cpu()))
+    for _ in range(2):
+        ci3.step(t, tn3, r3, energy=ci.energy_dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ci3.step(t, tn3, r3, energy=ci.energy_dev)
+    torch.cuda.synchronize()
+    ms3 = (time.perf_counter() - t0) / steps * 1e3
+    del ci3
     ov, oo, vv = o * v, o * o, v * v
     ladder, flops = spatial_step_flops(o, v)
     res = {"workload": f"spin-adapted CISD residual step (ci_wfn.py:457-486), o={o} v={v} spatial orbitals",
            "ms_per_step": round(ms, 3), "executed_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
            "ladder_flops": ladder, "step_flops": flops,
-           "plain_ladder_flops": 8.0 * oo * vv * vv}
+           "plain_ladder_flops": 8.0 * oo * vv * vv,
+           "gauss3m": {"what": "the same step with the ladder and ring products in the three-product complex form",
+                       "ms_per_step": round(ms3, 3),
+                       "effective_tflops": round(flops / (ms3 * 1e-3) / 1e12, 2),
+                       "executed_tflops": round(spatial_step_flops_gauss3m(o, v) / (ms3 * 1e-3) / 1e12, 2),
+                       "max_rel_diff_residual_vs_four_product": diff3}}
     log(f"spatial cisd: {res}")
     return res
 
@@ -669,7 +697,16 @@ def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=
     for _ in range(2):
         rs._step(X, Xn, R, sv)
     _, t_steps = sync_time(lambda: [rs._step(X, Xn, R, sv) for _ in range(steps)])
-    del rs, X, Xn, R
+    # the same batched step with the ladder and ring products in the three-product form
+    rs._step(X, Xn, R, sv)
+    R4 = R.clone()
+    rs.gauss3m = True
+    rs._step(X, Xn, R, sv)
+    diff3 = float((R - R4).abs().max().cpu()) / max(1e-300, float(R4.abs().max().cpu()))
+    for _ in range(2):
+        rs._step(X, Xn, R, sv)
+    _, t_steps3 = sync_time(lambda: [rs._step(X, Xn, R, sv) for _ in range(steps)])
+    del rs, X, Xn, R, R4
 
     def one_by_one():
         out = []
@@ -701,6 +738,12 @@ def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=
            "set_iterations_max": int(max(its)),
            "response_step_ms": round(t_steps / steps * 1e3, 3),
            "response_step_executed_tflops": round(nb * spatial_step_flops(o, v)[1] / (t_steps / steps) / 1e12, 2),
+           "response_step_gauss3m": {
+               "what": "the same batched step with the ladder and ring products in the three-product complex form",
+               "ms": round(t_steps3 / steps * 1e3, 3),
+               "effective_tflops": round(nb * spatial_step_flops(o, v)[1] / (t_steps3 / steps) / 1e12, 2),
+               "executed_tflops": round(nb * spatial_step_flops_gauss3m(o, v) / (t_steps3 / steps) / 1e12, 2),
+               "max_rel_diff_residual_vs_four_product": diff3},
            "one_by_one_ms": round(t_seq * 1e3, 1),
            "central_differences_ms": round(fd_scaled * 1e3, 1),
            "central_differences_measured": f"{nfd} perturbations, scaled to {nb}",

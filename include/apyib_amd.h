@@ -215,6 +215,8 @@ typedef struct apy_cisd_sp_args {
    * of this fock and ints->w21; for apy_cisd_sp_step_multi one per instance,
    * 2 (ov)^2 apart): the step then runs one GEMM fewer.  NULL: off. */
   const void* w21_f;
+  int gauss3m;              /* 1: the ladder and ring products in the three-
+                               product complex form (apy_gemm_desc.gauss3m) */
 } apy_cisd_sp_args;
 
 /* Spin-orbital amplitudes of nb closed-shell (spin-adapted) solutions t [nb][o v + o^2 v^2] in the

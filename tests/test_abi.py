@@ -76,8 +76,9 @@ def test_spatial_sizes_and_struct_layouts():
                            v * o ** 3, ov * ov, v * o * v * v, o ** 3 * v, v * v * o * v]
     assert lib.apy_sp_ints_sizes(0, v, sizes) == _lib.APY_E_ARG
     assert ctypes.sizeof(_lib.SpInts) == 8 * 2 + 8 * 13
-    # o, v, 5 pointers, int (padded), 4 pointers, 3 response pointers, 2 doubles, folded ring operator
-    assert ctypes.sizeof(_lib.CisdSpArgs) == 8 * 2 + 8 * 5 + 8 + 8 * 4 + 8 * 3 + 8 * 2 + 8
+    # o, v, 5 pointers, int (padded), 4 pointers, 3 response pointers, 2 doubles, folded ring operator,
+    # three-product flag (padded)
+    assert ctypes.sizeof(_lib.CisdSpArgs) == 8 * 2 + 8 * 5 + 8 + 8 * 4 + 8 * 3 + 8 * 2 + 8 + 8
 
 
 def test_argument_errors_return_before_any_device_work():

@@ -230,3 +230,31 @@ def test_segmented_reductions_are_bitwise_the_single_calls(dev):
         y1 = torch.zeros(seglen, dtype=torch.complex128, device=dev)
         assert lib.apy_zcombine(k, a1, c1.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), seglen, P(y1), st) == 0
         assert torch.equal(y1.cpu(), comb[sl])
+
+
+@pytest.mark.parametrize("singles", [True, False])
+def test_gauss3m_response_set_matches_four_product_set(dev, singles):
+    """ResponseSet(gauss3m=True) against the four-product set: same
+    projected derivatives (1e-10 relative) and amplitudes (1e-9)."""
+    from apyib_amd.spatial_ci import ResponseSet
+    o, v, seed = 5, 14, 6
+    E4, F, eps = _system(o, v, seed)
+    params = dict(PARAMS)
+    ints = SPIntegrals.from_mo(_d(E4, dev), o, singles=singles)
+    ci = SpatialCI(ints, _d(F, dev), eps, singles=singles)
+    E, _, _, _ = ci.solve(params)
+    t_ref = ci.t.clone()
+    perts = [_perturbation(o, v, seed + 30 + k, scale=0.02) for k in range(4)]
+    out = []
+    for g3 in (False, True):
+        rs = ResponseSet(ints, _d(F, dev), eps, singles, t_ref, E, gauss3m=g3)
+        for dE4, dF in perts:
+            rs.add(SPIntegrals.from_mo(_d(dE4, dev), o, singles=singles), _d(dF, dev))
+        dEs, g1, g2, its = rs.solve(params)
+        out.append((list(dEs), [x.clone() for x in g1] if singles else None, [x.clone() for x in g2]))
+    (d4, a1, a2), (d3, b1, b2) = out
+    for k in range(len(perts)):
+        assert abs(d3[k] - d4[k]) < 1e-10 * max(1.0, abs(d4[k]))
+        assert (b2[k] - a2[k]).abs().max().item() < 1e-9
+        if singles:
+            assert (b1[k] - a1[k]).abs().max().item() < 1e-9

@@ -123,3 +123,34 @@ def test_folded_ring_operator_matches_separate_fvv_product(dev, singles):
     scale = float(torch.abs(r).max().cpu())
     assert float(torch.abs(r_f - r).max().cpu()) < 1e-13 * max(1.0, scale)
     assert float(torch.abs(tn_f - tn).max().cpu()) < 1e-12
+
+
+@pytest.mark.parametrize("o,v,seed", [(1, 4, 8), (4, 7, 2), (6, 17, 3), (10, 40, 4)])
+@pytest.mark.parametrize("singles", [True, False])
+def test_gauss3m_residual_step_matches_oracle(dev, o, v, seed, singles):
+    """SpatialCI(gauss3m=True), the ladder and ring products in the
+    three-product complex form: one residual step against the oracle at the
+    bound the four-product form meets, and the Jacobi + DIIS solve to the
+    same energy."""
+    E4, F, eps = _system(o, v, seed)
+    eri = E4.swapaxes(1, 2)
+    t1, t2 = _amps(o, v, seed + 10)
+    if not singles:
+        t1 = np.zeros_like(t1)
+    Ec = 0.02 - 0.003j
+    ints = SPIntegrals.from_mo(_d(E4, dev), o, singles=singles)
+    ci = SpatialCI(ints, _d(F, dev), eps, singles=singles, gauss3m=True)
+    t = _d(np.concatenate([t1.ravel(), t2.ravel()]), dev)
+    tn, r = ci.new_vector(), ci.new_vector()
+    ci.step(t, tn, r, energy=torch.tensor([Ec], dtype=torch.complex128, device=dev))
+    g1, g2 = (x.cpu().numpy() for x in ci.split(r))
+    if singles:
+        r1, r2 = spatial.cisd_residual(F, eri, t1, t2, Ec, o)
+        assert np.abs(g1 - r1).max() < 1e-13 * max(1.0, np.abs(r1).max())
+    else:
+        r2 = spatial.cid_residual(F, eri, t2, Ec, o)
+    assert np.abs(g2 - r2).max() < 1e-13 * max(1.0, np.abs(r2).max())
+    if o * v <= 60:
+        E3 = ci.solve(dict(PARAMS))[0]
+        E4s = SpatialCI(ints, _d(F, dev), eps, singles=singles).solve(dict(PARAMS))[0]
+        assert abs(E3 - E4s) < 1e-11
