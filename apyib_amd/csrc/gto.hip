@@ -215,28 +215,74 @@ __device__ bool quartet_ok(const apy_gto& A, const apy_gto& B, const apy_gto& C,
   return true;
 }
 
-// One wavefront per output element (wave-uniform control flow): the lanes split the element's
-// (term, primitive) quartets and a fixed-order butterfly sums them.  A table entry outside the limits
-// (angular momentum, primitive count, term count, atom index) gives NaN before any indexing.
+// x = hi (hi + 1) / 2 + lo with 0 <= lo <= hi: the row-packed lower triangle.
+__device__ inline void tri_split(int64_t x, int64_t& hi, int64_t& lo) {
+  int64_t h = (int64_t)((sqrt(8.0 * (double)x + 1.0) - 1.0) * 0.5);
+  while (h * (h + 1) / 2 > x) --h;
+  while ((h + 1) * (h + 2) / 2 <= x) ++h;
+  hi = h;
+  lo = x - h * (h + 1) / 2;
+}
+
+// Which permutational symmetry of the real integrals (ab|cd) the launch uses: ERI_ALL when the four
+// tables are one table (8 images per computed quartet: i >= j, k >= l, (ij) >= (kl)), ERI_KET when the
+// ket tables are one table ((ab|cd) = (ab|dc): 2 images, k >= l), ERI_NONE otherwise.
+enum EriSym { ERI_NONE = 0, ERI_KET = 1, ERI_ALL = 2 };
+
+// One wavefront per computed element (wave-uniform control flow): the lanes split the element's
+// (term, primitive) quartets and a fixed-order butterfly sums them; lane 0 writes the value to every
+// image of the element under the launch's symmetry.  A table entry outside the limits (angular
+// momentum, primitive count, term count, atom index) gives NaN before any indexing.
 __global__ __launch_bounds__(256) void eri_kernel(int64_t ngeom, int64_t natom, const double* __restrict__ centers,
                                                   const apy_gto* __restrict__ fa, int64_t na,
                                                   const apy_gto* __restrict__ fb, int64_t nb,
                                                   const apy_gto* __restrict__ fc, int64_t nc,
-                                                  const apy_gto* __restrict__ fd, int64_t nd,
+                                                  const apy_gto* __restrict__ fd, int64_t nd, int sym,
                                                   double2* __restrict__ out) {
   const int lane = threadIdx.x & 63;
-  const int64_t per = na * nb * nc * nd, total = ngeom * per;
+  const int64_t full = na * nb * nc * nd;
+  const int64_t pc = nc * (nc + 1) / 2;
+  const int64_t per = sym == ERI_ALL ? pc * (pc + 1) / 2 : sym == ERI_KET ? na * nb * pc : full;
+  const int64_t total = ngeom * per;
   const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
   for (int64_t e = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; e < total; e += nwave) {
     const int64_t g = e / per, r = e % per;
-    const int64_t l = r % nd, k = (r / nd) % nc, j = (r / (nd * nc)) % nb, i = r / (nd * nc * nb);
+    int64_t i, j, k, l;
+    if (sym == ERI_ALL) {
+      int64_t bra, ket;
+      tri_split(r, bra, ket);
+      tri_split(bra, i, j);
+      tri_split(ket, k, l);
+    } else if (sym == ERI_KET) {
+      tri_split(r % pc, k, l);
+      j = (r / pc) % nb;
+      i = r / (pc * nb);
+    } else {
+      l = r % nd; k = (r / nd) % nc; j = (r / (nd * nc)) % nb; i = r / (nd * nc * nb);
+    }
+    double2* o = out + g * full;
+    // element (p, q, s, t) of the [na][nb][nc][nd] output
+    auto at = [&](int64_t p, int64_t q, int64_t s2, int64_t t) { return ((p * nb + q) * nc + s2) * nd + t; };
+    auto put = [&](double v) {
+      const double2 w = make_double2(v, 0.0);
+      o[at(i, j, k, l)] = w;
+      if (sym == ERI_NONE) return;
+      o[at(i, j, l, k)] = w;
+      if (sym == ERI_KET) return;
+      o[at(j, i, k, l)] = w;
+      o[at(j, i, l, k)] = w;
+      o[at(k, l, i, j)] = w;
+      o[at(l, k, i, j)] = w;
+      o[at(k, l, j, i)] = w;
+      o[at(l, k, j, i)] = w;
+    };
     const apy_gto& A = fa[i];
     const apy_gto& B = fb[j];
     const apy_gto& C = fc[k];
     const apy_gto& D = fd[l];
     if (!gto_ok(A, natom) || !gto_ok(B, natom) || !gto_ok(C, natom) || !gto_ok(D, natom) ||
         !quartet_ok(A, B, C, D)) {
-      if (lane == 0) out[e] = make_double2(kNaN, 0.0);
+      if (lane == 0) put(kNaN);
       continue;
     }
     const double* cen = centers + 3 * natom * g;
@@ -263,7 +309,7 @@ __global__ __launch_bounds__(256) void eri_kernel(int64_t ngeom, int64_t natom, 
       acc += eri_prim(A, B, C, D, ta, tb, tc, td, pa, pb, pc, pd, ca, cb, cc, cd, qab, qcd);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) out[e] = make_double2(acc, 0.0);
+    if (lane == 0) put(acc);
   }
 }
 
@@ -436,10 +482,16 @@ int apy_eri_gto(int64_t ngeom, int64_t natom, const double* centers, const apy_g
   if (ngeom < 1 || natom < 1 || !centers || !fa || !fb || !fc || !fd || !out || na < 1 || nb < 1 || nc < 1 ||
       nd < 1)
     return APY_E_ARG;
-  const int64_t total = ngeom * na * nb * nc * nd;
+  // the same table in every slot: 8-fold permutational symmetry; the same ket tables: (ab|cd) = (ab|dc)
+  const int sym = (fa == fb && fb == fc && fc == fd && na == nb && nb == nc && nc == nd) ? ERI_ALL
+                  : (fc == fd && nc == nd)                                             ? ERI_KET
+                                                                                       : ERI_NONE;
+  const int64_t pc = nc * (nc + 1) / 2;
+  const int64_t per = sym == ERI_ALL ? pc * (pc + 1) / 2 : sym == ERI_KET ? na * nb * pc : na * nb * nc * nd;
+  const int64_t total = ngeom * per;
   if (const int rc = ensure_boys_table((hipStream_t)stream); rc != APY_OK) return rc;
   eri_kernel<<<apy_grid1d(64 * total), 256, 0, (hipStream_t)stream>>>(ngeom, natom, centers, fa, na, fb, nb, fc, nc, fd,
-                                                                  nd, (double2*)out);
+                                                                  nd, sym, (double2*)out);
   APY_LAUNCH_CHECK();
   return APY_OK;
 }

@@ -212,3 +212,29 @@ def test_d_functions_match_host(dev):
             j, k, l = (int(q) for q in rng.integers(0, n, 3))
             want = sum(gaussian.eri(t, fns[j], fns[k], fns[l]) for t in gaussian.center_derivative(fns[i], x))
             assert abs(Dx[i, j, k, l] - want) < 1e-12 * max(1.0, abs(want)), (x, i, j, k, l)
+
+
+@pytest.mark.parametrize("geom,basis", [(H2O, "STO-3G"), (H2O, "6-31G*"), (H2O2, "STO-3G")],
+                         ids=["H2O-STO-3G", "H2O-6-31G*", "H2O2-STO-3G"])
+def test_eri_symmetric_launches_match_general_launch(dev, geom, basis):
+    """apy_eri_gto computes one quartet per permutational image class when
+    the four tables are one table (8 images) or the two ket tables are one
+    table (2 images); the same tables passed as distinct copies take the
+    general path, one quartet per element.  The three launches agree to
+    rounding (each image keeps the summation order of the quartet that
+    was computed), and the 8-fold result is exactly symmetric."""
+    mol = Molecule.from_string(geom)
+    bs = BasisSet(mol, basis)
+    n = len(bs.functions)
+    tab = basis_table(bs.functions, bs.atom_of, dev)
+    copies = [tab.clone() for _ in range(3)]
+    cnt = (n, n, n, n)
+    G8 = eri(tab, tab, tab, tab, cnt, mol.coords[None], dev).cpu().numpy()[0].real
+    G2 = eri(copies[0], copies[1], tab, tab, cnt, mol.coords[None], dev).cpu().numpy()[0].real
+    G1 = eri(copies[0], copies[1], copies[2], tab, cnt, mol.coords[None], dev).cpu().numpy()[0].real
+    scale = np.abs(G1).max()
+    assert np.abs(G8 - G1).max() <= 1e-14 * scale
+    assert np.abs(G2 - G1).max() <= 1e-14 * scale
+    for perm in ((1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1), (3, 2, 1, 0)):
+        assert np.array_equal(G8, G8.transpose(perm))
+    assert np.array_equal(G2, G2.transpose(0, 1, 3, 2))
