@@ -161,6 +161,13 @@ class Mp2NcArgs(ctypes.Structure):
     ]
 
 
+class Mp2AdjointItem(ctypes.Structure):
+    _fields_ = [
+        ("O", c_i64), ("V", c_i64), ("t", c_vp), ("lam", c_vp), ("grad_f", c_vp), ("ndir", c_i64),
+        ("dF", ctypes.POINTER(c_vp)), ("dW", ctypes.POINTER(c_vp)), ("dV", ctypes.POINTER(c_vp)),
+    ]
+
+
 class Mp2NcInstance(ctypes.Structure):
     _fields_ = [
         ("O", c_i64), ("V", c_i64),
@@ -258,6 +265,8 @@ SIGNATURES = {
     "apy_mp2_nc_step": (c_int, [ctypes.POINTER(Mp2NcArgs), c_vp, c_size, c_vp]),
     "apy_mp2_nc_density_workspace": (c_size, [c_i64, c_i64]),
     "apy_mp2_nc_densities": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    "apy_mp2_adjoint_sums_workspace": (c_size, [c_i64, ctypes.POINTER(Mp2AdjointItem)]),
+    "apy_mp2_adjoint_sums": (c_int, [c_i64, ctypes.POINTER(Mp2AdjointItem), c_vp, c_vp, c_size, c_vp]),
     "apy_eri_mo_workspace": (c_size, [c_i64, c_i64]),
     "apy_eri_mo": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     "apy_fock_build": (c_int, [c_i64, c_vp, c_vp, c_vp, c_dbl, c_vp, c_vp]),

@@ -182,60 +182,90 @@ class MP2NCBatch:
         return self.energy_out
 
 
-def mp2_adjoint_set(bases, directions, tol=1e-12, maxit=200, graph=False):
+class AdjointDensities:
+    """(G, t, lambda) of one base system: G = dE/dF [n*n] and the converged
+    amplitude and lambda vectors, unpacked to dense [O, O, V, V] on first
+    access.  Unpacks like a tuple: G, t, lam = dens."""
+
+    def __init__(self, batch, k_t, k_lam, G):
+        # k_lam None: lambda = conj(t) / 4 (Hermitian system)
+        self._batch, self._kt, self._kl = batch, k_t, k_lam
+        self.G = G
+        self._t = self._lam = None
+
+    @property
+    def t(self):
+        if self._t is None:
+            self._t = self._batch.amplitudes(self._kt).view(-1)
+        return self._t
+
+    @property
+    def lam(self):
+        if self._lam is None:
+            self._lam = (self._batch.amplitudes(self._kl).view(-1) if self._kl is not None
+                         else 0.25 * self.t.conj())
+        return self._lam
+
+    def __iter__(self):
+        return iter((self.G, self.t, self.lam))
+
+    def __getitem__(self, i):
+        return (self.G, self.t, self.lam)[i]
+
+
+def mp2_adjoint_set(bases, directions, tol=1e-12, maxit=200, graph=False, hermitian=False):
     """Directional derivatives of the MP2 energies of several base systems:
     for base b the directions directions[b] = (dF list [n*n], dW list or
     None, dV list or None; flattened device tensors).  The amplitude and the
     lambda equations (F^T, driver <ij||ab>/4) of every base are one batch
-    (one fused launch per iteration for all of them); then per base the
-    gradient densities (apy_mp2_nc_densities) and the directional sums
+    (one fused launch per iteration for all of them); with hermitian=True
+    (every base has F = F^H and <ab||ij> = conj(<ij||ab>), which the caller
+    asserts) the lambda equations are the conjugated amplitude equations
+    scaled by 1/4, so only the amplitudes are solved and lambda =
+    conj(t) / 4.  Then one
+    apy_mp2_adjoint_sums call forms every base's gradient densities and the
+    directional sums
         dE = sum dE/dF . dF + sum t/4 . dW + sum lambda . dV,
-    all read back with one host copy.  Returns ([array of dE per base],
-    [(G, t, lambda) per base])."""
+    read back with one host copy.  Returns ([array of dE per base],
+    [AdjointDensities (G, t, lambda) per base])."""
     lib = _lib.load()
     dev = bases[0].F.device
     nbase = len(bases)
-    lam_inst = [Instance(B.O, B.V, B.F, B.eps, B.w, None, transpose_f=True, drv_scale=0.25) for B in bases]
+    lam_inst = [] if hermitian else [Instance(B.O, B.V, B.F, B.eps, B.w, None, transpose_f=True, drv_scale=0.25)
+                                     for B in bases]
     batch = MP2NCBatch(list(bases) + lam_inst)
     batch.solve(tol, maxit, chunk=8, graph=graph)
-    out, dens = [], []
-    jobs = []
+    items = (_lib.Mp2AdjointItem * nbase)()
+    keep, counts = [], []
+    ncum = [0]
+    for B in bases:
+        ncum.append(ncum[-1] + (B.O + B.V) ** 2)
+    Gbuf = torch.empty(ncum[-1], dtype=CDT, device=dev)
+    dens = []
     for b, B in enumerate(bases):
-        O, V, n = B.O, B.V, B.O + B.V
-        t, lam = batch.amplitudes(b).view(-1), batch.amplitudes(nbase + b).view(-1)
-        G = torch.empty(n * n, dtype=CDT, device=dev)
-        nb = int(lib.apy_mp2_nc_density_workspace(O, V))
-        ws = workspace(nb, dev)
-        _lib.check(lib.apy_mp2_nc_densities(O, V, ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(lam.data_ptr()),
-                                            ctypes.c_void_p(G.data_ptr()), ctypes.c_void_p(ws.data_ptr()), nb,
-                                            stream_of(dev)), "apy_mp2_nc_densities")
-        dens.append((G, t, lam))
         dF, dW, dV = directions[b]
-        terms = [(xs, y, scale) for xs, y, scale in ((dF, G, 1.0), (dW, t, 0.25), (dV, lam, 1.0)) if xs is not None]
-        jobs.append((len(dF), terms))
-    total = sum(len(terms) * K for K, terms in jobs)
-    part = torch.zeros(max(total, 1), dtype=CDT, device=dev)      # every directional sum, read back once
-    off = 0
-    for K, terms in jobs:
-        for xs, y, _ in terms:
-            for s0 in range(0, K, _lib.APY_MAXVEC):
-                chunk = xs[s0:s0 + _lib.APY_MAXVEC]
-                arr = (ctypes.c_void_p * len(chunk))(*[x.data_ptr() for x in chunk])
-                nb = int(lib.apy_reduce_workspace(y.numel(), len(chunk)))
-                ws = workspace(nb, dev)
-                _lib.check(lib.apy_zdot_many(len(chunk), arr, ctypes.c_void_p(y.data_ptr()), y.numel(), 0,
-                                             ctypes.c_void_p(part[off + s0:].data_ptr()),
-                                             ctypes.c_void_p(ws.data_ptr()), nb, stream_of(dev)), "apy_zdot_many")
-            off += K
-    sums = part.cpu().numpy()
-    off = 0
-    for K, terms in jobs:
-        v = np.zeros(K, dtype=np.complex128)
-        for _, _, scale in terms:
-            v += scale * sums[off:off + K]
-            off += K
-        out.append(v)
-    return out, dens
+        K = len(dF)
+        arrs = [(ctypes.c_void_p * K)(*[x.data_ptr() for x in xs]) if xs is not None else None
+                for xs in (dF, dW, dV)]
+        keep.append(arrs)
+        G = Gbuf[ncum[b]:ncum[b + 1]]
+        as_ptr = lambda a: ctypes.cast(a, ctypes.POINTER(ctypes.c_void_p)) if a is not None else None
+        items[b] = _lib.Mp2AdjointItem(B.O, B.V, batch.x[b].data_ptr(),
+                                       None if hermitian else batch.x[nbase + b].data_ptr(),
+                                       G.data_ptr(), K, *[as_ptr(a) for a in arrs])
+        counts.append(K)
+        dens.append(AdjointDensities(batch, b, None if hermitian else nbase + b, G))
+    nbytes = int(lib.apy_mp2_adjoint_sums_workspace(nbase, items))
+    ws = workspace(nbytes, dev)
+    out = torch.empty(max(1, sum(counts)), dtype=CDT, device=dev)
+    _lib.check(lib.apy_mp2_adjoint_sums(nbase, items, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                                        nbytes, stream_of(dev)), "apy_mp2_adjoint_sums")
+    sums = out.cpu().numpy()
+    res, off = [], 0
+    for K in counts:
+        res.append(sums[off:off + K].copy())
+        off += K
+    return res, dens
 
 
 def mp2_adjoint_directional(base, dF, dW, dV, tol=1e-12, maxit=200):

@@ -170,3 +170,113 @@ int apy_mp2_nc_densities(int64_t O, int64_t V, const void* t, const void* lam, v
 }
 
 }  // extern "C"
+
+namespace {
+
+// lam = conj(t) / 4: the lambda amplitudes of a Hermitian system (F = F^H, <ab||ij> = conj(<ij||ab>))
+__global__ __launch_bounds__(256) void adj_lambda_hermitian(int64_t n, const double2* __restrict__ t,
+                                                            double2* __restrict__ lam) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = t[e];
+    lam[e] = make_double2(0.25 * v.x, -0.25 * v.y);
+  }
+}
+
+__global__ __launch_bounds__(256) void adj_combine_kernel(int64_t total, const double2* __restrict__ d,
+                                                          double2* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const double2 f = d[e], w = d[total + e], v = d[2 * total + e];
+    out[e] = make_double2(f.x + 0.25 * w.x + v.x, f.y + 0.25 * w.y + v.y);
+  }
+}
+
+// Workspace: dense t and lambda of the largest item, the [3][total] directional dots, then the scratch
+// of the density GEMMs / dot reductions.
+struct AdjWS {
+  size_t t, lam, dots, scratch, scratch_bytes, total;
+};
+
+bool adj_carve(int64_t nitem, const apy_mp2_adjoint_item* it, AdjWS& w, int64_t& ndir_total) {
+  auto rnd = [](size_t b) { return (b + 255) / 256 * 256; };
+  size_t dense = 0, scratch = 0;
+  ndir_total = 0;
+  for (int64_t b = 0; b < nitem; ++b) {
+    const int64_t O = it[b].O, V = it[b].V, n = O + V;
+    if (O < 1 || V < 1 || it[b].ndir < 0 || !it[b].t || !it[b].grad_f) return false;
+    if (it[b].ndir > 0 && !it[b].dF) return false;
+    dense = std::max(dense, (size_t)(O * O * V * V) * sizeof(double2));
+    scratch = std::max(scratch, apy_mp2_nc_density_workspace(O, V));
+    scratch = std::max(scratch, apy_reduce_workspace(std::max(n * n, O * O * V * V), APY_MAXVEC));
+    ndir_total += it[b].ndir;
+  }
+  size_t off = 0;
+  w.t = off; off += rnd(dense);
+  w.lam = off; off += rnd(dense);
+  w.dots = off; off += rnd((size_t)std::max<int64_t>(ndir_total, 1) * 3 * sizeof(double2));
+  w.scratch = off; w.scratch_bytes = rnd(scratch); off += w.scratch_bytes;
+  w.total = off;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t apy_mp2_adjoint_sums_workspace(int64_t nitem, const apy_mp2_adjoint_item* items) {
+  AdjWS w{};
+  int64_t nd = 0;
+  if (nitem < 1 || !items || !adj_carve(nitem, items, w, nd)) return 0;
+  return w.total;
+}
+
+int apy_mp2_adjoint_sums(int64_t nitem, const apy_mp2_adjoint_item* items, void* out, void* workspace,
+                         size_t ws_bytes, void* stream) {
+  AdjWS w{};
+  int64_t total = 0;
+  if (nitem < 1 || !items || !out || !adj_carve(nitem, items, w, total)) return APY_E_ARG;
+  if (!workspace || ws_bytes < w.total) return APY_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  double2* T = (double2*)(base + w.t);
+  double2* L = (double2*)(base + w.lam);
+  double2* D = (double2*)(base + w.dots);
+  void* scr = base + w.scratch;
+  if (total > 0) {
+    zero_kernel<<<apy_grid1d(3 * total), 256, 0, st>>>(D, 3 * total);
+    APY_LAUNCH_CHECK();
+  }
+  int64_t off = 0;
+  for (int64_t b = 0; b < nitem; ++b) {
+    const apy_mp2_adjoint_item& I = items[b];
+    const int64_t O = I.O, V = I.V, n = O + V, d2 = O * O * V * V;
+    int s = apy_t2_unpack(O, V, I.t, T, stream);
+    if (s == APY_OK && I.lam) s = apy_t2_unpack(O, V, I.lam, L, stream);
+    if (s == APY_OK && !I.lam) {
+      adj_lambda_hermitian<<<apy_grid1d(d2), 256, 0, st>>>(d2, T, L);
+      APY_LAUNCH_CHECK();
+    }
+    if (s == APY_OK) s = apy_mp2_nc_densities(O, V, T, L, I.grad_f, scr, w.scratch_bytes, stream);
+    if (s != APY_OK) return s;
+    // sum G.dF_q, sum t.dW_q, sum lambda.dV_q into the three rows of the dot block
+    const void* const* xs[3] = {I.dF, I.dW, I.dV};
+    const void* ys[3] = {I.grad_f, T, L};
+    const int64_t lens[3] = {n * n, d2, d2};
+    for (int term = 0; term < 3; ++term) {
+      if (!xs[term]) continue;
+      for (int64_t s0 = 0; s0 < I.ndir; s0 += APY_MAXVEC) {
+        const int k = (int)std::min<int64_t>(APY_MAXVEC, I.ndir - s0);
+        s = apy_zdot_many(k, xs[term] + s0, ys[term], lens[term], 0, D + term * total + off + s0, scr,
+                          w.scratch_bytes, stream);
+        if (s != APY_OK) return s;
+      }
+    }
+    off += I.ndir;
+  }
+  if (total > 0) {
+    adj_combine_kernel<<<apy_grid1d(total), 256, 0, st>>>(total, D, (double2*)out);
+    APY_LAUNCH_CHECK();
+  }
+  return APY_OK;
+}
+
+}  // extern "C"

@@ -210,7 +210,9 @@ def aat_set_measurement(dev, seed, tol=1e-12):
     and warm starts (the undisplaced systems solved first, one batch, then
     every displaced instance from its system's amplitudes).  Adjoint: the
     amplitude and lambda equations of every undisplaced system in one batch,
-    then the densities and all directional sums."""
+    then the densities and all directional sums (apy_mp2_adjoint_sums);
+    the Hermitian leg solves the amplitudes only (lambda = conj(t)/4 for
+    these Hermitian systems)."""
     import numpy as np
     import torch
     from apyib_amd.batched import MP2NCBatch, fd_mp2_instances, mp2_adjoint_set, plain_solve, warm_batch_solve
@@ -273,8 +275,11 @@ def aat_set_measurement(dev, seed, tol=1e-12):
         directions = [([dF[d] for d in g], [dW[d] for d in g], [dV[d] for d in g]) for g in groups]
         mp2_adjoint_set(bases[:1], directions[:1], tol)               # warm
         (an, dens), t_adj = timed(lambda: mp2_adjoint_set(bases, directions, tol))
+        # the base systems are Hermitian (F = F^H, <ab||ij> = conj(<ij||ab>)): lambda = conj(t)/4 without its solve
+        mp2_adjoint_set(bases[:1], directions[:1], tol, hermitian=True)
+        (anh, densh), t_adj_h = timed(lambda: mp2_adjoint_set(bases, directions, tol, hermitian=True))
         fd = ((Ep[0::2] - Ep[1::2]) / (2 * h)).cpu().numpy()
-        errs = []
+        errs, errs_h = [], []
         for b, g in enumerate(groups):
             G, t, lam = dens[b]
             for q, d in enumerate(g):
@@ -282,6 +287,8 @@ def aat_set_measurement(dev, seed, tol=1e-12):
                 scale = float((torch.sum((G * dF[d]).abs()) + 0.25 * torch.sum((t * dW[d]).abs())
                                + torch.sum((lam * dV[d]).abs())).cpu())
                 errs.append(abs(an[b][q] - fd[d]) / scale)
+                errs_h.append(abs(anh[b][q] - fd[d]) / scale)
+        diff_h = max(float(np.abs(anh[b] - an[b]).max()) for b in range(len(groups)))
         dE = lambda E: float(torch.abs(E - Ep).max().cpu())
         entry = {"instances": len(insts), "directions": nd, "systems": len(bases),
                  "sizes": sorted({(I.O, I.V) for I in insts}),
@@ -300,7 +307,11 @@ def aat_set_measurement(dev, seed, tol=1e-12):
                  "max_abs_dE_batched_vs_plain": dE(Eb), "max_abs_dE_graph_vs_plain": dE(Eg),
                  "max_abs_dE_warm_vs_plain": dE(Ew),
                  "adjoint_s": round(t_adj, 5), "speedup_adjoint_vs_plain_fd": round(t_plain / t_adj, 2),
-                 "max_err_adjoint_vs_fd_rel_to_scale": float(max(errs)), "fd_step": h}
+                 "max_err_adjoint_vs_fd_rel_to_scale": float(max(errs)), "fd_step": h,
+                 "adjoint_hermitian_s": round(t_adj_h, 5),
+                 "speedup_adjoint_hermitian_vs_plain_fd": round(t_plain / t_adj_h, 2),
+                 "max_err_adjoint_hermitian_vs_fd_rel_to_scale": float(max(errs_h)),
+                 "max_abs_diff_adjoint_hermitian_vs_solved_lambda": diff_h}
         res[name] = entry
         log(f"aat set {name}: {entry}")
     return res

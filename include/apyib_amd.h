@@ -510,6 +510,35 @@ int apy_mp2_nc_step(const apy_mp2_nc_args* a, void* workspace, size_t ws_bytes, 
 size_t apy_mp2_nc_density_workspace(int64_t O, int64_t V);
 int apy_mp2_nc_densities(int64_t O, int64_t V, const void* t, const void* lam, void* grad_f, void* workspace,
                          size_t ws_bytes, void* stream);
+/* Directional MP2 energy derivatives of several base systems in one call
+ * (the gradient step the reference's finite-difference loop replaces,
+ * fin_diff.py:285-370, in the Lagrangian form of ci_wfn.py:210-212 /
+ * analytic.py:307-338): for item b with converged packed amplitudes t and
+ * lambda (x[(i<j)][(a<b)], apy_t2_packed_length elements each), grad_f
+ * receives dE/dF as apy_mp2_nc_densities gives it, and for each of its ndir
+ * directions q
+ *   out[off_b + q] = sum G dF_q + 1/4 sum t dW_q + sum lambda dV_q
+ * (plain products; dF_q [n][n], dW_q and dV_q [O][O][V][V] dense, t and
+ * lambda taken dense; off_b = ndir of the items before b).  dF, dW, dV are
+ * host arrays of ndir device pointers (dW or dV null: that term is zero).
+ * lam null: the system is Hermitian (F = F^H, <ab||ij> = conj(<ij||ab>)),
+ * whose lambda equations are the conjugated amplitude equations scaled by
+ * 1/4, so lambda = conj(t) / 4 is used without a separate solve (the
+ * reference's MP2 gradient, analytic.py:320-324, likewise uses t alone).
+ * Workspace apy_mp2_adjoint_sums_workspace(nitem, items) bytes. */
+typedef struct apy_mp2_adjoint_item {
+  int64_t O, V;
+  const void* t;
+  const void* lam;
+  void* grad_f;
+  int64_t ndir;
+  const void* const* dF;
+  const void* const* dW;
+  const void* const* dV;
+} apy_mp2_adjoint_item;
+size_t apy_mp2_adjoint_sums_workspace(int64_t nitem, const apy_mp2_adjoint_item* items);
+int apy_mp2_adjoint_sums(int64_t nitem, const apy_mp2_adjoint_item* items, void* out, void* workspace,
+                         size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Spin-adapted (closed-shell) MP2 energy derivatives in the reference's own

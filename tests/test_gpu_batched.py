@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import torch
 
-from apyib_amd import synthetic
+from apyib_amd import _lib, synthetic
 from apyib_amd.adjoint import MP2Adjoint
 from apyib_amd.batched import Instance, MP2NCBatch, fd_mp2_instances, plain_solve
 from oracle import adjoint
@@ -217,3 +217,69 @@ def test_persistent_solve_set_b_sizes(dev):
     assert torch.equal(d.energies(), p.energies())
     for k in range(len(insts)):
         assert torch.equal(d.x[k], p.x[k])
+
+
+def test_adjoint_sums_match_separate_densities_and_dots(dev):
+    """apy_mp2_adjoint_sums (every base's densities and directional sums in
+    one call) against apy_mp2_nc_densities on the unpacked amplitudes and
+    torch sums of the three terms: uneven bases, more than 16 directions
+    on one base (chunked dots), and bases without the dW or dV terms."""
+    from apyib_amd.batched import mp2_adjoint_set
+    sizes = [(4, 9), (3, 11), (5, 7)]
+    bases = [_host_instance(O, V, 70 + k, dev)[0] for k, (O, V) in enumerate(sizes)]
+    g = torch.Generator(device="cpu").manual_seed(5)
+    rnd = lambda n: torch.complex(torch.randn(n, generator=g, dtype=torch.float64),
+                                  torch.randn(n, generator=g, dtype=torch.float64)).to(dev)
+    ndirs = [19, 3, 5]
+    directions = []
+    for b, (B, K) in enumerate(zip(bases, ndirs)):
+        n, d2 = B.O + B.V, B.O * B.O * B.V * B.V
+        dF = [rnd(n * n) for _ in range(K)]
+        dW = [rnd(d2) for _ in range(K)] if b != 1 else None
+        dV = [rnd(d2) for _ in range(K)] if b != 2 else None
+        directions.append((dF, dW, dV))
+    out, dens = mp2_adjoint_set(bases, directions, tol=1e-13)
+    lib = _lib.load()
+    for b, (B, (dF, dW, dV)) in enumerate(zip(bases, directions)):
+        G, t, lam = dens[b]
+        n = B.O + B.V
+        G2 = torch.empty(n * n, dtype=torch.complex128, device=dev)
+        nb = int(lib.apy_mp2_nc_density_workspace(B.O, B.V))
+        ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
+        assert lib.apy_mp2_nc_densities(B.O, B.V, _lib.c_vp(t.data_ptr()), _lib.c_vp(lam.data_ptr()),
+                                        _lib.c_vp(G2.data_ptr()), _lib.c_vp(ws.data_ptr()), nb,
+                                        _lib.c_vp(torch.cuda.current_stream(dev).cuda_stream)) == 0
+        assert torch.equal(G, G2)
+        for q in range(len(dF)):
+            want = torch.sum(G * dF[q])
+            if dW is not None:
+                want = want + 0.25 * torch.sum(t * dW[q])
+            if dV is not None:
+                want = want + torch.sum(lam * dV[q])
+            want = complex(want.cpu())
+            assert abs(out[b][q] - want) <= 1e-12 * max(1.0, abs(want)), (b, q)
+
+
+def test_adjoint_hermitian_lambda_matches_solved_lambda(dev):
+    """mp2_adjoint_set(hermitian=True) (amplitudes only, lambda = conj(t)/4
+    inside apy_mp2_adjoint_sums) against the solved lambda equations on
+    Hermitian synthetic bases: same densities and directional sums to the
+    convergence threshold."""
+    from apyib_amd.batched import mp2_adjoint_set
+    sizes = [(4, 9), (3, 11), (6, 13)]
+    bases = [_host_instance(O, V, 80 + k, dev)[0] for k, (O, V) in enumerate(sizes)]
+    for B in bases:
+        assert torch.allclose(B.drv, B.w.conj(), rtol=0, atol=1e-15)
+        assert torch.allclose(B.F, B.F.conj().T, rtol=0, atol=1e-15)
+    g = torch.Generator(device="cpu").manual_seed(7)
+    rnd = lambda n: torch.complex(torch.randn(n, generator=g, dtype=torch.float64),
+                                  torch.randn(n, generator=g, dtype=torch.float64)).to(dev)
+    directions = [([rnd((B.O + B.V) ** 2) for _ in range(4)], [rnd(B.O ** 2 * B.V ** 2) for _ in range(4)],
+                   [rnd(B.O ** 2 * B.V ** 2) for _ in range(4)]) for B in bases]
+    a, da = mp2_adjoint_set(bases, directions, tol=1e-14)
+    h, dh = mp2_adjoint_set(bases, directions, tol=1e-14, hermitian=True)
+    for b in range(len(bases)):
+        scale = max(1.0, float(np.abs(a[b]).max()))
+        assert np.abs(a[b] - h[b]).max() <= 1e-11 * scale
+        assert float((da[b].G - dh[b].G).abs().max().cpu()) <= 1e-11 * max(1.0, float(da[b].G.abs().max().cpu()))
+        assert float((da[b].lam - dh[b].lam).abs().max().cpu()) <= 1e-12

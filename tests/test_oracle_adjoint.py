@@ -59,3 +59,22 @@ def test_diagonal_fock_reduces_to_canonical_mp2():
     for a in range(V):
         dEdeps[O + a] = x[:, :, a].sum() + x[:, :, :, a].sum()
     assert np.abs(np.diag(GF) - dEdeps).max() < 1e-13
+
+
+def test_lambda_of_hermitian_system_is_conjugated_amplitudes():
+    """For F = F^H and <ab||ij> = conj(<ij||ab>) the lambda equations
+    (F^T, driver <ij||ab>/4) are the complex conjugate of the amplitude
+    equations scaled by 1/4, so lambda = conj(t)/4 (the identity
+    apy_mp2_adjoint_sums uses when it is given no lambda); a non-Hermitian
+    Fock matrix breaks it."""
+    O, V = 4, 7
+    F, A, eps = synthetic.system(O, V, 12)
+    W = A[:O, :O, O:, O:]
+    Vd = A[O:, O:, :O, :O].transpose(2, 3, 0, 1)
+    assert np.allclose(Vd, W.conj(), rtol=0, atol=1e-15) and np.allclose(F, F.conj().T, rtol=0, atol=1e-15)
+    _, t, lam, _, _, _ = adjoint.gradients(F, W, Vd, eps, O)
+    assert np.abs(lam - 0.25 * t.conj()).max() < 1e-13 * np.abs(t).max()
+    F2 = F.copy()
+    F2[O, O + 1] += 0.05j                     # no longer Hermitian
+    _, t2, lam2, _, _, _ = adjoint.gradients(F2, W, Vd, eps, O)
+    assert np.abs(lam2 - 0.25 * t2.conj()).max() > 1e-6 * np.abs(t2).max()
