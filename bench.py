@@ -580,7 +580,7 @@ def spatial_step_flops_gauss3m(o, v):
     return total - 0.25 * (ladder + 8.0 * 3 * (o * v) ** 3)
 
 
-def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10, tune=False):
+def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10, tune=False, gauss3m=True):
     """Spin-adapted closed-shell CISD residual steps (apy_cisd_sp_step) at
     o=10, v=80 spatial orbitals (the closed-shell counterpart of the
     O=20, V=160 spin-orbital workload), seeded chemists' integrals with the
@@ -610,6 +610,15 @@ def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10, tune=False):
         ci.step(t, tn, r)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / steps * 1e3
+    ov, oo, vv = o * v, o * o, v * v
+    ladder, flops = spatial_step_flops(o, v)
+    res = {"workload": f"spin-adapted CISD residual step (ci_wfn.py:457-486), o={o} v={v} spatial orbitals",
+           "ms_per_step": round(ms, 3), "executed_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+           "ladder_flops": ladder, "step_flops": flops,
+           "plain_ladder_flops": 8.0 * oo * vv * vv}
+    if not gauss3m:
+        log(f"spatial cisd: {res}")
+        return res
     # the same steps with the ladder and ring products in the three-product form
     ci3 = SpatialCI(ci.ints, F, eps, gauss3m=True)
     tn3, r3 = ci3.new_vector(), ci3.new_vector()
@@ -625,22 +634,34 @@ def spatial_cisd_measurement(dev, seed, o=10, v=80, steps=10, tune=False):
     torch.cuda.synchronize()
     ms3 = (time.perf_counter() - t0) / steps * 1e3
     del ci3
-    ov, oo, vv = o * v, o * o, v * v
-    ladder, flops = spatial_step_flops(o, v)
-    res = {"workload": f"spin-adapted CISD residual step (ci_wfn.py:457-486), o={o} v={v} spatial orbitals",
-           "ms_per_step": round(ms, 3), "executed_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
-           "ladder_flops": ladder, "step_flops": flops,
-           "plain_ladder_flops": 8.0 * oo * vv * vv,
-           "gauss3m": {"what": "the same step with the ladder and ring products in the three-product complex form",
+    res.update({"gauss3m": {"what": "the same step with the ladder and ring products in the three-product complex form",
                        "ms_per_step": round(ms3, 3),
                        "effective_tflops": round(flops / (ms3 * 1e-3) / 1e12, 2),
                        "executed_tflops": round(spatial_step_flops_gauss3m(o, v) / (ms3 * 1e-3) / 1e12, 2),
-                       "max_rel_diff_residual_vs_four_product": diff3}}
+                       "max_rel_diff_residual_vs_four_product": diff3}})
     log(f"spatial cisd: {res}")
     return res
 
 
-def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=2, h=1e-5, tune=False, steps=10):
+def _response_step_gauss3m(rs, X, Xn, R, sv, steps, sync_time):
+    """The batched response step with the ladder and ring products in the
+    three-product form: (time of `steps` steps, largest residual difference
+    from the four-product step relative to its largest element)."""
+    rs.gauss3m = False
+    rs._step(X, Xn, R, sv)
+    R4 = R.clone()
+    rs.gauss3m = True
+    rs._step(X, Xn, R, sv)
+    diff3 = float((R - R4).abs().max().cpu()) / max(1e-300, float(R4.abs().max().cpu()))
+    for _ in range(2):
+        rs._step(X, Xn, R, sv)
+    _, t = sync_time(lambda: [rs._step(X, Xn, R, sv) for _ in range(steps)])
+    rs.gauss3m = False
+    return t, diff3
+
+
+def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=2, h=1e-5, tune=False, steps=10,
+                                  gauss3m=True):
     """First-order CISD amplitudes of `nb` perturbations (the 3N+3 field
     and nuclear perturbations of a 4-atom molecule in the analytic AAT) of
     one spin-adapted CISD solution at o=10, v=80: seeded chemists' integrals
@@ -708,16 +729,10 @@ def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=
     for _ in range(2):
         rs._step(X, Xn, R, sv)
     _, t_steps = sync_time(lambda: [rs._step(X, Xn, R, sv) for _ in range(steps)])
-    # the same batched step with the ladder and ring products in the three-product form
-    rs._step(X, Xn, R, sv)
-    R4 = R.clone()
-    rs.gauss3m = True
-    rs._step(X, Xn, R, sv)
-    diff3 = float((R - R4).abs().max().cpu()) / max(1e-300, float(R4.abs().max().cpu()))
-    for _ in range(2):
-        rs._step(X, Xn, R, sv)
-    _, t_steps3 = sync_time(lambda: [rs._step(X, Xn, R, sv) for _ in range(steps)])
-    del rs, X, Xn, R, R4
+    t_steps3 = diff3 = None
+    if gauss3m:
+        t_steps3, diff3 = _response_step_gauss3m(rs, X, Xn, R, sv, steps, sync_time)
+    del rs, X, Xn, R
 
     def one_by_one():
         out = []
@@ -749,7 +764,7 @@ def analytic_response_measurement(dev, seed, o=10, v=80, nb=15, scale=0.01, nfd=
            "set_iterations_max": int(max(its)),
            "response_step_ms": round(t_steps / steps * 1e3, 3),
            "response_step_executed_tflops": round(nb * spatial_step_flops(o, v)[1] / (t_steps / steps) / 1e12, 2),
-           "response_step_gauss3m": {
+           "response_step_gauss3m": None if t_steps3 is None else {
                "what": "the same batched step with the ladder and ring products in the three-product complex form",
                "ms": round(t_steps3 / steps * 1e3, 3),
                "effective_tflops": round(nb * spatial_step_flops(o, v)[1] / (t_steps3 / steps) / 1e12, 2),
@@ -948,7 +963,8 @@ def main():
                          "iteration) instead of on the device")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replayed Davidson timing")
     ap.add_argument("--no-gauss3m", action="store_true",
-                    help="skip the three-product (Gauss) complex GEMM leg of the headline iterations")
+                    help="skip the three-product (Gauss) complex GEMM legs (headline iterations, spin-adapted "
+                         "and response steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-aat-set", action="store_true", help="skip the batched finite-difference set timing")
     ap.add_argument("--no-cisd-set", action="store_true", help="skip the CISD field-derivative set timing")
@@ -1108,10 +1124,12 @@ def main():
             aat_set["C"] = cisd_field_set_measurement(dev, args.seed)
         if not args.no_mp2_adjoint_set:
             aat_set["D"] = mp2_adjoint_set_measurement(dev, args.seed)
-    spatial = None if args.no_spatial else spatial_cisd_measurement(dev, args.seed, tune=args.tune)
+    spatial = None if args.no_spatial else spatial_cisd_measurement(dev, args.seed, tune=args.tune,
+                                                                    gauss3m=not args.no_gauss3m)
     eri_so = None if args.no_eri_so else eri_so_measurement(dev, args.seed)
     response = None if args.no_analytic_response else analytic_response_measurement(dev, args.seed,
-                                                                                     tune=args.tune)
+                                                                                     tune=args.tune,
+                                                                                     gauss3m=not args.no_gauss3m)
     h2o2 = None if args.no_h2o2_aat else h2o2_aat_measurement(dev)
 
     cpu = None
