@@ -437,6 +437,8 @@ std::map<int, DevInfo> g_dev;
 
 using Key = std::tuple<int64_t, int64_t, int64_t, int, int, int64_t, int, int>;  // m,n,k,al,bl,batch,reduce,device
 std::map<Key, std::pair<int, int>> g_tuned;
+// cost-model choices already made (the model's search over tiles and splits is deterministic per shape)
+std::map<Key, std::pair<int, int>> g_model;
 
 int current_device() {
   int d = 0;
@@ -489,7 +491,7 @@ int max_split(const apy_gemm_desc* d) {
   return std::max(1, s);
 }
 
-Plan plan_for(const apy_gemm_desc* d, int cfg, int split) {
+Plan plan_for(const apy_gemm_desc* d, int cfg, int split, const DevInfo& I) {
   const Cfg& c = CFGS[cfg];
   Plan pl;
   pl.cfg = cfg;
@@ -507,7 +509,6 @@ Plan plan_for(const apy_gemm_desc* d, int cfg, int split) {
   pl.npart = (int64_t)split * d->batch;
   // Cost model: rounds of resident blocks x stages per block x stage time,
   // plus the partial-sum traffic of the reduction.
-  const DevInfo& I = dev_info();
   const int lay = (d->a_layout == APY_MK ? 1 : 0) | (d->b_layout == APY_NK ? 2 : 0);
   const double slots = (double)I.cus * (double)I.occ[cfg][lay];
   const double rounds = std::ceil((double)pl.total / slots);
@@ -520,6 +521,8 @@ Plan plan_for(const apy_gemm_desc* d, int cfg, int split) {
   return pl;
 }
 
+Plan plan_for(const apy_gemm_desc* d, int cfg, int split) { return plan_for(d, cfg, split, dev_info()); }
+
 bool cfg_fits(const apy_gemm_desc* d, int cfg) {
   const Cfg& c = CFGS[cfg];
   // Tiles much larger than the matrix waste most of the MFMA work.
@@ -529,6 +532,16 @@ bool cfg_fits(const apy_gemm_desc* d, int cfg) {
 }
 
 Plan model_plan(const apy_gemm_desc* d) {
+  const Key key = key_of(d);
+  int mcfg = 0, msplit = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_model.find(key);
+    if (it != g_model.end()) { mcfg = it->second.first; msplit = it->second.second; }
+  }
+  // plan_for takes g_mu itself (dev_info), so it runs after the lock above is released
+  if (mcfg > 0) return plan_for(d, mcfg, msplit);
+  const DevInfo& I = dev_info();
   Plan best;
   best.cost = 1e300;
   best.cfg = 0;
@@ -537,11 +550,13 @@ Plan model_plan(const apy_gemm_desc* d) {
     if (!cfg_fits(d, cfg) && cfg != 7) continue;
     for (int s = 1; s <= smax; ++s) {
       if (s > 1 && (d->k + s - 1) / s < 4 * CFGS[cfg].bk) break;
-      Plan p = plan_for(d, cfg, s);
+      Plan p = plan_for(d, cfg, s, I);
       if (p.cost < best.cost) best = p;
     }
   }
-  if (best.cfg == 0) best = plan_for(d, 7, 1);
+  if (best.cfg == 0) best = plan_for(d, 7, 1, I);
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_model[key] = {best.cfg, best.split};
   return best;
 }
 
@@ -551,8 +566,9 @@ Plan make_plan(const apy_gemm_desc* d) {
     if (s > 0) return plan_for(d, d->tile, s);
     Plan best;
     best.cost = 1e300;
+    const DevInfo& I = dev_info();
     for (int q = 1; q <= max_split(d); ++q) {
-      Plan p = plan_for(d, d->tile, q);
+      Plan p = plan_for(d, d->tile, q, I);
       if (p.cost < best.cost) best = p;
     }
     return best;

@@ -165,3 +165,32 @@ def test_device_iteration_control_argument_errors():
     assert lib.apy_iter_check(ok, ok, 1, 1e-12, 1e-12, 10, ok, ok, 0, ok, None) == _lib.APY_E_ARG
     assert lib.apy_iter_check(ok, None, 1, 1e-12, 1e-12, 10, ok, ok, 4, ok, None) == _lib.APY_E_ARG
     assert lib.apy_iter_check(ok, ok, 1, 1e-12, 1e-12, 10, ok, ok, 4, None, None) == _lib.APY_E_ARG
+
+
+def test_model_plan_repeats_without_deadlock():
+    """apy_zgemm_plan of one shape three times (the second and later calls
+    read the cost model's memoized choice): the same plan each time, in a
+    child process with a time limit so that a lock taken twice fails the
+    test instead of hanging the suite.  Host-only (the device queries fall
+    back to defaults without a GPU)."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    import subprocess
+    import sys
+    code = (
+        "import ctypes\n"
+        "from apyib_amd import _lib\n"
+        "lib = _lib.load()\n"
+        "buf = (ctypes.c_byte * 64)()\n"
+        "p = ctypes.addressof(buf) + 16 - ctypes.addressof(buf) % 16\n"
+        "d = _lib.GemmDesc(64, 96, 40, _lib.APY_MK, _lib.APY_KN, 0, 0, 1.0, 0.0, 0.0, 0.0, p, 40, 0, p, 96, 0, p, 96,"
+        " 0, 1, 0, 0, 0, 0)\n"
+        "out = []\n"
+        "for _ in range(3):\n"
+        "    t, s = ctypes.c_int(), ctypes.c_int()\n"
+        "    assert lib.apy_zgemm_plan(ctypes.byref(d), ctypes.byref(t), ctypes.byref(s)) == 0\n"
+        "    out.append((t.value, s.value))\n"
+        "assert out[0] == out[1] == out[2] and out[0][0] >= 1, out\n"
+        "print('ok')\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
