@@ -30,9 +30,12 @@ class MP2Adjoint:
     """Device state of one MP2 energy/gradient evaluation.
 
     fock [n,n], W = <ij||ab> [O,O,V,V], Vd = <ab||ij> at [i,j,a,b], eps [n]
-    (denominators of the Jacobi preconditioner)."""
+    (denominators of the Jacobi preconditioner).  hermitian=True (the caller
+    asserts F = F^H and Vd = conj(W)): the lambda equations are the
+    conjugated amplitude equations scaled by 1/4, so gradients() takes
+    lambda = conj(t)/4 instead of solving them."""
 
-    def __init__(self, fock, W, Vd, eps, O, tol=1e-12, maxit=200):
+    def __init__(self, fock, W, Vd, eps, O, tol=1e-12, maxit=200, hermitian=False):
         self.lib = _lib.load()
         self.dev = fock.device
         self.O = int(O)
@@ -43,6 +46,7 @@ class MP2Adjoint:
         self.eps = torch.as_tensor(np.asarray(eps, dtype=np.float64) if not isinstance(eps, torch.Tensor) else eps,
                                    dtype=torch.float64).to(self.dev).contiguous()
         self.tol, self.maxit = tol, maxit
+        self.hermitian = bool(hermitian)
         self.ws_bytes = int(self.lib.apy_mp2_nc_workspace(self.O, self.V))
         self.rnorm = torch.zeros(1, dtype=CDT, device=self.dev)
         self.t = None
@@ -76,7 +80,7 @@ class MP2Adjoint:
         """(dE/dF [n,n], dE/d<ij||ab>, dE/d<ab||ij>) at the current solution."""
         if self.t is None:
             self.energy()
-        lam = self._solve(True, self.W, 0.25)
+        lam = 0.25 * self.t.conj() if self.hermitian else self._solve(True, self.W, 0.25)
         n = self.O + self.V
         G = torch.empty((n, n), dtype=CDT, device=self.dev)
         nb = int(self.lib.apy_mp2_nc_density_workspace(self.O, self.V))

@@ -860,7 +860,8 @@ def mp2_adjoint_set_measurement(dev, seed, O=30, V=270, ninst=8, ndir=200, nfd=2
     Hermitian directions (the last 4 on <pq||rs>: real draws with its pair
     antisymmetry, the rest on F).  Adjoint: one amplitude solve, one lambda
     solve and the gradient densities per instance give all directional
-    derivatives.  Plain loop: central differences, two solves per
+    derivatives (the Hermitian leg takes lambda = conj(t)/4 instead of its
+    solve).  Plain loop: central differences, two solves per
     direction, timed on `nfd` directions per instance (one on F, one on
     <pq||rs>) and scaled to `ndir`; those directions also check the adjoint
     against the differences."""
@@ -879,7 +880,7 @@ def mp2_adjoint_set_measurement(dev, seed, O=30, V=270, ninst=8, ndir=200, nfd=2
         return out, time.perf_counter() - t0
 
     n, N2 = O + V, O * O * V * V
-    t_adj = t_fd = 0.0
+    t_adj = t_fd = t_adj_h = diff_h = 0.0
     errs, gens = [], 0.0
     for k in range(ninst):
         sk = seed * 100 + k
@@ -892,8 +893,8 @@ def mp2_adjoint_set_measurement(dev, seed, O=30, V=270, ninst=8, ndir=200, nfd=2
         if k == 0:                                                   # warm the kernels and plans once
             MP2Adjoint(F, W, Vd, eps, O, tol=tol).gradients()
 
-        def adjoint():
-            s = MP2Adjoint(F, W, Vd, eps, O, tol=tol)
+        def adjoint(hermitian=False):
+            s = MP2Adjoint(F, W, Vd, eps, O, tol=tol, hermitian=hermitian)
             s.energy()
             G, gW, gV = s.gradients()
             out = list(zdot_many(dFs, G))
@@ -905,6 +906,10 @@ def mp2_adjoint_set_measurement(dev, seed, O=30, V=270, ninst=8, ndir=200, nfd=2
             return np.array(out), G, gW, gV
         (g, G, gW, gV), ta = sync_time(adjoint)
         t_adj += ta
+        # the same with lambda = conj(t)/4 (these seeded instances are Hermitian)
+        (gh, _, _, _), tah = sync_time(lambda: adjoint(hermitian=True))
+        t_adj_h += tah
+        diff_h = max(diff_h, float(np.abs(gh - g).max() / max(1e-300, np.abs(g).max())))
 
         def fd_pair(which):
             Ev = []
@@ -939,7 +944,10 @@ def mp2_adjoint_set_measurement(dev, seed, O=30, V=270, ninst=8, ndir=200, nfd=2
            "plain_fd_s_measured": round(t_fd, 3), "plain_fd_directions_measured": nfd * ninst,
            "plain_fd_s_scaled": round(plain, 2),
            "speedup_adjoint_vs_plain_fd": round(plain / t_adj, 1),
-           "fd_step": h, "max_err_adjoint_vs_fd_rel_to_scale": float(max(errs))}
+           "fd_step": h, "max_err_adjoint_vs_fd_rel_to_scale": float(max(errs)),
+           "adjoint_hermitian_s": round(t_adj_h, 3),
+           "speedup_adjoint_hermitian_vs_plain_fd": round(plain / t_adj_h, 1),
+           "max_rel_diff_adjoint_hermitian_vs_solved_lambda": diff_h}
     log(f"mp2 adjoint set: {res}")
     return res
 

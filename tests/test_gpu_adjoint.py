@@ -102,3 +102,18 @@ def test_autograd_function_gradients(dev):
     assert np.abs(gF.cpu().numpy() - GF.conj()).max() < 1e-12
     assert np.abs(gW.cpu().numpy() - GW.conj()).max() < 1e-13
     assert np.abs(gV.cpu().numpy() - GV.conj()).max() < 1e-12
+
+
+@pytest.mark.parametrize("O,V,seed", [(4, 6, 1), (6, 9, 2)])
+def test_hermitian_lambda_matches_oracle(dev, O, V, seed):
+    """MP2Adjoint(hermitian=True) (lambda = conj(t)/4, no lambda solve) on
+    Hermitian systems: the gradients of the oracle's solved lambda."""
+    F, W, Vd, eps = _host_system(O, V, seed)
+    assert np.allclose(Vd, W.conj(), rtol=0, atol=1e-15)
+    _, _, lam_ref, GF_ref, GW_ref, GV_ref = adjoint.gradients(F, W, Vd, eps, O)
+    s = MP2Adjoint(_d(F, dev), _d(W, dev), _d(Vd, dev), eps, O, hermitian=True)
+    s.energy()
+    GF, GW, GV = s.gradients()
+    assert np.abs(GF.cpu().numpy() - GF_ref).max() < 1e-12
+    assert np.abs(GW.cpu().numpy() - GW_ref).max() < 1e-13
+    assert np.abs(GV.cpu().numpy() - GV_ref).max() < 1e-12
