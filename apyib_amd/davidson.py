@@ -48,7 +48,9 @@ class CISDDavidson:
     packed=False keeps dense vectors (the layout MultiDavidson's batched sigma
     reads)."""
 
-    def __init__(self, soci, max_space=16, packed=True):
+    def __init__(self, soci, max_space=16, packed=True, restart="sigma"):
+        if restart not in ("sigma", "combine"):
+            raise ValueError("restart must be 'sigma' or 'combine'")
         if not soci.fov or not soci.singles:
             raise ValueError("Davidson needs the Hermitian CISD operator: SOCI(..., singles=True, fov_coupling=True)")
         self.soci = soci
@@ -57,6 +59,9 @@ class CISDDavidson:
         self.O, self.V = soci.O, soci.V
         self.OV, self.L = soci.OV, soci.L
         self.max_space = int(max_space)
+        # at a restart the Ritz vector's sigma is a fresh product ("sigma") or sum_j y_j S_j of the stored
+        # products ("combine": sigma is linear, one pass over the k products instead of a sigma)
+        self.restart = restart
         if not 2 <= self.max_space <= _lib.APY_MAXVEC:
             raise ValueError(f"max_space must be in [2, {_lib.APY_MAXVEC}]")
         self.packed = bool(packed) and self.O >= 2 and self.V >= 2
@@ -178,11 +183,14 @@ class CISDDavidson:
             self._ritz = (k, y)
             self.E = theta
             restart = k == self.max_space
-            if restart:                             # restart from the Ritz vector, with a fresh sigma
+            if restart:                             # restart from the Ritz vector
                 self._form_ritz()
                 B[0].copy_(x)
+                if self.restart == "combine":
+                    self.combine([S[j] for j in range(k)], y, S[0])
+                else:
+                    self._sigma(0)
                 self._ritz = (1, np.ones(1))
-                self._sigma(0)
                 self._dots_dev([B[0]], S[0], self._ibuf[2:3])
                 k = 1
             d = B[k]
@@ -274,9 +282,12 @@ class CISDDavidson:
         e_conv, r_conv = self._conv
         _lib.check(self.lib.apy_davidson_check(_vp(self.eig), _vp(self.rr), e_conv, r_conv, _vp(self.ctl),
                                                _vp(self.log), self.cap, st), "apy_davidson_check")
-        if k == M:                                  # restart from the Ritz vector, with a fresh sigma
+        if k == M:                                  # restart from the Ritz vector
             self._comb_dev([B[j] for j in range(k)], self.eig[1:], B[0])
-            self._sigma(0)
+            if self.restart == "combine":
+                self._comb_dev([S[j] for j in range(k)], self.eig[1:], S[0])
+            else:
+                self._sigma(0)
             self._dots_dev([B[0]], S[0], self.G_dev[0][0:1])
             k = 1
             self._ritz_dev = (1, self.one)

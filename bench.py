@@ -1121,6 +1121,35 @@ def main():
         log(f"graph-replayed step {ms_g:.3f} ms ({gsig} sigma in {K} iterations), E_corr = {theta_g:.12f}")
         del dg
 
+    restart_leg = None
+    if device_control and not args.no_graph:
+        # the same iterations with the restart sigma formed from the stored products (restart="combine")
+        dav._sigma = sigma_plain
+        dc = CISDDavidson(solver, max_space=args.max_space, restart="combine")
+        gc = dc.iterate_device(graph=True)
+        next(gc)
+        for _ in range(W):
+            next(gc)
+        torch.cuda.synchronize()
+        c0 = dc.nsigma
+        tc = time.perf_counter()
+        for _ in range(K):
+            next(gc)
+        torch.cuda.synchronize()
+        ec = time.perf_counter() - tc
+        csig = dc.nsigma - c0
+        _, theta_c, rn_c, _ = dc.read_state()
+        gc.close()
+        ms_c = ec * 1e3 / K
+        restart_leg = {"what": "the graph-replayed iterations with restart=\"combine\": at a restart the Ritz vector's "
+                               "sigma is sum_j y_j S_j of the stored products instead of a fresh sigma",
+                       "ms_per_step": round(ms_c, 4), "sigma_per_step": round(csig / K, 3),
+                       "tflops": round(flops_sigma * csig / K / (ms_c * 1e-3) / 1e12, 3),
+                       "E_corr_after_steps": theta_c, "residual_after_steps": rn_c,
+                       "E_corr_minus_fresh_sigma_restart": theta_c - theta}
+        log(f"combined-restart step {ms_c:.3f} ms ({csig} sigma in {K} iterations), E_corr = {theta_c:.12f}")
+        del dc
+
     gauss3m = None
     if not args.no_gauss3m:
         gauss3m = gauss3m_measurement(dev, ints, F, eps, hip, fl, args.max_space, K, W, theta)
@@ -1174,6 +1203,7 @@ def main():
         },
         "dense_equivalent_tflops": round(dense_flops(O, V) * nsig / K / (ms_step * 1e-3) / 1e12, 3),
         "graph": graph_leg,
+        "restart_combined": restart_leg,
         "gauss3m": gauss3m,
         "sigma": {"what": "the sigma calls of the timed iterations alone (HIP events on their stream around each "
                           "apy_cisd_so_step + reference-row energy); executed flops as value",

@@ -180,3 +180,29 @@ def test_device_davidson_matches_oracle_and_explicit_matrix(dev):
         assert abs(E - E_ref) < 1e-11 and abs(E - E_dav) < 1e-11
         assert np.abs(t1.cpu().numpy() - t1_ref).max() < 1e-8
         assert np.abs(t2.cpu().numpy() - t2_ref).max() < 1e-8
+
+
+@pytest.mark.parametrize("device_control,graph", [(False, False), (True, False), (True, True)])
+def test_combined_restart_matches_fresh_sigma_restart(dev, device_control, graph):
+    """restart="combine" (the Ritz vector's sigma formed as sum_j y_j S_j at
+    a restart) against the fresh-sigma restart with a small subspace (many
+    restarts): the same root to 1e-11 Eh and amplitudes to 1e-8, one sigma
+    fewer per restart."""
+    O, V, seed, M = 8, 20, 5, 4
+    F, A, eps, ints = _system(O, V, seed, dev)
+    res = {}
+    for mode in ("sigma", "combine"):
+        d = CISDDavidson(SOCI(ints, torch.from_numpy(F).to(dev), eps, singles=True, fov_coupling=True),
+                         max_space=M, restart=mode)
+        if device_control:
+            E = d.solve_device(e_conv=1e-13, r_conv=1e-9, chunk=3, graph=graph)
+        else:
+            E = d.solve(e_conv=1e-13, r_conv=1e-9)
+        res[mode] = (E, d.amplitudes(), d.nsigma, d.iterations)
+    (Es, (t1s, t2s), ns, its), (Ec, (t1c, t2c), nc, itc) = res["sigma"], res["combine"]
+    print("iterations", its, itc, "sigma", ns, nc)
+    assert abs(Ec - Es) < 1e-11
+    assert float((t1c - t1s).abs().max().cpu()) < 1e-8
+    assert float((t2c - t2s).abs().max().cpu()) < 1e-8
+    assert itc <= its + 3
+    assert nc < ns
