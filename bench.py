@@ -969,7 +969,8 @@ def main():
     ap.add_argument("--host-control", action="store_true",
                     help="Davidson with the projected eigenproblem on the host (numpy eigh, one host read per "
                          "iteration) instead of on the device")
-    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replayed Davidson timing")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the graph-replayed Davidson timings (fresh-sigma and combined restart)")
     ap.add_argument("--no-gauss3m", action="store_true",
                     help="skip the three-product (Gauss) complex GEMM legs (headline iterations, spin-adapted "
                          "and response steps)")
